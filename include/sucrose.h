@@ -1,0 +1,213 @@
+/*
+ * sucrose.h -- C ABI of the MI355X (gfx950) implementation of the SUCROSE
+ * per-step force-and-integrate path.
+ *
+ * The reference keeps one system in Fortran module globals and is driven by a
+ * user loop (README.md:58-118) that calls the routines below.  This ABI keeps
+ * the same routine names and argument meaning; the state lives on the GPU in
+ * a context and moves to/from host arrays only on explicit upload/download.
+ *
+ * Conventions
+ *  - Every entry point returns SX_OK (0) or a negative SX_ERR_* code; the
+ *    message is available from sx_last_error().  The reference's `stop`
+ *    conditions map to error codes (callers that want the reference's
+ *    behaviour stop on a non-zero return).
+ *  - Host arrays are caller-owned FP64 (Fortran real64), contiguous, read or
+ *    written only during the call.  Particle indices passed in are 1-based
+ *    (Fortran), exactly as the reference's I, J arguments.
+ *  - Calls are ordered on the context's HIP stream; only calls that return
+ *    data to the host synchronise.
+ *  - One host thread per context.  Contexts are independent.
+ */
+#ifndef SUCROSE_H
+#define SUCROSE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sx_ctx sx_ctx;
+
+/* ---- status codes ---- */
+#define SX_OK              0
+#define SX_ERR_ARG        -1   /* bad argument (index out of range, n <= 0, ...) */
+#define SX_ERR_HIP        -2   /* HIP runtime error                                */
+#define SX_ERR_STATE      -3   /* call order (e.g. forces before nlist_init)      */
+#define SX_ERR_CAPACITY   -4   /* device capacity exceeded (re-grown on next call) */
+#define SX_ERR_CONSTRAINT -5   /* RATTLE failure: system.f90:1101-1105            */
+#define SX_ERR_CUTOFF     -6   /* cutoff > box/2: system.f90:902-906              */
+#define SX_ERR_NOTEMP     -7   /* target temperature unset: integrators.f90:311   */
+
+/* ---- per-particle fields: the arrays of `module system` (system.f90:7-22)
+ *      and RX_old.. (:22) ---- */
+enum sx_field {
+    SX_RX, SX_RY, SX_RZ, SX_VX, SX_VY, SX_VZ, SX_FX, SX_FY, SX_FZ,
+    SX_QW, SX_QX, SX_QY, SX_QZ, SX_LX, SX_LY, SX_LZ, SX_TX, SX_TY, SX_TZ,
+    SX_MASS, SX_INV_MASS, SX_CHARGE, SX_SIG_X, SX_SIG_Y, SX_SIG_Z,
+    SX_EPS_X, SX_EPS_Y, SX_EPS_Z, SX_IXX, SX_IYY, SX_IZZ,
+    SX_IXX_INV, SX_IYY_INV, SX_IZZ_INV, SX_RX_OLD, SX_RY_OLD, SX_RZ_OLD,
+    SX_NFIELDS
+};
+
+/* ---- scalars: module system (system.f90:27-30) and nose_hoover
+ *      (integrators.f90:229-230) state ---- */
+enum sx_scalar {
+    SX_POT_ENG, SX_KIN_ENG, SX_KIN_ROT, SX_TEMP, SX_PRESSURE, SX_DENSITY,
+    SX_VIRIAL, SX_VIRIAL_BONDED, SX_TARGET_TEMP, SX_DOF,
+    SX_HB0, SX_HB1, SX_HB2, SX_ETA0, SX_ETA1, SX_ETA2, SX_PETA0, SX_PETA1, SX_PETA2,
+    SX_NSCALARS
+};
+
+/* ---- pair potentials: one code per `*_calculate_forces` routine ----
+ * params[] per kind (same order as the Fortran dummy arguments):
+ *   SX_LJ    sigma, epsilon, cutoff                 force.f90:6
+ *   SX_LJ12  sigma, epsilon, cutoff                 force.f90:51
+ *   SX_LJW   sigma, epsilon, cutoff, wf             force.f90:95
+ *   SX_WCA   sigma, epsilon                         force.f90:155
+ *   SX_MORSE dissoc, width, rmin, cutoff            force.f90:199
+ *   SX_GB    cutoff [, mu, nu]  (reference mu=1 nu=2, force.f90:240)   :308
+ *   SX_GBR   [unused, mu, nu]                       force.f90:430
+ *   SX_ECP   cutoff                                 force.f90:1202
+ *   SX_ECPR  (none)                                 force.f90:1387
+ *   SX_DH    cutoff, db_len, dielectric             force.f90:2589
+ */
+enum sx_kind {
+    SX_LJ = 1, SX_LJ12, SX_LJW, SX_WCA, SX_MORSE, SX_GB, SX_GBR, SX_ECP, SX_ECPR, SX_DH
+};
+
+/* ECP contact-parameter solver (sx_set_ecp_solver) */
+enum sx_ecp_solver {
+    SX_ECP_BRENT_REFERENCE = 0, /* the reference's Brent, atol 1e-3 rtol 1e-2 (force.f90:1046) */
+    SX_ECP_CONVERGED       = 1, /* Newton on dF/dlambda from a cold start to |dlambda| < 1e-10, Brent fallback */
+    SX_ECP_LEARNED         = 2  /* as CONVERGED, warm-started by the lambda* MLP */
+};
+
+/* neighbour-list rebuild rule (sx_nlist_set_rule) */
+enum sx_rebuild_rule {
+    SX_REBUILD_REFERENCE = 0,  /* 4 max|dr|^2 > (rc+skin)^2   (system.f90:1033 after :933) */
+    SX_REBUILD_HALF_SKIN = 1   /* max|dr| > skin/2: no interacting pair can be missed       */
+};
+
+/* ------------------------------------------------------------------ */
+/* context                                                             */
+/* ------------------------------------------------------------------ */
+int  sx_create(int device, sx_ctx **out);
+int  sx_destroy(sx_ctx *ctx);
+const char *sx_last_error(const sx_ctx *ctx);   /* ctx may be NULL (last create error) */
+int  sx_synchronize(sx_ctx *ctx);
+int  sx_version(void);
+
+/* ------------------------------------------------------------------ */
+/* system state (module system, system.f90)                            */
+/* ------------------------------------------------------------------ */
+/* set_num_atoms (system.f90:43): allocates n = num_atoms*rigid_atoms
+ * particles with the reference's sentinel values (-1 params, q = (1,0,0,0)). */
+int sx_set_num_atoms(sx_ctx *ctx, int num_atoms, int rigid_atoms);
+/* set_box (system.f90:185) + set_periodic_boundary (:209) */
+int sx_set_box(sx_ctx *ctx, const double box[3], int periodic);
+int sx_get_box(sx_ctx *ctx, double box[3]);
+int sx_upload(sx_ctx *ctx, int field, const double *host, int n);
+int sx_download(sx_ctx *ctx, int field, double *host, int n);
+/* device address of a field's n FP64 values (valid until the next sx_set_num_atoms) */
+int sx_field_ptr(sx_ctx *ctx, int field, double **dev_ptr);
+int sx_set_scalar(sx_ctx *ctx, int id, double value);
+int sx_get_scalar(sx_ctx *ctx, int id, double *value);
+/* set_num_constraints + add_constraint (system.f90:97, :345); 1-based atoms */
+int sx_set_constraints(sx_ctx *ctx, int nc, const int *bbi1, const int *bbj1, const double *len);
+/* initialize_system defaults (system.f90:525-718): mass 1, shape/energy 1,
+ * inertia (b^2+c^2)/20, dof = 6N - NC - 3*remove_com, then initialize_forces */
+int sx_initialize_system(sx_ctx *ctx, int remove_com);
+/* set_velocity_to_temperature (system.f90:362): Philox normals, COM removal, scaling */
+int sx_set_velocity_to_temperature(sx_ctx *ctx, double temperature, int seed, int remove_com);
+int sx_initialize_forces(sx_ctx *ctx);          /* system.f90:423 */
+int sx_calculate_thermo(sx_ctx *ctx);           /* system.f90:725 */
+
+/* ------------------------------------------------------------------ */
+/* neighbour list (module neighbour_list, system.f90:885-1053)         */
+/* ------------------------------------------------------------------ */
+int sx_nlist_init(sx_ctx *ctx, double skin, double cutoff);   /* init_nlist :895 (builds) */
+int sx_nlist_build(sx_ctx *ctx);                              /* build_nlist :966 */
+int sx_nlist_needs_rebuild(sx_ctx *ctx, int *out);            /* needs_rebuild :1005 */
+int sx_nlist_update(sx_ctx *ctx);   /* needs_rebuild -> build, decided on the device */
+int sx_nlist_set_rule(sx_ctx *ctx, int rule);
+int sx_nlist_rebuild_count(sx_ctx *ctx, long *count);
+/* the reference layout: offset(1..N) 1-based row starts with offset(N) = nnz+1,
+ * nlist(1..nnz) 1-based J ascending within each row */
+int sx_nlist_export(sx_ctx *ctx, int *offset1, int *nlist1, long cap, long *nnz);
+
+/* ------------------------------------------------------------------ */
+/* forces (module lennard_jones .. debye_huckel, force.f90)            */
+/* ------------------------------------------------------------------ */
+/* One call replaces the README pair loop (README.md:86-103) for one potential
+ * over the current neighbour list: F(I) += FXIJ, F(J) -= FXIJ, T(I) += TXI,
+ * T(J) += TXJ, pot_eng += PE. */
+int sx_forces(sx_ctx *ctx, int kind, const double *params, int nparams);
+/* Per-pair evaluation of `*_calculate_forces(I, J, ...)` for explicit 1-based
+ * index pairs; host outputs pe[np], f[3*np], ti[3*np], tj[3*np] (ti/tj may be
+ * NULL for isotropic kinds); lambda[np] (may be NULL) gets ECP's lambda. */
+int sx_pair_eval(sx_ctx *ctx, int kind, const double *params, int nparams, long npairs,
+                 const int *i1, const int *j1, double *pe, double *f, double *ti, double *tj,
+                 double *lambda);
+int sx_set_ecp_solver(sx_ctx *ctx, int solver);
+/* ECP diagnostics of the last sx_forces(SX_ECP*): pairs screened, pairs solved,
+ * total solver iterations, fallbacks from the warm start to Brent. */
+int sx_ecp_stats(sx_ctx *ctx, long out[4]);
+
+/* bonded terms (module bonded, force.f90:2244-2462); kind 1 bend_, 2 cosine_bend_,
+ * 3 dih_, 4 cosine_dih_; idx1 holds 3 (bends) or 4 (dihedrals) 1-based atoms
+ * per term.  Registered terms are evaluated by sx_bonded_forces. */
+int sx_set_bonded(sx_ctx *ctx, int kind, int count, const int *idx1, double coeff, double angle);
+int sx_bonded_forces(sx_ctx *ctx);
+
+/* ------------------------------------------------------------------ */
+/* integrators (integrators.f90)                                       */
+/* ------------------------------------------------------------------ */
+int sx_vv_initial_step(sx_ctx *ctx, double dt);                       /* :7   */
+int sx_vv_final_step(sx_ctx *ctx, double dt);                         /* :36  */
+int sx_vvc_initial_step(sx_ctx *ctx, double dt);                      /* :55  */
+int sx_vvc_final_step(sx_ctx *ctx, double dt);                        /* :89  */
+int sx_qq_initial_step(sx_ctx *ctx, double dt);                       /* :123 */
+int sx_qq_final_step(sx_ctx *ctx, double dt);                         /* :185 */
+/* coupling: NULL = the reference default (HB = T tau^2, HB(1) = n_dof T tau^2) */
+int sx_nht_initial_step(sx_ctx *ctx, double dt, const double *coupling);   /* :303 */
+int sx_nht_final_step(sx_ctx *ctx, double dt, const double *coupling);     /* :356 */
+int sx_nhtc_initial_step(sx_ctx *ctx, double dt, const double *coupling);  /* :394 */
+int sx_nhtc_final_step(sx_ctx *ctx, double dt, const double *coupling);    /* :452 */
+int sx_lgv_initial_step(sx_ctx *ctx, double dt, int step, double damping); /* :507 */
+int sx_lgv_final_step(sx_ctx *ctx, double dt, double damping);             /* :629 */
+int sx_lgvc_initial_step(sx_ctx *ctx, double dt, int step, double damping);/* :678 */
+int sx_lgvc_final_step(sx_ctx *ctx, double dt, double damping);            /* :806 */
+int sx_apply_constraints_a(sx_ctx *ctx, double dt);                   /* system.f90:1060 */
+int sx_apply_constraints_b(sx_ctx *ctx, double dt);                   /* system.f90:1144 */
+
+/* ------------------------------------------------------------------ */
+/* whole steps, captured once as a HIP graph and replayed              */
+/* ------------------------------------------------------------------ */
+enum sx_integrator { SX_INT_VV = 1, SX_INT_NHT = 4, SX_INT_LGV = 6, SX_INT_LGVC = 7 };
+typedef struct sx_step_spec {
+    int integrator;        /* SX_INT_*; NHT also runs the quaternion steps (qq_) */
+    int nforces;           /* number of pair potentials (<= 4)                    */
+    int kinds[4];
+    double params[4][8];
+    int bonded;            /* evaluate registered bonded terms                    */
+    double dt, damping;
+    int first_step;        /* Langevin counter of the first replayed step         */
+} sx_step_spec;
+/* Runs nsteps of: initial step, nlist update, initialize_forces, pair loops,
+ * bonded, final step, calculate_thermo.  The first call captures the graph. */
+int sx_run(sx_ctx *ctx, const sx_step_spec *spec, int nsteps);
+
+/* ------------------------------------------------------------------ */
+/* Philox4x32-10 (xmath.f90:14-113)                                    */
+/* ------------------------------------------------------------------ */
+/* host: the reference's fixed key */
+void sx_philox4x32_10(const uint32_t ctr[4], uint32_t out[4]);
+/* device batch of philox_normal_3seq(i, c2, c3, c4) for i = 1..n -> out[3*n] */
+int sx_philox_normals(sx_ctx *ctx, int n, int c2, int c3, int c4, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -238,6 +238,7 @@ class Oracle:
         return pe.value, f, ti, tj
 
     def bonded(self, kind, st: State, idx, coeff, angle=0.0):
+        idx = [int(x) for x in idx]
         s = self._sys(st)
         pe = C.c_double()
         f = np.zeros((4, 3))

@@ -1,0 +1,328 @@
+"""sucrose_amd -- the SUCROSE per-step force-and-integrate path on one MI355X.
+
+The product is ``libsucrose_hip.so`` (gfx950 HIP kernels behind the C ABI in
+``include/sucrose.h``).  This module is a thin ctypes mirror of that ABI that
+keeps the reference's routine names (``vv_initial_step``, ``init_nlist``,
+``lj_calculate_forces`` ...), for Python callers and the tests.  It never
+falls back to a CPU path: without the built library every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libsucrose_hip.so")
+
+# include/sucrose.h
+FIELDS = [
+    "RX", "RY", "RZ", "VX", "VY", "VZ", "FX", "FY", "FZ",
+    "QW", "QX", "QY", "QZ", "LX", "LY", "LZ", "TX", "TY", "TZ",
+    "MASS", "INV_MASS", "CHARGE", "SIG_X", "SIG_Y", "SIG_Z",
+    "EPS_X", "EPS_Y", "EPS_Z", "IXX", "IYY", "IZZ",
+    "IXX_INV", "IYY_INV", "IZZ_INV", "RX_OLD", "RY_OLD", "RZ_OLD",
+]
+FIELD = {f: k for k, f in enumerate(FIELDS)}
+SCALARS = ["POT_ENG", "KIN_ENG", "KIN_ROT", "TEMP", "PRESSURE", "DENSITY", "VIRIAL",
+           "VIRIAL_BONDED", "TARGET_TEMP", "DOF", "HB0", "HB1", "HB2", "ETA0", "ETA1", "ETA2",
+           "PETA0", "PETA1", "PETA2"]
+SCALAR = {s: k for k, s in enumerate(SCALARS)}
+LJ, LJ12, LJW, WCA, MORSE, GB, GBR, ECP, ECPR, DH = range(1, 11)
+ECP_BRENT_REFERENCE, ECP_CONVERGED, ECP_LEARNED = 0, 1, 2
+REBUILD_REFERENCE, REBUILD_HALF_SKIN = 0, 1
+INT_VV, INT_NHT, INT_LGV, INT_LGVC = 1, 4, 6, 7
+
+ERRORS = {-1: "SX_ERR_ARG", -2: "SX_ERR_HIP", -3: "SX_ERR_STATE", -4: "SX_ERR_CAPACITY",
+          -5: "SX_ERR_CONSTRAINT", -6: "SX_ERR_CUTOFF", -7: "SX_ERR_NOTEMP"}
+
+
+class SucroseError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_PD = C.POINTER(C.c_double)
+_PI = C.POINTER(C.c_int)
+_lib = None
+
+
+class StepSpec(C.Structure):
+    _fields_ = [("integrator", C.c_int), ("nforces", C.c_int), ("kinds", C.c_int * 4),
+                ("params", (C.c_double * 8) * 4), ("bonded", C.c_int),
+                ("dt", C.c_double), ("damping", C.c_double), ("first_step", C.c_int)]
+
+
+def lib():
+    """The loaded product library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise FileNotFoundError(f"{LIBPATH} is not built: run `python -m sucrose_amd.build`")
+        L = C.CDLL(LIBPATH)
+        L.sx_last_error.restype = C.c_char_p
+        L.sx_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+        vp = C.c_void_p
+        for name, args in {
+            "sx_destroy": [vp], "sx_synchronize": [vp],
+            "sx_set_num_atoms": [vp, C.c_int, C.c_int],
+            "sx_set_box": [vp, _PD, C.c_int], "sx_get_box": [vp, _PD],
+            "sx_upload": [vp, C.c_int, _PD, C.c_int], "sx_download": [vp, C.c_int, _PD, C.c_int],
+            "sx_field_ptr": [vp, C.c_int, C.POINTER(C.c_void_p)],
+            "sx_set_scalar": [vp, C.c_int, C.c_double], "sx_get_scalar": [vp, C.c_int, _PD],
+            "sx_set_constraints": [vp, C.c_int, _PI, _PI, _PD],
+            "sx_initialize_system": [vp, C.c_int],
+            "sx_set_velocity_to_temperature": [vp, C.c_double, C.c_int, C.c_int],
+            "sx_initialize_forces": [vp], "sx_calculate_thermo": [vp],
+            "sx_nlist_init": [vp, C.c_double, C.c_double], "sx_nlist_build": [vp],
+            "sx_nlist_needs_rebuild": [vp, _PI], "sx_nlist_update": [vp],
+            "sx_nlist_set_rule": [vp, C.c_int], "sx_nlist_rebuild_count": [vp, C.POINTER(C.c_long)],
+            "sx_nlist_export": [vp, _PI, _PI, C.c_long, C.POINTER(C.c_long)],
+            "sx_forces": [vp, C.c_int, _PD, C.c_int],
+            "sx_pair_eval": [vp, C.c_int, _PD, C.c_int, C.c_long, _PI, _PI, _PD, _PD, _PD, _PD, _PD],
+            "sx_set_ecp_solver": [vp, C.c_int], "sx_ecp_stats": [vp, C.POINTER(C.c_long)],
+            "sx_set_bonded": [vp, C.c_int, C.c_int, _PI, C.c_double, C.c_double],
+            "sx_bonded_forces": [vp],
+            "sx_vv_initial_step": [vp, C.c_double], "sx_vv_final_step": [vp, C.c_double],
+            "sx_vvc_initial_step": [vp, C.c_double], "sx_vvc_final_step": [vp, C.c_double],
+            "sx_qq_initial_step": [vp, C.c_double], "sx_qq_final_step": [vp, C.c_double],
+            "sx_nht_initial_step": [vp, C.c_double, _PD], "sx_nht_final_step": [vp, C.c_double, _PD],
+            "sx_nhtc_initial_step": [vp, C.c_double, _PD], "sx_nhtc_final_step": [vp, C.c_double, _PD],
+            "sx_lgv_initial_step": [vp, C.c_double, C.c_int, C.c_double],
+            "sx_lgv_final_step": [vp, C.c_double, C.c_double],
+            "sx_lgvc_initial_step": [vp, C.c_double, C.c_int, C.c_double],
+            "sx_lgvc_final_step": [vp, C.c_double, C.c_double],
+            "sx_apply_constraints_a": [vp, C.c_double], "sx_apply_constraints_b": [vp, C.c_double],
+            "sx_run": [vp, C.POINTER(StepSpec), C.c_int],
+            "sx_philox_normals": [vp, C.c_int, C.c_int, C.c_int, C.c_int, _PD],
+        }.items():
+            getattr(L, name).argtypes = args
+            getattr(L, name).restype = C.c_int
+        L.sx_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.sx_philox4x32_10.restype = None
+        _lib = L
+    return _lib
+
+
+def philox4x32_10(ctr):
+    """Host Philox4x32-10 with the reference's key (xmath.f90:42-76)."""
+    c = (C.c_uint32 * 4)(*[int(x) & 0xFFFFFFFF for x in ctr])
+    o = (C.c_uint32 * 4)()
+    lib().sx_philox4x32_10(c, o)
+    return np.array(list(o), dtype=np.uint32)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_PD)
+
+
+class System:
+    """One device-resident SUCROSE system (the reference's `module system`
+    globals), driven with the reference's routine names."""
+
+    def __init__(self, device: int = 0):
+        self.L = lib()
+        h = C.c_void_p()
+        r = self.L.sx_create(device, C.byref(h))
+        if r != 0:
+            raise SucroseError(r, self.L.sx_last_error(None).decode())
+        self.h = h
+        self.n = 0
+
+    def close(self):
+        if self.h:
+            self.L.sx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ok(self, r):
+        if r != 0:
+            raise SucroseError(r, self.L.sx_last_error(self.h).decode())
+
+    # ---- module system ----
+    def set_num_atoms(self, num_atoms, rigid_atoms=1):
+        self._ok(self.L.sx_set_num_atoms(self.h, int(num_atoms), int(rigid_atoms)))
+        self.n = int(num_atoms) * int(rigid_atoms)
+
+    def set_box(self, length, width=None, height=None, periodic=True):
+        box = np.array([length, width if width is not None else length,
+                        height if height is not None else length], dtype=np.float64)
+        self._ok(self.L.sx_set_box(self.h, _dp(box), 1 if periodic else 0))
+
+    def get_box(self):
+        b = np.zeros(3)
+        self._ok(self.L.sx_get_box(self.h, _dp(b)))
+        return b
+
+    def upload(self, field, values):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        if a.shape != (self.n,):
+            a = np.ascontiguousarray(np.broadcast_to(a, (self.n,)), dtype=np.float64)
+        self._ok(self.L.sx_upload(self.h, FIELD[field.upper()], _dp(a), self.n))
+
+    def download(self, field):
+        a = np.zeros(self.n)
+        self._ok(self.L.sx_download(self.h, FIELD[field.upper()], _dp(a), self.n))
+        return a
+
+    def field_ptr(self, field):
+        p = C.c_void_p()
+        self._ok(self.L.sx_field_ptr(self.h, FIELD[field.upper()], C.byref(p)))
+        return p.value
+
+    def set_scalar(self, name, v):
+        self._ok(self.L.sx_set_scalar(self.h, SCALAR[name.upper()], float(v)))
+
+    def get_scalar(self, name):
+        v = C.c_double()
+        self._ok(self.L.sx_get_scalar(self.h, SCALAR[name.upper()], C.byref(v)))
+        return v.value
+
+    def set_target_temperature(self, t):
+        self.set_scalar("TARGET_TEMP", t)
+
+    def set_constraints(self, bbi1, bbj1, lengths):
+        bi = np.ascontiguousarray(bbi1, dtype=np.int32)
+        bj = np.ascontiguousarray(bbj1, dtype=np.int32)
+        bl = np.ascontiguousarray(lengths, dtype=np.float64)
+        self._ok(self.L.sx_set_constraints(self.h, len(bi), bi.ctypes.data_as(_PI), bj.ctypes.data_as(_PI), _dp(bl)))
+
+    def initialize_system(self, remove_com=True):
+        self._ok(self.L.sx_initialize_system(self.h, 1 if remove_com else 0))
+
+    def set_velocity_to_temperature(self, t, seed, remove_com=True):
+        self._ok(self.L.sx_set_velocity_to_temperature(self.h, float(t), int(seed), 1 if remove_com else 0))
+
+    def initialize_forces(self):
+        self._ok(self.L.sx_initialize_forces(self.h))
+
+    def calculate_thermo(self):
+        self._ok(self.L.sx_calculate_thermo(self.h))
+
+    def synchronize(self):
+        self._ok(self.L.sx_synchronize(self.h))
+
+    # ---- module neighbour_list ----
+    def init_nlist(self, skin, cutoff):
+        self._ok(self.L.sx_nlist_init(self.h, float(skin), float(cutoff)))
+
+    def build_nlist(self):
+        self._ok(self.L.sx_nlist_build(self.h))
+
+    def needs_rebuild(self):
+        o = C.c_int()
+        self._ok(self.L.sx_nlist_needs_rebuild(self.h, C.byref(o)))
+        return bool(o.value)
+
+    def update_nlist(self):
+        self._ok(self.L.sx_nlist_update(self.h))
+
+    def set_rebuild_rule(self, rule):
+        self._ok(self.L.sx_nlist_set_rule(self.h, int(rule)))
+
+    def rebuild_count(self):
+        o = C.c_long()
+        self._ok(self.L.sx_nlist_rebuild_count(self.h, C.byref(o)))
+        return o.value
+
+    def export_nlist(self):
+        """(offset, nlist) in the reference layout: 1-based, offset(N) = nnz+1."""
+        nnz = C.c_long()
+        off = np.zeros(self.n, np.int32)
+        self._ok(self.L.sx_nlist_export(self.h, off.ctypes.data_as(_PI), None, 0, C.byref(nnz)))
+        nb = np.zeros(max(nnz.value, 1), np.int32)
+        self._ok(self.L.sx_nlist_export(self.h, off.ctypes.data_as(_PI), nb.ctypes.data_as(_PI),
+                                        len(nb), C.byref(nnz)))
+        return off, nb[:nnz.value]
+
+    # ---- forces ----
+    def forces(self, kind, *params):
+        p = np.array(params if params else [0.0], dtype=np.float64)
+        self._ok(self.L.sx_forces(self.h, int(kind), _dp(p), len(params)))
+
+    def pair_eval(self, kind, i1, j1, *params):
+        """Per-pair `*_calculate_forces(I, J, ...)` for 1-based index arrays."""
+        i1 = np.ascontiguousarray(i1, dtype=np.int32)
+        j1 = np.ascontiguousarray(j1, dtype=np.int32)
+        npairs = len(i1)
+        p = np.array(params if params else [0.0], dtype=np.float64)
+        pe = np.zeros(npairs); f = np.zeros((npairs, 3)); ti = np.zeros((npairs, 3))
+        tj = np.zeros((npairs, 3)); lam = np.zeros(npairs)
+        self._ok(self.L.sx_pair_eval(self.h, int(kind), _dp(p), len(params), npairs, i1.ctypes.data_as(_PI),
+                                     j1.ctypes.data_as(_PI), _dp(pe), _dp(f), _dp(ti), _dp(tj), _dp(lam)))
+        return pe, f, ti, tj, lam
+
+    def set_ecp_solver(self, solver):
+        self._ok(self.L.sx_set_ecp_solver(self.h, int(solver)))
+
+    def ecp_stats(self):
+        o = (C.c_long * 4)()
+        self._ok(self.L.sx_ecp_stats(self.h, o))
+        return list(o)
+
+    def set_bonded(self, kind, idx1, coeff, angle=0.0):
+        a = np.ascontiguousarray(idx1, dtype=np.int32).reshape(-1)
+        npos = 3 if kind <= 2 else 4
+        self._ok(self.L.sx_set_bonded(self.h, int(kind), len(a) // npos, a.ctypes.data_as(_PI),
+                                      float(coeff), float(angle)))
+
+    def bonded_forces(self):
+        self._ok(self.L.sx_bonded_forces(self.h))
+
+    # ---- integrators ----
+    def vv_initial_step(self, dt): self._ok(self.L.sx_vv_initial_step(self.h, dt))
+    def vv_final_step(self, dt): self._ok(self.L.sx_vv_final_step(self.h, dt))
+    def vvc_initial_step(self, dt): self._ok(self.L.sx_vvc_initial_step(self.h, dt))
+    def vvc_final_step(self, dt): self._ok(self.L.sx_vvc_final_step(self.h, dt))
+    def qq_initial_step(self, dt): self._ok(self.L.sx_qq_initial_step(self.h, dt))
+    def qq_final_step(self, dt): self._ok(self.L.sx_qq_final_step(self.h, dt))
+
+    def _cpl(self, coupling):
+        return C.byref(C.c_double(coupling)) if coupling is not None else None
+
+    def nht_initial_step(self, dt, coupling=None): self._ok(self.L.sx_nht_initial_step(self.h, dt, self._cpl(coupling)))
+    def nht_final_step(self, dt, coupling=None): self._ok(self.L.sx_nht_final_step(self.h, dt, self._cpl(coupling)))
+    def nhtc_initial_step(self, dt, coupling=None): self._ok(self.L.sx_nhtc_initial_step(self.h, dt, self._cpl(coupling)))
+    def nhtc_final_step(self, dt, coupling=None): self._ok(self.L.sx_nhtc_final_step(self.h, dt, self._cpl(coupling)))
+    def lgv_initial_step(self, dt, step, damping=1.0): self._ok(self.L.sx_lgv_initial_step(self.h, dt, int(step), damping))
+    def lgv_final_step(self, dt, damping=1.0): self._ok(self.L.sx_lgv_final_step(self.h, dt, damping))
+    def lgvc_initial_step(self, dt, step, damping=1.0): self._ok(self.L.sx_lgvc_initial_step(self.h, dt, int(step), damping))
+    def lgvc_final_step(self, dt, damping=1.0): self._ok(self.L.sx_lgvc_final_step(self.h, dt, damping))
+    def apply_constraints_a(self, dt): self._ok(self.L.sx_apply_constraints_a(self.h, dt))
+    def apply_constraints_b(self, dt): self._ok(self.L.sx_apply_constraints_b(self.h, dt))
+
+    def run(self, integrator, forces, nsteps, dt, damping=1.0, first_step=1, bonded=False):
+        """nsteps whole steps replayed from one captured HIP graph.
+        forces: list of (kind, params...) tuples."""
+        sp = StepSpec()
+        sp.integrator = integrator
+        sp.nforces = len(forces)
+        for k, fp in enumerate(forces):
+            sp.kinds[k] = fp[0]
+            for q, v in enumerate(fp[1:]):
+                sp.params[k][q] = v
+        sp.bonded = 1 if bonded else 0
+        sp.dt = dt
+        sp.damping = damping
+        sp.first_step = first_step
+        self._ok(self.L.sx_run(self.h, C.byref(sp), int(nsteps)))
+
+    def philox_normals(self, n, c2, c3, c4):
+        out = np.zeros(3 * n)
+        self._ok(self.L.sx_philox_normals(self.h, n, c2, c3, c4, _dp(out)))
+        return out.reshape(n, 3)
+
+    # ---- whole-state helpers ----
+    def load_arrays(self, arrays: dict):
+        for k, v in arrays.items():
+            self.upload(k, v)
+
+    def state(self, fields=FIELDS):
+        return {f: self.download(f) for f in fields}

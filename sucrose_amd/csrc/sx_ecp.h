@@ -1,0 +1,291 @@
+// ECP (Perram-Wertheim ellipsoid contact) device code, FP64 (force.f90:1034-1551).
+//
+// Per-particle quantities that the reference recomputes for every pair (the
+// rotation U = R(q)^T, AR = U^T S^2 U, AE = U^T E U, MM = U^T S and sigma_t,
+// force.f90:1427-1483) are computed once per step with the same expressions,
+// so each pair sees the reference's values bit-for-bit.
+#pragma once
+#include "sx_pairs.h"
+
+// per-particle record, SoA with stride n: q * n + i
+enum {
+    PP_AR = 0,     // 6: a11 a21 a31 a22 a32 a33
+    PP_AE = 6,     // 6
+    PP_MM = 12,    // 9: column-major MM(:,1) MM(:,2) MM(:,3)
+    PP_ST = 21,    // sigma_t of the particle
+    PP_AMAX = 22,  // largest semi-axis
+    PP_N = 23
+};
+
+struct Sym3 {
+    double a11, a21, a31, a22, a32, a33;
+};
+
+__device__ __forceinline__ Sym3 load_sym(const double *pp, long ld, int base, int i)
+{
+    Sym3 m;
+    m.a11 = pp[(base + 0) * ld + i]; m.a21 = pp[(base + 1) * ld + i]; m.a31 = pp[(base + 2) * ld + i];
+    m.a22 = pp[(base + 3) * ld + i]; m.a32 = pp[(base + 4) * ld + i]; m.a33 = pp[(base + 5) * ld + i];
+    return m;
+}
+
+// full 3x3 column-major from a symmetric record
+__device__ __forceinline__ void sym_full(const Sym3 &m, double o[9])
+{
+    o[0] = m.a11; o[1] = m.a21; o[2] = m.a31;
+    o[3] = m.a21; o[4] = m.a22; o[5] = m.a32;
+    o[6] = m.a31; o[7] = m.a32; o[8] = m.a33;
+}
+
+// optim_dist (force.f90:1188-1200)
+__device__ __forceinline__ double ecp_optim_dist(double l, const Sym3 &A, const Sym3 &B, const double r[3])
+{
+    double km[3];
+    sx_chol_solve((1 - l) * A.a11 + l * B.a11, (1 - l) * A.a21 + l * B.a21, (1 - l) * A.a31 + l * B.a31,
+                  (1 - l) * A.a22 + l * B.a22, (1 - l) * A.a32 + l * B.a32, (1 - l) * A.a33 + l * B.a33, r, km);
+    return -l * (1 - l) * (r[0] * km[0] + r[1] * km[1] + r[2] * km[2]);
+}
+
+#define ECP_PSI 0.3819660112501051  /* 0.5*(3-sqrt(5)) in FP64, force.f90:1050 */
+
+// brent (force.f90:1046-1172) continuing from the first evaluation fc = f(psi)
+__device__ __forceinline__ double ecp_brent(const Sym3 &A, const Sym3 &B, const double r[3], double fc, int &iters)
+{
+    const double psi = ECP_PSI;
+    double a = 0.0, b = 1.0, deltax = 0.0, rat = 0.0;
+    const double atol = 1e-3, rtol = 1e-2;
+    double c = psi;
+    double v = c, w = v, x = w;
+    double fv = fc, fw = fv, ft = fw;
+    int iter = 0;
+    while (iter < 10000) {
+        double tol1 = rtol * fabs(x) + atol;
+        double tol2 = 2.0 * tol1;
+        double xmid = 0.5 * (a + b);
+        if (fabs(x - xmid) < (tol2 - 0.5 * (b - a))) break;
+        if (fabs(deltax) <= tol1) {
+            deltax = (x >= xmid) ? a - x : b - x;
+            rat = psi * deltax;
+        } else {
+            double tmp1 = (x - w) * (ft - fv);
+            double tmp2 = (x - v) * (ft - fw);
+            double p = (x - v) * tmp2 - (x - w) * tmp1;
+            tmp2 = 2.0 * (tmp2 - tmp1);
+            if (tmp2 > 0.0) p = -p;
+            tmp2 = fabs(tmp2);
+            double dx_temp = deltax;
+            deltax = rat;
+            if ((p > tmp2 * (a - x)) && (p < tmp2 * (b - x)) && (fabs(p) < fabs(0.5 * tmp2 * dx_temp))) {
+                rat = p * 1.0 / tmp2;
+                double u = x + rat;
+                if ((u - a) < tol2 || (b - u) < tol2) rat = (xmid - x >= 0) ? tol1 : -tol1;
+            } else {
+                deltax = (x >= xmid) ? a - x : b - x;
+                rat = psi * deltax;
+            }
+        }
+        double u;
+        if (fabs(rat) < tol1) u = (rat >= 0) ? x + tol1 : x - tol1;
+        else u = x + rat;
+        double fu = ecp_optim_dist(u, A, B, r);
+        if (fu > ft) {
+            if (u < x) a = u; else b = u;
+            if ((fu <= fw) || (w == x)) {
+                v = w; w = u; fv = fw; fw = fu;
+            } else if ((fu <= fv) || (v == x) || (v == w)) {
+                v = u; fv = fu;
+            }
+        } else {
+            if (u >= x) a = x; else b = x;
+            v = w; w = x; x = u;
+            fv = fw; fw = ft; ft = fu;
+        }
+        iter = iter + 1;
+    }
+    iters = iter;
+    return x;
+}
+
+// F(l) = l(1-l) r^T G^-1 r and its first two derivatives (G = (1-l)A + lB)
+__device__ __forceinline__ void ecp_F_derivs(double l, const Sym3 &A, const Sym3 &B, const double r[3], double &F,
+                                             double &d1, double &d2)
+{
+    double g11 = (1 - l) * A.a11 + l * B.a11, g21 = (1 - l) * A.a21 + l * B.a21, g31 = (1 - l) * A.a31 + l * B.a31;
+    double g22 = (1 - l) * A.a22 + l * B.a22, g32 = (1 - l) * A.a32 + l * B.a32, g33 = (1 - l) * A.a33 + l * B.a33;
+    double k[3];
+    sx_chol_solve(g11, g21, g31, g22, g32, g33, r, k);
+    double d11 = B.a11 - A.a11, d21 = B.a21 - A.a21, d31 = B.a31 - A.a31;
+    double d22 = B.a22 - A.a22, d32 = B.a32 - A.a32, d33 = B.a33 - A.a33;
+    double u[3] = {d11 * k[0] + d21 * k[1] + d31 * k[2], d21 * k[0] + d22 * k[1] + d32 * k[2],
+                   d31 * k[0] + d32 * k[1] + d33 * k[2]};
+    double w[3];
+    sx_chol_solve(g11, g21, g31, g22, g32, g33, u, w);
+    double rk = r[0] * k[0] + r[1] * k[1] + r[2] * k[2];
+    double kdk = k[0] * u[0] + k[1] * u[1] + k[2] * u[2];
+    double udw = u[0] * w[0] + u[1] * w[1] + u[2] * w[2];
+    double ll = l * (1 - l);
+    F = ll * rk;
+    d1 = (1 - 2 * l) * rk - ll * kdk;
+    d2 = -2 * rk - 2 * (1 - 2 * l) * kdk + 2 * ll * udw;
+}
+
+// maximiser of F on (0,1) to |dl| < tol: safeguarded Newton on F' = 0.
+// F is concave on (0,1) with F'(0) > 0 > F'(1), so [lo,hi] always brackets.
+__device__ __forceinline__ double ecp_lambda_newton(const Sym3 &A, const Sym3 &B, const double r[3], double l0,
+                                                    double tol, int &iters, int &ok)
+{
+    double lo = 0.0, hi = 1.0, l = fmin(fmax(l0, 1e-6), 1.0 - 1e-6);
+    ok = 0;
+    int it = 0;
+    for (; it < 60; ++it) {
+        double F, d1, d2;
+        ecp_F_derivs(l, A, B, r, F, d1, d2);
+        if (d1 > 0) lo = l; else hi = l;
+        double step = (d2 < 0) ? -d1 / d2 : 0.0;
+        double ln = l + step;
+        if (!(d2 < 0) || !(ln > lo && ln < hi)) ln = 0.5 * (lo + hi);
+        double dl = fabs(ln - l);
+        l = ln;
+        if (dl < tol || (hi - lo) < tol) { ok = 1; ++it; break; }
+    }
+    iters = it;
+    return l;
+}
+
+// The force/torque evaluation of ecp/ecpr (force.f90:1487-1550) for a given
+// lambda; RIJ already minimum-imaged.
+template <int VARIANT>
+__device__ __forceinline__ void ecp_forces(const DState &s, const double *pp, long ld, int a, int b, double cutoff,
+                                           const double RIJ[3], double lambda_R, PairOut &o)
+{
+    const int meu = 2, neu = 1;
+    const double sigma_0 = 1.0;
+    o.pe = 0.0;
+    for (int k = 0; k < 3; ++k) { o.f[k] = 0.0; o.ti[k] = 0.0; o.tj[k] = 0.0; }
+    double rij_sq = RIJ[0] * RIJ[0] + RIJ[1] * RIJ[1] + RIJ[2] * RIJ[2];
+    double rij_mag = sqrt(rij_sq);
+    double rhat[3] = {RIJ[0] / rij_mag, RIJ[1] / rij_mag, RIJ[2] / rij_mag};
+    Sym3 AR = load_sym(pp, ld, PP_AR, a), BR = load_sym(pp, ld, PP_AR, b);
+    Sym3 AE = load_sym(pp, ld, PP_AE, a), BE = load_sym(pp, ld, PP_AE, b);
+    double lambda_E = lambda_R;
+    double KE[3], KR[3];
+    sx_chol_solve((1 - lambda_E) * AE.a11 + lambda_E * BE.a11, (1 - lambda_E) * AE.a21 + lambda_E * BE.a21,
+                  (1 - lambda_E) * AE.a31 + lambda_E * BE.a31, (1 - lambda_E) * AE.a22 + lambda_E * BE.a22,
+                  (1 - lambda_E) * AE.a32 + lambda_E * BE.a32, (1 - lambda_E) * AE.a33 + lambda_E * BE.a33, RIJ, KE);
+    double eps2 = lambda_E * (1.0 - lambda_E) * (rhat[0] * KE[0] + rhat[1] * KE[1] + rhat[2] * KE[2]) / rij_mag;
+    sx_chol_solve((1 - lambda_R) * AR.a11 + lambda_R * BR.a11, (1 - lambda_R) * AR.a21 + lambda_R * BR.a21,
+                  (1 - lambda_R) * AR.a31 + lambda_R * BR.a31, (1 - lambda_R) * AR.a22 + lambda_R * BR.a22,
+                  (1 - lambda_R) * AR.a32 + lambda_R * BR.a32, (1 - lambda_R) * AR.a33 + lambda_R * BR.a33, RIJ, KR);
+    double phi = lambda_R * (1.0 - lambda_R) * (rhat[0] * KR[0] + rhat[1] * KR[1] + rhat[2] * KR[2]) / rij_mag;
+    double sigma = 1.0 / sqrt(phi);
+    double efac = 2.0 * lambda_E * (1.0 - lambda_E) / rij_sq;
+    double rfac = 2.0 * lambda_R * (1.0 - lambda_R) / rij_sq;
+    double sigma_0_inv = 1.0 / sigma_0;
+    double p32 = sigma * sigma * sigma;
+    double switch_ = 0, factor = 0, sx = 0, sdx = 0, rlim;
+    if (VARIANT == 0) {
+        switch_ = 0.9 * (sigma + cutoff - sigma_0);
+        factor = 1.0 / ((sigma + cutoff - sigma_0) - switch_);
+        double xx = (rij_mag - switch_) * factor;
+        sx = 1.0 - 6.0 * xx * xx * xx * xx * xx + 15.0 * xx * xx * xx * xx - 10.0 * xx * xx * xx;
+        sdx = -30.0 * xx * xx * xx * xx + 60.0 * xx * xx * xx - 30.0 * xx * xx;
+        rlim = sigma + cutoff - sigma_0;
+    } else {
+        rlim = sigma;
+    }
+    bool inside = (VARIANT == 0) ? (rij_mag < rlim) : (rij_mag <= rlim);
+    if (!inside) return;
+    // eps1 = sigma_t sqrt(det(inverse(AR + BR)))  (force.f90:1482-1488)
+    double EAB[9], EM[9];
+    {
+        double ar[9], br[9];
+        sym_full(AR, ar);
+        sym_full(BR, br);
+        for (int k = 0; k < 9; ++k) EAB[k] = ar[k] + br[k];
+#define Mx(m, i, j) (m)[((i) - 1) + 3 * ((j) - 1)]
+        Mx(EM, 1, 1) = Mx(EAB, 2, 2) * Mx(EAB, 3, 3) - Mx(EAB, 3, 2) * Mx(EAB, 2, 3);
+        Mx(EM, 1, 2) = Mx(EAB, 1, 3) * Mx(EAB, 3, 2) - Mx(EAB, 1, 2) * Mx(EAB, 3, 3);
+        Mx(EM, 1, 3) = Mx(EAB, 1, 2) * Mx(EAB, 2, 3) - Mx(EAB, 1, 3) * Mx(EAB, 2, 2);
+        Mx(EM, 2, 1) = Mx(EAB, 2, 3) * Mx(EAB, 3, 1) - Mx(EAB, 2, 1) * Mx(EAB, 3, 3);
+        Mx(EM, 2, 2) = Mx(EAB, 1, 1) * Mx(EAB, 3, 3) - Mx(EAB, 1, 3) * Mx(EAB, 3, 1);
+        Mx(EM, 2, 3) = Mx(EAB, 1, 3) * Mx(EAB, 2, 1) - Mx(EAB, 1, 1) * Mx(EAB, 2, 3);
+        Mx(EM, 3, 1) = Mx(EAB, 2, 1) * Mx(EAB, 3, 2) - Mx(EAB, 2, 2) * Mx(EAB, 3, 1);
+        Mx(EM, 3, 2) = Mx(EAB, 1, 2) * Mx(EAB, 3, 1) - Mx(EAB, 1, 1) * Mx(EAB, 3, 2);
+        Mx(EM, 3, 3) = Mx(EAB, 1, 1) * Mx(EAB, 2, 2) - Mx(EAB, 1, 2) * Mx(EAB, 2, 1);
+        double det = Mx(EAB, 1, 1) * (Mx(EAB, 2, 2) * Mx(EAB, 3, 3) - Mx(EAB, 3, 2) * Mx(EAB, 2, 3))
+                   - Mx(EAB, 1, 2) * (Mx(EAB, 2, 1) * Mx(EAB, 3, 3) - Mx(EAB, 3, 1) * Mx(EAB, 2, 3))
+                   + Mx(EAB, 1, 3) * (Mx(EAB, 2, 1) * Mx(EAB, 3, 2) - Mx(EAB, 3, 1) * Mx(EAB, 2, 2));
+        for (int k = 0; k < 9; ++k) EM[k] = EM[k] / det;
+    }
+    double detEM = Mx(EM, 1, 1) * (Mx(EM, 2, 2) * Mx(EM, 3, 3) - Mx(EM, 3, 2) * Mx(EM, 2, 3))
+                 - Mx(EM, 1, 2) * (Mx(EM, 2, 1) * Mx(EM, 3, 3) - Mx(EM, 3, 1) * Mx(EM, 2, 3))
+                 + Mx(EM, 1, 3) * (Mx(EM, 2, 1) * Mx(EM, 3, 2) - Mx(EM, 3, 1) * Mx(EM, 2, 2));
+    double sigma_t = sqrt(pp[PP_ST * ld + a] * pp[PP_ST * ld + b]);
+    double eps1 = sigma_t * sqrt(detEM);
+
+    double sr = sigma_0 / (rij_mag - sigma + sigma_0);
+    double sr2 = sr * sr, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
+    double srx = (sr12 - 2.0 * sr6);
+    if (VARIANT == 1) srx = srx + 1.0;
+    double srd = 12.0 * (sr6 - sr12) * sr;
+    double e1n = eps1;                 // eps1**neu, neu = 1
+    double e2m = eps2 * eps2;          // eps2**meu, meu = 2
+    double e2m1 = eps2;                // eps2**(meu-1)
+    double dotKR = rhat[0] * KR[0] + rhat[1] * KR[1] + rhat[2] * KR[2];
+    double dotKE = rhat[0] * KE[0] + rhat[1] * KE[1] + rhat[2] * KE[2];
+    double pref1 = e2m * (neu * e1n) * srx;
+    double de1_u1[3], de1_u2[3];
+    for (int side = 0; side < 2; ++side) {
+        int who = side == 0 ? a : b;
+        double d[3][3];
+        for (int c = 0; c < 3; ++c) {
+            double col[3] = {pp[(PP_MM + 3 * c + 0) * ld + who], pp[(PP_MM + 3 * c + 1) * ld + who],
+                             pp[(PP_MM + 3 * c + 2) * ld + who]};
+            double t[3];
+            t[0] = Mx(EM, 1, 1) * col[0] + Mx(EM, 1, 2) * col[1] + Mx(EM, 1, 3) * col[2];
+            t[1] = Mx(EM, 2, 1) * col[0] + Mx(EM, 2, 2) * col[1] + Mx(EM, 2, 3) * col[2];
+            t[2] = Mx(EM, 3, 1) * col[0] + Mx(EM, 3, 2) * col[1] + Mx(EM, 3, 3) * col[2];
+            cross3(col, t, d[c]);
+            for (int k = 0; k < 3; ++k) d[c][k] = pref1 * d[c][k];
+        }
+        double *dst = side == 0 ? de1_u1 : de1_u2;
+        for (int k = 0; k < 3; ++k) dst[k] = (d[0][k] + d[1][k]) + d[2][k];
+    }
+#undef Mx
+    double aef[9], arf[9], bef[9], brf[9];
+    sym_full(AE, aef); sym_full(AR, arf); sym_full(BE, bef); sym_full(BR, brf);
+    double KE_AE[3], KR_AR[3], KE_BE[3], KR_BR[3];
+    // vmdot: a^T . B  (xmath.f90:219-229)
+    for (int j = 0; j < 3; ++j) {
+        KE_AE[j] = KE[0] * aef[0 + 3 * j] + KE[1] * aef[1 + 3 * j] + KE[2] * aef[2 + 3 * j];
+        KR_AR[j] = KR[0] * arf[0 + 3 * j] + KR[1] * arf[1 + 3 * j] + KR[2] * arf[2 + 3 * j];
+        KE_BE[j] = KE[0] * bef[0 + 3 * j] + KE[1] * bef[1 + 3 * j] + KE[2] * bef[2 + 3 * j];
+        KR_BR[j] = KR[0] * brf[0 + 3 * j] + KR[1] * brf[1 + 3 * j] + KR[2] * brf[2 + 3 * j];
+    }
+    double ce1[3], cr1[3], ce2[3], cr2[3];
+    cross3(KE_AE, KE, ce1);
+    cross3(KR_AR, KR, cr1);
+    cross3(KE_BE, KE, ce2);
+    cross3(KR_BR, KR, cr2);
+    double pot = e1n * e2m * srx;
+    for (int k = 0; k < 3; ++k) {
+        double dR_dr = sigma_0_inv * (rhat[k] + 0.5 * rfac * (KR[k] - dotKR * rhat[k]) * p32);
+        double de2_dr = e1n * (meu * e2m1) * efac * (KE[k] - dotKE * rhat[k]);
+        double de2_u1 = e1n * (meu * e2m1) * srx * (1 - lambda_E) * efac * ce1[k];
+        double dR_u1 = e1n * e2m * srd * (0.5 * sigma_0_inv) * (1 - lambda_R) * rfac * cr1[k] * p32;
+        double de2_u2 = e1n * (meu * e2m1) * srx * lambda_E * efac * ce2[k];
+        double dR_u2 = e1n * e2m * srd * (0.5 * sigma_0_inv) * lambda_R * rfac * cr2[k] * p32;
+        o.f[k] = (-1.0) * (e1n * e2m * srd * dR_dr + srx * de2_dr);
+        o.ti[k] = (de1_u1[k] + de2_u1) + dR_u1;
+        o.tj[k] = (de1_u2[k] + de2_u2) + dR_u2;
+    }
+    if (VARIANT == 0 && (rij_mag >= switch_) && (rij_mag <= rlim)) {
+        for (int k = 0; k < 3; ++k) {
+            o.f[k] = sx * o.f[k] - sdx * pot * factor;
+            o.ti[k] = sx * o.ti[k] - sdx * pot * factor;
+            o.tj[k] = sx * o.tj[k] - sdx * pot * factor;
+        }
+        pot = sx * pot;
+    }
+    o.pe = pot;
+}

@@ -1,0 +1,935 @@
+// Integrators (integrators.f90), RATTLE (system.f90:1055-1209) and bonded
+// terms (force.f90:2244-2462) on the device.  Per-particle updates keep the
+// reference's FP64 expressions; loop-invariant transcendental constants
+// (exp(-gamma dt), ...) are evaluated once on the host exactly as the
+// reference evaluates them and passed in.
+#include "sx_pairs.h"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+template <int W>
+__device__ __forceinline__ void block_sum_store(double (&v)[W], double *part)
+{
+    __shared__ double sh[W][256];
+    int t = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < W; ++q) sh[q][t] = v[q];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) {
+#pragma unroll
+            for (int q = 0; q < W; ++q) sh[q][t] = sh[q][t] + sh[q][t + s];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+#pragma unroll
+        for (int q = 0; q < W; ++q) part[blockIdx.x * W + q] = sh[q][0];
+    }
+}
+
+__device__ __forceinline__ void wrap_pos(const DState &s, int i)
+{
+    if (s.periodic) {
+        double x = s.f[SX_RX][i], y = s.f[SX_RY][i], z = s.f[SX_RZ][i];
+        s.f[SX_RX][i] = x - round(x / s.box[0]) * s.box[0];
+        s.f[SX_RY][i] = y - round(y / s.box[1]) * s.box[1];
+        s.f[SX_RZ][i] = z - round(z / s.box[2]) * s.box[2];
+    }
+}
+
+__device__ __forceinline__ void half_kick(const DState &s, int i, double dt)
+{
+    double im = s.f[SX_INV_MASS][i];
+    s.f[SX_VX][i] = s.f[SX_VX][i] + 0.5 * dt * (s.f[SX_FX][i] * im);
+    s.f[SX_VY][i] = s.f[SX_VY][i] + 0.5 * dt * (s.f[SX_FY][i] * im);
+    s.f[SX_VZ][i] = s.f[SX_VZ][i] + 0.5 * dt * (s.f[SX_FZ][i] * im);
+}
+
+__device__ __forceinline__ void save_old(const DState &s, int i)
+{
+    s.f[SX_RX_OLD][i] = s.f[SX_RX][i];
+    s.f[SX_RY_OLD][i] = s.f[SX_RY][i];
+    s.f[SX_RZ_OLD][i] = s.f[SX_RZ][i];
+}
+
+__device__ __forceinline__ double kin_trans(const DState &s, int i)
+{
+    double vx = s.f[SX_VX][i], vy = s.f[SX_VY][i], vz = s.f[SX_VZ][i];
+    return 0.5 * s.f[SX_MASS][i] * (vx * vx + vy * vy + vz * vz);
+}
+
+// RM = R(q)^T (integrators.f90:144-152)
+__device__ __forceinline__ void rm_of(double w, double x, double y, double z, double RM[9])
+{
+    RM[0] = w * w + x * x - y * y - z * z;   // (1,1)
+    RM[3] = 2.0 * (x * y + w * z);           // (1,2)
+    RM[6] = 2.0 * (x * z - w * y);           // (1,3)
+    RM[1] = 2.0 * (x * y - w * z);           // (2,1)
+    RM[4] = w * w - x * x + y * y - z * z;   // (2,2)
+    RM[7] = 2.0 * (y * z + w * x);           // (2,3)
+    RM[2] = 2.0 * (x * z + w * y);           // (3,1)
+    RM[5] = 2.0 * (y * z - w * x);           // (3,2)
+    RM[8] = w * w - x * x - y * y + z * z;   // (3,3)
+}
+
+__device__ __forceinline__ double kin_rot(const DState &s, int i)
+{
+    double RM[9];
+    rm_of(s.f[SX_QW][i], s.f[SX_QX][i], s.f[SX_QY][i], s.f[SX_QZ][i], RM);
+    double lx = s.f[SX_LX][i], ly = s.f[SX_LY][i], lz = s.f[SX_LZ][i];
+    double bx = (RM[0] * lx + RM[3] * ly + RM[6] * lz);
+    double by = (RM[1] * lx + RM[4] * ly + RM[7] * lz);
+    double bz = (RM[2] * lx + RM[5] * ly + RM[8] * lz);
+    return 0.5 * ((bx * bx) * s.f[SX_IXX_INV][i] + (by * by) * s.f[SX_IYY_INV][i] + (bz * bz) * s.f[SX_IZZ_INV][i]);
+}
+
+// ---------------------------------------------------------------------------
+// velocity Verlet (integrators.f90:7-108)
+// ---------------------------------------------------------------------------
+__global__ void k_vv_initial(DState s, double dt, int save)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    half_kick(s, i, dt);
+    if (save) save_old(s, i);
+    s.f[SX_RX][i] = s.f[SX_RX][i] + dt * s.f[SX_VX][i];
+    s.f[SX_RY][i] = s.f[SX_RY][i] + dt * s.f[SX_VY][i];
+    s.f[SX_RZ][i] = s.f[SX_RZ][i] + dt * s.f[SX_VZ][i];
+    wrap_pos(s, i);
+}
+
+__global__ void __launch_bounds__(256) k_vv_final(DState s, double dt, double *part)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[1] = {0.0};
+    if (i < s.n) {
+        half_kick(s, i, dt);
+        v[0] = kin_trans(s, i);
+    }
+    block_sum_store<1>(v, part);
+}
+
+// ---------------------------------------------------------------------------
+// quaternion (integrators.f90:123-218)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void quat_midpoint(const DState &s, int i, double dt, double &qdw, double &qdx, double &qdy,
+                                              double &qdz, double RM[9], double &bx, double &by, double &bz,
+                                              bool inertia_after)
+{
+    double qw0 = s.f[SX_QW][i], qx0 = s.f[SX_QX][i], qy0 = s.f[SX_QY][i], qz0 = s.f[SX_QZ][i];
+    double lx = s.f[SX_LX][i], ly = s.f[SX_LY][i], lz = s.f[SX_LZ][i];
+    double iix = s.f[SX_IXX_INV][i], iiy = s.f[SX_IYY_INV][i], iiz = s.f[SX_IZZ_INV][i];
+    double qwn = qw0, qxn = qx0, qyn = qy0, qzn = qz0;
+    for (int pass = 0; pass < 2; ++pass) {
+        double qwo = qwn, qxo = qxn, qyo = qyn, qzo = qzn;
+        rm_of(qwo, qxo, qyo, qzo, RM);
+        double wx, wy, wz;
+        bx = (RM[0] * lx + RM[3] * ly + RM[6] * lz);
+        by = (RM[1] * lx + RM[4] * ly + RM[7] * lz);
+        bz = (RM[2] * lx + RM[5] * ly + RM[8] * lz);
+        // qq_ multiplies the bracket by I^-1 in one expression; lgv_ stores L_body first:
+        // the same two roundings either way
+        wx = bx * iix; wy = by * iiy; wz = bz * iiz;
+        (void)inertia_after;
+        qdw = 0.5 * (-qxo * wx - qyo * wy - qzo * wz);
+        qdx = 0.5 * (qwo * wx + qyo * wz - qzo * wy);
+        qdy = 0.5 * (qwo * wy - qxo * wz + qzo * wx);
+        qdz = 0.5 * (qwo * wz + qxo * wy - qyo * wx);
+        qwn = qw0 + 0.5 * dt * qdw;
+        qxn = qx0 + 0.5 * dt * qdx;
+        qyn = qy0 + 0.5 * dt * qdy;
+        qzn = qz0 + 0.5 * dt * qdz;
+    }
+}
+
+__device__ __forceinline__ void quat_advance(const DState &s, int i, double dt, double qdw, double qdx, double qdy,
+                                             double qdz)
+{
+    double w = s.f[SX_QW][i] + dt * qdw, x = s.f[SX_QX][i] + dt * qdx;
+    double y = s.f[SX_QY][i] + dt * qdy, z = s.f[SX_QZ][i] + dt * qdz;
+    double qm = sqrt(w * w + x * x + y * y + z * z);
+    s.f[SX_QW][i] = w / qm; s.f[SX_QX][i] = x / qm; s.f[SX_QY][i] = y / qm; s.f[SX_QZ][i] = z / qm;
+}
+
+__device__ __forceinline__ void ang_kick(const DState &s, int i, double dt)
+{
+    s.f[SX_LX][i] = s.f[SX_LX][i] + 0.5 * dt * s.f[SX_TX][i];
+    s.f[SX_LY][i] = s.f[SX_LY][i] + 0.5 * dt * s.f[SX_TY][i];
+    s.f[SX_LZ][i] = s.f[SX_LZ][i] + 0.5 * dt * s.f[SX_TZ][i];
+}
+
+__global__ void k_qq_initial(DState s, double dt)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    ang_kick(s, i, dt);
+    double qdw, qdx, qdy, qdz, RM[9], bx, by, bz;
+    quat_midpoint(s, i, dt, qdw, qdx, qdy, qdz, RM, bx, by, bz, true);
+    quat_advance(s, i, dt, qdw, qdx, qdy, qdz);
+}
+
+__global__ void __launch_bounds__(256) k_qq_final(DState s, double dt, double *part)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[1] = {0.0};
+    if (i < s.n) {
+        ang_kick(s, i, dt);
+        v[0] = kin_rot(s, i);
+    }
+    block_sum_store<1>(v, part);
+}
+
+// ---------------------------------------------------------------------------
+// Langevin (integrators.f90:507-855)
+// ---------------------------------------------------------------------------
+struct LgvConst {
+    double dt, damp, T, e1, e2, kk1, kk2;   // e1 = exp(-damp dt), e2 = 1 - exp(-2 damp dt)
+    int temp_int, save;
+};
+
+__global__ void k_lgv_initial(DState s, LgvConst k, int step, const int *dstep)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    int st = dstep ? *dstep : step;
+    double dt = k.dt;
+    half_kick(s, i, dt);
+    if (k.save) save_old(s, i);
+    double im = s.f[SX_INV_MASS][i];
+    double rx = s.f[SX_RX][i] + 0.5 * dt * s.f[SX_VX][i];
+    double ry = s.f[SX_RY][i] + 0.5 * dt * s.f[SX_VY][i];
+    double rz = s.f[SX_RZ][i] + 0.5 * dt * s.f[SX_VZ][i];
+    double n1x, n1y, n1z, n2x, n2y, n2z;
+    sx_normal3(i + 1, st, k.temp_int, 0, n1x, n1y, n1z);
+    sx_normal3(i + 1, st, k.temp_int, 1, n2x, n2y, n2z);
+    double amp = sqrt(k.T * im * k.e2);
+    double vx = k.e1 * s.f[SX_VX][i] + amp * n1x;
+    double vy = k.e1 * s.f[SX_VY][i] + amp * n1y;
+    double vz = k.e1 * s.f[SX_VZ][i] + amp * n1z;
+    s.f[SX_VX][i] = vx; s.f[SX_VY][i] = vy; s.f[SX_VZ][i] = vz;
+    s.f[SX_RX][i] = rx + 0.5 * dt * vx;
+    s.f[SX_RY][i] = ry + 0.5 * dt * vy;
+    s.f[SX_RZ][i] = rz + 0.5 * dt * vz;
+    wrap_pos(s, i);
+    ang_kick(s, i, dt);
+    double qdw, qdx, qdy, qdz, RM[9], bx, by, bz;
+    quat_midpoint(s, i, dt, qdw, qdx, qdy, qdz, RM, bx, by, bz, false);
+    bx = k.kk2 * k.kk1 * bx + k.kk2 * sqrt(2.0 * k.damp * k.T * dt * s.f[SX_IXX][i]) * n2x;
+    by = k.kk2 * k.kk1 * by + k.kk2 * sqrt(2.0 * k.damp * k.T * dt * s.f[SX_IYY][i]) * n2y;
+    bz = k.kk2 * k.kk1 * bz + k.kk2 * sqrt(2.0 * k.damp * k.T * dt * s.f[SX_IZZ][i]) * n2z;
+    s.f[SX_LX][i] = (RM[0] * bx + RM[1] * by + RM[2] * bz);
+    s.f[SX_LY][i] = (RM[3] * bx + RM[4] * by + RM[5] * bz);
+    s.f[SX_LZ][i] = (RM[6] * bx + RM[7] * by + RM[8] * bz);
+    quat_advance(s, i, dt, qdw, qdx, qdy, qdz);
+}
+
+__global__ void __launch_bounds__(256) k_lgv_final(DState s, double dt, double *part)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (i < s.n) {
+        half_kick(s, i, dt);
+        ang_kick(s, i, dt);
+        v[1] = kin_rot(s, i);
+        v[0] = kin_trans(s, i);
+    }
+    block_sum_store<2>(v, part);
+}
+
+__global__ void k_step_inc(int *dstep) { *dstep = *dstep + 1; }
+
+// ---------------------------------------------------------------------------
+// Nose-Hoover chain (integrators.f90:222-492)
+//
+// The reference propagates the 3-link chain inside the per-particle loop and
+// re-sums sum_k m_k v_k^2 over all particles at every call (O(N^2)).  Here a
+// single lane walks the particles in order carrying that sum incrementally
+// (the velocity of particle i only changes by the scaling and kick applied
+// in its own turn), while the work-group stages per-particle data through
+// LDS and applies the resulting updates in parallel.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double exprel_d(double x)
+{
+    const double g0 = 1.0 / 1, g1 = 1.0 / 6, g2 = 1.0 / 120, g3 = 1.0 / 5040, g4 = 1.0 / 362880;
+    if (fabs(x) > 0.01) return (exp(x) - 1.0) / x;
+    double h = (x / 2) * (x / 2);
+    double f = g4;
+    f = f * h + g3;
+    f = f * h + g2;
+    f = f * h + g1;
+    f = f * h + g0;
+    return exp(x / 2) * f;
+}
+
+// u4_propagator (integrators.f90:274-301) with the j=1 sum passed in
+__device__ __forceinline__ void u4(double t, int dir, double sumv, double ndofT, double T, const double hb[3],
+                                   double p[3])
+{
+    for (int q = 0; q < 3; ++q) {
+        int j = dir < 0 ? 3 - q : 1 + q;
+        double gj = (j == 1) ? (sumv - ndofT) : ((p[j - 2] * p[j - 2] / hb[j - 2]) - T);
+        if (j == 3) {
+            p[2] = p[2] + t * gj;
+        } else {
+            double x = t * p[j] / hb[j];
+            double c = exprel_d(-x);
+            p[j - 1] = p[j - 1] * exp(-x) + t * gj * c;
+        }
+    }
+}
+
+#define NHT_CHUNK 256
+// phase 0: nht_initial (chain, scale, chain, kick, drift); phase 1: nht_final
+// (kick, chain, scale, chain).  scal holds HB, eta, p_eta; sum of m v^2 is
+// computed by this kernel first.
+__global__ void __launch_bounds__(NHT_CHUNK) k_nht(DState s, double dt, int phase, int save, double *scal, double ndof,
+                                                  double *part)
+{
+    __shared__ double sv[3][NHT_CHUNK], sa[3][NHT_CHUNK], sm[NHT_CHUNK], sfac[NHT_CHUNK];
+    __shared__ double red[NHT_CHUNK];
+    __shared__ double sum_sh;
+    int t = threadIdx.x;
+    int n = s.n;
+    // sum of m |v|^2 over all particles before any of this call's updates
+    // (in the final step each particle is kicked in its own turn)
+    double acc = 0.0;
+    for (int i = t; i < n; i += NHT_CHUNK) {
+        double vx = s.f[SX_VX][i], vy = s.f[SX_VY][i], vz = s.f[SX_VZ][i];
+        acc += s.f[SX_MASS][i] * (vx * vx + vy * vy + vz * vz);
+    }
+    red[t] = acc;
+    __syncthreads();
+    for (int o = NHT_CHUNK / 2; o > 0; o >>= 1) {
+        if (t < o) red[t] = red[t] + red[t + o];
+        __syncthreads();
+    }
+    if (t == 0) sum_sh = red[0];
+    __syncthreads();
+    double T = scal[SX_TARGET_TEMP];
+    double hb[3] = {scal[SX_HB0], scal[SX_HB1], scal[SX_HB2]};
+    double p[3] = {scal[SX_PETA0], scal[SX_PETA1], scal[SX_PETA2]};
+    double eta[3] = {scal[SX_ETA0], scal[SX_ETA1], scal[SX_ETA2]};
+    double ndofT = ndof * T;
+    double tq = dt / 4.0;
+    double sumv = sum_sh;
+    double ke = 0.0;
+    for (int c0 = 0; c0 < n; c0 += NHT_CHUNK) {
+        int i = c0 + t;
+        if (i < n) {
+            double vx = s.f[SX_VX][i], vy = s.f[SX_VY][i], vz = s.f[SX_VZ][i];
+            double im = s.f[SX_INV_MASS][i];
+            double ax = 0.5 * dt * (s.f[SX_FX][i] * im), ay = 0.5 * dt * (s.f[SX_FY][i] * im);
+            double az = 0.5 * dt * (s.f[SX_FZ][i] * im);
+            sv[0][t] = vx; sv[1][t] = vy; sv[2][t] = vz;
+            sa[0][t] = ax; sa[1][t] = ay; sa[2][t] = az;
+            sm[t] = s.f[SX_MASS][i];
+        }
+        __syncthreads();
+        if (t == 0) {
+            int cnt = min(NHT_CHUNK, n - c0);
+            for (int q = 0; q < cnt; ++q) {
+                double m = sm[q];
+                double vx = sv[0][q], vy = sv[1][q], vz = sv[2][q];
+                if (phase == 1) {
+                    // kick first (nht_final_step :374-376)
+                    double old = m * (vx * vx + vy * vy + vz * vz);
+                    vx = vx + sa[0][q]; vy = vy + sa[1][q]; vz = vz + sa[2][q];
+                    sumv = sumv - old + m * (vx * vx + vy * vy + vz * vz);
+                }
+                u4(tq, -1, sumv, ndofT, T, hb, p);
+                double fac = exp(-0.5 * dt * p[0] / hb[0]);
+                double before = m * (vx * vx + vy * vy + vz * vz);
+                vx = vx * fac; vy = vy * fac; vz = vz * fac;
+                sumv = sumv - before + m * (vx * vx + vy * vy + vz * vz);
+                for (int k = 0; k < 3; ++k) eta[k] = eta[k] + 0.5 * dt * p[k] / hb[k];
+                u4(tq, +1, sumv, ndofT, T, hb, p);
+                if (phase == 0) {
+                    double mid = m * (vx * vx + vy * vy + vz * vz);
+                    vx = vx + sa[0][q]; vy = vy + sa[1][q]; vz = vz + sa[2][q];
+                    sumv = sumv - mid + m * (vx * vx + vy * vy + vz * vz);
+                }
+                sv[0][q] = vx; sv[1][q] = vy; sv[2][q] = vz;
+                sfac[q] = fac;
+            }
+        }
+        __syncthreads();
+        if (i < n) {
+            double vx = sv[0][t], vy = sv[1][t], vz = sv[2][t];
+            s.f[SX_VX][i] = vx; s.f[SX_VY][i] = vy; s.f[SX_VZ][i] = vz;
+            if (phase == 0) {
+                if (save) save_old(s, i);
+                s.f[SX_RX][i] = s.f[SX_RX][i] + dt * vx;
+                s.f[SX_RY][i] = s.f[SX_RY][i] + dt * vy;
+                s.f[SX_RZ][i] = s.f[SX_RZ][i] + dt * vz;
+                wrap_pos(s, i);
+            } else {
+                ke += 0.5 * sm[t] * (vx * vx + vy * vy + vz * vz);
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        scal[SX_PETA0] = p[0]; scal[SX_PETA1] = p[1]; scal[SX_PETA2] = p[2];
+        scal[SX_ETA0] = eta[0]; scal[SX_ETA1] = eta[1]; scal[SX_ETA2] = eta[2];
+    }
+    if (phase == 1) {
+        red[t] = ke;
+        __syncthreads();
+        for (int o = NHT_CHUNK / 2; o > 0; o >>= 1) {
+            if (t < o) red[t] = red[t] + red[t + o];
+            __syncthreads();
+        }
+        if (t == 0) scal[SX_KIN_ENG] = red[0];
+    }
+}
+
+__global__ void k_nht_hb(double *scal, double ndof, double coupling, int has_coupling)
+{
+    if (threadIdx.x != 0) return;
+    double T = scal[SX_TARGET_TEMP];
+    if (has_coupling) {
+        scal[SX_HB0] = scal[SX_HB1] = scal[SX_HB2] = coupling;
+    } else {
+        scal[SX_HB0] = scal[SX_HB1] = scal[SX_HB2] = T * (2.0 * 2.0);
+        scal[SX_HB0] = ndof * T * (2.0 * 2.0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// RATTLE: one molecule per wavefront, bonds swept in the reference's order
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, SX_WAVE); }
+
+template <int PHASE>
+__global__ void __launch_bounds__(64) k_rattle(DState s, double dt, int nmol, const int *atom_start, const int *atoms,
+                                              const int *bond_start, const int *bond_a, const int *bond_b,
+                                              const double *bond_len, double *vir_part, int *err)
+{
+    int m = blockIdx.x;
+    int lane = threadIdx.x;
+    double vir = 0.0;
+    if (m < nmol) {
+        int a0 = atom_start[m], na = atom_start[m + 1] - a0;
+        int b0 = bond_start[m], nb = bond_start[m + 1] - b0;
+        // lane k holds atom k of the molecule (na <= 64)
+        int g = lane < na ? atoms[a0 + lane] : 0;
+        double rx = 0, ry = 0, rz = 0, vx = 0, vy = 0, vz = 0, ox = 0, oy = 0, oz = 0, im = 0;
+        if (lane < na) {
+            rx = s.f[SX_RX][g]; ry = s.f[SX_RY][g]; rz = s.f[SX_RZ][g];
+            vx = s.f[SX_VX][g]; vy = s.f[SX_VY][g]; vz = s.f[SX_VZ][g];
+            ox = s.f[SX_RX_OLD][g]; oy = s.f[SX_RY_OLD][g]; oz = s.f[SX_RZ_OLD][g];
+            im = s.f[SX_INV_MASS][g];
+        }
+        bool moved = true;
+        const double r_tol = 1.0e-5;
+        int failed = 0;
+        for (int rattle = 0; rattle < 100; ++rattle) {
+            if (!__any(lane < na && moved)) break;
+            for (int q = 0; q < nb; ++q) {
+                int la = bond_a[b0 + q], lb = bond_b[b0 + q];
+                double len = bond_len[b0 + q];
+                bool ma = __shfl((int)moved, la, SX_WAVE) != 0, mb = __shfl((int)moved, lb, SX_WAVE) != 0;
+                if (!(ma || mb)) continue;
+                double ax = shfl_d(rx, la), ay = shfl_d(ry, la), az = shfl_d(rz, la);
+                double bx = shfl_d(rx, lb), by = shfl_d(ry, lb), bz = shfl_d(rz, lb);
+                double R[3] = {ax - bx, ay - by, az - bz};
+                if (s.periodic)
+                    for (int k = 0; k < 3; ++k) R[k] = sx_mimg(R[k], s.box[k]);
+                double ima = shfl_d(im, la), imb = shfl_d(im, lb);
+                if (PHASE == 0) {
+                    double chi = len * len - (R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+                    if (fabs(chi) > 2 * r_tol * (len * len)) {
+                        double Ro[3] = {shfl_d(ox, la) - shfl_d(ox, lb), shfl_d(oy, la) - shfl_d(oy, lb),
+                                        shfl_d(oz, la) - shfl_d(oz, lb)};
+                        if (s.periodic)
+                            for (int k = 0; k < 3; ++k) Ro[k] = sx_mimg(Ro[k], s.box[k]);
+                        double vdot = R[0] * Ro[0] + R[1] * Ro[1] + R[2] * Ro[2];
+                        if (vdot < r_tol * (len * len)) { failed = 1; break; }
+                        double L = chi / (2.0 * vdot * (ima + imb));
+                        vir = vir + (L * (len * len)) / (dt * dt);
+                        if (lane == la) {
+                            double c = L * ima;
+                            rx = rx + c * Ro[0]; ry = ry + c * Ro[1]; rz = rz + c * Ro[2];
+                            vx = vx + c * Ro[0] / dt; vy = vy + c * Ro[1] / dt; vz = vz + c * Ro[2] / dt;
+                        }
+                        if (lane == lb) {
+                            double c = L * imb;
+                            rx = rx - c * Ro[0]; ry = ry - c * Ro[1]; rz = rz - c * Ro[2];
+                            vx = vx - c * Ro[0] / dt; vy = vy - c * Ro[1] / dt; vz = vz - c * Ro[2] / dt;
+                        }
+                        if (lane == la || lane == lb) moved = true;
+                    } else {
+                        if (lane == la || lane == lb) moved = false;
+                    }
+                } else {
+                    double V[3] = {shfl_d(vx, la) - shfl_d(vx, lb), shfl_d(vy, la) - shfl_d(vy, lb),
+                                   shfl_d(vz, la) - shfl_d(vz, lb)};
+                    double vdot = R[0] * V[0] + R[1] * V[1] + R[2] * V[2];
+                    double L = -vdot / ((ima + imb) * (len * len));
+                    if (fabs(L) > r_tol) {
+                        vir = vir + (L * (len * len)) / dt;
+                        if (lane == la) {
+                            double c = L * ima;
+                            vx = vx + c * R[0]; vy = vy + c * R[1]; vz = vz + c * R[2];
+                        }
+                        if (lane == lb) {
+                            double c = L * imb;
+                            vx = vx - c * R[0]; vy = vy - c * R[1]; vz = vz - c * R[2];
+                        }
+                        if (lane == la || lane == lb) moved = true;
+                    } else {
+                        if (lane == la || lane == lb) moved = false;
+                    }
+                }
+            }
+            if (failed) break;
+        }
+        if (failed && lane == 0) atomicCAS(err, 0, DERR_CONSTRAINT);
+        if (lane < na) {
+            s.f[SX_RX][g] = rx; s.f[SX_RY][g] = ry; s.f[SX_RZ][g] = rz;
+            s.f[SX_VX][g] = vx; s.f[SX_VY][g] = vy; s.f[SX_VZ][g] = vz;
+        }
+    }
+    if (lane == 0) vir_part[blockIdx.x] = vir;
+}
+
+// ---------------------------------------------------------------------------
+// bonded: per-term evaluation into a buffer, per-atom gather
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void bvec(const DState &s, int a, int b, double v[3])
+{
+    v[0] = s.f[SX_RX][b] - s.f[SX_RX][a];
+    v[1] = s.f[SX_RY][b] - s.f[SX_RY][a];
+    v[2] = s.f[SX_RZ][b] - s.f[SX_RZ][a];
+    if (s.periodic)
+        for (int k = 0; k < 3; ++k) v[k] = sx_mimg(v[k], s.box[k]);
+}
+
+__global__ void k_bonded_terms(DState s, int kind, int count, const int *idx, double coeff, double angle,
+                               double cos0, double *out)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int *q = idx + 4 * t;
+    double *o = out + 13L * t;
+    if (kind <= 2) {
+        double VI[3], VJ[3];
+        bvec(s, q[0], q[1], VI);
+        bvec(s, q[1], q[2], VJ);
+        double CC11 = VI[0] * VI[0] + VI[1] * VI[1] + VI[2] * VI[2];
+        double CC12 = VI[0] * VJ[0] + VI[1] * VJ[1] + VI[2] * VJ[2];
+        double CC22 = VJ[0] * VJ[0] + VJ[1] * VJ[1] + VJ[2] * VJ[2];
+        double prefac = 1.0 / sqrt(CC11 * CC22);
+        double fac = CC12, fac1 = fac / CC22, fac2 = fac / CC11;
+        double strength, pe;
+        if (kind == 1) {
+            pe = coeff * (1.0 - prefac * fac);
+            strength = coeff;
+        } else {
+            double cos_theta = prefac * fac;
+            strength = coeff * (cos_theta - cos0);
+            pe = (coeff / 2.0) * (cos_theta - cos0) * (cos_theta - cos0);
+        }
+        o[0] = pe;
+        for (int k = 0; k < 3; ++k) {
+            o[1 + k] = strength * prefac * (fac2 * VI[k] - VJ[k]);
+            o[4 + k] = strength * prefac * (fac1 * VJ[k] - fac2 * VI[k] + VJ[k] - VI[k]);
+            o[7 + k] = -strength * prefac * (fac1 * VJ[k] - VI[k]);
+            o[10 + k] = 0.0;
+        }
+    } else {
+        double VI[3], VJ[3], VK[3];
+        bvec(s, q[0], q[1], VI);
+        bvec(s, q[1], q[2], VJ);
+        bvec(s, q[2], q[3], VK);
+        double CC11 = VI[0] * VI[0] + VI[1] * VI[1] + VI[2] * VI[2];
+        double CC12 = VI[0] * VJ[0] + VI[1] * VJ[1] + VI[2] * VJ[2];
+        double CC13 = VI[0] * VK[0] + VI[1] * VK[1] + VI[2] * VK[2];
+        double CC22 = VJ[0] * VJ[0] + VJ[1] * VJ[1] + VJ[2] * VJ[2];
+        double CC23 = VJ[0] * VK[0] + VJ[1] * VK[1] + VJ[2] * VK[2];
+        double CC33 = VK[0] * VK[0] + VK[1] * VK[1] + VK[2] * VK[2];
+        double DD12 = CC11 * CC22 - CC12 * CC12;
+        double DD23 = CC22 * CC33 - CC23 * CC23;
+        double prefac = 1.0 / sqrt(DD23 * DD12);
+        double fac = CC23 * CC12 - CC13 * CC22;
+        double fac1 = fac / DD23, fac2 = fac / DD12;
+        double strength, pe;
+        if (kind == 3) {
+            pe = coeff * (1.0 + prefac * fac);
+            strength = coeff;
+        } else {
+            double vixvj[3];
+            cross3(VI, VJ, vixvj);
+            double fac_sin = vixvj[0] * VK[0] + vixvj[1] * VK[1] + vixvj[2] * VK[2];
+            double cos_phi = -prefac * fac;
+            double sin_phi = prefac * fac_sin * sqrt(VJ[0] * VJ[0] + VJ[1] * VJ[1] + VJ[2] * VJ[2]);
+            double sg = atan2(sin_phi, cos_phi);
+            strength = coeff * (cos(sg) - cos0);
+            pe = (coeff / 2.0) * (cos(sg) - cos0) * (cos(sg) - cos0);
+        }
+        o[0] = pe;
+        for (int k = 0; k < 3; ++k) {
+            o[10 + k] = -strength * prefac * (CC12 * VJ[k] - CC22 * VI[k] - fac1 * (CC22 * VK[k] - CC23 * VJ[k]));
+            o[7 + k] = -strength * prefac * (CC12 * VK[k] - CC12 * VJ[k] + CC23 * VI[k] + CC22 * VI[k] - 2.0 * CC13 * VJ[k]
+                                            - fac2 * (CC11 * VJ[k] - CC12 * VI[k]) - fac1 * (CC33 * VJ[k] - CC22 * VK[k] - CC23 * VK[k] + CC23 * VJ[k]));
+            o[4 + k] = -strength * prefac * (-CC12 * VK[k] + CC23 * VJ[k] - CC23 * VI[k] - CC22 * VK[k] + 2.0 * CC13 * VJ[k]
+                                            - fac2 * (CC22 * VI[k] - CC11 * VJ[k] - CC12 * VJ[k] + CC12 * VI[k]) - fac1 * (-CC33 * VJ[k] + CC23 * VK[k]));
+            o[1 + k] = -strength * prefac * (-CC23 * VJ[k] + CC22 * VK[k] - fac2 * (-CC22 * VI[k] + CC12 * VJ[k]));
+        }
+    }
+    (void)angle;
+}
+
+// atom_term entries: (global term slot * 4 + position), slots index a
+// concatenation of all registered term buffers
+__global__ void __launch_bounds__(256) k_bonded_gather(DState s, const int *atom_start, const int *atom_term,
+                                                       const double *const *bufs, const long *buf_off, int nbufs,
+                                                       double *part)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[1] = {0.0};
+    if (i < s.n) {
+        double fx = 0, fy = 0, fz = 0;
+        for (int q = atom_start[i]; q < atom_start[i + 1]; ++q) {
+            int code = atom_term[q];
+            long slot = code >> 2;
+            int pos = code & 3;
+            int b = 0;
+            while (b + 1 < nbufs && slot >= buf_off[b + 1]) ++b;
+            const double *o = bufs[b] + 13L * (slot - buf_off[b]);
+            fx = fx + o[1 + 3 * pos]; fy = fy + o[2 + 3 * pos]; fz = fz + o[3 + 3 * pos];
+            if (pos == 0) v[0] = v[0] + o[0];
+        }
+        s.f[SX_FX][i] = s.f[SX_FX][i] + fx;
+        s.f[SX_FY][i] = s.f[SX_FY][i] + fy;
+        s.f[SX_FZ][i] = s.f[SX_FZ][i] + fz;
+    }
+    block_sum_store<1>(v, part);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static void reduce_into(sx_ctx *c, int nb, int width, std::initializer_list<int> ids, int accumulate)
+{
+    std::vector<int> v(ids);
+    sx_reduce_to_scalar(c, nb, width, v.data(), (int)v.size(), accumulate);
+}
+
+void sx_vv_initial_impl(sx_ctx *c, double dt, bool save)
+{
+    hipLaunchKernelGGL(k_vv_initial, dim3(sx_blocks(c->n, 256)), dim3(256), 0, c->stream, c->dstate(), dt, save ? 1 : 0);
+}
+
+void sx_vv_final_impl(sx_ctx *c, double dt)
+{
+    int nb = sx_blocks(c->n, 256);
+    sx_ensure_part(c, nb, 1);
+    hipLaunchKernelGGL(k_vv_final, dim3(nb), dim3(256), 0, c->stream, c->dstate(), dt, c->d_part);
+    reduce_into(c, nb, 1, {SX_KIN_ENG}, 0);
+}
+
+void sx_qq_initial_impl(sx_ctx *c, double dt)
+{
+    hipLaunchKernelGGL(k_qq_initial, dim3(sx_blocks(c->n, 256)), dim3(256), 0, c->stream, c->dstate(), dt);
+}
+
+void sx_qq_final_impl(sx_ctx *c, double dt)
+{
+    int nb = sx_blocks(c->n, 256);
+    sx_ensure_part(c, nb, 1);
+    hipLaunchKernelGGL(k_qq_final, dim3(nb), dim3(256), 0, c->stream, c->dstate(), dt, c->d_part);
+    reduce_into(c, nb, 1, {SX_KIN_ROT}, 0);
+}
+
+static LgvConst lgv_const(sx_ctx *c, double dt, double damp, bool save)
+{
+    LgvConst k;
+    k.dt = dt;
+    k.damp = damp;
+    k.T = c->h_scal[SX_TARGET_TEMP];
+    k.e1 = std::exp(-damp * dt);
+    k.e2 = 1 - std::exp(-2.0 * damp * dt);
+    k.kk1 = 1.0 - (damp * dt) / 2.0;
+    k.kk2 = 1.0 / (1.0 + (damp * dt) / 2.0);
+    k.temp_int = (int)(k.T * 1000.0);
+    k.save = save ? 1 : 0;
+    return k;
+}
+
+void sx_lgv_initial_impl(sx_ctx *c, double dt, int step, const int *dstep, double damping, bool constrained)
+{
+    LgvConst k = lgv_const(c, dt, damping, constrained);
+    hipLaunchKernelGGL(k_lgv_initial, dim3(sx_blocks(c->n, 256)), dim3(256), 0, c->stream, c->dstate(), k, step, dstep);
+    if (constrained) sx_rattle_impl(c, dt, 0);
+}
+
+void sx_lgv_final_impl(sx_ctx *c, double dt, double damping, bool constrained)
+{
+    (void)damping;
+    int nb = sx_blocks(c->n, 256);
+    sx_ensure_part(c, nb, 2);
+    hipLaunchKernelGGL(k_lgv_final, dim3(nb), dim3(256), 0, c->stream, c->dstate(), dt, c->d_part);
+    reduce_into(c, nb, 2, {SX_KIN_ENG, SX_KIN_ROT}, 0);
+    if (constrained) sx_rattle_impl(c, dt, 1);
+}
+
+void sx_nht_impl(sx_ctx *c, double dt, const double *coupling, int phase, bool save)
+{
+    double ndof = (double)(3 * (c->n / c->nrigid) - 3);
+    hipLaunchKernelGGL(k_nht_hb, dim3(1), dim3(64), 0, c->stream, c->d_scal, ndof, coupling ? *coupling : 0.0,
+                       coupling ? 1 : 0);
+    sx_ensure_part(c, 1, 1);
+    hipLaunchKernelGGL(k_nht, dim3(1), dim3(NHT_CHUNK), 0, c->stream, c->dstate(), dt, phase, save ? 1 : 0, c->d_scal,
+                       ndof, c->d_part);
+}
+
+void sx_rattle_impl(sx_ctx *c, double dt, int phase)
+{
+    Molecules &m = c->mol;
+    if (m.nmol == 0) return;
+    sx_ensure_part(c, m.nmol, 1);
+    if (phase == 0)
+        hipLaunchKernelGGL(k_rattle<0>, dim3(m.nmol), dim3(64), 0, c->stream, c->dstate(), dt, m.nmol, m.atom_start,
+                           m.atoms, m.bond_start, m.bond_a, m.bond_b, m.bond_len, c->d_part, c->d_err);
+    else
+        hipLaunchKernelGGL(k_rattle<1>, dim3(m.nmol), dim3(64), 0, c->stream, c->dstate(), dt, m.nmol, m.atom_start,
+                           m.atoms, m.bond_start, m.bond_a, m.bond_b, m.bond_len, c->d_part, c->d_err);
+    reduce_into(c, m.nmol, 1, {SX_VIRIAL_BONDED}, 1);
+}
+
+
+void sx_bonded_impl(sx_ctx *c)
+{
+    if (c->bonded.empty()) return;
+    const double PI_REF = 3.1415926535897932384626433832795;
+    for (auto &b : c->bonded) {
+        double cos0 = std::cos(PI_REF - (PI_REF / 180.0) * b.angle);
+        hipLaunchKernelGGL(k_bonded_terms, dim3(sx_blocks(b.count, 256)), dim3(256), 0, c->stream, c->dstate(), b.kind,
+                           b.count, b.idx, b.coeff, b.angle, cos0, b.out);
+    }
+    int nb = sx_blocks(c->n, 256);
+    sx_ensure_part(c, nb, 1);
+    hipLaunchKernelGGL(k_bonded_gather, dim3(nb), dim3(256), 0, c->stream, c->dstate(), c->d_atom_term_start,
+                       c->d_atom_term, (const double *const *)c->d_bbufs, c->d_boff, (int)c->bonded.size(),
+                       c->d_part);
+    reduce_into(c, nb, 1, {SX_POT_ENG}, 1);
+}
+
+static void rebuild_bonded_index(sx_ctx *c)
+{
+    int n = c->n;
+    std::vector<std::vector<int>> per(n);
+    std::vector<const double *> bufs;
+    std::vector<long> off;
+    long slot0 = 0;
+    for (auto &b : c->bonded) {
+        std::vector<int> h(4 * (size_t)b.count);
+        HIPCHK(hipMemcpy(h.data(), b.idx, sizeof(int) * 4 * b.count, hipMemcpyDeviceToHost));
+        int npos = b.kind <= 2 ? 3 : 4;
+        for (int t = 0; t < b.count; ++t)
+            for (int p = 0; p < npos; ++p) per[h[4 * t + p]].push_back((int)(((slot0 + t) << 2) | p));
+        bufs.push_back(b.out);
+        off.push_back(slot0);
+        slot0 += b.count;
+    }
+    std::vector<int> start(n + 1, 0), items;
+    for (int i = 0; i < n; ++i) {
+        start[i + 1] = start[i] + (int)per[i].size();
+        for (int v : per[i]) items.push_back(v);
+    }
+    if (c->d_atom_term_start) hipFree(c->d_atom_term_start);
+    if (c->d_atom_term) hipFree(c->d_atom_term);
+    HIPCHK(hipMalloc(&c->d_atom_term_start, sizeof(int) * (n + 1)));
+    HIPCHK(hipMalloc(&c->d_atom_term, sizeof(int) * std::max<size_t>(1, items.size())));
+    HIPCHK(hipMemcpy(c->d_atom_term_start, start.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    if (!items.empty()) HIPCHK(hipMemcpy(c->d_atom_term, items.data(), sizeof(int) * items.size(), hipMemcpyHostToDevice));
+    if (c->d_bbufs) hipFree(c->d_bbufs);
+    if (c->d_boff) hipFree(c->d_boff);
+    HIPCHK(hipMalloc(&c->d_bbufs, sizeof(double *) * bufs.size()));
+    HIPCHK(hipMalloc(&c->d_boff, sizeof(long) * off.size()));
+    HIPCHK(hipMemcpy(c->d_bbufs, bufs.data(), sizeof(double *) * bufs.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_boff, off.data(), sizeof(long) * off.size(), hipMemcpyHostToDevice));
+}
+
+static void need_temp(sx_ctx *c)
+{
+    if (!c->have_target_temp) throw sx_error{SX_ERR_NOTEMP, "Error: Target temperature not set."};
+}
+
+extern "C" {
+
+int sx_vv_initial_step(sx_ctx *c, double dt) { return guard(c, [&] { need_system(c); sx_vv_initial_impl(c, dt, false); }); }
+int sx_vv_final_step(sx_ctx *c, double dt) { return guard(c, [&] { need_system(c); sx_vv_final_impl(c, dt); }); }
+int sx_vvc_initial_step(sx_ctx *c, double dt)
+{
+    return guard(c, [&] { need_system(c); sx_vv_initial_impl(c, dt, true); sx_rattle_impl(c, dt, 0); sx_check_device_error(c); });
+}
+int sx_vvc_final_step(sx_ctx *c, double dt)
+{
+    return guard(c, [&] { need_system(c); sx_vv_final_impl(c, dt); sx_rattle_impl(c, dt, 1); sx_check_device_error(c); });
+}
+int sx_qq_initial_step(sx_ctx *c, double dt) { return guard(c, [&] { need_system(c); sx_qq_initial_impl(c, dt); }); }
+int sx_qq_final_step(sx_ctx *c, double dt) { return guard(c, [&] { need_system(c); sx_qq_final_impl(c, dt); }); }
+
+int sx_nht_initial_step(sx_ctx *c, double dt, const double *coupling)
+{
+    return guard(c, [&] { need_system(c); need_temp(c); sx_nht_impl(c, dt, coupling, 0, false); });
+}
+int sx_nht_final_step(sx_ctx *c, double dt, const double *coupling)
+{
+    return guard(c, [&] { need_system(c); sx_nht_impl(c, dt, coupling, 1, false); });
+}
+int sx_nhtc_initial_step(sx_ctx *c, double dt, const double *coupling)
+{
+    return guard(c, [&] {
+        need_system(c); need_temp(c);
+        sx_nht_impl(c, dt, coupling, 0, true);
+        sx_rattle_impl(c, dt, 0);
+        sx_check_device_error(c);
+    });
+}
+int sx_nhtc_final_step(sx_ctx *c, double dt, const double *coupling)
+{
+    return guard(c, [&] { need_system(c); sx_nht_impl(c, dt, coupling, 1, false); sx_rattle_impl(c, dt, 1); sx_check_device_error(c); });
+}
+
+int sx_lgv_initial_step(sx_ctx *c, double dt, int step, double damping)
+{
+    return guard(c, [&] { need_system(c); need_temp(c); sx_lgv_initial_impl(c, dt, step, nullptr, damping, false); });
+}
+int sx_lgv_final_step(sx_ctx *c, double dt, double damping)
+{
+    return guard(c, [&] { need_system(c); sx_lgv_final_impl(c, dt, damping, false); });
+}
+int sx_lgvc_initial_step(sx_ctx *c, double dt, int step, double damping)
+{
+    return guard(c, [&] {
+        need_system(c);
+        if (c->h_scal[SX_TARGET_TEMP] <= 1e-3) throw sx_error{SX_ERR_NOTEMP, "Error: Target temperature too small."};
+        sx_lgv_initial_impl(c, dt, step, nullptr, damping, true);
+        sx_check_device_error(c);
+    });
+}
+int sx_lgvc_final_step(sx_ctx *c, double dt, double damping)
+{
+    return guard(c, [&] { need_system(c); sx_lgv_final_impl(c, dt, damping, true); sx_check_device_error(c); });
+}
+
+int sx_apply_constraints_a(sx_ctx *c, double dt)
+{
+    return guard(c, [&] { need_system(c); sx_rattle_impl(c, dt, 0); sx_check_device_error(c); });
+}
+int sx_apply_constraints_b(sx_ctx *c, double dt)
+{
+    return guard(c, [&] { need_system(c); sx_rattle_impl(c, dt, 1); sx_check_device_error(c); });
+}
+
+// set_num_constraints + add_constraint; molecules = connected components of the
+// bond graph, each keeping its bonds in the global order
+int sx_set_constraints(sx_ctx *c, int nc, const int *bbi1, const int *bbj1, const double *len)
+{
+    return guard(c, [&] {
+        need_system(c);
+        if (nc < 0) throw sx_error{SX_ERR_ARG, "nc < 0"};
+        int n = c->n;
+        std::vector<int> parent(n);
+        std::iota(parent.begin(), parent.end(), 0);
+        auto find = [&](int x) {
+            while (parent[x] != x) { parent[x] = parent[parent[x]]; x = parent[x]; }
+            return x;
+        };
+        for (int b = 0; b < nc; ++b) {
+            int i = bbi1[b] - 1, j = bbj1[b] - 1;
+            if (i < 0 || i >= n || j < 0 || j >= n || i == j)
+                throw sx_error{SX_ERR_ARG, "Error: Constraint index does not exist in system"};
+            int a = find(i), d = find(j);
+            if (a != d) parent[a] = d;
+        }
+        // molecules in order of their first bond
+        std::vector<int> mol_of_root(n, -1), mol_first;
+        std::vector<std::vector<int>> mb;
+        for (int b = 0; b < nc; ++b) {
+            int r = find(bbi1[b] - 1);
+            if (mol_of_root[r] < 0) { mol_of_root[r] = (int)mb.size(); mb.emplace_back(); }
+            mb[mol_of_root[r]].push_back(b);
+        }
+        std::vector<int> atom_start{0}, atoms, bond_start{0}, ba, bb, gidx;
+        std::vector<double> bl;
+        int max_atoms = 0;
+        for (auto &bonds : mb) {
+            std::vector<int> loc;  // atoms of the molecule in first-appearance order
+            auto local = [&](int g) {
+                for (size_t k = 0; k < loc.size(); ++k)
+                    if (loc[k] == g) return (int)k;
+                loc.push_back(g);
+                return (int)loc.size() - 1;
+            };
+            for (int b : bonds) {
+                int la = local(bbi1[b] - 1), lb = local(bbj1[b] - 1);
+                ba.push_back(la); bb.push_back(lb); bl.push_back(len[b]); gidx.push_back(b);
+            }
+            if ((int)loc.size() > SX_WAVE)
+                throw sx_error{SX_ERR_ARG, "a constrained molecule has more than 64 atoms"};
+            max_atoms = std::max(max_atoms, (int)loc.size());
+            for (int g : loc) atoms.push_back(g);
+            atom_start.push_back((int)atoms.size());
+            bond_start.push_back((int)ba.size());
+        }
+        Molecules &m = c->mol;
+        void *old[] = {m.atom_start, m.atoms, m.bond_start, m.bond_a, m.bond_b, m.bond_len, m.bond_gidx};
+        for (void *p : old)
+            if (p) hipFree(p);
+        m = Molecules();
+        m.nmol = (int)mb.size();
+        m.max_atoms = max_atoms;
+        auto up = [&](auto &vec, auto **dst) {
+            using T = typename std::decay<decltype(vec[0])>::type;
+            HIPCHK(hipMalloc(dst, sizeof(T) * std::max<size_t>(1, vec.size())));
+            if (!vec.empty()) HIPCHK(hipMemcpy(*dst, vec.data(), sizeof(T) * vec.size(), hipMemcpyHostToDevice));
+        };
+        up(atom_start, &m.atom_start);
+        up(atoms, &m.atoms);
+        up(bond_start, &m.bond_start);
+        up(ba, &m.bond_a);
+        up(bb, &m.bond_b);
+        up(bl, &m.bond_len);
+        up(gidx, &m.bond_gidx);
+        c->nc = nc;
+        c->have_graph = false;
+    });
+}
+
+int sx_set_bonded(sx_ctx *c, int kind, int count, const int *idx1, double coeff, double angle)
+{
+    return guard(c, [&] {
+        need_system(c);
+        if (kind < 1 || kind > 4 || count < 0) throw sx_error{SX_ERR_ARG, "bad bonded kind"};
+        int npos = kind <= 2 ? 3 : 4;
+        std::vector<int> h(4 * (size_t)count, 0);
+        for (int t = 0; t < count; ++t)
+            for (int p = 0; p < npos; ++p) {
+                int a = idx1[npos * t + p] - 1;
+                if (a < 0 || a >= c->n) throw sx_error{SX_ERR_ARG, "Error: Atom index does not exist in system"};
+                h[4 * t + p] = a;
+            }
+        Bonded b;
+        b.kind = kind; b.count = count; b.coeff = coeff; b.angle = angle;
+        HIPCHK(hipMalloc(&b.idx, sizeof(int) * std::max(1, 4 * count)));
+        HIPCHK(hipMalloc(&b.out, sizeof(double) * std::max(1, 13 * count)));
+        if (count) HIPCHK(hipMemcpy(b.idx, h.data(), sizeof(int) * 4 * count, hipMemcpyHostToDevice));
+        c->bonded.push_back(b);
+        rebuild_bonded_index(c);
+        c->have_graph = false;
+    });
+}
+
+int sx_bonded_forces(sx_ctx *c)
+{
+    return guard(c, [&] { need_system(c); sx_bonded_impl(c); });
+}
+
+}  // extern "C"

@@ -1,0 +1,253 @@
+// Internal declarations of the gfx950 implementation of sucrose.h.
+//
+// Numerics: every translation unit is compiled with -ffp-contract=off.  The
+// state is FP64 structure-of-arrays in HBM; routines that reproduce a
+// discrete decision of the reference (neighbour-list inclusion, cutoff masks,
+// the ECP Brent branches, RATTLE tolerances) evaluate the same FP64
+// expressions in the same order as the reference, so those decisions agree
+// exactly.  FP64 add/mul/div/sqrt are IEEE correctly rounded on gfx950;
+// exp/log/sin/cos come from the device math library (within 1 ulp of the
+// host libm the reference links).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "../../include/sucrose.h"
+
+#define SX_WAVE 64
+
+// ---------------------------------------------------------------------------
+// error plumbing
+// ---------------------------------------------------------------------------
+struct sx_error {
+    int code;
+    std::string msg;
+};
+
+#define HIPCHK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            throw sx_error{SX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+// device error word codes (first error wins)
+enum { DERR_NONE = 0, DERR_NLIST_CAP = 1, DERR_ROW_CAP = 2, DERR_CONSTRAINT = 3, DERR_ECP_CAP = 4 };
+
+// ---------------------------------------------------------------------------
+// state passed to kernels by value
+// ---------------------------------------------------------------------------
+struct DState {
+    double *f[SX_NFIELDS];
+    int n;
+    int periodic;
+    double box[3];
+};
+
+struct NList {
+    bool init = false;
+    double cutoff = 0, skin = 0, r_skin_sq = 0;
+    int rule = SX_REBUILD_REFERENCE;
+    // cell grid
+    bool use_cells = false;
+    int ncell[3] = {0, 0, 0};
+    int ncells = 0;
+    // device arrays
+    int *cell_of = nullptr, *cell_count = nullptr, *cell_start = nullptr, *cell_items = nullptr;
+    int *row_len = nullptr, *row_start = nullptr, *lower = nullptr, *half_start = nullptr;
+    int *nbr = nullptr, *pairk = nullptr;
+    int *half_i = nullptr, *half_j = nullptr;
+    double *x_last = nullptr, *y_last = nullptr, *z_last = nullptr;
+    long cap_full = 0, cap_half = 0;
+    int max_row = 0;
+    long rebuilds = 0;
+    // host copies of the sizes of the last build
+    long nnz_full = 0, nnz_half = 0;
+};
+
+struct Bonded {
+    int kind;        // 1 bend, 2 cosine_bend, 3 dih, 4 cosine_dih
+    int count;
+    double coeff, angle;
+    int *idx = nullptr;   // count * 4 (0-based atoms; 4th unused for bends)
+    double *out = nullptr; // count * 13: pe, 4x3 forces
+};
+
+struct Molecules {
+    int nmol = 0;
+    int *atom_start = nullptr, *atoms = nullptr;  // CSR of atoms per molecule
+    int *bond_start = nullptr, *bond_a = nullptr, *bond_b = nullptr;  // local atom ids
+    double *bond_len = nullptr;
+    int *bond_gidx = nullptr;
+    int max_atoms = 0;
+};
+
+struct sx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // system
+    int n = 0, nrigid = 1, nc = 0;
+    double box[3] = {0, 0, 0};
+    int periodic = 0;
+    bool have_target_temp = false;
+    double *d_fields = nullptr;  // SX_NFIELDS * ld doubles
+    long ld = 0;
+    double *d_scal = nullptr;    // SX_NSCALARS doubles (device copy of the scalars)
+    double h_scal[SX_NSCALARS] = {0};
+    bool scal_dirty_host = false; // host copy newer than device
+    double *d_part = nullptr;    // reduction partials
+    int part_cap = 0;
+    int *d_err = nullptr;
+    // constraints
+    int *d_bbi = nullptr, *d_bbj = nullptr;
+    double *d_bblen = nullptr;
+    Molecules mol;
+    // neighbour list
+    NList nl;
+    int *d_flag = nullptr;       // [0] rebuild flag, [1] full nnz, [2] half nnz, [4] ECP work count
+    int *d_step = nullptr;       // Langevin step counter for graph replays
+    long *d_counts = nullptr;    // device counters (ECP stats etc.)
+    // bonded
+    std::vector<Bonded> bonded;
+    int *d_atom_term_start = nullptr, *d_atom_term = nullptr;  // atom -> (term slot) CSR over all bonded
+    void *d_bbufs = nullptr;     // device table of the term buffers
+    long *d_boff = nullptr;      // first global slot of each buffer
+    bool bonded_index_dirty = true;
+    // pair buffers (anisotropic kernels)
+    double *d_pairbuf = nullptr;
+    unsigned char *d_live = nullptr;
+    int *d_work = nullptr;        // compacted pair list
+    long pair_cap = 0;
+    // per-particle ECP precompute
+    double *d_ecp_pp = nullptr;
+    int ecp_pp_cap = 0;
+    int ecp_solver = SX_ECP_BRENT_REFERENCE;
+    long ecp_stats[4] = {0, 0, 0, 0};
+    // MLP (learned warm start)
+    void *mlp = nullptr;
+    // graph
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    sx_step_spec graph_spec;
+    bool have_graph = false;
+    DState dstate() const;
+};
+
+struct SxRedIds {
+    int id[8];
+    int n;
+    int accumulate;
+};
+
+// Runs fn, converting thrown sx_error into the status code and message.
+template <typename F>
+static inline int guard(sx_ctx *c, F &&fn)
+{
+    if (!c) return SX_ERR_ARG;
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        fn();
+        return SX_OK;
+    } catch (const sx_error &e) {
+        c->err = e.msg;
+        return e.code;
+    } catch (const std::exception &e) {
+        c->err = e.what();
+        return SX_ERR_HIP;
+    }
+}
+
+static inline void need_system(sx_ctx *c)
+{
+    if (c->n <= 0) throw sx_error{SX_ERR_STATE, "Error: System has no atoms. Set number of atoms first."};
+}
+
+// ---------------------------------------------------------------------------
+// helpers shared across translation units
+// ---------------------------------------------------------------------------
+void sx_check_device_error(sx_ctx *c);                // syncs, throws on device error
+void sx_ensure_part(sx_ctx *c, int nblocks, int width);
+void sx_reduce_to_scalar(sx_ctx *c, int nblocks, int width, const int *ids, int nids, int accumulate);
+void sx_sync_scalars_to_device(sx_ctx *c);
+void sx_sync_scalars_to_host(sx_ctx *c);
+void sx_nlist_build_impl(sx_ctx *c, bool conditional);
+void sx_nlist_flag_impl(sx_ctx *c);
+void sx_forces_impl(sx_ctx *c, int kind, const double *p, int np);
+void sx_bonded_impl(sx_ctx *c);
+void sx_lgv_initial_impl(sx_ctx *c, double dt, int step, const int *dstep, double damping, bool constrained);
+void sx_lgv_final_impl(sx_ctx *c, double dt, double damping, bool constrained);
+void sx_vv_initial_impl(sx_ctx *c, double dt, bool save);
+void sx_vv_final_impl(sx_ctx *c, double dt);
+void sx_qq_initial_impl(sx_ctx *c, double dt);
+void sx_qq_final_impl(sx_ctx *c, double dt);
+void sx_nht_impl(sx_ctx *c, double dt, const double *coupling, int phase, bool save);
+void sx_rattle_impl(sx_ctx *c, double dt, int phase);
+void sx_thermo_impl(sx_ctx *c);
+void sx_init_forces_impl(sx_ctx *c);
+void sx_ecp_mlp_free(sx_ctx *c);
+
+static inline int sx_blocks(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+// ---------------------------------------------------------------------------
+// device helpers (FP64, reference evaluation order)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double sx_mimg(double d, double L) { return d - round(d / L) * L; }
+
+// Philox4x32-10 with the reference's constants and fixed key (xmath.f90:5-76)
+__device__ __forceinline__ void sx_philox(uint32_t c[4])
+{
+    uint32_t k0 = 0x6D2B79F5u, k1 = 0x1B873593u;
+#pragma unroll
+    for (int r = 1; r <= 10; ++r) {
+        uint32_t lo0 = 0xD2511F53u * c[0];
+        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
+        uint32_t lo1 = 0xCD9E8D57u * c[2];
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        uint32_t n0 = hi1 ^ (c[1] ^ k0);
+        uint32_t n2 = hi0 ^ (c[3] ^ k1);
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        if (r < 10) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    }
+}
+
+// philox_normal_3seq (xmath.f90:78-113)
+__device__ __forceinline__ void sx_normal3(int c1, int c2, int c3, int c4, double &n1, double &n2, double &n3)
+{
+    uint32_t c[4] = {(uint32_t)c1, (uint32_t)c2, (uint32_t)c3, (uint32_t)c4};
+    sx_philox(c);
+    const double s = 1.0 / 4294967296.0;
+    double u1 = (double)c[0] * s, u2 = (double)c[1] * s, u3 = (double)c[2] * s, u4 = (double)c[3] * s;
+    u1 = fmax(u1, 1.0e-35);
+    u3 = fmax(u3, 1.0e-35);
+    double r1 = sqrt(-2.0 * log(u1));
+    double th1 = 6.283185307179586 * u2;
+    double r2 = sqrt(-2.0 * log(u3));
+    double th2 = 6.283185307179586 * u4;
+    n1 = r1 * cos(th1);
+    n2 = r1 * sin(th1);
+    n3 = r2 * cos(th2);
+}
+
+// 3x3 SPD solve in the reference's order (xmath.f90:281-304); a is symmetric,
+// given by its lower triangle a11 a21 a31 a22 a32 a33.
+__device__ __forceinline__ void sx_chol_solve(double a11, double a21, double a31, double a22, double a32,
+                                              double a33, const double b[3], double out[3])
+{
+    double inv_l11 = 1.0 / sqrt(a11);
+    double l21 = a21 * inv_l11;
+    double l31 = a31 * inv_l11;
+    double inv_l22 = 1.0 / sqrt(a22 - l21 * l21);
+    double l32 = (a32 - l31 * l21) * inv_l22;
+    double inv_l33 = 1.0 / sqrt(a33 - l31 * l31 - l32 * l32);
+    double y1 = b[0] * inv_l11;
+    double y2 = (b[1] - l21 * y1) * inv_l22;
+    double y3 = (b[2] - l31 * y1 - l32 * y2) * inv_l33;
+    out[2] = y3 * inv_l33;
+    out[1] = (y2 - l32 * out[2]) * inv_l22;
+    out[0] = (y1 - l21 * out[1] - l31 * out[2]) * inv_l11;
+}

@@ -1,0 +1,553 @@
+// Device neighbour list: cell-binned build of the reference's Verlet list
+// (module neighbour_list, system.f90:885-1053).
+//
+// The pair set is the reference's: a pair (a < b) is included iff the FP64
+// value ((d0*d0 + d1*d1) + d2*d2), d = mimg(r_a - r_b), is < (rc+skin)^2,
+// evaluated in the reference's order for the canonical orientation a < b.
+// Cells (edge >= rc+skin, >= 3 per dimension) only choose which pairs are
+// tested.  Each row holds every neighbour j != i sorted ascending ("full"
+// list); the reference's half list (j > i) is the tail of each row.
+#include "sx_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+#define SORT_CAP 2048
+
+// canonical pair test: identical FP64 expression to build_nlist (system.f90:980-989)
+__device__ __forceinline__ bool in_list(const DState &s, int i, int j, double rsq_cut)
+{
+    int a = i < j ? i : j, b = i < j ? j : i;
+    double d0 = s.f[SX_RX][a] - s.f[SX_RX][b];
+    double d1 = s.f[SX_RY][a] - s.f[SX_RY][b];
+    double d2 = s.f[SX_RZ][a] - s.f[SX_RZ][b];
+    if (s.periodic) {
+        d0 = sx_mimg(d0, s.box[0]);
+        d1 = sx_mimg(d1, s.box[1]);
+        d2 = sx_mimg(d2, s.box[2]);
+    }
+    double rsq = d0 * d0 + d1 * d1 + d2 * d2;
+    return rsq < rsq_cut;
+}
+
+struct CellGrid {
+    int nc[3];
+    double w[3];   // cell edge
+};
+
+__device__ __forceinline__ int cell_coord(double x, double L, int nc, double w)
+{
+    double xw = x - floor(x / L) * L;   // [0, L) for binning only
+    int c = (int)(xw / w);
+    return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
+}
+
+__global__ void k_zero_int(int *p, int n, const int *flag)
+{
+    if (flag && *flag == 0) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ void k_cell_assign(DState s, CellGrid g, int *cell_of, int *cell_count, const int *flag)
+{
+    if (flag && *flag == 0) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    int cx = cell_coord(s.f[SX_RX][i], s.box[0], g.nc[0], g.w[0]);
+    int cy = cell_coord(s.f[SX_RY][i], s.box[1], g.nc[1], g.w[1]);
+    int cz = cell_coord(s.f[SX_RZ][i], s.box[2], g.nc[2], g.w[2]);
+    int c = (cx * g.nc[1] + cy) * g.nc[2] + cz;
+    cell_of[i] = c;
+    atomicAdd(&cell_count[c], 1);
+}
+
+// exclusive scan of n ints by one work-group; out has n+1 entries; optional
+// conditional flag (skip when *flag == 0)
+__global__ void __launch_bounds__(1024) k_scan(const int *in, int *out, int n, const int *flag, int *total)
+{
+    if (flag && *flag == 0) return;
+    __shared__ long sums[1024];
+    int t = threadIdx.x;
+    long chunk = ((long)n + 1023) / 1024;
+    long b = t * chunk, e = b + chunk < n ? b + chunk : n;
+    long s = 0;
+    for (long i = b; i < e; ++i) s += in[i];
+    sums[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        long v = (t >= off) ? sums[t - off] : 0;
+        __syncthreads();
+        sums[t] += v;
+        __syncthreads();
+    }
+    long run = sums[t] - s;
+    for (long i = b; i < e; ++i) {
+        out[i] = (int)run;
+        run += in[i];
+    }
+    if (t == 1023) {
+        out[n] = (int)sums[1023];
+        if (total) *total = (int)sums[1023];
+    }
+}
+
+__global__ void k_cell_scatter(int n, const int *cell_of, const int *cell_start, int *cell_fill, int *items,
+                               const int *flag)
+{
+    if (flag && *flag == 0) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c = cell_of[i];
+    int p = atomicAdd(&cell_fill[c], 1);
+    items[cell_start[c] + p] = i;
+}
+
+// count (pass 0) or fill (pass 1) the full rows using the cell grid
+template <int PASS>
+__global__ void k_rows_cells(DState s, CellGrid g, const int *cell_of, const int *cell_start, const int *items,
+                             double rsq_cut, int *row_len, int *lower, const int *row_start, int *nbr,
+                             const int *flag, long cap, int *err)
+{
+    if (flag && *flag == 0) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    int c = cell_of[i];
+    int cz = c % g.nc[2], cy = (c / g.nc[2]) % g.nc[1], cx = c / (g.nc[2] * g.nc[1]);
+    int cnt = 0, low = 0;
+    long base = PASS ? row_start[i] : 0;
+    if (PASS && (long)row_start[i + 1] > cap) {
+        atomicCAS(err, 0, DERR_NLIST_CAP);
+        return;
+    }
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz) {
+                int x = (cx + dx + g.nc[0]) % g.nc[0];
+                int y = (cy + dy + g.nc[1]) % g.nc[1];
+                int z = (cz + dz + g.nc[2]) % g.nc[2];
+                int cc = (x * g.nc[1] + y) * g.nc[2] + z;
+                for (int q = cell_start[cc]; q < cell_start[cc + 1]; ++q) {
+                    int j = items[q];
+                    if (j == i) continue;
+                    if (in_list(s, i, j, rsq_cut)) {
+                        if (PASS) nbr[base + cnt] = j;
+                        ++cnt;
+                        if (j < i) ++low;
+                    }
+                }
+            }
+    if (!PASS) {
+        row_len[i] = cnt;
+        lower[i] = low;
+    }
+}
+
+// brute force rows (small or non-periodic boxes)
+template <int PASS>
+__global__ void k_rows_brute(DState s, double rsq_cut, int *row_len, int *lower, const int *row_start, int *nbr,
+                             const int *flag, long cap, int *err)
+{
+    if (flag && *flag == 0) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    int cnt = 0, low = 0;
+    long base = PASS ? row_start[i] : 0;
+    if (PASS && (long)row_start[i + 1] > cap) {
+        atomicCAS(err, 0, DERR_NLIST_CAP);
+        return;
+    }
+    for (int j = 0; j < s.n; ++j) {
+        if (j == i) continue;
+        if (in_list(s, i, j, rsq_cut)) {
+            if (PASS) nbr[base + cnt] = j;
+            ++cnt;
+            if (j < i) ++low;
+        }
+    }
+    if (!PASS) {
+        row_len[i] = cnt;
+        lower[i] = low;
+    }
+}
+
+// one wavefront per row: bitonic sort of the row in LDS (ascending j)
+__global__ void __launch_bounds__(256) k_sort_rows(int n, const int *row_start, int *nbr, const int *flag, int *err)
+{
+    if (flag && *flag == 0) return;
+    __shared__ int buf[4][SORT_CAP];
+    int w = threadIdx.x / SX_WAVE, lane = threadIdx.x % SX_WAVE;
+    int i = blockIdx.x * 4 + w;
+    if (i >= n) return;
+    int b = row_start[i], len = row_start[i + 1] - b;
+    if (len <= 1) return;
+    if (len > SORT_CAP) {
+        if (lane == 0) atomicCAS(err, 0, DERR_ROW_CAP);
+        return;
+    }
+    int P = 1;
+    while (P < len) P <<= 1;
+    int *sh = buf[w];
+    for (int q = lane; q < P; q += SX_WAVE) sh[q] = q < len ? nbr[b + q] : 0x7fffffff;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int jj = k >> 1; jj > 0; jj >>= 1) {
+            for (int q = lane; q < P; q += SX_WAVE) {
+                int p = q ^ jj;
+                if (p > q) {
+                    int a = sh[q], c = sh[p];
+                    bool up = (q & k) == 0;
+                    if ((a > c) == up) { sh[q] = c; sh[p] = a; }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+    for (int q = lane; q < len; q += SX_WAVE) nbr[b + q] = sh[q];
+}
+
+__global__ void k_half_len(int n, const int *row_len, const int *lower, int *half_len, const int *flag)
+{
+    if (flag && *flag == 0) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) half_len[i] = row_len[i] - lower[i];
+}
+
+// half-pair index of every full-list entry, and the half list (i, j) pairs
+__global__ void k_half_index(int n, const int *row_start, const int *lower, const int *half_start, const int *nbr,
+                             int *pairk, int *half_i, int *half_j, const int *flag, long cap_half, int *err)
+{
+    if (flag && *flag == 0) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int b = row_start[i], e = row_start[i + 1], lo = lower[i];
+    if ((long)half_start[n] > cap_half) {
+        atomicCAS(err, 0, DERR_NLIST_CAP);
+        return;
+    }
+    for (int p = b; p < e; ++p) {
+        int j = nbr[p];
+        if (j > i) {
+            int k = half_start[i] + (p - b - lo);
+            pairk[p] = k;
+            half_i[k] = i;
+            half_j[k] = j;
+        } else {
+            // position of i in row j (sorted): binary search
+            int lb = row_start[j], ub = row_start[j + 1];
+            while (lb < ub) {
+                int m = (lb + ub) >> 1;
+                if (nbr[m] < i) lb = m + 1; else ub = m;
+            }
+            pairk[p] = half_start[j] + (lb - row_start[j] - lower[j]);
+        }
+    }
+}
+
+__global__ void k_save_last(DState s, double *xl, double *yl, double *zl, const int *flag)
+{
+    if (flag && *flag == 0) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    xl[i] = s.f[SX_RX][i]; yl[i] = s.f[SX_RY][i]; zl[i] = s.f[SX_RZ][i];
+}
+
+// needs_rebuild (system.f90:1005-1039): max over i of |mimg(r - r_last)|^2
+__global__ void __launch_bounds__(256) k_disp_max(DState s, const double *xl, const double *yl, const double *zl,
+                                                  double *part)
+{
+    __shared__ double sh[256];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if (i < s.n) {
+        double dx = s.f[SX_RX][i] - xl[i], dy = s.f[SX_RY][i] - yl[i], dz = s.f[SX_RZ][i] - zl[i];
+        if (s.periodic) {
+            dx = sx_mimg(dx, s.box[0]); dy = sx_mimg(dy, s.box[1]); dz = sx_mimg(dz, s.box[2]);
+        }
+        m = dx * dx + dy * dy + dz * dz;
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_disp_flag(const double *part, int nblocks, double thresh, int rule, int *flag, long *count)
+{
+    if (threadIdx.x != 0) return;
+    double m = 0.0;
+    for (int b = 0; b < nblocks; ++b) m = fmax(m, part[b]);
+    int r = rule == SX_REBUILD_REFERENCE ? (4.0 * m > thresh) : (m > thresh);
+    flag[0] = r;
+}
+
+__global__ void k_count_rebuild(const int *flag, long *count)
+{
+    if (threadIdx.x == 0 && flag[0]) count[7] += 1;
+}
+
+__global__ void k_set_flag(int *flag, int v) { flag[0] = v; }
+
+// ---------------------------------------------------------------------------
+static void nl_free(NList &nl)
+{
+    int *ip[] = {nl.cell_of, nl.cell_count, nl.cell_start, nl.cell_items, nl.row_len, nl.row_start, nl.lower,
+                 nl.half_start, nl.nbr, nl.pairk, nl.half_i, nl.half_j};
+    for (int *p : ip)
+        if (p) hipFree(p);
+    double *dp[] = {nl.x_last, nl.y_last, nl.z_last};
+    for (double *p : dp)
+        if (p) hipFree(p);
+    nl.cell_of = nl.cell_count = nl.cell_start = nl.cell_items = nl.row_len = nl.row_start = nl.lower = nullptr;
+    nl.half_start = nl.nbr = nl.pairk = nl.half_i = nl.half_j = nullptr;
+    nl.x_last = nl.y_last = nl.z_last = nullptr;
+}
+
+static void nl_alloc_lists(sx_ctx *c, long cap_full)
+{
+    NList &nl = c->nl;
+    if (nl.nbr) hipFree(nl.nbr);
+    if (nl.pairk) hipFree(nl.pairk);
+    if (nl.half_i) hipFree(nl.half_i);
+    if (nl.half_j) hipFree(nl.half_j);
+    if (cap_full > 0x7fffffffL) throw sx_error{SX_ERR_CAPACITY, "neighbour list larger than 2^31 entries"};
+    nl.cap_full = cap_full;
+    nl.cap_half = cap_full / 2 + 1;
+    HIPCHK(hipMalloc(&nl.nbr, sizeof(int) * nl.cap_full));
+    HIPCHK(hipMalloc(&nl.pairk, sizeof(int) * nl.cap_full));
+    HIPCHK(hipMalloc(&nl.half_i, sizeof(int) * nl.cap_half));
+    HIPCHK(hipMalloc(&nl.half_j, sizeof(int) * nl.cap_half));
+    c->have_graph = false;
+}
+
+static void nl_setup(sx_ctx *c)
+{
+    NList &nl = c->nl;
+    nl_free(nl);
+    int n = c->n;
+    double rskin = nl.cutoff + nl.skin;
+    nl.r_skin_sq = rskin * rskin;
+    nl.use_cells = c->periodic != 0;
+    nl.ncells = 1;
+    for (int d = 0; d < 3; ++d) {
+        nl.ncell[d] = (int)std::floor(c->box[d] / rskin);
+        if (nl.ncell[d] < 3) nl.use_cells = false;
+    }
+    if (nl.use_cells) {
+        // bound the grid (very dilute systems)
+        while ((long)nl.ncell[0] * nl.ncell[1] * nl.ncell[2] > 4L * n + 64) {
+            for (int d = 0; d < 3; ++d) nl.ncell[d] = std::max(3, nl.ncell[d] / 2);
+            if (nl.ncell[0] == 3 && nl.ncell[1] == 3 && nl.ncell[2] == 3) break;
+        }
+        nl.ncells = nl.ncell[0] * nl.ncell[1] * nl.ncell[2];
+    }
+    HIPCHK(hipMalloc(&nl.cell_of, sizeof(int) * n));
+    HIPCHK(hipMalloc(&nl.cell_count, sizeof(int) * (nl.ncells + 1)));
+    HIPCHK(hipMalloc(&nl.cell_start, sizeof(int) * (nl.ncells + 1)));
+    HIPCHK(hipMalloc(&nl.cell_items, sizeof(int) * n));
+    HIPCHK(hipMalloc(&nl.row_len, sizeof(int) * (n + 1)));
+    HIPCHK(hipMalloc(&nl.row_start, sizeof(int) * (n + 1)));
+    HIPCHK(hipMalloc(&nl.lower, sizeof(int) * (n + 1)));
+    HIPCHK(hipMalloc(&nl.half_start, sizeof(int) * (n + 1)));
+    HIPCHK(hipMalloc(&nl.x_last, sizeof(double) * n));
+    HIPCHK(hipMalloc(&nl.y_last, sizeof(double) * n));
+    HIPCHK(hipMalloc(&nl.z_last, sizeof(double) * n));
+    // initial capacity: density estimate of the list sphere x 2, at least 32 per row
+    double vol = c->box[0] * c->box[1] * c->box[2];
+    double est = (double)n * (n / vol) * 4.18879 * rskin * rskin * rskin;
+    long cap = std::max<long>((long)(2.0 * est) + 1024, 32L * n);
+    if (!c->periodic) cap = std::max<long>(cap, std::min<long>((long)n * (n - 1), 64L * n));
+    nl_alloc_lists(c, cap);
+    nl.init = true;
+}
+
+// Builds the list.  conditional: every kernel reads the device rebuild flag
+// (graph replay); otherwise the build is unconditional and capacity grows as
+// needed.
+void sx_nlist_build_impl(sx_ctx *c, bool conditional)
+{
+    NList &nl = c->nl;
+    int n = c->n;
+    DState s = c->dstate();
+    const int *flag = conditional ? c->d_flag : nullptr;
+    int nb = sx_blocks(n, 256);
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        if (nl.use_cells) {
+            CellGrid g;
+            for (int d = 0; d < 3; ++d) { g.nc[d] = nl.ncell[d]; g.w[d] = c->box[d] / nl.ncell[d]; }
+            // cell binning is cheap; it runs unconditionally
+            int ncb = sx_blocks(nl.ncells + 1, 256);
+            hipLaunchKernelGGL(k_zero_int, dim3(ncb), dim3(256), 0, c->stream, nl.cell_count, nl.ncells + 1, flag);
+            hipLaunchKernelGGL(k_cell_assign, dim3(nb), dim3(256), 0, c->stream, s, g, nl.cell_of, nl.cell_count,
+                               flag);
+            hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.cell_count, nl.cell_start, nl.ncells,
+                               flag, (int *)nullptr);
+            hipLaunchKernelGGL(k_zero_int, dim3(ncb), dim3(256), 0, c->stream, nl.cell_count, nl.ncells + 1, flag);
+            hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(256), 0, c->stream, n, nl.cell_of, nl.cell_start,
+                               nl.cell_count, nl.cell_items, flag);
+            hipLaunchKernelGGL(k_rows_cells<0>, dim3(nb), dim3(256), 0, c->stream, s, g, nl.cell_of, nl.cell_start,
+                               nl.cell_items, nl.r_skin_sq, nl.row_len, nl.lower, nl.row_start, nl.nbr, flag,
+                               nl.cap_full, c->d_err);
+            hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.row_len, nl.row_start, n, flag,
+                               c->d_flag + 1);
+            hipLaunchKernelGGL(k_rows_cells<1>, dim3(nb), dim3(256), 0, c->stream, s, g, nl.cell_of, nl.cell_start,
+                               nl.cell_items, nl.r_skin_sq, nl.row_len, nl.lower, nl.row_start, nl.nbr, flag,
+                               nl.cap_full, c->d_err);
+        } else {
+            hipLaunchKernelGGL(k_rows_brute<0>, dim3(nb), dim3(256), 0, c->stream, s, nl.r_skin_sq, nl.row_len,
+                               nl.lower, nl.row_start, nl.nbr, flag, nl.cap_full, c->d_err);
+            hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.row_len, nl.row_start, n, flag,
+                               c->d_flag + 1);
+            hipLaunchKernelGGL(k_rows_brute<1>, dim3(nb), dim3(256), 0, c->stream, s, nl.r_skin_sq, nl.row_len,
+                               nl.lower, nl.row_start, nl.nbr, flag, nl.cap_full, c->d_err);
+        }
+        if (!conditional) {
+            int total = 0;
+            HIPCHK(hipMemcpyAsync(&total, c->d_flag + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            if ((long)total > nl.cap_full) {
+                HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+                nl_alloc_lists(c, (long)(total * 1.5) + 1024);
+                continue;
+            }
+        }
+        break;
+    }
+    hipLaunchKernelGGL(k_sort_rows, dim3(sx_blocks(n, 4)), dim3(256), 0, c->stream, n, nl.row_start, nl.nbr, flag,
+                       c->d_err);
+    hipLaunchKernelGGL(k_half_len, dim3(nb), dim3(256), 0, c->stream, n, nl.row_len, nl.lower, nl.cell_items, flag);
+    // cell_items doubles as the half-length scratch after the fill pass
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.cell_items, nl.half_start, n, flag,
+                       c->d_flag + 2);
+    hipLaunchKernelGGL(k_half_index, dim3(nb), dim3(256), 0, c->stream, n, nl.row_start, nl.lower, nl.half_start,
+                       nl.nbr, nl.pairk, nl.half_i, nl.half_j, flag, nl.cap_half, c->d_err);
+    hipLaunchKernelGGL(k_save_last, dim3(nb), dim3(256), 0, c->stream, s, nl.x_last, nl.y_last, nl.z_last, flag);
+    if (conditional) hipLaunchKernelGGL(k_count_rebuild, dim3(1), dim3(64), 0, c->stream, flag, c->d_counts);
+    if (!conditional) {
+        int sizes[2];
+        HIPCHK(hipMemcpyAsync(sizes, c->d_flag + 1, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));
+        sx_check_device_error(c);
+        nl.nnz_full = sizes[0];
+        nl.nnz_half = sizes[1];
+        nl.rebuilds++;
+    }
+}
+
+// computes the rebuild flag on the device (d_flag[0])
+void sx_nlist_flag_impl(sx_ctx *c)
+{
+    NList &nl = c->nl;
+    int nb = sx_blocks(c->n, 256);
+    sx_ensure_part(c, nb, 1);
+    hipLaunchKernelGGL(k_disp_max, dim3(nb), dim3(256), 0, c->stream, c->dstate(), nl.x_last, nl.y_last, nl.z_last,
+                       c->d_part);
+    double thresh = nl.rule == SX_REBUILD_REFERENCE ? nl.r_skin_sq : (nl.skin / 2.0) * (nl.skin / 2.0);
+    hipLaunchKernelGGL(k_disp_flag, dim3(1), dim3(64), 0, c->stream, c->d_part, nb, thresh, nl.rule, c->d_flag,
+                       c->d_counts);
+}
+
+static void need_nlist(sx_ctx *c)
+{
+    if (!c->nl.init) throw sx_error{SX_ERR_STATE, "neighbour list not initialised (sx_nlist_init)"};
+}
+
+extern "C" {
+
+int sx_nlist_init(sx_ctx *c, double skin, double cutoff)
+{
+    return guard(c, [&] {
+        need_system(c);
+        for (int d = 0; d < 3; ++d)
+            if (cutoff > c->box[d] / 2.0) throw sx_error{SX_ERR_CUTOFF, "A cutoff is too large for the box dimensions"};
+        c->nl.cutoff = cutoff;
+        c->nl.skin = skin;
+        nl_setup(c);
+        sx_nlist_build_impl(c, false);
+    });
+}
+
+int sx_nlist_build(sx_ctx *c)
+{
+    return guard(c, [&] { need_nlist(c); sx_nlist_build_impl(c, false); });
+}
+
+int sx_nlist_needs_rebuild(sx_ctx *c, int *out)
+{
+    return guard(c, [&] {
+        need_nlist(c);
+        sx_nlist_flag_impl(c);
+        int f = 0;
+        HIPCHK(hipMemcpyAsync(&f, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        *out = f;
+    });
+}
+
+int sx_nlist_update(sx_ctx *c)
+{
+    return guard(c, [&] {
+        need_nlist(c);
+        sx_nlist_flag_impl(c);
+        int f = 0;
+        HIPCHK(hipMemcpyAsync(&f, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (f) sx_nlist_build_impl(c, false);
+    });
+}
+
+int sx_nlist_set_rule(sx_ctx *c, int rule)
+{
+    return guard(c, [&] {
+        if (rule != SX_REBUILD_REFERENCE && rule != SX_REBUILD_HALF_SKIN) throw sx_error{SX_ERR_ARG, "bad rule"};
+        c->nl.rule = rule;
+        c->have_graph = false;
+    });
+}
+
+int sx_nlist_rebuild_count(sx_ctx *c, long *count)
+{
+    return guard(c, [&] {
+        long dev[8];
+        HIPCHK(hipMemcpyAsync(dev, c->d_counts, sizeof(long) * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        *count = c->nl.rebuilds + dev[7];
+    });
+}
+
+int sx_nlist_export(sx_ctx *c, int *offset1, int *nlist1, long cap, long *nnz)
+{
+    return guard(c, [&] {
+        need_nlist(c);
+        sx_check_device_error(c);
+        int n = c->n;
+        int sizes[2];
+        HIPCHK(hipMemcpyAsync(sizes, c->d_flag + 1, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        long nf = sizes[0], nh = sizes[1];
+        *nnz = nh;
+        std::vector<int> hs(n + 1), hi(nh), hj(nh);
+        HIPCHK(hipMemcpyAsync(hs.data(), c->nl.half_start, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, c->stream));
+        if (nh > 0) {
+            HIPCHK(hipMemcpyAsync(hj.data(), c->nl.half_j, sizeof(int) * nh, hipMemcpyDeviceToHost, c->stream));
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        (void)nf;
+        if (offset1) {
+            for (int i = 0; i < n - 1; ++i) offset1[i] = hs[i] + 1;
+            if (n >= 1) offset1[n - 1] = (int)nh + 1;
+        }
+        if (nlist1) {
+            if (cap < nh) throw sx_error{SX_ERR_CAPACITY, "export buffer too small"};
+            for (long k = 0; k < nh; ++k) nlist1[k] = hj[k] + 1;
+        }
+    });
+}
+
+}  // extern "C"

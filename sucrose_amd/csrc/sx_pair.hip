@@ -1,0 +1,539 @@
+// Pair-force kernels over the device neighbour list (force.f90), replacing
+// the README pair loop (README.md:86-103) with one launch sequence per
+// potential.
+//
+//  * isotropic potentials and Gay-Berne: LPP lanes per particle walk the
+//    particle's sorted full row; each lane evaluates the canonical pair
+//    (min, max) exactly as the reference would and keeps the half it needs
+//    (+FXIJ / TXI when i is the reference's I, -FXIJ / TXJ when it is J);
+//    lanes combine with cross-lane shuffles in a fixed order.  No atomics:
+//    results are deterministic run to run.
+//  * ECP: a screen over the half list rejects pairs that provably give zero
+//    (F(psi) > 1 => the Brent minimum lies beyond contact), survivors are
+//    compacted and solved one lane per pair, results land in a per-pair
+//    buffer and a per-particle gather applies them.
+#include "sx_ecp.h"
+
+#include <cmath>
+#include <cstring>
+
+struct Params {
+    double p[8];
+};
+
+// ---------------------------------------------------------------------------
+// isotropic / GB gather
+// ---------------------------------------------------------------------------
+template <int KIND, int LPP, bool TORQUE>
+__global__ void __launch_bounds__(256) k_gather(DState s, const int *row_start, const int *nbr, Params P,
+                                                double *part)
+{
+    int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = gid / LPP, l = gid % LPP;
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};  // fx fy fz tx ty tz pe
+    if (i < s.n) {
+        int b = row_start[i], e = row_start[i + 1];
+        for (int q = b + l; q < e; q += LPP) {
+            int j = nbr[q];
+            PairOut o;
+            if (j > i) {
+                d_pair<KIND>(s, i, j, P.p, o);
+                acc[0] = acc[0] + o.f[0]; acc[1] = acc[1] + o.f[1]; acc[2] = acc[2] + o.f[2];
+                if (TORQUE) { acc[3] = acc[3] + o.ti[0]; acc[4] = acc[4] + o.ti[1]; acc[5] = acc[5] + o.ti[2]; }
+                acc[6] = acc[6] + o.pe;
+            } else {
+                d_pair<KIND>(s, j, i, P.p, o);
+                acc[0] = acc[0] - o.f[0]; acc[1] = acc[1] - o.f[1]; acc[2] = acc[2] - o.f[2];
+                if (TORQUE) { acc[3] = acc[3] + o.tj[0]; acc[4] = acc[4] + o.tj[1]; acc[5] = acc[5] + o.tj[2]; }
+            }
+        }
+    }
+#pragma unroll
+    for (int off = LPP / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int c = 0; c < 7; ++c) acc[c] = acc[c] + __shfl_xor(acc[c], off, LPP);
+    __shared__ double pe_sh[256];
+    pe_sh[threadIdx.x] = (l == 0 && i < s.n) ? acc[6] : 0.0;
+    if (l == 0 && i < s.n) {
+        s.f[SX_FX][i] = s.f[SX_FX][i] + acc[0];
+        s.f[SX_FY][i] = s.f[SX_FY][i] + acc[1];
+        s.f[SX_FZ][i] = s.f[SX_FZ][i] + acc[2];
+        if (TORQUE) {
+            s.f[SX_TX][i] = s.f[SX_TX][i] + acc[3];
+            s.f[SX_TY][i] = s.f[SX_TY][i] + acc[4];
+            s.f[SX_TZ][i] = s.f[SX_TZ][i] + acc[5];
+        }
+    }
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) pe_sh[threadIdx.x] = pe_sh[threadIdx.x] + pe_sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = pe_sh[0];
+}
+
+// ---------------------------------------------------------------------------
+// ECP
+// ---------------------------------------------------------------------------
+// per-particle precompute (force.f90:1427-1483 for one particle)
+__global__ void k_ecp_pp(DState s, double *pp, long ld)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    double w = s.f[SX_QW][i], x = s.f[SX_QX][i], y = s.f[SX_QY][i], z = s.f[SX_QZ][i];
+    double U[9];
+#define Ux(i_, j_) U[((i_) - 1) + 3 * ((j_) - 1)]
+    Ux(1, 1) = w * w + x * x - y * y - z * z;
+    Ux(1, 2) = 2.0 * (x * y + w * z);
+    Ux(1, 3) = 2.0 * (x * z - w * y);
+    Ux(2, 1) = 2.0 * (x * y - w * z);
+    Ux(2, 2) = w * w - x * x + y * y - z * z;
+    Ux(2, 3) = 2.0 * (y * z + w * x);
+    Ux(3, 1) = 2.0 * (x * z + w * y);
+    Ux(3, 2) = 2.0 * (y * z - w * x);
+    Ux(3, 3) = w * w - x * x - y * y + z * z;
+    double sx = s.f[SX_SIG_X][i], sy = s.f[SX_SIG_Y][i], sz = s.f[SX_SIG_Z][i];
+    double ex = s.f[SX_EPS_X][i], ey = s.f[SX_EPS_Y][i], ez = s.f[SX_EPS_Z][i];
+    double e0 = fmax(fmax(ex, ey), ez);
+    double S[3] = {sx / 2.0, sy / 2.0, sz / 2.0};
+    double B[3] = {S[0] * S[0], S[1] * S[1], S[2] * S[2]};
+    // E = ((e0/eps)**(1/2))/4  (force.f90:1452-1454)
+    double E[3] = {sqrt(e0 / ex) / 4.0, sqrt(e0 / ey) / 4.0, sqrt(e0 / ez) / 4.0};
+    // matmul_atba (xmath.f90:262-279) lower triangle
+    const int ij[6][2] = {{1, 1}, {2, 1}, {3, 1}, {2, 2}, {3, 2}, {3, 3}};
+    for (int t = 0; t < 6; ++t) {
+        // out(r,c) with r <= c written as out(1,2) etc.: use (c_, r_) ordering of the reference
+        int r_ = ij[t][1], c_ = ij[t][0];  // reference computes out(r_, c_) with r_ <= c_
+        double ar = Ux(1, r_) * B[0] * Ux(1, c_) + Ux(2, r_) * B[1] * Ux(2, c_) + Ux(3, r_) * B[2] * Ux(3, c_);
+        double ae = Ux(1, r_) * E[0] * Ux(1, c_) + Ux(2, r_) * E[1] * Ux(2, c_) + Ux(3, r_) * E[2] * Ux(3, c_);
+        pp[(PP_AR + t) * ld + i] = ar;
+        pp[(PP_AE + t) * ld + i] = ae;
+    }
+    // MM = U^T . S: column c is row c of U scaled by S(c)
+    for (int c = 0; c < 3; ++c)
+        for (int k = 0; k < 3; ++k) pp[(PP_MM + 3 * c + k) * ld + i] = Ux(c + 1, k + 1) * S[c];
+    pp[PP_ST * ld + i] = (sx * sy + sz * sz) * sqrt(2.0 * sx * sy) / 8.0;
+    pp[PP_AMAX * ld + i] = fmax(fmax(S[0], S[1]), S[2]);
+#undef Ux
+}
+
+struct EcpScreen {
+    int variant;       // 0 ecp (cutoff), 1 ecpr
+    double cutoff;
+};
+
+// Screen of one half pair.  Returns true if the pair can contribute.  The
+// Brent minimum satisfies f(lambda) <= f(psi), i.e. F(lambda) >= F(psi)
+// (force.f90:1132-1165 only ever accepts non-increasing f), and sigma =
+// r / sqrt(F(lambda)) up to rounding, so F(psi) above the contact threshold
+// proves a zero result.  Bound 1: F(psi) >= psi(1-psi) r^2 / lambda_max(G(psi)).
+__device__ __forceinline__ bool ecp_screen(const DState &s, const double *pp, long ld, int a, int b, EcpScreen sc,
+                                           double RIJ[3], double &fpsi)
+{
+    pair_rij(s, a, b, RIJ);
+    double rsq = RIJ[0] * RIJ[0] + RIJ[1] * RIJ[1] + RIJ[2] * RIJ[2];
+    // contact threshold on F: ecpr r <= sigma <=> F <= 1; ecp r < sigma + rc - 1
+    double thr;
+    if (sc.variant == 1) {
+        thr = 1.0;
+    } else {
+        double rmag = sqrt(rsq);
+        double den = rmag - sc.cutoff + 1.0;
+        if (den <= 0.0) { fpsi = 0.0; return true; }
+        thr = (rmag / den) * (rmag / den);
+    }
+    thr = thr * (1.0 + 1e-9);
+    const double psi = ECP_PSI;
+    double am = pp[PP_AMAX * ld + a], bm = pp[PP_AMAX * ld + b];
+    double lmax = (1.0 - psi) * am * am + psi * bm * bm;
+    if (psi * (1.0 - psi) * rsq > thr * lmax * (1.0 + 1e-9)) return false;
+    Sym3 A = load_sym(pp, ld, PP_AR, a), B = load_sym(pp, ld, PP_AR, b);
+    fpsi = ecp_optim_dist(psi, A, B, RIJ);
+    return -fpsi <= thr;
+}
+
+__global__ void k_ecp_screen(DState s, const double *pp, long ld, const int *half_i, const int *half_j,
+                             const int *nnz_half, EcpScreen sc, unsigned char *live, int *work, int *nwork, long cap)
+{
+    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= *nnz_half) return;
+    int a = half_i[k], b = half_j[k];
+    double RIJ[3], fpsi;
+    bool keep = ecp_screen(s, pp, ld, a, b, sc, RIJ, fpsi);
+    live[k] = keep ? 1 : 0;
+    if (keep) {
+        int slot = atomicAdd(nwork, 1);
+        if (slot < cap) work[slot] = (int)k;
+    }
+}
+
+template <int VARIANT>
+__device__ __forceinline__ double ecp_solve_lambda(int solver, const DState &s, const double *pp, long ld, int a,
+                                                   int b, const double RIJ[3], double fpsi, double lam0, int &iters,
+                                                   int &fallback)
+{
+    Sym3 A = load_sym(pp, ld, PP_AR, a), B = load_sym(pp, ld, PP_AR, b);
+    fallback = 0;
+    if (solver == SX_ECP_BRENT_REFERENCE) return ecp_brent(A, B, RIJ, fpsi, iters);
+    int ok = 0;
+    double l = ecp_lambda_newton(A, B, RIJ, lam0, 1e-10, iters, ok);
+    if (!ok) {
+        int it2 = 0;
+        l = ecp_lambda_newton(A, B, RIJ, 0.5, 1e-10, it2, ok);
+        iters += it2;
+        fallback = 1;
+    }
+    return l;
+}
+
+template <int VARIANT>
+__global__ void k_ecp_solve(DState s, const double *pp, long ld, const int *half_i, const int *half_j,
+                            const int *work, const int *nwork, long cap, EcpScreen sc, int solver, const float *lam0,
+                            double *res, long *stats, int *err)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long nw = *nwork;
+    if (t == 0 && nw > cap) atomicCAS(err, 0, DERR_ECP_CAP);
+    if (t >= nw || t >= cap) return;
+    int k = work[t];
+    int a = half_i[k], b = half_j[k];
+    double RIJ[3];
+    pair_rij(s, a, b, RIJ);
+    Sym3 A = load_sym(pp, ld, PP_AR, a), B = load_sym(pp, ld, PP_AR, b);
+    double fpsi = ecp_optim_dist(ECP_PSI, A, B, RIJ);
+    int iters = 0, fb = 0;
+    double l0 = lam0 ? (double)lam0[t] : 0.5;
+    double lam = ecp_solve_lambda<VARIANT>(solver, s, pp, ld, a, b, RIJ, fpsi, l0, iters, fb);
+    PairOut o;
+    ecp_forces<VARIANT>(s, pp, ld, a, b, sc.cutoff, RIJ, lam, o);
+    double *r = res + (long)k * 10;
+    r[0] = o.pe;
+    r[1] = o.f[0]; r[2] = o.f[1]; r[3] = o.f[2];
+    r[4] = o.ti[0]; r[5] = o.ti[1]; r[6] = o.ti[2];
+    r[7] = o.tj[0]; r[8] = o.tj[1]; r[9] = o.tj[2];
+    // stats: [0] solved pairs, [1] iterations, [2] fallbacks
+    atomicAdd((unsigned long long *)&stats[0], 1ull);
+    atomicAdd((unsigned long long *)&stats[1], (unsigned long long)iters);
+    if (fb) atomicAdd((unsigned long long *)&stats[2], 1ull);
+}
+
+// per-particle application of the per-pair results
+template <int LPP>
+__global__ void __launch_bounds__(256) k_aniso_apply(DState s, const int *row_start, const int *nbr, const int *pairk,
+                                                     const unsigned char *live, const double *res, double *part)
+{
+    int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = gid / LPP, l = gid % LPP;
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (i < s.n) {
+        int b = row_start[i], e = row_start[i + 1];
+        for (int q = b + l; q < e; q += LPP) {
+            int k = pairk[q];
+            if (!live[k]) continue;
+            const double *r = res + (long)k * 10;
+            if (nbr[q] > i) {
+                acc[0] = acc[0] + r[1]; acc[1] = acc[1] + r[2]; acc[2] = acc[2] + r[3];
+                acc[3] = acc[3] + r[4]; acc[4] = acc[4] + r[5]; acc[5] = acc[5] + r[6];
+                acc[6] = acc[6] + r[0];
+            } else {
+                acc[0] = acc[0] - r[1]; acc[1] = acc[1] - r[2]; acc[2] = acc[2] - r[3];
+                acc[3] = acc[3] + r[7]; acc[4] = acc[4] + r[8]; acc[5] = acc[5] + r[9];
+            }
+        }
+    }
+#pragma unroll
+    for (int off = LPP / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int c = 0; c < 7; ++c) acc[c] = acc[c] + __shfl_xor(acc[c], off, LPP);
+    __shared__ double pe_sh[256];
+    pe_sh[threadIdx.x] = (l == 0 && i < s.n) ? acc[6] : 0.0;
+    if (l == 0 && i < s.n) {
+        s.f[SX_FX][i] = s.f[SX_FX][i] + acc[0];
+        s.f[SX_FY][i] = s.f[SX_FY][i] + acc[1];
+        s.f[SX_FZ][i] = s.f[SX_FZ][i] + acc[2];
+        s.f[SX_TX][i] = s.f[SX_TX][i] + acc[3];
+        s.f[SX_TY][i] = s.f[SX_TY][i] + acc[4];
+        s.f[SX_TZ][i] = s.f[SX_TZ][i] + acc[5];
+    }
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) pe_sh[threadIdx.x] = pe_sh[threadIdx.x] + pe_sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = pe_sh[0];
+}
+
+__global__ void k_zero_counter(int *p) { *p = 0; }
+
+// explicit pairs (parity hook)
+template <int KIND>
+__global__ void k_pair_eval(DState s, long np, const int *pi, const int *pj, Params P, const double *pp, long ld,
+                            int solver, double *out)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= np) return;
+    int a = pi[t], b = pj[t];
+    PairOut o;
+    double lam = 0.0;
+    if (KIND == SX_ECP || KIND == SX_ECPR) {
+        double RIJ[3];
+        pair_rij(s, a, b, RIJ);
+        Sym3 A = load_sym(pp, ld, PP_AR, a), B = load_sym(pp, ld, PP_AR, b);
+        double fpsi = ecp_optim_dist(ECP_PSI, A, B, RIJ);
+        int it = 0, fb = 0;
+        if (KIND == SX_ECP) {
+            lam = ecp_solve_lambda<0>(solver, s, pp, ld, a, b, RIJ, fpsi, 0.5, it, fb);
+            ecp_forces<0>(s, pp, ld, a, b, P.p[0], RIJ, lam, o);
+        } else {
+            lam = ecp_solve_lambda<1>(solver, s, pp, ld, a, b, RIJ, fpsi, 0.5, it, fb);
+            ecp_forces<1>(s, pp, ld, a, b, 0.0, RIJ, lam, o);
+        }
+    } else {
+        for (int k = 0; k < 3; ++k) { o.ti[k] = 0.0; o.tj[k] = 0.0; }
+        d_pair<KIND>(s, a, b, P.p, o);
+    }
+    double *r = out + t * 11;
+    r[0] = o.pe;
+    for (int k = 0; k < 3; ++k) { r[1 + k] = o.f[k]; r[4 + k] = o.ti[k]; r[7 + k] = o.tj[k]; }
+    r[10] = lam;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static Params make_params(sx_ctx *c, int kind, const double *p, int np)
+{
+    Params P;
+    for (int k = 0; k < 8; ++k) P.p[k] = 0.0;
+    for (int k = 0; k < np && k < 8; ++k) P.p[k] = p[k];
+    if (kind == SX_GB || kind == SX_GBR) {
+        if (np < 2) P.p[1] = 1.0;   // mu (force.f90:240)
+        if (np < 3) P.p[2] = 2.0;   // nu
+    }
+    if (kind == SX_MORSE) P.p[4] = std::exp(-P.p[1] * (P.p[3] - P.p[2]));   // exp_cut (force.f90:224)
+    (void)c;
+    return P;
+}
+
+static void check_params(int kind, int np)
+{
+    int need = 0;
+    switch (kind) {
+    case SX_LJ: case SX_LJ12: case SX_DH: need = 3; break;
+    case SX_LJW: case SX_MORSE: need = 4; break;
+    case SX_WCA: need = 2; break;
+    case SX_GB: case SX_ECP: need = 1; break;
+    case SX_GBR: case SX_ECPR: need = 0; break;
+    default: throw sx_error{SX_ERR_ARG, "unknown force kind"};
+    }
+    if (np < need) throw sx_error{SX_ERR_ARG, "too few parameters for this force kind"};
+}
+
+template <int KIND, bool TORQUE>
+static void launch_gather(sx_ctx *c, const Params &P, int lpp, int nb_out[1])
+{
+    int n = c->n;
+    long threads = (long)n * lpp;
+    int nb = sx_blocks(threads, 256);
+    sx_ensure_part(c, nb, 1);
+    DState s = c->dstate();
+    switch (lpp) {
+    case 1: hipLaunchKernelGGL((k_gather<KIND, 1, TORQUE>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_part); break;
+    case 4: hipLaunchKernelGGL((k_gather<KIND, 4, TORQUE>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_part); break;
+    case 8: hipLaunchKernelGGL((k_gather<KIND, 8, TORQUE>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_part); break;
+    default: hipLaunchKernelGGL((k_gather<KIND, 16, TORQUE>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_part); break;
+    }
+    nb_out[0] = nb;
+}
+
+static int choose_lpp(sx_ctx *c, int kind)
+{
+    // enough lanes per particle to fill the chip for small systems; one lane
+    // per particle keeps the per-lane loop long for large ones
+    double avg = c->nl.nnz_full > 0 ? (double)c->nl.nnz_full / c->n : 32.0;
+    long waves1 = c->n / 64;
+    int lpp = 1;
+    while (lpp < 16 && waves1 * lpp < 4096 && avg / lpp >= 8) lpp *= 2;
+    if (lpp == 2) lpp = 4;
+    (void)kind;
+    return lpp;
+}
+
+static void ensure_pair_buffers(sx_ctx *c)
+{
+    long need = c->nl.cap_half;
+    if (need > c->pair_cap) {
+        if (c->d_pairbuf) HIPCHK(hipFree(c->d_pairbuf));
+        if (c->d_live) HIPCHK(hipFree(c->d_live));
+        if (c->d_work) HIPCHK(hipFree(c->d_work));
+        HIPCHK(hipMalloc(&c->d_pairbuf, sizeof(double) * 10 * need));
+        HIPCHK(hipMalloc(&c->d_live, need));
+        HIPCHK(hipMalloc(&c->d_work, sizeof(int) * (need + 64)));
+        c->pair_cap = need;
+        c->have_graph = false;
+    }
+    if (c->ecp_pp_cap < c->n) {
+        if (c->d_ecp_pp) HIPCHK(hipFree(c->d_ecp_pp));
+        HIPCHK(hipMalloc(&c->d_ecp_pp, sizeof(double) * PP_N * c->ld));
+        c->ecp_pp_cap = c->n;
+    }
+}
+
+// learned warm start: predictions for the work list (implemented with the MLP)
+const float *sx_ecp_mlp_predict(sx_ctx *c, const double *pp, long ld, const int *work, const int *nwork, long cap);
+
+static void ecp_forces_impl(sx_ctx *c, int kind, const Params &P)
+{
+    ensure_pair_buffers(c);
+    NList &nl = c->nl;
+    DState s = c->dstate();
+    int n = c->n;
+    long ld = c->ld;
+    hipLaunchKernelGGL(k_ecp_pp, dim3(sx_blocks(n, 256)), dim3(256), 0, c->stream, s, c->d_ecp_pp, ld);
+    int *nwork = c->d_flag + 4;
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, c->stream, nwork);
+    EcpScreen sc;
+    sc.variant = kind == SX_ECPR ? 1 : 0;
+    sc.cutoff = kind == SX_ECPR ? 0.0 : P.p[0];
+    long cap = nl.cap_half;
+    int nbh = sx_blocks(cap, 256);
+    hipLaunchKernelGGL(k_ecp_screen, dim3(nbh), dim3(256), 0, c->stream, s, c->d_ecp_pp, ld, nl.half_i, nl.half_j,
+                       c->d_flag + 2, sc, c->d_live, c->d_work, nwork, cap);
+    const float *lam0 = nullptr;
+    if (c->ecp_solver == SX_ECP_LEARNED) lam0 = sx_ecp_mlp_predict(c, c->d_ecp_pp, ld, c->d_work, nwork, cap);
+    if (sc.variant == 1)
+        hipLaunchKernelGGL(k_ecp_solve<1>, dim3(nbh), dim3(128), 0, c->stream, s, c->d_ecp_pp, ld, nl.half_i, nl.half_j,
+                           c->d_work, nwork, cap, sc, c->ecp_solver, lam0, c->d_pairbuf, c->d_counts, c->d_err);
+    else
+        hipLaunchKernelGGL(k_ecp_solve<0>, dim3(nbh), dim3(128), 0, c->stream, s, c->d_ecp_pp, ld, nl.half_i, nl.half_j,
+                           c->d_work, nwork, cap, sc, c->ecp_solver, lam0, c->d_pairbuf, c->d_counts, c->d_err);
+    int lpp = 4;
+    int nb = sx_blocks((long)n * lpp, 256);
+    sx_ensure_part(c, nb, 1);
+    hipLaunchKernelGGL(k_aniso_apply<4>, dim3(nb), dim3(256), 0, c->stream, s, nl.row_start, nl.nbr, nl.pairk,
+                       c->d_live, c->d_pairbuf, c->d_part);
+    int id = SX_POT_ENG;
+    sx_reduce_to_scalar(c, nb, 1, &id, 1, 1);
+}
+
+void sx_forces_impl(sx_ctx *c, int kind, const double *p, int np)
+{
+    check_params(kind, np);
+    if (!c->nl.init) throw sx_error{SX_ERR_STATE, "neighbour list not initialised (sx_nlist_init)"};
+    Params P = make_params(c, kind, p, np);
+    if (kind == SX_ECP || kind == SX_ECPR) {
+        ecp_forces_impl(c, kind, P);
+        return;
+    }
+    int lpp = choose_lpp(c, kind);
+    int nb[1];
+    switch (kind) {
+    case SX_LJ: launch_gather<SX_LJ, false>(c, P, lpp, nb); break;
+    case SX_LJ12: launch_gather<SX_LJ12, false>(c, P, lpp, nb); break;
+    case SX_LJW: launch_gather<SX_LJW, false>(c, P, lpp, nb); break;
+    case SX_WCA: launch_gather<SX_WCA, false>(c, P, lpp, nb); break;
+    case SX_MORSE: launch_gather<SX_MORSE, false>(c, P, lpp, nb); break;
+    case SX_DH: launch_gather<SX_DH, false>(c, P, lpp, nb); break;
+    case SX_GB: launch_gather<SX_GB, true>(c, P, lpp, nb); break;
+    case SX_GBR: launch_gather<SX_GBR, true>(c, P, lpp, nb); break;
+    }
+    int id = SX_POT_ENG;
+    sx_reduce_to_scalar(c, nb[0], 1, &id, 1, 1);
+}
+
+extern "C" {
+
+int sx_forces(sx_ctx *c, int kind, const double *params, int nparams)
+{
+    return guard(c, [&] {
+        need_system(c);
+        sx_forces_impl(c, kind, params, nparams);
+    });
+}
+
+int sx_set_ecp_solver(sx_ctx *c, int solver)
+{
+    return guard(c, [&] {
+        if (solver < SX_ECP_BRENT_REFERENCE || solver > SX_ECP_LEARNED) throw sx_error{SX_ERR_ARG, "bad solver"};
+        if (solver == SX_ECP_LEARNED && !c->mlp) throw sx_error{SX_ERR_STATE, "no lambda* network loaded"};
+        c->ecp_solver = solver;
+        c->have_graph = false;
+    });
+}
+
+int sx_ecp_stats(sx_ctx *c, long out[4])
+{
+    return guard(c, [&] {
+        long dev[8];
+        HIPCHK(hipMemcpyAsync(dev, c->d_counts, sizeof(long) * 8, hipMemcpyDeviceToHost, c->stream));
+        int sizes[8];
+        HIPCHK(hipMemcpyAsync(sizes, c->d_flag, sizeof(int) * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        out[0] = sizes[2];     // half pairs screened (last call)
+        out[1] = dev[0];       // pairs solved (cumulative)
+        out[2] = dev[1];       // solver iterations (cumulative)
+        out[3] = dev[2];       // fallbacks (cumulative)
+    });
+}
+
+int sx_pair_eval(sx_ctx *c, int kind, const double *params, int nparams, long np, const int *i1, const int *j1,
+                 double *pe, double *f, double *ti, double *tj, double *lambda)
+{
+    return guard(c, [&] {
+        need_system(c);
+        check_params(kind, nparams);
+        if (np <= 0) return;
+        for (long t = 0; t < np; ++t)
+            if (i1[t] < 1 || i1[t] > c->n || j1[t] < 1 || j1[t] > c->n)
+                throw sx_error{SX_ERR_ARG, "Error: Atom index does not exist in system"};
+        Params P = make_params(c, kind, params, nparams);
+        std::vector<int> hi(np), hj(np);
+        for (long t = 0; t < np; ++t) { hi[t] = i1[t] - 1; hj[t] = j1[t] - 1; }
+        int *di = nullptr, *dj = nullptr;
+        double *dout = nullptr;
+        HIPCHK(hipMalloc(&di, sizeof(int) * np));
+        HIPCHK(hipMalloc(&dj, sizeof(int) * np));
+        HIPCHK(hipMalloc(&dout, sizeof(double) * 11 * np));
+        HIPCHK(hipMemcpyAsync(di, hi.data(), sizeof(int) * np, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(dj, hj.data(), sizeof(int) * np, hipMemcpyHostToDevice, c->stream));
+        DState s = c->dstate();
+        if (kind == SX_ECP || kind == SX_ECPR) {
+            if (c->ecp_pp_cap < c->n) {
+                if (c->d_ecp_pp) HIPCHK(hipFree(c->d_ecp_pp));
+                HIPCHK(hipMalloc(&c->d_ecp_pp, sizeof(double) * PP_N * c->ld));
+                c->ecp_pp_cap = c->n;
+            }
+            hipLaunchKernelGGL(k_ecp_pp, dim3(sx_blocks(c->n, 256)), dim3(256), 0, c->stream, s, c->d_ecp_pp, c->ld);
+        }
+        int solver = c->ecp_solver == SX_ECP_LEARNED ? SX_ECP_CONVERGED : c->ecp_solver;
+        dim3 g(sx_blocks(np, 128)), b(128);
+        switch (kind) {
+        case SX_LJ: hipLaunchKernelGGL(k_pair_eval<SX_LJ>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, c->ld, solver, dout); break;
+        case SX_LJ12: hipLaunchKernelGGL(k_pair_eval<SX_LJ12>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, c->ld, solver, dout); break;
+        case SX_LJW: hipLaunchKernelGGL(k_pair_eval<SX_LJW>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, c->ld, solver, dout); break;
+        case SX_WCA: hipLaunchKernelGGL(k_pair_eval<SX_WCA>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, c->ld, solver, dout); break;
+        case SX_MORSE: hipLaunchKernelGGL(k_pair_eval<SX_MORSE>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, c->ld, solver, dout); break;
+        case SX_DH: hipLaunchKernelGGL(k_pair_eval<SX_DH>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, c->ld, solver, dout); break;
+        case SX_GB: hipLaunchKernelGGL(k_pair_eval<SX_GB>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, c->ld, solver, dout); break;
+        case SX_GBR: hipLaunchKernelGGL(k_pair_eval<SX_GBR>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, c->ld, solver, dout); break;
+        case SX_ECP: hipLaunchKernelGGL(k_pair_eval<SX_ECP>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, c->ld, solver, dout); break;
+        case SX_ECPR: hipLaunchKernelGGL(k_pair_eval<SX_ECPR>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, c->ld, solver, dout); break;
+        }
+        std::vector<double> h(11 * np);
+        HIPCHK(hipMemcpyAsync(h.data(), dout, sizeof(double) * 11 * np, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        hipFree(di); hipFree(dj); hipFree(dout);
+        for (long t = 0; t < np; ++t) {
+            const double *r = &h[11 * t];
+            if (pe) pe[t] = r[0];
+            for (int k = 0; k < 3; ++k) {
+                if (f) f[3 * t + k] = r[1 + k];
+                if (ti) ti[3 * t + k] = r[4 + k];
+                if (tj) tj[3 * t + k] = r[7 + k];
+            }
+            if (lambda) lambda[t] = r[10];
+        }
+    });
+}
+
+}  // extern "C"

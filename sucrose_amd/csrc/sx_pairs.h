@@ -1,0 +1,316 @@
+// Device pair kernels of force.f90, FP64, one canonical evaluation (I = a,
+// J = b, RIJ = R(a) - R(b)) per call.  Expressions keep the reference's
+// evaluation order (the translation unit is built with -ffp-contract=off), so
+// a pair's PE, FXIJ, TXI, TXJ equal the reference's to the last bit for the
+// same FP64 inputs, transcendental functions aside.
+#pragma once
+#include "sx_internal.h"
+
+struct PairOut {
+    double pe;
+    double f[3];
+    double ti[3];
+    double tj[3];
+};
+
+__device__ __forceinline__ void pair_rij(const DState &s, int a, int b, double r[3])
+{
+    r[0] = s.f[SX_RX][a] - s.f[SX_RX][b];
+    r[1] = s.f[SX_RY][a] - s.f[SX_RY][b];
+    r[2] = s.f[SX_RZ][a] - s.f[SX_RZ][b];
+    if (s.periodic) {
+        r[0] = sx_mimg(r[0], s.box[0]);
+        r[1] = sx_mimg(r[1], s.box[1]);
+        r[2] = sx_mimg(r[2], s.box[2]);
+    }
+}
+
+// lj_calculate_forces (force.f90:6-49)
+__device__ __forceinline__ void d_lj(const DState &s, int a, int b, const double *p, PairOut &o)
+{
+    double sigma = p[0], epsilon = p[1], cutoff = p[2];
+    double r[3];
+    pair_rij(s, a, b, r);
+    double rij_sq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    double rcut_sq = cutoff * cutoff;
+    bool cmask = rij_sq < rcut_sq;
+    double sr2 = (sigma * sigma) / rij_sq, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
+    double sc2 = (sigma * sigma) / rcut_sq, sc6 = sc2 * sc2 * sc2, sc12 = sc6 * sc6;
+    double pot = 4.0 * epsilon * (sr12 - sr6);
+    double pot_cut = 4.0 * epsilon * (sc12 - sc6);
+    double coeff = 24.0 * epsilon * (2.0 * sr12 - sr6);
+    o.pe = cmask ? (pot - pot_cut) : 0.0;
+    for (int k = 0; k < 3; ++k) o.f[k] = cmask ? (coeff * r[k] / rij_sq) : 0.0;
+}
+
+// lj12_calculate_forces (force.f90:51-93)
+__device__ __forceinline__ void d_lj12(const DState &s, int a, int b, const double *p, PairOut &o)
+{
+    double sigma = p[0], epsilon = p[1], cutoff = p[2];
+    double r[3];
+    pair_rij(s, a, b, r);
+    o.pe = 0.0; o.f[0] = o.f[1] = o.f[2] = 0.0;
+    double rij_sq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    double rcut_sq = cutoff * cutoff;
+    if (rij_sq < rcut_sq) {
+        double sr2 = (sigma * sigma) / rij_sq, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
+        double sc2 = (sigma * sigma) / rcut_sq, sc6 = sc2 * sc2 * sc2, sc12 = sc6 * sc6;
+        double pot = epsilon * (sr12 - 2.0 * sr6);
+        double pot_cut = epsilon * (sc12 - 2.0 * sc6);
+        double coeff = 12.0 * epsilon * (sr12 - sr6);
+        o.pe = pot - pot_cut;
+        for (int k = 0; k < 3; ++k) o.f[k] = coeff * r[k] / rij_sq;
+    }
+}
+
+// ljw_calculate_forces (force.f90:95-146)
+__device__ __forceinline__ void d_ljw(const DState &s, int a, int b, const double *p, PairOut &o)
+{
+    double sigma = p[0], epsilon = p[1], cutoff = p[2], wf = p[3];
+    double r[3];
+    pair_rij(s, a, b, r);
+    o.pe = 0.0; o.f[0] = o.f[1] = o.f[2] = 0.0;
+    double rij_sq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    double rij_mag = sqrt(rij_sq);
+    if (rij_mag < (cutoff + wf)) {
+        double sr;
+        if (rij_mag < sigma) sr = sigma / rij_mag;
+        else if ((rij_mag >= sigma) && (rij_mag < (sigma + wf))) sr = 1.0;
+        else sr = sigma / (rij_mag - wf);
+        double sr2 = sr * sr, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
+        double sc = sigma / cutoff, sc2 = sc * sc, sc6 = sc2 * sc2 * sc2, sc12 = sc6 * sc6;
+        double pot = epsilon * (sr12 - 2.0 * sr6);
+        double pot_cut = epsilon * (sc12 - 2.0 * sc6);
+        double coeff = 12.0 * epsilon * (sr12 - sr6) * (sr / sigma);
+        o.pe = pot - pot_cut;
+        for (int k = 0; k < 3; ++k) o.f[k] = coeff * r[k] / rij_mag;
+    }
+}
+
+// wca_calculate_forces (force.f90:155-190)
+__device__ __forceinline__ void d_wca(const DState &s, int a, int b, const double *p, PairOut &o)
+{
+    double sigma = p[0], epsilon = p[1];
+    double r[3];
+    pair_rij(s, a, b, r);
+    o.pe = 0.0; o.f[0] = o.f[1] = o.f[2] = 0.0;
+    double rij_sq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    double sigma_sq = sigma * sigma;
+    if (rij_sq <= sigma_sq) {
+        double sr2 = sigma_sq / rij_sq, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
+        o.pe = epsilon * (sr12 - 2.0 * sr6) + epsilon;
+        double coeff = 12.0 * epsilon * (sr12 - sr6);
+        for (int k = 0; k < 3; ++k) o.f[k] = coeff * r[k] / rij_sq;
+    }
+}
+
+// morse_calculate_forces (force.f90:199-233)
+__device__ __forceinline__ void d_morse(const DState &s, int a, int b, const double *p, PairOut &o)
+{
+    double dissoc = p[0], width = p[1], rmin = p[2], cutoff = p[3];
+    double r[3];
+    pair_rij(s, a, b, r);
+    o.pe = 0.0; o.f[0] = o.f[1] = o.f[2] = 0.0;
+    double rij_sq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    double rij_mag = sqrt(rij_sq);
+    if (rij_mag < cutoff) {
+        double e = exp(-width * (rij_mag - rmin));
+        double ec = p[4];   // exp(-width*(cutoff-rmin)), evaluated on the host
+        double pot = dissoc * ((1.0 - e) * (1.0 - e) - 1.0);
+        double pot_cut = dissoc * ((1.0 - ec) * (1.0 - ec) - 1.0);
+        o.pe = pot - pot_cut;
+        for (int k = 0; k < 3; ++k) o.f[k] = -2.0 * width * dissoc * (1.0 - e) * e * r[k] / rij_mag;
+    }
+}
+
+// dh_calculate_forces (force.f90:2589-2617)
+__device__ __forceinline__ void d_dh(const DState &s, int a, int b, const double *p, PairOut &o)
+{
+    const double PI_REF = 3.1415926535897932384626433832795;
+    double cutoff = p[0], db_len = p[1], dielectric = p[2];
+    double r[3];
+    pair_rij(s, a, b, r);
+    double rij_sq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    double rij_mag = sqrt(rij_sq);
+    bool cmask = rij_mag < cutoff;
+    double pe = (s.f[SX_CHARGE][a] * s.f[SX_CHARGE][b] * exp(-rij_mag / db_len)) / (4.0 * PI_REF * dielectric * rij_mag);
+    double fac = pe * (1.0 + rij_mag / db_len);
+    o.pe = cmask ? pe : 0.0;
+    for (int k = 0; k < 3; ++k) o.f[k] = cmask ? (fac * r[k] / rij_sq) : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Gay-Berne (force.f90:252-533)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double d_ipow(double x, int n)
+{
+    // binary powering, the lowering of the reference's constant integer powers
+    if (n == 0) return 1.0;
+    double result = 1.0, base = x;
+    bool first = true;
+    while (n > 0) {
+        if (n & 1) {
+            if (first) { result = base; first = false; }
+            else result = result * base;
+        }
+        n >>= 1;
+        if (n) base = base * base;
+    }
+    return result;
+}
+
+struct GbGeo {
+    double ru1, ru2, uu, rsq;
+    double u1[3], u2[3], r[3];
+};
+
+__device__ __forceinline__ double gb_g(double chie, const GbGeo &q)
+{
+    double t1 = (q.ru1 + q.ru2) * (q.ru1 + q.ru2) / (1 + chie * q.uu);
+    double t2 = (q.ru1 - q.ru2) * (q.ru1 - q.ru2) / (1 - chie * q.uu);
+    return 1 - (chie / 2.0 / q.rsq) * (t1 + t2);
+}
+
+__device__ __forceinline__ void gb_dgdr(double chie, const GbGeo &q, double g, double res[3])
+{
+    double a = (q.ru1 + q.ru2) / (1 + chie * q.uu);
+    double b = (q.ru1 - q.ru2) / (1 - chie * q.uu);
+    for (int k = 0; k < 3; ++k) {
+        double t1 = a * (q.u1[k] + q.u2[k]);
+        double t2 = b * (q.u1[k] - q.u2[k]);
+        res[k] = (-chie / q.rsq) * (t1 + t2) + (2 * q.r[k] / q.rsq) * (1 - g);
+    }
+}
+
+__device__ __forceinline__ void gb_dgdu(double chie, const GbGeo &q, int which, double res[3])
+{
+    double t1 = (q.ru1 + q.ru2) / (1 + chie * q.uu);
+    double t2 = (q.ru1 - q.ru2) / (1 - chie * q.uu);
+    for (int k = 0; k < 3; ++k) {
+        if (which == 1)
+            res[k] = (-chie * q.r[k] / q.rsq) * (t1 + t2) + ((chie * chie * q.u2[k]) / (2 * q.rsq)) * (t1 * t1 - t2 * t2);
+        else
+            res[k] = (-chie * q.r[k] / q.rsq) * (t1 - t2) + ((chie * chie * q.u1[k]) / (2 * q.rsq)) * (t1 * t1 - t2 * t2);
+    }
+}
+
+__device__ __forceinline__ void quat_u(const DState &s, int i, double u[3])
+{
+    double w = s.f[SX_QW][i], x = s.f[SX_QX][i], y = s.f[SX_QY][i], z = s.f[SX_QZ][i];
+    u[0] = 2.0 * (x * z + w * y);
+    u[1] = 2.0 * (y * z - w * x);
+    u[2] = w * w - x * x - y * y + z * z;
+}
+
+__device__ __forceinline__ void cross3(const double a[3], const double b[3], double o[3])
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// gb (variant 0, cutoff + switch) and gbr (variant 1); p = cutoff, mu, nu,
+// chi' of particle a's energies is recomputed per pair as in the reference
+template <int VARIANT>
+__device__ __forceinline__ void d_gb(const DState &s, int a, int b, const double *p, PairOut &o)
+{
+    const double sigma_0 = 1.0;
+    double cutoff = p[0];
+    int mu = (int)p[1], nu = (int)p[2];
+    GbGeo q;
+    o.pe = 0.0;
+    for (int k = 0; k < 3; ++k) { o.f[k] = 0.0; o.ti[k] = 0.0; o.tj[k] = 0.0; }
+    pair_rij(s, a, b, q.r);
+    q.rsq = q.r[0] * q.r[0] + q.r[1] * q.r[1] + q.r[2] * q.r[2];
+    double rij_mag = sqrt(q.rsq);
+    double rhat[3] = {q.r[0] / rij_mag, q.r[1] / rij_mag, q.r[2] / rij_mag};
+    quat_u(s, a, q.u1);
+    quat_u(s, b, q.u2);
+    double lpar = s.f[SX_SIG_Z][a], lprp = s.f[SX_SIG_X][a];
+    double epar = s.f[SX_EPS_Z][a], eprp = s.f[SX_EPS_X][a];
+    q.ru1 = q.r[0] * q.u1[0] + q.r[1] * q.u1[1] + q.r[2] * q.u1[2];
+    q.ru2 = q.r[0] * q.u2[0] + q.r[1] * q.u2[1] + q.r[2] * q.u2[2];
+    q.uu = q.u1[0] * q.u2[0] + q.u1[1] * q.u2[1] + q.u1[2] * q.u2[2];
+    double chi = (lpar * lpar - lprp * lprp) / (lpar * lpar + lprp * lprp);
+    double ep = eprp, ea = epar;
+    if (mu != 1) {
+        double muinv = 1.0 / mu;
+        ep = pow(eprp, muinv);
+        ea = pow(epar, muinv);
+    }
+    double xhi = (ep - ea) / (ep + ea);
+    double gchi = gb_g(chi, q);
+    double gxhi = gb_g(xhi, q);
+    double eps1 = 1.0 / sqrt(1 - (chi * q.uu) * (chi * q.uu));
+    double eps2 = gxhi / s.f[SX_EPS_Z][a];
+    double sigma = lprp / sqrt(gchi);
+    double sigma_0_inv = 1.0 / sigma_0;
+    double g32 = sigma / gchi;
+    double switch_ = 0, factor = 0, sx = 0, sdx = 0, rlim;
+    if (VARIANT == 0) {
+        switch_ = 0.9 * (sigma + cutoff - sigma_0);
+        factor = 1.0 / ((sigma + cutoff - sigma_0) - switch_);
+        double xx = (rij_mag - switch_) * factor;
+        sx = 1.0 - 6.0 * xx * xx * xx * xx * xx + 15.0 * xx * xx * xx * xx - 10.0 * xx * xx * xx;
+        sdx = -30.0 * xx * xx * xx * xx + 60.0 * xx * xx * xx - 30.0 * xx * xx;
+        rlim = sigma + cutoff - sigma_0;
+    } else {
+        rlim = sigma;
+    }
+    bool inside = (VARIANT == 0) ? (rij_mag < rlim) : (rij_mag <= rlim);
+    if (!inside) return;
+    double sr = sigma_0 / (rij_mag - sigma + sigma_0);
+    double sr2 = sr * sr, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
+    double srx = (sr12 - 2.0 * sr6);
+    if (VARIANT == 1) srx = srx + 1.0;
+    double srd = 12.0 * (sr6 - sr12) * sr;
+    double e1n = d_ipow(eps1, nu), e2m = d_ipow(eps2, mu);
+    double e2m1 = d_ipow(eps2, mu - 1), e1n2 = d_ipow(eps1, nu + 2);
+    double dgr_c[3], dgr_x[3], dgu1_x[3], dgu1_c[3], dgu2_x[3], dgu2_c[3];
+    gb_dgdr(chi, q, gchi, dgr_c);
+    gb_dgdr(xhi, q, gxhi, dgr_x);
+    gb_dgdu(xhi, q, 1, dgu1_x);
+    gb_dgdu(chi, q, 1, dgu1_c);
+    gb_dgdu(xhi, q, 2, dgu2_x);
+    gb_dgdu(chi, q, 2, dgu2_c);
+    double TI[3], TJ[3];
+    for (int k = 0; k < 3; ++k) {
+        double dR_dr = sigma_0_inv * (rhat[k] + 0.5 * dgr_c[k] * g32);
+        double de2_dr = e1n * (mu * e2m1) * dgr_x[k];
+        double de1_du1 = e2m * (nu * e1n2) * (chi * chi * q.uu * q.u2[k]);
+        double de2_du1 = e1n * (mu * e2m1) * dgu1_x[k];
+        double dR_du1 = srd * 0.5 * sigma_0_inv * dgu1_c[k] * g32;
+        double de1_du2 = e2m * (nu * e1n2) * (chi * chi * q.uu * q.u1[k]);
+        double de2_du2 = e1n * (mu * e2m1) * dgu2_x[k];
+        double dR_du2 = srd * 0.5 * sigma_0_inv * dgu2_c[k] * g32;
+        o.f[k] = (-1.0) * (e1n * e2m * srd * dR_dr + de2_dr * srx);
+        TI[k] = (-1.0) * (srx * (de1_du1 + de2_du1) + (e1n * e2m * dR_du1));
+        TJ[k] = (-1.0) * (srx * (de1_du2 + de2_du2) + (e1n * e2m * dR_du2));
+    }
+    double pot = e1n * e2m * srx;
+    cross3(q.u1, TI, o.ti);
+    cross3(q.u2, TJ, o.tj);
+    if (VARIANT == 0 && (rij_mag >= switch_) && (rij_mag <= rlim)) {
+        for (int k = 0; k < 3; ++k) {
+            o.f[k] = sx * o.f[k] - sdx * pot * factor;
+            o.ti[k] = sx * o.ti[k] - sdx * pot * factor;
+            o.tj[k] = sx * o.tj[k] - sdx * pot * factor;
+        }
+        pot = sx * pot;
+    }
+    o.pe = pot;
+}
+
+// isotropic dispatch
+template <int KIND>
+__device__ __forceinline__ void d_pair(const DState &s, int a, int b, const double *p, PairOut &o)
+{
+    if (KIND == SX_LJ) d_lj(s, a, b, p, o);
+    else if (KIND == SX_LJ12) d_lj12(s, a, b, p, o);
+    else if (KIND == SX_LJW) d_ljw(s, a, b, p, o);
+    else if (KIND == SX_WCA) d_wca(s, a, b, p, o);
+    else if (KIND == SX_MORSE) d_morse(s, a, b, p, o);
+    else if (KIND == SX_DH) d_dh(s, a, b, p, o);
+    else if (KIND == SX_GB) d_gb<0>(s, a, b, p, o);
+    else if (KIND == SX_GBR) d_gb<1>(s, a, b, p, o);
+}

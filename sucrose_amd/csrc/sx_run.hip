@@ -1,0 +1,97 @@
+// Whole-step driver: one step of the README loop (README.md:83-115) captured
+// once as a HIP graph and replayed.  The Langevin step counter and the
+// neighbour-list rebuild decision live on the device, so replays need no host
+// round trip.
+#include "sx_internal.h"
+
+#include <cstring>
+
+__global__ void k_step_set(int *dstep, int v) { *dstep = v; }
+__global__ void k_step_inc2(int *dstep) { *dstep = *dstep + 1; }
+
+static void enqueue_step(sx_ctx *c, const sx_step_spec &sp)
+{
+    switch (sp.integrator) {
+    case SX_INT_VV: sx_vv_initial_impl(c, sp.dt, false); break;
+    case SX_INT_NHT:
+        sx_nht_impl(c, sp.dt, nullptr, 0, false);
+        sx_qq_initial_impl(c, sp.dt);
+        break;
+    case SX_INT_LGV: sx_lgv_initial_impl(c, sp.dt, 0, c->d_step, sp.damping, false); break;
+    case SX_INT_LGVC: sx_lgv_initial_impl(c, sp.dt, 0, c->d_step, sp.damping, true); break;
+    }
+    sx_nlist_flag_impl(c);
+    sx_nlist_build_impl(c, true);
+    sx_init_forces_impl(c);
+    for (int f = 0; f < sp.nforces; ++f) {
+        int np = 0;
+        switch (sp.kinds[f]) {
+        case SX_LJ: case SX_LJ12: case SX_DH: np = 3; break;
+        case SX_LJW: case SX_MORSE: np = 4; break;
+        case SX_WCA: np = 2; break;
+        case SX_GB: case SX_GBR: np = 3; break;
+        case SX_ECP: np = 1; break;
+        default: np = 0;
+        }
+        sx_forces_impl(c, sp.kinds[f], sp.params[f], np);
+    }
+    if (sp.bonded) sx_bonded_impl(c);
+    switch (sp.integrator) {
+    case SX_INT_VV: sx_vv_final_impl(c, sp.dt); break;
+    case SX_INT_NHT:
+        sx_nht_impl(c, sp.dt, nullptr, 1, false);
+        sx_qq_final_impl(c, sp.dt);
+        break;
+    case SX_INT_LGV: sx_lgv_final_impl(c, sp.dt, sp.damping, false); break;
+    case SX_INT_LGVC: sx_lgv_final_impl(c, sp.dt, sp.damping, true); break;
+    }
+    sx_thermo_impl(c);
+    if (sp.integrator == SX_INT_LGV || sp.integrator == SX_INT_LGVC)
+        hipLaunchKernelGGL(k_step_inc2, dim3(1), dim3(1), 0, c->stream, c->d_step);
+}
+
+static bool same_spec(const sx_step_spec &a, const sx_step_spec &b)
+{
+    return memcmp(&a, &b, sizeof(sx_step_spec)) == 0;
+}
+
+extern "C" int sx_run(sx_ctx *c, const sx_step_spec *spec, int nsteps)
+{
+    return guard(c, [&] {
+        need_system(c);
+        if (!spec || nsteps < 0 || spec->nforces < 0 || spec->nforces > 4) throw sx_error{SX_ERR_ARG, "bad step spec"};
+        if (!c->nl.init) throw sx_error{SX_ERR_STATE, "neighbour list not initialised (sx_nlist_init)"};
+        if (spec->integrator != SX_INT_VV && !c->have_target_temp)
+            throw sx_error{SX_ERR_NOTEMP, "Error: Target temperature not set."};
+        if (!c->d_step) HIPCHK(hipMalloc(&c->d_step, sizeof(int)));
+        hipLaunchKernelGGL(k_step_set, dim3(1), dim3(1), 0, c->stream, c->d_step, spec->first_step);
+        sx_step_spec key = *spec;
+        key.first_step = 0;
+        if (!c->have_graph || !same_spec(key, c->graph_spec)) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            if (c->graph_exec) { hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
+            if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+            // one eager step sizes every lazily allocated buffer
+            sx_step_spec warm = *spec;
+            enqueue_step(c, warm);
+            sx_check_device_error(c);
+            hipLaunchKernelGGL(k_step_set, dim3(1), dim3(1), 0, c->stream, c->d_step, spec->first_step + 1);
+            --nsteps;
+            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue_step(c, *spec);
+            } catch (...) {
+                hipGraph_t g;
+                hipStreamEndCapture(c->stream, &g);
+                if (g) hipGraphDestroy(g);
+                throw;
+            }
+            HIPCHK(hipStreamEndCapture(c->stream, &c->graph));
+            HIPCHK(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+            c->graph_spec = key;
+            c->have_graph = true;
+        }
+        for (int t = 0; t < nsteps; ++t) HIPCHK(hipGraphLaunch(c->graph_exec, c->stream));
+        sx_check_device_error(c);
+    });
+}

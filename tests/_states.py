@@ -1,0 +1,97 @@
+"""Seeded synthetic states shared by the tests (host FP64, oracle layout)."""
+from __future__ import annotations
+
+import numpy as np
+
+from oracle.oracle import FIELDS, State
+
+
+def rand_quats(rng, n):
+    q = rng.normal(size=(n, 4))
+    return q / np.linalg.norm(q, axis=1)[:, None]
+
+
+def base_state(rng, n, box, sig=(1.0, 1.0, 1.0), eps=(1.0, 1.0, 1.0), mass=(0.5, 2.0), T=1.5,
+               forces=True, charges=False):
+    box = np.broadcast_to(np.asarray(box, dtype=np.float64), (3,)).copy()
+    st = State(n, box)
+    for k, f in enumerate(["rx", "ry", "rz"]):
+        st[f] = rng.uniform(-box[k] / 2, box[k] / 2, n)
+    for f in ["vx", "vy", "vz", "lx", "ly", "lz"]:
+        st[f] = rng.normal(size=n)
+    if forces:
+        for f in ["fx", "fy", "fz", "tx", "ty", "tz"]:
+            st[f] = rng.normal(size=n) * 5.0
+    q = rand_quats(rng, n)
+    for k, f in enumerate(["qw", "qx", "qy", "qz"]):
+        st[f] = q[:, k]
+    st["mass"] = rng.uniform(mass[0], mass[1], n)
+    st["inv_mass"] = 1.0 / st["mass"]
+    st["charge"] = rng.choice([-1.0, 1.0], n) if charges else 0.0
+    st["sig_x"], st["sig_y"], st["sig_z"] = sig
+    st["eps_x"], st["eps_y"], st["eps_z"] = eps
+    for a, (b, c) in {"ixx": ("sig_y", "sig_z"), "iyy": ("sig_x", "sig_z"), "izz": ("sig_x", "sig_y")}.items():
+        st[a] = (st[b] ** 2 + st[c] ** 2) / 20.0
+        st[a + "_inv"] = 1.0 / st[a]
+    st["target_temp"] = T
+    st["dof"] = 6 * n - 3
+    return st
+
+
+def chains(rng, nchains, nbeads, box, bond=1.0):
+    """Random-walk chains; returns (positions (n,3), bbi, bbj) 0-based."""
+    L = np.asarray(box, dtype=np.float64)
+    pos = []
+    bi, bj = [], []
+    for c in range(nchains):
+        p = rng.uniform(-L / 2, L / 2)
+        pts = [p]
+        for b in range(nbeads - 1):
+            d = rng.normal(size=3)
+            d /= np.linalg.norm(d)
+            pts.append(pts[-1] + bond * d)
+            bi.append(c * nbeads + b)
+            bj.append(c * nbeads + b + 1)
+        pos += pts
+    pos = np.array(pos)
+    pos = pos - np.round(pos / L) * L
+    return pos, np.array(bi, np.int32), np.array(bj, np.int32)
+
+
+def to_device(sysdev, st: State, nlist=None):
+    """Load an oracle State into a sucrose_amd.System."""
+    sysdev.set_num_atoms(st.n, 1)
+    sysdev.set_box(st.box[0], st.box[1], st.box[2], periodic=st.periodic)
+    for k, f in enumerate(FIELDS):
+        sysdev.upload(f, st.buf[:, k])
+    for name, idx in [("POT_ENG", "pot_eng"), ("KIN_ENG", "kin_eng"), ("KIN_ROT", "kin_rot"),
+                      ("VIRIAL_BONDED", "virial_bonded"), ("TARGET_TEMP", "target_temp"), ("DOF", "dof"),
+                      ("DENSITY", "density"), ("HB0", "hb0"), ("HB1", "hb1"), ("HB2", "hb2"),
+                      ("ETA0", "eta0"), ("ETA1", "eta1"), ("ETA2", "eta2"),
+                      ("PETA0", "peta0"), ("PETA1", "peta1"), ("PETA2", "peta2")]:
+        sysdev.set_scalar(name, float(st[idx]))
+    if st.nc > 0:
+        sysdev.set_constraints(st.bbi + 1, st.bbj + 1, st.bb_len)
+
+
+def from_device(sysdev, st: State) -> State:
+    out = st.copy()
+    for k, f in enumerate(FIELDS):
+        out.buf[:, k] = sysdev.download(f)
+    for name, idx in [("POT_ENG", "pot_eng"), ("KIN_ENG", "kin_eng"), ("KIN_ROT", "kin_rot"),
+                      ("VIRIAL_BONDED", "virial_bonded"), ("TEMP", "temp"), ("PRESSURE", "pressure"),
+                      ("VIRIAL", "virial"), ("ETA0", "eta0"), ("ETA1", "eta1"), ("ETA2", "eta2"),
+                      ("PETA0", "peta0"), ("PETA1", "peta1"), ("PETA2", "peta2"),
+                      ("HB0", "hb0"), ("HB1", "hb1"), ("HB2", "hb2")]:
+        out[idx] = sysdev.get_scalar(name)
+    return out
+
+
+def rel_err(a, b, scale=None):
+    """max |a-b| / scale, scale defaulting to the RMS of b (>0)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if scale is None:
+        scale = np.sqrt(np.mean(b * b)) if b.size else 1.0
+    scale = scale if scale > 0 else 1.0
+    return float(np.max(np.abs(a - b)) / scale) if b.size else 0.0
