@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Benchmark: SUCROSE ECP Langevin step on one MI355X.
+
+Workload (north-star configuration): 32768 prolate ellipsoids (semi-axes
+0.5, 0.5, 1.5), packing fraction 0.35, repulsive ECP (ecpr_calculate_forces,
+force.f90:1387) with the reference's Brent contact solve, translational and
+rotational Langevin (lgv_initial_step / lgv_final_step, integrators.f90:507,
+:629) at T* = 1, gamma = 1, dt = 0.002, Philox4x32-10 noise, neighbour list
+cutoff 3.1 + skin 0.3.  One "step" = initial step, list update, force
+evaluation, final step, thermo -- the reference driver loop (README.md:83-115).
+Inputs are resident in HBM; the timed region replays a captured HIP graph.
+
+Prints one JSON line (metric/value/unit/... plus roofline and cpu_baseline).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s on 1 MI355X (GB, ECP); learned vs plain ECP at 1e-5 force err"
+PEAK_FP64_TFLOPS = 78.6
+# algorithmic FP64 operation counts (add, mul, div, sqrt, round = 1 each)
+FLOP_OPTIM_DIST = 62        # one optim_dist evaluation (blend + 3x3 Cholesky solve + dot)
+FLOP_BRENT_ITER = 20        # Brent bookkeeping per iteration
+FLOP_SCREEN = 30 + FLOP_OPTIM_DIST   # min image, bound test, f(psi)
+FLOP_ECP_FORCE = 650        # force/torque evaluation at the solved lambda
+FLOP_APPLY_ENTRY = 8        # per full-list entry of the per-particle apply
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(state_arrays, box, params, seconds):
+    """The reference's own loop (built from source, oracle/_ref) on the host;
+    falls back to the C restatement when the reference build is absent."""
+    import numpy as np
+    from oracle.oracle import ECPR, FIELDS, LGV, Oracle, Reference, State, reference_available
+    n = params["n"]
+    st = State(n, [box, box, box])
+    for k, f in enumerate(FIELDS):
+        st.buf[:, k] = state_arrays[f.upper()]
+    st["target_temp"] = params["T"]
+    st["dof"] = 6 * n - 3
+    st["density"] = n / box ** 3
+    dt, gamma, step0 = params["dt"], params["gamma"], params["next_step"]
+    if reference_available(fast=True):
+        R = Reference(fast=True)
+        R.load(st)
+        t0 = time.time()
+        R.nlist_init(params["skin"], params["cutoff"])
+        log(f"[cpu] reference init_nlist {time.time() - t0:.1f}s")
+        t0 = time.time()
+        R.run_steps(LGV, ECPR, [0.0], 1, step0, dt, gamma)
+        t1 = time.time() - t0
+        more = int(max(1, min(60, seconds / max(t1, 1e-3) - 1)))
+        t0 = time.time()
+        R.run_steps(LGV, ECPR, [0.0], more, step0 + 1, dt, gamma)
+        el = time.time() - t0
+        steps = more
+        kind = "reference"
+        sample = (f"{steps} steps of the reference loop (lgv + ecpr pair loop over its Verlet list, "
+                  f"reference rebuild rule) on the same {n}-particle state after one untimed step; "
+                  f"reference fortran built from source with flang -O3 -ffast-math -march=x86-64-v3")
+    else:
+        O = Oracle()
+        nl = O.nlist_new(st, params["cutoff"], params["skin"])
+        O.nlist_build(st, nl)
+        t0 = time.time()
+        steps = 0
+        while time.time() - t0 < seconds or steps == 0:
+            O.integrate(LGV, 0, st, dt, step=step0 + steps, damping=gamma)
+            if O.needs_rebuild(st, nl):
+                O.nlist_build(st, nl)
+            O.call("or_initialize_forces", st)
+            O.pair_loop(st, nl, ECPR, [0.0])
+            O.integrate(LGV, 1, st, dt, damping=gamma)
+            O.call("or_calculate_thermo", st)
+            steps += 1
+        el = time.time() - t0
+        kind = "port"
+        sample = f"{steps} steps of the C restatement of the reference loop on the same {n}-particle state"
+    return {"value": n * steps / el, "unit": "particle-steps/s", "cores": 1, "kind": kind, "sample": sample,
+            "seconds": el}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--n", type=int, default=32768)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (this build runs one "
+              f"process on one MI355X; multi-GPU sharding is a design note, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import sucrose_amd as sx
+    from sucrose_amd.workloads import ecp_langevin_state
+
+    dev = sx.System(0)
+    p = ecp_langevin_state(dev, n=args.n, log=log)
+    n, dt, gamma = p["n"], p["dt"], p["gamma"]
+    forces = [(sx.ECPR,)]
+    step = p["next_step"]
+    dev.run(sx.INT_LGV, forces, args.warmup, dt, damping=gamma, first_step=step)
+    step += args.warmup
+    dev.synchronize()
+    rb0 = dev.rebuild_count()
+    t0 = time.perf_counter()
+    dev.run(sx.INT_LGV, forces, args.steps, dt, damping=gamma, first_step=step)
+    dev.synchronize()
+    el = time.perf_counter() - t0
+    step += args.steps
+    rebuilds = dev.rebuild_count() - rb0
+    value = n * args.steps / el
+    temp = dev.get_scalar("TEMP")
+    pe = dev.get_scalar("POT_ENG")
+    log(f"[gpu] {args.steps} steps in {el:.3f}s -> {value:.4e} particle-steps/s; T={temp:.4f} PE/N={pe / n:.5f} "
+        f"rebuilds={rebuilds}")
+
+    # roofline of the dominant kernel, timed live with HIP events on the library stream
+    ms, st = dev.profile_ecp(sx.ECPR, (), nrep=20)
+    screened, solved, iters = st
+    nnz_full = 2 * screened
+    flops = {
+        "ecp_precompute": n * 150.0,
+        "ecp_screen": screened * FLOP_SCREEN,
+        "ecp_solve": solved * (30 + FLOP_OPTIM_DIST + FLOP_ECP_FORCE) + iters * (FLOP_OPTIM_DIST + FLOP_BRENT_ITER),
+        "ecp_apply": nnz_full * FLOP_APPLY_ENTRY,
+    }
+    names = list(flops)
+    k = max(range(4), key=lambda q: ms[q])
+    ach = flops[names[k]] / (ms[k] * 1e-3) / 1e12
+    roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": ach / PEAK_FP64_TFLOPS, "traffic": None, "kernel": names[k],
+            "peak_kind": "fp64 (vector ALU; MI355X fp64 dense figure)",
+            "stage_ms": {names[q]: ms[q] for q in range(4)},
+            "pairs_screened": screened, "pairs_solved": solved, "brent_iters_per_solved_pair": iters / max(solved, 1)}
+    log(f"[gpu] stages ms {roof['stage_ms']} screened={screened:.0f} solved={solved:.0f} "
+        f"iters/pair={iters / max(solved, 1):.2f}")
+
+    cpu = None
+    if not args.no_cpu:
+        dev.initialize_forces()
+        dev.forces(sx.ECPR)
+        arrays = dev.state()
+        p["next_step"] = step
+        cpu = cpu_baseline(arrays, p["box"], p, args.cpu_seconds)
+        cpu["cores"] = 1
+        log(f"[cpu] {cpu['kind']} {cpu['value']:.4e} particle-steps/s ({cpu['seconds']:.1f}s)")
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "ecp_langevin_32768", "n": n, "phi": p["phi"], "semi_axes": [0.5, 0.5, 1.5],
+                   "T": p["T"], "gamma": gamma, "dt": dt, "cutoff": p["cutoff"], "skin": p["skin"],
+                   "potential": "ecpr", "lambda_solver": "brent_reference", "rebuild_rule": "half_skin",
+                   "parallelism": "single_gpu"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(out), flush=True)
+    dev.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -200,6 +200,25 @@ typedef struct sx_step_spec {
 int sx_run(sx_ctx *ctx, const sx_step_spec *spec, int nsteps);
 
 /* ------------------------------------------------------------------ */
+/* measurement and synthetic set-up (no counterpart in the reference:   */
+/* benchmark and test harness support)                                 */
+/* ------------------------------------------------------------------ */
+/* HIP-event timing of the ECP pipeline on the current state, averaged over
+ * nrep runs: ms = {per-particle precompute, screen, solve, apply};
+ * stats = {half pairs screened, pairs solved, solver iterations} per run.
+ * Forces accumulate; re-initialise them afterwards. */
+int sx_profile_ecp(sx_ctx *ctx, int kind, const double *params, int nparams, int nrep, double ms[4],
+                   double stats[3]);
+/* random sequential addition of identical ellipsoids (semi-axes a, b, c) with
+ * uniform orientations in a periodic cube of edge L; pos[3n], quat[4n]
+ * (w x y z); returns the number placed */
+int sx_gen_ellipsoids_rsa(int n, double a, double b, double c, double L, uint32_t seed, long max_attempts,
+                          double *pos, double *quat);
+/* number of overlapping pairs (Perram-Wertheim contact function < 1) */
+long sx_count_ellipsoid_overlaps(int n, double a, double b, double c, double L, const double *pos,
+                                 const double *quat);
+
+/* ------------------------------------------------------------------ */
 /* Philox4x32-10 (xmath.f90:14-113)                                    */
 /* ------------------------------------------------------------------ */
 /* host: the reference's fixed key */

@@ -102,6 +102,14 @@ def lib():
             getattr(L, name).restype = C.c_int
         L.sx_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.sx_philox4x32_10.restype = None
+        L.sx_profile_ecp.argtypes = [vp, C.c_int, _PD, C.c_int, C.c_int, _PD, _PD]
+        L.sx_profile_ecp.restype = C.c_int
+        L.sx_gen_ellipsoids_rsa.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                            C.c_uint32, C.c_long, _PD, _PD]
+        L.sx_gen_ellipsoids_rsa.restype = C.c_int
+        L.sx_count_ellipsoid_overlaps.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                                  _PD, _PD]
+        L.sx_count_ellipsoid_overlaps.restype = C.c_long
         _lib = L
     return _lib
 
@@ -116,6 +124,23 @@ def philox4x32_10(ctr):
 
 def _dp(a):
     return a.ctypes.data_as(_PD)
+
+
+def gen_ellipsoids_rsa(n, semi_axes, L, seed, max_attempts=200_000_000):
+    """Random sequential addition of identical ellipsoids; returns (pos (k,3), quat (k,4))."""
+    pos = np.zeros(3 * n)
+    quat = np.zeros(4 * n)
+    a, b, c = semi_axes
+    k = lib().sx_gen_ellipsoids_rsa(int(n), a, b, c, float(L), int(seed) & 0xFFFFFFFF, int(max_attempts),
+                                    _dp(pos), _dp(quat))
+    return pos.reshape(n, 3)[:k], quat.reshape(n, 4)[:k]
+
+
+def count_ellipsoid_overlaps(pos, quat, semi_axes, L):
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+    quat = np.ascontiguousarray(quat, dtype=np.float64).reshape(-1)
+    a, b, c = semi_axes
+    return lib().sx_count_ellipsoid_overlaps(len(pos) // 3, a, b, c, float(L), _dp(pos), _dp(quat))
 
 
 class System:
@@ -266,6 +291,14 @@ class System:
         o = (C.c_long * 4)()
         self._ok(self.L.sx_ecp_stats(self.h, o))
         return list(o)
+
+    def profile_ecp(self, kind, params=(), nrep=10):
+        """HIP-event timings (ms) of the ECP pipeline stages and per-run counts."""
+        p = np.array(params if params else [0.0], dtype=np.float64)
+        ms = np.zeros(4)
+        stats = np.zeros(3)
+        self._ok(self.L.sx_profile_ecp(self.h, int(kind), _dp(p), len(params), int(nrep), _dp(ms), _dp(stats)))
+        return ms, stats
 
     def set_bonded(self, kind, idx1, coeff, angle=0.0):
         a = np.ascontiguousarray(idx1, dtype=np.int32).reshape(-1)

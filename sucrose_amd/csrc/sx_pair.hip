@@ -14,6 +14,7 @@
 //    buffer and a per-particle gather applies them.
 #include "sx_ecp.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -448,6 +449,79 @@ int sx_forces(sx_ctx *c, int kind, const double *params, int nparams)
     return guard(c, [&] {
         need_system(c);
         sx_forces_impl(c, kind, params, nparams);
+    });
+}
+
+// Times the ECP force pipeline on the context's stream with HIP events,
+// averaged over nrep repetitions on the current state (forces accumulate into
+// F/T, so callers re-initialise forces afterwards): ms[0] per-particle
+// precompute, [1] screen, [2] solve, [3] apply; stats[0] pairs screened,
+// [1] pairs solved per repetition, [2] solver iterations per repetition.
+int sx_profile_ecp(sx_ctx *c, int kind, const double *params, int nparams, int nrep, double ms[4], double stats[3])
+{
+    return guard(c, [&] {
+        need_system(c);
+        if (kind != SX_ECP && kind != SX_ECPR) throw sx_error{SX_ERR_ARG, "profile_ecp needs an ECP kind"};
+        check_params(kind, nparams);
+        Params P = make_params(c, kind, params, nparams);
+        ensure_pair_buffers(c);
+        NList &nl = c->nl;
+        DState s = c->dstate();
+        long ld = c->ld;
+        int n = c->n;
+        hipEvent_t ev[5];
+        for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+        double acc[4] = {0, 0, 0, 0};
+        long dev0[8], dev1[8];
+        HIPCHK(hipMemcpyAsync(dev0, c->d_counts, sizeof(long) * 8, hipMemcpyDeviceToHost, c->stream));
+        EcpScreen sc;
+        sc.variant = kind == SX_ECPR ? 1 : 0;
+        sc.cutoff = kind == SX_ECPR ? 0.0 : P.p[0];
+        long cap = nl.cap_half;
+        int nbh = sx_blocks(cap, 256);
+        int *nwork = c->d_flag + 4;
+        int lpp = 4;
+        int nb = sx_blocks((long)n * lpp, 256);
+        sx_ensure_part(c, nb, 1);
+        for (int r = 0; r < nrep; ++r) {
+            HIPCHK(hipEventRecord(ev[0], c->stream));
+            hipLaunchKernelGGL(k_ecp_pp, dim3(sx_blocks(n, 256)), dim3(256), 0, c->stream, s, c->d_ecp_pp, ld);
+            HIPCHK(hipEventRecord(ev[1], c->stream));
+            hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, c->stream, nwork);
+            hipLaunchKernelGGL(k_ecp_screen, dim3(nbh), dim3(256), 0, c->stream, s, c->d_ecp_pp, ld, nl.half_i,
+                               nl.half_j, c->d_flag + 2, sc, c->d_live, c->d_work, nwork, cap);
+            HIPCHK(hipEventRecord(ev[2], c->stream));
+            const float *lam0 = nullptr;
+            if (c->ecp_solver == SX_ECP_LEARNED) lam0 = sx_ecp_mlp_predict(c, c->d_ecp_pp, ld, c->d_work, nwork, cap);
+            if (sc.variant == 1)
+                hipLaunchKernelGGL(k_ecp_solve<1>, dim3(nbh), dim3(128), 0, c->stream, s, c->d_ecp_pp, ld, nl.half_i,
+                                   nl.half_j, c->d_work, nwork, cap, sc, c->ecp_solver, lam0, c->d_pairbuf,
+                                   c->d_counts, c->d_err);
+            else
+                hipLaunchKernelGGL(k_ecp_solve<0>, dim3(nbh), dim3(128), 0, c->stream, s, c->d_ecp_pp, ld, nl.half_i,
+                                   nl.half_j, c->d_work, nwork, cap, sc, c->ecp_solver, lam0, c->d_pairbuf,
+                                   c->d_counts, c->d_err);
+            HIPCHK(hipEventRecord(ev[3], c->stream));
+            hipLaunchKernelGGL(k_aniso_apply<4>, dim3(nb), dim3(256), 0, c->stream, s, nl.row_start, nl.nbr, nl.pairk,
+                               c->d_live, c->d_pairbuf, c->d_part);
+            HIPCHK(hipEventRecord(ev[4], c->stream));
+            HIPCHK(hipEventSynchronize(ev[4]));
+            for (int q = 0; q < 4; ++q) {
+                float t = 0;
+                HIPCHK(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
+                acc[q] += t;
+            }
+        }
+        HIPCHK(hipMemcpyAsync(dev1, c->d_counts, sizeof(long) * 8, hipMemcpyDeviceToHost, c->stream));
+        int sizes[8];
+        HIPCHK(hipMemcpyAsync(sizes, c->d_flag, sizeof(int) * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (auto &e : ev) hipEventDestroy(e);
+        for (int q = 0; q < 4; ++q) ms[q] = acc[q] / std::max(1, nrep);
+        stats[0] = sizes[2];
+        stats[1] = (double)(dev1[0] - dev0[0]) / std::max(1, nrep);
+        stats[2] = (double)(dev1[1] - dev0[1]) / std::max(1, nrep);
+        sx_check_device_error(c);
     });
 }
 
