@@ -1,6 +1,7 @@
 // Context, state, scalars, reductions, thermo and set-up routines of sucrose.h.
 #include "sx_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -20,14 +21,27 @@ DState sx_ctx::dstate() const
 // ---------------------------------------------------------------------------
 // reductions: per-block partials (fixed tree order) then one ordered pass
 // ---------------------------------------------------------------------------
-__global__ void k_finish_reduce(const double *part, int nblocks, int width, SxRedIds ids, double *scal)
+// one block of 256 threads: strided partial sums then a fixed tree, per quantity
+__global__ void __launch_bounds__(256) k_finish_reduce(const double *part, int nblocks, int width, SxRedIds ids,
+                                                       double *scal)
 {
-    int q = threadIdx.x;
-    if (q >= ids.n) return;
-    double s = 0.0;
-    for (int b = 0; b < nblocks; ++b) s += part[(long)b * width + q];
-    if (ids.accumulate) scal[ids.id[q]] = scal[ids.id[q]] + s;
-    else scal[ids.id[q]] = s;
+    __shared__ double sh[256];
+    int t = threadIdx.x;
+    for (int q = 0; q < ids.n; ++q) {
+        double s = 0.0;
+        for (int b = t; b < nblocks; b += 256) s += part[(long)b * width + q];
+        sh[t] = s;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (t < o) sh[t] = sh[t] + sh[t + o];
+            __syncthreads();
+        }
+        if (t == 0) {
+            if (ids.accumulate) scal[ids.id[q]] = scal[ids.id[q]] + sh[0];
+            else scal[ids.id[q]] = sh[0];
+        }
+        __syncthreads();
+    }
 }
 
 void sx_ensure_part(sx_ctx *c, int nblocks, int width)
@@ -46,7 +60,7 @@ void sx_reduce_to_scalar(sx_ctx *c, int nblocks, int width, const int *ids, int 
     r.n = nids;
     r.accumulate = accumulate;
     for (int q = 0; q < nids && q < 8; ++q) r.id[q] = ids[q];
-    hipLaunchKernelGGL(k_finish_reduce, dim3(1), dim3(64), 0, c->stream, c->d_part, nblocks, width, r, c->d_scal);
+    hipLaunchKernelGGL(k_finish_reduce, dim3(1), dim3(256), 0, c->stream, c->d_part, nblocks, width, r, c->d_scal);
 }
 
 void sx_sync_scalars_to_device(sx_ctx *c)
@@ -88,20 +102,26 @@ __global__ void k_fill(double *p, long n, double v)
     if (i < n) p[i] = v;
 }
 
-// initialize_forces (system.f90:423-436)
-__global__ void k_init_forces(DState s)
+// initialize_forces (system.f90:423-436): F, T and the accumulators
+__global__ void k_init_forces(DState s, double *scal, double *pe_acc)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        scal[SX_POT_ENG] = 0.0; scal[SX_KIN_ENG] = 0.0; scal[SX_KIN_ROT] = 0.0; scal[SX_TEMP] = 0.0;
+        scal[SX_PRESSURE] = 0.0; scal[SX_VIRIAL] = 0.0; scal[SX_VIRIAL_BONDED] = 0.0;
+    }
     if (i >= s.n) return;
     s.f[SX_FX][i] = 0.0; s.f[SX_FY][i] = 0.0; s.f[SX_FZ][i] = 0.0;
     s.f[SX_TX][i] = 0.0; s.f[SX_TY][i] = 0.0; s.f[SX_TZ][i] = 0.0;
+    pe_acc[i] = 0.0;
 }
 
-__global__ void k_zero_scalars(double *scal)
+__global__ void __launch_bounds__(256) k_pe_total(int n, const double *pe_acc, double *part, unsigned *ticket,
+                                                  double *scal)
 {
-    // pot_eng kin_eng kin_rot temp pressure virial virial_bonded
-    scal[SX_POT_ENG] = 0.0; scal[SX_KIN_ENG] = 0.0; scal[SX_KIN_ROT] = 0.0; scal[SX_TEMP] = 0.0;
-    scal[SX_PRESSURE] = 0.0; scal[SX_VIRIAL] = 0.0; scal[SX_VIRIAL_BONDED] = 0.0;
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[0] = v[0] + pe_acc[i];
+    sx_reduce_last_block<1>(v, part, ticket, scal, sx_ids1(SX_POT_ENG, 0));
 }
 
 // defaults of initialize_system (system.f90:586-682)
@@ -165,23 +185,18 @@ __device__ __forceinline__ void block_sum_store(double (&v)[W], double *part)
     }
 }
 
-// virial = SUM(RX*FX + RY*FY + RZ*FZ) (system.f90:733) partials
-__global__ void __launch_bounds__(256) k_virial_part(DState s, double *part)
+// calculate_thermo (system.f90:725-741): virial = SUM(RX*FX + RY*FY + RZ*FZ)
+// (+ virial_bonded), temp = 2 (KE + KErot) / dof, pressure; one pass
+__global__ void __launch_bounds__(256) k_thermo(DState s, double *part, unsigned *ticket, double *scal, int nc,
+                                                int nrigid, double volume)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
     if (i < s.n)
         v[0] = s.f[SX_RX][i] * s.f[SX_FX][i] + s.f[SX_RY][i] * s.f[SX_FY][i] + s.f[SX_RZ][i] * s.f[SX_FZ][i];
-    block_sum_store<1>(v, part);
-}
-
-// calculate_thermo (system.f90:725-741) given the reduced virial sum in part
-__global__ void k_thermo_finish(const double *part, int nblocks, double *scal, int nc, int nrigid,
-                                double volume)
-{
-    if (threadIdx.x != 0) return;
-    double w = 0.0;
-    for (int b = 0; b < nblocks; ++b) w += part[b];
+    double tot[1];
+    if (!sx_block_reduce_last<1>(v, part, ticket, tot) || threadIdx.x != 0) return;
+    double w = tot[0];
     double kin_tot = scal[SX_KIN_ENG] + scal[SX_KIN_ROT];
     scal[SX_TEMP] = 2.0 * kin_tot / scal[SX_DOF];
     if (nc > 0 && scal[SX_VIRIAL_BONDED] >= 0.0) w = w + scal[SX_VIRIAL_BONDED];
@@ -264,24 +279,33 @@ __global__ void k_philox_normals(int n, int c2, int c3, int c4, double *out)
 // ---------------------------------------------------------------------------
 void sx_init_forces_impl(sx_ctx *c)
 {
-    if (c->n > 0) hipLaunchKernelGGL(k_init_forces, dim3(sx_blocks(c->n, 256)), dim3(256), 0, c->stream, c->dstate());
-    hipLaunchKernelGGL(k_zero_scalars, dim3(1), dim3(1), 0, c->stream, c->d_scal);
+    hipLaunchKernelGGL(k_init_forces, dim3(sx_blocks(std::max(c->n, 1), 256)), dim3(256), 0, c->stream, c->dstate(),
+                       c->d_scal, c->d_pe_acc);
+}
+
+void sx_pe_total_impl(sx_ctx *c)
+{
+    int nb = std::min(64, sx_blocks(std::max(c->n, 1), 256));
+    sx_ensure_part(c, nb, 1);
+    hipLaunchKernelGGL(k_pe_total, dim3(nb), dim3(256), 0, c->stream, c->n, c->d_pe_acc, c->d_part, c->d_ticket,
+                       c->d_scal);
 }
 
 void sx_thermo_impl(sx_ctx *c)
 {
     int nb = sx_blocks(c->n, 256);
     sx_ensure_part(c, nb, 1);
-    hipLaunchKernelGGL(k_virial_part, dim3(nb), dim3(256), 0, c->stream, c->dstate(), c->d_part);
     double volume = c->box[0] * c->box[1] * c->box[2];
-    hipLaunchKernelGGL(k_thermo_finish, dim3(1), dim3(64), 0, c->stream, c->d_part, nb, c->d_scal, c->nc,
-                       c->nrigid, volume);
+    hipLaunchKernelGGL(k_thermo, dim3(nb), dim3(256), 0, c->stream, c->dstate(), c->d_part, c->d_ticket, c->d_scal,
+                       c->nc, c->nrigid, volume);
 }
 
 static void free_system(sx_ctx *c)
 {
     if (c->d_fields) hipFree(c->d_fields);
+    if (c->d_pe_acc) hipFree(c->d_pe_acc);
     c->d_fields = nullptr;
+    c->d_pe_acc = nullptr;
 }
 
 extern "C" {
@@ -303,6 +327,10 @@ int sx_create(int device, sx_ctx **out)
         HIPCHK(hipMalloc(&c->d_scal, sizeof(double) * SX_NSCALARS));
         HIPCHK(hipMalloc(&c->d_err, sizeof(int) * 4));
         HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(int) * 4, c->stream));
+        HIPCHK(hipMalloc(&c->d_ticket, sizeof(unsigned) * 16));
+        HIPCHK(hipMemsetAsync(c->d_ticket, 0, sizeof(unsigned) * 16, c->stream));
+        HIPCHK(hipMalloc(&c->d_segcount, sizeof(int) * 4096));
+        HIPCHK(hipMemsetAsync(c->d_segcount, 0, sizeof(int) * 4096, c->stream));
         HIPCHK(hipMalloc(&c->d_flag, sizeof(int) * 8));
         HIPCHK(hipMemsetAsync(c->d_flag, 0, sizeof(int) * 8, c->stream));
         HIPCHK(hipMalloc(&c->d_counts, sizeof(long) * 8));
@@ -326,11 +354,11 @@ int sx_destroy(sx_ctx *c)
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     free_system(c);
-    void *ptrs[] = {c->d_scal, c->d_part, c->d_err, c->d_flag, c->d_counts, c->d_bbi, c->d_bbj, c->d_bblen,
+    void *ptrs[] = {c->d_scal, c->d_part, c->d_err, c->d_ticket, c->d_segcount, c->d_flag, c->d_counts, c->d_bbi, c->d_bbj, c->d_bblen,
                     c->nl.cell_of, c->nl.cell_count, c->nl.cell_start, c->nl.cell_items, c->nl.row_len,
                     c->nl.row_start, c->nl.lower, c->nl.half_start, c->nl.nbr, c->nl.pairk, c->nl.half_i,
                     c->nl.half_j, c->nl.x_last, c->nl.y_last, c->nl.z_last, c->d_pairbuf, c->d_live, c->d_work,
-                    c->d_ecp_pp, c->d_atom_term_start, c->d_atom_term, c->d_bbufs, c->d_boff, c->d_step, c->mol.atom_start, c->mol.atoms,
+                    c->d_ecp_pp, c->d_ecp_scr, c->d_pos4, c->d_atom_term_start, c->d_atom_term, c->d_bbufs, c->d_boff, c->d_step, c->mol.atom_start, c->mol.atoms,
                     c->mol.bond_start, c->mol.bond_a, c->mol.bond_b, c->mol.bond_len, c->mol.bond_gidx};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -363,6 +391,8 @@ int sx_set_num_atoms(sx_ctx *c, int num_atoms, int rigid_atoms)
         c->nc = 0;
         c->ld = ((long)c->n + 63) / 64 * 64;
         HIPCHK(hipMalloc(&c->d_fields, sizeof(double) * SX_NFIELDS * c->ld));
+        HIPCHK(hipMalloc(&c->d_pe_acc, sizeof(double) * c->ld));
+        hipLaunchKernelGGL(k_fill, dim3(sx_blocks(c->ld, 256)), dim3(256), 0, c->stream, c->d_pe_acc, c->ld, 0.0);
         // sentinels of set_num_atoms (system.f90:74-85)
         const double init[SX_NFIELDS] = {
             0, 0, 0, 0, 0, 0, 0, 0, 0,   // r v f

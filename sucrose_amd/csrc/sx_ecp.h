@@ -21,11 +21,18 @@ struct Sym3 {
     double a11, a21, a31, a22, a32, a33;
 };
 
+// records are stored per particle (array of structures, PP_STRIDE doubles):
+// element q of particle i lives at pp[i * PP_STRIDE + q]; `ld` is unused
+#define PP_STRIDE 24
+#define PPX(pp, q, i) (pp)[(long)(i) * PP_STRIDE + (q)]
+
 __device__ __forceinline__ Sym3 load_sym(const double *pp, long ld, int base, int i)
 {
+    (void)ld;
+    const double *r = pp + (long)i * PP_STRIDE + base;
     Sym3 m;
-    m.a11 = pp[(base + 0) * ld + i]; m.a21 = pp[(base + 1) * ld + i]; m.a31 = pp[(base + 2) * ld + i];
-    m.a22 = pp[(base + 3) * ld + i]; m.a32 = pp[(base + 4) * ld + i]; m.a33 = pp[(base + 5) * ld + i];
+    m.a11 = r[0]; m.a21 = r[1]; m.a31 = r[2];
+    m.a22 = r[3]; m.a32 = r[4]; m.a33 = r[5];
     return m;
 }
 
@@ -220,7 +227,7 @@ __device__ __forceinline__ void ecp_forces(const DState &s, const double *pp, lo
     double detEM = Mx(EM, 1, 1) * (Mx(EM, 2, 2) * Mx(EM, 3, 3) - Mx(EM, 3, 2) * Mx(EM, 2, 3))
                  - Mx(EM, 1, 2) * (Mx(EM, 2, 1) * Mx(EM, 3, 3) - Mx(EM, 3, 1) * Mx(EM, 2, 3))
                  + Mx(EM, 1, 3) * (Mx(EM, 2, 1) * Mx(EM, 3, 2) - Mx(EM, 3, 1) * Mx(EM, 2, 2));
-    double sigma_t = sqrt(pp[PP_ST * ld + a] * pp[PP_ST * ld + b]);
+    double sigma_t = sqrt(PPX(pp, PP_ST, a) * PPX(pp, PP_ST, b));
     double eps1 = sigma_t * sqrt(detEM);
 
     double sr = sigma_0 / (rij_mag - sigma + sigma_0);
@@ -239,8 +246,8 @@ __device__ __forceinline__ void ecp_forces(const DState &s, const double *pp, lo
         int who = side == 0 ? a : b;
         double d[3][3];
         for (int c = 0; c < 3; ++c) {
-            double col[3] = {pp[(PP_MM + 3 * c + 0) * ld + who], pp[(PP_MM + 3 * c + 1) * ld + who],
-                             pp[(PP_MM + 3 * c + 2) * ld + who]};
+            double col[3] = {PPX(pp, PP_MM + 3 * c + 0, who), PPX(pp, PP_MM + 3 * c + 1, who),
+                             PPX(pp, PP_MM + 3 * c + 2, who)};
             double t[3];
             t[0] = Mx(EM, 1, 1) * col[0] + Mx(EM, 1, 2) * col[1] + Mx(EM, 1, 3) * col[2];
             t[1] = Mx(EM, 2, 1) * col[0] + Mx(EM, 2, 2) * col[1] + Mx(EM, 2, 3) * col[2];

@@ -101,15 +101,45 @@ __global__ void k_vv_initial(DState s, double dt, int save)
     wrap_pos(s, i);
 }
 
-__global__ void __launch_bounds__(256) k_vv_final(DState s, double dt, double *part)
+// calculate_thermo (system.f90:725-741) folded into the final-step kernels
+// when the step driver asks for it
+struct ThermoArgs {
+    int on, nc, nrigid;
+    double volume;
+    const double *pe_acc;   // non-NULL with `on`: pot_eng = sum of per-particle energies
+};
+
+__device__ __forceinline__ void thermo_finish(double *scal, double vir, ThermoArgs th)
+{
+    double w = vir;
+    double kin_tot = scal[SX_KIN_ENG] + scal[SX_KIN_ROT];
+    scal[SX_TEMP] = 2.0 * kin_tot / scal[SX_DOF];
+    if (th.nc > 0 && scal[SX_VIRIAL_BONDED] >= 0.0) w = w + scal[SX_VIRIAL_BONDED];
+    scal[SX_VIRIAL] = w;
+    scal[SX_PRESSURE] = (scal[SX_DENSITY] * th.nrigid * scal[SX_TEMP]) + (scal[SX_VIRIAL] / (3.0 * th.volume));
+}
+
+__device__ __forceinline__ double virial_of(const DState &s, int i)
+{
+    return s.f[SX_RX][i] * s.f[SX_FX][i] + s.f[SX_RY][i] * s.f[SX_FY][i] + s.f[SX_RZ][i] * s.f[SX_FZ][i];
+}
+
+__global__ void __launch_bounds__(256) k_vv_final(DState s, double dt, double *part, unsigned *ticket, double *scal,
+                                                  ThermoArgs th)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double v[1] = {0.0};
+    double v[3] = {0.0, 0.0, 0.0};
     if (i < s.n) {
         half_kick(s, i, dt);
         v[0] = kin_trans(s, i);
+        if (th.on) v[1] = virial_of(s, i);
+        if (th.pe_acc) v[2] = th.pe_acc[i];
     }
-    block_sum_store<1>(v, part);
+    double tot[3];
+    if (!sx_block_reduce_last<3>(v, part, ticket, tot) || threadIdx.x != 0) return;
+    scal[SX_KIN_ENG] = tot[0];
+    if (th.pe_acc) scal[SX_POT_ENG] = tot[2];
+    if (th.on) thermo_finish(scal, tot[1], th);
 }
 
 // ---------------------------------------------------------------------------
@@ -171,15 +201,22 @@ __global__ void k_qq_initial(DState s, double dt)
     quat_advance(s, i, dt, qdw, qdx, qdy, qdz);
 }
 
-__global__ void __launch_bounds__(256) k_qq_final(DState s, double dt, double *part)
+__global__ void __launch_bounds__(256) k_qq_final(DState s, double dt, double *part, unsigned *ticket, double *scal,
+                                                  ThermoArgs th)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double v[1] = {0.0};
+    double v[3] = {0.0, 0.0, 0.0};
     if (i < s.n) {
         ang_kick(s, i, dt);
         v[0] = kin_rot(s, i);
+        if (th.on) v[1] = virial_of(s, i);
+        if (th.pe_acc) v[2] = th.pe_acc[i];
     }
-    block_sum_store<1>(v, part);
+    double tot[3];
+    if (!sx_block_reduce_last<3>(v, part, ticket, tot) || threadIdx.x != 0) return;
+    scal[SX_KIN_ROT] = tot[0];
+    if (th.pe_acc) scal[SX_POT_ENG] = tot[2];
+    if (th.on) thermo_finish(scal, tot[1], th);
 }
 
 // ---------------------------------------------------------------------------
@@ -226,17 +263,26 @@ __global__ void k_lgv_initial(DState s, LgvConst k, int step, const int *dstep)
     quat_advance(s, i, dt, qdw, qdx, qdy, qdz);
 }
 
-__global__ void __launch_bounds__(256) k_lgv_final(DState s, double dt, double *part)
+__global__ void __launch_bounds__(256) k_lgv_final(DState s, double dt, double *part, unsigned *ticket, double *scal,
+                                                   ThermoArgs th, int *dstep)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double v[2] = {0.0, 0.0};
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
     if (i < s.n) {
         half_kick(s, i, dt);
         ang_kick(s, i, dt);
         v[1] = kin_rot(s, i);
         v[0] = kin_trans(s, i);
+        if (th.on) v[2] = virial_of(s, i);
+        if (th.pe_acc) v[3] = th.pe_acc[i];
     }
-    block_sum_store<2>(v, part);
+    double tot[4];
+    if (!sx_block_reduce_last<4>(v, part, ticket, tot) || threadIdx.x != 0) return;
+    scal[SX_KIN_ENG] = tot[0];
+    scal[SX_KIN_ROT] = tot[1];
+    if (th.pe_acc) scal[SX_POT_ENG] = tot[3];
+    if (th.on) thermo_finish(scal, tot[2], th);
+    if (dstep) *dstep = *dstep + 1;   // Langevin counter of graph replays
 }
 
 __global__ void k_step_inc(int *dstep) { *dstep = *dstep + 1; }
@@ -587,7 +633,7 @@ __global__ void k_bonded_terms(DState s, int kind, int count, const int *idx, do
 // concatenation of all registered term buffers
 __global__ void __launch_bounds__(256) k_bonded_gather(DState s, const int *atom_start, const int *atom_term,
                                                        const double *const *bufs, const long *buf_off, int nbufs,
-                                                       double *part)
+                                                       double *pe_acc)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
@@ -606,8 +652,8 @@ __global__ void __launch_bounds__(256) k_bonded_gather(DState s, const int *atom
         s.f[SX_FX][i] = s.f[SX_FX][i] + fx;
         s.f[SX_FY][i] = s.f[SX_FY][i] + fy;
         s.f[SX_FZ][i] = s.f[SX_FZ][i] + fz;
+        pe_acc[i] = pe_acc[i] + v[0];
     }
-    block_sum_store<1>(v, part);
 }
 
 // ---------------------------------------------------------------------------
@@ -624,12 +670,23 @@ void sx_vv_initial_impl(sx_ctx *c, double dt, bool save)
     hipLaunchKernelGGL(k_vv_initial, dim3(sx_blocks(c->n, 256)), dim3(256), 0, c->stream, c->dstate(), dt, save ? 1 : 0);
 }
 
-void sx_vv_final_impl(sx_ctx *c, double dt)
+static ThermoArgs thermo_args(sx_ctx *c, bool on)
+{
+    ThermoArgs th;
+    th.on = on ? 1 : 0;
+    th.nc = c->nc;
+    th.nrigid = c->nrigid;
+    th.volume = c->box[0] * c->box[1] * c->box[2];
+    th.pe_acc = (on && c->defer_pe) ? c->d_pe_acc : nullptr;
+    return th;
+}
+
+void sx_vv_final_impl(sx_ctx *c, double dt, bool thermo)
 {
     int nb = sx_blocks(c->n, 256);
-    sx_ensure_part(c, nb, 1);
-    hipLaunchKernelGGL(k_vv_final, dim3(nb), dim3(256), 0, c->stream, c->dstate(), dt, c->d_part);
-    reduce_into(c, nb, 1, {SX_KIN_ENG}, 0);
+    sx_ensure_part(c, nb, 3);
+    hipLaunchKernelGGL(k_vv_final, dim3(nb), dim3(256), 0, c->stream, c->dstate(), dt, c->d_part, c->d_ticket,
+                       c->d_scal, thermo_args(c, thermo));
 }
 
 void sx_qq_initial_impl(sx_ctx *c, double dt)
@@ -637,12 +694,12 @@ void sx_qq_initial_impl(sx_ctx *c, double dt)
     hipLaunchKernelGGL(k_qq_initial, dim3(sx_blocks(c->n, 256)), dim3(256), 0, c->stream, c->dstate(), dt);
 }
 
-void sx_qq_final_impl(sx_ctx *c, double dt)
+void sx_qq_final_impl(sx_ctx *c, double dt, bool thermo)
 {
     int nb = sx_blocks(c->n, 256);
-    sx_ensure_part(c, nb, 1);
-    hipLaunchKernelGGL(k_qq_final, dim3(nb), dim3(256), 0, c->stream, c->dstate(), dt, c->d_part);
-    reduce_into(c, nb, 1, {SX_KIN_ROT}, 0);
+    sx_ensure_part(c, nb, 3);
+    hipLaunchKernelGGL(k_qq_final, dim3(nb), dim3(256), 0, c->stream, c->dstate(), dt, c->d_part, c->d_ticket,
+                       c->d_scal, thermo_args(c, thermo));
 }
 
 static LgvConst lgv_const(sx_ctx *c, double dt, double damp, bool save)
@@ -667,14 +724,21 @@ void sx_lgv_initial_impl(sx_ctx *c, double dt, int step, const int *dstep, doubl
     if (constrained) sx_rattle_impl(c, dt, 0);
 }
 
-void sx_lgv_final_impl(sx_ctx *c, double dt, double damping, bool constrained)
+void sx_lgv_final_impl(sx_ctx *c, double dt, double damping, bool constrained, bool thermo, int *dstep)
 {
     (void)damping;
     int nb = sx_blocks(c->n, 256);
-    sx_ensure_part(c, nb, 2);
-    hipLaunchKernelGGL(k_lgv_final, dim3(nb), dim3(256), 0, c->stream, c->dstate(), dt, c->d_part);
-    reduce_into(c, nb, 2, {SX_KIN_ENG, SX_KIN_ROT}, 0);
-    if (constrained) sx_rattle_impl(c, dt, 1);
+    sx_ensure_part(c, nb, 4);
+    // with constraints the thermo must follow RATTLE (virial_bonded)
+    hipLaunchKernelGGL(k_lgv_final, dim3(nb), dim3(256), 0, c->stream, c->dstate(), dt, c->d_part, c->d_ticket,
+                       c->d_scal, thermo_args(c, thermo && !constrained), dstep);
+    if (constrained) {
+        sx_rattle_impl(c, dt, 1);
+        if (thermo) {
+            if (c->defer_pe) sx_pe_total_impl(c);
+            sx_thermo_impl(c);
+        }
+    }
 }
 
 void sx_nht_impl(sx_ctx *c, double dt, const double *coupling, int phase, bool save)
@@ -715,8 +779,8 @@ void sx_bonded_impl(sx_ctx *c)
     sx_ensure_part(c, nb, 1);
     hipLaunchKernelGGL(k_bonded_gather, dim3(nb), dim3(256), 0, c->stream, c->dstate(), c->d_atom_term_start,
                        c->d_atom_term, (const double *const *)c->d_bbufs, c->d_boff, (int)c->bonded.size(),
-                       c->d_part);
-    reduce_into(c, nb, 1, {SX_POT_ENG}, 1);
+                       c->d_pe_acc);
+    if (!c->defer_pe) sx_pe_total_impl(c);
 }
 
 static void rebuild_bonded_index(sx_ctx *c)

@@ -58,10 +58,11 @@ struct NList {
     int ncells = 0;
     // device arrays
     int *cell_of = nullptr, *cell_count = nullptr, *cell_start = nullptr, *cell_items = nullptr;
-    int *row_len = nullptr, *row_start = nullptr, *lower = nullptr, *half_start = nullptr;
+    int *row_len = nullptr, *row_start = nullptr, *lower = nullptr, *half_start = nullptr, *half_len = nullptr;
     int *nbr = nullptr, *pairk = nullptr;
     int *half_i = nullptr, *half_j = nullptr;
     double *x_last = nullptr, *y_last = nullptr, *z_last = nullptr;
+    double *cell_pos = nullptr;  // positions in cell order (x y z pad)
     long cap_full = 0, cap_half = 0;
     int max_row = 0;
     long rebuilds = 0;
@@ -97,12 +98,16 @@ struct sx_ctx {
     bool have_target_temp = false;
     double *d_fields = nullptr;  // SX_NFIELDS * ld doubles
     long ld = 0;
+    double *d_pe_acc = nullptr;  // per-particle potential energy since initialize_forces
+    bool defer_pe = false;       // step driver: pot_eng is reduced in the final-step kernel
+    int *d_segcount = nullptr;   // ECP work-list segment counts
     double *d_scal = nullptr;    // SX_NSCALARS doubles (device copy of the scalars)
     double h_scal[SX_NSCALARS] = {0};
     bool scal_dirty_host = false; // host copy newer than device
     double *d_part = nullptr;    // reduction partials
     int part_cap = 0;
     int *d_err = nullptr;
+    unsigned *d_ticket = nullptr; // last-block reduction counters (self re-arming)
     // constraints
     int *d_bbi = nullptr, *d_bbj = nullptr;
     double *d_bblen = nullptr;
@@ -124,7 +129,9 @@ struct sx_ctx {
     int *d_work = nullptr;        // compacted pair list
     long pair_cap = 0;
     // per-particle ECP precompute
-    double *d_ecp_pp = nullptr;
+    double *d_ecp_pp = nullptr;   // FP64 records, 24 doubles per particle
+    float *d_ecp_scr = nullptr;   // FP32 screen records, 8 floats per particle
+    double *d_pos4 = nullptr;     // positions packed x y z pad
     int ecp_pp_cap = 0;
     int ecp_solver = SX_ECP_BRENT_REFERENCE;
     long ecp_stats[4] = {0, 0, 0, 0};
@@ -180,15 +187,20 @@ void sx_nlist_flag_impl(sx_ctx *c);
 void sx_forces_impl(sx_ctx *c, int kind, const double *p, int np);
 void sx_bonded_impl(sx_ctx *c);
 void sx_lgv_initial_impl(sx_ctx *c, double dt, int step, const int *dstep, double damping, bool constrained);
-void sx_lgv_final_impl(sx_ctx *c, double dt, double damping, bool constrained);
+// thermo = also run calculate_thermo in the same kernel; dstep = device step
+// counter to advance (graph replays)
+void sx_lgv_final_impl(sx_ctx *c, double dt, double damping, bool constrained, bool thermo = false,
+                       int *dstep = nullptr);
 void sx_vv_initial_impl(sx_ctx *c, double dt, bool save);
-void sx_vv_final_impl(sx_ctx *c, double dt);
+void sx_vv_final_impl(sx_ctx *c, double dt, bool thermo = false);
 void sx_qq_initial_impl(sx_ctx *c, double dt);
-void sx_qq_final_impl(sx_ctx *c, double dt);
+void sx_qq_final_impl(sx_ctx *c, double dt, bool thermo = false);
 void sx_nht_impl(sx_ctx *c, double dt, const double *coupling, int phase, bool save);
 void sx_rattle_impl(sx_ctx *c, double dt, int phase);
 void sx_thermo_impl(sx_ctx *c);
 void sx_init_forces_impl(sx_ctx *c);
+// pot_eng = sum of the per-particle energies accumulated since initialize_forces
+void sx_pe_total_impl(sx_ctx *c);
 void sx_ecp_mlp_free(sx_ctx *c);
 
 static inline int sx_blocks(long n, int bs) { return (int)((n + bs - 1) / bs); }
@@ -197,6 +209,98 @@ static inline int sx_blocks(long n, int bs) { return (int)((n + bs - 1) / bs); }
 // device helpers (FP64, reference evaluation order)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double sx_mimg(double d, double L) { return d - round(d / L) * L; }
+
+// Same value as sx_mimg whenever |d| is not within a few ulp of an odd
+// multiple of L/2: round(d*invL) == round(d/L) there, and the subtraction
+// d - k*L is identical.  Callers use it only for pairs whose distance is
+// compared against a radius below L/2, where a half-box component excludes
+// the pair under either image.
+__device__ __forceinline__ double sx_mimg_fast(double d, double L, double invL) { return d - round(d * invL) * L; }
+
+// Deterministic one-pass block reduction of W doubles per thread.  Every
+// block writes its tree-reduced partial; the last block to finish (ticket
+// counter) sums the partials in block order and stores (or adds) the totals
+// into scal[ids[q]], then re-arms the counter for the next launch.
+// blockDim.x must be 256.
+// Returns true in the last block, where tot[] (valid in thread 0) holds the
+// grid totals.
+template <int W>
+__device__ __forceinline__ bool sx_block_reduce_last(double (&v)[W], double *part, unsigned *ticket, double (&tot)[W])
+{
+    __shared__ double sh[W][256];
+    __shared__ bool last;
+    int t = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < W; ++q) sh[q][t] = v[q];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) {
+#pragma unroll
+            for (int q = 0; q < W; ++q) sh[q][t] = sh[q][t] + sh[q][t + s];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+#pragma unroll
+        for (int q = 0; q < W; ++q) part[blockIdx.x * W + q] = sh[q][0];
+        __threadfence();
+        unsigned k = atomicAdd(ticket, 1u);
+        last = (k == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+    int nb = gridDim.x;
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+        double a = 0.0;
+        for (int b = t; b < nb; b += 256)
+            a = a + __hip_atomic_load(&part[b * W + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sh[q][t] = a;
+    }
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) {
+#pragma unroll
+            for (int q = 0; q < W; ++q) sh[q][t] = sh[q][t] + sh[q][t + s];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+#pragma unroll
+        for (int q = 0; q < W; ++q) tot[q] = sh[q][0];
+        *ticket = 0u;
+    }
+    return true;
+}
+
+template <int W>
+__device__ __forceinline__ void sx_reduce_last_block(double (&v)[W], double *part, unsigned *ticket, double *scal,
+                                                     SxRedIds ids)
+{
+    double tot[W];
+    if (!sx_block_reduce_last<W>(v, part, ticket, tot)) return;
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < W && q < ids.n; ++q) {
+            if (ids.accumulate) scal[ids.id[q]] = scal[ids.id[q]] + tot[q];
+            else scal[ids.id[q]] = tot[q];
+        }
+    }
+}
+
+__host__ __device__ static inline SxRedIds sx_ids1(int a, int accumulate)
+{
+    SxRedIds r;
+    r.id[0] = a; r.n = 1; r.accumulate = accumulate;
+    return r;
+}
+
+__host__ __device__ static inline SxRedIds sx_ids2(int a, int b, int accumulate)
+{
+    SxRedIds r;
+    r.id[0] = a; r.id[1] = b; r.n = 2; r.accumulate = accumulate;
+    return r;
+}
 
 // Philox4x32-10 with the reference's constants and fixed key (xmath.f90:5-76)
 __device__ __forceinline__ void sx_philox(uint32_t c[4])

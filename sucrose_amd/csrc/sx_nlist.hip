@@ -32,8 +32,31 @@ __device__ __forceinline__ bool in_list(const DState &s, int i, int j, double rs
 
 struct CellGrid {
     int nc[3];
-    double w[3];   // cell edge
+    double w[3];     // cell edge
+    double inv[3];   // 1 / box
+    int fast;        // (rc+skin) < box/2: reciprocal min-image gives the reference's value
 };
+
+// the same test from packed positions (pa of particle i, pb of particle j)
+__device__ __forceinline__ bool in_list_pos(double4 pi, double4 pj, int i, int j, const DState &s, const CellGrid &g,
+                                            double rsq_cut)
+{
+    double4 pa = i < j ? pi : pj, pb = i < j ? pj : pi;
+    double d0 = pa.x - pb.x, d1 = pa.y - pb.y, d2 = pa.z - pb.z;
+    if (s.periodic) {
+        if (g.fast) {
+            d0 = sx_mimg_fast(d0, s.box[0], g.inv[0]);
+            d1 = sx_mimg_fast(d1, s.box[1], g.inv[1]);
+            d2 = sx_mimg_fast(d2, s.box[2], g.inv[2]);
+        } else {
+            d0 = sx_mimg(d0, s.box[0]);
+            d1 = sx_mimg(d1, s.box[1]);
+            d2 = sx_mimg(d2, s.box[2]);
+        }
+    }
+    double rsq = d0 * d0 + d1 * d1 + d2 * d2;
+    return rsq < rsq_cut;
+}
 
 __device__ __forceinline__ int cell_coord(double x, double L, int nc, double w)
 {
@@ -67,52 +90,70 @@ __global__ void k_cell_assign(DState s, CellGrid g, int *cell_of, int *cell_coun
 __global__ void __launch_bounds__(1024) k_scan(const int *in, int *out, int n, const int *flag, int *total)
 {
     if (flag && *flag == 0) return;
-    __shared__ long sums[1024];
-    int t = threadIdx.x;
-    long chunk = ((long)n + 1023) / 1024;
-    long b = t * chunk, e = b + chunk < n ? b + chunk : n;
-    long s = 0;
-    for (long i = b; i < e; ++i) s += in[i];
-    sums[t] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        long v = (t >= off) ? sums[t - off] : 0;
+    // tiles of 4096: 4 consecutive elements per thread, wavefront shuffle
+    // scans, one LDS pass over the 16 wavefront totals
+    __shared__ int wsum[16];
+    int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    long carry = 0;
+    for (long base = 0; base < n; base += 4096) {
+        long i0 = base + 4L * t;
+        int v[4];
+        for (int k = 0; k < 4; ++k) v[k] = (i0 + k < n) ? in[i0 + k] : 0;
+        int local = v[0] + v[1] + v[2] + v[3];
+        int x = local;
+        for (int off = 1; off < 64; off <<= 1) {
+            int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
         __syncthreads();
-        sums[t] += v;
+        if (w == 0) {
+            int ws = lane < 16 ? wsum[lane] : 0;
+            for (int off = 1; off < 16; off <<= 1) {
+                int y = __shfl_up(ws, off, 64);
+                if (lane >= off) ws += y;
+            }
+            if (lane < 16) wsum[lane] = ws;
+        }
         __syncthreads();
+        long excl = carry + (w > 0 ? wsum[w - 1] : 0) + (x - local);
+        for (int k = 0; k < 4; ++k) {
+            if (i0 + k < n) out[i0 + k] = (int)excl;
+            excl += v[k];
+        }
+        long tile = wsum[15];
+        __syncthreads();
+        carry += tile;
     }
-    long run = sums[t] - s;
-    for (long i = b; i < e; ++i) {
-        out[i] = (int)run;
-        run += in[i];
-    }
-    if (t == 1023) {
-        out[n] = (int)sums[1023];
-        if (total) *total = (int)sums[1023];
+    if (t == 0) {
+        out[n] = (int)carry;
+        if (total) *total = (int)carry;
     }
 }
 
-__global__ void k_cell_scatter(int n, const int *cell_of, const int *cell_start, int *cell_fill, int *items,
-                               const int *flag)
-{
-    if (flag && *flag == 0) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int c = cell_of[i];
-    int p = atomicAdd(&cell_fill[c], 1);
-    items[cell_start[c] + p] = i;
-}
-
-// count (pass 0) or fill (pass 1) the full rows using the cell grid
-template <int PASS>
-__global__ void k_rows_cells(DState s, CellGrid g, const int *cell_of, const int *cell_start, const int *items,
-                             double rsq_cut, int *row_len, int *lower, const int *row_start, int *nbr,
-                             const int *flag, long cap, int *err)
+__global__ void k_cell_scatter(DState s, const int *cell_of, const int *cell_start, int *cell_fill, int *items,
+                               double4 *cell_pos, const int *flag)
 {
     if (flag && *flag == 0) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.n) return;
     int c = cell_of[i];
+    int p = cell_start[c] + atomicAdd(&cell_fill[c], 1);
+    items[p] = i;
+    cell_pos[p] = make_double4(s.f[SX_RX][i], s.f[SX_RY][i], s.f[SX_RZ][i], 0.0);
+}
+
+// count (pass 0) or fill (pass 1) the full rows using the cell grid
+template <int PASS>
+__global__ void k_rows_cells(DState s, CellGrid g, const int *cell_of, const int *cell_start, const int *items,
+                             const double4 *cell_pos, double rsq_cut, int *row_len, int *lower, int *half_len,
+                             const int *row_start, int *nbr, const int *flag, long cap, int *err)
+{
+    if (flag && *flag == 0) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    int c = cell_of[i];
+    double4 pi = make_double4(s.f[SX_RX][i], s.f[SX_RY][i], s.f[SX_RZ][i], 0.0);
     int cz = c % g.nc[2], cy = (c / g.nc[2]) % g.nc[1], cx = c / (g.nc[2] * g.nc[1]);
     int cnt = 0, low = 0;
     long base = PASS ? row_start[i] : 0;
@@ -130,7 +171,7 @@ __global__ void k_rows_cells(DState s, CellGrid g, const int *cell_of, const int
                 for (int q = cell_start[cc]; q < cell_start[cc + 1]; ++q) {
                     int j = items[q];
                     if (j == i) continue;
-                    if (in_list(s, i, j, rsq_cut)) {
+                    if (in_list_pos(pi, cell_pos[q], i, j, s, g, rsq_cut)) {
                         if (PASS) nbr[base + cnt] = j;
                         ++cnt;
                         if (j < i) ++low;
@@ -140,13 +181,14 @@ __global__ void k_rows_cells(DState s, CellGrid g, const int *cell_of, const int
     if (!PASS) {
         row_len[i] = cnt;
         lower[i] = low;
+        half_len[i] = cnt - low;
     }
 }
 
 // brute force rows (small or non-periodic boxes)
 template <int PASS>
-__global__ void k_rows_brute(DState s, double rsq_cut, int *row_len, int *lower, const int *row_start, int *nbr,
-                             const int *flag, long cap, int *err)
+__global__ void k_rows_brute(DState s, double rsq_cut, int *row_len, int *lower, int *half_len, const int *row_start,
+                             int *nbr, const int *flag, long cap, int *err)
 {
     if (flag && *flag == 0) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -168,6 +210,7 @@ __global__ void k_rows_brute(DState s, double rsq_cut, int *row_len, int *lower,
     if (!PASS) {
         row_len[i] = cnt;
         lower[i] = low;
+        half_len[i] = cnt - low;
     }
 }
 
@@ -177,37 +220,40 @@ __global__ void __launch_bounds__(256) k_sort_rows(int n, const int *row_start, 
     if (flag && *flag == 0) return;
     __shared__ int buf[4][SORT_CAP];
     int w = threadIdx.x / SX_WAVE, lane = threadIdx.x % SX_WAVE;
-    int i = blockIdx.x * 4 + w;
-    if (i >= n) return;
-    int b = row_start[i], len = row_start[i + 1] - b;
-    if (len <= 1) return;
-    if (len > SORT_CAP) {
-        if (lane == 0) atomicCAS(err, 0, DERR_ROW_CAP);
-        return;
-    }
-    int P = 1;
-    while (P < len) P <<= 1;
     int *sh = buf[w];
-    for (int q = lane; q < P; q += SX_WAVE) sh[q] = q < len ? nbr[b + q] : 0x7fffffff;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int k = 2; k <= P; k <<= 1) {
-        for (int jj = k >> 1; jj > 0; jj >>= 1) {
-            for (int q = lane; q < P; q += SX_WAVE) {
-                int p = q ^ jj;
-                if (p > q) {
-                    int a = sh[q], c = sh[p];
-                    bool up = (q & k) == 0;
-                    if ((a > c) == up) { sh[q] = c; sh[p] = a; }
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int i = blockIdx.x * 4 + w; i < n; i += gridDim.x * 4) {
+        int b = row_start[i], len = row_start[i + 1] - b;
+        if (len <= 1) continue;
+        if (len > SORT_CAP) {
+            if (lane == 0) atomicCAS(err, 0, DERR_ROW_CAP);
+            continue;
         }
+        int P = 1;
+        while (P < len) P <<= 1;
+        for (int q = lane; q < P; q += SX_WAVE) sh[q] = q < len ? nbr[b + q] : 0x7fffffff;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int k = 2; k <= P; k <<= 1) {
+            for (int jj = k >> 1; jj > 0; jj >>= 1) {
+                for (int q = lane; q < P; q += SX_WAVE) {
+                    int p = q ^ jj;
+                    if (p > q) {
+                        int a = sh[q], c = sh[p];
+                        bool up = (q & k) == 0;
+                        if ((a > c) == up) { sh[q] = c; sh[p] = a; }
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+        }
+        for (int q = lane; q < len; q += SX_WAVE) nbr[b + q] = sh[q];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    for (int q = lane; q < len; q += SX_WAVE) nbr[b + q] = sh[q];
 }
 
 __global__ void k_half_len(int n, const int *row_len, const int *lower, int *half_len, const int *flag)
@@ -248,19 +294,23 @@ __global__ void k_half_index(int n, const int *row_start, const int *lower, cons
     }
 }
 
-__global__ void k_save_last(DState s, double *xl, double *yl, double *zl, const int *flag)
+__global__ void k_save_last(DState s, double *xl, double *yl, double *zl, const int *flag, long *count)
 {
     if (flag && *flag == 0) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && count) count[7] += 1;   // rebuilds done inside graph replays
     if (i >= s.n) return;
     xl[i] = s.f[SX_RX][i]; yl[i] = s.f[SX_RY][i]; zl[i] = s.f[SX_RZ][i];
 }
 
 // needs_rebuild (system.f90:1005-1039): max over i of |mimg(r - r_last)|^2
+// needs_rebuild in one pass: per-block maxima, the last block to finish takes
+// the maximum over blocks and writes the rebuild flag (max is order-free)
 __global__ void __launch_bounds__(256) k_disp_max(DState s, const double *xl, const double *yl, const double *zl,
-                                                  double *part)
+                                                  double *part, unsigned *ticket, double thresh, int rule, int *flag)
 {
     __shared__ double sh[256];
+    __shared__ bool last;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double m = 0.0;
     if (i < s.n) {
@@ -276,16 +326,28 @@ __global__ void __launch_bounds__(256) k_disp_max(DState s, const double *xl, co
         if (threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
-}
-
-__global__ void k_disp_flag(const double *part, int nblocks, double thresh, int rule, int *flag, long *count)
-{
-    if (threadIdx.x != 0) return;
-    double m = 0.0;
-    for (int b = 0; b < nblocks; ++b) m = fmax(m, part[b]);
-    int r = rule == SX_REBUILD_REFERENCE ? (4.0 * m > thresh) : (m > thresh);
-    flag[0] = r;
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = sh[0];
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double v = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += 256)
+        v = fmax(v, __hip_atomic_load(&part[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double mx = sh[0];
+        flag[0] = rule == SX_REBUILD_REFERENCE ? (4.0 * mx > thresh) : (mx > thresh);
+        *ticket = 0u;
+    }
 }
 
 __global__ void k_count_rebuild(const int *flag, long *count)
@@ -299,15 +361,15 @@ __global__ void k_set_flag(int *flag, int v) { flag[0] = v; }
 static void nl_free(NList &nl)
 {
     int *ip[] = {nl.cell_of, nl.cell_count, nl.cell_start, nl.cell_items, nl.row_len, nl.row_start, nl.lower,
-                 nl.half_start, nl.nbr, nl.pairk, nl.half_i, nl.half_j};
+                 nl.half_start, nl.half_len, nl.nbr, nl.pairk, nl.half_i, nl.half_j};
     for (int *p : ip)
         if (p) hipFree(p);
-    double *dp[] = {nl.x_last, nl.y_last, nl.z_last};
+    double *dp[] = {nl.x_last, nl.y_last, nl.z_last, nl.cell_pos};
     for (double *p : dp)
         if (p) hipFree(p);
     nl.cell_of = nl.cell_count = nl.cell_start = nl.cell_items = nl.row_len = nl.row_start = nl.lower = nullptr;
-    nl.half_start = nl.nbr = nl.pairk = nl.half_i = nl.half_j = nullptr;
-    nl.x_last = nl.y_last = nl.z_last = nullptr;
+    nl.half_start = nl.half_len = nl.nbr = nl.pairk = nl.half_i = nl.half_j = nullptr;
+    nl.x_last = nl.y_last = nl.z_last = nl.cell_pos = nullptr;
 }
 
 static void nl_alloc_lists(sx_ctx *c, long cap_full)
@@ -352,10 +414,12 @@ static void nl_setup(sx_ctx *c)
     HIPCHK(hipMalloc(&nl.cell_count, sizeof(int) * (nl.ncells + 1)));
     HIPCHK(hipMalloc(&nl.cell_start, sizeof(int) * (nl.ncells + 1)));
     HIPCHK(hipMalloc(&nl.cell_items, sizeof(int) * n));
+    HIPCHK(hipMalloc(&nl.cell_pos, sizeof(double) * 4 * (size_t)n));
     HIPCHK(hipMalloc(&nl.row_len, sizeof(int) * (n + 1)));
     HIPCHK(hipMalloc(&nl.row_start, sizeof(int) * (n + 1)));
     HIPCHK(hipMalloc(&nl.lower, sizeof(int) * (n + 1)));
     HIPCHK(hipMalloc(&nl.half_start, sizeof(int) * (n + 1)));
+    HIPCHK(hipMalloc(&nl.half_len, sizeof(int) * (n + 1)));
     HIPCHK(hipMalloc(&nl.x_last, sizeof(double) * n));
     HIPCHK(hipMalloc(&nl.y_last, sizeof(double) * n));
     HIPCHK(hipMalloc(&nl.z_last, sizeof(double) * n));
@@ -381,7 +445,13 @@ void sx_nlist_build_impl(sx_ctx *c, bool conditional)
     for (int attempt = 0; attempt < 4; ++attempt) {
         if (nl.use_cells) {
             CellGrid g;
-            for (int d = 0; d < 3; ++d) { g.nc[d] = nl.ncell[d]; g.w[d] = c->box[d] / nl.ncell[d]; }
+            g.fast = 1;
+            for (int d = 0; d < 3; ++d) {
+                g.nc[d] = nl.ncell[d];
+                g.w[d] = c->box[d] / nl.ncell[d];
+                g.inv[d] = 1.0 / c->box[d];
+                if (!(nl.cutoff + nl.skin < 0.5 * c->box[d] * (1.0 - 1e-12))) g.fast = 0;
+            }
             // cell binning is cheap; it runs unconditionally
             int ncb = sx_blocks(nl.ncells + 1, 256);
             hipLaunchKernelGGL(k_zero_int, dim3(ncb), dim3(256), 0, c->stream, nl.cell_count, nl.ncells + 1, flag);
@@ -390,23 +460,23 @@ void sx_nlist_build_impl(sx_ctx *c, bool conditional)
             hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.cell_count, nl.cell_start, nl.ncells,
                                flag, (int *)nullptr);
             hipLaunchKernelGGL(k_zero_int, dim3(ncb), dim3(256), 0, c->stream, nl.cell_count, nl.ncells + 1, flag);
-            hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(256), 0, c->stream, n, nl.cell_of, nl.cell_start,
-                               nl.cell_count, nl.cell_items, flag);
+            hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(256), 0, c->stream, s, nl.cell_of, nl.cell_start,
+                               nl.cell_count, nl.cell_items, (double4 *)nl.cell_pos, flag);
             hipLaunchKernelGGL(k_rows_cells<0>, dim3(nb), dim3(256), 0, c->stream, s, g, nl.cell_of, nl.cell_start,
-                               nl.cell_items, nl.r_skin_sq, nl.row_len, nl.lower, nl.row_start, nl.nbr, flag,
-                               nl.cap_full, c->d_err);
+                               nl.cell_items, (const double4 *)nl.cell_pos, nl.r_skin_sq, nl.row_len, nl.lower,
+                               nl.half_len, nl.row_start, nl.nbr, flag, nl.cap_full, c->d_err);
             hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.row_len, nl.row_start, n, flag,
                                c->d_flag + 1);
             hipLaunchKernelGGL(k_rows_cells<1>, dim3(nb), dim3(256), 0, c->stream, s, g, nl.cell_of, nl.cell_start,
-                               nl.cell_items, nl.r_skin_sq, nl.row_len, nl.lower, nl.row_start, nl.nbr, flag,
-                               nl.cap_full, c->d_err);
+                               nl.cell_items, (const double4 *)nl.cell_pos, nl.r_skin_sq, nl.row_len, nl.lower,
+                               nl.half_len, nl.row_start, nl.nbr, flag, nl.cap_full, c->d_err);
         } else {
             hipLaunchKernelGGL(k_rows_brute<0>, dim3(nb), dim3(256), 0, c->stream, s, nl.r_skin_sq, nl.row_len,
-                               nl.lower, nl.row_start, nl.nbr, flag, nl.cap_full, c->d_err);
+                               nl.lower, nl.half_len, nl.row_start, nl.nbr, flag, nl.cap_full, c->d_err);
             hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.row_len, nl.row_start, n, flag,
                                c->d_flag + 1);
             hipLaunchKernelGGL(k_rows_brute<1>, dim3(nb), dim3(256), 0, c->stream, s, nl.r_skin_sq, nl.row_len,
-                               nl.lower, nl.row_start, nl.nbr, flag, nl.cap_full, c->d_err);
+                               nl.lower, nl.half_len, nl.row_start, nl.nbr, flag, nl.cap_full, c->d_err);
         }
         if (!conditional) {
             int total = 0;
@@ -420,16 +490,13 @@ void sx_nlist_build_impl(sx_ctx *c, bool conditional)
         }
         break;
     }
-    hipLaunchKernelGGL(k_sort_rows, dim3(sx_blocks(n, 4)), dim3(256), 0, c->stream, n, nl.row_start, nl.nbr, flag,
-                       c->d_err);
-    hipLaunchKernelGGL(k_half_len, dim3(nb), dim3(256), 0, c->stream, n, nl.row_len, nl.lower, nl.cell_items, flag);
-    // cell_items doubles as the half-length scratch after the fill pass
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.cell_items, nl.half_start, n, flag,
-                       c->d_flag + 2);
+    hipLaunchKernelGGL(k_sort_rows, dim3(std::min(sx_blocks(n, 4), 2048)), dim3(256), 0, c->stream, n, nl.row_start,
+                       nl.nbr, flag, c->d_err);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.half_len, nl.half_start, n, flag, c->d_flag + 2);
     hipLaunchKernelGGL(k_half_index, dim3(nb), dim3(256), 0, c->stream, n, nl.row_start, nl.lower, nl.half_start,
                        nl.nbr, nl.pairk, nl.half_i, nl.half_j, flag, nl.cap_half, c->d_err);
-    hipLaunchKernelGGL(k_save_last, dim3(nb), dim3(256), 0, c->stream, s, nl.x_last, nl.y_last, nl.z_last, flag);
-    if (conditional) hipLaunchKernelGGL(k_count_rebuild, dim3(1), dim3(64), 0, c->stream, flag, c->d_counts);
+    hipLaunchKernelGGL(k_save_last, dim3(nb), dim3(256), 0, c->stream, s, nl.x_last, nl.y_last, nl.z_last, flag,
+                       conditional ? c->d_counts : (long *)nullptr);
     if (!conditional) {
         int sizes[2];
         HIPCHK(hipMemcpyAsync(sizes, c->d_flag + 1, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));
@@ -446,11 +513,9 @@ void sx_nlist_flag_impl(sx_ctx *c)
     NList &nl = c->nl;
     int nb = sx_blocks(c->n, 256);
     sx_ensure_part(c, nb, 1);
-    hipLaunchKernelGGL(k_disp_max, dim3(nb), dim3(256), 0, c->stream, c->dstate(), nl.x_last, nl.y_last, nl.z_last,
-                       c->d_part);
     double thresh = nl.rule == SX_REBUILD_REFERENCE ? nl.r_skin_sq : (nl.skin / 2.0) * (nl.skin / 2.0);
-    hipLaunchKernelGGL(k_disp_flag, dim3(1), dim3(64), 0, c->stream, c->d_part, nb, thresh, nl.rule, c->d_flag,
-                       c->d_counts);
+    hipLaunchKernelGGL(k_disp_max, dim3(nb), dim3(256), 0, c->stream, c->dstate(), nl.x_last, nl.y_last, nl.z_last,
+                       c->d_part, c->d_ticket, thresh, nl.rule, c->d_flag);
 }
 
 static void need_nlist(sx_ctx *c)

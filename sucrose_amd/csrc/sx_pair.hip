@@ -6,12 +6,13 @@
 //    particle's sorted full row; each lane evaluates the canonical pair
 //    (min, max) exactly as the reference would and keeps the half it needs
 //    (+FXIJ / TXI when i is the reference's I, -FXIJ / TXJ when it is J);
-//    lanes combine with cross-lane shuffles in a fixed order.  No atomics:
-//    results are deterministic run to run.
-//  * ECP: a screen over the half list rejects pairs that provably give zero
-//    (F(psi) > 1 => the Brent minimum lies beyond contact), survivors are
-//    compacted and solved one lane per pair, results land in a per-pair
-//    buffer and a per-particle gather applies them.
+//    lanes combine with cross-lane shuffles in a fixed order.  No atomics on
+//    forces: results are deterministic run to run.
+//  * ECP: a conservative FP32 screen over the half list rejects pairs that
+//    provably give zero (F(psi) > contact threshold => the Brent minimum lies
+//    beyond contact), survivors are compacted and solved one lane per pair in
+//    FP64 along the reference's Brent path, results land in a per-pair buffer
+//    and a per-particle gather applies them.
 #include "sx_ecp.h"
 
 #include <algorithm>
@@ -27,7 +28,7 @@ struct Params {
 // ---------------------------------------------------------------------------
 template <int KIND, int LPP, bool TORQUE>
 __global__ void __launch_bounds__(256) k_gather(DState s, const int *row_start, const int *nbr, Params P,
-                                                double *part)
+                                                double *pe_acc)
 {
     int gid = blockIdx.x * blockDim.x + threadIdx.x;
     int i = gid / LPP, l = gid % LPP;
@@ -53,8 +54,6 @@ __global__ void __launch_bounds__(256) k_gather(DState s, const int *row_start, 
     for (int off = LPP / 2; off > 0; off >>= 1)
 #pragma unroll
         for (int c = 0; c < 7; ++c) acc[c] = acc[c] + __shfl_xor(acc[c], off, LPP);
-    __shared__ double pe_sh[256];
-    pe_sh[threadIdx.x] = (l == 0 && i < s.n) ? acc[6] : 0.0;
     if (l == 0 && i < s.n) {
         s.f[SX_FX][i] = s.f[SX_FX][i] + acc[0];
         s.f[SX_FY][i] = s.f[SX_FY][i] + acc[1];
@@ -64,22 +63,20 @@ __global__ void __launch_bounds__(256) k_gather(DState s, const int *row_start, 
             s.f[SX_TY][i] = s.f[SX_TY][i] + acc[4];
             s.f[SX_TZ][i] = s.f[SX_TZ][i] + acc[5];
         }
+        pe_acc[i] = pe_acc[i] + acc[6];
     }
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) pe_sh[threadIdx.x] = pe_sh[threadIdx.x] + pe_sh[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = pe_sh[0];
 }
 
 // ---------------------------------------------------------------------------
 // ECP
 // ---------------------------------------------------------------------------
-// per-particle precompute (force.f90:1427-1483 for one particle)
-__global__ void k_ecp_pp(DState s, double *pp, long ld)
+// per-particle records (force.f90:1427-1483 for one particle): FP64 record,
+// FP32 screen record (AR lower triangle, largest semi-axis, smallest) and the
+// packed position; thread 0 also clears the work counter of this call
+__global__ void k_ecp_pp(DState s, double *pp, float *scr, double *pos4, int *nwork)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && nwork) *nwork = 0;
     if (i >= s.n) return;
     double w = s.f[SX_QW][i], x = s.f[SX_QX][i], y = s.f[SX_QY][i], z = s.f[SX_QZ][i];
     double U[9];
@@ -100,77 +97,144 @@ __global__ void k_ecp_pp(DState s, double *pp, long ld)
     double B[3] = {S[0] * S[0], S[1] * S[1], S[2] * S[2]};
     // E = ((e0/eps)**(1/2))/4  (force.f90:1452-1454)
     double E[3] = {sqrt(e0 / ex) / 4.0, sqrt(e0 / ey) / 4.0, sqrt(e0 / ez) / 4.0};
-    // matmul_atba (xmath.f90:262-279) lower triangle
+    double *rec = pp + (long)i * PP_STRIDE;
+    // matmul_atba (xmath.f90:262-279): the reference computes out(r, c) for
+    // r <= c; the record keeps (c, r) = (r, c) in lower-triangle order
     const int ij[6][2] = {{1, 1}, {2, 1}, {3, 1}, {2, 2}, {3, 2}, {3, 3}};
+    double ar_l[6];
     for (int t = 0; t < 6; ++t) {
-        // out(r,c) with r <= c written as out(1,2) etc.: use (c_, r_) ordering of the reference
-        int r_ = ij[t][1], c_ = ij[t][0];  // reference computes out(r_, c_) with r_ <= c_
+        int r_ = ij[t][1], c_ = ij[t][0];
         double ar = Ux(1, r_) * B[0] * Ux(1, c_) + Ux(2, r_) * B[1] * Ux(2, c_) + Ux(3, r_) * B[2] * Ux(3, c_);
         double ae = Ux(1, r_) * E[0] * Ux(1, c_) + Ux(2, r_) * E[1] * Ux(2, c_) + Ux(3, r_) * E[2] * Ux(3, c_);
-        pp[(PP_AR + t) * ld + i] = ar;
-        pp[(PP_AE + t) * ld + i] = ae;
+        rec[PP_AR + t] = ar;
+        rec[PP_AE + t] = ae;
+        ar_l[t] = ar;
     }
     // MM = U^T . S: column c is row c of U scaled by S(c)
     for (int c = 0; c < 3; ++c)
-        for (int k = 0; k < 3; ++k) pp[(PP_MM + 3 * c + k) * ld + i] = Ux(c + 1, k + 1) * S[c];
-    pp[PP_ST * ld + i] = (sx * sy + sz * sz) * sqrt(2.0 * sx * sy) / 8.0;
-    pp[PP_AMAX * ld + i] = fmax(fmax(S[0], S[1]), S[2]);
+        for (int k = 0; k < 3; ++k) rec[PP_MM + 3 * c + k] = Ux(c + 1, k + 1) * S[c];
+    rec[PP_ST] = (sx * sy + sz * sz) * sqrt(2.0 * sx * sy) / 8.0;
+    double amax = fmax(fmax(S[0], S[1]), S[2]), amin = fmin(fmin(S[0], S[1]), S[2]);
+    rec[PP_AMAX] = amax;
 #undef Ux
+    float4 *s4 = reinterpret_cast<float4 *>(scr) + 2L * i;
+    s4[0] = make_float4((float)ar_l[0], (float)ar_l[1], (float)ar_l[2], (float)ar_l[3]);
+    s4[1] = make_float4((float)ar_l[4], (float)ar_l[5], (float)amax, (float)amin);
+    double4 *p4 = reinterpret_cast<double4 *>(pos4) + i;
+    *p4 = make_double4(s.f[SX_RX][i], s.f[SX_RY][i], s.f[SX_RZ][i], 0.0);
 }
 
 struct EcpScreen {
     int variant;       // 0 ecp (cutoff), 1 ecpr
     double cutoff;
+    double L[3], invL[3];
+    int periodic;
 };
 
-// Screen of one half pair.  Returns true if the pair can contribute.  The
-// Brent minimum satisfies f(lambda) <= f(psi), i.e. F(lambda) >= F(psi)
-// (force.f90:1132-1165 only ever accepts non-increasing f), and sigma =
-// r / sqrt(F(lambda)) up to rounding, so F(psi) above the contact threshold
-// proves a zero result.  Bound 1: F(psi) >= psi(1-psi) r^2 / lambda_max(G(psi)).
-__device__ __forceinline__ bool ecp_screen(const DState &s, const double *pp, long ld, int a, int b, EcpScreen sc,
-                                           double RIJ[3], double &fpsi)
+#define ECP_SCREEN_MARGIN 1e-2f
+
+// Conservative screen of one half pair.  Brent only ever accepts a point
+// with f(u) <= f(x) (force.f90:1132-1165), so its final lambda has F(lambda)
+// >= F(psi); sigma = r / sqrt(F(lambda)) up to rounding, so F(psi) above the
+// contact threshold proves a zero result.  F(psi) is evaluated in FP32 and a
+// pair is kept unless F32(psi) exceeds the threshold by the 1 % margin, far
+// above the FP32 error of a 3x3 solve whose condition number is bounded by
+// (largest/smallest semi-axis)^2 (checked on the host).  Before the solve,
+// F(psi) >= psi(1-psi) r^2 / lambda_max(G(psi)) with lambda_max(G) <=
+// (1-psi) amax_a^2 + psi amax_b^2 rejects distant pairs without a solve.
+__device__ __forceinline__ bool ecp_screen32(const double4 *pos4, const float *scr, int a, int b,
+                                             const EcpScreen &sc)
 {
-    pair_rij(s, a, b, RIJ);
-    double rsq = RIJ[0] * RIJ[0] + RIJ[1] * RIJ[1] + RIJ[2] * RIJ[2];
-    // contact threshold on F: ecpr r <= sigma <=> F <= 1; ecp r < sigma + rc - 1
-    double thr;
+    double4 pa = pos4[a], pb = pos4[b];
+    double d0 = pa.x - pb.x, d1 = pa.y - pb.y, d2 = pa.z - pb.z;
+    if (sc.periodic) {
+        d0 = sx_mimg_fast(d0, sc.L[0], sc.invL[0]);
+        d1 = sx_mimg_fast(d1, sc.L[1], sc.invL[1]);
+        d2 = sx_mimg_fast(d2, sc.L[2], sc.invL[2]);
+    }
+    float r0 = (float)d0, r1 = (float)d1, r2 = (float)d2;
+    float rsq = r0 * r0 + r1 * r1 + r2 * r2;
+    float thr;
     if (sc.variant == 1) {
-        thr = 1.0;
+        thr = 1.0f;
     } else {
-        double rmag = sqrt(rsq);
-        double den = rmag - sc.cutoff + 1.0;
-        if (den <= 0.0) { fpsi = 0.0; return true; }
+        float rmag = sqrtf(rsq);
+        float den = rmag - (float)sc.cutoff + 1.0f;
+        if (den <= 0.05f) return true;
         thr = (rmag / den) * (rmag / den);
     }
-    thr = thr * (1.0 + 1e-9);
-    const double psi = ECP_PSI;
-    double am = pp[PP_AMAX * ld + a], bm = pp[PP_AMAX * ld + b];
-    double lmax = (1.0 - psi) * am * am + psi * bm * bm;
-    if (psi * (1.0 - psi) * rsq > thr * lmax * (1.0 + 1e-9)) return false;
-    Sym3 A = load_sym(pp, ld, PP_AR, a), B = load_sym(pp, ld, PP_AR, b);
-    fpsi = ecp_optim_dist(psi, A, B, RIJ);
-    return -fpsi <= thr;
+    thr = thr * (1.0f + ECP_SCREEN_MARGIN);
+    const float psi = 0.381966011250105f;
+    const float4 *s4 = reinterpret_cast<const float4 *>(scr);
+    float4 A0 = s4[2L * a], A1 = s4[2L * a + 1], B0 = s4[2L * b], B1 = s4[2L * b + 1];
+    // aspect ratios beyond 30 are not screened (condition number of G)
+    if (A1.z > 30.0f * A1.w || B1.z > 30.0f * B1.w) return true;
+    float lmax = (1.0f - psi) * A1.z * A1.z + psi * B1.z * B1.z;
+    if (psi * (1.0f - psi) * rsq > thr * lmax * (1.0f + ECP_SCREEN_MARGIN)) return false;
+    float q = 1.0f - psi;
+    float g11 = q * A0.x + psi * B0.x, g21 = q * A0.y + psi * B0.y, g31 = q * A0.z + psi * B0.z;
+    float g22 = q * A0.w + psi * B0.w, g32 = q * A1.x + psi * B1.x, g33 = q * A1.y + psi * B1.y;
+    float i11 = rsqrtf(g11);
+    float l21 = g21 * i11, l31 = g31 * i11;
+    float i22 = rsqrtf(g22 - l21 * l21);
+    float l32 = (g32 - l31 * l21) * i22;
+    float i33 = rsqrtf(g33 - l31 * l31 - l32 * l32);
+    float y1 = r0 * i11;
+    float y2 = (r1 - l21 * y1) * i22;
+    float y3 = (r2 - l31 * y1 - l32 * y2) * i33;
+    // r^T G^-1 r = |L^-1 r|^2
+    float F = psi * (1.0f - psi) * (y1 * y1 + y2 * y2 + y3 * y3);
+    return F <= thr;
 }
 
-__global__ void k_ecp_screen(DState s, const double *pp, long ld, const int *half_i, const int *half_j,
-                             const int *nnz_half, EcpScreen sc, unsigned char *live, int *work, int *nwork, long cap)
+// The half list is cut into ECP_NSEG contiguous segments (one per block);
+// each block compacts the survivors of its segment in order into the same
+// segment of the work array and records their count.  No atomics: the work
+// list is deterministic and the solve walks the segments.
+#define ECP_NSEG 1024
+
+__device__ __forceinline__ void ecp_segment(long nnz, int b, long &start, long &end)
 {
-    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= *nnz_half) return;
-    int a = half_i[k], b = half_j[k];
-    double RIJ[3], fpsi;
-    bool keep = ecp_screen(s, pp, ld, a, b, sc, RIJ, fpsi);
-    live[k] = keep ? 1 : 0;
-    if (keep) {
-        int slot = atomicAdd(nwork, 1);
-        if (slot < cap) work[slot] = (int)k;
+    long chunk = (nnz + ECP_NSEG - 1) / ECP_NSEG;
+    start = (long)b * chunk;
+    end = start + chunk < nnz ? start + chunk : nnz;
+    if (start > nnz) start = nnz;
+}
+
+__global__ void __launch_bounds__(256) k_ecp_screen(const double4 *pos4, const float *scr, const int *half_i,
+                                                    const int *half_j, const int *nnz_half, EcpScreen sc,
+                                                    unsigned char *live, int *work, int *segcount)
+{
+    __shared__ int wcnt[4];
+    long start, end;
+    ecp_segment(*nnz_half, blockIdx.x, start, end);
+    int t = threadIdx.x, lane = t & (SX_WAVE - 1), w = t / SX_WAVE;
+    int run = 0;
+    for (long base = start; base < end; base += 256) {
+        long k = base + t;
+        bool keep = false;
+        if (k < end) {
+            keep = ecp_screen32(pos4, scr, half_i[k], half_j[k], sc);
+            live[k] = keep ? 1 : 0;
+        }
+        unsigned long long m = __ballot(keep);
+        if (lane == 0) wcnt[w] = __popcll(m);
+        __syncthreads();
+        int before = 0, total = 0;
+        for (int q = 0; q < 4; ++q) {
+            if (q < w) before += wcnt[q];
+            total += wcnt[q];
+        }
+        if (keep) work[start + run + before + __popcll(m & ((1ull << lane) - 1ull))] = (int)k;
+        run += total;
+        __syncthreads();
     }
+    if (t == 0) segcount[blockIdx.x] = run;
 }
 
 template <int VARIANT>
-__device__ __forceinline__ double ecp_solve_lambda(int solver, const DState &s, const double *pp, long ld, int a,
-                                                   int b, const double RIJ[3], double fpsi, double lam0, int &iters,
+__device__ __forceinline__ double ecp_solve_lambda(int solver, const double *pp, long ld, int a, int b,
+                                                   const double RIJ[3], double fpsi, double lam0, int &iters,
                                                    int &fallback)
 {
     Sym3 A = load_sym(pp, ld, PP_AR, a), B = load_sym(pp, ld, PP_AR, b);
@@ -187,41 +251,61 @@ __device__ __forceinline__ double ecp_solve_lambda(int solver, const DState &s, 
     return l;
 }
 
+#define ECP_SOLVE_THREADS 64
+
+// one block per segment of the work list; stats (may be NULL) accumulates
+// solved pairs, solver iterations and fallbacks
 template <int VARIANT>
-__global__ void k_ecp_solve(DState s, const double *pp, long ld, const int *half_i, const int *half_j,
-                            const int *work, const int *nwork, long cap, EcpScreen sc, int solver, const float *lam0,
-                            double *res, long *stats, int *err)
+__global__ void __launch_bounds__(ECP_SOLVE_THREADS) k_ecp_solve(DState s, const double *pp, long ld, const int *half_i,
+                                                                 const int *half_j, const int *work, const int *segcount,
+                                                                 const int *nnz_half, EcpScreen sc, int solver,
+                                                                 const float *lam0, double *res, long *stats)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long nw = *nwork;
-    if (t == 0 && nw > cap) atomicCAS(err, 0, DERR_ECP_CAP);
-    if (t >= nw || t >= cap) return;
-    int k = work[t];
-    int a = half_i[k], b = half_j[k];
-    double RIJ[3];
-    pair_rij(s, a, b, RIJ);
-    Sym3 A = load_sym(pp, ld, PP_AR, a), B = load_sym(pp, ld, PP_AR, b);
-    double fpsi = ecp_optim_dist(ECP_PSI, A, B, RIJ);
-    int iters = 0, fb = 0;
-    double l0 = lam0 ? (double)lam0[t] : 0.5;
-    double lam = ecp_solve_lambda<VARIANT>(solver, s, pp, ld, a, b, RIJ, fpsi, l0, iters, fb);
-    PairOut o;
-    ecp_forces<VARIANT>(s, pp, ld, a, b, sc.cutoff, RIJ, lam, o);
-    double *r = res + (long)k * 10;
-    r[0] = o.pe;
-    r[1] = o.f[0]; r[2] = o.f[1]; r[3] = o.f[2];
-    r[4] = o.ti[0]; r[5] = o.ti[1]; r[6] = o.ti[2];
-    r[7] = o.tj[0]; r[8] = o.tj[1]; r[9] = o.tj[2];
-    // stats: [0] solved pairs, [1] iterations, [2] fallbacks
-    atomicAdd((unsigned long long *)&stats[0], 1ull);
-    atomicAdd((unsigned long long *)&stats[1], (unsigned long long)iters);
-    if (fb) atomicAdd((unsigned long long *)&stats[2], 1ull);
+    long start, end;
+    ecp_segment(*nnz_half, blockIdx.x, start, end);
+    long cnt = segcount[blockIdx.x];
+    long my_iters = 0, my_solved = 0, my_fb = 0;
+    for (long u = threadIdx.x; u < cnt; u += blockDim.x) {
+        long t = start + u;
+        int k = work[t];
+        int a = half_i[k], b = half_j[k];
+        double RIJ[3];
+        pair_rij(s, a, b, RIJ);
+        Sym3 A = load_sym(pp, ld, PP_AR, a), B = load_sym(pp, ld, PP_AR, b);
+        double fpsi = ecp_optim_dist(ECP_PSI, A, B, RIJ);
+        int iters = 0, fb = 0;
+        double l0 = lam0 ? (double)lam0[t] : 0.5;
+        double lam = ecp_solve_lambda<VARIANT>(solver, pp, ld, a, b, RIJ, fpsi, l0, iters, fb);
+        PairOut o;
+        ecp_forces<VARIANT>(s, pp, ld, a, b, sc.cutoff, RIJ, lam, o);
+        double *r = res + (long)k * 10;
+        r[0] = o.pe;
+        r[1] = o.f[0]; r[2] = o.f[1]; r[3] = o.f[2];
+        r[4] = o.ti[0]; r[5] = o.ti[1]; r[6] = o.ti[2];
+        r[7] = o.tj[0]; r[8] = o.tj[1]; r[9] = o.tj[2];
+        my_iters += iters;
+        my_solved += 1;
+        my_fb += fb;
+    }
+    (void)end;
+    if (!stats) return;
+    // stats: [0] solved pairs, [1] iterations, [2] fallbacks (wave-reduced)
+    for (int off = SX_WAVE / 2; off > 0; off >>= 1) {
+        my_iters += __shfl_xor(my_iters, off, SX_WAVE);
+        my_solved += __shfl_xor(my_solved, off, SX_WAVE);
+        my_fb += __shfl_xor(my_fb, off, SX_WAVE);
+    }
+    if ((threadIdx.x & (SX_WAVE - 1)) == 0 && my_solved) {
+        atomicAdd((unsigned long long *)&stats[0], (unsigned long long)my_solved);
+        atomicAdd((unsigned long long *)&stats[1], (unsigned long long)my_iters);
+        if (my_fb) atomicAdd((unsigned long long *)&stats[2], (unsigned long long)my_fb);
+    }
 }
 
 // per-particle application of the per-pair results
 template <int LPP>
 __global__ void __launch_bounds__(256) k_aniso_apply(DState s, const int *row_start, const int *nbr, const int *pairk,
-                                                     const unsigned char *live, const double *res, double *part)
+                                                     const unsigned char *live, const double *res, double *pe_acc)
 {
     int gid = blockIdx.x * blockDim.x + threadIdx.x;
     int i = gid / LPP, l = gid % LPP;
@@ -246,8 +330,6 @@ __global__ void __launch_bounds__(256) k_aniso_apply(DState s, const int *row_st
     for (int off = LPP / 2; off > 0; off >>= 1)
 #pragma unroll
         for (int c = 0; c < 7; ++c) acc[c] = acc[c] + __shfl_xor(acc[c], off, LPP);
-    __shared__ double pe_sh[256];
-    pe_sh[threadIdx.x] = (l == 0 && i < s.n) ? acc[6] : 0.0;
     if (l == 0 && i < s.n) {
         s.f[SX_FX][i] = s.f[SX_FX][i] + acc[0];
         s.f[SX_FY][i] = s.f[SX_FY][i] + acc[1];
@@ -255,16 +337,9 @@ __global__ void __launch_bounds__(256) k_aniso_apply(DState s, const int *row_st
         s.f[SX_TX][i] = s.f[SX_TX][i] + acc[3];
         s.f[SX_TY][i] = s.f[SX_TY][i] + acc[4];
         s.f[SX_TZ][i] = s.f[SX_TZ][i] + acc[5];
+        pe_acc[i] = pe_acc[i] + acc[6];
     }
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) pe_sh[threadIdx.x] = pe_sh[threadIdx.x] + pe_sh[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = pe_sh[0];
 }
-
-__global__ void k_zero_counter(int *p) { *p = 0; }
 
 // explicit pairs (parity hook)
 template <int KIND>
@@ -283,10 +358,10 @@ __global__ void k_pair_eval(DState s, long np, const int *pi, const int *pj, Par
         double fpsi = ecp_optim_dist(ECP_PSI, A, B, RIJ);
         int it = 0, fb = 0;
         if (KIND == SX_ECP) {
-            lam = ecp_solve_lambda<0>(solver, s, pp, ld, a, b, RIJ, fpsi, 0.5, it, fb);
+            lam = ecp_solve_lambda<0>(solver, pp, ld, a, b, RIJ, fpsi, 0.5, it, fb);
             ecp_forces<0>(s, pp, ld, a, b, P.p[0], RIJ, lam, o);
         } else {
-            lam = ecp_solve_lambda<1>(solver, s, pp, ld, a, b, RIJ, fpsi, 0.5, it, fb);
+            lam = ecp_solve_lambda<1>(solver, pp, ld, a, b, RIJ, fpsi, 0.5, it, fb);
             ecp_forces<1>(s, pp, ld, a, b, 0.0, RIJ, lam, o);
         }
     } else {
@@ -331,7 +406,7 @@ static void check_params(int kind, int np)
 }
 
 template <int KIND, bool TORQUE>
-static void launch_gather(sx_ctx *c, const Params &P, int lpp, int nb_out[1])
+static void launch_gather(sx_ctx *c, const Params &P, int lpp)
 {
     int n = c->n;
     long threads = (long)n * lpp;
@@ -339,12 +414,11 @@ static void launch_gather(sx_ctx *c, const Params &P, int lpp, int nb_out[1])
     sx_ensure_part(c, nb, 1);
     DState s = c->dstate();
     switch (lpp) {
-    case 1: hipLaunchKernelGGL((k_gather<KIND, 1, TORQUE>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_part); break;
-    case 4: hipLaunchKernelGGL((k_gather<KIND, 4, TORQUE>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_part); break;
-    case 8: hipLaunchKernelGGL((k_gather<KIND, 8, TORQUE>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_part); break;
-    default: hipLaunchKernelGGL((k_gather<KIND, 16, TORQUE>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_part); break;
+    case 1: hipLaunchKernelGGL((k_gather<KIND, 1, TORQUE>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_pe_acc); break;
+    case 4: hipLaunchKernelGGL((k_gather<KIND, 4, TORQUE>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_pe_acc); break;
+    case 8: hipLaunchKernelGGL((k_gather<KIND, 8, TORQUE>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_pe_acc); break;
+    default: hipLaunchKernelGGL((k_gather<KIND, 16, TORQUE>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_pe_acc); break;
     }
-    nb_out[0] = nb;
 }
 
 static int choose_lpp(sx_ctx *c, int kind)
@@ -360,6 +434,20 @@ static int choose_lpp(sx_ctx *c, int kind)
     return lpp;
 }
 
+static void ensure_pp(sx_ctx *c)
+{
+    if (c->ecp_pp_cap < c->n) {
+        if (c->d_ecp_pp) HIPCHK(hipFree(c->d_ecp_pp));
+        if (c->d_ecp_scr) HIPCHK(hipFree(c->d_ecp_scr));
+        if (c->d_pos4) HIPCHK(hipFree(c->d_pos4));
+        HIPCHK(hipMalloc(&c->d_ecp_pp, sizeof(double) * PP_STRIDE * (size_t)c->n));
+        HIPCHK(hipMalloc(&c->d_ecp_scr, sizeof(float) * 8 * (size_t)c->n));
+        HIPCHK(hipMalloc(&c->d_pos4, sizeof(double) * 4 * (size_t)c->n));
+        c->ecp_pp_cap = c->n;
+        c->have_graph = false;
+    }
+}
+
 static void ensure_pair_buffers(sx_ctx *c)
 {
     long need = c->nl.cap_half;
@@ -373,48 +461,85 @@ static void ensure_pair_buffers(sx_ctx *c)
         c->pair_cap = need;
         c->have_graph = false;
     }
-    if (c->ecp_pp_cap < c->n) {
-        if (c->d_ecp_pp) HIPCHK(hipFree(c->d_ecp_pp));
-        HIPCHK(hipMalloc(&c->d_ecp_pp, sizeof(double) * PP_N * c->ld));
-        c->ecp_pp_cap = c->n;
-    }
+    ensure_pp(c);
 }
 
 // learned warm start: predictions for the work list (implemented with the MLP)
-const float *sx_ecp_mlp_predict(sx_ctx *c, const double *pp, long ld, const int *work, const int *nwork, long cap);
+const float *sx_ecp_mlp_predict(sx_ctx *c, const double *pp, long ld, const int *work, const int *segcount, long cap);
+
+static EcpScreen make_screen(sx_ctx *c, int kind, const Params &P)
+{
+    EcpScreen sc;
+    sc.variant = kind == SX_ECPR ? 1 : 0;
+    sc.cutoff = kind == SX_ECPR ? 0.0 : P.p[0];
+    sc.periodic = c->periodic;
+    for (int d = 0; d < 3; ++d) {
+        sc.L[d] = c->box[d];
+        sc.invL[d] = 1.0 / c->box[d];
+    }
+    return sc;
+}
+
+struct EcpLaunch {
+    long cap;
+    int lpp, nb;
+    bool stats;   // count solved pairs / iterations (profiling)
+};
+
+static EcpLaunch ecp_launch_shape(sx_ctx *c, bool stats)
+{
+    EcpLaunch L;
+    L.cap = c->nl.cap_half;
+    L.lpp = 4;
+    L.nb = sx_blocks((long)c->n * L.lpp, 256);
+    L.stats = stats;
+    return L;
+}
+
+static void ecp_stage(sx_ctx *c, int stage, int kind, const EcpScreen &sc, const EcpLaunch &L)
+{
+    NList &nl = c->nl;
+    DState s = c->dstate();
+    int n = c->n;
+    long ld = 0;
+    const int *nnz_half = c->d_flag + 2;
+    switch (stage) {
+    case 0:
+        hipLaunchKernelGGL(k_ecp_pp, dim3(sx_blocks(n, 256)), dim3(256), 0, c->stream, s, c->d_ecp_pp, c->d_ecp_scr,
+                           c->d_pos4, (int *)nullptr);
+        break;
+    case 1:
+        hipLaunchKernelGGL(k_ecp_screen, dim3(ECP_NSEG), dim3(256), 0, c->stream, (const double4 *)c->d_pos4,
+                           c->d_ecp_scr, nl.half_i, nl.half_j, nnz_half, sc, c->d_live, c->d_work, c->d_segcount);
+        break;
+    case 2: {
+        const float *lam0 = nullptr;
+        if (c->ecp_solver == SX_ECP_LEARNED)
+            lam0 = sx_ecp_mlp_predict(c, c->d_ecp_pp, ld, c->d_work, c->d_segcount, L.cap);
+        long *stats = L.stats ? c->d_counts : nullptr;
+        if (kind == SX_ECPR)
+            hipLaunchKernelGGL(k_ecp_solve<1>, dim3(ECP_NSEG), dim3(ECP_SOLVE_THREADS), 0, c->stream, s, c->d_ecp_pp,
+                               ld, nl.half_i, nl.half_j, c->d_work, c->d_segcount, nnz_half, sc, c->ecp_solver, lam0,
+                               c->d_pairbuf, stats);
+        else
+            hipLaunchKernelGGL(k_ecp_solve<0>, dim3(ECP_NSEG), dim3(ECP_SOLVE_THREADS), 0, c->stream, s, c->d_ecp_pp,
+                               ld, nl.half_i, nl.half_j, c->d_work, c->d_segcount, nnz_half, sc, c->ecp_solver, lam0,
+                               c->d_pairbuf, stats);
+        break;
+    }
+    case 3:
+        hipLaunchKernelGGL(k_aniso_apply<4>, dim3(L.nb), dim3(256), 0, c->stream, s, nl.row_start, nl.nbr, nl.pairk,
+                           c->d_live, c->d_pairbuf, c->d_pe_acc);
+        break;
+    }
+}
 
 static void ecp_forces_impl(sx_ctx *c, int kind, const Params &P)
 {
     ensure_pair_buffers(c);
-    NList &nl = c->nl;
-    DState s = c->dstate();
-    int n = c->n;
-    long ld = c->ld;
-    hipLaunchKernelGGL(k_ecp_pp, dim3(sx_blocks(n, 256)), dim3(256), 0, c->stream, s, c->d_ecp_pp, ld);
-    int *nwork = c->d_flag + 4;
-    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, c->stream, nwork);
-    EcpScreen sc;
-    sc.variant = kind == SX_ECPR ? 1 : 0;
-    sc.cutoff = kind == SX_ECPR ? 0.0 : P.p[0];
-    long cap = nl.cap_half;
-    int nbh = sx_blocks(cap, 256);
-    hipLaunchKernelGGL(k_ecp_screen, dim3(nbh), dim3(256), 0, c->stream, s, c->d_ecp_pp, ld, nl.half_i, nl.half_j,
-                       c->d_flag + 2, sc, c->d_live, c->d_work, nwork, cap);
-    const float *lam0 = nullptr;
-    if (c->ecp_solver == SX_ECP_LEARNED) lam0 = sx_ecp_mlp_predict(c, c->d_ecp_pp, ld, c->d_work, nwork, cap);
-    if (sc.variant == 1)
-        hipLaunchKernelGGL(k_ecp_solve<1>, dim3(nbh), dim3(128), 0, c->stream, s, c->d_ecp_pp, ld, nl.half_i, nl.half_j,
-                           c->d_work, nwork, cap, sc, c->ecp_solver, lam0, c->d_pairbuf, c->d_counts, c->d_err);
-    else
-        hipLaunchKernelGGL(k_ecp_solve<0>, dim3(nbh), dim3(128), 0, c->stream, s, c->d_ecp_pp, ld, nl.half_i, nl.half_j,
-                           c->d_work, nwork, cap, sc, c->ecp_solver, lam0, c->d_pairbuf, c->d_counts, c->d_err);
-    int lpp = 4;
-    int nb = sx_blocks((long)n * lpp, 256);
-    sx_ensure_part(c, nb, 1);
-    hipLaunchKernelGGL(k_aniso_apply<4>, dim3(nb), dim3(256), 0, c->stream, s, nl.row_start, nl.nbr, nl.pairk,
-                       c->d_live, c->d_pairbuf, c->d_part);
-    int id = SX_POT_ENG;
-    sx_reduce_to_scalar(c, nb, 1, &id, 1, 1);
+    EcpScreen sc = make_screen(c, kind, P);
+    EcpLaunch L = ecp_launch_shape(c, false);
+    for (int stage = 0; stage < 4; ++stage) ecp_stage(c, stage, kind, sc, L);
 }
 
 void sx_forces_impl(sx_ctx *c, int kind, const double *p, int np)
@@ -424,22 +549,21 @@ void sx_forces_impl(sx_ctx *c, int kind, const double *p, int np)
     Params P = make_params(c, kind, p, np);
     if (kind == SX_ECP || kind == SX_ECPR) {
         ecp_forces_impl(c, kind, P);
-        return;
+    } else {
+        int lpp = choose_lpp(c, kind);
+        switch (kind) {
+        case SX_LJ: launch_gather<SX_LJ, false>(c, P, lpp); break;
+        case SX_LJ12: launch_gather<SX_LJ12, false>(c, P, lpp); break;
+        case SX_LJW: launch_gather<SX_LJW, false>(c, P, lpp); break;
+        case SX_WCA: launch_gather<SX_WCA, false>(c, P, lpp); break;
+        case SX_MORSE: launch_gather<SX_MORSE, false>(c, P, lpp); break;
+        case SX_DH: launch_gather<SX_DH, false>(c, P, lpp); break;
+        case SX_GB: launch_gather<SX_GB, true>(c, P, lpp); break;
+        case SX_GBR: launch_gather<SX_GBR, true>(c, P, lpp); break;
+        }
     }
-    int lpp = choose_lpp(c, kind);
-    int nb[1];
-    switch (kind) {
-    case SX_LJ: launch_gather<SX_LJ, false>(c, P, lpp, nb); break;
-    case SX_LJ12: launch_gather<SX_LJ12, false>(c, P, lpp, nb); break;
-    case SX_LJW: launch_gather<SX_LJW, false>(c, P, lpp, nb); break;
-    case SX_WCA: launch_gather<SX_WCA, false>(c, P, lpp, nb); break;
-    case SX_MORSE: launch_gather<SX_MORSE, false>(c, P, lpp, nb); break;
-    case SX_DH: launch_gather<SX_DH, false>(c, P, lpp, nb); break;
-    case SX_GB: launch_gather<SX_GB, true>(c, P, lpp, nb); break;
-    case SX_GBR: launch_gather<SX_GBR, true>(c, P, lpp, nb); break;
-    }
-    int id = SX_POT_ENG;
-    sx_reduce_to_scalar(c, nb[0], 1, &id, 1, 1);
+    // the step driver folds this reduction into the final-step kernel
+    if (!c->defer_pe) sx_pe_total_impl(c);
 }
 
 extern "C" {
@@ -452,58 +576,27 @@ int sx_forces(sx_ctx *c, int kind, const double *params, int nparams)
     });
 }
 
-// Times the ECP force pipeline on the context's stream with HIP events,
-// averaged over nrep repetitions on the current state (forces accumulate into
-// F/T, so callers re-initialise forces afterwards): ms[0] per-particle
-// precompute, [1] screen, [2] solve, [3] apply; stats[0] pairs screened,
-// [1] pairs solved per repetition, [2] solver iterations per repetition.
 int sx_profile_ecp(sx_ctx *c, int kind, const double *params, int nparams, int nrep, double ms[4], double stats[3])
 {
     return guard(c, [&] {
         need_system(c);
         if (kind != SX_ECP && kind != SX_ECPR) throw sx_error{SX_ERR_ARG, "profile_ecp needs an ECP kind"};
+        if (!c->nl.init) throw sx_error{SX_ERR_STATE, "neighbour list not initialised (sx_nlist_init)"};
         check_params(kind, nparams);
         Params P = make_params(c, kind, params, nparams);
         ensure_pair_buffers(c);
-        NList &nl = c->nl;
-        DState s = c->dstate();
-        long ld = c->ld;
-        int n = c->n;
+        EcpScreen sc = make_screen(c, kind, P);
+        EcpLaunch L = ecp_launch_shape(c, true);
         hipEvent_t ev[5];
         for (auto &e : ev) HIPCHK(hipEventCreate(&e));
         double acc[4] = {0, 0, 0, 0};
         long dev0[8], dev1[8];
         HIPCHK(hipMemcpyAsync(dev0, c->d_counts, sizeof(long) * 8, hipMemcpyDeviceToHost, c->stream));
-        EcpScreen sc;
-        sc.variant = kind == SX_ECPR ? 1 : 0;
-        sc.cutoff = kind == SX_ECPR ? 0.0 : P.p[0];
-        long cap = nl.cap_half;
-        int nbh = sx_blocks(cap, 256);
-        int *nwork = c->d_flag + 4;
-        int lpp = 4;
-        int nb = sx_blocks((long)n * lpp, 256);
-        sx_ensure_part(c, nb, 1);
         for (int r = 0; r < nrep; ++r) {
-            HIPCHK(hipEventRecord(ev[0], c->stream));
-            hipLaunchKernelGGL(k_ecp_pp, dim3(sx_blocks(n, 256)), dim3(256), 0, c->stream, s, c->d_ecp_pp, ld);
-            HIPCHK(hipEventRecord(ev[1], c->stream));
-            hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, c->stream, nwork);
-            hipLaunchKernelGGL(k_ecp_screen, dim3(nbh), dim3(256), 0, c->stream, s, c->d_ecp_pp, ld, nl.half_i,
-                               nl.half_j, c->d_flag + 2, sc, c->d_live, c->d_work, nwork, cap);
-            HIPCHK(hipEventRecord(ev[2], c->stream));
-            const float *lam0 = nullptr;
-            if (c->ecp_solver == SX_ECP_LEARNED) lam0 = sx_ecp_mlp_predict(c, c->d_ecp_pp, ld, c->d_work, nwork, cap);
-            if (sc.variant == 1)
-                hipLaunchKernelGGL(k_ecp_solve<1>, dim3(nbh), dim3(128), 0, c->stream, s, c->d_ecp_pp, ld, nl.half_i,
-                                   nl.half_j, c->d_work, nwork, cap, sc, c->ecp_solver, lam0, c->d_pairbuf,
-                                   c->d_counts, c->d_err);
-            else
-                hipLaunchKernelGGL(k_ecp_solve<0>, dim3(nbh), dim3(128), 0, c->stream, s, c->d_ecp_pp, ld, nl.half_i,
-                                   nl.half_j, c->d_work, nwork, cap, sc, c->ecp_solver, lam0, c->d_pairbuf,
-                                   c->d_counts, c->d_err);
-            HIPCHK(hipEventRecord(ev[3], c->stream));
-            hipLaunchKernelGGL(k_aniso_apply<4>, dim3(nb), dim3(256), 0, c->stream, s, nl.row_start, nl.nbr, nl.pairk,
-                               c->d_live, c->d_pairbuf, c->d_part);
+            for (int stage = 0; stage < 4; ++stage) {
+                HIPCHK(hipEventRecord(ev[stage], c->stream));
+                ecp_stage(c, stage, kind, sc, L);
+            }
             HIPCHK(hipEventRecord(ev[4], c->stream));
             HIPCHK(hipEventSynchronize(ev[4]));
             for (int q = 0; q < 4; ++q) {
@@ -572,26 +665,24 @@ int sx_pair_eval(sx_ctx *c, int kind, const double *params, int nparams, long np
         HIPCHK(hipMemcpyAsync(dj, hj.data(), sizeof(int) * np, hipMemcpyHostToDevice, c->stream));
         DState s = c->dstate();
         if (kind == SX_ECP || kind == SX_ECPR) {
-            if (c->ecp_pp_cap < c->n) {
-                if (c->d_ecp_pp) HIPCHK(hipFree(c->d_ecp_pp));
-                HIPCHK(hipMalloc(&c->d_ecp_pp, sizeof(double) * PP_N * c->ld));
-                c->ecp_pp_cap = c->n;
-            }
-            hipLaunchKernelGGL(k_ecp_pp, dim3(sx_blocks(c->n, 256)), dim3(256), 0, c->stream, s, c->d_ecp_pp, c->ld);
+            ensure_pp(c);
+            hipLaunchKernelGGL(k_ecp_pp, dim3(sx_blocks(c->n, 256)), dim3(256), 0, c->stream, s, c->d_ecp_pp,
+                               c->d_ecp_scr, c->d_pos4, (int *)nullptr);
         }
         int solver = c->ecp_solver == SX_ECP_LEARNED ? SX_ECP_CONVERGED : c->ecp_solver;
+        long ld = 0;
         dim3 g(sx_blocks(np, 128)), b(128);
         switch (kind) {
-        case SX_LJ: hipLaunchKernelGGL(k_pair_eval<SX_LJ>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, c->ld, solver, dout); break;
-        case SX_LJ12: hipLaunchKernelGGL(k_pair_eval<SX_LJ12>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, c->ld, solver, dout); break;
-        case SX_LJW: hipLaunchKernelGGL(k_pair_eval<SX_LJW>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, c->ld, solver, dout); break;
-        case SX_WCA: hipLaunchKernelGGL(k_pair_eval<SX_WCA>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, c->ld, solver, dout); break;
-        case SX_MORSE: hipLaunchKernelGGL(k_pair_eval<SX_MORSE>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, c->ld, solver, dout); break;
-        case SX_DH: hipLaunchKernelGGL(k_pair_eval<SX_DH>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, c->ld, solver, dout); break;
-        case SX_GB: hipLaunchKernelGGL(k_pair_eval<SX_GB>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, c->ld, solver, dout); break;
-        case SX_GBR: hipLaunchKernelGGL(k_pair_eval<SX_GBR>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, c->ld, solver, dout); break;
-        case SX_ECP: hipLaunchKernelGGL(k_pair_eval<SX_ECP>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, c->ld, solver, dout); break;
-        case SX_ECPR: hipLaunchKernelGGL(k_pair_eval<SX_ECPR>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, c->ld, solver, dout); break;
+        case SX_LJ: hipLaunchKernelGGL(k_pair_eval<SX_LJ>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
+        case SX_LJ12: hipLaunchKernelGGL(k_pair_eval<SX_LJ12>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
+        case SX_LJW: hipLaunchKernelGGL(k_pair_eval<SX_LJW>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
+        case SX_WCA: hipLaunchKernelGGL(k_pair_eval<SX_WCA>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
+        case SX_MORSE: hipLaunchKernelGGL(k_pair_eval<SX_MORSE>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
+        case SX_DH: hipLaunchKernelGGL(k_pair_eval<SX_DH>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
+        case SX_GB: hipLaunchKernelGGL(k_pair_eval<SX_GB>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
+        case SX_GBR: hipLaunchKernelGGL(k_pair_eval<SX_GBR>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
+        case SX_ECP: hipLaunchKernelGGL(k_pair_eval<SX_ECP>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
+        case SX_ECPR: hipLaunchKernelGGL(k_pair_eval<SX_ECPR>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
         }
         std::vector<double> h(11 * np);
         HIPCHK(hipMemcpyAsync(h.data(), dout, sizeof(double) * 11 * np, hipMemcpyDeviceToHost, c->stream));
