@@ -724,9 +724,12 @@ static void quat_U(const or_sys *s, int i, double U[9])
 
 static double max3(double a, double b, double c) { return fmax(fmax(a, b), c); }
 
-void or_ecp(const or_sys *s, int i, int j, int variant, double cutoff,
-            double *pe, double f[3], double ti[3], double tj[3],
-            double *lambda_out, int *niter_out)
+/* lambda_in < 0: the reference's Brent solve; otherwise the contact
+   parameter is taken as given (used to evaluate the force expressions at a
+   converged lambda*, where the envelope theorem makes F = -grad U exact). */
+static void ecp_eval(const or_sys *s, int i, int j, int variant, double cutoff, double lambda_in,
+                     double *pe, double f[3], double ti[3], double tj[3],
+                     double *lambda_out, int *niter_out)
 {
     const int meu = 2, neu = 1; /* force.f90:1038 */
     const double sigma_0 = 1.0;
@@ -770,7 +773,7 @@ void or_ecp(const or_sys *s, int i, int j, int variant, double cutoff,
     or_inverse(EAB, EM);
     double eps1 = sigma_t * sqrt(or_determinant(EM));
     int niter = 0;
-    double lambda_R = or_ecp_brent(AR, BR, RIJ, &niter);
+    double lambda_R = lambda_in < 0.0 ? or_ecp_brent(AR, BR, RIJ, &niter) : lambda_in;
     double lambda_E = lambda_R;
     if (lambda_out) *lambda_out = lambda_R;
     if (niter_out) *niter_out = niter;
@@ -868,6 +871,37 @@ void or_ecp(const or_sys *s, int i, int j, int variant, double cutoff,
     }
     *pe = pot;
     for (int k = 0; k < 3; ++k) { f[k] = FIJ[k]; ti[k] = T1[k]; tj[k] = T2[k]; }
+}
+
+void or_ecp(const or_sys *s, int i, int j, int variant, double cutoff,
+            double *pe, double f[3], double ti[3], double tj[3],
+            double *lambda_out, int *niter_out)
+{
+    ecp_eval(s, i, j, variant, cutoff, -1.0, pe, f, ti, tj, lambda_out, niter_out);
+}
+
+void or_ecp_at_lambda(const or_sys *s, int i, int j, int variant, double cutoff, double lambda,
+                      double *pe, double f[3], double ti[3], double tj[3])
+{
+    ecp_eval(s, i, j, variant, cutoff, lambda, pe, f, ti, tj, NULL, NULL);
+}
+
+/* the pair's shape matrices AR, BR (force.f90:1254-1259) and min-image
+   separation, so a caller can maximise the contact function itself */
+void or_ecp_shape(const or_sys *s, int i, int j, double ar[9], double br[9], double rij[3])
+{
+    double U1[9], U2[9], S1sq[9] = {0}, S2sq[9] = {0};
+    pair_rij(s, i, j, rij);
+    quat_U(s, i, U1);
+    quat_U(s, j, U2);
+    M(S1sq, 1, 1) = (s->sig_x[i] / 2.0) * (s->sig_x[i] / 2.0);
+    M(S1sq, 2, 2) = (s->sig_y[i] / 2.0) * (s->sig_y[i] / 2.0);
+    M(S1sq, 3, 3) = (s->sig_z[i] / 2.0) * (s->sig_z[i] / 2.0);
+    M(S2sq, 1, 1) = (s->sig_x[j] / 2.0) * (s->sig_x[j] / 2.0);
+    M(S2sq, 2, 2) = (s->sig_y[j] / 2.0) * (s->sig_y[j] / 2.0);
+    M(S2sq, 3, 3) = (s->sig_z[j] / 2.0) * (s->sig_z[j] / 2.0);
+    matmul_atba(U1, S1sq, ar);
+    matmul_atba(U2, S2sq, br);
 }
 
 /* ------------------------------------------------------------------ */

@@ -100,6 +100,9 @@ void or_ecp(const or_sys *s, int i, int j, int variant, double cutoff,
             double *pe, double f[3], double ti[3], double tj[3],
             double *lambda_out, int *niter_out);
 /* brent(optim_dist) on given AR, BR (column-major) and r (force.f90:1046-1200). */
+void or_ecp_at_lambda(const or_sys *s, int i, int j, int variant, double cutoff, double lambda,
+                      double *pe, double f[3], double ti[3], double tj[3]);
+void or_ecp_shape(const or_sys *s, int i, int j, double ar[9], double br[9], double rij[3]);
 double or_ecp_brent(const double ar[9], const double br[9], const double r[3], int *niter);
 double or_ecp_optim_dist(double l, const double ar[9], const double br[9], const double r[3]);
 

@@ -237,6 +237,27 @@ class Oracle:
             raise ValueError(kind)
         return pe.value, f, ti, tj
 
+    def ecp_shape(self, st: State, i, j):
+        """(AR, BR, RIJ) of a pair: column-major 3x3 shape matrices and the
+        min-image separation."""
+        s = self._sys(st)
+        ar = np.zeros(9); br = np.zeros(9); r = np.zeros(3)
+        self.lib.or_ecp_shape(C.byref(s), int(i), int(j), _dp(ar), _dp(br), _dp(r))
+        return ar, br, r
+
+    def optim_dist(self, lam, ar, br, r):
+        return self.lib.or_ecp_optim_dist(C.c_double(lam), _dp(ar), _dp(br), _dp(r))
+
+    def ecp_at_lambda(self, kind, st: State, i, j, p, lam):
+        """ECP/ECPR pair terms evaluated at a given contact parameter."""
+        s = self._sys(st)
+        pe = C.c_double()
+        f = np.zeros(3); ti = np.zeros(3); tj = np.zeros(3)
+        cut = p[0] if kind == ECP else 0.0
+        self.lib.or_ecp_at_lambda(C.byref(s), int(i), int(j), 0 if kind == ECP else 1, C.c_double(cut),
+                                  C.c_double(lam), C.byref(pe), _dp(f), _dp(ti), _dp(tj))
+        return pe.value, f, ti, tj
+
     def bonded(self, kind, st: State, idx, coeff, angle=0.0):
         idx = [int(x) for x in idx]
         s = self._sys(st)
