@@ -31,6 +31,16 @@ FLOP_BRENT_ITER = 20        # Brent bookkeeping per iteration
 FLOP_SCREEN = 30 + FLOP_OPTIM_DIST   # min image, bound test, f(psi)
 FLOP_ECP_FORCE = 650        # force/torque evaluation at the solved lambda
 FLOP_APPLY_ENTRY = 8        # per full-list entry of the per-particle apply
+# L2-miss bytes per launch of each ECP stage kernel at this workload, from
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch) scaled by a 256 MiB
+# copy measured the same way (FETCH x2.000, WRITE x1.000); the runs and the
+# arithmetic are in profiles/ecp_langevin_32768_fetch_write.txt
+TRAFFIC_BYTES = {
+    "ecp_precompute": (1694.5 * 2.0 + 8192.0) * 1024,
+    "ecp_screen": (10238.6 * 2.0 + 914.2) * 1024,
+    "ecp_solve": (10438.2 * 2.0 + 3885.0) * 1024,
+    "ecp_apply": (14661.7 * 2.0 + 1792.0) * 1024,
+}
 
 
 def log(*a):
@@ -98,17 +108,37 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n", type=int, default=32768)
+    ap.add_argument("--save-state", default=None, help="write the prepared workload state to this .npz")
+    ap.add_argument("--load-state", default=None,
+                    help="start from a state written by --save-state instead of preparing the workload")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 256 MiB device copy 10 times (reference point for HBM counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4, "copies": 10}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (this build runs one "
               f"process on one MI355X; multi-GPU sharding is a design note, not built)", file=sys.stderr)
         sys.exit(2)
 
     import sucrose_amd as sx
-    from sucrose_amd.workloads import ecp_langevin_state
+    from sucrose_amd.workloads import ecp_langevin_state, load_state, save_state
 
     dev = sx.System(0)
-    p = ecp_langevin_state(dev, n=args.n, log=log)
+    if args.load_state:
+        p = load_state(dev, args.load_state)
+        log(f"[workload] loaded {args.load_state}")
+    else:
+        p = ecp_langevin_state(dev, n=args.n, log=log)
+    if args.save_state:
+        save_state(dev, p, args.save_state)
     n, dt, gamma = p["n"], p["dt"], p["gamma"]
     forces = [(sx.ECPR,)]
     step = p["next_step"]
@@ -142,7 +172,8 @@ def main():
     k = max(range(4), key=lambda q: ms[q])
     ach = flops[names[k]] / (ms[k] * 1e-3) / 1e12
     roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-            "frac": ach / PEAK_FP64_TFLOPS, "traffic": None, "kernel": names[k],
+            "frac": ach / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_BYTES.get(names[k]), "kernel": names[k],
+            "traffic_unit": "bytes per launch (L2 misses: rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
             "peak_kind": "fp64 (vector ALU; MI355X fp64 dense figure)",
             "stage_ms": {names[q]: ms[q] for q in range(4)},
             "pairs_screened": screened, "pairs_solved": solved, "brent_iters_per_solved_pair": iters / max(solved, 1)}

@@ -285,8 +285,6 @@ __global__ void __launch_bounds__(256) k_lgv_final(DState s, double dt, double *
     if (dstep) *dstep = *dstep + 1;   // Langevin counter of graph replays
 }
 
-__global__ void k_step_inc(int *dstep) { *dstep = *dstep + 1; }
-
 // ---------------------------------------------------------------------------
 // Nose-Hoover chain (integrators.f90:222-492)
 //

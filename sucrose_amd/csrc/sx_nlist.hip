@@ -256,13 +256,6 @@ __global__ void __launch_bounds__(256) k_sort_rows(int n, const int *row_start, 
     }
 }
 
-__global__ void k_half_len(int n, const int *row_len, const int *lower, int *half_len, const int *flag)
-{
-    if (flag && *flag == 0) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) half_len[i] = row_len[i] - lower[i];
-}
-
 // half-pair index of every full-list entry, and the half list (i, j) pairs
 __global__ void k_half_index(int n, const int *row_start, const int *lower, const int *half_start, const int *nbr,
                              int *pairk, int *half_i, int *half_j, const int *flag, long cap_half, int *err)
@@ -349,13 +342,6 @@ __global__ void __launch_bounds__(256) k_disp_max(DState s, const double *xl, co
         *ticket = 0u;
     }
 }
-
-__global__ void k_count_rebuild(const int *flag, long *count)
-{
-    if (threadIdx.x == 0 && flag[0]) count[7] += 1;
-}
-
-__global__ void k_set_flag(int *flag, int v) { flag[0] = v; }
 
 // ---------------------------------------------------------------------------
 static void nl_free(NList &nl)

@@ -8,7 +8,6 @@
 #include <cstring>
 
 __global__ void k_step_set(int *dstep, int v) { *dstep = v; }
-__global__ void k_step_inc2(int *dstep) { *dstep = *dstep + 1; }
 
 struct DeferPe {
     sx_ctx *c;

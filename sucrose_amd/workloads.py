@@ -12,6 +12,7 @@ creates a deep overlap), then an equilibration run at the final density.
 """
 from __future__ import annotations
 
+import json
 import math
 import time
 
@@ -68,6 +69,32 @@ def ecp_langevin_state(dev: System, n=32768, phi=0.35, phi0=0.25, seed=2024, T=1
     log(f"[workload] compressed to phi={phi} (L={L1:.4f}) and equilibrated, {time.time() - t0:.1f}s total")
     return dict(n=n, box=L1, phi=phi, T=T, gamma=gamma, dt=dt, cutoff=cutoff, skin=skin, next_step=step,
                 seed=seed)
+
+
+def save_state(dev: System, params: dict, path):
+    """Writes the device state and the workload parameters to an .npz file."""
+    arrays = {f"f_{k}": v for k, v in dev.state().items()}
+    np.savez(path, params=np.array(json.dumps(params)), **arrays)
+
+
+def load_state(dev: System, path):
+    """Restores a state written by save_state (same particle count and box),
+    re-initialises the neighbour list; returns the workload parameters."""
+    z = np.load(path, allow_pickle=False)
+    params = json.loads(str(z["params"]))
+    dev.set_num_atoms(params["n"])
+    dev.set_box(params["box"])
+    for key in z.files:
+        if key.startswith("f_"):
+            dev.upload(key[2:], z[key])
+    dev.set_target_temperature(params["T"])
+    dev.initialize_system(remove_com=False)
+    for key in z.files:   # initialize_system resets velocities' derived sums only; restore fields again
+        if key.startswith("f_"):
+            dev.upload(key[2:], z[key])
+    dev.set_rebuild_rule(REBUILD_HALF_SKIN)
+    dev.init_nlist(params["skin"], params["cutoff"])
+    return params
 
 
 def overlaps_on_device(dev: System, box):
