@@ -100,6 +100,93 @@ def cpu_baseline(state_arrays, box, params, seconds):
             "seconds": el}
 
 
+TRAIN_SCHEDULE = [(3e-3, 10000), (1.5e-3, 10000), (7e-4, 10000), (3e-4, 6000), (1e-4, 4000)]
+
+
+def learned_vs_plain(dev, sx, p, step, nsteps):
+    """The lambda* network trained on the GPU by the library's own kernels
+    on pairs the contact solver generates along this trajectory, then the
+    converged contact solve with and without its warm start: each solver
+    timed alone on the same work list, forces compared, and the whole loop
+    timed in each mode on the same trajectory start."""
+    import numpy as np
+    n, dt, gamma = p["n"], p["dt"], p["gamma"]
+    t0 = time.perf_counter()
+    dev.mlp_init(seed=2024)
+    for _ in range(48):       # ~2M pairs: 48 work lists, 10 steps apart
+        dev.run(sx.INT_LGV, [(sx.ECPR,)], 10, dt, damping=gamma, first_step=step)
+        step += 10
+        samples = dev.mlp_collect(sx.ECPR, capacity=1 << 21)
+    t_collect = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    losses = []
+    for k, (lr, nst) in enumerate(TRAIN_SCHEDULE):
+        tr = dev.mlp_train(nst, batch=4096, lr=lr, w=(1.0, 0.1), seed=100 + k)
+        losses.append((float(tr[:50].mean()), float(tr[-200:].mean())))
+    dev.synchronize()
+    t_train = time.perf_counter() - t0
+    train_steps = sum(x[1] for x in TRAIN_SCHEDULE)
+    log(f"[learned] {samples} samples in {t_collect:.2f}s; {train_steps} Adam steps (batch 4096) in {t_train:.2f}s; "
+        f"loss {losses[0][0]:.3e} -> {losses[-1][1]:.3e}")
+    dev.run(sx.INT_LGV, [(sx.ECPR,)], 10, dt, damping=gamma, first_step=step)
+    step += 10
+    cmp = dev.compare_solvers(sx.ECPR, nrep=20)
+    # forces and torques on one state: reference Brent, plain converged, learned
+    ft = {}
+    for name, solver in (("brent", sx.ECP_BRENT_REFERENCE), ("plain", sx.ECP_CONVERGED),
+                         ("learned", sx.ECP_LEARNED)):
+        dev.set_ecp_solver(solver)
+        dev.initialize_forces()
+        dev.forces(sx.ECPR)
+        ft[name] = np.stack([dev.download(f) for f in ("FX", "FY", "FZ", "TX", "TY", "TZ")])
+    scale = np.abs(ft["plain"]).max()
+    err_learned = float(np.abs(ft["learned"] - ft["plain"]).max() / scale)
+    err_brent = float(np.abs(ft["brent"] - ft["plain"]).max() / scale)
+    # the whole loop in each mode from the same state
+    state = dev.state()
+    rates = {}
+    for name, solver in (("plain", sx.ECP_CONVERGED), ("learned", sx.ECP_LEARNED)):
+        for f, v in state.items():
+            dev.upload(f, v)
+        dev.build_nlist()
+        dev.set_ecp_solver(solver)
+        dev.run(sx.INT_LGV, [(sx.ECPR,)], 50, dt, damping=gamma, first_step=step)
+        dev.synchronize()
+        t0 = time.perf_counter()
+        dev.run(sx.INT_LGV, [(sx.ECPR,)], nsteps, dt, damping=gamma, first_step=step + 50)
+        dev.synchronize()
+        rates[name] = n * nsteps / (time.perf_counter() - t0)
+    dev.set_ecp_solver(sx.ECP_BRENT_REFERENCE)
+    for f, v in state.items():
+        dev.upload(f, v)
+    dev.build_nlist()
+    lam_plain = cmp["ms_newton_plain"]
+    lam_learned = cmp["ms_network"] + cmp["ms_newton_warm"]
+    out = {
+        "training": {"samples": samples, "adam_steps": train_steps, "batch": 4096, "seconds": t_train,
+                     "loss_first": losses[0][0], "loss_last": losses[-1][1], "dtype": "bf16 mfma, fp32 accumulate"},
+        "lambda0_err_rms": cmp["lambda0_err_rms"], "lambda0_err_max": cmp["lambda0_err_max"],
+        "pairs": cmp["pairs"], "proven_out_of_range_by_warm_start": cmp["proven_out_of_range"],
+        "newton_iters_per_pair": {"plain": cmp["iters_plain"], "warm": cmp["iters_warm"]},
+        "wave_max_iters_per_round": {"plain": cmp["wave_iters_plain"], "warm": cmp["wave_iters_warm"],
+                                     "brent_1e-10": cmp["wave_iters_brent_tight"]},
+        "lambda_solve_ms": {"brent_reference_tol": cmp["ms_brent"], "brent_1e-10": cmp["ms_brent_tight"],
+                            "newton_plain": lam_plain, "network": cmp["ms_network"],
+                            "newton_warm": cmp["ms_newton_warm"], "learned_total": lam_learned},
+        "speedup_vs_plain_newton": lam_plain / lam_learned,
+        "speedup_vs_brent_1e-10": cmp["ms_brent_tight"] / lam_learned,
+        "solve_stage_ms": {"plain": cmp["ms_solve_plain"], "learned": cmp["ms_solve_learned"]},
+        "force_rel_err_learned_vs_plain": err_learned,
+        "force_rel_err_brent_reference_vs_converged": err_brent,
+        "step_particle_steps_per_s": rates,
+        "_next_step": step + 50 + nsteps,
+    }
+    log(f"[learned] lambda0 rms {cmp['lambda0_err_rms']:.2e}; lambda solve ms plain {lam_plain:.4f} learned "
+        f"{lam_learned:.4f} (network {cmp['ms_network']:.4f}) brent-1e-10 {cmp['ms_brent_tight']:.4f}; "
+        f"force err learned {err_learned:.1e}, reference-brent {err_brent:.1e}; steps/s {rates}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +195,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n", type=int, default=32768)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain contact solver section")
+    ap.add_argument("--learned-steps", type=int, default=1000,
+                    help="steps of the whole loop timed for each contact solver")
     ap.add_argument("--save-state", default=None, help="write the prepared workload state to this .npz")
     ap.add_argument("--load-state", default=None,
                     help="start from a state written by --save-state instead of preparing the workload")
@@ -180,6 +270,10 @@ def main():
     log(f"[gpu] stages ms {roof['stage_ms']} screened={screened:.0f} solved={solved:.0f} "
         f"iters/pair={iters / max(solved, 1):.2f}")
 
+    learned = None if args.no_learned else learned_vs_plain(dev, sx, p, step, args.learned_steps)
+    if learned:
+        step = learned.pop("_next_step")
+
     cpu = None
     if not args.no_cpu:
         dev.initialize_forces()
@@ -199,6 +293,7 @@ def main():
                    "potential": "ecpr", "lambda_solver": "brent_reference", "rebuild_rule": "half_skin",
                    "parallelism": "single_gpu"},
         "roofline": roof,
+        "learned_vs_plain": learned,
         "cpu_baseline": cpu,
     }
     if cpu:

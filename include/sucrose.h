@@ -219,6 +219,45 @@ long sx_count_ellipsoid_overlaps(int n, double a, double b, double c, double L, 
                                  const double *quat);
 
 /* ------------------------------------------------------------------ */
+/* learned lambda* warm start for the ECP contact solve (no counterpart  */
+/* in the reference: it accelerates the solve of force.f90:1046-1200)    */
+/* ------------------------------------------------------------------ */
+/* Network 7 -> 64 -> 64 -> 64 -> 2 (SiLU), BF16 matrix-core products with FP32
+ * accumulation.  Inputs: unit separation in particle i's body frame and the
+ * relative quaternion conj(q_i) q_j (w >= 0); outputs: lambda* and the contact
+ * distance over 2 max(semi-axis).  Parameters: one flat FP32 vector
+ * W1 (64x7) b1 W2 (64x64) b2 W3 (64x64) b3 W4 (2x64) b4, W row-major (out, in). */
+int sx_mlp_num_params(void);
+int sx_mlp_init(sx_ctx *ctx, uint32_t seed);                 /* uniform +-1/sqrt(fan_in), Philox */
+int sx_mlp_set_params(sx_ctx *ctx, const float *params);
+int sx_mlp_get_params(sx_ctx *ctx, float *params);
+/* outputs Y[n][2] for inputs X[n][7] */
+int sx_mlp_forward(sx_ctx *ctx, long n, const float *X, float *Y);
+/* loss sum_o w_o (y_o - t_o)^2 / n of an explicit batch and its gradient */
+int sx_mlp_loss_grad(sx_ctx *ctx, long n, const float *X, const float *T, const float w[2], float *loss,
+                     float *grad);
+/* appends the ECP work list of the current state (kind SX_ECP/SX_ECPR) as
+ * training samples: inputs, converged lambda*, contact distance / (2 max a);
+ * the device store holds up to `capacity` samples; *total = samples held */
+int sx_mlp_collect(sx_ctx *ctx, int kind, const double *params, int nparams, long capacity, long *total);
+int sx_mlp_samples(sx_ctx *ctx, long first, long n, float *X, float *T);
+int sx_mlp_clear_samples(sx_ctx *ctx);
+/* Adam (lr, beta 0.9/0.999, eps 1e-8) on minibatches of `batch` samples (a
+ * multiple of 64) drawn with Philox(seed); loss_trace[steps] may be NULL */
+int sx_mlp_train(sx_ctx *ctx, int steps, int batch, double lr, const float w[2], uint32_t seed, float *loss_trace);
+/* contact-parameter solvers on the current state's ECP work list, each timed
+ * alone (HIP events, nrep runs).  out[19]: [0] live pairs, ms of [1] Brent
+ * (reference tolerances) [2] Newton from psi [3] network [4] Newton from the
+ * network's warm start [5] full converged solve (lambda + forces) [6] full
+ * learned solve incl. network [7] Brent at 1e-10; iterations per pair [8]
+ * plain Newton [9] warm Newton; mean per-wave maximum iterations [10] plain
+ * [11] warm; |lambda0 - lambda*| [12] mean [13] max [14] rms; [15] max
+ * |lambda_warm - lambda_plain|; [16] iterations per pair and [17] mean
+ * per-wave maximum of the 1e-10 Brent; [18] pairs the learned solve proved out
+ * of range from the warm start alone (F(lambda0) <= F(lambda*)) */
+int sx_ecp_compare_solvers(sx_ctx *ctx, int kind, const double *params, int nparams, int nrep, double out[19]);
+
+/* ------------------------------------------------------------------ */
 /* Philox4x32-10 (xmath.f90:14-113)                                    */
 /* ------------------------------------------------------------------ */
 /* host: the reference's fixed key */

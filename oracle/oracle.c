@@ -904,6 +904,85 @@ void or_ecp_shape(const or_sys *s, int i, int j, double ar[9], double br[9], dou
     matmul_atba(U2, S2sq, br);
 }
 
+/* ---- converged contact parameter and network features (no reference
+   counterpart: the converged and learned ECP solver modes) ---- */
+
+/* F(l) = l(1-l) r^T G^-1 r with G = (1-l) A + l B, and dF/dl, d2F/dl2 */
+static void ecp_F_derivs(double l, const double ar[9], const double br[9], const double r[3],
+                         double *F, double *d1, double *d2)
+{
+    double G[9], D[9], k[3], u[3], w[3];
+    for (int q = 0; q < 9; ++q) { G[q] = (1 - l) * ar[q] + l * br[q]; D[q] = br[q] - ar[q]; }
+    or_cholesky_solve(G, r, k);
+    for (int a = 0; a < 3; ++a) u[a] = D[a] * k[0] + D[a + 3] * k[1] + D[a + 6] * k[2];
+    or_cholesky_solve(G, u, w);
+    double rk = r[0] * k[0] + r[1] * k[1] + r[2] * k[2];
+    double kdk = k[0] * u[0] + k[1] * u[1] + k[2] * u[2];
+    double udw = u[0] * w[0] + u[1] * w[1] + u[2] * w[2];
+    double ll = l * (1 - l);
+    *F = ll * rk;
+    *d1 = (1 - 2 * l) * rk - ll * kdk;
+    *d2 = -2 * rk - 2 * (1 - 2 * l) * kdk + 2 * ll * udw;
+}
+
+/* maximiser of F on (0,1) (F is concave there) by Newton on F' = 0 from l0:
+   a step below tol ends the solve; a step that leaves the bracket [lo, hi]
+   kept by the sign of F' is replaced by bisection */
+double or_ecp_lambda_converged(const double ar[9], const double br[9], const double r[3], double l0, double tol,
+                               int *iters)
+{
+    double lo = 0.0, hi = 1.0, l = fmin(fmax(l0, 1e-6), 1.0 - 1e-6);
+    int it = 0;
+    for (; it < 100; ++it) {
+        double F, d1, d2;
+        ecp_F_derivs(l, ar, br, r, &F, &d1, &d2);
+        if (d1 > 0) lo = l; else hi = l;
+        if (d2 < 0) {
+            double step = -d1 / d2;
+            if (fabs(step) < tol) { l = l + step; ++it; break; }
+            double ln = l + step;
+            l = (ln > lo && ln < hi) ? ln : 0.5 * (lo + hi);
+        } else {
+            l = 0.5 * (lo + hi);
+        }
+        if (hi - lo < tol) { ++it; break; }
+    }
+    if (iters) *iters = it;
+    return l;
+}
+
+/* contact distance sigma = 1/sqrt(l(1-l) rhat^T G(l)^-1 rhat) */
+double or_ecp_contact_distance(const double ar[9], const double br[9], const double r[3], double l)
+{
+    double G[9], k[3];
+    double m = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    double rh[3] = {r[0] / m, r[1] / m, r[2] / m};
+    for (int q = 0; q < 9; ++q) G[q] = (1 - l) * ar[q] + l * br[q];
+    or_cholesky_solve(G, rh, k);
+    return 1.0 / sqrt(l * (1 - l) * (rh[0] * k[0] + rh[1] * k[1] + rh[2] * k[2]));
+}
+
+/* network inputs of a pair: the unit separation rhat = RIJ/|RIJ| in particle
+   i's body frame (U_i rhat, U = R(q)^T) and the relative rotation
+   conj(q_i) q_j with its sign chosen so that w >= 0 */
+void or_ecp_features(const or_sys *s, int i, int j, double x[7])
+{
+    double RIJ[3], U[9];
+    pair_rij(s, i, j, RIJ);
+    double m = sqrt(RIJ[0] * RIJ[0] + RIJ[1] * RIJ[1] + RIJ[2] * RIJ[2]);
+    double rh[3] = {RIJ[0] / m, RIJ[1] / m, RIJ[2] / m};
+    quat_U(s, i, U);
+    for (int a = 1; a <= 3; ++a) x[a - 1] = M(U, a, 1) * rh[0] + M(U, a, 2) * rh[1] + M(U, a, 3) * rh[2];
+    double aw = s->qw[i], ax = -s->qx[i], ay = -s->qy[i], az = -s->qz[i];
+    double bw = s->qw[j], bx = s->qx[j], by = s->qy[j], bz = s->qz[j];
+    double w = aw * bw - ax * bx - ay * by - az * bz;
+    double qx = aw * bx + ax * bw + ay * bz - az * by;
+    double qy = aw * by - ax * bz + ay * bw + az * bx;
+    double qz = aw * bz + ax * by - ay * bx + az * bw;
+    double sg = (w < 0) ? -1.0 : 1.0;
+    x[3] = sg * w; x[4] = sg * qx; x[5] = sg * qy; x[6] = sg * qz;
+}
+
 /* ------------------------------------------------------------------ */
 /* bonded (force.f90:2244-2462)                                        */
 /* ------------------------------------------------------------------ */

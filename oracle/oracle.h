@@ -103,6 +103,10 @@ void or_ecp(const or_sys *s, int i, int j, int variant, double cutoff,
 void or_ecp_at_lambda(const or_sys *s, int i, int j, int variant, double cutoff, double lambda,
                       double *pe, double f[3], double ti[3], double tj[3]);
 void or_ecp_shape(const or_sys *s, int i, int j, double ar[9], double br[9], double rij[3]);
+double or_ecp_lambda_converged(const double ar[9], const double br[9], const double r[3], double l0, double tol,
+                               int *iters);
+double or_ecp_contact_distance(const double ar[9], const double br[9], const double r[3], double l);
+void or_ecp_features(const or_sys *s, int i, int j, double x[7]);
 double or_ecp_brent(const double ar[9], const double br[9], const double r[3], int *niter);
 double or_ecp_optim_dist(double l, const double ar[9], const double br[9], const double r[3]);
 

@@ -248,6 +248,28 @@ class Oracle:
     def optim_dist(self, lam, ar, br, r):
         return self.lib.or_ecp_optim_dist(C.c_double(lam), _dp(ar), _dp(br), _dp(r))
 
+    def lambda_converged(self, ar, br, r, l0=0.3819660112501051, tol=1e-10):
+        """Converged contact parameter (safeguarded Newton) and its iterations."""
+        it = C.c_int()
+        f = self.lib.or_ecp_lambda_converged
+        f.restype = C.c_double
+        ar, br, r = (np.ascontiguousarray(x, dtype=np.float64) for x in (ar, br, r))
+        lam = f(_dp(ar), _dp(br), _dp(r), C.c_double(l0), C.c_double(tol), C.byref(it))
+        return lam, it.value
+
+    def contact_distance(self, ar, br, r, lam):
+        f = self.lib.or_ecp_contact_distance
+        f.restype = C.c_double
+        ar, br, r = (np.ascontiguousarray(x, dtype=np.float64) for x in (ar, br, r))
+        return f(_dp(ar), _dp(br), _dp(r), C.c_double(lam))
+
+    def features(self, st: State, i, j):
+        """The 7 network inputs of pair (i, j)."""
+        s = self._sys(st)
+        x = np.zeros(7)
+        self.lib.or_ecp_features(C.byref(s), int(i), int(j), _dp(x))
+        return x
+
     def ecp_at_lambda(self, kind, st: State, i, j, p, lam):
         """ECP/ECPR pair terms evaluated at a given contact parameter."""
         s = self._sys(st)

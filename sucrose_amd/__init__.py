@@ -110,6 +110,19 @@ def lib():
         L.sx_count_ellipsoid_overlaps.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                                   _PD, _PD]
         L.sx_count_ellipsoid_overlaps.restype = C.c_long
+        PF = C.POINTER(C.c_float)
+        for name, args in {
+            "sx_mlp_init": [vp, C.c_uint32], "sx_mlp_set_params": [vp, PF], "sx_mlp_get_params": [vp, PF],
+            "sx_mlp_forward": [vp, C.c_long, PF, PF],
+            "sx_mlp_loss_grad": [vp, C.c_long, PF, PF, PF, PF, PF],
+            "sx_mlp_collect": [vp, C.c_int, _PD, C.c_int, C.c_long, C.POINTER(C.c_long)],
+            "sx_mlp_samples": [vp, C.c_long, C.c_long, PF, PF], "sx_mlp_clear_samples": [vp],
+            "sx_mlp_train": [vp, C.c_int, C.c_int, C.c_double, PF, C.c_uint32, PF],
+            "sx_ecp_compare_solvers": [vp, C.c_int, _PD, C.c_int, C.c_int, _PD],
+        }.items():
+            getattr(L, name).argtypes = args
+            getattr(L, name).restype = C.c_int
+        L.sx_mlp_num_params.restype = C.c_int
         _lib = L
     return _lib
 
@@ -120,6 +133,10 @@ def philox4x32_10(ctr):
     o = (C.c_uint32 * 4)()
     lib().sx_philox4x32_10(c, o)
     return np.array(list(o), dtype=np.uint32)
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
 def _dp(a):
@@ -299,6 +316,69 @@ class System:
         stats = np.zeros(3)
         self._ok(self.L.sx_profile_ecp(self.h, int(kind), _dp(p), len(params), int(nrep), _dp(ms), _dp(stats)))
         return ms, stats
+
+    # ---- learned lambda* warm start ----
+    def mlp_init(self, seed=1):
+        self._ok(self.L.sx_mlp_init(self.h, int(seed) & 0xFFFFFFFF))
+
+    def mlp_set_params(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        if a.size != self.L.sx_mlp_num_params():
+            raise ValueError("wrong parameter count")
+        self._ok(self.L.sx_mlp_set_params(self.h, _fp(a)))
+
+    def mlp_get_params(self):
+        a = np.zeros(self.L.sx_mlp_num_params(), dtype=np.float32)
+        self._ok(self.L.sx_mlp_get_params(self.h, _fp(a)))
+        return a
+
+    def mlp_forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 7)
+        Y = np.zeros((len(X), 2), dtype=np.float32)
+        self._ok(self.L.sx_mlp_forward(self.h, len(X), _fp(X), _fp(Y)))
+        return Y
+
+    def mlp_loss_grad(self, X, T, w=(1.0, 1.0)):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 7)
+        T = np.ascontiguousarray(T, dtype=np.float32).reshape(-1, 2)
+        wv = np.array(w, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        g = np.zeros(self.L.sx_mlp_num_params(), dtype=np.float32)
+        self._ok(self.L.sx_mlp_loss_grad(self.h, len(X), _fp(X), _fp(T), _fp(wv), _fp(loss), _fp(g)))
+        return float(loss[0]), g
+
+    def mlp_collect(self, kind, params=(), capacity=1 << 21):
+        p = np.array(params if params else [0.0], dtype=np.float64)
+        tot = C.c_long()
+        self._ok(self.L.sx_mlp_collect(self.h, int(kind), _dp(p), len(params), int(capacity), C.byref(tot)))
+        return tot.value
+
+    def mlp_samples(self, first, n):
+        X = np.zeros((n, 7), dtype=np.float32)
+        T = np.zeros((n, 2), dtype=np.float32)
+        self._ok(self.L.sx_mlp_samples(self.h, int(first), int(n), _fp(X), _fp(T)))
+        return X, T
+
+    def mlp_clear_samples(self):
+        self._ok(self.L.sx_mlp_clear_samples(self.h))
+
+    def mlp_train(self, steps, batch=4096, lr=1e-3, w=(1.0, 1.0), seed=7):
+        wv = np.array(w, dtype=np.float32)
+        trace = np.zeros(steps, dtype=np.float32)
+        self._ok(self.L.sx_mlp_train(self.h, int(steps), int(batch), float(lr), _fp(wv), int(seed) & 0xFFFFFFFF,
+                                     _fp(trace)))
+        return trace
+
+    def compare_solvers(self, kind, params=(), nrep=10):
+        """Contact-parameter solvers on the current ECP work list (times in ms)."""
+        p = np.array(params if params else [0.0], dtype=np.float64)
+        o = np.zeros(19)
+        self._ok(self.L.sx_ecp_compare_solvers(self.h, int(kind), _dp(p), len(params), int(nrep), _dp(o)))
+        keys = ["pairs", "ms_brent", "ms_newton_plain", "ms_network", "ms_newton_warm", "ms_solve_plain",
+                "ms_solve_learned", "ms_brent_tight", "iters_plain", "iters_warm", "wave_iters_plain",
+                "wave_iters_warm", "lambda0_err_mean", "lambda0_err_max", "lambda0_err_rms",
+                "lambda_warm_vs_plain_max", "iters_brent_tight", "wave_iters_brent_tight", "proven_out_of_range"]
+        return dict(zip(keys, o.tolist()))
 
     def set_bonded(self, kind, idx1, coeff, angle=0.0):
         a = np.ascontiguousarray(idx1, dtype=np.int32).reshape(-1)

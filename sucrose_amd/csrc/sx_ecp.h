@@ -53,14 +53,29 @@ __device__ __forceinline__ double ecp_optim_dist(double l, const Sym3 &A, const 
     return -l * (1 - l) * (r[0] * km[0] + r[1] * km[1] + r[2] * km[2]);
 }
 
+// The half list is cut into ECP_NSEG contiguous segments (one per block);
+// the screen compacts the survivors of each segment in order into the same
+// segment of the work array and records their count (segcount).  No atomics:
+// the work list is deterministic and later stages walk the segments.
+#define ECP_NSEG 1024
+
+__device__ __forceinline__ void ecp_segment(long nnz, int b, long &start, long &end)
+{
+    long chunk = (nnz + ECP_NSEG - 1) / ECP_NSEG;
+    start = (long)b * chunk;
+    end = start + chunk < nnz ? start + chunk : nnz;
+    if (start > nnz) start = nnz;
+}
+
 #define ECP_PSI 0.3819660112501051  /* 0.5*(3-sqrt(5)) in FP64, force.f90:1050 */
 
-// brent (force.f90:1046-1172) continuing from the first evaluation fc = f(psi)
-__device__ __forceinline__ double ecp_brent(const Sym3 &A, const Sym3 &B, const double r[3], double fc, int &iters)
+// brent (force.f90:1046-1172) continuing from the first evaluation fc = f(psi);
+// atol/rtol are the reference's 1e-3/1e-2 unless a caller tightens them
+__device__ __forceinline__ double ecp_brent(const Sym3 &A, const Sym3 &B, const double r[3], double fc, int &iters,
+                                            double atol = 1e-3, double rtol = 1e-2)
 {
     const double psi = ECP_PSI;
     double a = 0.0, b = 1.0, deltax = 0.0, rat = 0.0;
-    const double atol = 1e-3, rtol = 1e-2;
     double c = psi;
     double v = c, w = v, x = w;
     double fv = fc, fw = fv, ft = fw;
@@ -136,27 +151,48 @@ __device__ __forceinline__ void ecp_F_derivs(double l, const Sym3 &A, const Sym3
     d2 = -2 * rk - 2 * (1 - 2 * l) * kdk + 2 * ll * udw;
 }
 
-// maximiser of F on (0,1) to |dl| < tol: safeguarded Newton on F' = 0.
-// F is concave on (0,1) with F'(0) > 0 > F'(1), so [lo,hi] always brackets.
+// maximiser of F on (0,1): Newton on F' = 0 from l0.  F is concave on (0,1)
+// with F'(0) > 0 > F'(1), so the sign of F' keeps a bracket [lo, hi]; a Newton
+// step below tol ends the solve (that step is taken), a step that leaves the
+// bracket is replaced by bisection.  `ok` is 0 only if 100 iterations pass.
 __device__ __forceinline__ double ecp_lambda_newton(const Sym3 &A, const Sym3 &B, const double r[3], double l0,
                                                     double tol, int &iters, int &ok)
 {
     double lo = 0.0, hi = 1.0, l = fmin(fmax(l0, 1e-6), 1.0 - 1e-6);
     ok = 0;
     int it = 0;
-    for (; it < 60; ++it) {
+    for (; it < 100; ++it) {
         double F, d1, d2;
         ecp_F_derivs(l, A, B, r, F, d1, d2);
         if (d1 > 0) lo = l; else hi = l;
-        double step = (d2 < 0) ? -d1 / d2 : 0.0;
-        double ln = l + step;
-        if (!(d2 < 0) || !(ln > lo && ln < hi)) ln = 0.5 * (lo + hi);
-        double dl = fabs(ln - l);
-        l = ln;
-        if (dl < tol || (hi - lo) < tol) { ok = 1; ++it; break; }
+        if (d2 < 0) {
+            double step = -d1 / d2;
+            if (fabs(step) < tol) { l = l + step; ++it; ok = 1; break; }
+            double ln = l + step;
+            l = (ln > lo && ln < hi) ? ln : 0.5 * (lo + hi);
+        } else {
+            l = 0.5 * (lo + hi);
+        }
+        if (hi - lo < tol) { ++it; ok = 1; break; }
     }
     iters = it;
     return l;
+}
+
+// F(l) <= F(lambda*) for every l (lambda* maximises F), so an F(l) beyond the
+// contact threshold proves that the pair gives zero at the converged lambda*:
+// ecpr is non-zero only for r <= sigma = r / sqrt(F*), i.e. F* <= 1; ecp only
+// for r < sigma + cutoff - sigma0, i.e. F* < (r / (r - cutoff + 1))^2.
+__device__ __forceinline__ bool ecp_zero_proven(double F, double rsq, int variant, double cutoff)
+{
+    double thr = 1.0;
+    if (variant == 0) {
+        double r = sqrt(rsq);
+        double den = r - cutoff + 1.0;
+        if (den <= 0.0) return false;
+        thr = (r / den) * (r / den);
+    }
+    return F > thr * (1.0 + 1e-12);
 }
 
 // The force/torque evaluation of ecp/ecpr (force.f90:1487-1550) for a given

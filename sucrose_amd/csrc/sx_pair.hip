@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 struct Params {
     double p[8];
@@ -187,19 +188,6 @@ __device__ __forceinline__ bool ecp_screen32(const double4 *pos4, const float *s
     return F <= thr;
 }
 
-// The half list is cut into ECP_NSEG contiguous segments (one per block);
-// each block compacts the survivors of its segment in order into the same
-// segment of the work array and records their count.  No atomics: the work
-// list is deterministic and the solve walks the segments.
-#define ECP_NSEG 1024
-
-__device__ __forceinline__ void ecp_segment(long nnz, int b, long &start, long &end)
-{
-    long chunk = (nnz + ECP_NSEG - 1) / ECP_NSEG;
-    start = (long)b * chunk;
-    end = start + chunk < nnz ? start + chunk : nnz;
-    if (start > nnz) start = nnz;
-}
 
 __global__ void __launch_bounds__(256) k_ecp_screen(const double4 *pos4, const float *scr, const int *half_i,
                                                     const int *half_j, const int *nnz_half, EcpScreen sc,
@@ -244,7 +232,7 @@ __device__ __forceinline__ double ecp_solve_lambda(int solver, const double *pp,
     double l = ecp_lambda_newton(A, B, RIJ, lam0, 1e-10, iters, ok);
     if (!ok) {
         int it2 = 0;
-        l = ecp_lambda_newton(A, B, RIJ, 0.5, 1e-10, it2, ok);
+        l = ecp_lambda_newton(A, B, RIJ, ECP_PSI, 1e-10, it2, ok);
         iters += it2;
         fallback = 1;
     }
@@ -254,28 +242,45 @@ __device__ __forceinline__ double ecp_solve_lambda(int solver, const double *pp,
 #define ECP_SOLVE_THREADS 64
 
 // one block per segment of the work list; stats (may be NULL) accumulates
-// solved pairs, solver iterations and fallbacks
+// solved pairs, solver iterations, fallbacks and, for the learned solver,
+// pairs whose warm start already proves them out of range (their live flag
+// is cleared so the apply stage skips them)
 template <int VARIANT>
 __global__ void __launch_bounds__(ECP_SOLVE_THREADS) k_ecp_solve(DState s, const double *pp, long ld, const int *half_i,
                                                                  const int *half_j, const int *work, const int *segcount,
                                                                  const int *nnz_half, EcpScreen sc, int solver,
-                                                                 const float *lam0, double *res, long *stats)
+                                                                 const float *lam0, double *res, unsigned char *live,
+                                                                 long *stats)
 {
     long start, end;
     ecp_segment(*nnz_half, blockIdx.x, start, end);
     long cnt = segcount[blockIdx.x];
-    long my_iters = 0, my_solved = 0, my_fb = 0;
+    long my_iters = 0, my_solved = 0, my_fb = 0, my_skip = 0;
     for (long u = threadIdx.x; u < cnt; u += blockDim.x) {
         long t = start + u;
         int k = work[t];
         int a = half_i[k], b = half_j[k];
         double RIJ[3];
         pair_rij(s, a, b, RIJ);
-        Sym3 A = load_sym(pp, ld, PP_AR, a), B = load_sym(pp, ld, PP_AR, b);
-        double fpsi = ecp_optim_dist(ECP_PSI, A, B, RIJ);
         int iters = 0, fb = 0;
-        double l0 = lam0 ? (double)lam0[t] : 0.5;
-        double lam = ecp_solve_lambda<VARIANT>(solver, pp, ld, a, b, RIJ, fpsi, l0, iters, fb);
+        double lam;
+        if (solver == SX_ECP_BRENT_REFERENCE) {
+            Sym3 A = load_sym(pp, ld, PP_AR, a), B = load_sym(pp, ld, PP_AR, b);
+            lam = ecp_brent(A, B, RIJ, ecp_optim_dist(ECP_PSI, A, B, RIJ), iters);
+        } else {
+            double l0 = ECP_PSI;
+            if (lam0) {
+                l0 = (double)lam0[t];
+                Sym3 A = load_sym(pp, ld, PP_AR, a), B = load_sym(pp, ld, PP_AR, b);
+                double rsq = RIJ[0] * RIJ[0] + RIJ[1] * RIJ[1] + RIJ[2] * RIJ[2];
+                if (ecp_zero_proven(-ecp_optim_dist(l0, A, B, RIJ), rsq, VARIANT, sc.cutoff)) {
+                    live[k] = 0;
+                    ++my_skip;
+                    continue;
+                }
+            }
+            lam = ecp_solve_lambda<VARIANT>(solver, pp, ld, a, b, RIJ, 0.0, l0, iters, fb);
+        }
         PairOut o;
         ecp_forces<VARIANT>(s, pp, ld, a, b, sc.cutoff, RIJ, lam, o);
         double *r = res + (long)k * 10;
@@ -289,16 +294,18 @@ __global__ void __launch_bounds__(ECP_SOLVE_THREADS) k_ecp_solve(DState s, const
     }
     (void)end;
     if (!stats) return;
-    // stats: [0] solved pairs, [1] iterations, [2] fallbacks (wave-reduced)
+    // stats: [0] solved pairs, [1] iterations, [2] fallbacks, [3] proven zero by the warm start
     for (int off = SX_WAVE / 2; off > 0; off >>= 1) {
         my_iters += __shfl_xor(my_iters, off, SX_WAVE);
         my_solved += __shfl_xor(my_solved, off, SX_WAVE);
         my_fb += __shfl_xor(my_fb, off, SX_WAVE);
+        my_skip += __shfl_xor(my_skip, off, SX_WAVE);
     }
-    if ((threadIdx.x & (SX_WAVE - 1)) == 0 && my_solved) {
+    if ((threadIdx.x & (SX_WAVE - 1)) == 0 && (my_solved || my_skip)) {
         atomicAdd((unsigned long long *)&stats[0], (unsigned long long)my_solved);
         atomicAdd((unsigned long long *)&stats[1], (unsigned long long)my_iters);
         if (my_fb) atomicAdd((unsigned long long *)&stats[2], (unsigned long long)my_fb);
+        if (my_skip) atomicAdd((unsigned long long *)&stats[3], (unsigned long long)my_skip);
     }
 }
 
@@ -358,10 +365,10 @@ __global__ void k_pair_eval(DState s, long np, const int *pi, const int *pj, Par
         double fpsi = ecp_optim_dist(ECP_PSI, A, B, RIJ);
         int it = 0, fb = 0;
         if (KIND == SX_ECP) {
-            lam = ecp_solve_lambda<0>(solver, pp, ld, a, b, RIJ, fpsi, 0.5, it, fb);
+            lam = ecp_solve_lambda<0>(solver, pp, ld, a, b, RIJ, fpsi, ECP_PSI, it, fb);
             ecp_forces<0>(s, pp, ld, a, b, P.p[0], RIJ, lam, o);
         } else {
-            lam = ecp_solve_lambda<1>(solver, pp, ld, a, b, RIJ, fpsi, 0.5, it, fb);
+            lam = ecp_solve_lambda<1>(solver, pp, ld, a, b, RIJ, fpsi, ECP_PSI, it, fb);
             ecp_forces<1>(s, pp, ld, a, b, 0.0, RIJ, lam, o);
         }
     } else {
@@ -464,8 +471,8 @@ static void ensure_pair_buffers(sx_ctx *c)
     ensure_pp(c);
 }
 
-// learned warm start: predictions for the work list (implemented with the MLP)
-const float *sx_ecp_mlp_predict(sx_ctx *c, const double *pp, long ld, const int *work, const int *segcount, long cap);
+// learned warm start: predictions for the work list of the current call
+const float *sx_ecp_mlp_predict(sx_ctx *c);
 
 static EcpScreen make_screen(sx_ctx *c, int kind, const Params &P)
 {
@@ -514,17 +521,16 @@ static void ecp_stage(sx_ctx *c, int stage, int kind, const EcpScreen &sc, const
         break;
     case 2: {
         const float *lam0 = nullptr;
-        if (c->ecp_solver == SX_ECP_LEARNED)
-            lam0 = sx_ecp_mlp_predict(c, c->d_ecp_pp, ld, c->d_work, c->d_segcount, L.cap);
+        if (c->ecp_solver == SX_ECP_LEARNED) lam0 = sx_ecp_mlp_predict(c);
         long *stats = L.stats ? c->d_counts : nullptr;
         if (kind == SX_ECPR)
             hipLaunchKernelGGL(k_ecp_solve<1>, dim3(ECP_NSEG), dim3(ECP_SOLVE_THREADS), 0, c->stream, s, c->d_ecp_pp,
                                ld, nl.half_i, nl.half_j, c->d_work, c->d_segcount, nnz_half, sc, c->ecp_solver, lam0,
-                               c->d_pairbuf, stats);
+                               c->d_pairbuf, c->d_live, stats);
         else
             hipLaunchKernelGGL(k_ecp_solve<0>, dim3(ECP_NSEG), dim3(ECP_SOLVE_THREADS), 0, c->stream, s, c->d_ecp_pp,
                                ld, nl.half_i, nl.half_j, c->d_work, c->d_segcount, nnz_half, sc, c->ecp_solver, lam0,
-                               c->d_pairbuf, stats);
+                               c->d_pairbuf, c->d_live, stats);
         break;
     }
     case 3:
@@ -532,6 +538,88 @@ static void ecp_stage(sx_ctx *c, int stage, int kind, const EcpScreen &sc, const
                            c->d_live, c->d_pairbuf, c->d_pe_acc);
         break;
     }
+}
+
+// contact parameter only, for the solver comparison: MODE 0 the reference's
+// Brent, 1 Newton from psi, 2 the learned solve (warm start lam0; pairs whose
+// warm start proves them out of range end there), 3 the reference's Brent
+// with its tolerances tightened to atol = rtol = 1e-10.  iters (may be NULL):
+// [0] += iterations, [1] += the wave's maximum iterations of each round of
+// 64 pairs, [2] += rounds, [3] += pairs proven out of range
+template <int MODE>
+__global__ void __launch_bounds__(ECP_SOLVE_THREADS) k_ecp_lambda(DState s, const double *pp, const int *half_i,
+                                                                  const int *half_j, const int *work,
+                                                                  const int *segcount, const int *nnz_half,
+                                                                  EcpScreen sc, const float *lam0, double *lam_out,
+                                                                  long *iters)
+{
+    long start, end;
+    ecp_segment(*nnz_half, blockIdx.x, start, end);
+    const int cnt = segcount[blockIdx.x];
+    long my_it = 0, my_wmax = 0, rounds = 0, my_skip = 0;
+    for (int base = 0; base < cnt; base += ECP_SOLVE_THREADS, ++rounds) {
+        int u = base + threadIdx.x;
+        int it = 0;
+        if (u < cnt) {
+            long t = start + u;
+            int k = work[t];
+            int a = half_i[k], b = half_j[k];
+            double RIJ[3];
+            pair_rij(s, a, b, RIJ);
+            Sym3 A = load_sym(pp, 0, PP_AR, a), B = load_sym(pp, 0, PP_AR, b);
+            double l;
+            if (MODE == 0) {
+                l = ecp_brent(A, B, RIJ, ecp_optim_dist(ECP_PSI, A, B, RIJ), it);
+            } else if (MODE == 3) {
+                l = ecp_brent(A, B, RIJ, ecp_optim_dist(ECP_PSI, A, B, RIJ), it, 1e-10, 1e-10);
+            } else if (MODE == 1) {
+                int ok = 0;
+                l = ecp_lambda_newton(A, B, RIJ, ECP_PSI, 1e-10, it, ok);
+            } else {
+                l = (double)lam0[t];
+                double rsq = RIJ[0] * RIJ[0] + RIJ[1] * RIJ[1] + RIJ[2] * RIJ[2];
+                if (ecp_zero_proven(-ecp_optim_dist(l, A, B, RIJ), rsq, sc.variant, sc.cutoff)) {
+                    ++my_skip;
+                    l = -1.0;   // out of range: no contact parameter needed
+                } else {
+                    int ok = 0;
+                    l = ecp_lambda_newton(A, B, RIJ, l, 1e-10, it, ok);
+                }
+            }
+            lam_out[t] = l;
+        }
+        my_it += it;
+        int wm = it;
+        for (int off = SX_WAVE / 2; off > 0; off >>= 1) wm = max(wm, __shfl_xor(wm, off, SX_WAVE));
+        my_wmax += wm;
+    }
+    (void)end;
+    if (!iters) return;
+    for (int off = SX_WAVE / 2; off > 0; off >>= 1) {
+        my_it += __shfl_xor(my_it, off, SX_WAVE);
+        my_skip += __shfl_xor(my_skip, off, SX_WAVE);
+    }
+    if ((threadIdx.x & (SX_WAVE - 1)) == 0 && cnt) {
+        atomicAdd((unsigned long long *)&iters[0], (unsigned long long)my_it);
+        atomicAdd((unsigned long long *)&iters[1], (unsigned long long)my_wmax);
+        atomicAdd((unsigned long long *)&iters[2], (unsigned long long)rounds);
+        if (my_skip) atomicAdd((unsigned long long *)&iters[3], (unsigned long long)my_skip);
+    }
+}
+
+// per-particle records and the screened work list of kind on the current
+// state (the first two stages of a force call)
+void sx_ecp_prepare(sx_ctx *c, int kind, const double *p, int np)
+{
+    if (kind != SX_ECP && kind != SX_ECPR) throw sx_error{SX_ERR_ARG, "an ECP kind is required"};
+    if (!c->nl.init) throw sx_error{SX_ERR_STATE, "neighbour list not initialised (sx_nlist_init)"};
+    check_params(kind, np);
+    Params P = make_params(c, kind, p, np);
+    ensure_pair_buffers(c);
+    EcpScreen sc = make_screen(c, kind, P);
+    EcpLaunch L = ecp_launch_shape(c, false);
+    ecp_stage(c, 0, kind, sc, L);
+    ecp_stage(c, 1, kind, sc, L);
 }
 
 static void ecp_forces_impl(sx_ctx *c, int kind, const Params &P)
@@ -706,3 +794,136 @@ int sx_pair_eval(sx_ctx *c, int kind, const double *params, int nparams, long np
 }
 
 }  // extern "C"
+
+// Contact-parameter solvers on the current state's ECP work list, each timed
+// alone with HIP events over nrep runs: the reference's Brent, Brent with
+// tolerances tightened to 1e-10, Newton from psi (plain) and Newton from the
+// network's warm start (learned), plus the full solve stage (lambda +
+// force/torque) of the converged and learned modes.  out[19]:
+//  [0] live pairs  [1] ms Brent  [2] ms plain Newton  [3] ms network
+//  [4] ms warm Newton  [5] ms full plain solve  [6] ms full learned solve
+//  (network included)  [7] ms tight Brent  [8] iterations/pair plain
+//  [9] iterations/pair warm  [10] mean per-wave maximum iterations plain
+//  [11] same, warm  [12] mean |lambda0 - lambda*|  [13] max  [14] rms
+//  [15] max |lambda_warm - lambda_plain|  [16] iterations/pair tight Brent
+//  [17] mean per-wave maximum iterations, tight Brent  [18] pairs the
+//  learned solve proved out of range from its warm start alone
+extern "C" int sx_ecp_compare_solvers(sx_ctx *c, int kind, const double *params, int nparams, int nrep,
+                                      double out[19])
+{
+    return guard(c, [&] {
+        need_system(c);
+        if (!c->mlp) throw sx_error{SX_ERR_STATE, "lambda* network not initialised (sx_mlp_init)"};
+        if (nrep < 1) nrep = 1;
+        sx_ecp_prepare(c, kind, params, nparams);
+        Params P = make_params(c, kind, params, nparams);
+        EcpScreen sc = make_screen(c, kind, P);
+        NList &nl = c->nl;
+        DState s = c->dstate();
+        const int *nnz_half = c->d_flag + 2;
+        long cap = nl.cap_half;
+        double *lam_b = nullptr, *lam_p = nullptr, *lam_w = nullptr;
+        long *cnt = nullptr;
+        HIPCHK(hipMalloc(&lam_b, sizeof(double) * cap));
+        HIPCHK(hipMalloc(&lam_p, sizeof(double) * cap));
+        HIPCHK(hipMalloc(&lam_w, sizeof(double) * cap));
+        HIPCHK(hipMalloc(&cnt, sizeof(long) * 12));
+        HIPCHK(hipMemsetAsync(cnt, 0, sizeof(long) * 12, c->stream));
+        const float *lam0 = sx_ecp_mlp_predict(c);
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        auto timed = [&](auto &&launch) {
+            launch();   // warm
+            HIPCHK(hipEventRecord(e0, c->stream));
+            for (int r = 0; r < nrep; ++r) launch();
+            HIPCHK(hipEventRecord(e1, c->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            return (double)ms / nrep;
+        };
+        dim3 g(ECP_NSEG), b(ECP_SOLVE_THREADS);
+        double t_brent = timed([&] {
+            hipLaunchKernelGGL(k_ecp_lambda<0>, g, b, 0, c->stream, s, c->d_ecp_pp, nl.half_i, nl.half_j, c->d_work,
+                               c->d_segcount, nnz_half, sc, lam0, lam_b, (long *)nullptr);
+        });
+        double t_brent_tight = timed([&] {
+            hipLaunchKernelGGL(k_ecp_lambda<3>, g, b, 0, c->stream, s, c->d_ecp_pp, nl.half_i, nl.half_j, c->d_work,
+                               c->d_segcount, nnz_half, sc, lam0, lam_b, (long *)nullptr);
+        });
+        double t_plain = timed([&] {
+            hipLaunchKernelGGL(k_ecp_lambda<1>, g, b, 0, c->stream, s, c->d_ecp_pp, nl.half_i, nl.half_j, c->d_work,
+                               c->d_segcount, nnz_half, sc, lam0, lam_p, (long *)nullptr);
+        });
+        double t_mlp = timed([&] { sx_ecp_mlp_predict(c); });
+        double t_warm = timed([&] {
+            hipLaunchKernelGGL(k_ecp_lambda<2>, g, b, 0, c->stream, s, c->d_ecp_pp, nl.half_i, nl.half_j, c->d_work,
+                               c->d_segcount, nnz_half, sc, lam0, lam_w, (long *)nullptr);
+        });
+        auto full = [&](int solver, const float *l0) {
+            if (kind == SX_ECPR)
+                hipLaunchKernelGGL(k_ecp_solve<1>, g, b, 0, c->stream, s, c->d_ecp_pp, 0L, nl.half_i, nl.half_j,
+                                   c->d_work, c->d_segcount, nnz_half, sc, solver, l0, c->d_pairbuf, c->d_live,
+                                   (long *)nullptr);
+            else
+                hipLaunchKernelGGL(k_ecp_solve<0>, g, b, 0, c->stream, s, c->d_ecp_pp, 0L, nl.half_i, nl.half_j,
+                                   c->d_work, c->d_segcount, nnz_half, sc, solver, l0, c->d_pairbuf, c->d_live,
+                                   (long *)nullptr);
+        };
+        double t_full_plain = timed([&] { full(SX_ECP_CONVERGED, nullptr); });
+        double t_full_learned = timed([&] { full(SX_ECP_LEARNED, sx_ecp_mlp_predict(c)); });
+        // iteration counts (4 counters per solver)
+        hipLaunchKernelGGL(k_ecp_lambda<3>, g, b, 0, c->stream, s, c->d_ecp_pp, nl.half_i, nl.half_j, c->d_work,
+                           c->d_segcount, nnz_half, sc, lam0, lam_b, cnt + 0);
+        hipLaunchKernelGGL(k_ecp_lambda<1>, g, b, 0, c->stream, s, c->d_ecp_pp, nl.half_i, nl.half_j, c->d_work,
+                           c->d_segcount, nnz_half, sc, lam0, lam_p, cnt + 4);
+        hipLaunchKernelGGL(k_ecp_lambda<2>, g, b, 0, c->stream, s, c->d_ecp_pp, nl.half_i, nl.half_j, c->d_work,
+                           c->d_segcount, nnz_half, sc, lam0, lam_w, cnt + 8);
+        long hc[12];
+        int nnz = 0;
+        std::vector<int> seg(ECP_NSEG);
+        std::vector<double> hp(cap), hw(cap);
+        std::vector<float> h0(cap);
+        HIPCHK(hipMemcpyAsync(hc, cnt, sizeof(long) * 12, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(&nnz, nnz_half, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(seg.data(), c->d_segcount, sizeof(int) * ECP_NSEG, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(hp.data(), lam_p, sizeof(double) * cap, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(hw.data(), lam_w, sizeof(double) * cap, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(h0.data(), lam0, sizeof(float) * cap, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        long live = 0;
+        double s_err = 0, s_err2 = 0, m_err = 0, m_dw = 0;
+        long chunk = ((long)nnz + ECP_NSEG - 1) / ECP_NSEG;
+        for (int q = 0; q < ECP_NSEG; ++q) {
+            long st = std::min<long>((long)q * chunk, nnz);
+            for (int u = 0; u < seg[q]; ++u) {
+                long t = st + u;
+                double e = std::fabs((double)h0[t] - hp[t]);
+                s_err += e;
+                s_err2 += e * e;
+                m_err = std::max(m_err, e);
+                if (hw[t] >= 0.0) m_dw = std::max(m_dw, std::fabs(hw[t] - hp[t]));   // pairs the learned solve solved
+                ++live;
+            }
+        }
+        double L = (double)std::max<long>(live, 1);
+        auto per = [](long a, long b) { return b > 0 ? (double)a / (double)b : 0.0; };
+        out[0] = (double)live;
+        out[1] = t_brent; out[2] = t_plain; out[3] = t_mlp; out[4] = t_warm;
+        out[5] = t_full_plain; out[6] = t_full_learned;
+        out[7] = t_brent_tight;
+        out[8] = hc[4] / L; out[9] = hc[8] / L;
+        out[10] = per(hc[5], hc[6]); out[11] = per(hc[9], hc[10]);
+        out[12] = s_err / L; out[13] = m_err; out[14] = std::sqrt(s_err2 / L); out[15] = m_dw;
+        out[16] = hc[0] / L; out[17] = per(hc[1], hc[2]);
+        out[18] = (double)hc[11];
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        HIPCHK(hipFree(lam_b));
+        HIPCHK(hipFree(lam_p));
+        HIPCHK(hipFree(lam_w));
+        HIPCHK(hipFree(cnt));
+        sx_check_device_error(c);
+    });
+}

@@ -438,3 +438,60 @@ def test_rattle_satisfies_constraints(oracle):
     oracle.integrate(VVC, 1, st, 0.005)
     v = np.stack([st[f][bi] - st[f][bj] for f in ["vx", "vy", "vz"]], 1)
     assert np.abs((d * v).sum(1)).max() < 1e-4
+
+
+def test_converged_lambda_and_contact_distance(oracle):
+    """The converged contact parameter maximises the contact function (scipy's
+    bounded minimiser to 1e-12 agrees), equals a/(a+b) for spheres of radii
+    a, b, whose contact distance is a+b."""
+    from scipy.optimize import minimize_scalar
+    rng = np.random.default_rng(16)
+    for ar, br, r in _shape_pairs(rng, 100):
+        ar, br = ar.ravel(order="F"), br.ravel(order="F")
+        lam, it = oracle.lambda_converged(ar, br, r)
+        ref = minimize_scalar(lambda l: oracle.optim_dist(l, ar, br, r), bounds=(0, 1), method="bounded",
+                              options={"xatol": 1e-12})
+        # the optimum is quadratic, so the minimiser's argument is good to ~1e-8
+        # and its value to ~1e-16: lambda agrees and is at least as good
+        f = oracle.optim_dist(lam, ar, br, r)
+        assert abs(lam - ref.x) < 1e-6 and f <= ref.fun + 1e-14 * abs(ref.fun) and it < 40
+    for _ in range(20):
+        a, b = rng.uniform(0.3, 2.0, 2)
+        r = rng.normal(size=3) * 4
+        lam, _ = oracle.lambda_converged(np.eye(3).ravel() * a * a, np.eye(3).ravel() * b * b, r)
+        assert abs(lam - a / (a + b)) < 1e-12
+        sig = oracle.contact_distance(np.eye(3).ravel() * a * a, np.eye(3).ravel() * b * b, r, lam)
+        assert abs(sig - (a + b)) < 1e-12
+
+
+def test_network_features(oracle):
+    """Features of a pair: the unit separation in particle i's body frame and
+    conj(q_i) q_j with w >= 0.  Identity orientations give (rhat, 1, 0, 0, 0);
+    rotating both particles and the separation together leaves them fixed."""
+    rng = np.random.default_rng(17)
+    st = base_state(rng, 2, 30.0)
+    d = rng.normal(size=3) * 2
+    st["rx"], st["ry"], st["rz"] = [d[0], 0.0], [d[1], 0.0], [d[2], 0.0]
+    for f, v in zip(["qw", "qx", "qy", "qz"], [1.0, 0.0, 0.0, 0.0]):
+        st[f] = v
+    x0 = oracle.features(st, 0, 1)
+    assert np.allclose(x0[:3], d / np.linalg.norm(d)) and np.allclose(x0[3:], [1, 0, 0, 0])
+    qi, qj = rand_quats(rng, 2)
+    st2 = st.copy()
+    for k, f in enumerate(["qw", "qx", "qy", "qz"]):
+        st2[f] = [qi[k], qj[k]]
+    x1 = oracle.features(st2, 0, 1)
+    g = rand_quats(rng, 1)[0]                      # rotate the whole pair by g
+    Rg = _rot_body_from_space(g).T                 # body -> space of g
+    st3 = st2.copy()
+    dn = Rg @ d
+    st3["rx"], st3["ry"], st3["rz"] = [dn[0], 0.0], [dn[1], 0.0], [dn[2], 0.0]
+    for who, q in ((0, qi), (1, qj)):
+        a0, a1, a2, a3 = g
+        w, x, y, z = q
+        qq = [a0 * w - a1 * x - a2 * y - a3 * z, a0 * x + a1 * w + a2 * z - a3 * y,
+              a0 * y - a1 * z + a2 * w + a3 * x, a0 * z + a1 * y - a2 * x + a3 * w]
+        for k, f in enumerate(["qw", "qx", "qy", "qz"]):
+            st3[f][who] = qq[k]
+    assert np.allclose(oracle.features(st3, 0, 1), x1, atol=1e-12)
+    assert x1[3] >= 0 and abs(np.linalg.norm(x1[3:]) - 1) < 1e-12
