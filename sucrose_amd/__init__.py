@@ -110,6 +110,11 @@ def lib():
         L.sx_count_ellipsoid_overlaps.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                                   _PD, _PD]
         L.sx_count_ellipsoid_overlaps.restype = C.c_long
+        L.sx_gen_uniform_quaternions.argtypes = [C.c_int, C.c_uint32, _PD]
+        L.sx_gen_uniform_quaternions.restype = None
+        L.sx_gen_chains.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint32, C.c_long,
+                                    _PD]
+        L.sx_gen_chains.restype = C.c_int
         PF = C.POINTER(C.c_float)
         for name, args in {
             "sx_mlp_init": [vp, C.c_uint32], "sx_mlp_set_params": [vp, PF], "sx_mlp_get_params": [vp, PF],
@@ -151,6 +156,22 @@ def gen_ellipsoids_rsa(n, semi_axes, L, seed, max_attempts=200_000_000):
     k = lib().sx_gen_ellipsoids_rsa(int(n), a, b, c, float(L), int(seed) & 0xFFFFFFFF, int(max_attempts),
                                     _dp(pos), _dp(quat))
     return pos.reshape(n, 3)[:k], quat.reshape(n, 4)[:k]
+
+
+def uniform_quaternions(n, seed):
+    q = np.zeros(4 * n)
+    lib().sx_gen_uniform_quaternions(int(n), int(seed) & 0xFFFFFFFF, _dp(q))
+    return q.reshape(n, 4)
+
+
+def gen_chains(nchains, nbeads, L, bond=1.0, min_dist=0.8, seed=1, max_attempts=2_000_000_000):
+    """Self-avoiding random-walk chains; returns positions (nchains*nbeads, 3)."""
+    pos = np.zeros(3 * nchains * nbeads)
+    k = lib().sx_gen_chains(int(nchains), int(nbeads), float(L), float(bond), float(min_dist),
+                            int(seed) & 0xFFFFFFFF, int(max_attempts), _dp(pos))
+    if k != nchains:
+        raise RuntimeError(f"placed {k} of {nchains} chains")
+    return pos.reshape(-1, 3)
 
 
 def count_ellipsoid_overlaps(pos, quat, semi_axes, L):

@@ -430,6 +430,291 @@ __global__ void __launch_bounds__(NHT_CHUNK) k_nht(DState s, double dt, int phas
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same chain recurrence evaluated in parallel (DEER: Newton iteration on
+// the whole sequence).  The particles are cut into NHT_P consecutive chunks,
+// one per thread.  A chunk maps the chain state entering it, s = (p_eta(1:3),
+// sum of m v^2), to the state leaving it; a pass with forward-mode dual
+// numbers gives that map's value and 4x4 Jacobian at a guessed entry state.
+// The affine models of all chunks are composed with a parallel prefix scan,
+// which yields new entry states; the iteration converges quadratically to the
+// states the serial walk produces, and a last pass applies the updates.  The
+// results equal the serial recurrence to rounding (the particle updates are
+// the same expressions in the same order; only each chunk's entry state is
+// reached through the scan).
+// ---------------------------------------------------------------------------
+struct D4 {
+    double v, d[4];
+};
+__device__ __forceinline__ D4 dd(double c) { return D4{c, {0, 0, 0, 0}}; }
+__device__ __forceinline__ D4 operator+(const D4 &a, const D4 &b)
+{
+    return D4{a.v + b.v, {a.d[0] + b.d[0], a.d[1] + b.d[1], a.d[2] + b.d[2], a.d[3] + b.d[3]}};
+}
+__device__ __forceinline__ D4 operator-(const D4 &a, const D4 &b)
+{
+    return D4{a.v - b.v, {a.d[0] - b.d[0], a.d[1] - b.d[1], a.d[2] - b.d[2], a.d[3] - b.d[3]}};
+}
+__device__ __forceinline__ D4 operator*(const D4 &a, const D4 &b)
+{
+    D4 r;
+    r.v = a.v * b.v;
+    for (int k = 0; k < 4; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+}
+__device__ __forceinline__ D4 operator/(const D4 &a, const D4 &b)
+{
+    D4 r;
+    r.v = a.v / b.v;
+    double ib = 1.0 / b.v;
+    for (int k = 0; k < 4; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) * ib;
+    return r;
+}
+__device__ __forceinline__ D4 operator+(const D4 &a, double b) { return a + dd(b); }
+__device__ __forceinline__ D4 operator-(const D4 &a, double b) { return a - dd(b); }
+__device__ __forceinline__ D4 operator*(const D4 &a, double b)
+{
+    return D4{a.v * b, {a.d[0] * b, a.d[1] * b, a.d[2] * b, a.d[3] * b}};
+}
+__device__ __forceinline__ D4 operator/(const D4 &a, double b) { return a / dd(b); }
+__device__ __forceinline__ D4 operator*(double a, const D4 &b) { return b * a; }
+__device__ __forceinline__ D4 operator-(double a, const D4 &b) { return dd(a) - b; }
+__device__ __forceinline__ D4 operator-(const D4 &a) { return dd(0.0) - a; }
+__device__ __forceinline__ double val(double x) { return x; }
+__device__ __forceinline__ double val(const D4 &x) { return x.v; }
+__device__ __forceinline__ double texp(double x) { return exp(x); }
+__device__ __forceinline__ D4 texp(const D4 &x)
+{
+    double e = exp(x.v);
+    return D4{e, {e * x.d[0], e * x.d[1], e * x.d[2], e * x.d[3]}};
+}
+
+// exprel and u4 of the serial kernel above, for doubles or dual numbers
+template <class T>
+__device__ __forceinline__ T exprel_t(const T &x)
+{
+    const double g0 = 1.0 / 1, g1 = 1.0 / 6, g2 = 1.0 / 120, g3 = 1.0 / 5040, g4 = 1.0 / 362880;
+    if (fabs(val(x)) > 0.01) return (texp(x) - 1.0) / x;
+    T h = (x / 2.0) * (x / 2.0);
+    T f = h * g4 + g3;
+    f = f * h + g2;
+    f = f * h + g1;
+    f = f * h + g0;
+    return texp(x / 2.0) * f;
+}
+
+template <class T>
+__device__ __forceinline__ void u4_t(double t, int dir, const T &sumv, double ndofT, double Temp, const double hb[3],
+                                     T p[3])
+{
+    for (int q = 0; q < 3; ++q) {
+        int j = dir < 0 ? 3 - q : 1 + q;
+        T gj = (j == 1) ? (sumv - ndofT) : ((p[j - 2] * p[j - 2] / hb[j - 2]) - Temp);
+        if (j == 3) {
+            p[2] = p[2] + gj * t;
+        } else {
+            T x = p[j] * t / hb[j];
+            T c = exprel_t(-x);
+            p[j - 1] = p[j - 1] * texp(-x) + gj * t * c;
+        }
+    }
+}
+
+// one particle's turn of nht_initial (phase 0) / nht_final (phase 1): chain
+// state (p, sumv) in, out; v: the particle's velocity (updated); eta_add: its
+// eta increments
+template <class T>
+__device__ __forceinline__ void nht_turn(int phase, double dt, double tq, double ndofT, double Temp, const double hb[3],
+                                         double m, const double a[3], T p[3], T &sumv, T v[3], T eta_add[3])
+{
+    if (phase == 1) {
+        T old = (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * m;
+        v[0] = v[0] + a[0]; v[1] = v[1] + a[1]; v[2] = v[2] + a[2];
+        sumv = sumv - old + (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * m;
+    }
+    u4_t(tq, -1, sumv, ndofT, Temp, hb, p);
+    T fac = texp(p[0] * (-0.5 * dt) / hb[0]);
+    T before = (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * m;
+    v[0] = v[0] * fac; v[1] = v[1] * fac; v[2] = v[2] * fac;
+    sumv = sumv - before + (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * m;
+    for (int k = 0; k < 3; ++k) eta_add[k] = p[k] * (0.5 * dt) / hb[k];
+    u4_t(tq, +1, sumv, ndofT, Temp, hb, p);
+    if (phase == 0) {
+        T mid = (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * m;
+        v[0] = v[0] + a[0]; v[1] = v[1] + a[1]; v[2] = v[2] + a[2];
+        sumv = sumv - mid + (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * m;
+    }
+}
+
+#define NHT_P 512
+#define NHT_MAXIT 16
+struct AffMap {
+    double M[16];   // row-major 4x4
+    double c[4];
+};
+
+// (b after a): x -> Mb (Ma x + ca) + cb
+__device__ __forceinline__ AffMap aff_compose(const AffMap &b, const AffMap &a)
+{
+    AffMap r;
+    for (int i = 0; i < 4; ++i) {
+        for (int j = 0; j < 4; ++j) {
+            double acc = 0.0;
+            for (int k = 0; k < 4; ++k) acc += b.M[i * 4 + k] * a.M[k * 4 + j];
+            r.M[i * 4 + j] = acc;
+        }
+        double acc = b.c[i];
+        for (int k = 0; k < 4; ++k) acc += b.M[i * 4 + k] * a.c[k];
+        r.c[i] = acc;
+    }
+    return r;
+}
+
+__global__ void __launch_bounds__(NHT_P) k_nht_deer(DState s, double dt, int phase, int save, double *scal, double ndof)
+{
+    __shared__ AffMap maps[NHT_P];
+    __shared__ double g[NHT_P][4];
+    __shared__ double red[NHT_P];
+    __shared__ double s0[4];
+    __shared__ int converged;
+    const int t = threadIdx.x, n = s.n;
+    const int chunk = (n + NHT_P - 1) / NHT_P;
+    const int i0 = min(n, t * chunk), i1 = min(n, i0 + chunk);
+    // sum of m |v|^2 before this call (tree order of the serial kernel)
+    double acc = 0.0;
+    for (int i = t; i < n; i += NHT_P) {
+        double vx = s.f[SX_VX][i], vy = s.f[SX_VY][i], vz = s.f[SX_VZ][i];
+        acc += s.f[SX_MASS][i] * (vx * vx + vy * vy + vz * vz);
+    }
+    red[t] = acc;
+    __syncthreads();
+    for (int o = NHT_P / 2; o > 0; o >>= 1) {
+        if (t < o) red[t] = red[t] + red[t + o];
+        __syncthreads();
+    }
+    const double Temp = scal[SX_TARGET_TEMP];
+    const double hb[3] = {scal[SX_HB0], scal[SX_HB1], scal[SX_HB2]};
+    const double ndofT = ndof * Temp, tq = dt / 4.0;
+    if (t == 0) {
+        s0[0] = scal[SX_PETA0]; s0[1] = scal[SX_PETA1]; s0[2] = scal[SX_PETA2]; s0[3] = red[0];
+        converged = 0;
+    }
+    __syncthreads();
+    // first guess of every chunk's entry state: the state entering the call
+    for (int k = 0; k < 4; ++k) g[t][k] = s0[k];
+    __syncthreads();
+    for (int it = 0; it < NHT_MAXIT && !converged; ++it) {
+        // chunk pass with dual numbers seeded at the guessed entry state
+        D4 p[3], sv;
+        for (int k = 0; k < 3; ++k) { p[k] = dd(g[t][k]); p[k].d[k] = 1.0; }
+        sv = dd(g[t][3]);
+        sv.d[3] = 1.0;
+        for (int i = i0; i < i1; ++i) {
+            double m = s.f[SX_MASS][i], im = s.f[SX_INV_MASS][i];
+            double a[3] = {0.5 * dt * (s.f[SX_FX][i] * im), 0.5 * dt * (s.f[SX_FY][i] * im),
+                           0.5 * dt * (s.f[SX_FZ][i] * im)};
+            D4 v[3] = {dd(s.f[SX_VX][i]), dd(s.f[SX_VY][i]), dd(s.f[SX_VZ][i])};
+            D4 ea[3];
+            nht_turn(phase, dt, tq, ndofT, Temp, hb, m, a, p, sv, v, ea);
+        }
+        // affine model of this chunk: x -> J x + (E - J g)
+        AffMap A;
+        const D4 *out[4] = {&p[0], &p[1], &p[2], &sv};
+        for (int r = 0; r < 4; ++r) {
+            double cr = out[r]->v;
+            for (int k = 0; k < 4; ++k) {
+                A.M[r * 4 + k] = out[r]->d[k];
+                cr -= out[r]->d[k] * g[t][k];
+            }
+            A.c[r] = cr;
+        }
+        maps[t] = A;
+        __syncthreads();
+        // inclusive prefix of the maps (Hillis-Steele, later after earlier; the
+        // neighbour is read before anyone overwrites it)
+        for (int o = 1; o < NHT_P; o <<= 1) {
+            AffMap m2 = maps[t];
+            if (t >= o) m2 = aff_compose(m2, maps[t - o]);
+            __syncthreads();
+            maps[t] = m2;
+            __syncthreads();
+        }
+        // new entry states: g(t) = prefix(t-1)(s0)
+        double ng[4];
+        if (t == 0) {
+            for (int k = 0; k < 4; ++k) ng[k] = s0[k];
+        } else {
+            const AffMap &P = maps[t - 1];
+            for (int r = 0; r < 4; ++r) {
+                double v = P.c[r];
+                for (int k = 0; k < 4; ++k) v += P.M[r * 4 + k] * s0[k];
+                ng[r] = v;
+            }
+        }
+        double dev = 0.0;
+        for (int k = 0; k < 4; ++k) {
+            double d = fabs(ng[k] - g[t][k]) / (fabs(ng[k]) + 1e-300);
+            dev = (d == d && dev == dev) ? fmax(dev, d) : INFINITY;   // a NaN never counts as converged
+        }
+        red[t] = dev;
+        __syncthreads();
+        for (int k = 0; k < 4; ++k) g[t][k] = ng[k];
+        for (int o = NHT_P / 2; o > 0; o >>= 1) {
+            if (t < o) red[t] = fmax(red[t], red[t + o]);
+            __syncthreads();
+        }
+        if (t == 0 && red[0] < 1e-12) converged = 1;
+        __syncthreads();
+    }
+    // not converged (a far-from-equilibrium chain can need more Newton steps):
+    // the serial walk from the start, one lane, as the small-system kernel
+    if (!converged) {
+        if (t == 0)
+            for (int k = 0; k < 4; ++k) g[0][k] = s0[k];
+        __syncthreads();
+    }
+    const int i0f = converged ? i0 : (t == 0 ? 0 : n), i1f = converged ? i1 : (t == 0 ? n : n);
+    // final pass from the converged entry states
+    double p[3] = {g[t][0], g[t][1], g[t][2]}, sv = g[t][3];
+    double eta_loc[3] = {0.0, 0.0, 0.0}, ke = 0.0;
+    for (int i = i0f; i < i1f; ++i) {
+        double m = s.f[SX_MASS][i], im = s.f[SX_INV_MASS][i];
+        double a[3] = {0.5 * dt * (s.f[SX_FX][i] * im), 0.5 * dt * (s.f[SX_FY][i] * im),
+                       0.5 * dt * (s.f[SX_FZ][i] * im)};
+        double v[3] = {s.f[SX_VX][i], s.f[SX_VY][i], s.f[SX_VZ][i]};
+        double ea[3];
+        nht_turn(phase, dt, tq, ndofT, Temp, hb, m, a, p, sv, v, ea);
+        for (int k = 0; k < 3; ++k) eta_loc[k] += ea[k];
+        s.f[SX_VX][i] = v[0]; s.f[SX_VY][i] = v[1]; s.f[SX_VZ][i] = v[2];
+        if (phase == 0) {
+            if (save) save_old(s, i);
+            s.f[SX_RX][i] = s.f[SX_RX][i] + dt * v[0];
+            s.f[SX_RY][i] = s.f[SX_RY][i] + dt * v[1];
+            s.f[SX_RZ][i] = s.f[SX_RZ][i] + dt * v[2];
+            wrap_pos(s, i);
+        } else {
+            ke += 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        }
+    }
+    // the chain state after the last particle belongs to the last non-empty chunk
+    if (i1f == n && i0f < n) {
+        scal[SX_PETA0] = p[0]; scal[SX_PETA1] = p[1]; scal[SX_PETA2] = p[2];
+    }
+    for (int k = 0; k < 4; ++k) {
+        red[t] = k < 3 ? eta_loc[k] : ke;
+        __syncthreads();
+        for (int o = NHT_P / 2; o > 0; o >>= 1) {
+            if (t < o) red[t] = red[t] + red[t + o];
+            __syncthreads();
+        }
+        if (t == 0) {
+            if (k < 3) scal[SX_ETA0 + k] = scal[SX_ETA0 + k] + red[0];
+            else if (phase == 1) scal[SX_KIN_ENG] = red[0];
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void k_nht_hb(double *scal, double ndof, double coupling, int has_coupling)
 {
     if (threadIdx.x != 0) return;
@@ -744,9 +1029,16 @@ void sx_nht_impl(sx_ctx *c, double dt, const double *coupling, int phase, bool s
     double ndof = (double)(3 * (c->n / c->nrigid) - 3);
     hipLaunchKernelGGL(k_nht_hb, dim3(1), dim3(64), 0, c->stream, c->d_scal, ndof, coupling ? *coupling : 0.0,
                        coupling ? 1 : 0);
-    sx_ensure_part(c, 1, 1);
-    hipLaunchKernelGGL(k_nht, dim3(1), dim3(NHT_CHUNK), 0, c->stream, c->dstate(), dt, phase, save ? 1 : 0, c->d_scal,
-                       ndof, c->d_part);
+    // the serial walk for small systems, the parallel (DEER) evaluation above
+    // a few thousand particles
+    if (c->n < 2048) {
+        sx_ensure_part(c, 1, 1);
+        hipLaunchKernelGGL(k_nht, dim3(1), dim3(NHT_CHUNK), 0, c->stream, c->dstate(), dt, phase, save ? 1 : 0,
+                           c->d_scal, ndof, c->d_part);
+    } else {
+        hipLaunchKernelGGL(k_nht_deer, dim3(1), dim3(NHT_P), 0, c->stream, c->dstate(), dt, phase, save ? 1 : 0,
+                           c->d_scal, ndof);
+    }
 }
 
 void sx_rattle_impl(sx_ctx *c, double dt, int phase)

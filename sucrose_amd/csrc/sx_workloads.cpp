@@ -6,6 +6,11 @@
 //    by the Perram-Wertheim contact function F = max_l l(1-l) r^T G(l)^-1 r
 //    (overlap iff F < 1), the function whose maximiser the ECP kernels solve
 //    for (force.f90:1188-1200).
+//  sx_gen_uniform_quaternions: uniformly distributed unit quaternions
+//    (Shoemake's subgroup algorithm from three uniforms).
+//  sx_gen_chains: self-avoiding random-walk chains of fixed bond length in a
+//    cubic periodic box (a bead is rejected closer than min_dist to any
+//    placed bead; a chain that cannot grow is withdrawn and restarted).
 //  Random numbers come from the reference's Philox4x32-10 stream, counter
 //  (seed, attempt, k, 0x57524b4c).
 #include <algorithm>
@@ -192,6 +197,110 @@ long sx_count_ellipsoid_overlaps(int n, double a, double b, double c, double L, 
                 }
     }
     return cnt;
+}
+
+// n uniformly distributed unit quaternions (w x y z)
+void sx_gen_uniform_quaternions(int n, uint32_t seed, double *quat)
+{
+    Rng rng{seed};
+    const double tp = 6.283185307179586;
+    for (int i = 0; i < n; ++i) {
+        double u1 = rng.u(), u2 = rng.u(), u3 = rng.u();
+        quat[4 * i + 0] = std::sqrt(1 - u1) * std::sin(tp * u2);
+        quat[4 * i + 1] = std::sqrt(1 - u1) * std::cos(tp * u2);
+        quat[4 * i + 2] = std::sqrt(u1) * std::sin(tp * u3);
+        quat[4 * i + 3] = std::sqrt(u1) * std::cos(tp * u3);
+    }
+}
+
+// nchains self-avoiding random walks of nbeads beads, bond length `bond`, no
+// two beads closer than min_dist (< bond), in a periodic cube of edge L;
+// pos[3 * nchains * nbeads] wrapped into [-L/2, L/2).  Returns the number of
+// complete chains placed.
+int sx_gen_chains(int nchains, int nbeads, double L, double bond, double min_dist, uint32_t seed,
+                  long max_attempts, double *pos)
+{
+    Rng rng{seed};
+    const long n = (long)nchains * nbeads;
+    int nc = std::max(1, (int)std::floor(L / std::max(min_dist, 1e-9)));
+    if (nc < 3) nc = 1;
+    double cw = L / nc;
+    std::vector<long> head((size_t)nc * nc * nc, -1), next((size_t)n, -1), prev((size_t)n, -1);
+    std::vector<size_t> cellof((size_t)n);
+    auto wrap = [&](double x) { return x - std::floor(x / L + 0.5) * L; };
+    auto cell_of = [&](const double *p) {
+        int ci[3];
+        for (int k = 0; k < 3; ++k) ci[k] = std::min(std::max((int)std::floor((p[k] + L / 2) / cw), 0), nc - 1);
+        return ((size_t)ci[0] * nc + ci[1]) * nc + ci[2];
+    };
+    auto clash = [&](const double *p, long skip) {
+        int c0 = (int)cell_of(p);
+        int cz = c0 % nc, cy = (c0 / nc) % nc, cx = c0 / (nc * nc);
+        int span = nc >= 3 ? 1 : 0;
+        for (int dx = -span; dx <= span; ++dx)
+            for (int dy = -span; dy <= span; ++dy)
+                for (int dz = -span; dz <= span; ++dz) {
+                    size_t c = ((size_t)((cx + dx + nc) % nc) * nc + (cy + dy + nc) % nc) * nc + (cz + dz + nc) % nc;
+                    for (long j = head[c]; j >= 0; j = next[j]) {
+                        if (j == skip) continue;
+                        double r2 = 0;
+                        for (int k = 0; k < 3; ++k) {
+                            double d = p[k] - pos[3 * j + k];
+                            d -= std::round(d / L) * L;
+                            r2 += d * d;
+                        }
+                        if (r2 < min_dist * min_dist) return true;
+                    }
+                }
+        return false;
+    };
+    auto insert = [&](long i) {
+        size_t c = cell_of(&pos[3 * i]);
+        cellof[i] = c;
+        next[i] = head[c];
+        prev[i] = -1;
+        if (head[c] >= 0) prev[head[c]] = i;
+        head[c] = i;
+    };
+    auto remove = [&](long i) {
+        size_t c = cellof[i];
+        if (prev[i] >= 0) next[prev[i]] = next[i]; else head[c] = next[i];
+        if (next[i] >= 0) prev[next[i]] = prev[i];
+    };
+    long attempts = 0;
+    int done = 0;
+    for (int ch = 0; ch < nchains && attempts < max_attempts; ++ch) {
+        long base = (long)ch * nbeads;
+        int b = 0;
+        int stuck = 0;
+        while (b < nbeads && attempts < max_attempts) {
+            ++attempts;
+            double p[3];
+            if (b == 0) {
+                for (int k = 0; k < 3; ++k) p[k] = (rng.u() - 0.5) * L;
+            } else {
+                double z = 2 * rng.u() - 1, ph = 6.283185307179586 * rng.u(), s = std::sqrt(1 - z * z);
+                const double *q = &pos[3 * (base + b - 1)];
+                p[0] = wrap(q[0] + bond * s * std::cos(ph));
+                p[1] = wrap(q[1] + bond * s * std::sin(ph));
+                p[2] = wrap(q[2] + bond * z);
+            }
+            if (clash(p, b > 0 ? base + b - 1 : -1)) {
+                if (++stuck > 200) {   // withdraw the chain and start it again
+                    for (int k = 0; k < b; ++k) remove(base + k);
+                    b = 0;
+                    stuck = 0;
+                }
+                continue;
+            }
+            for (int k = 0; k < 3; ++k) pos[3 * (base + b) + k] = p[k];
+            insert(base + b);
+            ++b;
+            stuck = 0;
+        }
+        if (b == nbeads) ++done; else break;
+    }
+    return done;
 }
 
 }  // extern "C"

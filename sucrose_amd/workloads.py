@@ -1,6 +1,20 @@
 """Seeded synthetic workloads of the benchmark configurations, prepared on the
 device with the library's own routines.
 
+GB 8000 (Gay-Berne GB(3,5,2,1) with a Nose-Hoover chain): 20^3 particles on a
+cubic lattice with uniformly random orientations.  At number density 0.30
+(lattice spacing 1.494) randomly oriented 3:1 ellipsoids overlap, so the
+lattice starts at spacing 3.2 (no overlap for any orientation) and is
+compressed affinely to spacing 1.494 in stages interleaved with short
+Langevin runs of the Gay-Berne potential itself, then equilibrated with the
+Nose-Hoover chain.
+
+Chains 100000: 2000 self-avoiding random-walk chains of 50 beads (bond 1.0,
+no two beads closer than 0.95) at bead density 0.5, charges alternating +1/-1
+along each chain, RATTLE bonds, cosine bend k=20 about the straight chain,
+dihedral k(1 + cos phi) with k=1, WCA (sigma=1) and Debye-Hueckel (kappa=1,
+cutoff 5, dielectric 1) over one neighbour list.
+
 ECP 32k (the headline): identical prolate ellipsoids, semi-axes
 (0.5, 0.5, 1.5) (sig = 1, 1, 3), N = 32768, packing fraction 0.35, random
 positions and orientations with no overlaps.  Random sequential addition
@@ -18,7 +32,8 @@ import time
 
 import numpy as np
 
-from . import ECPR, INT_LGV, REBUILD_HALF_SKIN, System, count_ellipsoid_overlaps, gen_ellipsoids_rsa
+from . import (DH, ECPR, GB, INT_LGV, INT_LGVC, INT_NHT, REBUILD_HALF_SKIN, WCA, System, count_ellipsoid_overlaps,
+               gen_chains, gen_ellipsoids_rsa, uniform_quaternions)
 
 ELLIPSOID_SEMI_AXES = (0.5, 0.5, 1.5)
 
@@ -69,6 +84,94 @@ def ecp_langevin_state(dev: System, n=32768, phi=0.35, phi0=0.25, seed=2024, T=1
     log(f"[workload] compressed to phi={phi} (L={L1:.4f}) and equilibrated, {time.time() - t0:.1f}s total")
     return dict(n=n, box=L1, phi=phi, T=T, gamma=gamma, dt=dt, cutoff=cutoff, skin=skin, next_step=step,
                 seed=seed)
+
+
+GB_SIG = (1.0, 1.0, 3.0)   # kappa = 3
+GB_EPS = (5.0, 5.0, 1.0)   # kappa' = 5
+
+
+def gb_nhc_state(dev: System, n_side=20, rho=0.30, seed=15, T=3.0, dt=0.0015, cutoff=4.0, skin=0.4, mu=2, nu=1,
+                 a0=3.2, stages=60, steps_per_stage=30, dt_compress=0.001, equil_steps=200, log=print):
+    """Builds the Gay-Berne + Nose-Hoover workload on dev; returns its parameters."""
+    t0 = time.time()
+    n = n_side ** 3
+    L1 = (n / rho) ** (1.0 / 3.0)
+    L0 = a0 * n_side
+    g = (np.arange(n_side) + 0.5) * a0 - L0 / 2
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    quat = uniform_quaternions(n, seed)
+    dev.set_num_atoms(n)
+    dev.set_box(L0)
+    for k, f in enumerate(("RX", "RY", "RZ")):
+        dev.upload(f, pos[:, k])
+    for k, f in enumerate(("QW", "QX", "QY", "QZ")):
+        dev.upload(f, quat[:, k])
+    for k, ax in enumerate("XYZ"):
+        dev.upload("SIG_" + ax, GB_SIG[k])
+        dev.upload("EPS_" + ax, GB_EPS[k])
+    dev.set_target_temperature(T)
+    dev.initialize_system()
+    dev.set_velocity_to_temperature(T, seed)
+    dev.set_rebuild_rule(REBUILD_HALF_SKIN)
+    dev.init_nlist(skin, cutoff)
+    gb = [(GB, cutoff, mu, nu)]
+    step = 1
+    Lc = L0
+    for s in range(1, stages + 1):
+        Ln = L0 * (L1 / L0) ** (s / stages)
+        f = Ln / Lc
+        for c in ("RX", "RY", "RZ"):
+            dev.upload(c, dev.download(c) * f)
+        dev.set_box(Ln)
+        dev.init_nlist(skin, cutoff)
+        dev.run(INT_LGV, gb, steps_per_stage, dt_compress, damping=1.0, first_step=step)
+        step += steps_per_stage
+        Lc = Ln
+    dev.run(INT_NHT, gb, equil_steps, dt, first_step=step)
+    step += equil_steps
+    pe = dev.get_scalar("POT_ENG")
+    if not np.isfinite(pe):
+        raise RuntimeError("Gay-Berne workload preparation diverged")
+    log(f"[workload] GB {n}: lattice {a0} -> {L1 / n_side:.4f} spacing (L={L1:.4f}), NHC equilibrated, "
+        f"PE/N={pe / n:.4f} T={dev.get_scalar('TEMP'):.4f}, {time.time() - t0:.1f}s")
+    return dict(n=n, box=L1, rho=rho, T=T, dt=dt, cutoff=cutoff, skin=skin, mu=mu, nu=nu, next_step=step, seed=seed)
+
+
+def chains_state(dev: System, nchains=2000, nbeads=50, rho=0.5, seed=7, T=1.0, dt=0.005, gamma=1.0, bond=1.0,
+                 min_dist=0.95, k_bend=20.0, bend_angle=180.0, k_dih=1.0, dh=(5.0, 1.0, 1.0), skin=0.3,
+                 equil_steps=200, log=print):
+    """Builds the charged-chain workload on dev; returns its parameters and the force list."""
+    t0 = time.time()
+    n = nchains * nbeads
+    L = (n / rho) ** (1.0 / 3.0)
+    pos = gen_chains(nchains, nbeads, L, bond=bond, min_dist=min_dist, seed=seed)
+    dev.set_num_atoms(n)
+    dev.set_box(L)
+    for k, f in enumerate(("RX", "RY", "RZ")):
+        dev.upload(f, pos[:, k])
+    charge = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
+    dev.upload("CHARGE", charge)
+    ids = np.arange(n).reshape(nchains, nbeads) + 1          # 1-based
+    bi = ids[:, :-1].ravel()
+    bj = ids[:, 1:].ravel()
+    dev.set_constraints(bi, bj, np.full(len(bi), bond))
+    dev.set_target_temperature(T)
+    dev.initialize_system()
+    dev.set_velocity_to_temperature(T, seed)
+    # cosine bend about the straight chain (cosine_bend_, force.f90:2345), dihedral k(1 + cos phi) (dih_, :2286)
+    dev.set_bonded(2, np.stack([ids[:, :-2], ids[:, 1:-1], ids[:, 2:]], -1).reshape(-1), k_bend, bend_angle)
+    dev.set_bonded(3, np.stack([ids[:, :-3], ids[:, 1:-2], ids[:, 2:-1], ids[:, 3:]], -1).reshape(-1), k_dih)
+    dev.set_rebuild_rule(REBUILD_HALF_SKIN)
+    dev.init_nlist(skin, dh[0])
+    forces = [(WCA, 1.0, 1.0), (DH,) + tuple(dh)]
+    dev.run(INT_LGVC, forces, equil_steps, dt, damping=gamma, first_step=1, bonded=True)
+    pe = dev.get_scalar("POT_ENG")
+    if not np.isfinite(pe):
+        raise RuntimeError("chain workload preparation diverged")
+    log(f"[workload] chains {nchains}x{nbeads} (L={L:.4f}) equilibrated {equil_steps} steps, PE/N={pe / n:.4f} "
+        f"T={dev.get_scalar('TEMP'):.4f}, {time.time() - t0:.1f}s")
+    return dict(n=n, box=L, rho=rho, T=T, dt=dt, gamma=gamma, skin=skin, next_step=1 + equil_steps, seed=seed,
+                forces=forces, nchains=nchains, nbeads=nbeads)
 
 
 def save_state(dev: System, params: dict, path):

@@ -158,9 +158,8 @@ def test_forces_pair_loop(dev, oracle, kind, params, rc, kw, n, L):
     assert np.any(ref["fx"] != 0)
 
 
-def _integrate_case(dev, oracle, kind, phase, constrained=False):
+def _integrate_case(dev, oracle, kind, phase, constrained=False, n=640):
     rng = np.random.default_rng(31 + kind * 2 + phase)
-    n = 640
     st = base_state(rng, n, 9.0, sig=(1, 1, 3))
     st["peta0"], st["peta1"], st["peta2"] = 0.3, -0.2, 0.1
     if constrained:
@@ -194,6 +193,18 @@ def test_integrator_one_step(dev, oracle, kind, phase):
     for sname in ("kin_eng", "kin_rot", "peta0", "peta1", "peta2", "eta0"):
         if phase == 1 or sname.startswith(("peta", "eta")):
             assert abs(got[sname] - ref[sname]) <= TOL * max(1.0, abs(ref[sname])), sname
+
+
+@pytest.mark.parametrize("phase", [0, 1])
+def test_nose_hoover_parallel_chain(dev, oracle, phase):
+    """6000 particles: the chain recurrence is evaluated by the parallel
+    (Newton-scan) kernel; it must reproduce the oracle's serial walk."""
+    say(f"nose-hoover parallel chain n=6000 phase={phase}")
+    got, ref = _integrate_case(dev, oracle, NHT, phase, n=6000)
+    for f in ("rx", "ry", "rz", "vx", "vy", "vz"):
+        assert rel_err(got[f], ref[f]) < TOL, f
+    for sname in ("peta0", "peta1", "peta2", "eta0", "eta1", "eta2") + (("kin_eng",) if phase == 1 else ()):
+        assert abs(got[sname] - ref[sname]) <= TOL * max(1.0, abs(ref[sname])), sname
 
 
 @pytest.mark.parametrize("kind", [VVC, LGVC])
