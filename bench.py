@@ -100,6 +100,83 @@ def cpu_baseline(state_arrays, box, params, seconds):
             "seconds": el}
 
 
+def gb_cpu_baseline(state, box, params, nsteps):
+    """The reference's own NHT + Gay-Berne loop on the GB state, in a private
+    copy of the reference library (the reference hard-codes GB(mu=1, nu=2);
+    the cost per pair is the same)."""
+    from oracle.oracle import FIELDS, GB, NHT, Reference, State, reference_available
+    if not reference_available(fast=True):
+        return None
+    n = params["n"]
+    st = State(n, [box, box, box])
+    for k, f in enumerate(FIELDS):
+        st.buf[:, k] = state[f.upper()]
+    st["target_temp"] = params["T"]
+    st["dof"] = 6 * n - 3
+    st["density"] = n / box ** 3
+    R = Reference(fast=True, private=True)
+    R.load(st)
+    R.nlist_init(params["skin"], params["cutoff"])
+    R.run_steps(NHT, GB, [params["cutoff"]], 1, 1, params["dt"], 1.0)
+    t0 = time.time()
+    R.run_steps(NHT, GB, [params["cutoff"]], nsteps, 2, params["dt"], 1.0)
+    el = time.time() - t0
+    return {"value": n * nsteps / el, "unit": "particle-steps/s", "cores": 1, "kind": "reference",
+            "sample": f"{nsteps} steps of the reference loop (nht + qq steps, gb pair loop over its Verlet list) on "
+                      f"the same {n}-particle state after one untimed step", "seconds": el}
+
+
+def other_workloads(sx, args):
+    """The Gay-Berne + Nose-Hoover and the charged-chain configurations, each
+    prepared on the device and timed over whole graph-replayed steps."""
+    from sucrose_amd.workloads import chains_state, gb_nhc_state
+    out = {}
+    d = sx.System(0)
+    p = gb_nhc_state(d, log=log)
+    gb = [(sx.GB, p["cutoff"], p["mu"], p["nu"])]
+    step = p["next_step"]
+    d.run(sx.INT_NHT, gb, 100, p["dt"], first_step=step)
+    d.synchronize()
+    rb0 = d.rebuild_count()
+    t0 = time.perf_counter()
+    d.run(sx.INT_NHT, gb, args.gb_steps, p["dt"], first_step=step + 100)
+    d.synchronize()
+    el = time.perf_counter() - t0
+    rec = {"value": p["n"] * args.gb_steps / el, "unit": "particle-steps/s", "steps": args.gb_steps,
+           "ms_per_step": el / args.gb_steps * 1e3, "rebuilds": d.rebuild_count() - rb0,
+           "T": d.get_scalar("TEMP"), "pe_per_particle": d.get_scalar("POT_ENG") / p["n"],
+           "config": {"workload": "gb_nhc_8000", "n": p["n"], "rho": p["rho"], "gb": "GB(3,5,2,1)",
+                      "sig": [1, 1, 3], "eps": [5, 5, 1], "T": p["T"], "dt": p["dt"], "cutoff": p["cutoff"],
+                      "skin": p["skin"], "integrator": "nht (chain 3, tau 2) + quaternion", "dtype": "f64"}}
+    log(f"[gb] {args.gb_steps} steps in {el:.3f}s -> {rec['value']:.4e} particle-steps/s")
+    if not args.no_cpu:
+        cpu = gb_cpu_baseline(d.state(), p["box"], p, 10)
+        if cpu:
+            rec["cpu_baseline"] = cpu
+            log(f"[gb cpu] reference {cpu['value']:.4e} particle-steps/s ({cpu['seconds']:.1f}s)")
+    out["gb_nhc_8000"] = rec
+    d.close()
+    d = sx.System(0)
+    p = chains_state(d, log=log)
+    rb0 = d.rebuild_count()
+    t0 = time.perf_counter()
+    d.run(sx.INT_LGVC, p["forces"], args.chain_steps, p["dt"], damping=p["gamma"], first_step=p["next_step"],
+          bonded=True)
+    d.synchronize()
+    el = time.perf_counter() - t0
+    rec = {"value": p["n"] * args.chain_steps / el, "unit": "particle-steps/s", "steps": args.chain_steps,
+           "ms_per_step": el / args.chain_steps * 1e3, "rebuilds": d.rebuild_count() - rb0,
+           "T": d.get_scalar("TEMP"), "pe_per_particle": d.get_scalar("POT_ENG") / p["n"],
+           "config": {"workload": "chains_100000", "chains": p["nchains"], "beads": p["nbeads"], "rho": p["rho"],
+                      "T": p["T"], "dt": p["dt"], "forces": "wca(1,1) + debye_huckel(rc 5, kappa 1, eps_r 1)",
+                      "bonded": "cosine_bend k=20 about straight + dih k(1+cos) k=1", "constraints": "rattle",
+                      "integrator": "lgvc", "dtype": "f64"}}
+    log(f"[chains] {args.chain_steps} steps in {el:.3f}s -> {rec['value']:.4e} particle-steps/s")
+    out["chains_100000"] = rec
+    d.close()
+    return out
+
+
 TRAIN_SCHEDULE = [(3e-3, 10000), (1.5e-3, 10000), (7e-4, 10000), (3e-4, 6000), (1e-4, 4000)]
 
 
@@ -196,6 +273,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n", type=int, default=32768)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain contact solver section")
+    ap.add_argument("--no-others", action="store_true", help="skip the Gay-Berne and chain workloads")
+    ap.add_argument("--gb-steps", type=int, default=2000)
+    ap.add_argument("--chain-steps", type=int, default=1000)
     ap.add_argument("--learned-steps", type=int, default=1000,
                     help="steps of the whole loop timed for each contact solver")
     ap.add_argument("--save-state", default=None, help="write the prepared workload state to this .npz")
@@ -274,6 +354,8 @@ def main():
     if learned:
         step = learned.pop("_next_step")
 
+    others = None if args.no_others else other_workloads(sx, args)
+
     cpu = None
     if not args.no_cpu:
         dev.initialize_forces()
@@ -294,6 +376,7 @@ def main():
                    "parallelism": "single_gpu"},
         "roofline": roof,
         "learned_vs_plain": learned,
+        "other_workloads": others,
         "cpu_baseline": cpu,
     }
     if cpu:

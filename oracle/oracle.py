@@ -377,10 +377,19 @@ class Reference:
     The library holds one system in module globals, so each instance drives
     that single global system."""
 
-    def __init__(self, fast=False):
+    def __init__(self, fast=False, private=False):
         path = reference_path(fast)
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path} not built (make -C oracle ref)")
+        if private:
+            # a private copy of the library gets its own module globals (the
+            # reference's neighbour list can be initialised once per copy)
+            import shutil
+            import tempfile
+            fd, copy = tempfile.mkstemp(suffix=".so")
+            os.close(fd)
+            shutil.copyfile(path, copy)
+            path = copy
         self.lib = C.CDLL(path)
         L = self.lib
         L.ref_brent.restype = C.c_double
