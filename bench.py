@@ -174,7 +174,95 @@ def other_workloads(sx, args):
     log(f"[chains] {args.chain_steps} steps in {el:.3f}s -> {rec['value']:.4e} particle-steps/s")
     out["chains_100000"] = rec
     d.close()
+    if args.batch_steps > 0:
+        out["batched_1024"] = batched_workload(sx, args)
     return out
+
+
+def batched_workload(sx, args):
+    """1024 independent LJ / Gay-Berne systems advanced together: one launch
+    per step, graph-replayed, launch shape picked by the autotuner."""
+    import numpy as np
+    from sucrose_amd.workloads import batched_state
+    d = sx.System(0)
+    p = batched_state(d, log=log)
+    step = p["next_step"]
+    choice, ms = d.batch_autotune(p["dt"], damping=p["gamma"], first_step=step, nrep=5)
+    shape = sx.System.BATCH_SHAPES[choice]
+    log(f"[batch] autotune: " + " ".join(f"{s[0]}x{s[1]}:{m:.3f}ms" for s, m in zip(sx.System.BATCH_SHAPES, ms))
+        + f" -> workgroup {shape[0]}, {shape[1]} lanes/particle")
+    d.batch_run(50, p["dt"], damping=p["gamma"], first_step=step)
+    step += 50
+    state = d.state() if not args.no_cpu else None
+    d.synchronize()
+    t0 = time.perf_counter()
+    d.batch_run(args.batch_steps, p["dt"], damping=p["gamma"], first_step=step)
+    d.synchronize()
+    el = time.perf_counter() - t0
+    th = d.batch_thermo()
+    temps = np.asarray(p["temps"])
+    kinds = np.asarray(p["kinds"])
+    lj = kinds == sx.LJ
+    rec = {"value": p["n"] * args.batch_steps / el, "unit": "particle-steps/s", "steps": args.batch_steps,
+           "ms_per_step": el / args.batch_steps * 1e3, "launch_shape": {"workgroup": shape[0],
+                                                                       "lanes_per_particle": shape[1]},
+           "autotune_ms": [round(float(m), 4) for m in ms],
+           "T_over_T0": {"lj": float(np.mean(th[lj, 3] / temps[lj])), "gb": float(np.mean(th[~lj, 3] / temps[~lj]))},
+           "config": {"workload": "batched_1024", "systems": p["nsys"], "particles": p["n"],
+                      "n_per_system": "U[64,512]", "lj": "rho U[0.3,0.8], T U[0.7,1.5], rc min(2.5, L/2)",
+                      "gb": "GB(3,5,2,1), rho U[0.08,0.15], T U[2.5,3.5], cutoff min(4, L/2-2)",
+                      "integrator": "lgv (gamma 1)", "dt": p["dt"], "pairs": "all pairs I<J per system",
+                      "dtype": "f64"}}
+    log(f"[batch] {args.batch_steps} steps in {el:.3f}s -> {rec['value']:.4e} particle-steps/s")
+    if state is not None:
+        cpu = batch_cpu_baseline(state, p)
+        if cpu:
+            rec["cpu_baseline"] = cpu
+            log(f"[batch cpu] reference {cpu['value']:.4e} particle-steps/s ({cpu['seconds']:.1f}s)")
+    d.close()
+    return rec
+
+
+def batch_cpu_baseline(state, p, nsample=8, seconds_each=1.5):
+    """The reference's own Langevin loop (Verlet list, skin 0.3, its rebuild
+    rule) run system by system on evenly spaced systems of the batch, each in
+    a private copy of the reference library."""
+    from oracle.oracle import FIELDS, GB, LGV, LJ, Reference, State, reference_available
+    if not reference_available(fast=True):
+        return None
+    counts = p["counts"]
+    off = [0]
+    for c in counts:
+        off.append(off[-1] + c)
+    picks = [int(k) for k in range(0, p["nsys"], max(1, p["nsys"] // nsample))][:nsample]
+    tot_ps, tot_t = 0.0, 0.0
+    for s in picks:
+        n, L, kind, prm = counts[s], p["boxes"][s], p["kinds"][s], p["params"][s]
+        st = State(n, [L, L, L])
+        for k, f in enumerate(FIELDS):
+            st.buf[:, k] = state[f.upper()][off[s]:off[s] + n]
+        st["target_temp"] = p["temps"][s]
+        st["dof"] = 6 * n - 3
+        st["density"] = n / L ** 3
+        R = Reference(fast=True, private=True)
+        R.load(st)
+        if kind == LJ:
+            fk, fp, reach = LJ, prm[:3], prm[2]
+        else:
+            fk, fp, reach = GB, [prm[0]], prm[0] + 2.0
+        R.nlist_init(0.3, reach)
+        t0 = time.time()
+        R.run_steps(LGV, fk, fp, 1, 1, p["dt"], p["gamma"])
+        t1 = max(time.time() - t0, 1e-5)
+        k = int(max(1, min(2000, seconds_each / t1)))
+        t0 = time.time()
+        R.run_steps(LGV, fk, fp, k, 2, p["dt"], p["gamma"])
+        tot_t += time.time() - t0
+        tot_ps += n * k
+    return {"value": tot_ps / tot_t, "unit": "particle-steps/s", "cores": 1, "kind": "reference",
+            "sample": f"the reference loop (lgv, lj or gb pair loop over its Verlet list) on {len(picks)} evenly "
+                      f"spaced systems of the batch, one after another, ~{seconds_each}s each",
+            "seconds": tot_t}
 
 
 TRAIN_SCHEDULE = [(3e-3, 10000), (1.5e-3, 10000), (7e-4, 10000), (3e-4, 6000), (1e-4, 4000)]
@@ -276,6 +364,7 @@ def main():
     ap.add_argument("--no-others", action="store_true", help="skip the Gay-Berne and chain workloads")
     ap.add_argument("--gb-steps", type=int, default=2000)
     ap.add_argument("--chain-steps", type=int, default=1000)
+    ap.add_argument("--batch-steps", type=int, default=5000, help="steps of the batched replicas (0 skips them)")
     ap.add_argument("--learned-steps", type=int, default=1000,
                     help="steps of the whole loop timed for each contact solver")
     ap.add_argument("--save-state", default=None, help="write the prepared workload state to this .npz")

@@ -200,6 +200,33 @@ typedef struct sx_step_spec {
 int sx_run(sx_ctx *ctx, const sx_step_spec *spec, int nsteps);
 
 /* ------------------------------------------------------------------ */
+/* batched replicas: independent small systems, one workgroup per system */
+/* and one launch per step (the README loop of each system)            */
+/* ------------------------------------------------------------------ */
+/* The context's particles are split into nsys contiguous systems of counts[s]
+ * particles, each in its own cubic periodic box of edge box[s], with pair
+ * kind kind[s] (SX_LJ: params sigma, epsilon, cutoff; SX_GB: cutoff, mu, nu)
+ * from params[4 s ..], and Langevin temperature temp[s].  Each step runs
+ * lgv_initial_step (Philox counters: the particle's 1-based index within its
+ * system), initialize_forces, the pair loop over all pairs I < J of the
+ * system, lgv_final_step and the system's thermo.  Interaction ranges must not
+ * exceed half the box (SX_ERR_CUTOFF). */
+int sx_batch_set(sx_ctx *ctx, int nsys, const int *counts, const double *box, const int *kind, const double *params,
+                 const double *temp);
+int sx_batch_run(sx_ctx *ctx, int nsteps, double dt, double damping, int first_step);
+/* per system: KE, KErot, PE, T = 2 (KE + KErot) / (6 N - 3); out[4 nsys] */
+int sx_batch_thermo(sx_ctx *ctx, double *out);
+/* times nrep steps of each of the 8 launch shapes from the same state,
+ * restores it, keeps the fastest; ms[8] (may be NULL) gets ms per step.
+ * Shapes (workgroup, lanes per particle): 0 (64,1) 1 (128,1) 2 (128,2)
+ * 3 (256,2) 4 (256,4) 5 (256,8) 6 (256,64) 7 (512,64); 64 lanes means one
+ * wavefront per particle that queues the pairs in range and evaluates them
+ * 64 at a time.  Shape 0 adds each particle's pair terms in the README
+ * loop's order; the others in a fixed tree order. */
+int sx_batch_autotune(sx_ctx *ctx, double dt, double damping, int first_step, int nrep, int *choice, double ms[8]);
+int sx_batch_set_shape(sx_ctx *ctx, int choice);
+
+/* ------------------------------------------------------------------ */
 /* measurement and synthetic set-up (no counterpart in the reference:   */
 /* benchmark and test harness support)                                 */
 /* ------------------------------------------------------------------ */
@@ -217,6 +244,13 @@ int sx_gen_ellipsoids_rsa(int n, double a, double b, double c, double L, uint32_
 /* number of overlapping pairs (Perram-Wertheim contact function < 1) */
 long sx_count_ellipsoid_overlaps(int n, double a, double b, double c, double L, const double *pos,
                                  const double *quat);
+/* n uniformly distributed unit quaternions (w x y z), Philox stream of seed */
+void sx_gen_uniform_quaternions(int n, uint32_t seed, double *quat);
+/* self-avoiding random-walk chains (bond length `bond`, no two beads closer
+ * than min_dist) in a periodic cube of edge L; pos[3 nchains nbeads];
+ * returns the number of complete chains */
+int sx_gen_chains(int nchains, int nbeads, double L, double bond, double min_dist, uint32_t seed,
+                  long max_attempts, double *pos);
 
 /* ------------------------------------------------------------------ */
 /* learned lambda* warm start for the ECP contact solve (no counterpart  */

@@ -124,6 +124,11 @@ def lib():
             "sx_mlp_samples": [vp, C.c_long, C.c_long, PF, PF], "sx_mlp_clear_samples": [vp],
             "sx_mlp_train": [vp, C.c_int, C.c_int, C.c_double, PF, C.c_uint32, PF],
             "sx_ecp_compare_solvers": [vp, C.c_int, _PD, C.c_int, C.c_int, _PD],
+            "sx_batch_set": [vp, C.c_int, _PI, _PD, _PI, _PD, _PD],
+            "sx_batch_run": [vp, C.c_int, C.c_double, C.c_double, C.c_int],
+            "sx_batch_thermo": [vp, _PD],
+            "sx_batch_autotune": [vp, C.c_double, C.c_double, C.c_int, C.c_int, _PI, _PD],
+            "sx_batch_set_shape": [vp, C.c_int],
         }.items():
             getattr(L, name).argtypes = args
             getattr(L, name).restype = C.c_int
@@ -400,6 +405,46 @@ class System:
                 "wave_iters_warm", "lambda0_err_mean", "lambda0_err_max", "lambda0_err_rms",
                 "lambda_warm_vs_plain_max", "iters_brent_tight", "wave_iters_brent_tight", "proven_out_of_range"]
         return dict(zip(keys, o.tolist()))
+
+    # ---- batched replicas ----
+    BATCH_SHAPES = ((64, 1), (128, 1), (128, 2), (256, 2), (256, 4), (256, 8), (256, 64), (512, 64))
+
+    def batch_set(self, counts, boxes, kinds, params, temps):
+        """Splits the particles into len(counts) independent systems (contiguous
+        ranges); params is (nsys, <=4) per system (LJ: sigma, epsilon, cutoff;
+        GB: cutoff, mu, nu)."""
+        cnt = np.ascontiguousarray(counts, dtype=np.int32)
+        ns = len(cnt)
+        bx = np.ascontiguousarray(boxes, dtype=np.float64).reshape(ns)
+        kd = np.ascontiguousarray(kinds, dtype=np.int32).reshape(ns)
+        pr = np.zeros((ns, 4))
+        pa = np.asarray(params, dtype=np.float64).reshape(ns, -1)
+        pr[:, :pa.shape[1]] = pa
+        tp = np.ascontiguousarray(temps, dtype=np.float64).reshape(ns)
+        self._ok(self.L.sx_batch_set(self.h, ns, cnt.ctypes.data_as(_PI), _dp(bx), kd.ctypes.data_as(_PI),
+                                     _dp(pr), _dp(tp)))
+        self._batch_n = ns
+
+    def batch_run(self, nsteps, dt, damping=1.0, first_step=1):
+        self._ok(self.L.sx_batch_run(self.h, int(nsteps), float(dt), float(damping), int(first_step)))
+
+    def batch_thermo(self):
+        """(nsys, 4): KE, KErot, PE, T of each system after the last step."""
+        out = np.zeros(4 * self._batch_n)
+        self._ok(self.L.sx_batch_thermo(self.h, _dp(out)))
+        return out.reshape(-1, 4)
+
+    def batch_autotune(self, dt, damping=1.0, first_step=1, nrep=5):
+        """Times every launch shape from the current state (restored after);
+        returns (chosen index, ms per step of each shape)."""
+        ch = C.c_int(0)
+        ms = np.zeros(len(self.BATCH_SHAPES))
+        self._ok(self.L.sx_batch_autotune(self.h, float(dt), float(damping), int(first_step), int(nrep),
+                                          C.byref(ch), _dp(ms)))
+        return ch.value, ms
+
+    def batch_set_shape(self, choice):
+        self._ok(self.L.sx_batch_set_shape(self.h, int(choice)))
 
     def set_bonded(self, kind, idx1, coeff, angle=0.0):
         a = np.ascontiguousarray(idx1, dtype=np.int32).reshape(-1)

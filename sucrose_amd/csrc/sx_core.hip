@@ -367,6 +367,7 @@ int sx_destroy(sx_ctx *c)
         if (b.out) hipFree(b.out);
     }
     sx_ecp_mlp_free(c);
+    sx_batch_free(c);
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     if (c->graph) hipGraphDestroy(c->graph);
     if (c->stream) hipStreamDestroy(c->stream);

@@ -137,6 +137,8 @@ struct sx_ctx {
     long ecp_stats[4] = {0, 0, 0, 0};
     // MLP (learned warm start)
     void *mlp = nullptr;
+    // batched replicas (sx_batch_set)
+    void *batch = nullptr;
     // graph
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -202,6 +204,7 @@ void sx_init_forces_impl(sx_ctx *c);
 // pot_eng = sum of the per-particle energies accumulated since initialize_forces
 void sx_pe_total_impl(sx_ctx *c);
 void sx_ecp_mlp_free(sx_ctx *c);
+void sx_batch_free(sx_ctx *c);
 
 static inline int sx_blocks(long n, int bs) { return (int)((n + bs - 1) / bs); }
 

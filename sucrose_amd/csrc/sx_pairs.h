@@ -25,12 +25,10 @@ __device__ __forceinline__ void pair_rij(const DState &s, int a, int b, double r
     }
 }
 
-// lj_calculate_forces (force.f90:6-49)
-__device__ __forceinline__ void d_lj(const DState &s, int a, int b, const double *p, PairOut &o)
+// lj_calculate_forces (force.f90:6-49) on a minimum-image separation r = R(a) - R(b)
+__device__ __forceinline__ void lj_core(const double r[3], const double *p, PairOut &o)
 {
     double sigma = p[0], epsilon = p[1], cutoff = p[2];
-    double r[3];
-    pair_rij(s, a, b, r);
     double rij_sq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
     double rcut_sq = cutoff * cutoff;
     bool cmask = rij_sq < rcut_sq;
@@ -41,6 +39,13 @@ __device__ __forceinline__ void d_lj(const DState &s, int a, int b, const double
     double coeff = 24.0 * epsilon * (2.0 * sr12 - sr6);
     o.pe = cmask ? (pot - pot_cut) : 0.0;
     for (int k = 0; k < 3; ++k) o.f[k] = cmask ? (coeff * r[k] / rij_sq) : 0.0;
+}
+
+__device__ __forceinline__ void d_lj(const DState &s, int a, int b, const double *p, PairOut &o)
+{
+    double r[3];
+    pair_rij(s, a, b, r);
+    lj_core(r, p, o);
 }
 
 // lj12_calculate_forces (force.f90:51-93)
@@ -209,40 +214,53 @@ __device__ __forceinline__ void cross3(const double a[3], const double b[3], dou
     o[2] = a[0] * b[1] - a[1] * b[0];
 }
 
-// gb (variant 0, cutoff + switch) and gbr (variant 1); p = cutoff, mu, nu,
-// chi' of particle a's energies is recomputed per pair as in the reference
-template <int VARIANT>
-__device__ __forceinline__ void d_gb(const DState &s, int a, int b, const double *p, PairOut &o)
+// The per-particle factors of gb_ that depend on particle I alone: its axis
+// u = R(q) e_z, chi from its sig, chi' from its eps (eps^(1/mu) for mu != 1)
+// and the sig_x / eps_z that scale sigma and epsilon.
+struct GbSelf {
+    double u[3];
+    double chi, xhi, lprp, epsz;
+};
+
+__device__ __forceinline__ void gb_self(const DState &s, int a, int mu, GbSelf &g)
 {
-    const double sigma_0 = 1.0;
-    double cutoff = p[0];
-    int mu = (int)p[1], nu = (int)p[2];
-    GbGeo q;
-    o.pe = 0.0;
-    for (int k = 0; k < 3; ++k) { o.f[k] = 0.0; o.ti[k] = 0.0; o.tj[k] = 0.0; }
-    pair_rij(s, a, b, q.r);
-    q.rsq = q.r[0] * q.r[0] + q.r[1] * q.r[1] + q.r[2] * q.r[2];
-    double rij_mag = sqrt(q.rsq);
-    double rhat[3] = {q.r[0] / rij_mag, q.r[1] / rij_mag, q.r[2] / rij_mag};
-    quat_u(s, a, q.u1);
-    quat_u(s, b, q.u2);
+    quat_u(s, a, g.u);
     double lpar = s.f[SX_SIG_Z][a], lprp = s.f[SX_SIG_X][a];
     double epar = s.f[SX_EPS_Z][a], eprp = s.f[SX_EPS_X][a];
-    q.ru1 = q.r[0] * q.u1[0] + q.r[1] * q.u1[1] + q.r[2] * q.u1[2];
-    q.ru2 = q.r[0] * q.u2[0] + q.r[1] * q.u2[1] + q.r[2] * q.u2[2];
-    q.uu = q.u1[0] * q.u2[0] + q.u1[1] * q.u2[1] + q.u1[2] * q.u2[2];
-    double chi = (lpar * lpar - lprp * lprp) / (lpar * lpar + lprp * lprp);
+    g.chi = (lpar * lpar - lprp * lprp) / (lpar * lpar + lprp * lprp);
     double ep = eprp, ea = epar;
     if (mu != 1) {
         double muinv = 1.0 / mu;
         ep = pow(eprp, muinv);
         ea = pow(epar, muinv);
     }
-    double xhi = (ep - ea) / (ep + ea);
+    g.xhi = (ep - ea) / (ep + ea);
+    g.lprp = lprp;
+    g.epsz = epar;
+}
+
+// gb (variant 0, cutoff + switch) and gbr (variant 1) for I = a with factors A,
+// J with axis u2, RIJ = r; cutoff, mu, nu as in p
+template <int VARIANT>
+__device__ __forceinline__ void gb_core(const double r[3], const GbSelf &A, const double u2[3], double cutoff, int mu,
+                                        int nu, PairOut &o)
+{
+    const double sigma_0 = 1.0;
+    GbGeo q;
+    o.pe = 0.0;
+    for (int k = 0; k < 3; ++k) { o.f[k] = 0.0; o.ti[k] = 0.0; o.tj[k] = 0.0; }
+    for (int k = 0; k < 3; ++k) { q.r[k] = r[k]; q.u1[k] = A.u[k]; q.u2[k] = u2[k]; }
+    q.rsq = q.r[0] * q.r[0] + q.r[1] * q.r[1] + q.r[2] * q.r[2];
+    double rij_mag = sqrt(q.rsq);
+    double rhat[3] = {q.r[0] / rij_mag, q.r[1] / rij_mag, q.r[2] / rij_mag};
+    q.ru1 = q.r[0] * q.u1[0] + q.r[1] * q.u1[1] + q.r[2] * q.u1[2];
+    q.ru2 = q.r[0] * q.u2[0] + q.r[1] * q.u2[1] + q.r[2] * q.u2[2];
+    q.uu = q.u1[0] * q.u2[0] + q.u1[1] * q.u2[1] + q.u1[2] * q.u2[2];
+    double chi = A.chi, xhi = A.xhi, lprp = A.lprp;
     double gchi = gb_g(chi, q);
     double gxhi = gb_g(xhi, q);
     double eps1 = 1.0 / sqrt(1 - (chi * q.uu) * (chi * q.uu));
-    double eps2 = gxhi / s.f[SX_EPS_Z][a];
+    double eps2 = gxhi / A.epsz;
     double sigma = lprp / sqrt(gchi);
     double sigma_0_inv = 1.0 / sigma_0;
     double g32 = sigma / gchi;
@@ -299,6 +317,20 @@ __device__ __forceinline__ void d_gb(const DState &s, int a, int b, const double
         pot = sx * pot;
     }
     o.pe = pot;
+}
+
+// p = cutoff, mu, nu; chi' of particle a's energies is recomputed per pair as
+// in the reference
+template <int VARIANT>
+__device__ __forceinline__ void d_gb(const DState &s, int a, int b, const double *p, PairOut &o)
+{
+    int mu = (int)p[1], nu = (int)p[2];
+    double r[3], u2[3];
+    GbSelf A;
+    pair_rij(s, a, b, r);
+    gb_self(s, a, mu, A);
+    quat_u(s, b, u2);
+    gb_core<VARIANT>(r, A, u2, p[0], mu, nu, o);
 }
 
 // isotropic dispatch

@@ -15,6 +15,20 @@ along each chain, RATTLE bonds, cosine bend k=20 about the straight chain,
 dihedral k(1 + cos phi) with k=1, WCA (sigma=1) and Debye-Hueckel (kappa=1,
 cutoff 5, dielectric 1) over one neighbour list.
 
+Batched replicas: nsys independent systems of N ~ U[64, 512] particles, even
+indices Lennard-Jones and odd ones Gay-Berne GB(3,5,2,1), each in its own
+cubic periodic box.  LJ systems: number density U[0.3, 0.8], T U[0.7, 1.5],
+cutoff min(2.5, L/2), a simple cubic lattice of ceil(N^(1/3))^3 sites with N
+of them occupied at random and a small jitter.  GB systems: T U[2.5, 3.5] and
+number density U[0.08, 0.15] (a 3:1 ellipsoid reaches its neighbours' images
+through the minimum image only if the interaction range, cutoff + 2, stays
+within L/2; at N = 64 and density 0.3 that leaves a cutoff below the
+potential's minimum), cutoff min(4, L/2 - 2), random sequential addition of
+hard (0.5, 0.5, 1.5) ellipsoids.  Velocities and angular momenta are drawn
+per system from the Maxwell-Boltzmann distribution at that system's
+temperature, then all systems are equilibrated together by the batched
+Langevin step itself.
+
 ECP 32k (the headline): identical prolate ellipsoids, semi-axes
 (0.5, 0.5, 1.5) (sig = 1, 1, 3), N = 32768, packing fraction 0.35, random
 positions and orientations with no overlaps.  Random sequential addition
@@ -32,7 +46,7 @@ import time
 
 import numpy as np
 
-from . import (DH, ECPR, GB, INT_LGV, INT_LGVC, INT_NHT, REBUILD_HALF_SKIN, WCA, System, count_ellipsoid_overlaps,
+from . import (DH, ECPR, GB, INT_LGV, LJ, INT_LGVC, INT_NHT, REBUILD_HALF_SKIN, WCA, System, count_ellipsoid_overlaps,
                gen_chains, gen_ellipsoids_rsa, uniform_quaternions)
 
 ELLIPSOID_SEMI_AXES = (0.5, 0.5, 1.5)
@@ -172,6 +186,108 @@ def chains_state(dev: System, nchains=2000, nbeads=50, rho=0.5, seed=7, T=1.0, d
         f"T={dev.get_scalar('TEMP'):.4f}, {time.time() - t0:.1f}s")
     return dict(n=n, box=L, rho=rho, T=T, dt=dt, gamma=gamma, skin=skin, next_step=1 + equil_steps, seed=seed,
                 forces=forces, nchains=nchains, nbeads=nbeads)
+
+
+def batched_layout(nsys=1024, nmin=64, nmax=512, seed=31):
+    """Host arrays of the batched workload: per-system counts, boxes, kinds,
+    parameters (LJ: sigma, epsilon, cutoff; GB: cutoff, mu, nu) and
+    temperatures, and per-particle positions (centred boxes), quaternions and
+    sig/eps axes in system order."""
+    rng = np.random.default_rng(seed)
+    counts = rng.integers(nmin, nmax + 1, nsys).astype(np.int32)
+    kinds = np.where(np.arange(nsys) % 2 == 0, LJ, GB).astype(np.int32)
+    boxes = np.zeros(nsys)
+    params = np.zeros((nsys, 4))
+    temps = np.zeros(nsys)
+    pos, quat, sig, eps = [], [], [], []
+    for s in range(nsys):
+        n = int(counts[s])
+        if kinds[s] == LJ:
+            rho = rng.uniform(0.3, 0.8)
+            L = (n / rho) ** (1.0 / 3.0)
+            m = int(math.ceil(n ** (1.0 / 3.0) - 1e-9))
+            a = L / m
+            g = (np.arange(m) + 0.5) * a - L / 2
+            sites = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+            p = sites[np.sort(rng.permutation(len(sites))[:n])] + rng.uniform(-0.05, 0.05, (n, 3)) * a
+            q = np.zeros((n, 4)); q[:, 0] = 1.0
+            params[s, :3] = (1.0, 1.0, min(2.5, 0.5 * L - 1e-9))
+            temps[s] = rng.uniform(0.7, 1.5)
+            sig.append(np.ones((n, 3))); eps.append(np.ones((n, 3)))
+        else:
+            rho = rng.uniform(0.08, 0.15)
+            L = (n / rho) ** (1.0 / 3.0)
+            p, q = gen_ellipsoids_rsa(n, ELLIPSOID_SEMI_AXES, L, int(rng.integers(1, 2 ** 31)))
+            if len(p) != n:
+                raise RuntimeError(f"placed {len(p)} of {n} ellipsoids")
+            # interaction range cutoff + sig_z - sigma_0 = cutoff + 2 within L/2 (with a rounding margin)
+            params[s, :3] = (min(4.0, 0.5 * L - 2.0 - 1e-9), 2, 1)
+            temps[s] = rng.uniform(2.5, 3.5)
+            sig.append(np.tile(GB_SIG, (n, 1)).astype(np.float64)); eps.append(np.tile(GB_EPS, (n, 1)).astype(float))
+        boxes[s] = L
+        pos.append(p); quat.append(q)
+    return dict(counts=counts, boxes=boxes, kinds=kinds, params=params, temps=temps, pos=np.concatenate(pos),
+                quat=np.concatenate(quat), sig=np.concatenate(sig), eps=np.concatenate(eps))
+
+
+def _rot(q):
+    """R(q) of each unit quaternion (w x y z): space = R body."""
+    w, x, y, z = q.T
+    return np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+                     np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+                     np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], -2)
+
+
+def batched_state(dev: System, nsys=1024, nmin=64, nmax=512, seed=31, dt=0.0015, gamma=1.0, relax_steps=100,
+                  equil_steps=400, log=print):
+    """Builds the batched-replica workload on dev and equilibrates it with the
+    batched Langevin step; returns its parameters."""
+    t0 = time.time()
+    lay = batched_layout(nsys, nmin, nmax, seed)
+    counts = lay["counts"]
+    n = int(counts.sum())
+    dev.set_num_atoms(n)
+    dev.set_box(float(lay["boxes"].max()))
+    for k, f in enumerate(("RX", "RY", "RZ")):
+        dev.upload(f, lay["pos"][:, k])
+    for k, f in enumerate(("QW", "QX", "QY", "QZ")):
+        dev.upload(f, lay["quat"][:, k])
+    for k, ax in enumerate("XYZ"):
+        dev.upload("SIG_" + ax, lay["sig"][:, k])
+        dev.upload("EPS_" + ax, lay["eps"][:, k])
+    dev.set_target_temperature(1.0)
+    dev.initialize_system()
+    rng = np.random.default_rng(seed + 1)
+    T = np.repeat(lay["temps"], counts)
+    mass = dev.download("MASS")
+    v = rng.normal(size=(n, 3)) * np.sqrt(T / mass)[:, None]
+    sys_id = np.repeat(np.arange(nsys), counts)
+    for k in range(3):   # no centre-of-mass drift within a system
+        mom = np.bincount(sys_id, weights=mass * v[:, k], minlength=nsys)
+        v[:, k] -= (mom / np.bincount(sys_id, weights=mass, minlength=nsys))[sys_id]
+    inertia = np.stack([dev.download(f) for f in ("IXX", "IYY", "IZZ")], 1)
+    lb = rng.normal(size=(n, 3)) * np.sqrt(T[:, None] * inertia)
+    ls = np.einsum("nij,nj->ni", _rot(lay["quat"]), lb)
+    for k, f in enumerate(("VX", "VY", "VZ")):
+        dev.upload(f, v[:, k])
+    for k, f in enumerate(("LX", "LY", "LZ")):
+        dev.upload(f, ls[:, k])
+    dev.batch_set(counts, lay["boxes"], lay["kinds"], lay["params"], lay["temps"])
+    step = 1
+    dev.batch_run(relax_steps, dt / 5, damping=gamma, first_step=step)
+    step += relax_steps
+    dev.batch_run(equil_steps, dt, damping=gamma, first_step=step)
+    step += equil_steps
+    th = dev.batch_thermo()
+    if not np.all(np.isfinite(th)):
+        raise RuntimeError("batched workload preparation diverged")
+    lj = lay["kinds"] == LJ
+    log(f"[workload] batch {nsys} systems, {n} particles: equilibrated {relax_steps}+{equil_steps} steps, "
+        f"T/T0 LJ {np.mean(th[lj, 3] / lay['temps'][lj]):.3f} GB {np.mean(th[~lj, 3] / lay['temps'][~lj]):.3f}, "
+        f"{time.time() - t0:.1f}s")
+    return dict(n=n, nsys=nsys, dt=dt, gamma=gamma, next_step=step, seed=seed, counts=counts.tolist(),
+                boxes=lay["boxes"].tolist(), kinds=lay["kinds"].tolist(), params=lay["params"].tolist(),
+                temps=lay["temps"].tolist())
 
 
 def save_state(dev: System, params: dict, path):
