@@ -127,53 +127,56 @@ def gb_cpu_baseline(state, box, params, nsteps):
 
 
 def other_workloads(sx, args):
-    """The Gay-Berne + Nose-Hoover and the charged-chain configurations, each
-    prepared on the device and timed over whole graph-replayed steps."""
+    """The Gay-Berne + Nose-Hoover, the charged-chain and the batched-replica
+    configurations, each prepared on the device and timed over whole
+    graph-replayed steps (a section with 0 steps is skipped)."""
     from sucrose_amd.workloads import chains_state, gb_nhc_state
     out = {}
-    d = sx.System(0)
-    p = gb_nhc_state(d, log=log)
-    gb = [(sx.GB, p["cutoff"], p["mu"], p["nu"])]
-    step = p["next_step"]
-    d.run(sx.INT_NHT, gb, 100, p["dt"], first_step=step)
-    d.synchronize()
-    rb0 = d.rebuild_count()
-    t0 = time.perf_counter()
-    d.run(sx.INT_NHT, gb, args.gb_steps, p["dt"], first_step=step + 100)
-    d.synchronize()
-    el = time.perf_counter() - t0
-    rec = {"value": p["n"] * args.gb_steps / el, "unit": "particle-steps/s", "steps": args.gb_steps,
-           "ms_per_step": el / args.gb_steps * 1e3, "rebuilds": d.rebuild_count() - rb0,
-           "T": d.get_scalar("TEMP"), "pe_per_particle": d.get_scalar("POT_ENG") / p["n"],
-           "config": {"workload": "gb_nhc_8000", "n": p["n"], "rho": p["rho"], "gb": "GB(3,5,2,1)",
-                      "sig": [1, 1, 3], "eps": [5, 5, 1], "T": p["T"], "dt": p["dt"], "cutoff": p["cutoff"],
-                      "skin": p["skin"], "integrator": "nht (chain 3, tau 2) + quaternion", "dtype": "f64"}}
-    log(f"[gb] {args.gb_steps} steps in {el:.3f}s -> {rec['value']:.4e} particle-steps/s")
-    if not args.no_cpu:
-        cpu = gb_cpu_baseline(d.state(), p["box"], p, 10)
-        if cpu:
-            rec["cpu_baseline"] = cpu
-            log(f"[gb cpu] reference {cpu['value']:.4e} particle-steps/s ({cpu['seconds']:.1f}s)")
-    out["gb_nhc_8000"] = rec
-    d.close()
-    d = sx.System(0)
-    p = chains_state(d, log=log)
-    rb0 = d.rebuild_count()
-    t0 = time.perf_counter()
-    d.run(sx.INT_LGVC, p["forces"], args.chain_steps, p["dt"], damping=p["gamma"], first_step=p["next_step"],
-          bonded=True)
-    d.synchronize()
-    el = time.perf_counter() - t0
-    rec = {"value": p["n"] * args.chain_steps / el, "unit": "particle-steps/s", "steps": args.chain_steps,
-           "ms_per_step": el / args.chain_steps * 1e3, "rebuilds": d.rebuild_count() - rb0,
-           "T": d.get_scalar("TEMP"), "pe_per_particle": d.get_scalar("POT_ENG") / p["n"],
-           "config": {"workload": "chains_100000", "chains": p["nchains"], "beads": p["nbeads"], "rho": p["rho"],
-                      "T": p["T"], "dt": p["dt"], "forces": "wca(1,1) + debye_huckel(rc 5, kappa 1, eps_r 1)",
-                      "bonded": "cosine_bend k=20 about straight + dih k(1+cos) k=1", "constraints": "rattle",
-                      "integrator": "lgvc", "dtype": "f64"}}
-    log(f"[chains] {args.chain_steps} steps in {el:.3f}s -> {rec['value']:.4e} particle-steps/s")
-    out["chains_100000"] = rec
-    d.close()
+    if args.gb_steps > 0:
+        d = sx.System(0)
+        p = gb_nhc_state(d, log=log)
+        gb = [(sx.GB, p["cutoff"], p["mu"], p["nu"])]
+        step = p["next_step"]
+        d.run(sx.INT_NHT, gb, 100, p["dt"], first_step=step)
+        d.synchronize()
+        rb0 = d.rebuild_count()
+        t0 = time.perf_counter()
+        d.run(sx.INT_NHT, gb, args.gb_steps, p["dt"], first_step=step + 100)
+        d.synchronize()
+        el = time.perf_counter() - t0
+        rec = {"value": p["n"] * args.gb_steps / el, "unit": "particle-steps/s", "steps": args.gb_steps,
+               "ms_per_step": el / args.gb_steps * 1e3, "rebuilds": d.rebuild_count() - rb0,
+               "T": d.get_scalar("TEMP"), "pe_per_particle": d.get_scalar("POT_ENG") / p["n"],
+               "config": {"workload": "gb_nhc_8000", "n": p["n"], "rho": p["rho"], "gb": "GB(3,5,2,1)",
+                          "sig": [1, 1, 3], "eps": [5, 5, 1], "T": p["T"], "dt": p["dt"], "cutoff": p["cutoff"],
+                          "skin": p["skin"], "integrator": "nht (chain 3, tau 2) + quaternion", "dtype": "f64"}}
+        log(f"[gb] {args.gb_steps} steps in {el:.3f}s -> {rec['value']:.4e} particle-steps/s")
+        if not args.no_cpu:
+            cpu = gb_cpu_baseline(d.state(), p["box"], p, 10)
+            if cpu:
+                rec["cpu_baseline"] = cpu
+                log(f"[gb cpu] reference {cpu['value']:.4e} particle-steps/s ({cpu['seconds']:.1f}s)")
+        out["gb_nhc_8000"] = rec
+        d.close()
+    if args.chain_steps > 0:
+        d = sx.System(0)
+        p = chains_state(d, log=log)
+        rb0 = d.rebuild_count()
+        t0 = time.perf_counter()
+        d.run(sx.INT_LGVC, p["forces"], args.chain_steps, p["dt"], damping=p["gamma"], first_step=p["next_step"],
+              bonded=True)
+        d.synchronize()
+        el = time.perf_counter() - t0
+        rec = {"value": p["n"] * args.chain_steps / el, "unit": "particle-steps/s", "steps": args.chain_steps,
+               "ms_per_step": el / args.chain_steps * 1e3, "rebuilds": d.rebuild_count() - rb0,
+               "T": d.get_scalar("TEMP"), "pe_per_particle": d.get_scalar("POT_ENG") / p["n"],
+               "config": {"workload": "chains_100000", "chains": p["nchains"], "beads": p["nbeads"], "rho": p["rho"],
+                          "T": p["T"], "dt": p["dt"], "forces": "wca(1,1) + debye_huckel(rc 5, kappa 1, eps_r 1)",
+                          "bonded": "cosine_bend k=20 about straight + dih k(1+cos) k=1", "constraints": "rattle",
+                          "integrator": "lgvc", "dtype": "f64"}}
+        log(f"[chains] {args.chain_steps} steps in {el:.3f}s -> {rec['value']:.4e} particle-steps/s")
+        out["chains_100000"] = rec
+        d.close()
     if args.batch_steps > 0:
         out["batched_1024"] = batched_workload(sx, args)
     return out
