@@ -60,11 +60,14 @@ struct NList {
     int *cell_of = nullptr, *cell_count = nullptr, *cell_start = nullptr, *cell_items = nullptr;
     int *row_len = nullptr, *row_start = nullptr, *lower = nullptr, *half_start = nullptr, *half_len = nullptr;
     int *nbr = nullptr, *pairk = nullptr;
+    int *fwdpos = nullptr, *revpos = nullptr;   // half pair k -> its entry in row i / in row j
     int *half_i = nullptr, *half_j = nullptr;
     double *x_last = nullptr, *y_last = nullptr, *z_last = nullptr;
     double *cell_pos = nullptr;  // positions in cell order (x y z pad)
     long cap_full = 0, cap_half = 0;
     int max_row = 0;
+    int lpr_rows = 16, lpr_half = 16;   // lanes per row of the row and half-index kernels
+    bool need_half = false;             // half list and pair index kept (ECP paths, export)
     long rebuilds = 0;
     // host copies of the sizes of the last build
     long nnz_full = 0, nnz_half = 0;
@@ -128,6 +131,10 @@ struct sx_ctx {
     unsigned char *d_live = nullptr;
     int *d_work = nullptr;        // compacted pair list
     long pair_cap = 0;
+    // per full-list entry: 1 + the half pair whose results the apply stage
+    // adds, or 0; set by the solve stage, cleared again by the apply stage
+    int *d_idx_full = nullptr;
+    long idx_cap = 0;
     // per-particle ECP precompute
     double *d_ecp_pp = nullptr;   // FP64 records, 24 doubles per particle
     float *d_ecp_scr = nullptr;   // FP32 screen records, 8 floats per particle
@@ -186,6 +193,7 @@ void sx_sync_scalars_to_device(sx_ctx *c);
 void sx_sync_scalars_to_host(sx_ctx *c);
 void sx_nlist_build_impl(sx_ctx *c, bool conditional);
 void sx_nlist_flag_impl(sx_ctx *c);
+void sx_nlist_want_half(sx_ctx *c);
 void sx_forces_impl(sx_ctx *c, int kind, const double *p, int np);
 void sx_bonded_impl(sx_ctx *c);
 void sx_lgv_initial_impl(sx_ctx *c, double dt, int step, const int *dstep, double damping, bool constrained);

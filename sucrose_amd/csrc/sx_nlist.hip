@@ -11,6 +11,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 
 #define SORT_CAP 2048
 
@@ -185,6 +187,115 @@ __global__ void k_rows_cells(DState s, CellGrid g, const int *cell_of, const int
     }
 }
 
+// The same rows with LPR lanes per particle.  For each (dx, dy) of the 3x3
+// column of neighbour cells, the three z cells are consecutive in the
+// cell-ordered item array (two ranges where z wraps), so a row is 9-18
+// contiguous candidate ranges; the group's lanes test LPR candidates at a
+// time and place hits with a group-masked ballot and prefix count.  Every loop
+// bound is uniform within a group, so the group's ballots stay in step.
+__device__ __forceinline__ unsigned long long group_mask(int LPR)
+{
+    const int l64 = threadIdx.x & 63;
+    return LPR == 64 ? ~0ull : (((1ull << LPR) - 1ull) << (l64 - l64 % LPR));
+}
+
+template <int PASS, int LPR>
+__global__ void __launch_bounds__(256) k_rows_cells_g(DState s, CellGrid g, const int *cell_of, const int *cell_start,
+                                                      const int *items, const double4 *cell_pos, double rsq_cut,
+                                                      int *row_len, int *lower, int *half_len, const int *row_start,
+                                                      int *nbr, const int *flag, long cap, int *err)
+{
+    if (flag && *flag == 0) return;
+    const int lane = threadIdx.x % LPR;
+    // rows are taken in cell order, so neighbouring groups read the same cells
+    const int slot = blockIdx.x * (256 / LPR) + threadIdx.x / LPR;
+    if (slot >= s.n) return;   // whole groups leave together
+    const int i = items[slot];
+    const unsigned long long gm = group_mask(LPR);
+    const unsigned long long below = gm & ((1ull << (threadIdx.x & 63)) - 1ull);
+    const int c = cell_of[i];
+    const double4 pi = cell_pos[slot];
+    const int nx = g.nc[0], ny = g.nc[1], nz = g.nc[2];
+    const int cz = c % nz, cy = (c / nz) % ny, cx = c / (nz * ny);
+    int cnt = 0, low = 0;
+    long base = 0;
+    if (PASS) {
+        base = row_start[i];
+        if ((long)row_start[i + 1] > cap) {
+            if (lane == 0) atomicCAS(err, 0, DERR_NLIST_CAP);
+            return;
+        }
+    }
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int x = (cx + dx + nx) % nx, y = (cy + dy + ny) % ny;
+            const int col = (x * ny + y) * nz;
+            // z cells cz-1..cz+1 as up to two consecutive runs
+            int r0[2], r1[2], nr = 0;
+            const int zlo = cz - 1 < 0 ? 0 : cz - 1, zhi = cz + 1 > nz - 1 ? nz - 1 : cz + 1;
+            r0[nr] = cell_start[col + zlo]; r1[nr] = cell_start[col + zhi + 1]; ++nr;
+            if (cz - 1 < 0) { r0[nr] = cell_start[col + nz - 1]; r1[nr] = cell_start[col + nz]; ++nr; }
+            else if (cz + 1 > nz - 1) { r0[nr] = cell_start[col]; r1[nr] = cell_start[col + 1]; ++nr; }
+            for (int rr = 0; rr < nr; ++rr)
+                for (int q0 = r0[rr]; q0 < r1[rr]; q0 += LPR) {
+                    const int q = q0 + lane;
+                    int j = -1;
+                    bool hit = false;
+                    if (q < r1[rr]) {
+                        j = items[q];
+                        hit = j != i && in_list_pos(pi, cell_pos[q], i, j, s, g, rsq_cut);
+                    }
+                    const unsigned long long m = __ballot(hit) & gm;
+                    if (PASS && hit) nbr[base + cnt + __popcll(m & below)] = j;
+                    cnt += __popcll(m);
+                    if (!PASS) low += __popcll(__ballot(hit && j < i) & gm);
+                }
+        }
+    if (!PASS && lane == 0) {
+        row_len[i] = cnt;
+        lower[i] = low;
+        half_len[i] = cnt - low;
+    }
+}
+
+// half-pair index with LPR lanes per row (entries of a row are independent)
+template <int LPR>
+__global__ void __launch_bounds__(256) k_half_index_g(int n, const int *row_start, const int *lower,
+                                                      const int *half_start, const int *nbr, int *pairk, int *half_i,
+                                                      int *half_j, int *fwdpos, int *revpos, const int *flag,
+                                                      long cap_half, int *err)
+{
+    if (flag && *flag == 0) return;
+    const int lane = threadIdx.x % LPR;
+    const int i = blockIdx.x * (256 / LPR) + threadIdx.x / LPR;
+    if (i >= n) return;
+    if ((long)half_start[n] > cap_half) {
+        if (lane == 0) atomicCAS(err, 0, DERR_NLIST_CAP);
+        return;
+    }
+    const int b = row_start[i], e = row_start[i + 1], lo = lower[i];
+    for (int p = b + lane; p < e; p += LPR) {
+        const int j = nbr[p];
+        if (j > i) {
+            const int k = half_start[i] + (p - b - lo);
+            pairk[p] = k;
+            half_i[k] = i;
+            half_j[k] = j;
+            fwdpos[k] = p;
+        } else {
+            // position of i in row j (sorted): binary search
+            int lb = row_start[j], ub = row_start[j + 1];
+            while (lb < ub) {
+                const int m = (lb + ub) >> 1;
+                if (nbr[m] < i) lb = m + 1; else ub = m;
+            }
+            const int k = half_start[j] + (lb - row_start[j] - lower[j]);
+            pairk[p] = k;
+            revpos[k] = p;
+        }
+    }
+}
+
 // brute force rows (small or non-periodic boxes)
 template <int PASS>
 __global__ void k_rows_brute(DState s, double rsq_cut, int *row_len, int *lower, int *half_len, const int *row_start,
@@ -214,42 +325,50 @@ __global__ void k_rows_brute(DState s, double rsq_cut, int *row_len, int *lower,
     }
 }
 
-// one wavefront per row: bitonic sort of the row in LDS (ascending j)
+// one wavefront per row: the row goes to LDS, each lane takes up to 8 of its
+// entries and ranks them against the whole row (the entries of a row are
+// distinct, so rank = number of smaller entries is the sorted position), then
+// writes them back in ascending j
+#define SORT_PER_LANE 8
 __global__ void __launch_bounds__(256) k_sort_rows(int n, const int *row_start, int *nbr, const int *flag, int *err)
 {
     if (flag && *flag == 0) return;
-    __shared__ int buf[4][SORT_CAP];
-    int w = threadIdx.x / SX_WAVE, lane = threadIdx.x % SX_WAVE;
+    __shared__ __align__(16) int buf[4][SORT_CAP];
+    const int w = threadIdx.x / SX_WAVE, lane = threadIdx.x % SX_WAVE;
     int *sh = buf[w];
     for (int i = blockIdx.x * 4 + w; i < n; i += gridDim.x * 4) {
-        int b = row_start[i], len = row_start[i + 1] - b;
+        const int b = row_start[i], len = row_start[i + 1] - b;
         if (len <= 1) continue;
         if (len > SORT_CAP) {
             if (lane == 0) atomicCAS(err, 0, DERR_ROW_CAP);
             continue;
         }
-        int P = 1;
-        while (P < len) P <<= 1;
-        for (int q = lane; q < P; q += SX_WAVE) sh[q] = q < len ? nbr[b + q] : 0x7fffffff;
+        const int len4 = (len + 3) & ~3;
+        for (int q = lane; q < len4; q += SX_WAVE) sh[q] = q < len ? nbr[b + q] : 0x7fffffff;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (int k = 2; k <= P; k <<= 1) {
-            for (int jj = k >> 1; jj > 0; jj >>= 1) {
-                for (int q = lane; q < P; q += SX_WAVE) {
-                    int p = q ^ jj;
-                    if (p > q) {
-                        int a = sh[q], c = sh[p];
-                        bool up = (q & k) == 0;
-                        if ((a > c) == up) { sh[q] = c; sh[p] = a; }
-                    }
-                }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int q0 = 0; q0 < len; q0 += SX_WAVE * SORT_PER_LANE) {
+            int mine[SORT_PER_LANE], rank[SORT_PER_LANE];
+#pragma unroll
+            for (int t = 0; t < SORT_PER_LANE; ++t) {
+                const int q = q0 + t * SX_WAVE + lane;
+                mine[t] = q < len ? sh[q] : 0x7fffffff;
+                rank[t] = 0;
+            }
+            const int4 *sh4 = (const int4 *)sh;
+            for (int m = 0; m < len4 / 4; ++m) {
+                const int4 v = sh4[m];   // the same address in every lane: a broadcast read
+#pragma unroll
+                for (int t = 0; t < SORT_PER_LANE; ++t)
+                    rank[t] += (v.x < mine[t]) + (v.y < mine[t]) + (v.z < mine[t]) + (v.w < mine[t]);
+            }
+#pragma unroll
+            for (int t = 0; t < SORT_PER_LANE; ++t) {
+                const int q = q0 + t * SX_WAVE + lane;
+                if (q < len) nbr[b + rank[t]] = mine[t];
             }
         }
-        for (int q = lane; q < len; q += SX_WAVE) nbr[b + q] = sh[q];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -344,17 +463,71 @@ __global__ void __launch_bounds__(256) k_disp_max(DState s, const double *xl, co
 }
 
 // ---------------------------------------------------------------------------
+static void launch_rows_g(sx_ctx *c, int pass, const CellGrid &g, const int *flag)
+{
+    NList &nl = c->nl;
+    const int L = nl.lpr_rows;
+    dim3 grid((c->n + 256 / L - 1) / (256 / L)), blk(256);
+#define SX_ROWS_G(P, LL)                                                                                            \
+    hipLaunchKernelGGL((k_rows_cells_g<P, LL>), grid, blk, 0, c->stream, c->dstate(), g, nl.cell_of, nl.cell_start, \
+                       nl.cell_items, (const double4 *)nl.cell_pos, nl.r_skin_sq, nl.row_len, nl.lower, nl.half_len,  \
+                       nl.row_start, nl.nbr, flag, nl.cap_full, c->d_err)
+    if (pass == 0) {
+        switch (L) {
+        case 4: SX_ROWS_G(0, 4); break;
+        case 8: SX_ROWS_G(0, 8); break;
+        case 16: SX_ROWS_G(0, 16); break;
+        case 32: SX_ROWS_G(0, 32); break;
+        default: SX_ROWS_G(0, 64); break;
+        }
+    } else {
+        switch (L) {
+        case 4: SX_ROWS_G(1, 4); break;
+        case 8: SX_ROWS_G(1, 8); break;
+        case 16: SX_ROWS_G(1, 16); break;
+        case 32: SX_ROWS_G(1, 32); break;
+        default: SX_ROWS_G(1, 64); break;
+        }
+    }
+#undef SX_ROWS_G
+}
+
+static void launch_half_index_g(sx_ctx *c, const int *flag)
+{
+    NList &nl = c->nl;
+    const int L = nl.lpr_half;
+    dim3 grid((c->n + 256 / L - 1) / (256 / L)), blk(256);
+#define SX_HALF_G(LL)                                                                                               \
+    hipLaunchKernelGGL((k_half_index_g<LL>), grid, blk, 0, c->stream, c->n, nl.row_start, nl.lower, nl.half_start, \
+                       nl.nbr, nl.pairk, nl.half_i, nl.half_j, nl.fwdpos, nl.revpos, flag, nl.cap_half, c->d_err)
+    switch (L) {
+    case 4: SX_HALF_G(4); break;
+    case 8: SX_HALF_G(8); break;
+    case 16: SX_HALF_G(16); break;
+    case 32: SX_HALF_G(32); break;
+    default: SX_HALF_G(64); break;
+    }
+#undef SX_HALF_G
+}
+
+static int pow2_clamp(double x, int lo, int hi)
+{
+    int p = lo;
+    while (p < hi && p < x) p <<= 1;
+    return p;
+}
+
 static void nl_free(NList &nl)
 {
     int *ip[] = {nl.cell_of, nl.cell_count, nl.cell_start, nl.cell_items, nl.row_len, nl.row_start, nl.lower,
-                 nl.half_start, nl.half_len, nl.nbr, nl.pairk, nl.half_i, nl.half_j};
+                 nl.half_start, nl.half_len, nl.nbr, nl.pairk, nl.half_i, nl.half_j, nl.fwdpos, nl.revpos};
     for (int *p : ip)
         if (p) hipFree(p);
     double *dp[] = {nl.x_last, nl.y_last, nl.z_last, nl.cell_pos};
     for (double *p : dp)
         if (p) hipFree(p);
     nl.cell_of = nl.cell_count = nl.cell_start = nl.cell_items = nl.row_len = nl.row_start = nl.lower = nullptr;
-    nl.half_start = nl.half_len = nl.nbr = nl.pairk = nl.half_i = nl.half_j = nullptr;
+    nl.half_start = nl.half_len = nl.nbr = nl.pairk = nl.half_i = nl.half_j = nl.fwdpos = nl.revpos = nullptr;
     nl.x_last = nl.y_last = nl.z_last = nl.cell_pos = nullptr;
 }
 
@@ -365,6 +538,8 @@ static void nl_alloc_lists(sx_ctx *c, long cap_full)
     if (nl.pairk) hipFree(nl.pairk);
     if (nl.half_i) hipFree(nl.half_i);
     if (nl.half_j) hipFree(nl.half_j);
+    if (nl.fwdpos) hipFree(nl.fwdpos);
+    if (nl.revpos) hipFree(nl.revpos);
     if (cap_full > 0x7fffffffL) throw sx_error{SX_ERR_CAPACITY, "neighbour list larger than 2^31 entries"};
     nl.cap_full = cap_full;
     nl.cap_half = cap_full / 2 + 1;
@@ -372,6 +547,8 @@ static void nl_alloc_lists(sx_ctx *c, long cap_full)
     HIPCHK(hipMalloc(&nl.pairk, sizeof(int) * nl.cap_full));
     HIPCHK(hipMalloc(&nl.half_i, sizeof(int) * nl.cap_half));
     HIPCHK(hipMalloc(&nl.half_j, sizeof(int) * nl.cap_half));
+    HIPCHK(hipMalloc(&nl.fwdpos, sizeof(int) * nl.cap_half));
+    HIPCHK(hipMalloc(&nl.revpos, sizeof(int) * nl.cap_half));
     c->have_graph = false;
 }
 
@@ -415,7 +592,43 @@ static void nl_setup(sx_ctx *c)
     long cap = std::max<long>((long)(2.0 * est) + 1024, 32L * n);
     if (!c->periodic) cap = std::max<long>(cap, std::min<long>((long)n * (n - 1), 64L * n));
     nl_alloc_lists(c, cap);
+    // lanes per row: about the candidates of three cells for the row builder,
+    // a quarter of the expected row for the half index
+    double row_est = est / std::max(1, n);
+    nl.lpr_rows = nl.use_cells ? pow2_clamp(3.0 * (double)n / nl.ncells, 4, 64) : 16;
+    nl.lpr_half = pow2_clamp(row_est / 4.0, 4, 64);
+    if (const char *e = getenv("SUCROSE_NLIST_LANES")) {   // rows,half override for tuning runs
+        int a = 0, b = 0;
+        if (sscanf(e, "%d,%d", &a, &b) == 2) {
+            if (a >= 1) nl.lpr_rows = pow2_clamp(a, 4, 64);
+            if (b >= 1) nl.lpr_half = pow2_clamp(b, 4, 64);
+        }
+    }
     nl.init = true;
+}
+
+// the reference's half list (half_start, half_i, half_j; d_flag[2] = its
+// length) and pairk, from the sorted rows
+static void half_index_launch(sx_ctx *c, const int *flag)
+{
+    NList &nl = c->nl;
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.half_len, nl.half_start, c->n, flag,
+                       c->d_flag + 2);
+    launch_half_index_g(c, flag);
+}
+
+void sx_nlist_want_half(sx_ctx *c)
+{
+    NList &nl = c->nl;
+    if (nl.need_half) return;
+    nl.need_half = true;
+    c->have_graph = false;
+    if (!nl.init) return;
+    half_index_launch(c, nullptr);
+    int nh = 0;
+    HIPCHK(hipMemcpyAsync(&nh, c->d_flag + 2, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    sx_check_device_error(c);
+    nl.nnz_half = nh;
 }
 
 // Builds the list.  conditional: every kernel reads the device rebuild flag
@@ -448,14 +661,10 @@ void sx_nlist_build_impl(sx_ctx *c, bool conditional)
             hipLaunchKernelGGL(k_zero_int, dim3(ncb), dim3(256), 0, c->stream, nl.cell_count, nl.ncells + 1, flag);
             hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(256), 0, c->stream, s, nl.cell_of, nl.cell_start,
                                nl.cell_count, nl.cell_items, (double4 *)nl.cell_pos, flag);
-            hipLaunchKernelGGL(k_rows_cells<0>, dim3(nb), dim3(256), 0, c->stream, s, g, nl.cell_of, nl.cell_start,
-                               nl.cell_items, (const double4 *)nl.cell_pos, nl.r_skin_sq, nl.row_len, nl.lower,
-                               nl.half_len, nl.row_start, nl.nbr, flag, nl.cap_full, c->d_err);
+            launch_rows_g(c, 0, g, flag);
             hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.row_len, nl.row_start, n, flag,
                                c->d_flag + 1);
-            hipLaunchKernelGGL(k_rows_cells<1>, dim3(nb), dim3(256), 0, c->stream, s, g, nl.cell_of, nl.cell_start,
-                               nl.cell_items, (const double4 *)nl.cell_pos, nl.r_skin_sq, nl.row_len, nl.lower,
-                               nl.half_len, nl.row_start, nl.nbr, flag, nl.cap_full, c->d_err);
+            launch_rows_g(c, 1, g, flag);
         } else {
             hipLaunchKernelGGL(k_rows_brute<0>, dim3(nb), dim3(256), 0, c->stream, s, nl.r_skin_sq, nl.row_len,
                                nl.lower, nl.half_len, nl.row_start, nl.nbr, flag, nl.cap_full, c->d_err);
@@ -478,9 +687,7 @@ void sx_nlist_build_impl(sx_ctx *c, bool conditional)
     }
     hipLaunchKernelGGL(k_sort_rows, dim3(std::min(sx_blocks(n, 4), 2048)), dim3(256), 0, c->stream, n, nl.row_start,
                        nl.nbr, flag, c->d_err);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.half_len, nl.half_start, n, flag, c->d_flag + 2);
-    hipLaunchKernelGGL(k_half_index, dim3(nb), dim3(256), 0, c->stream, n, nl.row_start, nl.lower, nl.half_start,
-                       nl.nbr, nl.pairk, nl.half_i, nl.half_j, flag, nl.cap_half, c->d_err);
+    if (nl.need_half) half_index_launch(c, flag);
     hipLaunchKernelGGL(k_save_last, dim3(nb), dim3(256), 0, c->stream, s, nl.x_last, nl.y_last, nl.z_last, flag,
                        conditional ? c->d_counts : (long *)nullptr);
     if (!conditional) {
@@ -576,6 +783,7 @@ int sx_nlist_export(sx_ctx *c, int *offset1, int *nlist1, long cap, long *nnz)
 {
     return guard(c, [&] {
         need_nlist(c);
+        sx_nlist_want_half(c);
         sx_check_device_error(c);
         int n = c->n;
         int sizes[2];

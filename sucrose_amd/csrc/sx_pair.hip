@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -245,12 +246,16 @@ __device__ __forceinline__ double ecp_solve_lambda(int solver, const double *pp,
 // solved pairs, solver iterations, fallbacks and, for the learned solver,
 // pairs whose warm start already proves them out of range (their live flag
 // is cleared so the apply stage skips them)
+// idx_full (may be NULL): the pair's two full-list entries (fwdpos, revpos)
+// get 1 + k, so the apply stage finds the solved pairs of a row without
+// reading the others
 template <int VARIANT>
 __global__ void __launch_bounds__(ECP_SOLVE_THREADS) k_ecp_solve(DState s, const double *pp, long ld, const int *half_i,
                                                                  const int *half_j, const int *work, const int *segcount,
                                                                  const int *nnz_half, EcpScreen sc, int solver,
                                                                  const float *lam0, double *res, unsigned char *live,
-                                                                 long *stats)
+                                                                 long *stats, const int *fwdpos, const int *revpos,
+                                                                 int *idx_full)
 {
     long start, end;
     ecp_segment(*nnz_half, blockIdx.x, start, end);
@@ -288,6 +293,10 @@ __global__ void __launch_bounds__(ECP_SOLVE_THREADS) k_ecp_solve(DState s, const
         r[1] = o.f[0]; r[2] = o.f[1]; r[3] = o.f[2];
         r[4] = o.ti[0]; r[5] = o.ti[1]; r[6] = o.ti[2];
         r[7] = o.tj[0]; r[8] = o.tj[1]; r[9] = o.tj[2];
+        if (idx_full) {
+            idx_full[fwdpos[k]] = k + 1;
+            idx_full[revpos[k]] = k + 1;
+        }
         my_iters += iters;
         my_solved += 1;
         my_fb += fb;
@@ -309,10 +318,12 @@ __global__ void __launch_bounds__(ECP_SOLVE_THREADS) k_ecp_solve(DState s, const
     }
 }
 
-// per-particle application of the per-pair results
+// per-particle application of the per-pair results: the row's entries marked
+// by the solve stage, in row order; each mark is cleared after use, so the
+// marks are all zero again when the stage ends
 template <int LPP>
-__global__ void __launch_bounds__(256) k_aniso_apply(DState s, const int *row_start, const int *nbr, const int *pairk,
-                                                     const unsigned char *live, const double *res, double *pe_acc)
+__global__ void __launch_bounds__(256) k_aniso_apply(DState s, const int *row_start, const int *nbr, int *idx_full,
+                                                     const double *res, double *pe_acc)
 {
     int gid = blockIdx.x * blockDim.x + threadIdx.x;
     int i = gid / LPP, l = gid % LPP;
@@ -320,8 +331,10 @@ __global__ void __launch_bounds__(256) k_aniso_apply(DState s, const int *row_st
     if (i < s.n) {
         int b = row_start[i], e = row_start[i + 1];
         for (int q = b + l; q < e; q += LPP) {
-            int k = pairk[q];
-            if (!live[k]) continue;
+            const int v = idx_full[q];
+            if (v == 0) continue;
+            idx_full[q] = 0;
+            const int k = v - 1;
             const double *r = res + (long)k * 10;
             if (nbr[q] > i) {
                 acc[0] = acc[0] + r[1]; acc[1] = acc[1] + r[2]; acc[2] = acc[2] + r[3];
@@ -468,6 +481,13 @@ static void ensure_pair_buffers(sx_ctx *c)
         c->pair_cap = need;
         c->have_graph = false;
     }
+    if (c->nl.cap_full > c->idx_cap) {
+        if (c->d_idx_full) HIPCHK(hipFree(c->d_idx_full));
+        HIPCHK(hipMalloc(&c->d_idx_full, sizeof(int) * c->nl.cap_full));
+        HIPCHK(hipMemsetAsync(c->d_idx_full, 0, sizeof(int) * c->nl.cap_full, c->stream));
+        c->idx_cap = c->nl.cap_full;
+        c->have_graph = false;
+    }
     ensure_pp(c);
 }
 
@@ -497,7 +517,15 @@ static EcpLaunch ecp_launch_shape(sx_ctx *c, bool stats)
 {
     EcpLaunch L;
     L.cap = c->nl.cap_half;
+    // lanes per particle of the apply stage: 4-8 row entries per lane (8 lanes
+    // at 36 entries per row measured fastest)
     L.lpp = 4;
+    const double row = c->n > 0 ? (double)c->nl.nnz_full / c->n : 0.0;
+    while (L.lpp < 16 && 8.0 * L.lpp < row) L.lpp <<= 1;
+    if (const char *e = getenv("SUCROSE_APPLY_LANES")) {   // tuning override: 4, 8 or 16
+        int v = atoi(e);
+        if (v == 4 || v == 8 || v == 16) L.lpp = v;
+    }
     L.nb = sx_blocks((long)c->n * L.lpp, 256);
     L.stats = stats;
     return L;
@@ -526,16 +554,23 @@ static void ecp_stage(sx_ctx *c, int stage, int kind, const EcpScreen &sc, const
         if (kind == SX_ECPR)
             hipLaunchKernelGGL(k_ecp_solve<1>, dim3(ECP_NSEG), dim3(ECP_SOLVE_THREADS), 0, c->stream, s, c->d_ecp_pp,
                                ld, nl.half_i, nl.half_j, c->d_work, c->d_segcount, nnz_half, sc, c->ecp_solver, lam0,
-                               c->d_pairbuf, c->d_live, stats);
+                               c->d_pairbuf, c->d_live, stats, nl.fwdpos, nl.revpos, c->d_idx_full);
         else
             hipLaunchKernelGGL(k_ecp_solve<0>, dim3(ECP_NSEG), dim3(ECP_SOLVE_THREADS), 0, c->stream, s, c->d_ecp_pp,
                                ld, nl.half_i, nl.half_j, c->d_work, c->d_segcount, nnz_half, sc, c->ecp_solver, lam0,
-                               c->d_pairbuf, c->d_live, stats);
+                               c->d_pairbuf, c->d_live, stats, nl.fwdpos, nl.revpos, c->d_idx_full);
         break;
     }
     case 3:
-        hipLaunchKernelGGL(k_aniso_apply<4>, dim3(L.nb), dim3(256), 0, c->stream, s, nl.row_start, nl.nbr, nl.pairk,
-                           c->d_live, c->d_pairbuf, c->d_pe_acc);
+        if (L.lpp == 16)
+            hipLaunchKernelGGL(k_aniso_apply<16>, dim3(L.nb), dim3(256), 0, c->stream, s, nl.row_start, nl.nbr,
+                               c->d_idx_full, c->d_pairbuf, c->d_pe_acc);
+        else if (L.lpp == 8)
+            hipLaunchKernelGGL(k_aniso_apply<8>, dim3(L.nb), dim3(256), 0, c->stream, s, nl.row_start, nl.nbr,
+                               c->d_idx_full, c->d_pairbuf, c->d_pe_acc);
+        else
+            hipLaunchKernelGGL(k_aniso_apply<4>, dim3(L.nb), dim3(256), 0, c->stream, s, nl.row_start, nl.nbr,
+                               c->d_idx_full, c->d_pairbuf, c->d_pe_acc);
         break;
     }
 }
@@ -614,6 +649,7 @@ void sx_ecp_prepare(sx_ctx *c, int kind, const double *p, int np)
     if (kind != SX_ECP && kind != SX_ECPR) throw sx_error{SX_ERR_ARG, "an ECP kind is required"};
     if (!c->nl.init) throw sx_error{SX_ERR_STATE, "neighbour list not initialised (sx_nlist_init)"};
     check_params(kind, np);
+    sx_nlist_want_half(c);
     Params P = make_params(c, kind, p, np);
     ensure_pair_buffers(c);
     EcpScreen sc = make_screen(c, kind, P);
@@ -660,6 +696,7 @@ int sx_forces(sx_ctx *c, int kind, const double *params, int nparams)
 {
     return guard(c, [&] {
         need_system(c);
+        if ((kind == SX_ECP || kind == SX_ECPR) && c->nl.init) sx_nlist_want_half(c);
         sx_forces_impl(c, kind, params, nparams);
     });
 }
@@ -671,6 +708,7 @@ int sx_profile_ecp(sx_ctx *c, int kind, const double *params, int nparams, int n
         if (kind != SX_ECP && kind != SX_ECPR) throw sx_error{SX_ERR_ARG, "profile_ecp needs an ECP kind"};
         if (!c->nl.init) throw sx_error{SX_ERR_STATE, "neighbour list not initialised (sx_nlist_init)"};
         check_params(kind, nparams);
+        sx_nlist_want_half(c);
         Params P = make_params(c, kind, params, nparams);
         ensure_pair_buffers(c);
         EcpScreen sc = make_screen(c, kind, P);
@@ -865,11 +903,11 @@ extern "C" int sx_ecp_compare_solvers(sx_ctx *c, int kind, const double *params,
             if (kind == SX_ECPR)
                 hipLaunchKernelGGL(k_ecp_solve<1>, g, b, 0, c->stream, s, c->d_ecp_pp, 0L, nl.half_i, nl.half_j,
                                    c->d_work, c->d_segcount, nnz_half, sc, solver, l0, c->d_pairbuf, c->d_live,
-                                   (long *)nullptr);
+                                   (long *)nullptr, (const int *)nullptr, (const int *)nullptr, (int *)nullptr);
             else
                 hipLaunchKernelGGL(k_ecp_solve<0>, g, b, 0, c->stream, s, c->d_ecp_pp, 0L, nl.half_i, nl.half_j,
                                    c->d_work, c->d_segcount, nnz_half, sc, solver, l0, c->d_pairbuf, c->d_live,
-                                   (long *)nullptr);
+                                   (long *)nullptr, (const int *)nullptr, (const int *)nullptr, (int *)nullptr);
         };
         double t_full_plain = timed([&] { full(SX_ECP_CONVERGED, nullptr); });
         double t_full_learned = timed([&] { full(SX_ECP_LEARNED, sx_ecp_mlp_predict(c)); });

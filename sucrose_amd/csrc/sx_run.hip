@@ -68,6 +68,8 @@ extern "C" int sx_run(sx_ctx *c, const sx_step_spec *spec, int nsteps)
         if (!c->nl.init) throw sx_error{SX_ERR_STATE, "neighbour list not initialised (sx_nlist_init)"};
         if (spec->integrator != SX_INT_VV && !c->have_target_temp)
             throw sx_error{SX_ERR_NOTEMP, "Error: Target temperature not set."};
+        for (int f = 0; f < spec->nforces; ++f)
+            if (spec->kinds[f] == SX_ECP || spec->kinds[f] == SX_ECPR) sx_nlist_want_half(c);
         if (!c->d_step) HIPCHK(hipMalloc(&c->d_step, sizeof(int)));
         hipLaunchKernelGGL(k_step_set, dim3(1), dim3(1), 0, c->stream, c->d_step, spec->first_step);
         if (nsteps == 0) return;
