@@ -154,6 +154,10 @@ int sx_set_ecp_solver(sx_ctx *ctx, int solver);
 /* ECP diagnostics of the last sx_forces(SX_ECP*): pairs screened, pairs solved,
  * total solver iterations, fallbacks from the warm start to Brent. */
 int sx_ecp_stats(sx_ctx *ctx, long out[4]);
+/* Nose-Hoover diagnostics (cumulative over sx_nht_* calls of 2048 particles or
+ * more, which evaluate the chain recurrence in parallel by Newton iteration):
+ * Newton iterations, calls, calls that fell back to the serial walk. */
+int sx_nht_stats(sx_ctx *ctx, long out[3]);
 
 /* bonded terms (module bonded, force.f90:2244-2462); kind 1 bend_, 2 cosine_bend_,
  * 3 dih_, 4 cosine_dih_; idx1 holds 3 (bends) or 4 (dihedrals) 1-based atoms

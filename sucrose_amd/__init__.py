@@ -83,6 +83,7 @@ def lib():
             "sx_forces": [vp, C.c_int, _PD, C.c_int],
             "sx_pair_eval": [vp, C.c_int, _PD, C.c_int, C.c_long, _PI, _PI, _PD, _PD, _PD, _PD, _PD],
             "sx_set_ecp_solver": [vp, C.c_int], "sx_ecp_stats": [vp, C.POINTER(C.c_long)],
+            "sx_nht_stats": [vp, C.POINTER(C.c_long)],
             "sx_set_bonded": [vp, C.c_int, C.c_int, _PI, C.c_double, C.c_double],
             "sx_bonded_forces": [vp],
             "sx_vv_initial_step": [vp, C.c_double], "sx_vv_final_step": [vp, C.c_double],
@@ -333,6 +334,13 @@ class System:
     def ecp_stats(self):
         o = (C.c_long * 4)()
         self._ok(self.L.sx_ecp_stats(self.h, o))
+        return list(o)
+
+    def nht_stats(self):
+        """Cumulative Nose-Hoover (parallel evaluation) counts: Newton
+        iterations, calls, serial fallbacks."""
+        o = (C.c_long * 3)()
+        self._ok(self.L.sx_nht_stats(self.h, o))
         return list(o)
 
     def profile_ecp(self, kind, params=(), nrep=10):

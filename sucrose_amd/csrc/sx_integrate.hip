@@ -297,16 +297,20 @@ __global__ void __launch_bounds__(NHT_CHUNK) k_nht(DState s, double dt, int phas
 
 // ---------------------------------------------------------------------------
 // The same chain recurrence evaluated in parallel (DEER: Newton iteration on
-// the whole sequence).  The particles are cut into NHT_P consecutive chunks,
-// one per thread.  A chunk maps the chain state entering it, s = (p_eta(1:3),
-// sum of m v^2), to the state leaving it; a pass with forward-mode dual
-// numbers gives that map's value and 4x4 Jacobian at a guessed entry state.
-// The affine models of all chunks are composed with a parallel prefix scan,
-// which yields new entry states; the iteration converges quadratically to the
-// states the serial walk produces, and a last pass applies the updates.  The
-// results equal the serial recurrence to rounding (the particle updates are
-// the same expressions in the same order; only each chunk's entry state is
-// reached through the scan).
+// the whole sequence).  The particles are cut into NHT_C consecutive chunks,
+// one per thread, spread over many workgroups.  A chunk maps the chain state
+// entering it, s = (p_eta(1:3), sum of m v^2), to the state leaving it; a pass
+// with forward-mode dual numbers gives that map's value and 4x4 Jacobian at a
+// guessed entry state.  One workgroup composes the affine models of all chunks
+// with a prefix scan, which yields new entry states; the iteration converges
+// quadratically to the states the serial walk produces (three iterations at
+// equilibrium), and a last pass applies the updates.  The results equal the
+// serial recurrence to rounding (the particle updates are the same expressions
+// in the same order; only each chunk's entry state is reached through the
+// scan).  Each call is a fixed sequence of launches (graph-capturable):
+// prep, NHT_MAXIT iterations that return at once after convergence, final
+// pass, finish; without convergence the finish kernel walks the recurrence
+// serially.
 // ---------------------------------------------------------------------------
 struct D4 {
     double v, d[4];
@@ -411,8 +415,9 @@ __device__ __forceinline__ void nht_turn(int phase, double dt, double tq, double
     }
 }
 
-#define NHT_P 512
-#define NHT_MAXIT 16
+#define NHT_C 2048       // chunks
+#define NHT_ST 512       // threads of the scan and the reductions (NHT_C / NHT_ST chunks each)
+#define NHT_MAXIT 12
 struct AffMap {
     double M[16];   // row-major 4x4
     double c[4];
@@ -435,120 +440,194 @@ __device__ __forceinline__ AffMap aff_compose(const AffMap &b, const AffMap &a)
     return r;
 }
 
-__global__ void __launch_bounds__(NHT_P) k_nht_deer(DState s, double dt, int phase, int save, double *scal, double ndof)
+__device__ __forceinline__ void aff_apply(const AffMap &A, const double x[4], double y[4])
 {
-    __shared__ AffMap maps[NHT_P];
-    __shared__ double g[NHT_P][4];
-    __shared__ double red[NHT_P];
-    __shared__ double s0[4];
-    __shared__ int converged;
+    for (int r = 0; r < 4; ++r) {
+        double v = A.c[r];
+        for (int k = 0; k < 4; ++k) v += A.M[r * 4 + k] * x[k];
+        y[r] = v;
+    }
+}
+
+// device scratch of the parallel evaluation: maps (20 per chunk), guesses
+// (4 per chunk), per-chunk sums (4), then s0 (4) and the end state (3)
+struct NhtBuf {
+    double *maps, *g, *part, *s0, *pend;
+    int *conv;
+};
+
+__device__ __forceinline__ void nht_chunk(int n, int c, int &i0, int &i1)
+{
+    const int chunk = (n + NHT_C - 1) / NHT_C;
+    i0 = min(n, c * chunk);
+    i1 = min(n, i0 + chunk);
+}
+
+struct NhtArgs {
+    double dt, ndofT, Temp, tq, hb[3];
+};
+
+__device__ __forceinline__ NhtArgs nht_args(const double *scal, double dt, double ndof)
+{
+    NhtArgs a;
+    a.dt = dt;
+    a.Temp = scal[SX_TARGET_TEMP];
+    a.hb[0] = scal[SX_HB0]; a.hb[1] = scal[SX_HB1]; a.hb[2] = scal[SX_HB2];
+    a.ndofT = ndof * a.Temp;
+    a.tq = dt / 4.0;
+    return a;
+}
+
+// sum of m |v|^2 before the call (strided + tree order), the chain state
+// entering the call, flat first guesses
+__global__ void __launch_bounds__(NHT_ST) k_nhtp_prep(DState s, const double *scal, NhtBuf B,
+                                                      unsigned long long *stats)
+{
+    __shared__ double red[NHT_ST];
     const int t = threadIdx.x, n = s.n;
-    const int chunk = (n + NHT_P - 1) / NHT_P;
-    const int i0 = min(n, t * chunk), i1 = min(n, i0 + chunk);
-    // sum of m |v|^2 before this call (tree order of the serial kernel)
     double acc = 0.0;
-    for (int i = t; i < n; i += NHT_P) {
+    for (int i = t; i < n; i += NHT_ST) {
         double vx = s.f[SX_VX][i], vy = s.f[SX_VY][i], vz = s.f[SX_VZ][i];
         acc += s.f[SX_MASS][i] * (vx * vx + vy * vy + vz * vz);
     }
     red[t] = acc;
     __syncthreads();
-    for (int o = NHT_P / 2; o > 0; o >>= 1) {
+    for (int o = NHT_ST / 2; o > 0; o >>= 1) {
         if (t < o) red[t] = red[t] + red[t + o];
         __syncthreads();
     }
-    const double Temp = scal[SX_TARGET_TEMP];
-    const double hb[3] = {scal[SX_HB0], scal[SX_HB1], scal[SX_HB2]};
-    const double ndofT = ndof * Temp, tq = dt / 4.0;
+    const double s0[4] = {scal[SX_PETA0], scal[SX_PETA1], scal[SX_PETA2], red[0]};
+    for (int c = t; c < NHT_C; c += NHT_ST)
+        for (int k = 0; k < 4; ++k) B.g[c * 4 + k] = s0[k];
     if (t == 0) {
-        s0[0] = scal[SX_PETA0]; s0[1] = scal[SX_PETA1]; s0[2] = scal[SX_PETA2]; s0[3] = red[0];
-        converged = 0;
+        for (int k = 0; k < 4; ++k) B.s0[k] = s0[k];
+        *B.conv = 0;
+        if (stats) atomicAdd(&stats[1], 1ull);
     }
-    __syncthreads();
-    // first guess of every chunk's entry state: the state entering the call
-    for (int k = 0; k < 4; ++k) g[t][k] = s0[k];
-    __syncthreads();
-    for (int it = 0; it < NHT_MAXIT && !converged; ++it) {
-        // chunk pass with dual numbers seeded at the guessed entry state
-        D4 p[3], sv;
-        for (int k = 0; k < 3; ++k) { p[k] = dd(g[t][k]); p[k].d[k] = 1.0; }
-        sv = dd(g[t][3]);
-        sv.d[3] = 1.0;
-        for (int i = i0; i < i1; ++i) {
-            double m = s.f[SX_MASS][i], im = s.f[SX_INV_MASS][i];
-            double a[3] = {0.5 * dt * (s.f[SX_FX][i] * im), 0.5 * dt * (s.f[SX_FY][i] * im),
-                           0.5 * dt * (s.f[SX_FZ][i] * im)};
-            D4 v[3] = {dd(s.f[SX_VX][i]), dd(s.f[SX_VY][i]), dd(s.f[SX_VZ][i])};
-            D4 ea[3];
-            nht_turn(phase, dt, tq, ndofT, Temp, hb, m, a, p, sv, v, ea);
-        }
-        // affine model of this chunk: x -> J x + (E - J g)
-        AffMap A;
-        const D4 *out[4] = {&p[0], &p[1], &p[2], &sv};
-        for (int r = 0; r < 4; ++r) {
-            double cr = out[r]->v;
-            for (int k = 0; k < 4; ++k) {
-                A.M[r * 4 + k] = out[r]->d[k];
-                cr -= out[r]->d[k] * g[t][k];
-            }
-            A.c[r] = cr;
-        }
-        maps[t] = A;
-        __syncthreads();
-        // inclusive prefix of the maps (Hillis-Steele, later after earlier; the
-        // neighbour is read before anyone overwrites it)
-        for (int o = 1; o < NHT_P; o <<= 1) {
-            AffMap m2 = maps[t];
-            if (t >= o) m2 = aff_compose(m2, maps[t - o]);
-            __syncthreads();
-            maps[t] = m2;
-            __syncthreads();
-        }
-        // new entry states: g(t) = prefix(t-1)(s0)
-        double ng[4];
-        if (t == 0) {
-            for (int k = 0; k < 4; ++k) ng[k] = s0[k];
-        } else {
-            const AffMap &P = maps[t - 1];
-            for (int r = 0; r < 4; ++r) {
-                double v = P.c[r];
-                for (int k = 0; k < 4; ++k) v += P.M[r * 4 + k] * s0[k];
-                ng[r] = v;
-            }
-        }
-        double dev = 0.0;
-        for (int k = 0; k < 4; ++k) {
-            double d = fabs(ng[k] - g[t][k]) / (fabs(ng[k]) + 1e-300);
-            dev = (d == d && dev == dev) ? fmax(dev, d) : INFINITY;   // a NaN never counts as converged
-        }
-        red[t] = dev;
-        __syncthreads();
-        for (int k = 0; k < 4; ++k) g[t][k] = ng[k];
-        for (int o = NHT_P / 2; o > 0; o >>= 1) {
-            if (t < o) red[t] = fmax(red[t], red[t + o]);
-            __syncthreads();
-        }
-        if (t == 0 && red[0] < 1e-12) converged = 1;
-        __syncthreads();
-    }
-    // not converged (a far-from-equilibrium chain can need more Newton steps):
-    // the serial walk from the start, one lane, as the small-system kernel
-    if (!converged) {
-        if (t == 0)
-            for (int k = 0; k < 4; ++k) g[0][k] = s0[k];
-        __syncthreads();
-    }
-    const int i0f = converged ? i0 : (t == 0 ? 0 : n), i1f = converged ? i1 : (t == 0 ? n : n);
-    // final pass from the converged entry states
-    double p[3] = {g[t][0], g[t][1], g[t][2]}, sv = g[t][3];
-    double eta_loc[3] = {0.0, 0.0, 0.0}, ke = 0.0;
-    for (int i = i0f; i < i1f; ++i) {
+}
+
+// one chunk per thread: dual-number pass from the guessed entry state ->
+// the chunk's affine model x -> J x + (E - J g)
+__global__ void __launch_bounds__(64) k_nhtp_pass(DState s, double dt, int phase, const double *scal, double ndof,
+                                                  NhtBuf B)
+{
+    if (*B.conv) return;
+    const int c = blockIdx.x * 64 + threadIdx.x;
+    if (c >= NHT_C) return;
+    const NhtArgs a = nht_args(scal, dt, ndof);
+    int i0, i1;
+    nht_chunk(s.n, c, i0, i1);
+    double g[4];
+    for (int k = 0; k < 4; ++k) g[k] = B.g[c * 4 + k];
+    D4 p[3], sv;
+    for (int k = 0; k < 3; ++k) { p[k] = dd(g[k]); p[k].d[k] = 1.0; }
+    sv = dd(g[3]);
+    sv.d[3] = 1.0;
+    for (int i = i0; i < i1; ++i) {
         double m = s.f[SX_MASS][i], im = s.f[SX_INV_MASS][i];
-        double a[3] = {0.5 * dt * (s.f[SX_FX][i] * im), 0.5 * dt * (s.f[SX_FY][i] * im),
-                       0.5 * dt * (s.f[SX_FZ][i] * im)};
+        double acc3[3] = {0.5 * dt * (s.f[SX_FX][i] * im), 0.5 * dt * (s.f[SX_FY][i] * im),
+                          0.5 * dt * (s.f[SX_FZ][i] * im)};
+        D4 v[3] = {dd(s.f[SX_VX][i]), dd(s.f[SX_VY][i]), dd(s.f[SX_VZ][i])};
+        D4 ea[3];
+        nht_turn(phase, dt, a.tq, a.ndofT, a.Temp, a.hb, m, acc3, p, sv, v, ea);
+    }
+    const D4 *out[4] = {&p[0], &p[1], &p[2], &sv};
+    double *A = B.maps + (size_t)c * 20;
+    for (int r = 0; r < 4; ++r) {
+        double cr = out[r]->v;
+        for (int k = 0; k < 4; ++k) {
+            A[r * 4 + k] = out[r]->d[k];
+            cr -= out[r]->d[k] * g[k];
+        }
+        A[16 + r] = cr;
+    }
+}
+
+// prefix of the chunk models -> new entry states; converged when no entry
+// state component moved by more than 1e-12 of its magnitude plus its thermal
+// scale (sqrt(Q_k T) for p_eta(k), n_dof T for sum m v^2), so a p_eta passing
+// through zero does not stall the test; a NaN never counts as converged
+__global__ void __launch_bounds__(NHT_ST) k_nhtp_scan(NhtBuf B, const double *scal, double ndof,
+                                                      unsigned long long *stats)
+{
+    if (*B.conv) return;
+    const int PER = NHT_C / NHT_ST;
+    __shared__ AffMap tot[NHT_ST];
+    __shared__ double red[NHT_ST];
+    const int t = threadIdx.x;
+    AffMap loc[PER];
+    for (int q = 0; q < PER; ++q) {
+        const double *A = B.maps + (size_t)(t * PER + q) * 20;
+        for (int k = 0; k < 16; ++k) loc[q].M[k] = A[k];
+        for (int k = 0; k < 4; ++k) loc[q].c[k] = A[16 + k];
+    }
+    AffMap P = loc[0];
+    for (int q = 1; q < PER; ++q) P = aff_compose(loc[q], P);
+    tot[t] = P;
+    __syncthreads();
+    // inclusive prefix over the threads (Hillis-Steele, later after earlier)
+    for (int o = 1; o < NHT_ST; o <<= 1) {
+        AffMap m2 = tot[t];
+        if (t >= o) m2 = aff_compose(m2, tot[t - o]);
+        __syncthreads();
+        tot[t] = m2;
+        __syncthreads();
+    }
+    double s0[4] = {B.s0[0], B.s0[1], B.s0[2], B.s0[3]};
+    double e[4];
+    if (t == 0) {
+        for (int k = 0; k < 4; ++k) e[k] = s0[k];
+    } else {
+        aff_apply(tot[t - 1], s0, e);
+    }
+    const double T = scal[SX_TARGET_TEMP];
+    const double scale[4] = {sqrt(fabs(scal[SX_HB0] * T)), sqrt(fabs(scal[SX_HB1] * T)), sqrt(fabs(scal[SX_HB2] * T)),
+                             fabs(ndof * T)};
+    double dev = 0.0;
+    for (int q = 0; q < PER; ++q) {
+        const int c = t * PER + q;
+        for (int k = 0; k < 4; ++k) {
+            double d = fabs(e[k] - B.g[c * 4 + k]) / (fabs(e[k]) + scale[k] + 1e-300);
+            dev = (d == d && dev == dev) ? fmax(dev, d) : INFINITY;
+            B.g[c * 4 + k] = e[k];
+        }
+        if (q + 1 < PER) {
+            double nx[4];
+            aff_apply(loc[q], e, nx);
+            for (int k = 0; k < 4; ++k) e[k] = nx[k];
+        }
+    }
+    red[t] = dev;
+    __syncthreads();
+    for (int o = NHT_ST / 2; o > 0; o >>= 1) {
+        if (t < o) red[t] = fmax(red[t], red[t + o]);
+        __syncthreads();
+    }
+    if (t == 0) {
+        if (stats) atomicAdd(&stats[0], 1ull);
+        if (red[0] < 1e-12) *B.conv = 1;
+    }
+}
+
+// the particle updates of every chunk from its converged entry state
+__global__ void __launch_bounds__(64) k_nhtp_final(DState s, double dt, int phase, int save, const double *scal,
+                                                   double ndof, NhtBuf B)
+{
+    if (!*B.conv) return;
+    const int c = blockIdx.x * 64 + threadIdx.x;
+    if (c >= NHT_C) return;
+    const NhtArgs a = nht_args(scal, dt, ndof);
+    int i0, i1;
+    nht_chunk(s.n, c, i0, i1);
+    double p[3] = {B.g[c * 4 + 0], B.g[c * 4 + 1], B.g[c * 4 + 2]}, sv = B.g[c * 4 + 3];
+    double eta_loc[3] = {0.0, 0.0, 0.0}, ke = 0.0;
+    for (int i = i0; i < i1; ++i) {
+        double m = s.f[SX_MASS][i], im = s.f[SX_INV_MASS][i];
+        double acc3[3] = {0.5 * dt * (s.f[SX_FX][i] * im), 0.5 * dt * (s.f[SX_FY][i] * im),
+                          0.5 * dt * (s.f[SX_FZ][i] * im)};
         double v[3] = {s.f[SX_VX][i], s.f[SX_VY][i], s.f[SX_VZ][i]};
         double ea[3];
-        nht_turn(phase, dt, tq, ndofT, Temp, hb, m, a, p, sv, v, ea);
+        nht_turn(phase, dt, a.tq, a.ndofT, a.Temp, a.hb, m, acc3, p, sv, v, ea);
         for (int k = 0; k < 3; ++k) eta_loc[k] += ea[k];
         s.f[SX_VX][i] = v[0]; s.f[SX_VY][i] = v[1]; s.f[SX_VZ][i] = v[2];
         if (phase == 0) {
@@ -561,23 +640,69 @@ __global__ void __launch_bounds__(NHT_P) k_nht_deer(DState s, double dt, int pha
             ke += 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
         }
     }
+    for (int k = 0; k < 3; ++k) B.part[c * 4 + k] = eta_loc[k];
+    B.part[c * 4 + 3] = ke;
     // the chain state after the last particle belongs to the last non-empty chunk
-    if (i1f == n && i0f < n) {
-        scal[SX_PETA0] = p[0]; scal[SX_PETA1] = p[1]; scal[SX_PETA2] = p[2];
-    }
-    for (int k = 0; k < 4; ++k) {
-        red[t] = k < 3 ? eta_loc[k] : ke;
-        __syncthreads();
-        for (int o = NHT_P / 2; o > 0; o >>= 1) {
-            if (t < o) red[t] = red[t] + red[t + o];
+    if (i1 == s.n && i0 < s.n)
+        for (int k = 0; k < 3; ++k) B.pend[k] = p[k];
+}
+
+// converged: sums of the chunks' eta increments and kinetic energy (fixed
+// order), the chain state after the last particle.  Not converged: one lane
+// walks the whole recurrence from the state entering the call.
+__global__ void __launch_bounds__(NHT_ST) k_nhtp_finish(DState s, double dt, int phase, int save, double *scal,
+                                                        double ndof, NhtBuf B, unsigned long long *stats)
+{
+    __shared__ double red[NHT_ST];
+    __shared__ double tot[4];
+    const int t = threadIdx.x;
+    if (*B.conv) {
+        for (int k = 0; k < 4; ++k) {
+            double acc = 0.0;
+            for (int c = t; c < NHT_C; c += NHT_ST) acc += B.part[c * 4 + k];
+            red[t] = acc;
+            __syncthreads();
+            for (int o = NHT_ST / 2; o > 0; o >>= 1) {
+                if (t < o) red[t] = red[t] + red[t + o];
+                __syncthreads();
+            }
+            if (t == 0) tot[k] = red[0];
             __syncthreads();
         }
         if (t == 0) {
-            if (k < 3) scal[SX_ETA0 + k] = scal[SX_ETA0 + k] + red[0];
-            else if (phase == 1) scal[SX_KIN_ENG] = red[0];
+            for (int k = 0; k < 3; ++k) scal[SX_ETA0 + k] = scal[SX_ETA0 + k] + tot[k];
+            if (phase == 1) scal[SX_KIN_ENG] = tot[3];
+            scal[SX_PETA0] = B.pend[0]; scal[SX_PETA1] = B.pend[1]; scal[SX_PETA2] = B.pend[2];
         }
-        __syncthreads();
+        return;
     }
+    if (t != 0) return;
+    if (stats) atomicAdd(&stats[2], 1ull);
+    const NhtArgs a = nht_args(scal, dt, ndof);
+    double p[3] = {B.s0[0], B.s0[1], B.s0[2]}, sv = B.s0[3];
+    double eta_loc[3] = {0.0, 0.0, 0.0}, ke = 0.0;
+    for (int i = 0; i < s.n; ++i) {
+        double m = s.f[SX_MASS][i], im = s.f[SX_INV_MASS][i];
+        double acc3[3] = {0.5 * dt * (s.f[SX_FX][i] * im), 0.5 * dt * (s.f[SX_FY][i] * im),
+                          0.5 * dt * (s.f[SX_FZ][i] * im)};
+        double v[3] = {s.f[SX_VX][i], s.f[SX_VY][i], s.f[SX_VZ][i]};
+        double ea[3];
+        nht_turn(phase, dt, a.tq, a.ndofT, a.Temp, a.hb, m, acc3, p, sv, v, ea);
+        for (int k = 0; k < 3; ++k) eta_loc[k] += ea[k];
+        s.f[SX_VX][i] = v[0]; s.f[SX_VY][i] = v[1]; s.f[SX_VZ][i] = v[2];
+        if (phase == 0) {
+            if (save) save_old(s, i);
+            s.f[SX_RX][i] = s.f[SX_RX][i] + dt * v[0];
+            s.f[SX_RY][i] = s.f[SX_RY][i] + dt * v[1];
+            s.f[SX_RZ][i] = s.f[SX_RZ][i] + dt * v[2];
+            wrap_pos(s, i);
+        } else {
+            ke += 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        }
+    }
+    for (int k = 0; k < 3; ++k) scal[SX_ETA0 + k] = scal[SX_ETA0 + k] + eta_loc[k];
+    if (phase == 1) scal[SX_KIN_ENG] = ke;
+    scal[SX_PETA0] = p[0]; scal[SX_PETA1] = p[1]; scal[SX_PETA2] = p[2];
 }
 
 __global__ void k_nht_hb(double *scal, double ndof, double coupling, int has_coupling)
@@ -901,9 +1026,42 @@ void sx_nht_impl(sx_ctx *c, double dt, const double *coupling, int phase, bool s
         hipLaunchKernelGGL(k_nht, dim3(1), dim3(NHT_CHUNK), 0, c->stream, c->dstate(), dt, phase, save ? 1 : 0,
                            c->d_scal, ndof, c->d_part);
     } else {
-        hipLaunchKernelGGL(k_nht_deer, dim3(1), dim3(NHT_P), 0, c->stream, c->dstate(), dt, phase, save ? 1 : 0,
-                           c->d_scal, ndof);
+        if (!c->d_nht) {
+            const size_t nd = (size_t)NHT_C * (20 + 4 + 4) + 8;
+            HIPCHK(hipMalloc(&c->d_nht, sizeof(double) * nd + sizeof(int) * 4));
+        }
+        NhtBuf B;
+        B.maps = c->d_nht;
+        B.g = B.maps + (size_t)NHT_C * 20;
+        B.part = B.g + (size_t)NHT_C * 4;
+        B.s0 = B.part + (size_t)NHT_C * 4;
+        B.pend = B.s0 + 4;
+        B.conv = (int *)(B.s0 + 8);
+        unsigned long long *st = (unsigned long long *)(c->d_counts + 4);
+        DState s = c->dstate();
+        const int nb = NHT_C / 64;
+        hipLaunchKernelGGL(k_nhtp_prep, dim3(1), dim3(NHT_ST), 0, c->stream, s, c->d_scal, B, st);
+        for (int it = 0; it < NHT_MAXIT; ++it) {
+            hipLaunchKernelGGL(k_nhtp_pass, dim3(nb), dim3(64), 0, c->stream, s, dt, phase, c->d_scal, ndof, B);
+            hipLaunchKernelGGL(k_nhtp_scan, dim3(1), dim3(NHT_ST), 0, c->stream, B, c->d_scal, ndof, st);
+        }
+        hipLaunchKernelGGL(k_nhtp_final, dim3(nb), dim3(64), 0, c->stream, s, dt, phase, save ? 1 : 0, c->d_scal,
+                           ndof, B);
+        hipLaunchKernelGGL(k_nhtp_finish, dim3(1), dim3(NHT_ST), 0, c->stream, s, dt, phase, save ? 1 : 0,
+                           c->d_scal, ndof, B, st);
     }
+}
+
+extern "C" int sx_nht_stats(sx_ctx *c, long out[3])
+{
+    return guard(c, [&] {
+        long dev[8];
+        HIPCHK(hipMemcpyAsync(dev, c->d_counts, sizeof(long) * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        out[0] = dev[4];
+        out[1] = dev[5];
+        out[2] = dev[6];
+    });
 }
 
 void sx_rattle_impl(sx_ctx *c, double dt, int phase)

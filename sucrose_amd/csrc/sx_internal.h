@@ -135,6 +135,8 @@ struct sx_ctx {
     // adds, or 0; set by the solve stage, cleared again by the apply stage
     int *d_idx_full = nullptr;
     long idx_cap = 0;
+    // parallel Nose-Hoover scratch (maps, guesses, chunk sums, control)
+    double *d_nht = nullptr;
     // per-particle ECP precompute
     double *d_ecp_pp = nullptr;   // FP64 records, 24 doubles per particle
     float *d_ecp_scr = nullptr;   // FP32 screen records, 8 floats per particle
