@@ -344,6 +344,7 @@ def learned_vs_plain(dev, sx, p, step, nsteps):
         "speedup_vs_plain_newton": lam_plain / lam_learned,
         "speedup_vs_brent_1e-10": cmp["ms_brent_tight"] / lam_learned,
         "solve_stage_ms": {"plain": cmp["ms_solve_plain"], "learned": cmp["ms_solve_learned"]},
+        "solve_stage_speedup_vs_plain": cmp["ms_solve_plain"] / cmp["ms_solve_learned"],
         "force_rel_err_learned_vs_plain": err_learned,
         "force_rel_err_brent_reference_vs_converged": err_brent,
         "step_particle_steps_per_s": rates,
@@ -351,6 +352,7 @@ def learned_vs_plain(dev, sx, p, step, nsteps):
     }
     log(f"[learned] lambda0 rms {cmp['lambda0_err_rms']:.2e}; lambda solve ms plain {lam_plain:.4f} learned "
         f"{lam_learned:.4f} (network {cmp['ms_network']:.4f}) brent-1e-10 {cmp['ms_brent_tight']:.4f}; "
+        f"solve stage ms plain {cmp['ms_solve_plain']:.4f} learned (one kernel) {cmp['ms_solve_learned']:.4f}; "
         f"force err learned {err_learned:.1e}, reference-brent {err_brent:.1e}; steps/s {rates}")
     return out
 

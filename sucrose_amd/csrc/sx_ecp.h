@@ -156,14 +156,19 @@ __device__ __forceinline__ void ecp_F_derivs(double l, const Sym3 &A, const Sym3
 // step below tol ends the solve (that step is taken), a step that leaves the
 // bracket is replaced by bisection.  `ok` is 0 only if 100 iterations pass.
 __device__ __forceinline__ double ecp_lambda_newton(const Sym3 &A, const Sym3 &B, const double r[3], double l0,
-                                                    double tol, int &iters, int &ok)
+                                                    double tol, int &iters, int &ok, const double *pre = nullptr)
 {
-    double lo = 0.0, hi = 1.0, l = fmin(fmax(l0, 1e-6), 1.0 - 1e-6);
+    // pre (may be NULL): F, F', F'' already evaluated at l0 (inside (0, 1))
+    double lo = 0.0, hi = 1.0, l = pre ? l0 : fmin(fmax(l0, 1e-6), 1.0 - 1e-6);
     ok = 0;
     int it = 0;
     for (; it < 100; ++it) {
         double F, d1, d2;
-        ecp_F_derivs(l, A, B, r, F, d1, d2);
+        if (it == 0 && pre) {
+            F = pre[0]; d1 = pre[1]; d2 = pre[2];
+        } else {
+            ecp_F_derivs(l, A, B, r, F, d1, d2);
+        }
         if (d1 > 0) lo = l; else hi = l;
         if (d2 < 0) {
             double step = -d1 / d2;
@@ -331,4 +336,83 @@ __device__ __forceinline__ void ecp_forces(const DState &s, const double *pp, lo
         pot = sx * pot;
     }
     o.pe = pot;
+}
+
+// ---------------------------------------------------------------------------
+// one pair of the solve stage
+// ---------------------------------------------------------------------------
+template <int VARIANT>
+__device__ __forceinline__ double ecp_solve_lambda(int solver, const double *pp, long ld, int a, int b,
+                                                   const double RIJ[3], double fpsi, double lam0, int &iters,
+                                                   int &fallback)
+{
+    Sym3 A = load_sym(pp, ld, PP_AR, a), B = load_sym(pp, ld, PP_AR, b);
+    fallback = 0;
+    if (solver == SX_ECP_BRENT_REFERENCE) return ecp_brent(A, B, RIJ, fpsi, iters);
+    int ok = 0;
+    double l = ecp_lambda_newton(A, B, RIJ, lam0, 1e-10, iters, ok);
+    if (!ok) {
+        int it2 = 0;
+        l = ecp_lambda_newton(A, B, RIJ, ECP_PSI, 1e-10, it2, ok);
+        iters += it2;
+        fallback = 1;
+    }
+    return l;
+}
+
+// Solves pair k = (a, b) of the work list and stores its energy, force and
+// torques at res[10 k]; idx_full (may be NULL) gets 1 + k at the pair's two
+// full-list entries.  warm: l0 is a warm start (learned), and a pair whose
+// F(l0) already proves it out of range ends there (returns 1, live[k] = 0).
+template <int VARIANT>
+__device__ __forceinline__ int ecp_pair_solve(const DState &s, const double *pp, long ld, int k, int a, int b,
+                                              int solver, bool warm, double l0, double cutoff, double *res,
+                                              unsigned char *live, const int *fwdpos, const int *revpos, int *idx_full,
+                                              int &iters, int &fb)
+{
+    double RIJ[3];
+    pair_rij(s, a, b, RIJ);
+    iters = 0;
+    fb = 0;
+    double lam;
+    if (solver == SX_ECP_BRENT_REFERENCE) {
+        Sym3 A = load_sym(pp, ld, PP_AR, a), B = load_sym(pp, ld, PP_AR, b);
+        lam = ecp_brent(A, B, RIJ, ecp_optim_dist(ECP_PSI, A, B, RIJ), iters);
+    } else {
+        if (warm) {
+            // F and its derivatives at the warm start serve both the
+            // out-of-range proof and the first Newton step
+            Sym3 A = load_sym(pp, ld, PP_AR, a), B = load_sym(pp, ld, PP_AR, b);
+            double rsq = RIJ[0] * RIJ[0] + RIJ[1] * RIJ[1] + RIJ[2] * RIJ[2];
+            l0 = fmin(fmax(l0, 1e-6), 1.0 - 1e-6);
+            double d[3];
+            ecp_F_derivs(l0, A, B, RIJ, d[0], d[1], d[2]);
+            if (ecp_zero_proven(d[0], rsq, VARIANT, cutoff)) {
+                live[k] = 0;
+                return 1;
+            }
+            int ok = 0;
+            lam = ecp_lambda_newton(A, B, RIJ, l0, 1e-10, iters, ok, d);
+            if (!ok) {
+                int it2 = 0;
+                lam = ecp_lambda_newton(A, B, RIJ, ECP_PSI, 1e-10, it2, ok);
+                iters += it2;
+                fb = 1;
+            }
+        } else {
+            lam = ecp_solve_lambda<VARIANT>(solver, pp, ld, a, b, RIJ, 0.0, ECP_PSI, iters, fb);
+        }
+    }
+    PairOut o;
+    ecp_forces<VARIANT>(s, pp, ld, a, b, cutoff, RIJ, lam, o);
+    double *r = res + (long)k * 10;
+    r[0] = o.pe;
+    r[1] = o.f[0]; r[2] = o.f[1]; r[3] = o.f[2];
+    r[4] = o.ti[0]; r[5] = o.ti[1]; r[6] = o.ti[2];
+    r[7] = o.tj[0]; r[8] = o.tj[1]; r[9] = o.tj[2];
+    if (idx_full) {
+        idx_full[fwdpos[k]] = k + 1;
+        idx_full[revpos[k]] = k + 1;
+    }
+    return 0;
 }

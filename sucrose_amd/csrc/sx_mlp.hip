@@ -498,6 +498,98 @@ __global__ void __launch_bounds__(256) k_mlp_lambda0(DState s, const float *p, c
     (void)end;
 }
 
+// The learned solve stage in one kernel: one wavefront per work-list segment
+// (as the plain solve stage);
+// for each round of 64 pairs the network runs on four groups of 16 (its
+// outputs are this kernel's warm starts, clamped to [0.02, 0.98] as in
+// k_mlp_lambda0), the warm starts pass through LDS to the lanes that own the
+// pairs, and every lane then finishes its pair as the plain solve stage does.
+// stats (may be NULL): [0] solved, [1] iterations, [2] fallbacks, [3] pairs
+// proven out of range by the warm start.
+template <int VARIANT>
+__global__ void __launch_bounds__(64) k_ecp_solve_learned(DState s, const double *pp, const float *p, const bf16x4 *wf,
+                                                          const bf16x8 *wi, const int *half_i, const int *half_j,
+                                                          const int *work, const int *segcount, const int *nnz_half,
+                                                          double cutoff, double *res, unsigned char *live,
+                                                          long *stats, const int *fwdpos, const int *revpos,
+                                                          int *idx_full)
+{
+    const int cnt = segcount[blockIdx.x];
+    if (cnt == 0) return;
+    __shared__ float sp[3 * 64 + 2];
+    __shared__ float l0s[64];
+    __shared__ float xs[64 * 8];   // features of the round's 64 pairs, one row per pair
+    const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
+    // operand tiles are read from global memory (each lane its own 8 or 16
+    // bytes per tile; the 20 KB of tiles stay in the caches), biases from LDS
+    const bf16x4 *s1 = wf + F_L1 * 64;
+    const bf16x8 *si = wi;
+    sp[lane] = p[OFF_B1 + lane];
+    sp[64 + lane] = p[OFF_B2 + lane];
+    sp[128 + lane] = p[OFF_B3 + lane];
+    if (lane < 2) sp[192 + lane] = p[OFF_B4 + lane];
+    __syncthreads();
+    long start, end;
+    ecp_segment(*nnz_half, blockIdx.x, start, end);
+    long my_iters = 0, my_solved = 0, my_fb = 0, my_skip = 0;
+    for (int base = 0; base < cnt; base += 64) {
+        // every lane the features of its own pair (one round of loads), then
+        // the network reads them from LDS in the operand layout
+        const int u = base + lane;
+        int k = 0, a = 0, b = 0;
+        {
+            float x[7] = {0, 0, 0, 0, 0, 0, 0};
+            if (u < cnt) {
+                k = work[start + u];
+                a = half_i[k];
+                b = half_j[k];
+                double RIJ[3];
+                pair_rij(s, a, b, RIJ);
+                mlp_features(s, a, b, RIJ, x);
+            }
+#pragma unroll
+            for (int j = 0; j < 7; ++j) xs[lane * 8 + j] = x[j];
+            xs[lane * 8 + 7] = 0.f;
+        }
+        __syncthreads();
+        for (int q = 0; q < 4 && base + 16 * q < cnt; ++q) {
+            bf16x4 xb;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) xb[j] = to_bf16(xs[(16 * q + r) * 8 + 4 * g + j]);
+            if (g >= 2) xb = bf16x4{0, 0, 0, 0};
+            f32x4 y = forward_regs(s1, si, sp, xb, lane);
+            if (g == 0) l0s[16 * q + r] = fminf(fmaxf(y[0], 0.02f), 0.98f);
+        }
+        __syncthreads();
+        if (u < cnt) {
+            int iters = 0, fb = 0;
+            if (ecp_pair_solve<VARIANT>(s, pp, 0L, k, a, b, SX_ECP_LEARNED, true, (double)l0s[lane], cutoff, res,
+                                        live, fwdpos, revpos, idx_full, iters, fb)) {
+                ++my_skip;
+            } else {
+                my_iters += iters;
+                my_solved += 1;
+                my_fb += fb;
+            }
+        }
+        __syncthreads();
+    }
+    (void)end;
+    if (!stats) return;
+    for (int off = 32; off > 0; off >>= 1) {
+        my_iters += __shfl_xor(my_iters, off, 64);
+        my_solved += __shfl_xor(my_solved, off, 64);
+        my_fb += __shfl_xor(my_fb, off, 64);
+        my_skip += __shfl_xor(my_skip, off, 64);
+    }
+    if (lane == 0) {
+        atomicAdd((unsigned long long *)&stats[0], (unsigned long long)my_solved);
+        atomicAdd((unsigned long long *)&stats[1], (unsigned long long)my_iters);
+        if (my_fb) atomicAdd((unsigned long long *)&stats[2], (unsigned long long)my_fb);
+        if (my_skip) atomicAdd((unsigned long long *)&stats[3], (unsigned long long)my_skip);
+    }
+}
+
 // training samples from the current work list: inputs, converged lambda*
 // (Newton to 1e-12) and the contact distance over 2 max(a); slots are
 // reserved per wave with one atomic on the sample counter
@@ -592,6 +684,23 @@ const float *sx_ecp_mlp_predict(sx_ctx *c)
     hipLaunchKernelGGL(k_mlp_lambda0, dim3(ECP_NSEG), dim3(256), 0, c->stream, c->dstate(), M->p, M->wf, M->wi,
                        c->nl.half_i, c->nl.half_j, c->d_work, c->d_segcount, c->d_flag + 2, M->lam0);
     return M->lam0;
+}
+
+// the learned solve stage over the current work list (results, live flags
+// and full-list marks as the plain solve stage)
+void sx_ecp_solve_learned(sx_ctx *c, int variant, double cutoff, long *stats, bool mark)
+{
+    MlpState *M = mlp_of(c);
+    NList &nl = c->nl;
+    int *idx = mark ? c->d_idx_full : nullptr;
+    if (variant)
+        hipLaunchKernelGGL(k_ecp_solve_learned<1>, dim3(ECP_NSEG), dim3(64), 0, c->stream, c->dstate(), c->d_ecp_pp,
+                           M->p, M->wf, M->wi, nl.half_i, nl.half_j, c->d_work, c->d_segcount, c->d_flag + 2, cutoff,
+                           c->d_pairbuf, c->d_live, stats, nl.fwdpos, nl.revpos, idx);
+    else
+        hipLaunchKernelGGL(k_ecp_solve_learned<0>, dim3(ECP_NSEG), dim3(64), 0, c->stream, c->dstate(), c->d_ecp_pp,
+                           M->p, M->wf, M->wi, nl.half_i, nl.half_j, c->d_work, c->d_segcount, c->d_flag + 2, cutoff,
+                           c->d_pairbuf, c->d_live, stats, nl.fwdpos, nl.revpos, idx);
 }
 
 extern "C" int sx_mlp_init(sx_ctx *c, uint32_t seed)

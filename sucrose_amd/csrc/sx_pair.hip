@@ -221,25 +221,6 @@ __global__ void __launch_bounds__(256) k_ecp_screen(const double4 *pos4, const f
     if (t == 0) segcount[blockIdx.x] = run;
 }
 
-template <int VARIANT>
-__device__ __forceinline__ double ecp_solve_lambda(int solver, const double *pp, long ld, int a, int b,
-                                                   const double RIJ[3], double fpsi, double lam0, int &iters,
-                                                   int &fallback)
-{
-    Sym3 A = load_sym(pp, ld, PP_AR, a), B = load_sym(pp, ld, PP_AR, b);
-    fallback = 0;
-    if (solver == SX_ECP_BRENT_REFERENCE) return ecp_brent(A, B, RIJ, fpsi, iters);
-    int ok = 0;
-    double l = ecp_lambda_newton(A, B, RIJ, lam0, 1e-10, iters, ok);
-    if (!ok) {
-        int it2 = 0;
-        l = ecp_lambda_newton(A, B, RIJ, ECP_PSI, 1e-10, it2, ok);
-        iters += it2;
-        fallback = 1;
-    }
-    return l;
-}
-
 #define ECP_SOLVE_THREADS 64
 
 // one block per segment of the work list; stats (may be NULL) accumulates
@@ -264,38 +245,12 @@ __global__ void __launch_bounds__(ECP_SOLVE_THREADS) k_ecp_solve(DState s, const
     for (long u = threadIdx.x; u < cnt; u += blockDim.x) {
         long t = start + u;
         int k = work[t];
-        int a = half_i[k], b = half_j[k];
-        double RIJ[3];
-        pair_rij(s, a, b, RIJ);
         int iters = 0, fb = 0;
-        double lam;
-        if (solver == SX_ECP_BRENT_REFERENCE) {
-            Sym3 A = load_sym(pp, ld, PP_AR, a), B = load_sym(pp, ld, PP_AR, b);
-            lam = ecp_brent(A, B, RIJ, ecp_optim_dist(ECP_PSI, A, B, RIJ), iters);
-        } else {
-            double l0 = ECP_PSI;
-            if (lam0) {
-                l0 = (double)lam0[t];
-                Sym3 A = load_sym(pp, ld, PP_AR, a), B = load_sym(pp, ld, PP_AR, b);
-                double rsq = RIJ[0] * RIJ[0] + RIJ[1] * RIJ[1] + RIJ[2] * RIJ[2];
-                if (ecp_zero_proven(-ecp_optim_dist(l0, A, B, RIJ), rsq, VARIANT, sc.cutoff)) {
-                    live[k] = 0;
-                    ++my_skip;
-                    continue;
-                }
-            }
-            lam = ecp_solve_lambda<VARIANT>(solver, pp, ld, a, b, RIJ, 0.0, l0, iters, fb);
-        }
-        PairOut o;
-        ecp_forces<VARIANT>(s, pp, ld, a, b, sc.cutoff, RIJ, lam, o);
-        double *r = res + (long)k * 10;
-        r[0] = o.pe;
-        r[1] = o.f[0]; r[2] = o.f[1]; r[3] = o.f[2];
-        r[4] = o.ti[0]; r[5] = o.ti[1]; r[6] = o.ti[2];
-        r[7] = o.tj[0]; r[8] = o.tj[1]; r[9] = o.tj[2];
-        if (idx_full) {
-            idx_full[fwdpos[k]] = k + 1;
-            idx_full[revpos[k]] = k + 1;
+        if (ecp_pair_solve<VARIANT>(s, pp, ld, k, half_i[k], half_j[k], solver, lam0 != nullptr,
+                                    lam0 ? (double)lam0[t] : ECP_PSI, sc.cutoff, res, live, fwdpos, revpos, idx_full,
+                                    iters, fb)) {
+            ++my_skip;
+            continue;
         }
         my_iters += iters;
         my_solved += 1;
@@ -493,6 +448,8 @@ static void ensure_pair_buffers(sx_ctx *c)
 
 // learned warm start: predictions for the work list of the current call
 const float *sx_ecp_mlp_predict(sx_ctx *c);
+// learned solve stage (network and solve in one kernel)
+void sx_ecp_solve_learned(sx_ctx *c, int variant, double cutoff, long *stats, bool mark);
 
 static EcpScreen make_screen(sx_ctx *c, int kind, const Params &P)
 {
@@ -548,9 +505,12 @@ static void ecp_stage(sx_ctx *c, int stage, int kind, const EcpScreen &sc, const
                            c->d_ecp_scr, nl.half_i, nl.half_j, nnz_half, sc, c->d_live, c->d_work, c->d_segcount);
         break;
     case 2: {
-        const float *lam0 = nullptr;
-        if (c->ecp_solver == SX_ECP_LEARNED) lam0 = sx_ecp_mlp_predict(c);
         long *stats = L.stats ? c->d_counts : nullptr;
+        if (c->ecp_solver == SX_ECP_LEARNED) {
+            sx_ecp_solve_learned(c, kind == SX_ECPR ? 1 : 0, sc.cutoff, stats, true);
+            break;
+        }
+        const float *lam0 = nullptr;
         if (kind == SX_ECPR)
             hipLaunchKernelGGL(k_ecp_solve<1>, dim3(ECP_NSEG), dim3(ECP_SOLVE_THREADS), 0, c->stream, s, c->d_ecp_pp,
                                ld, nl.half_i, nl.half_j, c->d_work, c->d_segcount, nnz_half, sc, c->ecp_solver, lam0,
@@ -611,14 +571,18 @@ __global__ void __launch_bounds__(ECP_SOLVE_THREADS) k_ecp_lambda(DState s, cons
                 int ok = 0;
                 l = ecp_lambda_newton(A, B, RIJ, ECP_PSI, 1e-10, it, ok);
             } else {
-                l = (double)lam0[t];
+                // F and its derivatives at the warm start serve both the
+                // out-of-range proof and the first Newton step
+                l = fmin(fmax((double)lam0[t], 1e-6), 1.0 - 1e-6);
                 double rsq = RIJ[0] * RIJ[0] + RIJ[1] * RIJ[1] + RIJ[2] * RIJ[2];
-                if (ecp_zero_proven(-ecp_optim_dist(l, A, B, RIJ), rsq, sc.variant, sc.cutoff)) {
+                double d[3];
+                ecp_F_derivs(l, A, B, RIJ, d[0], d[1], d[2]);
+                if (ecp_zero_proven(d[0], rsq, sc.variant, sc.cutoff)) {
                     ++my_skip;
                     l = -1.0;   // out of range: no contact parameter needed
                 } else {
                     int ok = 0;
-                    l = ecp_lambda_newton(A, B, RIJ, l, 1e-10, it, ok);
+                    l = ecp_lambda_newton(A, B, RIJ, l, 1e-10, it, ok, d);
                 }
             }
             lam_out[t] = l;
@@ -910,7 +874,7 @@ extern "C" int sx_ecp_compare_solvers(sx_ctx *c, int kind, const double *params,
                                    (long *)nullptr, (const int *)nullptr, (const int *)nullptr, (int *)nullptr);
         };
         double t_full_plain = timed([&] { full(SX_ECP_CONVERGED, nullptr); });
-        double t_full_learned = timed([&] { full(SX_ECP_LEARNED, sx_ecp_mlp_predict(c)); });
+        double t_full_learned = timed([&] { sx_ecp_solve_learned(c, kind == SX_ECPR ? 1 : 0, sc.cutoff, nullptr, false); });
         // iteration counts (4 counters per solver)
         hipLaunchKernelGGL(k_ecp_lambda<3>, g, b, 0, c->stream, s, c->d_ecp_pp, nl.half_i, nl.half_j, c->d_work,
                            c->d_segcount, nnz_half, sc, lam0, lam_b, cnt + 0);
