@@ -193,7 +193,7 @@ def batched_workload(sx, args):
     choice, ms = d.batch_autotune(p["dt"], damping=p["gamma"], first_step=step, nrep=5)
     shape = sx.System.BATCH_SHAPES[choice]
     log(f"[batch] autotune: " + " ".join(f"{s[0]}x{s[1]}:{m:.3f}ms" for s, m in zip(sx.System.BATCH_SHAPES, ms))
-        + f" -> workgroup {shape[0]}, {shape[1]} lanes/particle")
+        + f" -> {shape[0]} particles per force workgroup, {shape[1]} lanes/particle")
     d.batch_run(50, p["dt"], damping=p["gamma"], first_step=step)
     step += 50
     state = d.state() if not args.no_cpu else None
@@ -207,7 +207,7 @@ def batched_workload(sx, args):
     kinds = np.asarray(p["kinds"])
     lj = kinds == sx.LJ
     rec = {"value": p["n"] * args.batch_steps / el, "unit": "particle-steps/s", "steps": args.batch_steps,
-           "ms_per_step": el / args.batch_steps * 1e3, "launch_shape": {"workgroup": shape[0],
+           "ms_per_step": el / args.batch_steps * 1e3, "launch_shape": {"particles_per_workgroup": shape[0],
                                                                        "lanes_per_particle": shape[1]},
            "autotune_ms": [round(float(m), 4) for m in ms],
            "T_over_T0": {"lj": float(np.mean(th[lj, 3] / temps[lj])), "gb": float(np.mean(th[~lj, 3] / temps[~lj]))},

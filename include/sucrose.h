@@ -204,8 +204,8 @@ typedef struct sx_step_spec {
 int sx_run(sx_ctx *ctx, const sx_step_spec *spec, int nsteps);
 
 /* ------------------------------------------------------------------ */
-/* batched replicas: independent small systems, one workgroup per system */
-/* and one launch per step (the README loop of each system)            */
+/* batched replicas: independent small systems advanced together, three     */
+/* launches per step (the README loop of each system), graph-replayed      */
 /* ------------------------------------------------------------------ */
 /* The context's particles are split into nsys contiguous systems of counts[s]
  * particles, each in its own cubic periodic box of edge box[s], with pair
@@ -222,11 +222,11 @@ int sx_batch_run(sx_ctx *ctx, int nsteps, double dt, double damping, int first_s
 int sx_batch_thermo(sx_ctx *ctx, double *out);
 /* times nrep steps of each of the 8 launch shapes from the same state,
  * restores it, keeps the fastest; ms[8] (may be NULL) gets ms per step.
- * Shapes (workgroup, lanes per particle): 0 (64,1) 1 (128,1) 2 (128,2)
- * 3 (256,2) 4 (256,4) 5 (256,8) 6 (256,64) 7 (512,64); 64 lanes means one
- * wavefront per particle that queues the pairs in range and evaluates them
- * 64 at a time.  Shape 0 adds each particle's pair terms in the README
- * loop's order; the others in a fixed tree order. */
+ * Shapes (particles per force workgroup, lanes per particle): 0 (64,1)
+ * 1 (16,4) 2 (32,4) 3 (32,8) 4 (64,8) 5 (16,64) 6 (32,64) 7 (64,64); 64 lanes
+ * means one wavefront per particle that queues the pairs in range and
+ * evaluates them 64 at a time.  Shape 0 adds each particle's pair terms in
+ * the README loop's order; the others in a fixed tree order. */
 int sx_batch_autotune(sx_ctx *ctx, double dt, double damping, int first_step, int nrep, int *choice, double ms[8]);
 int sx_batch_set_shape(sx_ctx *ctx, int choice);
 

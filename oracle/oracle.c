@@ -550,11 +550,17 @@ static inline void quat_u(const or_sys *s, int i, double u[3])
     u[2] = w * w - x * x - y * y + z * z;
 }
 
-void or_gb(const or_sys *s, int i, int j, int variant, double cutoff, int mu, int nu,
-           double *pe, double f[3], double ti[3], double tj[3])
+/* gb_ (variant 0, force.f90:308-428), gbr_ (1, :430-533), gbw_ (2, welded:
+   :535-663), gbc_ (3, chiral: :665-843), gbwc_ (4, welded and chiral:
+   :845-1030).  The chiral forms regroup the plain terms (the energy factors
+   move into dR and the srx/srd factors out, :694-708) before adding the
+   chirality-weighted pseudoscalar terms (:712-759). */
+void or_gbx(const or_sys *s, int i, int j, int variant, double cutoff, double wf, double chirality, int mu, int nu,
+            double *pe, double f[3], double ti[3], double tj[3])
 {
     gb_par par;
     gb_pr pr;
+    const int welded = (variant == 2 || variant == 4), chiral = (variant == 3 || variant == 4);
     double pot = 0.0, FIJ[3] = {0, 0, 0}, T1[3] = {0, 0, 0}, T2[3] = {0, 0, 0};
     const double sigma_0 = 1.0;
     pair_rij(s, i, j, pr.r);
@@ -581,19 +587,23 @@ void or_gb(const or_sys *s, int i, int j, int variant, double cutoff, int mu, in
     double sigma_0_inv = 1.0 / sigma_0;
     double g32 = sigma / gchi;
     double switch_ = 0, factor = 0, sx = 0, sdx = 0, rlim;
-    if (variant == 0) {
-        switch_ = 0.9 * (sigma + cutoff - sigma_0);
-        factor = 1.0 / ((sigma + cutoff - sigma_0) - switch_);
+    if (variant != 1) {
+        rlim = welded ? (sigma + cutoff + wf - sigma_0) : (sigma + cutoff - sigma_0);
+        switch_ = 0.9 * rlim;
+        factor = 1.0 / (rlim - switch_);
         double xx = (rij_mag - switch_) * factor;
         sx = 1.0 - 6.0 * xx * xx * xx * xx * xx + 15.0 * xx * xx * xx * xx - 10.0 * xx * xx * xx;
         sdx = -30.0 * xx * xx * xx * xx + 60.0 * xx * xx * xx - 30.0 * xx * xx;
-        rlim = sigma + cutoff - sigma_0;
     } else {
         rlim = sigma;
     }
-    int inside = (variant == 0) ? (rij_mag < rlim) : (rij_mag <= rlim);
+    int inside = (variant == 1) ? (rij_mag <= rlim) : (rij_mag < rlim);
     if (inside) {
-        double sr = sigma_0 / (rij_mag - sigma + sigma_0);
+        double sr;
+        if (!welded) sr = sigma_0 / (rij_mag - sigma + sigma_0);
+        else if (rij_mag < sigma) sr = sigma_0 / (rij_mag - sigma + sigma_0);
+        else if (rij_mag >= sigma && rij_mag <= sigma + wf) sr = 1.0;
+        else sr = sigma_0 / (rij_mag - sigma + sigma_0 - wf);
         double sr2 = sr * sr, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
         double srx = (sr12 - 2.0 * sr6);
         if (variant == 1) srx = srx + 1.0;
@@ -608,23 +618,73 @@ void or_gb(const or_sys *s, int i, int j, int variant, double cutoff, int mu, in
         dG_du2(xhi, &par, &pr, dgu2_x);
         dG_du2(chi, &par, &pr, dgu2_c);
         double TI[3], TJ[3];
+        double dR_dr[3], de2_dr[3], de1_du1[3], de2_du1[3], dR_du1[3], de1_du2[3], de2_du2[3], dR_du2[3];
         for (int k = 0; k < 3; ++k) {
-            double dR_dr = sigma_0_inv * (rhat[k] + 0.5 * dgr_c[k] * g32);
-            double de2_dr = e1n * (mu * e2m1) * dgr_x[k];
-            double de1_du1 = e2m * (nu * e1n2) * (chi * chi * par.uu * pr.u2[k]);
-            double de2_du1 = e1n * (mu * e2m1) * dgu1_x[k];
-            double dR_du1 = srd * 0.5 * sigma_0_inv * dgu1_c[k] * g32;
-            double de1_du2 = e2m * (nu * e1n2) * (chi * chi * par.uu * pr.u1[k]);
-            double de2_du2 = e1n * (mu * e2m1) * dgu2_x[k];
-            double dR_du2 = srd * 0.5 * sigma_0_inv * dgu2_c[k] * g32;
-            FIJ[k] = (-1.0) * (e1n * e2m * srd * dR_dr + de2_dr * srx);
-            TI[k] = (-1.0) * (srx * (de1_du1 + de2_du1) + (e1n * e2m * dR_du1));
-            TJ[k] = (-1.0) * (srx * (de1_du2 + de2_du2) + (e1n * e2m * dR_du2));
+            de2_dr[k] = e1n * (mu * e2m1) * dgr_x[k];
+            de1_du1[k] = e2m * (nu * e1n2) * (chi * chi * par.uu * pr.u2[k]);
+            de2_du1[k] = e1n * (mu * e2m1) * dgu1_x[k];
+            de1_du2[k] = e2m * (nu * e1n2) * (chi * chi * par.uu * pr.u1[k]);
+            de2_du2[k] = e1n * (mu * e2m1) * dgu2_x[k];
+            if (!chiral) {
+                dR_dr[k] = sigma_0_inv * (rhat[k] + 0.5 * dgr_c[k] * g32);
+                dR_du1[k] = srd * 0.5 * sigma_0_inv * dgu1_c[k] * g32;
+                dR_du2[k] = srd * 0.5 * sigma_0_inv * dgu2_c[k] * g32;
+                FIJ[k] = (-1.0) * (e1n * e2m * srd * dR_dr[k] + de2_dr[k] * srx);
+                TI[k] = (-1.0) * (srx * (de1_du1[k] + de2_du1[k]) + (e1n * e2m * dR_du1[k]));
+                TJ[k] = (-1.0) * (srx * (de1_du2[k] + de2_du2[k]) + (e1n * e2m * dR_du2[k]));
+            } else {
+                dR_dr[k] = e1n * e2m * sigma_0_inv * (rhat[k] + 0.5 * dgr_c[k] * g32);
+                dR_du1[k] = e1n * e2m * 0.5 * sigma_0_inv * dgu1_c[k] * g32;
+                dR_du2[k] = e1n * e2m * 0.5 * sigma_0_inv * dgu2_c[k] * g32;
+                FIJ[k] = (-1.0) * (srx * de2_dr[k] + srd * dR_dr[k]);
+                TI[k] = (-1.0) * (srx * (de1_du1[k] + de2_du1[k]) + srd * dR_du1[k]);
+                TJ[k] = (-1.0) * (srx * (de1_du2[k] + de2_du2[k]) + srd * dR_du2[k]);
+            }
         }
         pot = e1n * e2m * srx;
         cross(pr.u1, TI, T1);
         cross(pr.u2, TJ, T2);
-        if (variant == 0 && (rij_mag >= switch_) && (rij_mag <= rlim)) {
+        if (chiral) {
+            double srxc, srdc;
+            if (welded) {
+                srxc = (sr12 - (12.0 / 7.0) * sr6 * sr);
+                srdc = 12.0 * (sr6 * sr2 - sr12 * sr);
+            } else {
+                srxc = sr6 * sr;
+                srdc = -7.0 * sr6 * sr2;
+            }
+            double u1xu2[3], u2xrij[3], rijxu1[3];
+            cross(pr.u1, pr.u2, u1xu2);
+            cross(pr.u2, rhat, u2xrij);
+            cross(rhat, pr.u1, rijxu1);
+            double uur = u1xu2[0] * rhat[0] + u1xu2[1] * rhat[1] + u1xu2[2] * rhat[2];
+            double uu = par.uu;
+            double FC[3], TIc[3], TJc[3], T1c[3], T2c[3];
+            for (int k = 0; k < 3; ++k) {
+                double dRr = dR_dr[k] * srdc * uur * uu;
+                double de2r = de2_dr[k] * srxc * uur * uu;
+                double duur_r = e1n * e2m * srxc * uu * (u1xu2[k] - uur * rhat[k]) / rij_mag;
+                double a1 = de1_du1[k] * srxc * uur * uu, b1 = de2_du1[k] * srxc * uur * uu;
+                double c1 = dR_du1[k] * srdc * uur * uu;
+                double duur1 = e1n * e2m * srxc * uu * u2xrij[k], duu1 = e1n * e2m * srxc * uur * pr.u2[k];
+                double a2 = de1_du2[k] * srxc * uur * uu, b2 = de2_du2[k] * srxc * uur * uu;
+                double c2 = dR_du2[k] * srdc * uur * uu;
+                double duur2 = e1n * e2m * srxc * uu * rijxu1[k], duu2 = e1n * e2m * srxc * uur * pr.u1[k];
+                FC[k] = (-1.0) * (de2r + dRr + duur_r);
+                TIc[k] = (-1.0) * (a1 + b1 + c1 + duur1 + duu1);
+                TJc[k] = (-1.0) * (a2 + b2 + c2 + duur2 + duu2);
+            }
+            cross(pr.u1, TIc, T1c);
+            cross(pr.u2, TJc, T2c);
+            double potc = e1n * e2m * srxc * uur * uu;
+            pot = pot + chirality * potc;
+            for (int k = 0; k < 3; ++k) {
+                FIJ[k] = FIJ[k] + chirality * FC[k];
+                T1[k] = T1[k] + chirality * T1c[k];
+                T2[k] = T2[k] + chirality * T2c[k];
+            }
+        }
+        if (variant != 1 && (rij_mag >= switch_) && (rij_mag <= rlim)) {
             for (int k = 0; k < 3; ++k) {
                 FIJ[k] = sx * FIJ[k] - sdx * pot * factor;
                 T1[k] = sx * T1[k] - sdx * pot * factor;
@@ -635,6 +695,12 @@ void or_gb(const or_sys *s, int i, int j, int variant, double cutoff, int mu, in
     }
     *pe = pot;
     for (int k = 0; k < 3; ++k) { f[k] = FIJ[k]; ti[k] = T1[k]; tj[k] = T2[k]; }
+}
+
+void or_gb(const or_sys *s, int i, int j, int variant, double cutoff, int mu, int nu,
+           double *pe, double f[3], double ti[3], double tj[3])
+{
+    or_gbx(s, i, j, variant, cutoff, 0.0, 0.0, mu, nu, pe, f, ti, tj);
 }
 
 /* ------------------------------------------------------------------ */
@@ -727,10 +793,14 @@ static double max3(double a, double b, double c) { return fmax(fmax(a, b), c); }
 /* lambda_in < 0: the reference's Brent solve; otherwise the contact
    parameter is taken as given (used to evaluate the force expressions at a
    converged lambda*, where the envelope theorem makes F = -grad U exact). */
-static void ecp_eval(const or_sys *s, int i, int j, int variant, double cutoff, double lambda_in,
-                     double *pe, double f[3], double ti[3], double tj[3],
+/* ecp_ (variant 0, force.f90:1202-1385), ecpr_ (1, :1387-1551), ecpw_ (2,
+   welded: :1553-1742), ecpc_ (3, chiral: :1744-1984), ecpwc_ (4, welded and
+   chiral: :1986-2234); lambda_in >= 0 replaces the Brent solve */
+static void ecp_eval(const or_sys *s, int i, int j, int variant, double cutoff, double wf, double chirality,
+                     double lambda_in, double *pe, double f[3], double ti[3], double tj[3],
                      double *lambda_out, int *niter_out)
 {
+    const int welded = (variant == 2 || variant == 4), chiral = (variant == 3 || variant == 4);
     const int meu = 2, neu = 1; /* force.f90:1038 */
     const double sigma_0 = 1.0;
     double pot = 0.0, FIJ[3] = {0, 0, 0}, T1[3] = {0, 0, 0}, T2[3] = {0, 0, 0};
@@ -790,19 +860,23 @@ static void ecp_eval(const or_sys *s, int i, int j, int variant, double cutoff, 
     double sigma_0_inv = 1.0 / sigma_0;
     double phi_inv_3by2 = sigma * sigma * sigma;
     double switch_ = 0, factor = 0, sx = 0, sdx = 0, rlim;
-    if (variant == 0) {
-        switch_ = 0.9 * (sigma + cutoff - sigma_0);
-        factor = 1.0 / ((sigma + cutoff - sigma_0) - switch_);
+    if (variant != 1) {
+        rlim = welded ? (sigma + cutoff + wf - sigma_0) : (sigma + cutoff - sigma_0);
+        switch_ = 0.9 * rlim;
+        factor = 1.0 / (rlim - switch_);
         double xx = (rij_mag - switch_) * factor;
         sx = 1.0 - 6.0 * xx * xx * xx * xx * xx + 15.0 * xx * xx * xx * xx - 10.0 * xx * xx * xx;
         sdx = -30.0 * xx * xx * xx * xx + 60.0 * xx * xx * xx - 30.0 * xx * xx;
-        rlim = sigma + cutoff - sigma_0;
     } else {
         rlim = sigma;
     }
-    int inside = (variant == 0) ? (rij_mag < rlim) : (rij_mag <= rlim);
+    int inside = (variant == 1) ? (rij_mag <= rlim) : (rij_mag < rlim);
     if (inside) {
-        double sr = sigma_0 / (rij_mag - sigma + sigma_0);
+        double sr;
+        if (!welded) sr = sigma_0 / (rij_mag - sigma + sigma_0);
+        else if (rij_mag < sigma) sr = sigma_0 / (rij_mag - sigma + sigma_0);
+        else if (rij_mag >= sigma && rij_mag <= sigma + wf) sr = 1.0;
+        else sr = sigma_0 / (rij_mag - sigma + sigma_0 - wf);
         double sr2 = sr * sr, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
         double srx = (sr12 - 2.0 * sr6);
         if (variant == 1) srx = srx + 1.0;
@@ -812,11 +886,15 @@ static void ecp_eval(const or_sys *s, int i, int j, int variant, double cutoff, 
         double dotKE = rhat[0] * KE[0] + rhat[1] * KE[1] + rhat[2] * KE[2];
         double dR_dr[3], de2_dr[3];
         for (int k = 0; k < 3; ++k) {
-            dR_dr[k] = sigma_0_inv * (rhat[k] + 0.5 * rfac * (KR[k] - dotKR * rhat[k]) * phi_inv_3by2);
+            if (chiral)
+                dR_dr[k] = e1n * e2m * sigma_0_inv * (rhat[k] + 0.5 * rfac * (KR[k] - dotKR * rhat[k]) * phi_inv_3by2);
+            else
+                dR_dr[k] = sigma_0_inv * (rhat[k] + 0.5 * rfac * (KR[k] - dotKR * rhat[k]) * phi_inv_3by2);
             de2_dr[k] = e1n * (meu * e2m1) * efac * (KE[k] - dotKE * rhat[k]);
         }
         double de1_u1[3] = {0, 0, 0}, de1_u2[3] = {0, 0, 0};
-        double pref1 = e2m * (neu * e1n) * srx;
+        /* the chiral forms leave srx_fac / srd_fac out of the derivative terms */
+        double pref1 = chiral ? e2m * (neu * e1n) : e2m * (neu * e1n) * srx;
         double d1x[3], d1y[3], d1z[3], tmp[3], col[3];
         for (int c = 0; c < 3; ++c) {
             for (int k = 0; k < 3; ++k) col[k] = MM1[k + 3 * c];
@@ -833,8 +911,13 @@ static void ecp_eval(const or_sys *s, int i, int j, int variant, double cutoff, 
         cross(KR_AR, KR, cr);
         double de2_u1[3], dR_u1[3];
         for (int k = 0; k < 3; ++k) {
-            de2_u1[k] = e1n * (meu * e2m1) * srx * (1 - lambda_E) * efac * ce[k];
-            dR_u1[k] = e1n * e2m * srd * (0.5 * sigma_0_inv) * (1 - lambda_R) * rfac * cr[k] * phi_inv_3by2;
+            if (chiral) {
+                de2_u1[k] = e1n * (meu * e2m1) * (1 - lambda_E) * efac * ce[k];
+                dR_u1[k] = e1n * e2m * (0.5 * sigma_0_inv) * (1 - lambda_R) * rfac * cr[k] * phi_inv_3by2;
+            } else {
+                de2_u1[k] = e1n * (meu * e2m1) * srx * (1 - lambda_E) * efac * ce[k];
+                dR_u1[k] = e1n * e2m * srd * (0.5 * sigma_0_inv) * (1 - lambda_R) * rfac * cr[k] * phi_inv_3by2;
+            }
         }
         for (int c = 0; c < 3; ++c) {
             for (int k = 0; k < 3; ++k) col[k] = MM2[k + 3 * c];
@@ -851,16 +934,74 @@ static void ecp_eval(const or_sys *s, int i, int j, int variant, double cutoff, 
         cross(KR_BR, KR, cr);
         double de2_u2[3], dR_u2[3];
         for (int k = 0; k < 3; ++k) {
-            de2_u2[k] = e1n * (meu * e2m1) * srx * lambda_E * efac * ce[k];
-            dR_u2[k] = e1n * e2m * srd * (0.5 * sigma_0_inv) * lambda_R * rfac * cr[k] * phi_inv_3by2;
+            if (chiral) {
+                de2_u2[k] = e1n * (meu * e2m1) * lambda_E * efac * ce[k];
+                dR_u2[k] = e1n * e2m * (0.5 * sigma_0_inv) * lambda_R * rfac * cr[k] * phi_inv_3by2;
+            } else {
+                de2_u2[k] = e1n * (meu * e2m1) * srx * lambda_E * efac * ce[k];
+                dR_u2[k] = e1n * e2m * srd * (0.5 * sigma_0_inv) * lambda_R * rfac * cr[k] * phi_inv_3by2;
+            }
         }
         pot = e1n * e2m * srx;
         for (int k = 0; k < 3; ++k) {
-            FIJ[k] = (-1.0) * (e1n * e2m * srd * dR_dr[k] + srx * de2_dr[k]);
-            T1[k] = (de1_u1[k] + de2_u1[k]) + dR_u1[k];
-            T2[k] = (de1_u2[k] + de2_u2[k]) + dR_u2[k];
+            if (chiral) {
+                FIJ[k] = (-1.0) * (srx * de2_dr[k] + srd * dR_dr[k]);
+                T1[k] = (srx * (de1_u1[k] + de2_u1[k]) + srd * dR_u1[k]);
+                T2[k] = (srx * (de1_u2[k] + de2_u2[k]) + srd * dR_u2[k]);
+            } else {
+                FIJ[k] = (-1.0) * (e1n * e2m * srd * dR_dr[k] + srx * de2_dr[k]);
+                T1[k] = (de1_u1[k] + de2_u1[k]) + dR_u1[k];
+                T2[k] = (de1_u2[k] + de2_u2[k]) + dR_u2[k];
+            }
         }
-        if (variant == 0 && (rij_mag >= switch_) && (rij_mag <= rlim)) {
+        if (chiral) {
+            /* force.f90:1916-1968 (ecpc_), :2170-2222 (ecpwc_) */
+            double srxc, srdc;
+            if (welded) {
+                srxc = (sr12 - (12.0 / 7.0) * sr6 * sr);
+                srdc = 12.0 * (sr6 * sr2 - sr12 * sr);
+            } else {
+                srxc = sr6 * sr;
+                srdc = -7.0 * sr6 * sr2;
+            }
+            double u1z[3] = {M(U1, 3, 1), M(U1, 3, 2), M(U1, 3, 3)};
+            double u2z[3] = {M(U2, 3, 1), M(U2, 3, 2), M(U2, 3, 3)};
+            double u1xu2[3], u2xrij[3], rijxu1[3];
+            cross(u1z, u2z, u1xu2);
+            cross(u2z, rhat, u2xrij);
+            cross(rhat, u1z, rijxu1);
+            double uur = u1xu2[0] * rhat[0] + u1xu2[1] * rhat[1] + u1xu2[2] * rhat[2];
+            double uu = u1z[0] * u2z[0] + u1z[1] * u2z[1] + u1z[2] * u2z[2];
+            double term1[3], term2[3], tu1[3], tu2[3], FC[3], T1c[3], T2c[3];
+            for (int k = 0; k < 3; ++k) {
+                double duur1 = e1n * e2m * srxc * uu * u2xrij[k], duu1 = e1n * e2m * srxc * uur * u2z[k];
+                double duur2 = e1n * e2m * srxc * uu * rijxu1[k], duu2 = e1n * e2m * srxc * uur * u1z[k];
+                term1[k] = (duur1 + duu1);
+                term2[k] = (duur2 + duu2);
+            }
+            cross(u1z, term1, tu1);
+            cross(u2z, term2, tu2);
+            for (int k = 0; k < 3; ++k) {
+                double dRr = dR_dr[k] * srdc * uur * uu;
+                double de2r = de2_dr[k] * srxc * uur * uu;
+                double duur_r = e1n * e2m * srxc * uu * (u1xu2[k] - uur * rhat[k]) / rij_mag;
+                double a1 = de1_u1[k] * srxc * uur * uu, b1 = de2_u1[k] * srxc * uur * uu;
+                double c1 = dR_u1[k] * srdc * uur * uu;
+                double a2 = de1_u2[k] * srxc * uur * uu, b2 = de2_u2[k] * srxc * uur * uu;
+                double c2 = dR_u2[k] * srdc * uur * uu;
+                FC[k] = (-1.0) * (de2r + dRr + duur_r);
+                T1c[k] = (a1 + b1 + c1 - tu1[k]);
+                T2c[k] = (a2 + b2 + c2 - tu2[k]);
+            }
+            double potc = e1n * e2m * srxc * uur * uu;
+            pot = pot + chirality * potc;
+            for (int k = 0; k < 3; ++k) {
+                FIJ[k] = FIJ[k] + chirality * FC[k];
+                T1[k] = T1[k] + chirality * T1c[k];
+                T2[k] = T2[k] + chirality * T2c[k];
+            }
+        }
+        if (variant != 1 && (rij_mag >= switch_) && (rij_mag <= rlim)) {
             for (int k = 0; k < 3; ++k) {
                 FIJ[k] = sx * FIJ[k] - sdx * pot * factor;
                 T1[k] = sx * T1[k] - sdx * pot * factor;
@@ -877,13 +1018,25 @@ void or_ecp(const or_sys *s, int i, int j, int variant, double cutoff,
             double *pe, double f[3], double ti[3], double tj[3],
             double *lambda_out, int *niter_out)
 {
-    ecp_eval(s, i, j, variant, cutoff, -1.0, pe, f, ti, tj, lambda_out, niter_out);
+    ecp_eval(s, i, j, variant, cutoff, 0.0, 0.0, -1.0, pe, f, ti, tj, lambda_out, niter_out);
+}
+
+void or_ecpx(const or_sys *s, int i, int j, int variant, double cutoff, double wf, double chirality,
+             double *pe, double f[3], double ti[3], double tj[3], double *lambda_out, int *niter_out)
+{
+    ecp_eval(s, i, j, variant, cutoff, wf, chirality, -1.0, pe, f, ti, tj, lambda_out, niter_out);
 }
 
 void or_ecp_at_lambda(const or_sys *s, int i, int j, int variant, double cutoff, double lambda,
                       double *pe, double f[3], double ti[3], double tj[3])
 {
-    ecp_eval(s, i, j, variant, cutoff, lambda, pe, f, ti, tj, NULL, NULL);
+    ecp_eval(s, i, j, variant, cutoff, 0.0, 0.0, lambda, pe, f, ti, tj, NULL, NULL);
+}
+
+void or_ecpx_at_lambda(const or_sys *s, int i, int j, int variant, double cutoff, double wf, double chirality,
+                       double lambda, double *pe, double f[3], double ti[3], double tj[3])
+{
+    ecp_eval(s, i, j, variant, cutoff, wf, chirality, lambda, pe, f, ti, tj, NULL, NULL);
 }
 
 /* the pair's shape matrices AR, BR (force.f90:1254-1259) and min-image
@@ -1504,6 +1657,12 @@ void or_pair_loop(or_sys *s, const or_nlist *nl, int kind, const double *p)
             case 8: or_ecp(s, i, j, 0, p[0], &pe, f, ti, tj, NULL, NULL); torque = 1; break;
             case 9: or_ecp(s, i, j, 1, 0.0, &pe, f, ti, tj, NULL, NULL); torque = 1; break;
             case 10: or_dh(s, i, j, p[0], p[1], p[2], &pe, f); break;
+            case 11: or_gbx(s, i, j, 2, p[0], p[3], 0.0, (int)p[1], (int)p[2], &pe, f, ti, tj); torque = 1; break;
+            case 12: or_gbx(s, i, j, 3, p[0], 0.0, p[3], (int)p[1], (int)p[2], &pe, f, ti, tj); torque = 1; break;
+            case 13: or_gbx(s, i, j, 4, p[0], p[3], p[4], (int)p[1], (int)p[2], &pe, f, ti, tj); torque = 1; break;
+            case 14: ecp_eval(s, i, j, 2, p[0], p[1], 0.0, -1.0, &pe, f, ti, tj, NULL, NULL); torque = 1; break;
+            case 15: ecp_eval(s, i, j, 3, p[0], 0.0, p[1], -1.0, &pe, f, ti, tj, NULL, NULL); torque = 1; break;
+            case 16: ecp_eval(s, i, j, 4, p[0], p[1], p[2], -1.0, &pe, f, ti, tj, NULL, NULL); torque = 1; break;
             default: return;
             }
             s->pot_eng = s->pot_eng + pe;

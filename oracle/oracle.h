@@ -91,11 +91,19 @@ void or_dh(const or_sys *s, int i, int j, double cutoff, double db_len, double d
            double *pe, double f[3]);
 /* Gay-Berne; mu, nu are the exponents the reference fixes to (1, 2)
  * (force.f90:240).  variant: 0 gb (cutoff + smooth switch), 1 gbr (repulsive). */
+/* variant 0 gb_, 1 gbr_, 2 gbw_, 3 gbc_, 4 gbwc_ */
+void or_gbx(const or_sys *s, int i, int j, int variant, double cutoff, double wf, double chirality, int mu, int nu,
+            double *pe, double f[3], double ti[3], double tj[3]);
 void or_gb(const or_sys *s, int i, int j, int variant, double cutoff, int mu, int nu,
            double *pe, double f[3], double ti[3], double tj[3]);
 /* ECP (Perram-Wertheim).  variant: 0 ecp (cutoff + switch), 1 ecpr (repulsive).
  * lambda_out (may be NULL) receives the Brent lambda; niter_out the iteration
  * count of the Brent loop. */
+/* variant 0 ecp_, 1 ecpr_, 2 ecpw_, 3 ecpc_, 4 ecpwc_ */
+void or_ecpx(const or_sys *s, int i, int j, int variant, double cutoff, double wf, double chirality,
+             double *pe, double f[3], double ti[3], double tj[3], double *lambda_out, int *niter_out);
+void or_ecpx_at_lambda(const or_sys *s, int i, int j, int variant, double cutoff, double wf, double chirality,
+                       double lambda, double *pe, double f[3], double ti[3], double tj[3]);
 void or_ecp(const or_sys *s, int i, int j, int variant, double cutoff,
             double *pe, double f[3], double ti[3], double tj[3],
             double *lambda_out, int *niter_out);

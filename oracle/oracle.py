@@ -39,7 +39,7 @@ SCALARS = [
 SIDX = {s: k for k, s in enumerate(SCALARS)}
 
 # force kinds (include/sucrose.h SX_KIND_*)
-LJ, LJ12, LJW, WCA, MORSE, GB, GBR, ECP, ECPR, DH = range(1, 11)
+LJ, LJ12, LJW, WCA, MORSE, GB, GBR, ECP, ECPR, DH, GBW, GBC, GBWC, ECPW, ECPC, ECPWC = range(1, 17)
 # integrator kinds used by ref_integrate
 VV, VVC, QQ, NHT, NHTC, LGV, LGVC = range(1, 8)
 
@@ -227,11 +227,20 @@ class Oracle:
             nu = int(p[2]) if len(p) > 2 else 2
             L.or_gb(C.byref(s), i, j, 0 if kind == GB else 1, d(cut), mu, nu,
                     C.byref(pe), fp, tip, tjp)
-        elif kind in (ECP, ECPR):
-            cut = p[0] if kind == ECP else 0.0
+        elif kind in (GBW, GBC, GBWC):
+            # p = cutoff, mu, nu, then wf (GBW, GBWC) and chirality (GBC, GBWC)
+            wf = p[3] if kind in (GBW, GBWC) else 0.0
+            ch = p[3] if kind == GBC else (p[4] if kind == GBWC else 0.0)
+            L.or_gbx(C.byref(s), i, j, {GBW: 2, GBC: 3, GBWC: 4}[kind], d(p[0]), d(wf), d(ch), int(p[1]), int(p[2]),
+                     C.byref(pe), fp, tip, tjp)
+        elif kind in (ECP, ECPR, ECPW, ECPC, ECPWC):
+            # p = cutoff, then wf (ECPW, ECPWC) and chirality (ECPC, ECPWC)
+            cut = 0.0 if kind == ECPR else p[0]
+            wf = p[1] if kind in (ECPW, ECPWC) else 0.0
+            ch = p[1] if kind == ECPC else (p[2] if kind == ECPWC else 0.0)
             lam = C.c_double(); nit = C.c_int()
-            L.or_ecp(C.byref(s), i, j, 0 if kind == ECP else 1, d(cut), C.byref(pe), fp, tip, tjp,
-                     C.byref(lam), C.byref(nit))
+            L.or_ecpx(C.byref(s), i, j, {ECP: 0, ECPR: 1, ECPW: 2, ECPC: 3, ECPWC: 4}[kind], d(cut), d(wf), d(ch),
+                      C.byref(pe), fp, tip, tjp, C.byref(lam), C.byref(nit))
             return pe.value, f, ti, tj, lam.value, nit.value
         else:
             raise ValueError(kind)
@@ -271,13 +280,16 @@ class Oracle:
         return x
 
     def ecp_at_lambda(self, kind, st: State, i, j, p, lam):
-        """ECP/ECPR pair terms evaluated at a given contact parameter."""
+        """ECP-family pair terms evaluated at a given contact parameter."""
         s = self._sys(st)
         pe = C.c_double()
         f = np.zeros(3); ti = np.zeros(3); tj = np.zeros(3)
-        cut = p[0] if kind == ECP else 0.0
-        self.lib.or_ecp_at_lambda(C.byref(s), int(i), int(j), 0 if kind == ECP else 1, C.c_double(cut),
-                                  C.c_double(lam), C.byref(pe), _dp(f), _dp(ti), _dp(tj))
+        cut = 0.0 if kind == ECPR else p[0]
+        wf = p[1] if kind in (ECPW, ECPWC) else 0.0
+        ch = p[1] if kind == ECPC else (p[2] if kind == ECPWC else 0.0)
+        d = C.c_double
+        self.lib.or_ecpx_at_lambda(C.byref(s), int(i), int(j), {ECP: 0, ECPR: 1, ECPW: 2, ECPC: 3, ECPWC: 4}[kind],
+                                   d(cut), d(wf), d(ch), d(lam), C.byref(pe), _dp(f), _dp(ti), _dp(tj))
         return pe.value, f, ti, tj
 
     def bonded(self, kind, st: State, idx, coeff, angle=0.0):

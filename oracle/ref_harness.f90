@@ -219,6 +219,18 @@ contains
             call ecpr_calculate_forces(i1, j1, pe, f, ti, tj)
         case (10)
             call dh_calculate_forces(i1, j1, p(1), p(2), p(3), pe, f)
+        case (11)   ! p = cutoff, (mu, nu fixed by the reference), wf
+            call gbw_calculate_forces(i1, j1, p(1), pe, f, ti, tj, p(4))
+        case (12)   ! p = cutoff, -, -, chirality
+            call gbc_calculate_forces(i1, j1, p(1), pe, f, ti, tj, p(4))
+        case (13)   ! p = cutoff, -, -, wf, chirality
+            call gbwc_calculate_forces(i1, j1, p(1), pe, f, ti, tj, p(4), p(5))
+        case (14)   ! p = cutoff, wf
+            call ecpw_calculate_forces(i1, j1, p(1), pe, f, ti, tj, p(2))
+        case (15)   ! p = cutoff, chirality
+            call ecpc_calculate_forces(i1, j1, p(1), pe, f, ti, tj, p(2))
+        case (16)   ! p = cutoff, wf, chirality
+            call ecpwc_calculate_forces(i1, j1, p(1), pe, f, ti, tj, p(2), p(3))
         end select
     end subroutine ref_pair
 

@@ -415,7 +415,8 @@ class System:
         return dict(zip(keys, o.tolist()))
 
     # ---- batched replicas ----
-    BATCH_SHAPES = ((64, 1), (128, 1), (128, 2), (256, 2), (256, 4), (256, 8), (256, 64), (512, 64))
+    # (particles per force workgroup, lanes per particle)
+    BATCH_SHAPES = ((64, 1), (16, 4), (32, 4), (32, 8), (64, 8), (16, 64), (32, 64), (64, 64))
 
     def batch_set(self, counts, boxes, kinds, params, temps):
         """Splits the particles into len(counts) independent systems (contiguous

@@ -1,19 +1,25 @@
-// Batched replicas: many independent small systems advanced together, one
-// workgroup per system and ONE kernel launch per step (captured in a HIP
-// graph and replayed).  Each system is a contiguous range of the context's
-// particle arrays with its own cubic periodic box, pair potential (LJ or
-// Gay-Berne), parameters and Langevin temperature.  Per system and step the
-// workgroup runs the reference's loop (README.md:83-115):
+// Batched replicas: many independent small systems advanced together, each a
+// contiguous range of the context's particle arrays with its own cubic
+// periodic box, pair potential (LJ or Gay-Berne), parameters and Langevin
+// temperature.  Per system and step the reference's loop (README.md:83-115):
 // lgv_initial_step, initialize_forces, the pair loop over all pairs I < J of
 // the system (the README's own double loop; the cutoff of each potential does
 // the screening, so no neighbour list is built), lgv_final_step and
 // calculate_thermo's KE / rotational KE / PE / T for that system.
 //
+// A step is three launches, captured once in a HIP graph and replayed:
+//   k_batch_initial  one thread per particle: lgv_initial_step, then the
+//                    particle's row of its system's table;
+//   k_batch_forces   workgroups over (system, slice of its particles), largest
+//                    work first, so a large system spreads over several CUs;
+//                    each stages its system's table in LDS and gathers the
+//                    forces, torques and pair energies of its slice;
+//   k_batch_final    one workgroup per system: lgv_final_step and the thermo
+//                    sums; the last workgroup advances the step counter.
 // Forces are gathered per particle (LPP lanes per particle, combined with
-// cross-lane shuffles in a fixed order): no atomics, deterministic results.
-// Systems are dealt to workgroups largest first so the hardware scheduler
-// balances the uneven sizes.  The launch shape (workgroup size, lanes per
-// particle) is picked by sx_batch_autotune from timed trial steps.
+// cross-lane shuffles in a fixed order, or one compacting wavefront per
+// particle): no atomics, deterministic results.  The slice size and lanes per
+// particle are picked by sx_batch_autotune from timed trial steps.
 #include "sx_steps.h"
 
 #include <algorithm>
@@ -34,13 +40,17 @@ struct BatchSys {
 struct BatchState {
     int nsys = 0;
     std::vector<BatchSys> h_sys;
-    std::vector<int> h_order;
     BatchSys *d_sys = nullptr;
-    int *d_order = nullptr;
+    int *d_sys_of = nullptr;       // system of each particle
+    double *d_tab = nullptr;       // per-particle tables, TB_N doubles per particle
+    double *d_pe = nullptr;        // per particle: its pair energies (pairs where it is I)
+    int4 *d_blocks = nullptr;      // force workgroups: system, first, end particle
+    int nblocks = 0, blocks_choice = -1, blocks_cap = 0;
     double *d_out = nullptr;       // per system: KE, KErot, PE, T
     unsigned *d_ticket = nullptr;
     int *d_step = nullptr;
-    int choice = 3;
+    int choice = 6;
+    int cap = 0;                   // largest system
     double dt = -1, damp = -1;     // constants currently uploaded
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -48,19 +58,24 @@ struct BatchState {
     double graph_dt = -1, graph_damp = -1;   // kernel arguments baked into the graph
     int n_total = 0;                         // particle count and field storage at sx_batch_set
     const double *fields = nullptr;
-    int cap = 0;                             // largest system
-    bool lds = true;                         // per-particle table in LDS (else d_tab)
-    double *d_tab = nullptr;
 };
 
-// (workgroup size, lanes per particle); 64 lanes = one compacting wavefront per particle
-static const int BATCH_SHAPES[][2] = {{64, 1}, {128, 1}, {128, 2}, {256, 2}, {256, 4}, {256, 8}, {256, 64}, {512, 64}};
+// (particles per force workgroup, lanes per particle); 64 lanes = one
+// compacting wavefront per particle.  Shape 0 adds each particle's pair terms
+// in the README loop's order.
+static const int BATCH_SHAPES[][2] = {{64, 1}, {16, 4}, {32, 4}, {32, 8}, {64, 8}, {16, 64}, {32, 64}, {64, 64}};
 static const int BATCH_NSHAPES = 8;
+static int shape_threads(int choice)
+{
+    const int P = BATCH_SHAPES[choice][0], LPP = BATCH_SHAPES[choice][1];
+    return LPP == 64 ? 256 : std::min(256, P * LPP);
+}
 
-// Per-particle table of the system being stepped, filled after
-// lgv_initial_step: position, and for Gay-Berne the factors of gb_ that depend
-// on one particle only (GbSelf) plus its reach, an upper bound on the distance
-// at which the pair with that particle as I can interact:
+// Per-particle table of every system, written right after lgv_initial_step
+// (one SoA block of TB_N rows per system, row length N_s): position, and for
+// Gay-Berne the factors of gb_ that depend on one particle only (GbSelf) plus
+// its reach, an upper bound on the distance at which the pair with that
+// particle as I can interact:
 // max(sig_x, sig_z) + cutoff - sigma_0 bounds gb_'s sigma + cutoff - sigma_0
 // because sigma(u1, u2, r) never exceeds the larger of the two diameters.
 enum { TB_X, TB_Y, TB_Z, TB_U0, TB_U1, TB_U2, TB_CHI, TB_XHI, TB_LPRP, TB_EPSZ, TB_REACH, TB_N };
@@ -144,58 +159,69 @@ __device__ __forceinline__ void batch_eval(const double *tab, int ld, int q, int
     }
 }
 
-template <int BT, int LPP, bool LDS>
-__global__ void __launch_bounds__(BT) k_batch_step(DState s, const BatchSys *sys, const int *order, double dt,
-                                                   double damp, int step, int *dstep, double *out, unsigned *ticket,
-                                                   int nsys, double *gtab, int cap)
+// lgv_initial_step of particle i (Philox counter: its 1-based index in its
+// system) and its table row
+__global__ void __launch_bounds__(256) k_batch_initial(DState s, const BatchSys *sys, const int *sys_of, double dt,
+                                                       double damp, const int *dstep, double *tab)
 {
-    extern __shared__ double lds_tab[];
-    __shared__ double red[BT][3];
-    __shared__ int queue[LPP == 64 ? BT / 64 : 1][128];   // per-wavefront pair queue (LPP = 64)
-    const int sid = order[blockIdx.x];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    const int sid = sys_of[i];
     const BatchSys S = sys[sid];
     DState ls = s;
     ls.box[0] = ls.box[1] = ls.box[2] = S.L;
     ls.periodic = 1;
-    const int st = dstep ? *(volatile const int *)dstep : step;
     LgvConst k;
     k.dt = dt; k.damp = damp; k.T = S.T; k.e1 = S.e1; k.e2 = S.e2; k.kk1 = S.kk1; k.kk2 = S.kk2;
     k.temp_int = S.temp_int; k.save = 0;
-    // lgv_initial_step; the Philox counter is the particle's index in its system
-    for (int q = threadIdx.x; q < S.n; q += BT) lgv_initial_particle(ls, S.off + q, q + 1, k, st);
-    __syncthreads();
-    double *tab = LDS ? lds_tab : gtab + (size_t)TB_N * S.off;
-    const int ld = LDS ? cap : S.n;
+    const int q = i - S.off;
+    lgv_initial_particle(ls, i, q + 1, k, *dstep);
+    double *t = tab + (size_t)TB_N * S.off;
+    const int ld = S.n;
+    t[TB_X * ld + q] = s.f[SX_RX][i];
+    t[TB_Y * ld + q] = s.f[SX_RY][i];
+    t[TB_Z * ld + q] = s.f[SX_RZ][i];
+    if (S.kind == SX_GB) {
+        GbSelf g;
+        gb_self(ls, i, (int)S.p[1], g);
+        t[TB_U0 * ld + q] = g.u[0];
+        t[TB_U1 * ld + q] = g.u[1];
+        t[TB_U2 * ld + q] = g.u[2];
+        t[TB_CHI * ld + q] = g.chi;
+        t[TB_XHI * ld + q] = g.xhi;
+        t[TB_LPRP * ld + q] = g.lprp;
+        t[TB_EPSZ * ld + q] = g.epsz;
+        t[TB_REACH * ld + q] = (fmax(s.f[SX_SIG_X][i], s.f[SX_SIG_Z][i]) + S.p[0] - 1.0) * (1.0 + 1e-9);
+    }
+}
+
+// initialize_forces + the pair loop for particles [q0, q1) of one system
+template <int BT, int LPP, bool LDS>
+__global__ void __launch_bounds__(BT) k_batch_forces(DState s, const BatchSys *sys, const int4 *blocks,
+                                                     const double *gtab, double *pe)
+{
+    extern __shared__ double lds_tab[];
+    __shared__ int queue[LPP == 64 ? BT / 64 : 1][128];   // per-wavefront pair queue (LPP = 64)
+    const int4 blk = blocks[blockIdx.x];
+    const BatchSys S = sys[blk.x];
+    const int q0 = blk.y, q1 = blk.z;
     const bool gb = S.kind == SX_GB;
     const int mu = (int)S.p[1], nu = (int)S.p[2];
     const double L = S.L;
-    for (int q = threadIdx.x; q < S.n; q += BT) {
-        const int i = S.off + q;
-        tab[TB_X * ld + q] = s.f[SX_RX][i];
-        tab[TB_Y * ld + q] = s.f[SX_RY][i];
-        tab[TB_Z * ld + q] = s.f[SX_RZ][i];
-        if (gb) {
-            GbSelf g;
-            gb_self(ls, i, mu, g);
-            tab[TB_U0 * ld + q] = g.u[0];
-            tab[TB_U1 * ld + q] = g.u[1];
-            tab[TB_U2 * ld + q] = g.u[2];
-            tab[TB_CHI * ld + q] = g.chi;
-            tab[TB_XHI * ld + q] = g.xhi;
-            tab[TB_LPRP * ld + q] = g.lprp;
-            tab[TB_EPSZ * ld + q] = g.epsz;
-            tab[TB_REACH * ld + q] = (fmax(s.f[SX_SIG_X][i], s.f[SX_SIG_Z][i]) + S.p[0] - 1.0) * (1.0 + 1e-9);
-        }
+    const int ld = S.n;
+    const double *src = gtab + (size_t)TB_N * S.off;
+    const double *tab = src;
+    if (LDS) {
+        const int rows = gb ? TB_N : 3;
+        for (int e = threadIdx.x; e < rows * ld; e += BT) lds_tab[e] = src[e];
+        __syncthreads();
+        tab = lds_tab;
     }
-    __syncthreads();
-    // initialize_forces + the pair loop, gathered per particle
-    double pe_loc = 0.0;
     if constexpr (LPP < 64) {
-        // LPP lanes per particle; for particle q the contributions arrive in
-        // the README loop's order (pairs (J, q), J < q, then (q, J), J > q)
-        // when LPP = 1
+        // LPP lanes per particle; with LPP = 1 a particle's terms arrive in the
+        // README loop's order (pairs (J, q), J < q, then (q, J), J > q)
         const int lane = threadIdx.x % LPP, grp = threadIdx.x / LPP;
-        for (int q = grp; q < S.n; q += BT / LPP) {
+        for (int q = q0 + grp; q < q1; q += BT / LPP) {
             const int i = S.off + q;
             BatchSelf me;
             batch_self(tab, ld, q, gb, S.p[2], me);
@@ -211,7 +237,7 @@ __global__ void __launch_bounds__(BT) k_batch_step(DState s, const BatchSys *sys
             if (lane == 0) {
                 s.f[SX_FX][i] = acc[0]; s.f[SX_FY][i] = acc[1]; s.f[SX_FZ][i] = acc[2];
                 s.f[SX_TX][i] = acc[3]; s.f[SX_TY][i] = acc[4]; s.f[SX_TZ][i] = acc[5];
-                pe_loc += acc[6];
+                pe[i] = acc[6];
             }
         }
     } else {
@@ -219,11 +245,10 @@ __global__ void __launch_bounds__(BT) k_batch_step(DState s, const BatchSys *sys
         // candidates at a time, the ones in reach are appended (ballot +
         // prefix count) to the wavefront's queue in LDS, and every time the
         // queue holds 64 pairs all lanes evaluate one each.  Lanes never idle
-        // on pairs beyond the cutoff.  Sums are combined in a fixed order, so
-        // results are deterministic.
+        // on pairs beyond the cutoff.
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
         int *jq_buf = queue[wv];
-        for (int q = wv; q < S.n; q += BT / 64) {
+        for (int q = q0 + wv; q < q1; q += BT / 64) {
             const int i = S.off + q;
             BatchSelf me;
             batch_self(tab, ld, q, gb, S.p[2], me);
@@ -265,23 +290,36 @@ __global__ void __launch_bounds__(BT) k_batch_step(DState s, const BatchSys *sys
             if (lane == 0) {
                 s.f[SX_FX][i] = acc[0]; s.f[SX_FY][i] = acc[1]; s.f[SX_FZ][i] = acc[2];
                 s.f[SX_TX][i] = acc[3]; s.f[SX_TY][i] = acc[4]; s.f[SX_TZ][i] = acc[5];
-                pe_loc += acc[6];
+                pe[i] = acc[6];
             }
         }
     }
-    __syncthreads();
-    // lgv_final_step and the system's thermo sums
-    double ke = 0.0, kr = 0.0;
-    for (int q = threadIdx.x; q < S.n; q += BT) {
+}
+
+// lgv_final_step of one system and its thermo: KE, rotational KE, PE (the
+// particles' pair energies summed in a fixed order) and T; the last
+// workgroup to finish advances the step counter
+__global__ void __launch_bounds__(256) k_batch_final(DState s, const BatchSys *sys, double dt, const double *pe,
+                                                     double *out, unsigned *ticket, int *dstep, int nsys)
+{
+    __shared__ double red[256][3];
+    const int sid = blockIdx.x;
+    const BatchSys S = sys[sid];
+    DState ls = s;
+    ls.box[0] = ls.box[1] = ls.box[2] = S.L;
+    ls.periodic = 1;
+    double ke = 0.0, kr = 0.0, pl = 0.0;
+    for (int q = threadIdx.x; q < S.n; q += 256) {
         const int i = S.off + q;
         half_kick(ls, i, dt);
         ang_kick(ls, i, dt);
         ke += kin_trans(ls, i);
         kr += kin_rot(ls, i);
+        pl += pe[i];
     }
-    red[threadIdx.x][0] = ke; red[threadIdx.x][1] = kr; red[threadIdx.x][2] = pe_loc;
+    red[threadIdx.x][0] = ke; red[threadIdx.x][1] = kr; red[threadIdx.x][2] = pl;
     __syncthreads();
-    for (int o = BT / 2; o > 0; o >>= 1) {
+    for (int o = 128; o > 0; o >>= 1) {
         if (threadIdx.x < o)
             for (int c = 0; c < 3; ++c) red[threadIdx.x][c] = red[threadIdx.x][c] + red[threadIdx.x + o][c];
         __syncthreads();
@@ -291,12 +329,11 @@ __global__ void __launch_bounds__(BT) k_batch_step(DState s, const BatchSys *sys
         out[4 * sid + 1] = red[0][1];
         out[4 * sid + 2] = red[0][2];
         out[4 * sid + 3] = 2.0 * (red[0][0] + red[0][1]) / (double)(6 * S.n - 3);
-        // the step counter advances once every system has read it
+        // every system's initial step has read the counter (an earlier launch)
         __threadfence();
-        unsigned t = atomicAdd(ticket, 1u);
-        if (t == (unsigned)nsys - 1) {
+        if (atomicAdd(ticket, 1u) == (unsigned)nsys - 1) {
             *ticket = 0u;
-            if (dstep) *dstep = st + 1;
+            *dstep = *dstep + 1;
         }
     }
 }
@@ -318,8 +355,8 @@ void sx_batch_free(sx_ctx *c)
     if (!B) return;
     if (B->exec) hipGraphExecDestroy(B->exec);
     if (B->graph) hipGraphDestroy(B->graph);
-    for (void *q : {(void *)B->d_sys, (void *)B->d_order, (void *)B->d_out, (void *)B->d_ticket, (void *)B->d_step,
-                    (void *)B->d_tab})
+    for (void *q : {(void *)B->d_sys, (void *)B->d_sys_of, (void *)B->d_tab, (void *)B->d_pe, (void *)B->d_blocks,
+                    (void *)B->d_out, (void *)B->d_ticket, (void *)B->d_step})
         if (q) hipFree(q);
     delete B;
     c->batch = nullptr;
@@ -339,34 +376,71 @@ static void batch_constants(sx_ctx *c, BatchState *B, double dt, double damp)
     B->damp = damp;
 }
 
-static void batch_launch(sx_ctx *c, BatchState *B, int choice, double dt, double damp, int step, int *dstep)
+// force workgroups of a shape: slices of P particles, the most expensive
+// first (work ~ slice length x N_s, Gay-Berne weighted 4x)
+static void batch_blocks(sx_ctx *c, BatchState *B, int choice)
 {
-    dim3 g(B->nsys);
+    if (B->blocks_choice == choice) return;
+    const int P = BATCH_SHAPES[choice][0];
+    std::vector<int4> blk;
+    std::vector<double> cost;
+    for (int q = 0; q < B->nsys; ++q) {
+        const BatchSys &S = B->h_sys[q];
+        for (int a = 0; a < S.n; a += P) {
+            int4 b;
+            b.x = q; b.y = a; b.z = std::min(S.n, a + P); b.w = 0;
+            blk.push_back(b);
+            cost.push_back((double)(b.z - b.y) * S.n * (S.kind == SX_GB ? 4.0 : 1.0));
+        }
+    }
+    std::vector<int> ord(blk.size());
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<int4> sorted(blk.size());
+    for (size_t k = 0; k < ord.size(); ++k) sorted[k] = blk[ord[k]];
+    if ((int)sorted.size() > B->blocks_cap) {
+        if (B->d_blocks) HIPCHK(hipFree(B->d_blocks));
+        HIPCHK(hipMalloc(&B->d_blocks, sizeof(int4) * sorted.size()));
+        B->blocks_cap = (int)sorted.size();
+        B->graph_choice = -1;   // a captured step would use the old buffer
+    }
+    HIPCHK(hipMemcpyAsync(B->d_blocks, sorted.data(), sizeof(int4) * sorted.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    B->nblocks = (int)sorted.size();
+    B->blocks_choice = choice;
+}
+
+static void batch_launch(sx_ctx *c, BatchState *B, int choice, double dt, double damp)
+{
     DState s = c->dstate();
-    const size_t lds = B->lds ? sizeof(double) * TB_N * (size_t)B->cap : 0;
-#define SX_BATCH_CASE(K, BT, LPP)                                                                                 \
-    case K:                                                                                                       \
-        if (B->lds)                                                                                               \
-            hipLaunchKernelGGL((k_batch_step<BT, LPP, true>), g, dim3(BT), lds, c->stream, s, B->d_sys,          \
-                               B->d_order, dt, damp, step, dstep, B->d_out, B->d_ticket, B->nsys, B->d_tab,       \
-                               B->cap);                                                                           \
-        else                                                                                                      \
-            hipLaunchKernelGGL((k_batch_step<BT, LPP, false>), g, dim3(BT), 0, c->stream, s, B->d_sys,           \
-                               B->d_order, dt, damp, step, dstep, B->d_out, B->d_ticket, B->nsys, B->d_tab,       \
-                               B->cap);                                                                           \
+    hipLaunchKernelGGL(k_batch_initial, dim3(sx_blocks(c->n, 256)), dim3(256), 0, c->stream, s, B->d_sys,
+                       B->d_sys_of, dt, damp, B->d_step, B->d_tab);
+    const bool lds = sizeof(double) * TB_N * (size_t)B->cap <= (size_t)BATCH_LDS_BYTES;
+    const size_t lbytes = lds ? sizeof(double) * TB_N * (size_t)B->cap : 0;
+    dim3 g(B->nblocks);
+#define SX_BATCH_CASE(K, BT, LPP)                                                                                  \
+    case K:                                                                                                        \
+        if (lds)                                                                                                   \
+            hipLaunchKernelGGL((k_batch_forces<BT, LPP, true>), g, dim3(BT), lbytes, c->stream, s, B->d_sys,       \
+                               B->d_blocks, B->d_tab, B->d_pe);                                                    \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_batch_forces<BT, LPP, false>), g, dim3(BT), 0, c->stream, s, B->d_sys,           \
+                               B->d_blocks, B->d_tab, B->d_pe);                                                    \
         break;
     switch (choice) {
         SX_BATCH_CASE(0, 64, 1)
-        SX_BATCH_CASE(1, 128, 1)
-        SX_BATCH_CASE(2, 128, 2)
-        SX_BATCH_CASE(3, 256, 2)
-        SX_BATCH_CASE(4, 256, 4)
-        SX_BATCH_CASE(5, 256, 8)
+        SX_BATCH_CASE(1, 64, 4)
+        SX_BATCH_CASE(2, 128, 4)
+        SX_BATCH_CASE(3, 256, 8)
+        SX_BATCH_CASE(4, 256, 8)
+        SX_BATCH_CASE(5, 256, 64)
         SX_BATCH_CASE(6, 256, 64)
-        SX_BATCH_CASE(7, 512, 64)
+        SX_BATCH_CASE(7, 256, 64)
     default: throw sx_error{SX_ERR_ARG, "bad launch shape"};
     }
 #undef SX_BATCH_CASE
+    hipLaunchKernelGGL(k_batch_final, dim3(B->nsys), dim3(256), 0, c->stream, s, B->d_sys, dt, B->d_pe, B->d_out,
+                       B->d_ticket, B->d_step, B->nsys);
 }
 
 extern "C" {
@@ -380,12 +454,31 @@ int sx_batch_set(sx_ctx *c, int nsys, const int *counts, const double *box, cons
         long total = 0;
         for (int q = 0; q < nsys; ++q) total += counts[q];
         if (total != c->n) throw sx_error{SX_ERR_ARG, "system sizes do not add up to the particle count"};
-        std::vector<double> sigz(c->n);
+        std::vector<double> sigz(c->n), sigx(c->n);
         HIPCHK(hipMemcpy(sigz.data(), c->d_fields + (size_t)SX_SIG_Z * c->ld, sizeof(double) * c->n,
                          hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(sigx.data(), c->d_fields + (size_t)SX_SIG_X * c->ld, sizeof(double) * c->n,
+                         hipMemcpyDeviceToHost));
         sx_batch_free(c);
-        BatchState *B = new BatchState();
-        c->batch = B;
+        struct Owner {
+            sx_ctx *c;
+            BatchState *b;
+            ~Owner()
+            {
+                if (b) {   // a failed set-up: release what was allocated
+                    c->batch = b;
+                    sx_batch_free(c);
+                }
+            }
+            BatchState *get() { return b; }
+            BatchState *release()
+            {
+                BatchState *r = b;
+                b = nullptr;
+                return r;
+            }
+        } owner{c, new BatchState()};
+        BatchState *B = owner.get();
         B->nsys = nsys;
         B->n_total = c->n;
         B->fields = c->d_fields;
@@ -402,7 +495,8 @@ int sx_batch_set(sx_ctx *c, int nsys, const int *counts, const double *box, cons
             S.T = temp[q];
             S.temp_int = (int)(S.T * 1000.0);   // integrators.f90:532 (host FP64)
             for (int k = 0; k < 4; ++k) S.p[k] = params[4 * q + k];
-            double smax = *std::max_element(sigz.begin() + off, sigz.begin() + off + counts[q]);
+            double smax = 0.0;
+            for (int a = off; a < off + counts[q]; ++a) smax = std::max(smax, std::max(sigz[a], sigx[a]));
             // the minimum image must see every interacting pair (system.f90:902-906)
             double range = S.kind == SX_LJ ? S.p[2] : smax + S.p[0] - 1.0;
             if (range > 0.5 * S.L)
@@ -410,21 +504,21 @@ int sx_batch_set(sx_ctx *c, int nsys, const int *counts, const double *box, cons
             off += counts[q];
         }
         B->cap = *std::max_element(counts, counts + nsys);
-        B->lds = sizeof(double) * TB_N * (size_t)B->cap <= (size_t)BATCH_LDS_BYTES;
-        if (!B->lds) HIPCHK(hipMalloc(&B->d_tab, sizeof(double) * TB_N * (size_t)c->n));
-        B->h_order.resize(nsys);
-        std::iota(B->h_order.begin(), B->h_order.end(), 0);
-        std::stable_sort(B->h_order.begin(), B->h_order.end(),
-                         [&](int a, int b) { return counts[a] > counts[b]; });
+        std::vector<int> sys_of(c->n);
+        for (int q = 0; q < nsys; ++q)
+            for (int a = 0; a < counts[q]; ++a) sys_of[B->h_sys[q].off + a] = q;
         HIPCHK(hipMalloc(&B->d_sys, sizeof(BatchSys) * nsys));
-        HIPCHK(hipMalloc(&B->d_order, sizeof(int) * nsys));
+        HIPCHK(hipMalloc(&B->d_sys_of, sizeof(int) * c->n));
+        HIPCHK(hipMalloc(&B->d_tab, sizeof(double) * TB_N * (size_t)c->n));
+        HIPCHK(hipMalloc(&B->d_pe, sizeof(double) * (size_t)c->n));
+        HIPCHK(hipMemcpyAsync(B->d_sys_of, sys_of.data(), sizeof(int) * c->n, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMalloc(&B->d_out, sizeof(double) * 4 * nsys));
         HIPCHK(hipMalloc(&B->d_ticket, sizeof(unsigned)));
         HIPCHK(hipMalloc(&B->d_step, sizeof(int)));
-        HIPCHK(hipMemcpyAsync(B->d_order, B->h_order.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemsetAsync(B->d_out, 0, sizeof(double) * 4 * nsys, c->stream));
         HIPCHK(hipMemsetAsync(B->d_ticket, 0, sizeof(unsigned), c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+        c->batch = owner.release();
     });
 }
 
@@ -435,11 +529,12 @@ int sx_batch_run(sx_ctx *c, int nsteps, double dt, double damping, int first_ste
         BatchState *B = batch_of(c);
         if (nsteps < 0) throw sx_error{SX_ERR_ARG, "nsteps < 0"};
         batch_constants(c, B, dt, damping);
+        batch_blocks(c, B, B->choice);
         hipLaunchKernelGGL(k_batch_step_set, dim3(1), dim3(1), 0, c->stream, B->d_step, first_step);
         if (nsteps == 0) return;
         const char *ng = getenv("SUCROSE_NO_GRAPH");
         if (ng && ng[0] == '1') {
-            for (int t = 0; t < nsteps; ++t) batch_launch(c, B, B->choice, dt, damping, 0, B->d_step);
+            for (int t = 0; t < nsteps; ++t) batch_launch(c, B, B->choice, dt, damping);
             sx_check_device_error(c);
             return;
         }
@@ -447,7 +542,7 @@ int sx_batch_run(sx_ctx *c, int nsteps, double dt, double damping, int first_ste
             if (B->exec) { hipGraphExecDestroy(B->exec); B->exec = nullptr; }
             if (B->graph) { hipGraphDestroy(B->graph); B->graph = nullptr; }
             HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            batch_launch(c, B, B->choice, dt, damping, 0, B->d_step);
+            batch_launch(c, B, B->choice, dt, damping);
             HIPCHK(hipStreamEndCapture(c->stream, &B->graph));
             HIPCHK(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
             B->graph_choice = B->choice;
@@ -489,10 +584,11 @@ int sx_batch_autotune(sx_ctx *c, double dt, double damping, int first_step, int 
         double best = 1e300;
         int best_k = B->choice;
         for (int k = 0; k < BATCH_NSHAPES; ++k) {
+            batch_blocks(c, B, k);
             hipLaunchKernelGGL(k_batch_step_set, dim3(1), dim3(1), 0, c->stream, B->d_step, first_step);
-            batch_launch(c, B, k, dt, damping, 0, B->d_step);   // warm
+            batch_launch(c, B, k, dt, damping);   // warm
             HIPCHK(hipEventRecord(e0, c->stream));
-            for (int r = 0; r < nrep; ++r) batch_launch(c, B, k, dt, damping, 0, B->d_step);
+            for (int r = 0; r < nrep; ++r) batch_launch(c, B, k, dt, damping);
             HIPCHK(hipEventRecord(e1, c->stream));
             HIPCHK(hipEventSynchronize(e1));
             float t = 0;

@@ -13,7 +13,8 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from oracle.oracle import (DH, ECP, ECPR, GB, GBR, LGV, LJ, LJ12, LJW, MORSE, QQ, VV, VVC, WCA, State)
+from oracle.oracle import (DH, ECP, ECPC, ECPR, ECPW, ECPWC, GB, GBC, GBR, GBW, GBWC, LGV, LJ, LJ12, LJW, MORSE, QQ,
+                           VV, VVC, WCA, State)
 from tests._states import base_state, chains, rand_quats
 
 PD = C.POINTER(C.c_double)
@@ -226,9 +227,13 @@ def test_lj_special_values(oracle):
 
 
 @pytest.mark.parametrize("kind,p,r", [(GB, [4.0, 1, 2], (1.6, 3.2)), (GB, [4.0, 2, 1], (1.6, 3.2)),
-                                      (GBR, [0.0, 1, 2], (0.9, 1.6))])
+                                      (GBR, [0.0, 1, 2], (0.9, 1.6)), (GBW, [4.0, 1, 2, 0.4], (1.6, 3.2)),
+                                      (GBC, [4.0, 1, 2, 0.7], (1.6, 3.2)), (GBC, [4.0, 2, 1, -0.7], (1.6, 3.2)),
+                                      (GBWC, [4.0, 1, 2, 0.4, 0.7], (1.6, 3.2))])
 def test_gay_berne_forces_torques_are_minus_gradient(oracle, kind, p, r):
-    """Outside the switching band (r < 0.9 (sigma + rc - sigma0))."""
+    """Outside the switching band (r < 0.9 (sigma + rc - sigma0)); the welded
+    variants' kinks at r = sigma and sigma + wf are met with probability ~0 by
+    the random separations."""
     rng = np.random.default_rng(8 + kind + int(p[1]))
     st = base_state(rng, 2, 30.0, sig=(1, 1, 3), eps=(5, 5, 1))
     checked = 0
@@ -267,7 +272,10 @@ def test_gay_berne_sphere_limit(oracle):
 
 
 @pytest.mark.parametrize("kind,sig,p,rmax", [(ECPR, (1, 1, 3), [0.0], 3.3), (ECP, (1, 1, 3), [1.5], 1.35),
-                                             (ECPR, (1.0, 2.0, 3.0), [0.0], 3.3)])
+                                             (ECPR, (1.0, 2.0, 3.0), [0.0], 3.3),
+                                             (ECPW, (1, 1, 3), [1.5, 0.3], 1.35),
+                                             (ECPC, (1, 1, 3), [1.5, 0.6], 1.35),
+                                             (ECPWC, (1.0, 2.0, 3.0), [1.5, 0.3, -0.6], 1.35)])
 def test_ecp_forces_torques_are_partial_derivatives(oracle, kind, sig, p, rmax):
     """At a fixed contact parameter lambda the ECP force and torques
     (force.f90:1202-1420) are the exact partial derivatives of the pair energy.

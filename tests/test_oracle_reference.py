@@ -8,8 +8,8 @@ when ``oracle/_ref`` has not been built (``make -C oracle ref``).
 import numpy as np
 import pytest
 
-from oracle.oracle import (DH, ECP, ECPR, GB, GBR, LGV, LGVC, LJ, LJ12, LJW, MORSE, NHT, NHTC, QQ,
-                           VV, VVC, WCA, State, reference_available)
+from oracle.oracle import (DH, ECP, ECPC, ECPR, ECPW, ECPWC, GB, GBC, GBR, GBW, GBWC, LGV, LGVC, LJ, LJ12, LJW, MORSE,
+                           NHT, NHTC, QQ, VV, VVC, WCA, State, reference_available)
 from tests._states import base_state, chains
 
 pytestmark = [pytest.mark.ref,
@@ -83,6 +83,14 @@ PAIRS = [
     (ECP, [1.5], dict(sig=(1, 1, 3)), 0.6, 3.4),
     (ECPR, [0.0], dict(sig=(1, 1, 3)), 0.6, 3.4),
     (ECPR, [0.0], dict(sig=(1.0, 2.0, 3.0)), 0.6, 3.4),
+    # welded and chiral variants (force.f90:535-1030, 1553-2234); GB (mu, nu)
+    # fixed at the reference's (1, 2)
+    (GBW, [4.0, 1, 2, 0.4], dict(sig=(1, 1, 3), eps=(5, 5, 1)), 0.7, 4.9),
+    (GBC, [4.0, 1, 2, 0.7], dict(sig=(1, 1, 3), eps=(5, 5, 1)), 0.7, 4.5),
+    (GBWC, [4.0, 1, 2, 0.4, -0.7], dict(sig=(1, 1, 3), eps=(5, 5, 1)), 0.7, 4.9),
+    (ECPW, [1.5, 0.3], dict(sig=(1, 1, 3)), 0.6, 3.7),
+    (ECPC, [1.5, 0.6], dict(sig=(1, 1, 3)), 0.6, 3.4),
+    (ECPWC, [1.5, 0.3, -0.6], dict(sig=(1.0, 2.0, 3.0)), 0.6, 3.7),
 ]
 
 
