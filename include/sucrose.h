@@ -72,9 +72,16 @@ enum sx_scalar {
  *   SX_ECP   cutoff                                 force.f90:1202
  *   SX_ECPR  (none)                                 force.f90:1387
  *   SX_DH    cutoff, db_len, dielectric             force.f90:2589
+ *   SX_GBW   cutoff, mu, nu, wf                     force.f90:535 (welded)
+ *   SX_GBC   cutoff, mu, nu, chirality              force.f90:665 (chiral)
+ *   SX_GBWC  cutoff, mu, nu, wf, chirality          force.f90:845
+ *   SX_ECPW  cutoff, wf                             force.f90:1553 (welded)
+ *   SX_ECPC  cutoff, chirality                      force.f90:1744 (chiral)
+ *   SX_ECPWC cutoff, wf, chirality                  force.f90:1986
  */
 enum sx_kind {
-    SX_LJ = 1, SX_LJ12, SX_LJW, SX_WCA, SX_MORSE, SX_GB, SX_GBR, SX_ECP, SX_ECPR, SX_DH
+    SX_LJ = 1, SX_LJ12, SX_LJW, SX_WCA, SX_MORSE, SX_GB, SX_GBR, SX_ECP, SX_ECPR, SX_DH,
+    SX_GBW, SX_GBC, SX_GBWC, SX_ECPW, SX_ECPC, SX_ECPWC
 };
 
 /* ECP contact-parameter solver (sx_set_ecp_solver) */

@@ -202,10 +202,18 @@ __device__ __forceinline__ bool ecp_zero_proven(double F, double rsq, int varian
 
 // The force/torque evaluation of ecp/ecpr (force.f90:1487-1550) for a given
 // lambda; RIJ already minimum-imaged.
+// parameters of the ECP family: ecp_ (variant 0), ecpr_ (1), ecpw_ (2, weld
+// width wf), ecpc_ (3, chirality), ecpwc_ (4, both)
+struct EcpPar {
+    double cutoff, wf, chir;
+};
+
 template <int VARIANT>
-__device__ __forceinline__ void ecp_forces(const DState &s, const double *pp, long ld, int a, int b, double cutoff,
+__device__ __forceinline__ void ecp_forces(const DState &s, const double *pp, long ld, int a, int b, const EcpPar &P,
                                            const double RIJ[3], double lambda_R, PairOut &o)
 {
+    constexpr bool welded = VARIANT == 2 || VARIANT == 4, chiral = VARIANT == 3 || VARIANT == 4;
+    const double cutoff = P.cutoff, wf = P.wf;
     const int meu = 2, neu = 1;
     const double sigma_0 = 1.0;
     o.pe = 0.0;
@@ -231,17 +239,17 @@ __device__ __forceinline__ void ecp_forces(const DState &s, const double *pp, lo
     double sigma_0_inv = 1.0 / sigma_0;
     double p32 = sigma * sigma * sigma;
     double switch_ = 0, factor = 0, sx = 0, sdx = 0, rlim;
-    if (VARIANT == 0) {
-        switch_ = 0.9 * (sigma + cutoff - sigma_0);
-        factor = 1.0 / ((sigma + cutoff - sigma_0) - switch_);
+    if (VARIANT != 1) {
+        rlim = welded ? (sigma + cutoff + wf - sigma_0) : (sigma + cutoff - sigma_0);
+        switch_ = 0.9 * rlim;
+        factor = 1.0 / (rlim - switch_);
         double xx = (rij_mag - switch_) * factor;
         sx = 1.0 - 6.0 * xx * xx * xx * xx * xx + 15.0 * xx * xx * xx * xx - 10.0 * xx * xx * xx;
         sdx = -30.0 * xx * xx * xx * xx + 60.0 * xx * xx * xx - 30.0 * xx * xx;
-        rlim = sigma + cutoff - sigma_0;
     } else {
         rlim = sigma;
     }
-    bool inside = (VARIANT == 0) ? (rij_mag < rlim) : (rij_mag <= rlim);
+    bool inside = (VARIANT == 1) ? (rij_mag <= rlim) : (rij_mag < rlim);
     if (!inside) return;
     // eps1 = sigma_t sqrt(det(inverse(AR + BR)))  (force.f90:1482-1488)
     double EAB[9], EM[9];
@@ -271,7 +279,11 @@ __device__ __forceinline__ void ecp_forces(const DState &s, const double *pp, lo
     double sigma_t = sqrt(PPX(pp, PP_ST, a) * PPX(pp, PP_ST, b));
     double eps1 = sigma_t * sqrt(detEM);
 
-    double sr = sigma_0 / (rij_mag - sigma + sigma_0);
+    double sr;
+    if (!welded) sr = sigma_0 / (rij_mag - sigma + sigma_0);
+    else if (rij_mag < sigma) sr = sigma_0 / (rij_mag - sigma + sigma_0);
+    else if (rij_mag >= sigma && rij_mag <= sigma + wf) sr = 1.0;
+    else sr = sigma_0 / (rij_mag - sigma + sigma_0 - wf);
     double sr2 = sr * sr, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
     double srx = (sr12 - 2.0 * sr6);
     if (VARIANT == 1) srx = srx + 1.0;
@@ -281,7 +293,8 @@ __device__ __forceinline__ void ecp_forces(const DState &s, const double *pp, lo
     double e2m1 = eps2;                // eps2**(meu-1)
     double dotKR = rhat[0] * KR[0] + rhat[1] * KR[1] + rhat[2] * KR[2];
     double dotKE = rhat[0] * KE[0] + rhat[1] * KE[1] + rhat[2] * KE[2];
-    double pref1 = e2m * (neu * e1n) * srx;
+    // the chiral forms leave srx / srd out of the derivative terms (force.f90:1889-1914)
+    double pref1 = chiral ? e2m * (neu * e1n) : e2m * (neu * e1n) * srx;
     double de1_u1[3], de1_u2[3];
     for (int side = 0; side < 2; ++side) {
         int who = side == 0 ? a : b;
@@ -316,18 +329,69 @@ __device__ __forceinline__ void ecp_forces(const DState &s, const double *pp, lo
     cross3(KE_BE, KE, ce2);
     cross3(KR_BR, KR, cr2);
     double pot = e1n * e2m * srx;
-    for (int k = 0; k < 3; ++k) {
-        double dR_dr = sigma_0_inv * (rhat[k] + 0.5 * rfac * (KR[k] - dotKR * rhat[k]) * p32);
-        double de2_dr = e1n * (meu * e2m1) * efac * (KE[k] - dotKE * rhat[k]);
-        double de2_u1 = e1n * (meu * e2m1) * srx * (1 - lambda_E) * efac * ce1[k];
-        double dR_u1 = e1n * e2m * srd * (0.5 * sigma_0_inv) * (1 - lambda_R) * rfac * cr1[k] * p32;
-        double de2_u2 = e1n * (meu * e2m1) * srx * lambda_E * efac * ce2[k];
-        double dR_u2 = e1n * e2m * srd * (0.5 * sigma_0_inv) * lambda_R * rfac * cr2[k] * p32;
-        o.f[k] = (-1.0) * (e1n * e2m * srd * dR_dr + srx * de2_dr);
-        o.ti[k] = (de1_u1[k] + de2_u1) + dR_u1;
-        o.tj[k] = (de1_u2[k] + de2_u2) + dR_u2;
+    if (!chiral) {
+        for (int k = 0; k < 3; ++k) {
+            double dR_dr = sigma_0_inv * (rhat[k] + 0.5 * rfac * (KR[k] - dotKR * rhat[k]) * p32);
+            double de2_dr = e1n * (meu * e2m1) * efac * (KE[k] - dotKE * rhat[k]);
+            double de2_u1 = e1n * (meu * e2m1) * srx * (1 - lambda_E) * efac * ce1[k];
+            double dR_u1 = e1n * e2m * srd * (0.5 * sigma_0_inv) * (1 - lambda_R) * rfac * cr1[k] * p32;
+            double de2_u2 = e1n * (meu * e2m1) * srx * lambda_E * efac * ce2[k];
+            double dR_u2 = e1n * e2m * srd * (0.5 * sigma_0_inv) * lambda_R * rfac * cr2[k] * p32;
+            o.f[k] = (-1.0) * (e1n * e2m * srd * dR_dr + srx * de2_dr);
+            o.ti[k] = (de1_u1[k] + de2_u1) + dR_u1;
+            o.tj[k] = (de1_u2[k] + de2_u2) + dR_u2;
+        }
+    } else {
+        // ecpc_ / ecpwc_ (force.f90:1889-1968, 2143-2222): the plain terms
+        // regrouped, then the chirality-weighted pseudoscalar terms built on
+        // the particles' z axes u = R(q) e_z
+        double srxc, srdc;
+        if (welded) {
+            srxc = (sr12 - (12.0 / 7.0) * sr6 * sr);
+            srdc = 12.0 * (sr6 * sr2 - sr12 * sr);
+        } else {
+            srxc = sr6 * sr;
+            srdc = -7.0 * sr6 * sr2;
+        }
+        double u1z[3], u2z[3];
+        quat_u(s, a, u1z);
+        quat_u(s, b, u2z);
+        double u1xu2[3], u2xrij[3], rijxu1[3];
+        cross3(u1z, u2z, u1xu2);
+        cross3(u2z, rhat, u2xrij);
+        cross3(rhat, u1z, rijxu1);
+        const double uur = u1xu2[0] * rhat[0] + u1xu2[1] * rhat[1] + u1xu2[2] * rhat[2];
+        const double uu = u1z[0] * u2z[0] + u1z[1] * u2z[1] + u1z[2] * u2z[2];
+        double term1[3], term2[3], tu1[3], tu2[3];
+        for (int k = 0; k < 3; ++k) {
+            term1[k] = (e1n * e2m * srxc * uu * u2xrij[k] + e1n * e2m * srxc * uur * u2z[k]);
+            term2[k] = (e1n * e2m * srxc * uu * rijxu1[k] + e1n * e2m * srxc * uur * u1z[k]);
+        }
+        cross3(u1z, term1, tu1);
+        cross3(u2z, term2, tu2);
+        const double chir = P.chir;
+        for (int k = 0; k < 3; ++k) {
+            double dR_dr = e1n * e2m * sigma_0_inv * (rhat[k] + 0.5 * rfac * (KR[k] - dotKR * rhat[k]) * p32);
+            double de2_dr = e1n * (meu * e2m1) * efac * (KE[k] - dotKE * rhat[k]);
+            double de2_u1 = e1n * (meu * e2m1) * (1 - lambda_E) * efac * ce1[k];
+            double dR_u1 = e1n * e2m * (0.5 * sigma_0_inv) * (1 - lambda_R) * rfac * cr1[k] * p32;
+            double de2_u2 = e1n * (meu * e2m1) * lambda_E * efac * ce2[k];
+            double dR_u2 = e1n * e2m * (0.5 * sigma_0_inv) * lambda_R * rfac * cr2[k] * p32;
+            double f = (-1.0) * (srx * de2_dr + srd * dR_dr);
+            double t1 = (srx * (de1_u1[k] + de2_u1) + srd * dR_u1);
+            double t2 = (srx * (de1_u2[k] + de2_u2) + srd * dR_u2);
+            double dRr = dR_dr * srdc * uur * uu, de2r = de2_dr * srxc * uur * uu;
+            double duur_r = e1n * e2m * srxc * uu * (u1xu2[k] - uur * rhat[k]) / rij_mag;
+            double fc = (-1.0) * (de2r + dRr + duur_r);
+            double t1c = (de1_u1[k] * srxc * uur * uu + de2_u1 * srxc * uur * uu + dR_u1 * srdc * uur * uu - tu1[k]);
+            double t2c = (de1_u2[k] * srxc * uur * uu + de2_u2 * srxc * uur * uu + dR_u2 * srdc * uur * uu - tu2[k]);
+            o.f[k] = f + chir * fc;
+            o.ti[k] = t1 + chir * t1c;
+            o.tj[k] = t2 + chir * t2c;
+        }
+        pot = pot + chir * (e1n * e2m * srxc * uur * uu);
     }
-    if (VARIANT == 0 && (rij_mag >= switch_) && (rij_mag <= rlim)) {
+    if (VARIANT != 1 && (rij_mag >= switch_) && (rij_mag <= rlim)) {
         for (int k = 0; k < 3; ++k) {
             o.f[k] = sx * o.f[k] - sdx * pot * factor;
             o.ti[k] = sx * o.ti[k] - sdx * pot * factor;
@@ -366,7 +430,7 @@ __device__ __forceinline__ double ecp_solve_lambda(int solver, const double *pp,
 // F(l0) already proves it out of range ends there (returns 1, live[k] = 0).
 template <int VARIANT>
 __device__ __forceinline__ int ecp_pair_solve(const DState &s, const double *pp, long ld, int k, int a, int b,
-                                              int solver, bool warm, double l0, double cutoff, double *res,
+                                              int solver, bool warm, double l0, const EcpPar &P, double *res,
                                               unsigned char *live, const int *fwdpos, const int *revpos, int *idx_full,
                                               int &iters, int &fb)
 {
@@ -387,7 +451,7 @@ __device__ __forceinline__ int ecp_pair_solve(const DState &s, const double *pp,
             l0 = fmin(fmax(l0, 1e-6), 1.0 - 1e-6);
             double d[3];
             ecp_F_derivs(l0, A, B, RIJ, d[0], d[1], d[2]);
-            if (ecp_zero_proven(d[0], rsq, VARIANT, cutoff)) {
+            if (ecp_zero_proven(d[0], rsq, VARIANT == 1 ? 1 : 0, P.cutoff + P.wf)) {
                 live[k] = 0;
                 return 1;
             }
@@ -404,7 +468,7 @@ __device__ __forceinline__ int ecp_pair_solve(const DState &s, const double *pp,
         }
     }
     PairOut o;
-    ecp_forces<VARIANT>(s, pp, ld, a, b, cutoff, RIJ, lam, o);
+    ecp_forces<VARIANT>(s, pp, ld, a, b, P, RIJ, lam, o);
     double *r = res + (long)k * 10;
     r[0] = o.pe;
     r[1] = o.f[0]; r[2] = o.f[1]; r[3] = o.f[2];

@@ -510,7 +510,7 @@ template <int VARIANT>
 __global__ void __launch_bounds__(64) k_ecp_solve_learned(DState s, const double *pp, const float *p, const bf16x4 *wf,
                                                           const bf16x8 *wi, const int *half_i, const int *half_j,
                                                           const int *work, const int *segcount, const int *nnz_half,
-                                                          double cutoff, double *res, unsigned char *live,
+                                                          EcpPar par, double *res, unsigned char *live,
                                                           long *stats, const int *fwdpos, const int *revpos,
                                                           int *idx_full)
 {
@@ -563,7 +563,7 @@ __global__ void __launch_bounds__(64) k_ecp_solve_learned(DState s, const double
         __syncthreads();
         if (u < cnt) {
             int iters = 0, fb = 0;
-            if (ecp_pair_solve<VARIANT>(s, pp, 0L, k, a, b, SX_ECP_LEARNED, true, (double)l0s[lane], cutoff, res,
+            if (ecp_pair_solve<VARIANT>(s, pp, 0L, k, a, b, SX_ECP_LEARNED, true, (double)l0s[lane], par, res,
                                         live, fwdpos, revpos, idx_full, iters, fb)) {
                 ++my_skip;
             } else {
@@ -688,19 +688,23 @@ const float *sx_ecp_mlp_predict(sx_ctx *c)
 
 // the learned solve stage over the current work list (results, live flags
 // and full-list marks as the plain solve stage)
-void sx_ecp_solve_learned(sx_ctx *c, int variant, double cutoff, long *stats, bool mark)
+void sx_ecp_solve_learned(sx_ctx *c, int variant, const EcpPar &par, long *stats, bool mark)
 {
     MlpState *M = mlp_of(c);
     NList &nl = c->nl;
     int *idx = mark ? c->d_idx_full : nullptr;
-    if (variant)
-        hipLaunchKernelGGL(k_ecp_solve_learned<1>, dim3(ECP_NSEG), dim3(64), 0, c->stream, c->dstate(), c->d_ecp_pp,
-                           M->p, M->wf, M->wi, nl.half_i, nl.half_j, c->d_work, c->d_segcount, c->d_flag + 2, cutoff,
-                           c->d_pairbuf, c->d_live, stats, nl.fwdpos, nl.revpos, idx);
-    else
-        hipLaunchKernelGGL(k_ecp_solve_learned<0>, dim3(ECP_NSEG), dim3(64), 0, c->stream, c->dstate(), c->d_ecp_pp,
-                           M->p, M->wf, M->wi, nl.half_i, nl.half_j, c->d_work, c->d_segcount, c->d_flag + 2, cutoff,
-                           c->d_pairbuf, c->d_live, stats, nl.fwdpos, nl.revpos, idx);
+#define SX_LEARNED(V)                                                                                               \
+    hipLaunchKernelGGL(k_ecp_solve_learned<V>, dim3(ECP_NSEG), dim3(64), 0, c->stream, c->dstate(), c->d_ecp_pp,  \
+                       M->p, M->wf, M->wi, nl.half_i, nl.half_j, c->d_work, c->d_segcount, c->d_flag + 2, par,      \
+                       c->d_pairbuf, c->d_live, stats, nl.fwdpos, nl.revpos, idx)
+    switch (variant) {
+    case 0: SX_LEARNED(0); break;
+    case 1: SX_LEARNED(1); break;
+    case 2: SX_LEARNED(2); break;
+    case 3: SX_LEARNED(3); break;
+    default: SX_LEARNED(4); break;
+    }
+#undef SX_LEARNED
 }
 
 extern "C" int sx_mlp_init(sx_ctx *c, uint32_t seed)

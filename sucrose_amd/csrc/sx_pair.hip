@@ -127,11 +127,18 @@ __global__ void k_ecp_pp(DState s, double *pp, float *scr, double *pos4, int *nw
 }
 
 struct EcpScreen {
-    int variant;       // 0 ecp (cutoff), 1 ecpr
-    double cutoff;
+    int variant;       // screening class: 0 cut (ecp, ecpw, ecpc, ecpwc), 1 ecpr
+    double cutoff;     // reach of the cut forms: cutoff + wf
     double L[3], invL[3];
     int periodic;
+    int fvar;          // force variant: 0 ecp, 1 ecpr, 2 ecpw, 3 ecpc, 4 ecpwc
+    EcpPar par;        // the potential's cutoff, weld width, chirality
 };
+
+static inline bool is_ecp_kind(int kind)
+{
+    return kind == SX_ECP || kind == SX_ECPR || kind == SX_ECPW || kind == SX_ECPC || kind == SX_ECPWC;
+}
 
 #define ECP_SCREEN_MARGIN 1e-2f
 
@@ -247,7 +254,7 @@ __global__ void __launch_bounds__(ECP_SOLVE_THREADS) k_ecp_solve(DState s, const
         int k = work[t];
         int iters = 0, fb = 0;
         if (ecp_pair_solve<VARIANT>(s, pp, ld, k, half_i[k], half_j[k], solver, lam0 != nullptr,
-                                    lam0 ? (double)lam0[t] : ECP_PSI, sc.cutoff, res, live, fwdpos, revpos, idx_full,
+                                    lam0 ? (double)lam0[t] : ECP_PSI, sc.par, res, live, fwdpos, revpos, idx_full,
                                     iters, fb)) {
             ++my_skip;
             continue;
@@ -326,19 +333,20 @@ __global__ void k_pair_eval(DState s, long np, const int *pi, const int *pj, Par
     int a = pi[t], b = pj[t];
     PairOut o;
     double lam = 0.0;
-    if (KIND == SX_ECP || KIND == SX_ECPR) {
+    constexpr int EV = KIND == SX_ECP ? 0 : KIND == SX_ECPR ? 1 : KIND == SX_ECPW ? 2 : KIND == SX_ECPC ? 3
+                     : KIND == SX_ECPWC ? 4 : -1;
+    if constexpr (EV >= 0) {
         double RIJ[3];
         pair_rij(s, a, b, RIJ);
         Sym3 A = load_sym(pp, ld, PP_AR, a), B = load_sym(pp, ld, PP_AR, b);
         double fpsi = ecp_optim_dist(ECP_PSI, A, B, RIJ);
         int it = 0, fb = 0;
-        if (KIND == SX_ECP) {
-            lam = ecp_solve_lambda<0>(solver, pp, ld, a, b, RIJ, fpsi, ECP_PSI, it, fb);
-            ecp_forces<0>(s, pp, ld, a, b, P.p[0], RIJ, lam, o);
-        } else {
-            lam = ecp_solve_lambda<1>(solver, pp, ld, a, b, RIJ, fpsi, ECP_PSI, it, fb);
-            ecp_forces<1>(s, pp, ld, a, b, 0.0, RIJ, lam, o);
-        }
+        EcpPar ep;
+        ep.cutoff = EV == 1 ? 0.0 : P.p[0];
+        ep.wf = (EV == 2 || EV == 4) ? P.p[1] : 0.0;
+        ep.chir = EV == 3 ? P.p[1] : (EV == 4 ? P.p[2] : 0.0);
+        lam = ecp_solve_lambda<EV>(solver, pp, ld, a, b, RIJ, fpsi, ECP_PSI, it, fb);
+        ecp_forces<EV>(s, pp, ld, a, b, ep, RIJ, lam, o);
     } else {
         for (int k = 0; k < 3; ++k) { o.ti[k] = 0.0; o.tj[k] = 0.0; }
         d_pair<KIND>(s, a, b, P.p, o);
@@ -357,7 +365,7 @@ static Params make_params(sx_ctx *c, int kind, const double *p, int np)
     Params P;
     for (int k = 0; k < 8; ++k) P.p[k] = 0.0;
     for (int k = 0; k < np && k < 8; ++k) P.p[k] = p[k];
-    if (kind == SX_GB || kind == SX_GBR) {
+    if (kind == SX_GB || kind == SX_GBR || kind == SX_GBW || kind == SX_GBC || kind == SX_GBWC) {
         if (np < 2) P.p[1] = 1.0;   // mu (force.f90:240)
         if (np < 3) P.p[2] = 2.0;   // nu
     }
@@ -375,6 +383,10 @@ static void check_params(int kind, int np)
     case SX_WCA: need = 2; break;
     case SX_GB: case SX_ECP: need = 1; break;
     case SX_GBR: case SX_ECPR: need = 0; break;
+    case SX_GBW: case SX_GBC: need = 4; break;   // cutoff, mu, nu, wf | chirality
+    case SX_GBWC: need = 5; break;
+    case SX_ECPW: case SX_ECPC: need = 2; break;
+    case SX_ECPWC: need = 3; break;
     default: throw sx_error{SX_ERR_ARG, "unknown force kind"};
     }
     if (np < need) throw sx_error{SX_ERR_ARG, "too few parameters for this force kind"};
@@ -449,13 +461,17 @@ static void ensure_pair_buffers(sx_ctx *c)
 // learned warm start: predictions for the work list of the current call
 const float *sx_ecp_mlp_predict(sx_ctx *c);
 // learned solve stage (network and solve in one kernel)
-void sx_ecp_solve_learned(sx_ctx *c, int variant, double cutoff, long *stats, bool mark);
+void sx_ecp_solve_learned(sx_ctx *c, int variant, const EcpPar &par, long *stats, bool mark);
 
 static EcpScreen make_screen(sx_ctx *c, int kind, const Params &P)
 {
     EcpScreen sc;
-    sc.variant = kind == SX_ECPR ? 1 : 0;
-    sc.cutoff = kind == SX_ECPR ? 0.0 : P.p[0];
+    sc.fvar = kind == SX_ECP ? 0 : kind == SX_ECPR ? 1 : kind == SX_ECPW ? 2 : kind == SX_ECPC ? 3 : 4;
+    sc.par.cutoff = sc.fvar == 1 ? 0.0 : P.p[0];
+    sc.par.wf = (sc.fvar == 2 || sc.fvar == 4) ? P.p[1] : 0.0;
+    sc.par.chir = sc.fvar == 3 ? P.p[1] : (sc.fvar == 4 ? P.p[2] : 0.0);
+    sc.variant = sc.fvar == 1 ? 1 : 0;
+    sc.cutoff = sc.par.cutoff + sc.par.wf;
     sc.periodic = c->periodic;
     for (int d = 0; d < 3; ++d) {
         sc.L[d] = c->box[d];
@@ -507,18 +523,22 @@ static void ecp_stage(sx_ctx *c, int stage, int kind, const EcpScreen &sc, const
     case 2: {
         long *stats = L.stats ? c->d_counts : nullptr;
         if (c->ecp_solver == SX_ECP_LEARNED) {
-            sx_ecp_solve_learned(c, kind == SX_ECPR ? 1 : 0, sc.cutoff, stats, true);
+            sx_ecp_solve_learned(c, sc.fvar, sc.par, stats, true);
             break;
         }
         const float *lam0 = nullptr;
-        if (kind == SX_ECPR)
-            hipLaunchKernelGGL(k_ecp_solve<1>, dim3(ECP_NSEG), dim3(ECP_SOLVE_THREADS), 0, c->stream, s, c->d_ecp_pp,
-                               ld, nl.half_i, nl.half_j, c->d_work, c->d_segcount, nnz_half, sc, c->ecp_solver, lam0,
-                               c->d_pairbuf, c->d_live, stats, nl.fwdpos, nl.revpos, c->d_idx_full);
-        else
-            hipLaunchKernelGGL(k_ecp_solve<0>, dim3(ECP_NSEG), dim3(ECP_SOLVE_THREADS), 0, c->stream, s, c->d_ecp_pp,
-                               ld, nl.half_i, nl.half_j, c->d_work, c->d_segcount, nnz_half, sc, c->ecp_solver, lam0,
-                               c->d_pairbuf, c->d_live, stats, nl.fwdpos, nl.revpos, c->d_idx_full);
+#define SX_ECP_SOLVE(V)                                                                                          \
+    hipLaunchKernelGGL(k_ecp_solve<V>, dim3(ECP_NSEG), dim3(ECP_SOLVE_THREADS), 0, c->stream, s, c->d_ecp_pp, ld, \
+                       nl.half_i, nl.half_j, c->d_work, c->d_segcount, nnz_half, sc, c->ecp_solver, lam0,         \
+                       c->d_pairbuf, c->d_live, stats, nl.fwdpos, nl.revpos, c->d_idx_full)
+        switch (sc.fvar) {
+        case 0: SX_ECP_SOLVE(0); break;
+        case 1: SX_ECP_SOLVE(1); break;
+        case 2: SX_ECP_SOLVE(2); break;
+        case 3: SX_ECP_SOLVE(3); break;
+        default: SX_ECP_SOLVE(4); break;
+        }
+#undef SX_ECP_SOLVE
         break;
     }
     case 3:
@@ -610,7 +630,7 @@ __global__ void __launch_bounds__(ECP_SOLVE_THREADS) k_ecp_lambda(DState s, cons
 // state (the first two stages of a force call)
 void sx_ecp_prepare(sx_ctx *c, int kind, const double *p, int np)
 {
-    if (kind != SX_ECP && kind != SX_ECPR) throw sx_error{SX_ERR_ARG, "an ECP kind is required"};
+    if (!is_ecp_kind(kind)) throw sx_error{SX_ERR_ARG, "an ECP kind is required"};
     if (!c->nl.init) throw sx_error{SX_ERR_STATE, "neighbour list not initialised (sx_nlist_init)"};
     check_params(kind, np);
     sx_nlist_want_half(c);
@@ -635,7 +655,7 @@ void sx_forces_impl(sx_ctx *c, int kind, const double *p, int np)
     check_params(kind, np);
     if (!c->nl.init) throw sx_error{SX_ERR_STATE, "neighbour list not initialised (sx_nlist_init)"};
     Params P = make_params(c, kind, p, np);
-    if (kind == SX_ECP || kind == SX_ECPR) {
+    if (is_ecp_kind(kind)) {
         ecp_forces_impl(c, kind, P);
     } else {
         int lpp = choose_lpp(c, kind);
@@ -648,6 +668,9 @@ void sx_forces_impl(sx_ctx *c, int kind, const double *p, int np)
         case SX_DH: launch_gather<SX_DH, false>(c, P, lpp); break;
         case SX_GB: launch_gather<SX_GB, true>(c, P, lpp); break;
         case SX_GBR: launch_gather<SX_GBR, true>(c, P, lpp); break;
+        case SX_GBW: launch_gather<SX_GBW, true>(c, P, lpp); break;
+        case SX_GBC: launch_gather<SX_GBC, true>(c, P, lpp); break;
+        case SX_GBWC: launch_gather<SX_GBWC, true>(c, P, lpp); break;
         }
     }
     // the step driver folds this reduction into the final-step kernel
@@ -660,7 +683,7 @@ int sx_forces(sx_ctx *c, int kind, const double *params, int nparams)
 {
     return guard(c, [&] {
         need_system(c);
-        if ((kind == SX_ECP || kind == SX_ECPR) && c->nl.init) sx_nlist_want_half(c);
+        if (is_ecp_kind(kind) && c->nl.init) sx_nlist_want_half(c);
         sx_forces_impl(c, kind, params, nparams);
     });
 }
@@ -669,7 +692,7 @@ int sx_profile_ecp(sx_ctx *c, int kind, const double *params, int nparams, int n
 {
     return guard(c, [&] {
         need_system(c);
-        if (kind != SX_ECP && kind != SX_ECPR) throw sx_error{SX_ERR_ARG, "profile_ecp needs an ECP kind"};
+        if (!is_ecp_kind(kind)) throw sx_error{SX_ERR_ARG, "profile_ecp needs an ECP kind"};
         if (!c->nl.init) throw sx_error{SX_ERR_STATE, "neighbour list not initialised (sx_nlist_init)"};
         check_params(kind, nparams);
         sx_nlist_want_half(c);
@@ -758,7 +781,7 @@ int sx_pair_eval(sx_ctx *c, int kind, const double *params, int nparams, long np
         HIPCHK(hipMemcpyAsync(di, hi.data(), sizeof(int) * np, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(dj, hj.data(), sizeof(int) * np, hipMemcpyHostToDevice, c->stream));
         DState s = c->dstate();
-        if (kind == SX_ECP || kind == SX_ECPR) {
+        if (is_ecp_kind(kind)) {
             ensure_pp(c);
             hipLaunchKernelGGL(k_ecp_pp, dim3(sx_blocks(c->n, 256)), dim3(256), 0, c->stream, s, c->d_ecp_pp,
                                c->d_ecp_scr, c->d_pos4, (int *)nullptr);
@@ -777,6 +800,12 @@ int sx_pair_eval(sx_ctx *c, int kind, const double *params, int nparams, long np
         case SX_GBR: hipLaunchKernelGGL(k_pair_eval<SX_GBR>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
         case SX_ECP: hipLaunchKernelGGL(k_pair_eval<SX_ECP>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
         case SX_ECPR: hipLaunchKernelGGL(k_pair_eval<SX_ECPR>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
+        case SX_GBW: hipLaunchKernelGGL(k_pair_eval<SX_GBW>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
+        case SX_GBC: hipLaunchKernelGGL(k_pair_eval<SX_GBC>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
+        case SX_GBWC: hipLaunchKernelGGL(k_pair_eval<SX_GBWC>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
+        case SX_ECPW: hipLaunchKernelGGL(k_pair_eval<SX_ECPW>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
+        case SX_ECPC: hipLaunchKernelGGL(k_pair_eval<SX_ECPC>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
+        case SX_ECPWC: hipLaunchKernelGGL(k_pair_eval<SX_ECPWC>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
         }
         std::vector<double> h(11 * np);
         HIPCHK(hipMemcpyAsync(h.data(), dout, sizeof(double) * 11 * np, hipMemcpyDeviceToHost, c->stream));
@@ -864,17 +893,21 @@ extern "C" int sx_ecp_compare_solvers(sx_ctx *c, int kind, const double *params,
                                c->d_segcount, nnz_half, sc, lam0, lam_w, (long *)nullptr);
         });
         auto full = [&](int solver, const float *l0) {
-            if (kind == SX_ECPR)
-                hipLaunchKernelGGL(k_ecp_solve<1>, g, b, 0, c->stream, s, c->d_ecp_pp, 0L, nl.half_i, nl.half_j,
-                                   c->d_work, c->d_segcount, nnz_half, sc, solver, l0, c->d_pairbuf, c->d_live,
-                                   (long *)nullptr, (const int *)nullptr, (const int *)nullptr, (int *)nullptr);
-            else
-                hipLaunchKernelGGL(k_ecp_solve<0>, g, b, 0, c->stream, s, c->d_ecp_pp, 0L, nl.half_i, nl.half_j,
-                                   c->d_work, c->d_segcount, nnz_half, sc, solver, l0, c->d_pairbuf, c->d_live,
-                                   (long *)nullptr, (const int *)nullptr, (const int *)nullptr, (int *)nullptr);
+#define SX_ECP_FULL(V)                                                                                            \
+    hipLaunchKernelGGL(k_ecp_solve<V>, g, b, 0, c->stream, s, c->d_ecp_pp, 0L, nl.half_i, nl.half_j, c->d_work,  \
+                       c->d_segcount, nnz_half, sc, solver, l0, c->d_pairbuf, c->d_live, (long *)nullptr,        \
+                       (const int *)nullptr, (const int *)nullptr, (int *)nullptr)
+            switch (sc.fvar) {
+            case 0: SX_ECP_FULL(0); break;
+            case 1: SX_ECP_FULL(1); break;
+            case 2: SX_ECP_FULL(2); break;
+            case 3: SX_ECP_FULL(3); break;
+            default: SX_ECP_FULL(4); break;
+            }
+#undef SX_ECP_FULL
         };
         double t_full_plain = timed([&] { full(SX_ECP_CONVERGED, nullptr); });
-        double t_full_learned = timed([&] { sx_ecp_solve_learned(c, kind == SX_ECPR ? 1 : 0, sc.cutoff, nullptr, false); });
+        double t_full_learned = timed([&] { sx_ecp_solve_learned(c, sc.fvar, sc.par, nullptr, false); });
         // iteration counts (4 counters per solver)
         hipLaunchKernelGGL(k_ecp_lambda<3>, g, b, 0, c->stream, s, c->d_ecp_pp, nl.half_i, nl.half_j, c->d_work,
                            c->d_segcount, nnz_half, sc, lam0, lam_b, cnt + 0);

@@ -239,12 +239,14 @@ __device__ __forceinline__ void gb_self(const DState &s, int a, int mu, GbSelf &
     g.epsz = epar;
 }
 
-// gb (variant 0, cutoff + switch) and gbr (variant 1) for I = a with factors A,
-// J with axis u2, RIJ = r; cutoff, mu, nu as in p
+// gb_ (variant 0, cutoff + switch), gbr_ (1), gbw_ (2, weld width wf), gbc_
+// (3, chirality) and gbwc_ (4, both) for I = a with factors A, J with axis
+// u2, RIJ = r
 template <int VARIANT>
 __device__ __forceinline__ void gb_core(const double r[3], const GbSelf &A, const double u2[3], double cutoff, int mu,
-                                        int nu, PairOut &o)
+                                        int nu, PairOut &o, double wf = 0.0, double chir = 0.0)
 {
+    constexpr bool welded = VARIANT == 2 || VARIANT == 4, chiral = VARIANT == 3 || VARIANT == 4;
     const double sigma_0 = 1.0;
     GbGeo q;
     o.pe = 0.0;
@@ -265,19 +267,23 @@ __device__ __forceinline__ void gb_core(const double r[3], const GbSelf &A, cons
     double sigma_0_inv = 1.0 / sigma_0;
     double g32 = sigma / gchi;
     double switch_ = 0, factor = 0, sx = 0, sdx = 0, rlim;
-    if (VARIANT == 0) {
-        switch_ = 0.9 * (sigma + cutoff - sigma_0);
-        factor = 1.0 / ((sigma + cutoff - sigma_0) - switch_);
+    if (VARIANT != 1) {
+        rlim = welded ? (sigma + cutoff + wf - sigma_0) : (sigma + cutoff - sigma_0);
+        switch_ = 0.9 * rlim;
+        factor = 1.0 / (rlim - switch_);
         double xx = (rij_mag - switch_) * factor;
         sx = 1.0 - 6.0 * xx * xx * xx * xx * xx + 15.0 * xx * xx * xx * xx - 10.0 * xx * xx * xx;
         sdx = -30.0 * xx * xx * xx * xx + 60.0 * xx * xx * xx - 30.0 * xx * xx;
-        rlim = sigma + cutoff - sigma_0;
     } else {
         rlim = sigma;
     }
-    bool inside = (VARIANT == 0) ? (rij_mag < rlim) : (rij_mag <= rlim);
+    bool inside = (VARIANT == 1) ? (rij_mag <= rlim) : (rij_mag < rlim);
     if (!inside) return;
-    double sr = sigma_0 / (rij_mag - sigma + sigma_0);
+    double sr;
+    if (!welded) sr = sigma_0 / (rij_mag - sigma + sigma_0);
+    else if (rij_mag < sigma) sr = sigma_0 / (rij_mag - sigma + sigma_0);
+    else if (rij_mag >= sigma && rij_mag <= sigma + wf) sr = 1.0;
+    else sr = sigma_0 / (rij_mag - sigma + sigma_0 - wf);
     double sr2 = sr * sr, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
     double srx = (sr12 - 2.0 * sr6);
     if (VARIANT == 1) srx = srx + 1.0;
@@ -292,23 +298,74 @@ __device__ __forceinline__ void gb_core(const double r[3], const GbSelf &A, cons
     gb_dgdu(xhi, q, 2, dgu2_x);
     gb_dgdu(chi, q, 2, dgu2_c);
     double TI[3], TJ[3];
-    for (int k = 0; k < 3; ++k) {
-        double dR_dr = sigma_0_inv * (rhat[k] + 0.5 * dgr_c[k] * g32);
-        double de2_dr = e1n * (mu * e2m1) * dgr_x[k];
-        double de1_du1 = e2m * (nu * e1n2) * (chi * chi * q.uu * q.u2[k]);
-        double de2_du1 = e1n * (mu * e2m1) * dgu1_x[k];
-        double dR_du1 = srd * 0.5 * sigma_0_inv * dgu1_c[k] * g32;
-        double de1_du2 = e2m * (nu * e1n2) * (chi * chi * q.uu * q.u1[k]);
-        double de2_du2 = e1n * (mu * e2m1) * dgu2_x[k];
-        double dR_du2 = srd * 0.5 * sigma_0_inv * dgu2_c[k] * g32;
-        o.f[k] = (-1.0) * (e1n * e2m * srd * dR_dr + de2_dr * srx);
-        TI[k] = (-1.0) * (srx * (de1_du1 + de2_du1) + (e1n * e2m * dR_du1));
-        TJ[k] = (-1.0) * (srx * (de1_du2 + de2_du2) + (e1n * e2m * dR_du2));
-    }
     double pot = e1n * e2m * srx;
-    cross3(q.u1, TI, o.ti);
-    cross3(q.u2, TJ, o.tj);
-    if (VARIANT == 0 && (rij_mag >= switch_) && (rij_mag <= rlim)) {
+    if (!chiral) {
+        for (int k = 0; k < 3; ++k) {
+            double dR_dr = sigma_0_inv * (rhat[k] + 0.5 * dgr_c[k] * g32);
+            double de2_dr = e1n * (mu * e2m1) * dgr_x[k];
+            double de1_du1 = e2m * (nu * e1n2) * (chi * chi * q.uu * q.u2[k]);
+            double de2_du1 = e1n * (mu * e2m1) * dgu1_x[k];
+            double dR_du1 = srd * 0.5 * sigma_0_inv * dgu1_c[k] * g32;
+            double de1_du2 = e2m * (nu * e1n2) * (chi * chi * q.uu * q.u1[k]);
+            double de2_du2 = e1n * (mu * e2m1) * dgu2_x[k];
+            double dR_du2 = srd * 0.5 * sigma_0_inv * dgu2_c[k] * g32;
+            o.f[k] = (-1.0) * (e1n * e2m * srd * dR_dr + de2_dr * srx);
+            TI[k] = (-1.0) * (srx * (de1_du1 + de2_du1) + (e1n * e2m * dR_du1));
+            TJ[k] = (-1.0) * (srx * (de1_du2 + de2_du2) + (e1n * e2m * dR_du2));
+        }
+        cross3(q.u1, TI, o.ti);
+        cross3(q.u2, TJ, o.tj);
+    } else {
+        // gbc_ / gbwc_ (force.f90:751-801, 931-981): the plain terms regrouped
+        // (energy factors inside dR, srx / srd outside), then the
+        // chirality-weighted pseudoscalar terms
+        double srxc, srdc;
+        if (welded) {
+            srxc = (sr12 - (12.0 / 7.0) * sr6 * sr);
+            srdc = 12.0 * (sr6 * sr2 - sr12 * sr);
+        } else {
+            srxc = sr6 * sr;
+            srdc = -7.0 * sr6 * sr2;
+        }
+        double u1xu2[3], u2xrij[3], rijxu1[3];
+        cross3(q.u1, q.u2, u1xu2);
+        cross3(q.u2, rhat, u2xrij);
+        cross3(rhat, q.u1, rijxu1);
+        const double uur = u1xu2[0] * rhat[0] + u1xu2[1] * rhat[1] + u1xu2[2] * rhat[2];
+        const double uu = q.uu;
+        double FC[3], TIc[3], TJc[3];
+        for (int k = 0; k < 3; ++k) {
+            double dR_dr = e1n * e2m * sigma_0_inv * (rhat[k] + 0.5 * dgr_c[k] * g32);
+            double de2_dr = e1n * (mu * e2m1) * dgr_x[k];
+            double de1_du1 = e2m * (nu * e1n2) * (chi * chi * q.uu * q.u2[k]);
+            double de2_du1 = e1n * (mu * e2m1) * dgu1_x[k];
+            double dR_du1 = e1n * e2m * 0.5 * sigma_0_inv * dgu1_c[k] * g32;
+            double de1_du2 = e2m * (nu * e1n2) * (chi * chi * q.uu * q.u1[k]);
+            double de2_du2 = e1n * (mu * e2m1) * dgu2_x[k];
+            double dR_du2 = e1n * e2m * 0.5 * sigma_0_inv * dgu2_c[k] * g32;
+            o.f[k] = (-1.0) * (srx * de2_dr + srd * dR_dr);
+            TI[k] = (-1.0) * (srx * (de1_du1 + de2_du1) + srd * dR_du1);
+            TJ[k] = (-1.0) * (srx * (de1_du2 + de2_du2) + srd * dR_du2);
+            double duur_r = e1n * e2m * srxc * uu * (u1xu2[k] - uur * rhat[k]) / rij_mag;
+            FC[k] = (-1.0) * (de2_dr * srxc * uur * uu + dR_dr * srdc * uur * uu + duur_r);
+            TIc[k] = (-1.0) * (de1_du1 * srxc * uur * uu + de2_du1 * srxc * uur * uu + dR_du1 * srdc * uur * uu +
+                               e1n * e2m * srxc * uu * u2xrij[k] + e1n * e2m * srxc * uur * q.u2[k]);
+            TJc[k] = (-1.0) * (de1_du2 * srxc * uur * uu + de2_du2 * srxc * uur * uu + dR_du2 * srdc * uur * uu +
+                               e1n * e2m * srxc * uu * rijxu1[k] + e1n * e2m * srxc * uur * q.u1[k]);
+        }
+        double T1[3], T2[3], T1c[3], T2c[3];
+        cross3(q.u1, TI, T1);
+        cross3(q.u2, TJ, T2);
+        cross3(q.u1, TIc, T1c);
+        cross3(q.u2, TJc, T2c);
+        pot = pot + chir * (e1n * e2m * srxc * uur * uu);
+        for (int k = 0; k < 3; ++k) {
+            o.f[k] = o.f[k] + chir * FC[k];
+            o.ti[k] = T1[k] + chir * T1c[k];
+            o.tj[k] = T2[k] + chir * T2c[k];
+        }
+    }
+    if (VARIANT != 1 && (rij_mag >= switch_) && (rij_mag <= rlim)) {
         for (int k = 0; k < 3; ++k) {
             o.f[k] = sx * o.f[k] - sdx * pot * factor;
             o.ti[k] = sx * o.ti[k] - sdx * pot * factor;
@@ -325,12 +382,14 @@ template <int VARIANT>
 __device__ __forceinline__ void d_gb(const DState &s, int a, int b, const double *p, PairOut &o)
 {
     int mu = (int)p[1], nu = (int)p[2];
+    const double wf = (VARIANT == 2 || VARIANT == 4) ? p[3] : 0.0;
+    const double chir = VARIANT == 3 ? p[3] : (VARIANT == 4 ? p[4] : 0.0);
     double r[3], u2[3];
     GbSelf A;
     pair_rij(s, a, b, r);
     gb_self(s, a, mu, A);
     quat_u(s, b, u2);
-    gb_core<VARIANT>(r, A, u2, p[0], mu, nu, o);
+    gb_core<VARIANT>(r, A, u2, p[0], mu, nu, o, wf, chir);
 }
 
 // isotropic dispatch
@@ -345,4 +404,7 @@ __device__ __forceinline__ void d_pair(const DState &s, int a, int b, const doub
     else if (KIND == SX_DH) d_dh(s, a, b, p, o);
     else if (KIND == SX_GB) d_gb<0>(s, a, b, p, o);
     else if (KIND == SX_GBR) d_gb<1>(s, a, b, p, o);
+    else if (KIND == SX_GBW) d_gb<2>(s, a, b, p, o);
+    else if (KIND == SX_GBC) d_gb<3>(s, a, b, p, o);
+    else if (KIND == SX_GBWC) d_gb<4>(s, a, b, p, o);
 }

@@ -37,7 +37,11 @@ static void enqueue_step(sx_ctx *c, const sx_step_spec &sp)
         case SX_LJW: case SX_MORSE: np = 4; break;
         case SX_WCA: np = 2; break;
         case SX_GB: case SX_GBR: np = 3; break;
+        case SX_GBW: case SX_GBC: np = 4; break;
+        case SX_GBWC: np = 5; break;
         case SX_ECP: np = 1; break;
+        case SX_ECPW: case SX_ECPC: np = 2; break;
+        case SX_ECPWC: np = 3; break;
         default: np = 0;
         }
         sx_forces_impl(c, sp.kinds[f], sp.params[f], np);
@@ -69,7 +73,9 @@ extern "C" int sx_run(sx_ctx *c, const sx_step_spec *spec, int nsteps)
         if (spec->integrator != SX_INT_VV && !c->have_target_temp)
             throw sx_error{SX_ERR_NOTEMP, "Error: Target temperature not set."};
         for (int f = 0; f < spec->nforces; ++f)
-            if (spec->kinds[f] == SX_ECP || spec->kinds[f] == SX_ECPR) sx_nlist_want_half(c);
+            if (spec->kinds[f] == SX_ECP || spec->kinds[f] == SX_ECPR || spec->kinds[f] == SX_ECPW ||
+                spec->kinds[f] == SX_ECPC || spec->kinds[f] == SX_ECPWC)
+                sx_nlist_want_half(c);
         if (!c->d_step) HIPCHK(hipMalloc(&c->d_step, sizeof(int)));
         hipLaunchKernelGGL(k_step_set, dim3(1), dim3(1), 0, c->stream, c->d_step, spec->first_step);
         if (nsteps == 0) return;

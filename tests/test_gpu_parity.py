@@ -11,7 +11,10 @@ list and the Philox words are compared for exact equality.
 import numpy as np
 import pytest
 
-from oracle.oracle import (DH, ECP, ECPR, GB, GBR, LGV, LGVC, LJ, LJ12, LJW, MORSE, NHT, QQ, VV, VVC, WCA)
+from oracle.oracle import (DH, ECP, ECPC, ECPR, ECPW, ECPWC, GB, GBC, GBR, GBW, GBWC, LGV, LGVC, LJ, LJ12, LJW, MORSE,
+                           NHT, QQ, VV, VVC, WCA)
+
+ECP_KINDS = (ECP, ECPR, ECPW, ECPC, ECPWC)
 from tests._states import base_state, chains, from_device, rel_err, to_device
 
 pytestmark = pytest.mark.gpu
@@ -51,6 +54,15 @@ PAIR_CASES = [
     (ECP, (1.5,), dict(sig=(1, 1, 3))),
     (ECPR, (), dict(sig=(1, 1, 3))),
     (ECPR, (), dict(sig=(1.0, 2.0, 3.0), eps=(1.0, 2.0, 3.0))),
+    # welded (weld width 0.3) and chiral (chirality 0.7) forms
+    (GBW, (4.0, 1, 2, 0.3), dict(sig=(1, 1, 3), eps=(5, 5, 1))),
+    (GBC, (4.0, 1, 2, 0.7), dict(sig=(1, 1, 3), eps=(5, 5, 1))),
+    (GBWC, (4.0, 1, 2, 0.3, 0.7), dict(sig=(1, 1, 3), eps=(5, 5, 1))),
+    (GBWC, (3.0, 2, 1, 0.5, -1.3), dict(sig=(1.0, 2.0, 3.0), eps=(1.0, 2.0, 3.0))),
+    (ECPW, (1.5, 0.3), dict(sig=(1, 1, 3))),
+    (ECPC, (1.5, 0.7), dict(sig=(1, 1, 3))),
+    (ECPWC, (1.5, 0.3, 0.7), dict(sig=(1, 1, 3))),
+    (ECPWC, (1.2, 0.5, -1.3), dict(sig=(1.0, 2.0, 3.0), eps=(1.0, 2.0, 3.0))),
 ]
 
 
@@ -72,7 +84,7 @@ def test_pair_eval(dev, oracle, kind, params, kw):
     for t in range(len(i)):
         r = oracle.pair(kind, st, i[t], j[t], p if kind != ECPR else [0.0])
         ope[t], of[t], oti[t], otj[t] = r[0], r[1], r[2], r[3]
-        if kind in (ECP, ECPR):
+        if kind in ECP_KINDS:
             olam[t] = r[4]
     nz = np.count_nonzero(ope)
     say(f"  {nz} of {len(i)} pairs non-zero")
@@ -82,7 +94,7 @@ def test_pair_eval(dev, oracle, kind, params, kw):
     for a, b in ((f, of), (ti, oti), (tj, otj)):
         s = max(np.abs(b).max(), 1e-300)
         assert np.max(np.abs(a - b)) <= TOL * s
-    if kind in (ECP, ECPR):
+    if kind in ECP_KINDS:
         assert np.array_equal(lam, olam), "Brent lambda must follow the reference's path exactly"
 
 
@@ -129,6 +141,12 @@ FORCE_CASES = [
     (GB, (4.0, 2, 1), 6.0, dict(sig=(1, 1, 3), eps=(5, 5, 1)), 1500, 20.0),
     (ECPR, (), 3.1, dict(sig=(1, 1, 3)), 2000, 24.0),
     (ECP, (1.2,), 3.3, dict(sig=(1, 1, 3)), 2000, 24.0),
+    (GBW, (4.0, 1, 2, 0.3), 6.4, dict(sig=(1, 1, 3), eps=(5, 5, 1)), 1500, 20.0),
+    (GBC, (4.0, 1, 2, 0.7), 6.0, dict(sig=(1, 1, 3), eps=(5, 5, 1)), 1500, 20.0),
+    (GBWC, (4.0, 1, 2, 0.3, 0.7), 6.4, dict(sig=(1, 1, 3), eps=(5, 5, 1)), 1500, 20.0),
+    (ECPW, (1.2, 0.3), 3.6, dict(sig=(1, 1, 3)), 2000, 24.0),
+    (ECPC, (1.2, 0.7), 3.3, dict(sig=(1, 1, 3)), 2000, 24.0),
+    (ECPWC, (1.2, 0.3, 0.7), 3.6, dict(sig=(1, 1, 3)), 2000, 24.0),
 ]
 
 
@@ -156,6 +174,54 @@ def test_forces_pair_loop(dev, oracle, kind, params, rc, kw, n, L):
         assert e < TOL, (f, e)
     assert abs(got["pot_eng"] - ref["pot_eng"]) <= TOL * max(abs(ref["pot_eng"]), 1.0)
     assert np.any(ref["fx"] != 0)
+
+
+@pytest.mark.parametrize("kind,params,rc", [(ECP, (1.2,), 3.3), (ECPR, (), 3.1), (ECPW, (1.2, 0.3), 3.6),
+                                             (ECPC, (1.2, 0.7), 3.3), (ECPWC, (1.2, 0.3, 0.7), 3.6)])
+def test_ecp_converged_and_learned_forces(dev, oracle, kind, params, rc):
+    """The ECP pipeline with the converged solver (and with the learned warm
+    start, here an untrained network) against the oracle's pair terms
+    evaluated at the oracle's own converged contact parameter, summed over
+    the oracle's neighbour list."""
+    import sucrose_amd as sx
+    say(f"ECP forces kind={kind}: converged and learned solvers vs oracle")
+    rng = np.random.default_rng(40 + kind)
+    n, L = 600, 14.0
+    st = base_state(rng, n, L, forces=False, sig=(1, 1, 3))
+    d = sx.System(0)
+    to_device(d, st)
+    d.init_nlist(0.3, rc)
+    d.mlp_init(seed=3)
+    got = {}
+    for solver in (sx.ECP_CONVERGED, sx.ECP_LEARNED):
+        d.set_ecp_solver(solver)
+        d.initialize_forces()
+        d.forces(kind, *params)
+        got[solver] = (np.stack([d.download(f) for f in ("FX", "FY", "FZ", "TX", "TY", "TZ")]),
+                       d.get_scalar("POT_ENG"))
+    d.close()
+    nl = oracle.nlist_new(st, rc, 0.3)
+    oracle.nlist_build(st, nl)
+    p = list(params) if params else [0.0]
+    F = np.zeros((6, n))
+    pe = 0.0
+    off, nb = nl["offset"], nl["nbr"]
+    for i in range(n - 1):
+        for j in nb[off[i]:off[i + 1]]:
+            ar, br, r = oracle.ecp_shape(st, i, j)
+            lr, _ = oracle.lambda_converged(ar, br, r)
+            e, f, ti, tj = oracle.ecp_at_lambda(kind, st, i, j, p, lr)
+            pe += e
+            F[0:3, i] += f
+            F[0:3, j] -= f
+            F[3:6, i] += ti
+            F[3:6, j] += tj
+    assert np.count_nonzero(F[0]) > n // 4
+    for solver, (g, gpe) in got.items():
+        for c in range(6):
+            e = rel_err(g[c], F[c])
+            assert e < TOL, (solver, c, e)
+        assert abs(gpe - pe) <= TOL * max(abs(pe), 1.0), (solver, gpe, pe)
 
 
 def _integrate_case(dev, oracle, kind, phase, constrained=False, n=640):
