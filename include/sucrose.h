@@ -166,9 +166,10 @@ int sx_ecp_stats(sx_ctx *ctx, long out[4]);
  * Newton iterations, calls, calls that fell back to the serial walk. */
 int sx_nht_stats(sx_ctx *ctx, long out[3]);
 
-/* bonded terms (module bonded, force.f90:2244-2462); kind 1 bend_, 2 cosine_bend_,
- * 3 dih_, 4 cosine_dih_; idx1 holds 3 (bends) or 4 (dihedrals) 1-based atoms
- * per term.  Registered terms are evaluated by sx_bonded_forces. */
+/* bonded terms (module bonded, force.f90:2244-2579); kind 1 bend_, 2 cosine_bend_,
+ * 3 dih_, 4 cosine_dih_, 5 angle_bend_ (:2464, harmonic in the angle),
+ * 6 angle_dih_ (:2508); idx1 holds 3 (bends: 1, 2, 5) or 4 (dihedrals: 3, 4, 6)
+ * 1-based atoms per term.  Registered terms are evaluated by sx_bonded_forces. */
 int sx_set_bonded(sx_ctx *ctx, int kind, int count, const int *idx1, double coeff, double angle);
 int sx_bonded_forces(sx_ctx *ctx);
 

@@ -1167,6 +1167,11 @@ static void bend_common(const or_sys *s, int i, int j, int k, double strength, i
     if (mode == 0) { /* bend_ : k (1 - cos) */
         *pe = coeff * (1.0 - prefac * fac);
         strength = coeff;
+    } else if (mode == 2) { /* angle_bend_ (force.f90:2464-2506): k/2 (theta - theta0)^2 */
+        double theta0 = PI_REF - (PI_REF / 180.0) * angle;
+        double theta = acos(prefac * fac);
+        strength = coeff * (theta - theta0) / sin(theta);
+        *pe = (coeff / 2.0) * (theta - theta0) * (theta - theta0);
     } else {         /* cosine_bend_ */
         double theta0 = PI_REF - (PI_REF / 180.0) * angle;
         double cos_theta0 = cos(theta0);
@@ -1216,6 +1221,16 @@ static void dih_common(const or_sys *s, int i, int j, int k, int l, int mode,
     if (mode == 0) { /* dih_ : k (1 + prefac*fac) */
         *pe = coeff * (1.0 + prefac * fac);
         strength = coeff;
+    } else if (mode == 2) { /* angle_dih_ (force.f90:2508-2579): k (1 - cos(phi - phi0)) */
+        double vixvj[3];
+        cross(VI, VJ, vixvj);
+        double fac_sin = vixvj[0] * VK[0] + vixvj[1] * VK[1] + vixvj[2] * VK[2];
+        double phi0 = PI_REF - (PI_REF / 180.0) * angle;
+        double cos_phi0 = cos(phi0), sin_phi0 = sin(phi0);
+        double cos_phi = -prefac * fac;
+        double sin_phi = prefac * fac_sin * sqrt(VJ[0] * VJ[0] + VJ[1] * VJ[1] + VJ[2] * VJ[2]);
+        strength = coeff * (cos_phi0 - sin_phi0 * (cos_phi / sin_phi));
+        *pe = coeff * (1.0 - cos_phi * cos_phi0 - sin_phi * sin_phi0);
     } else {         /* cosine_dih_ */
         double vixvj[3];
         cross(VI, VJ, vixvj);
@@ -1238,6 +1253,12 @@ static void dih_common(const or_sys *s, int i, int j, int k, int l, int mode,
     }
 }
 
+void or_angle_bend(const or_sys *s, int i, int j, int k, double coeff, double angle,
+                   double *pe, double fi[3], double fj[3], double fk[3])
+{
+    bend_common(s, i, j, k, 0.0, 2, coeff, angle, pe, fi, fj, fk);
+}
+
 void or_dih(const or_sys *s, int i, int j, int k, int l, double coeff,
             double *pe, double fi[3], double fj[3], double fk[3], double fl[3])
 {
@@ -1248,6 +1269,12 @@ void or_cosine_dih(const or_sys *s, int i, int j, int k, int l, double coeff, do
                    double *pe, double fi[3], double fj[3], double fk[3], double fl[3])
 {
     dih_common(s, i, j, k, l, 1, coeff, angle, pe, fi, fj, fk, fl);
+}
+
+void or_angle_dih(const or_sys *s, int i, int j, int k, int l, double coeff, double angle,
+                  double *pe, double fi[3], double fj[3], double fk[3], double fl[3])
+{
+    dih_common(s, i, j, k, l, 2, coeff, angle, pe, fi, fj, fk, fl);
 }
 
 /* ------------------------------------------------------------------ */

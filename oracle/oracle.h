@@ -127,6 +127,10 @@ void or_dih(const or_sys *s, int i, int j, int k, int l, double coeff,
             double *pe, double fi[3], double fj[3], double fk[3], double fl[3]);
 void or_cosine_dih(const or_sys *s, int i, int j, int k, int l, double coeff, double angle,
                    double *pe, double fi[3], double fj[3], double fk[3], double fl[3]);
+void or_angle_bend(const or_sys *s, int i, int j, int k, double coeff, double angle,
+                   double *pe, double fi[3], double fj[3], double fk[3]);
+void or_angle_dih(const or_sys *s, int i, int j, int k, int l, double coeff, double angle,
+                  double *pe, double fi[3], double fj[3], double fk[3], double fl[3]);
 
 /* ---- integrators (integrators.f90) ---- */
 void or_vv_initial_step(or_sys *s, double dt);

@@ -308,6 +308,12 @@ class Oracle:
             L.or_dih(C.byref(s), *idx[:4], d(coeff), C.byref(pe), *rows)
         elif kind == 4:
             L.or_cosine_dih(C.byref(s), *idx[:4], d(coeff), d(angle), C.byref(pe), *rows)
+        elif kind == 5:
+            L.or_angle_bend(C.byref(s), idx[0], idx[1], idx[2], d(coeff), d(angle), C.byref(pe), *rows[:3])
+        elif kind == 6:
+            L.or_angle_dih(C.byref(s), *idx[:4], d(coeff), d(angle), C.byref(pe), *rows)
+        else:
+            raise ValueError(kind)
         return pe.value, f
 
     def pair_loop(self, st: State, nl: dict, kind, p):

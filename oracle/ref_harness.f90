@@ -249,6 +249,10 @@ contains
             call dih_calculate_forces(i1, j1, k1, l1, pe, f(:, 1), f(:, 2), f(:, 3), f(:, 4), coeff)
         case (4)
             call cosine_dih_calculate_forces(i1, j1, k1, l1, pe, f(:, 1), f(:, 2), f(:, 3), f(:, 4), coeff, angle)
+        case (5)
+            call angle_bend_calculate_forces(i1, j1, k1, pe, f(:, 1), f(:, 2), f(:, 3), coeff, angle)
+        case (6)
+            call angle_dih_calculate_forces(i1, j1, k1, l1, pe, f(:, 1), f(:, 2), f(:, 3), f(:, 4), coeff, angle)
         end select
     end subroutine ref_bonded
 

@@ -457,7 +457,7 @@ class System:
 
     def set_bonded(self, kind, idx1, coeff, angle=0.0):
         a = np.ascontiguousarray(idx1, dtype=np.int32).reshape(-1)
-        npos = 3 if kind <= 2 else 4
+        npos = 3 if kind in (1, 2, 5) else 4
         self._ok(self.L.sx_set_bonded(self.h, int(kind), len(a) // npos, a.ctypes.data_as(_PI),
                                       float(coeff), float(angle)))
 

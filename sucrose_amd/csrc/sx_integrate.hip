@@ -827,14 +827,20 @@ __device__ __forceinline__ void bvec(const DState &s, int a, int b, double v[3])
         for (int k = 0; k < 3; ++k) v[k] = sx_mimg(v[k], s.box[k]);
 }
 
-__global__ void k_bonded_terms(DState s, int kind, int count, const int *idx, double coeff, double angle,
-                               double cos0, double *out)
+// bends: kind 1 bend_ (force.f90:2244), 2 cosine_bend_ (:2345), 5 angle_bend_
+// (:2464, harmonic in theta); dihedrals: 3 dih_ (:2286), 4 cosine_dih_
+// (:2392), 6 angle_dih_ (:2508).  th0 = pi - angle in radians, cos0/sin0 its
+// cosine and sine (computed once on the host).
+__device__ __forceinline__ bool bonded_is_bend(int kind) { return kind <= 2 || kind == 5; }
+
+__global__ void k_bonded_terms(DState s, int kind, int count, const int *idx, double coeff, double th0,
+                               double cos0, double sin0, double *out)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
     const int *q = idx + 4 * t;
     double *o = out + 13L * t;
-    if (kind <= 2) {
+    if (bonded_is_bend(kind)) {
         double VI[3], VJ[3];
         bvec(s, q[0], q[1], VI);
         bvec(s, q[1], q[2], VJ);
@@ -847,6 +853,10 @@ __global__ void k_bonded_terms(DState s, int kind, int count, const int *idx, do
         if (kind == 1) {
             pe = coeff * (1.0 - prefac * fac);
             strength = coeff;
+        } else if (kind == 5) {
+            double theta = acos(prefac * fac);
+            strength = coeff * (theta - th0) / sin(theta);
+            pe = (coeff / 2.0) * (theta - th0) * (theta - th0);
         } else {
             double cos_theta = prefac * fac;
             strength = coeff * (cos_theta - cos0);
@@ -885,9 +895,14 @@ __global__ void k_bonded_terms(DState s, int kind, int count, const int *idx, do
             double fac_sin = vixvj[0] * VK[0] + vixvj[1] * VK[1] + vixvj[2] * VK[2];
             double cos_phi = -prefac * fac;
             double sin_phi = prefac * fac_sin * sqrt(VJ[0] * VJ[0] + VJ[1] * VJ[1] + VJ[2] * VJ[2]);
-            double sg = atan2(sin_phi, cos_phi);
-            strength = coeff * (cos(sg) - cos0);
-            pe = (coeff / 2.0) * (cos(sg) - cos0) * (cos(sg) - cos0);
+            if (kind == 6) {
+                strength = coeff * (cos0 - sin0 * (cos_phi / sin_phi));
+                pe = coeff * (1.0 - cos_phi * cos0 - sin_phi * sin0);
+            } else {
+                double sg = atan2(sin_phi, cos_phi);
+                strength = coeff * (cos(sg) - cos0);
+                pe = (coeff / 2.0) * (cos(sg) - cos0) * (cos(sg) - cos0);
+            }
         }
         o[0] = pe;
         for (int k = 0; k < 3; ++k) {
@@ -899,7 +914,6 @@ __global__ void k_bonded_terms(DState s, int kind, int count, const int *idx, do
             o[1 + k] = -strength * prefac * (-CC23 * VJ[k] + CC22 * VK[k] - fac2 * (-CC22 * VI[k] + CC12 * VJ[k]));
         }
     }
-    (void)angle;
 }
 
 // atom_term entries: (global term slot * 4 + position), slots index a
@@ -1084,9 +1098,9 @@ void sx_bonded_impl(sx_ctx *c)
     if (c->bonded.empty()) return;
     const double PI_REF = 3.1415926535897932384626433832795;
     for (auto &b : c->bonded) {
-        double cos0 = std::cos(PI_REF - (PI_REF / 180.0) * b.angle);
+        double th0 = PI_REF - (PI_REF / 180.0) * b.angle;
         hipLaunchKernelGGL(k_bonded_terms, dim3(sx_blocks(b.count, 256)), dim3(256), 0, c->stream, c->dstate(), b.kind,
-                           b.count, b.idx, b.coeff, b.angle, cos0, b.out);
+                           b.count, b.idx, b.coeff, th0, std::cos(th0), std::sin(th0), b.out);
     }
     int nb = sx_blocks(c->n, 256);
     sx_ensure_part(c, nb, 1);
@@ -1106,7 +1120,7 @@ static void rebuild_bonded_index(sx_ctx *c)
     for (auto &b : c->bonded) {
         std::vector<int> h(4 * (size_t)b.count);
         HIPCHK(hipMemcpy(h.data(), b.idx, sizeof(int) * 4 * b.count, hipMemcpyDeviceToHost));
-        int npos = b.kind <= 2 ? 3 : 4;
+        int npos = (b.kind <= 2 || b.kind == 5) ? 3 : 4;
         for (int t = 0; t < b.count; ++t)
             for (int p = 0; p < npos; ++p) per[h[4 * t + p]].push_back((int)(((slot0 + t) << 2) | p));
         bufs.push_back(b.out);
@@ -1284,8 +1298,8 @@ int sx_set_bonded(sx_ctx *c, int kind, int count, const int *idx1, double coeff,
 {
     return guard(c, [&] {
         need_system(c);
-        if (kind < 1 || kind > 4 || count < 0) throw sx_error{SX_ERR_ARG, "bad bonded kind"};
-        int npos = kind <= 2 ? 3 : 4;
+        if (kind < 1 || kind > 6 || count < 0) throw sx_error{SX_ERR_ARG, "bad bonded kind"};
+        int npos = (kind <= 2 || kind == 5) ? 3 : 4;
         std::vector<int> h(4 * (size_t)count, 0);
         for (int t = 0; t < count; ++t)
             for (int p = 0; p < npos; ++p) {

@@ -74,7 +74,7 @@ struct NList {
 };
 
 struct Bonded {
-    int kind;        // 1 bend, 2 cosine_bend, 3 dih, 4 cosine_dih
+    int kind;        // 1 bend, 2 cosine_bend, 3 dih, 4 cosine_dih, 5 angle_bend, 6 angle_dih
     int count;
     double coeff, angle;
     int *idx = nullptr;   // count * 4 (0-based atoms; 4th unused for bends)

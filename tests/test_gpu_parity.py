@@ -370,22 +370,22 @@ def test_bonded_terms(dev, oracle):
     to_device(dev, st)
     bends = np.array([[c * 12 + b, c * 12 + b + 1, c * 12 + b + 2] for c in range(20) for b in range(10)])
     dihs = np.array([[c * 12 + b, c * 12 + b + 1, c * 12 + b + 2, c * 12 + b + 3] for c in range(20) for b in range(9)])
-    dev.set_bonded(1, bends + 1, 20.0)
-    dev.set_bonded(3, dihs + 1, 1.0)
+    # every bonded kind: bend_, cosine_bend_, angle_bend_ on the triplets;
+    # dih_, cosine_dih_, angle_dih_ on the quadruplets
+    terms = [(1, bends, 20.0, 0.0), (2, bends, 5.0, 110.0), (5, bends, 5.0, 110.0),
+             (3, dihs, 1.0, 0.0), (4, dihs, 1.5, 60.0), (6, dihs, 2.0, -120.0)]
+    for kind, idx, coeff, angle in terms:
+        dev.set_bonded(kind, idx + 1, coeff, angle)
     dev.initialize_forces()
     dev.bonded_forces()
     got = from_device(dev, st)
     F = np.zeros((n, 3)); pe = 0.0
-    for t in bends:
-        e, f = oracle.bonded(1, st, list(t), 20.0)
-        pe += e
-        for p in range(3):
-            F[t[p]] += f[p]
-    for t in dihs:
-        e, f = oracle.bonded(3, st, list(t), 1.0)
-        pe += e
-        for p in range(4):
-            F[t[p]] += f[p]
+    for kind, idx, coeff, angle in terms:
+        for t in idx:
+            e, f = oracle.bonded(kind, st, list(t), coeff, angle)
+            pe += e
+            for p in range(len(t)):
+                F[t[p]] += f[p]
     for k, fn in enumerate(("fx", "fy", "fz")):
         assert rel_err(got[fn], F[:, k]) < TOL
     assert abs(got["pot_eng"] - pe) <= TOL * abs(pe)

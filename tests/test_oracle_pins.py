@@ -427,6 +427,66 @@ def test_langevin_equipartition(oracle):
     assert abs(mv2 - 1.3) < 0.05 and abs(lrot - 1.3) < 0.1
 
 
+def _bond_angle(p0, p1, p2):
+    a, b = p0 - p1, p2 - p1
+    return np.arctan2(np.linalg.norm(np.cross(a, b)), a @ b)
+
+
+def _dihedral(p0, p1, p2, p3):
+    """IUPAC torsion angle (trans = pi) by the atan2 route."""
+    b1, b2, b3 = p1 - p0, p2 - p1, p3 - p2
+    n1, n2 = np.cross(b1, b2), np.cross(b2, b3)
+    return np.arctan2(np.linalg.norm(b2) * (b1 @ n2), n1 @ n2)
+
+
+# kind, atoms, coeff, angle (degrees), closed form of the energy in the bond
+# angle alpha at the middle atom or the torsion angle phi
+BONDED_FORMS = [
+    (1, 3, 20.0, 0.0, lambda x, k, a: k * (1.0 + np.cos(x))),
+    (2, 3, 5.0, 110.0, lambda x, k, a: 0.5 * k * (np.cos(np.pi - x) - np.cos(np.pi - np.radians(a))) ** 2),
+    (5, 3, 5.0, 110.0, lambda x, k, a: 0.5 * k * (x - np.radians(a)) ** 2),
+    (5, 3, 7.0, 180.0, lambda x, k, a: 0.5 * k * (x - np.radians(a)) ** 2),
+    (3, 4, 1.0, 0.0, lambda x, k, a: k * (1.0 + np.cos(x))),
+    (4, 4, 1.5, 60.0, lambda x, k, a: 0.5 * k * (np.cos(np.pi - x) - np.cos(np.pi - np.radians(a))) ** 2),
+    (6, 4, 1.5, 60.0, lambda x, k, a: k * (1.0 - np.cos(x - np.radians(a)))),
+    (6, 4, 2.0, -120.0, lambda x, k, a: k * (1.0 - np.cos(x - np.radians(a)))),
+]
+
+
+@pytest.mark.parametrize("kind,nat,coeff,angle,form", BONDED_FORMS)
+def test_bonded_closed_forms_and_gradients(oracle, kind, nat, coeff, angle, form):
+    """Bonded terms (force.f90:2244-2579): the energy equals its closed form
+    in the bond angle (bends) or the IUPAC torsion (dihedrals) computed here
+    by an independent atan2 route, and each atom's force is minus the
+    central-difference gradient of that energy, except for cosine_bend_
+    and cosine_dih_, whose forces are PLUS the gradient: they reuse the
+    k(1 -/+ cos) force expressions with strength +k (cos - cos0)
+    (force.f90:2383-2388, :2449-2460), so they push the angle away from the
+    energy minimum.  The device reproduces the reference, sign included.
+    Periodic images are used: the atoms straddle the box edge."""
+    sign = 1.0 if kind in (2, 4) else -1.0
+    rng = np.random.default_rng(50 + kind)
+    L = 6.0
+    st = base_state(rng, nat, L)
+    h = 1e-6
+    for _ in range(25):
+        pos = np.cumsum(rng.normal(size=(nat, 3)) * 0.7, axis=0) + L / 2 - 0.3
+        wrapped = pos - np.round(pos / L) * L
+        st["rx"], st["ry"], st["rz"] = wrapped[:, 0], wrapped[:, 1], wrapped[:, 2]
+        idx = list(range(nat))
+        pe, f = oracle.bonded(kind, st, idx, coeff, angle)
+        x = _bond_angle(*pos) if nat == 3 else _dihedral(*pos)
+        assert abs(pe - form(x, coeff, angle)) <= 1e-10 * max(1.0, abs(pe)), (pe, form(x, coeff, angle))
+        scale = max(np.abs(f[:nat]).max(), 1e-3)
+        for a in range(nat):
+            for k, fld in enumerate(["rx", "ry", "rz"]):
+                up = st.copy(); up[fld][a] += h
+                dn = st.copy(); dn[fld][a] -= h
+                fd = sign * (oracle.bonded(kind, up, idx, coeff, angle)[0]
+                             - oracle.bonded(kind, dn, idx, coeff, angle)[0]) / (2 * h)
+                assert abs(fd - f[a][k]) <= 2e-5 * scale, (kind, a, k, fd, f[a][k])
+
+
 def test_rattle_satisfies_constraints(oracle):
     """apply_constraints_a/b (system.f90:1060-1211): bond lengths to 2e-5 L^2
     after the position stage, r.v near zero after the velocity stage."""

@@ -132,7 +132,8 @@ def test_ecp_brent_bit_exact(oracle, reference):
         assert lam == reference.brent(mats[0], mats[1], r)
 
 
-BONDED = [(1, 3, 20.0, 0.0), (2, 3, 20.0, np.pi), (2, 3, 5.0, 2.0), (3, 4, 1.0, 0.0), (4, 4, 1.0, 0.7)]
+BONDED = [(1, 3, 20.0, 0.0), (2, 3, 20.0, np.pi), (2, 3, 5.0, 2.0), (3, 4, 1.0, 0.0), (4, 4, 1.0, 0.7),
+          (5, 3, 5.0, 110.0), (5, 3, 20.0, 0.0), (6, 4, 1.0, 60.0), (6, 4, 2.0, 0.0)]
 
 
 @pytest.mark.parametrize("kind,nat,coeff,angle", BONDED)
