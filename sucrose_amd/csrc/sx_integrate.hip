@@ -3,6 +3,7 @@
 // reference's FP64 expressions; loop-invariant transcendental constants
 // (exp(-gamma dt), ...) are evaluated once on the host exactly as the
 // reference evaluates them and passed in.
+#include "sx_ecp.h"
 #include "sx_steps.h"
 
 #include <algorithm>
@@ -126,6 +127,69 @@ __global__ void k_lgv_initial(DState s, LgvConst k, int step, const int *dstep)
     if (i >= s.n) return;
     int st = dstep ? *dstep : step;
     lgv_initial_particle(s, i, i + 1, k, st);
+}
+
+// The start of a whole Langevin step in one launch: lgv_initial_step
+// (integrators.f90:507-627), the rebuild decision of needs_rebuild
+// (system.f90:1005-1039: maximum displacement since the last build, taken by
+// the last workgroup to finish) and initialize_forces (system.f90:423-436:
+// each particle's F, T and energy slot after its half kick has read F); with
+// PP also the ECP per-particle records of the new orientations.
+template <bool PP>
+__global__ void __launch_bounds__(256) k_lgv_step_begin(DState s, LgvConst k, const int *dstep, const double *xl,
+                                                        const double *yl, const double *zl, double *part,
+                                                        unsigned *ticket, double thresh, int rule, int *flag,
+                                                        double *scal, double *pe_acc, double *pp, float *scr,
+                                                        double *pos4)
+{
+    __shared__ double sh[256];
+    __shared__ bool last;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if (i == 0) {
+        scal[SX_POT_ENG] = 0.0; scal[SX_KIN_ENG] = 0.0; scal[SX_KIN_ROT] = 0.0; scal[SX_TEMP] = 0.0;
+        scal[SX_PRESSURE] = 0.0; scal[SX_VIRIAL] = 0.0; scal[SX_VIRIAL_BONDED] = 0.0;
+    }
+    if (i < s.n) {
+        lgv_initial_particle(s, i, i + 1, k, *dstep);
+        double dx = s.f[SX_RX][i] - xl[i], dy = s.f[SX_RY][i] - yl[i], dz = s.f[SX_RZ][i] - zl[i];
+        if (s.periodic) {
+            dx = sx_mimg(dx, s.box[0]); dy = sx_mimg(dy, s.box[1]); dz = sx_mimg(dz, s.box[2]);
+        }
+        m = dx * dx + dy * dy + dz * dz;
+        s.f[SX_FX][i] = 0.0; s.f[SX_FY][i] = 0.0; s.f[SX_FZ][i] = 0.0;
+        s.f[SX_TX][i] = 0.0; s.f[SX_TY][i] = 0.0; s.f[SX_TZ][i] = 0.0;
+        pe_acc[i] = 0.0;
+        if (PP) ecp_pp_particle(s, i, pp, scr, pos4);
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = sh[0];
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double v = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += 256)
+        v = fmax(v, __hip_atomic_load(&part[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double mx = sh[0];
+        flag[0] = rule == SX_REBUILD_REFERENCE ? (4.0 * mx > thresh) : (mx > thresh);
+        *ticket = 0u;
+    }
 }
 
 __global__ void __launch_bounds__(256) k_lgv_final(DState s, double dt, double *part, unsigned *ticket, double *scal,
@@ -1009,6 +1073,26 @@ void sx_lgv_initial_impl(sx_ctx *c, double dt, int step, const int *dstep, doubl
     LgvConst k = lgv_const(c, dt, damping, constrained);
     hipLaunchKernelGGL(k_lgv_initial, dim3(sx_blocks(c->n, 256)), dim3(256), 0, c->stream, c->dstate(), k, step, dstep);
     if (constrained) sx_rattle_impl(c, dt, 0);
+}
+
+void sx_lgv_step_begin_impl(sx_ctx *c, double dt, double damping, bool ecp_pp)
+{
+    LgvConst k = lgv_const(c, dt, damping, false);
+    NList &nl = c->nl;
+    int nb = sx_blocks(c->n, 256);
+    sx_ensure_part(c, nb, 1);
+    double thresh = nl.rule == SX_REBUILD_REFERENCE ? nl.r_skin_sq : (nl.skin / 2.0) * (nl.skin / 2.0);
+    double *pp = nullptr, *pos4 = nullptr;
+    float *scr = nullptr;
+    if (ecp_pp) sx_ecp_records(c, &pp, &scr, &pos4);
+    if (ecp_pp)
+        hipLaunchKernelGGL(k_lgv_step_begin<true>, dim3(nb), dim3(256), 0, c->stream, c->dstate(), k, c->d_step,
+                           nl.x_last, nl.y_last, nl.z_last, c->d_part, c->d_ticket, thresh, nl.rule, c->d_flag,
+                           c->d_scal, c->d_pe_acc, pp, scr, pos4);
+    else
+        hipLaunchKernelGGL(k_lgv_step_begin<false>, dim3(nb), dim3(256), 0, c->stream, c->dstate(), k, c->d_step,
+                           nl.x_last, nl.y_last, nl.z_last, c->d_part, c->d_ticket, thresh, nl.rule, c->d_flag,
+                           c->d_scal, c->d_pe_acc, pp, scr, pos4);
 }
 
 void sx_lgv_final_impl(sx_ctx *c, double dt, double damping, bool constrained, bool thermo, int *dstep)

@@ -103,6 +103,7 @@ struct sx_ctx {
     long ld = 0;
     double *d_pe_acc = nullptr;  // per-particle potential energy since initialize_forces
     bool defer_pe = false;       // step driver: pot_eng is reduced in the final-step kernel
+    bool ecp_pp_by_step = false; // step driver: the initial-step kernel wrote the ECP per-particle records
     int *d_segcount = nullptr;   // ECP work-list segment counts
     double *d_scal = nullptr;    // SX_NSCALARS doubles (device copy of the scalars)
     double h_scal[SX_NSCALARS] = {0};
@@ -199,6 +200,11 @@ void sx_nlist_want_half(sx_ctx *c);
 void sx_forces_impl(sx_ctx *c, int kind, const double *p, int np);
 void sx_bonded_impl(sx_ctx *c);
 void sx_lgv_initial_impl(sx_ctx *c, double dt, int step, const int *dstep, double damping, bool constrained);
+// lgv_initial_step, the rebuild decision and initialize_forces in one launch
+// (and the ECP per-particle records of the new orientations when ecp_pp)
+void sx_lgv_step_begin_impl(sx_ctx *c, double dt, double damping, bool ecp_pp);
+// the ECP per-particle record buffers (allocated on first use)
+void sx_ecp_records(sx_ctx *c, double **pp, float **scr, double **pos4);
 // thermo = also run calculate_thermo in the same kernel; dstep = device step
 // counter to advance (graph replays)
 void sx_lgv_final_impl(sx_ctx *c, double dt, double damping, bool constrained, bool thermo = false,

@@ -67,21 +67,21 @@ __device__ __forceinline__ int cell_coord(double x, double L, int nc, double w)
     return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
 }
 
-__global__ void k_zero_int(int *p, int n, const int *flag)
+// bins the particles and records the positions the list is built from
+// (x_last of build_nlist, system.f90:1001); count[7] counts the builds
+// done inside graph replays
+__global__ void k_cell_assign(DState s, CellGrid g, int *cell_of, int *cell_count, double *xl, double *yl,
+                              double *zl, const int *flag, long *count)
 {
     if (flag && *flag == 0) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
-
-__global__ void k_cell_assign(DState s, CellGrid g, int *cell_of, int *cell_count, const int *flag)
-{
-    if (flag && *flag == 0) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && count) count[7] += 1;
     if (i >= s.n) return;
-    int cx = cell_coord(s.f[SX_RX][i], s.box[0], g.nc[0], g.w[0]);
-    int cy = cell_coord(s.f[SX_RY][i], s.box[1], g.nc[1], g.w[1]);
-    int cz = cell_coord(s.f[SX_RZ][i], s.box[2], g.nc[2], g.w[2]);
+    double x = s.f[SX_RX][i], y = s.f[SX_RY][i], z = s.f[SX_RZ][i];
+    xl[i] = x; yl[i] = y; zl[i] = z;
+    int cx = cell_coord(x, s.box[0], g.nc[0], g.w[0]);
+    int cy = cell_coord(y, s.box[1], g.nc[1], g.w[1]);
+    int cz = cell_coord(z, s.box[2], g.nc[2], g.w[2]);
     int c = (cx * g.nc[1] + cy) * g.nc[2] + cz;
     cell_of[i] = c;
     atomicAdd(&cell_count[c], 1);
@@ -133,14 +133,16 @@ __global__ void __launch_bounds__(1024) k_scan(const int *in, int *out, int n, c
     }
 }
 
-__global__ void k_cell_scatter(DState s, const int *cell_of, const int *cell_start, int *cell_fill, int *items,
+// places each particle in its cell's slot range by counting the cell's
+// count back down, which leaves every count at zero for the next build
+__global__ void k_cell_scatter(DState s, const int *cell_of, const int *cell_start, int *cell_count, int *items,
                                double4 *cell_pos, const int *flag)
 {
     if (flag && *flag == 0) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.n) return;
     int c = cell_of[i];
-    int p = cell_start[c] + atomicAdd(&cell_fill[c], 1);
+    int p = cell_start[c + 1] - atomicSub(&cell_count[c], 1);
     items[p] = i;
     cell_pos[p] = make_double4(s.f[SX_RX][i], s.f[SX_RY][i], s.f[SX_RZ][i], 0.0);
 }
@@ -575,6 +577,7 @@ static void nl_setup(sx_ctx *c)
     }
     HIPCHK(hipMalloc(&nl.cell_of, sizeof(int) * n));
     HIPCHK(hipMalloc(&nl.cell_count, sizeof(int) * (nl.ncells + 1)));
+    HIPCHK(hipMemsetAsync(nl.cell_count, 0, sizeof(int) * (nl.ncells + 1), c->stream));
     HIPCHK(hipMalloc(&nl.cell_start, sizeof(int) * (nl.ncells + 1)));
     HIPCHK(hipMalloc(&nl.cell_items, sizeof(int) * n));
     HIPCHK(hipMalloc(&nl.cell_pos, sizeof(double) * 4 * (size_t)n));
@@ -651,14 +654,12 @@ void sx_nlist_build_impl(sx_ctx *c, bool conditional)
                 g.inv[d] = 1.0 / c->box[d];
                 if (!(nl.cutoff + nl.skin < 0.5 * c->box[d] * (1.0 - 1e-12))) g.fast = 0;
             }
-            // cell binning is cheap; it runs unconditionally
-            int ncb = sx_blocks(nl.ncells + 1, 256);
-            hipLaunchKernelGGL(k_zero_int, dim3(ncb), dim3(256), 0, c->stream, nl.cell_count, nl.ncells + 1, flag);
+            // the cell counts are zero between builds (k_cell_scatter counts
+            // them back down)
             hipLaunchKernelGGL(k_cell_assign, dim3(nb), dim3(256), 0, c->stream, s, g, nl.cell_of, nl.cell_count,
-                               flag);
+                               nl.x_last, nl.y_last, nl.z_last, flag, conditional ? c->d_counts : (long *)nullptr);
             hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.cell_count, nl.cell_start, nl.ncells,
                                flag, (int *)nullptr);
-            hipLaunchKernelGGL(k_zero_int, dim3(ncb), dim3(256), 0, c->stream, nl.cell_count, nl.ncells + 1, flag);
             hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(256), 0, c->stream, s, nl.cell_of, nl.cell_start,
                                nl.cell_count, nl.cell_items, (double4 *)nl.cell_pos, flag);
             launch_rows_g(c, 0, g, flag);
@@ -688,8 +689,9 @@ void sx_nlist_build_impl(sx_ctx *c, bool conditional)
     hipLaunchKernelGGL(k_sort_rows, dim3(std::min(sx_blocks(n, 4), 2048)), dim3(256), 0, c->stream, n, nl.row_start,
                        nl.nbr, flag, c->d_err);
     if (nl.need_half) half_index_launch(c, flag);
-    hipLaunchKernelGGL(k_save_last, dim3(nb), dim3(256), 0, c->stream, s, nl.x_last, nl.y_last, nl.z_last, flag,
-                       conditional ? c->d_counts : (long *)nullptr);
+    if (!nl.use_cells)
+        hipLaunchKernelGGL(k_save_last, dim3(nb), dim3(256), 0, c->stream, s, nl.x_last, nl.y_last, nl.z_last, flag,
+                           conditional ? c->d_counts : (long *)nullptr);
     if (!conditional) {
         int sizes[2];
         HIPCHK(hipMemcpyAsync(sizes, c->d_flag + 1, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));

@@ -72,58 +72,14 @@ __global__ void __launch_bounds__(256) k_gather(DState s, const int *row_start, 
 // ---------------------------------------------------------------------------
 // ECP
 // ---------------------------------------------------------------------------
-// per-particle records (force.f90:1427-1483 for one particle): FP64 record,
-// FP32 screen record (AR lower triangle, largest semi-axis, smallest) and the
-// packed position; thread 0 also clears the work counter of this call
+// per-particle records of every particle (ecp_pp_particle); thread 0 also
+// clears the work counter of this call
 __global__ void k_ecp_pp(DState s, double *pp, float *scr, double *pos4, int *nwork)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0 && nwork) *nwork = 0;
     if (i >= s.n) return;
-    double w = s.f[SX_QW][i], x = s.f[SX_QX][i], y = s.f[SX_QY][i], z = s.f[SX_QZ][i];
-    double U[9];
-#define Ux(i_, j_) U[((i_) - 1) + 3 * ((j_) - 1)]
-    Ux(1, 1) = w * w + x * x - y * y - z * z;
-    Ux(1, 2) = 2.0 * (x * y + w * z);
-    Ux(1, 3) = 2.0 * (x * z - w * y);
-    Ux(2, 1) = 2.0 * (x * y - w * z);
-    Ux(2, 2) = w * w - x * x + y * y - z * z;
-    Ux(2, 3) = 2.0 * (y * z + w * x);
-    Ux(3, 1) = 2.0 * (x * z + w * y);
-    Ux(3, 2) = 2.0 * (y * z - w * x);
-    Ux(3, 3) = w * w - x * x - y * y + z * z;
-    double sx = s.f[SX_SIG_X][i], sy = s.f[SX_SIG_Y][i], sz = s.f[SX_SIG_Z][i];
-    double ex = s.f[SX_EPS_X][i], ey = s.f[SX_EPS_Y][i], ez = s.f[SX_EPS_Z][i];
-    double e0 = fmax(fmax(ex, ey), ez);
-    double S[3] = {sx / 2.0, sy / 2.0, sz / 2.0};
-    double B[3] = {S[0] * S[0], S[1] * S[1], S[2] * S[2]};
-    // E = ((e0/eps)**(1/2))/4  (force.f90:1452-1454)
-    double E[3] = {sqrt(e0 / ex) / 4.0, sqrt(e0 / ey) / 4.0, sqrt(e0 / ez) / 4.0};
-    double *rec = pp + (long)i * PP_STRIDE;
-    // matmul_atba (xmath.f90:262-279): the reference computes out(r, c) for
-    // r <= c; the record keeps (c, r) = (r, c) in lower-triangle order
-    const int ij[6][2] = {{1, 1}, {2, 1}, {3, 1}, {2, 2}, {3, 2}, {3, 3}};
-    double ar_l[6];
-    for (int t = 0; t < 6; ++t) {
-        int r_ = ij[t][1], c_ = ij[t][0];
-        double ar = Ux(1, r_) * B[0] * Ux(1, c_) + Ux(2, r_) * B[1] * Ux(2, c_) + Ux(3, r_) * B[2] * Ux(3, c_);
-        double ae = Ux(1, r_) * E[0] * Ux(1, c_) + Ux(2, r_) * E[1] * Ux(2, c_) + Ux(3, r_) * E[2] * Ux(3, c_);
-        rec[PP_AR + t] = ar;
-        rec[PP_AE + t] = ae;
-        ar_l[t] = ar;
-    }
-    // MM = U^T . S: column c is row c of U scaled by S(c)
-    for (int c = 0; c < 3; ++c)
-        for (int k = 0; k < 3; ++k) rec[PP_MM + 3 * c + k] = Ux(c + 1, k + 1) * S[c];
-    rec[PP_ST] = (sx * sy + sz * sz) * sqrt(2.0 * sx * sy) / 8.0;
-    double amax = fmax(fmax(S[0], S[1]), S[2]), amin = fmin(fmin(S[0], S[1]), S[2]);
-    rec[PP_AMAX] = amax;
-#undef Ux
-    float4 *s4 = reinterpret_cast<float4 *>(scr) + 2L * i;
-    s4[0] = make_float4((float)ar_l[0], (float)ar_l[1], (float)ar_l[2], (float)ar_l[3]);
-    s4[1] = make_float4((float)ar_l[4], (float)ar_l[5], (float)amax, (float)amin);
-    double4 *p4 = reinterpret_cast<double4 *>(pos4) + i;
-    *p4 = make_double4(s.f[SX_RX][i], s.f[SX_RY][i], s.f[SX_RZ][i], 0.0);
+    ecp_pp_particle(s, i, pp, scr, pos4);
 }
 
 struct EcpScreen {
@@ -647,7 +603,16 @@ static void ecp_forces_impl(sx_ctx *c, int kind, const Params &P)
     ensure_pair_buffers(c);
     EcpScreen sc = make_screen(c, kind, P);
     EcpLaunch L = ecp_launch_shape(c, false);
-    for (int stage = 0; stage < 4; ++stage) ecp_stage(c, stage, kind, sc, L);
+    // the whole-step driver writes the records in its initial-step kernel
+    for (int stage = c->ecp_pp_by_step ? 1 : 0; stage < 4; ++stage) ecp_stage(c, stage, kind, sc, L);
+}
+
+void sx_ecp_records(sx_ctx *c, double **pp, float **scr, double **pos4)
+{
+    ensure_pp(c);
+    *pp = c->d_ecp_pp;
+    *scr = c->d_ecp_scr;
+    *pos4 = c->d_pos4;
 }
 
 void sx_forces_impl(sx_ctx *c, int kind, const double *p, int np)

@@ -12,24 +12,38 @@ __global__ void k_step_set(int *dstep, int v) { *dstep = v; }
 struct DeferPe {
     sx_ctx *c;
     explicit DeferPe(sx_ctx *c_) : c(c_) { c->defer_pe = true; }
-    ~DeferPe() { c->defer_pe = false; }
+    ~DeferPe() { c->defer_pe = false; c->ecp_pp_by_step = false; }
 };
+
+static bool is_ecp(int kind)
+{
+    return kind == SX_ECP || kind == SX_ECPR || kind == SX_ECPW || kind == SX_ECPC || kind == SX_ECPWC;
+}
 
 static void enqueue_step(sx_ctx *c, const sx_step_spec &sp)
 {
     DeferPe defer(c);   // pot_eng is reduced by the final-step kernel
-    switch (sp.integrator) {
-    case SX_INT_VV: sx_vv_initial_impl(c, sp.dt, false); break;
-    case SX_INT_NHT:
-        sx_nht_impl(c, sp.dt, nullptr, 0, false);
-        sx_qq_initial_impl(c, sp.dt);
-        break;
-    case SX_INT_LGV: sx_lgv_initial_impl(c, sp.dt, 0, c->d_step, sp.damping, false); break;
-    case SX_INT_LGVC: sx_lgv_initial_impl(c, sp.dt, 0, c->d_step, sp.damping, true); break;
+    if (sp.integrator == SX_INT_LGV) {
+        // initial step, rebuild decision, initialize_forces (and the ECP
+        // per-particle records) in one kernel, then the conditional rebuild
+        bool ecp = false;
+        for (int f = 0; f < sp.nforces; ++f) ecp = ecp || is_ecp(sp.kinds[f]);
+        sx_lgv_step_begin_impl(c, sp.dt, sp.damping, ecp);
+        c->ecp_pp_by_step = ecp;
+        sx_nlist_build_impl(c, true);
+    } else {
+        switch (sp.integrator) {
+        case SX_INT_VV: sx_vv_initial_impl(c, sp.dt, false); break;
+        case SX_INT_NHT:
+            sx_nht_impl(c, sp.dt, nullptr, 0, false);
+            sx_qq_initial_impl(c, sp.dt);
+            break;
+        case SX_INT_LGVC: sx_lgv_initial_impl(c, sp.dt, 0, c->d_step, sp.damping, true); break;
+        }
+        sx_nlist_flag_impl(c);
+        sx_nlist_build_impl(c, true);
+        sx_init_forces_impl(c);
     }
-    sx_nlist_flag_impl(c);
-    sx_nlist_build_impl(c, true);
-    sx_init_forces_impl(c);
     for (int f = 0; f < sp.nforces; ++f) {
         int np = 0;
         switch (sp.kinds[f]) {
@@ -73,9 +87,7 @@ extern "C" int sx_run(sx_ctx *c, const sx_step_spec *spec, int nsteps)
         if (spec->integrator != SX_INT_VV && !c->have_target_temp)
             throw sx_error{SX_ERR_NOTEMP, "Error: Target temperature not set."};
         for (int f = 0; f < spec->nforces; ++f)
-            if (spec->kinds[f] == SX_ECP || spec->kinds[f] == SX_ECPR || spec->kinds[f] == SX_ECPW ||
-                spec->kinds[f] == SX_ECPC || spec->kinds[f] == SX_ECPWC)
-                sx_nlist_want_half(c);
+            if (is_ecp(spec->kinds[f])) sx_nlist_want_half(c);
         if (!c->d_step) HIPCHK(hipMalloc(&c->d_step, sizeof(int)));
         hipLaunchKernelGGL(k_step_set, dim3(1), dim3(1), 0, c->stream, c->d_step, spec->first_step);
         if (nsteps == 0) return;

@@ -425,3 +425,64 @@ def test_run_graph_lj_nve(dev, oracle):
         assert rel_err(got[f], ref[f]) < 1e-9, f
     assert abs(got["pot_eng"] - ref["pot_eng"]) < 1e-9 * abs(ref["pot_eng"])
     assert abs(got["kin_eng"] - ref["kin_eng"]) < 1e-9 * abs(ref["kin_eng"])
+
+
+@pytest.mark.parametrize("kind,params,rc", [(ECP, (1.2,), 3.3), (ECPR, (), 3.1), (LJ, (1.0, 1.0, 2.5), 2.5)])
+def test_run_graph_langevin(dev, oracle, kind, params, rc):
+    """sx_run (graph replay) with the Langevin integrator, whose first kernel
+    also takes the rebuild decision, clears the forces and, for ECP kinds,
+    writes the per-particle records, vs the oracle's README loop: 12 steps
+    under the skin/2 rebuild rule with a small skin (0.02), so that the list
+    is rebuilt on the way."""
+    import sucrose_amd as sx
+    say(f"run graph Langevin kind={kind}")
+    rng = np.random.default_rng(60 + kind)
+    if kind == LJ:
+        n, L = 864, 10.0776
+        st = base_state(rng, n, L, forces=False, T=1.0)
+        g = np.arange(10) * (L / 10) - L / 2
+        X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+        P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)[:n] + rng.normal(size=(n, 3)) * 0.05
+    else:
+        n, L = 1500, 20.0
+        st = base_state(rng, n, L, forces=False, sig=(1, 1, 3), T=1.0)
+        P, q = sx.gen_ellipsoids_rsa(n, (0.5, 0.5, 1.5), L, 77 + kind)
+        assert len(P) == n
+        for k, f in enumerate(("qw", "qx", "qy", "qz")):
+            st[f] = q[:, k]
+    st["rx"], st["ry"], st["rz"] = P[:, 0], P[:, 1], P[:, 2]
+    to_device(dev, st)
+    skin = 0.02
+    dev.init_nlist(skin, rc)
+    dev.set_rebuild_rule(sx.REBUILD_HALF_SKIN)
+    dev.initialize_forces()
+    dev.forces(kind, *params)
+    ref = from_device(dev, st)
+    nl = oracle.nlist_new(ref, rc, skin)
+    oracle.nlist_build(ref, nl)
+    p = list(params) if params else [0.0]
+    nsteps, dt, first = 12, 0.002, 5
+    dev.run(sx.INT_LGV, [(kind,) + tuple(params)], nsteps, dt, damping=1.0, first_step=first)
+    rebuilds = 0
+    for t in range(nsteps):
+        oracle.integrate(LGV, 0, ref, dt, step=first + t, damping=1.0)
+        d = np.stack([ref[f] - nl[x] for f, x in (("rx", "x_last"), ("ry", "y_last"), ("rz", "z_last"))], 1)
+        d -= np.round(d / L) * L
+        if (d * d).sum(1).max() > (skin / 2) ** 2:
+            oracle.nlist_build(ref, nl)
+            rebuilds += 1
+        oracle.call("or_initialize_forces", ref)
+        oracle.pair_loop(ref, nl, kind, p)
+        oracle.integrate(LGV, 1, ref, dt, damping=1.0)
+        oracle.call("or_calculate_thermo", ref)
+    got = from_device(dev, st)
+    dev.set_rebuild_rule(sx.REBUILD_REFERENCE)
+    say(f"  {rebuilds} rebuilds")
+    assert rebuilds >= 1
+    assert np.any(ref["fx"] != 0)
+    for f in ("rx", "ry", "rz", "vx", "qw", "lx", "fx", "tz"):
+        if kind == LJ and f == "tz":
+            continue
+        assert rel_err(got[f], ref[f]) < 1e-9, (f, rel_err(got[f], ref[f]))
+    for s in ("pot_eng", "kin_eng", "kin_rot", "temp"):
+        assert abs(got[s] - ref[s]) <= 1e-9 * max(1.0, abs(ref[s])), (s, got[s], ref[s])
