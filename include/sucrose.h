@@ -115,6 +115,11 @@ int sx_set_num_atoms(sx_ctx *ctx, int num_atoms, int rigid_atoms);
 /* set_box (system.f90:185) + set_periodic_boundary (:209) */
 int sx_set_box(sx_ctx *ctx, const double box[3], int periodic);
 int sx_get_box(sx_ctx *ctx, double box[3]);
+/* set_atom_type (system.f90:109) for all n atoms at once; rtype is host
+ * metadata (-1 until set; initialize_system turns negatives into 0), read by
+ * the trajectory and checkpoint reporters */
+int sx_set_atom_types(sx_ctx *ctx, const int *rtype);
+int sx_get_atom_types(sx_ctx *ctx, int *rtype);
 int sx_upload(sx_ctx *ctx, int field, const double *host, int n);
 int sx_download(sx_ctx *ctx, int field, double *host, int n);
 /* device address of a field's n FP64 values (valid until the next sx_set_num_atoms) */
@@ -210,6 +215,52 @@ typedef struct sx_step_spec {
 /* Runs nsteps of: initial step, nlist update, initialize_forces, pair loops,
  * bonded, final step, calculate_thermo.  The first call captures the graph. */
 int sx_run(sx_ctx *ctx, const sx_step_spec *spec, int nsteps);
+
+/* ------------------------------------------------------------------ */
+/* reporters (reporters.f90): text files in the reference's formats     */
+/* ------------------------------------------------------------------ */
+enum sx_report_kind {
+    SX_REPORT_STATE_DATA = 1,  /* module state_data_reporter   reporters.f90:1-46    */
+    SX_REPORT_TRAJECTORY = 2,  /* module trajectory_reporter   reporters.f90:48-101  */
+    SX_REPORT_CONFIG     = 3,  /* module config_reporter       reporters.f90:103-139 */
+    SX_REPORT_CHECKPOINT = 4   /* module checkpoint_reporter   reporters.f90:141-231 */
+};
+typedef struct sx_reporter sx_reporter;
+/* open_state_data_file / open_trajectory_file / open_confwriter_file /
+ * open_checkpoint_file (reporters.f90:7, :54, :109, :148): creates (truncates)
+ * the file; the state-data reporter writes its header line.  ctx may be NULL
+ * for a reporter fed only by sx_report_host. */
+int sx_reporter_open(sx_ctx *ctx, int kind, const char *filename, sx_reporter **out);
+/* report_state_data / report_trajectory / report_confwriter /
+ * report_checkpoint (reporters.f90:20, :65, :120, :160) of the context's state
+ * as it stands after the work queued before the call: the frame is copied on
+ * the device, in stream order, and formatted and written by the reporter's
+ * host thread; the call does not wait for the device.  step is printed by the
+ * state-data and trajectory frames.  A checkpoint also writes <file>.state
+ * (target temperature, dof, Nose-Hoover HB/eta/p_eta, Langevin step counter,
+ * periodic flag), which the reference's checkpoint does not hold. */
+int sx_report(sx_reporter *r, int step);
+/* the same frame from host arrays (fields: SX_NFIELDS pointers indexed by
+ * sx_field, only the frame's fields read; scal: SX_NSCALARS values; bonds
+ * 1-based), written before the call returns */
+int sx_report_host(sx_reporter *r, int step, int n, const double *const *fields, const int *rtype,
+                   const double *scal, const double box[3], int nc, const int *bbi1, const int *bbj1,
+                   const double *bblen);
+/* config_index written at the end of each checkpoint (0 at open, reporters.f90:156) */
+int sx_reporter_set_config_index(sx_reporter *r, int config_index);
+int sx_reporter_flush(sx_reporter *r);        /* waits until every frame is written */
+const char *sx_reporter_error(const sx_reporter *r);
+int sx_reporter_close(sx_reporter *r);        /* close_*_file: flush, close, free   */
+/* load_checkpoint (reporters.f90:188-221): set_num_atoms, set_num_constraints,
+ * set_box, then the atom and bond lines (the mass of a line goes to the atom
+ * the line names, the other columns to the line's position; inv_mass and the
+ * inverse inertia stay -1 as set_num_atoms leaves them) and config_index
+ * (may be NULL); restores <file>.state when present */
+int sx_load_checkpoint(sx_ctx *ctx, const char *filename, int *config_index);
+/* the parse of sx_load_checkpoint into host arrays (no device): sizes first;
+ * SX_ERR_CAPACITY when n > cap_n or nc > cap_nc; fields[k] may be NULL */
+int sx_read_checkpoint_host(const char *filename, int *n, int *nc, double box[3], int *config_index, int cap_n,
+                            int cap_nc, double *const *fields, int *rtype, int *bbi1, int *bbj1, double *bblen);
 
 /* ------------------------------------------------------------------ */
 /* batched replicas: independent small systems advanced together, three     */

@@ -472,6 +472,32 @@ class Reference:
                           _dp(f), _dp(ti), _dp(tj))
         return pe.value, f, ti, tj
 
+    def report(self, kind, path, step, rtype):
+        """One frame of reporters.f90 written by the reference to a new file
+        (kind 1 state data, 2 trajectory, 3 config, 4 checkpoint)."""
+        p = os.fsencode(path)
+        rt = np.ascontiguousarray(rtype, dtype=np.int32)
+        self.lib.ref_report(C.c_int(kind), p, C.c_int(len(p)), C.c_int(step), rt.ctypes.data_as(_PI))
+
+    def load_checkpoint(self, path):
+        """load_checkpoint (reporters.f90:188-221) into the module arrays;
+        returns n, nc, config_index, the field buffer, rtype, bonds, box."""
+        p = os.fsencode(path)
+        n, nc, cfg = C.c_int(), C.c_int(), C.c_int()
+        self.lib.ref_load_checkpoint(p, C.c_int(len(p)), C.byref(n), C.byref(nc), C.byref(cfg))
+        self.n = n.value
+        buf = np.zeros((n.value, len(FIELDS)), order="F")
+        self.lib.ref_get_state(C.c_int(n.value), _dp(buf))
+        rt = np.zeros(n.value, np.int32)
+        self.lib.ref_get_types(rt.ctypes.data_as(_PI))
+        m = max(nc.value, 1)
+        bi = np.zeros(m, np.int32); bj = np.zeros(m, np.int32); bl = np.zeros(m)
+        self.lib.ref_get_bonds(bi.ctypes.data_as(_PI), bj.ctypes.data_as(_PI), _dp(bl))
+        box = np.zeros(3)
+        self.lib.ref_get_box(_dp(box))
+        return n.value, nc.value, cfg.value, np.ascontiguousarray(buf), rt, (bi[:nc.value], bj[:nc.value],
+                                                                             bl[:nc.value]), box
+
     def bonded(self, kind, idx, coeff, angle=0.0):
         idx = list(idx) + [0] * (4 - len(idx))
         pe = C.c_double()

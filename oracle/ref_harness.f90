@@ -22,6 +22,8 @@ module ref_harness
     use quaternion
     use nose_hoover
     use langevin
+    use monte_carlo_barostat
+    use reporters
     implicit none
 
     integer, parameter :: NFIELD = 37
@@ -401,5 +403,92 @@ contains
             call calculate_thermo
         enddo
     end subroutine ref_run_steps
+
+    ! ---- reporters (reporters.f90): kind 1 state data, 2 trajectory,
+    !      3 config, 4 checkpoint; one frame written to a new file ----
+    function cstr(c, n) result(f)
+        character(kind=c_char), intent(in) :: c(*)
+        integer, intent(in) :: n
+        character(len=n) :: f
+        integer :: k
+        do k = 1, n
+            f(k:k) = c(k)
+        enddo
+    end function cstr
+
+    subroutine ref_report(kind, fname, flen, step, rt) bind(C, name="ref_report")
+        integer(c_int), value :: kind, flen, step
+        character(kind=c_char), intent(in) :: fname(*)
+        integer(c_int), intent(in) :: rt(*)
+        integer, parameter :: u = 91
+        rtype = rt(1:N)
+        select case (kind)
+        case (1)
+            call open_state_data_file(u, cstr(fname, flen))
+            call report_state_data(u, step)
+            call close_state_data_file(u)
+        case (2)
+            call open_trajectory_file(u, cstr(fname, flen))
+            call report_trajectory(u, step)
+            call close_trajectory_file(u)
+        case (3)
+            call open_confwriter_file(u, cstr(fname, flen))
+            call report_confwriter(u)
+            call close_confwriter_file(u)
+        case (4)
+            call open_checkpoint_file(u, cstr(fname, flen))
+            call report_checkpoint(u)
+            call close_checkpoint_file(u)
+        end select
+    end subroutine ref_report
+
+    ! load_checkpoint (reporters.f90:186-225) into freshly allocated module
+    ! arrays; returns N and NC
+    subroutine ref_load_checkpoint(fname, flen, n_out, nc_out, cfg) bind(C, name="ref_load_checkpoint")
+        integer(c_int), value :: flen
+        character(kind=c_char), intent(in) :: fname(*)
+        integer(c_int), intent(out) :: n_out, nc_out, cfg
+        integer, parameter :: u = 92
+        call free_if(mass); call free_if(inv_mass); call free_if(charge)
+        call free_if(sig_x); call free_if(sig_y); call free_if(sig_z)
+        call free_if(eps_x); call free_if(eps_y); call free_if(eps_z)
+        call free_if(Ixx); call free_if(Iyy); call free_if(Izz)
+        call free_if(Ixx_inv); call free_if(Iyy_inv); call free_if(Izz_inv)
+        call free_if(RX); call free_if(RY); call free_if(RZ)
+        call free_if(VX); call free_if(VY); call free_if(VZ)
+        call free_if(FX); call free_if(FY); call free_if(FZ)
+        call free_if(LX); call free_if(LY); call free_if(LZ)
+        call free_if(TX); call free_if(TY); call free_if(TZ)
+        call free_if(QW); call free_if(QX); call free_if(QY); call free_if(QZ)
+        call free_if(RX_old); call free_if(RY_old); call free_if(RZ_old)
+        if (allocated(rtype)) deallocate(rtype)
+        if (allocated(BBI)) deallocate(BBI)
+        if (allocated(BBJ)) deallocate(BBJ)
+        if (allocated(BB_len)) deallocate(BB_len)
+        open(unit=u, file=cstr(fname, flen), action='read')
+        call load_checkpoint(u)
+        close(u)
+        n_out = N
+        nc_out = NC
+        cfg = config_index
+    end subroutine ref_load_checkpoint
+
+    subroutine ref_get_types(rt) bind(C, name="ref_get_types")
+        integer(c_int), intent(out) :: rt(*)
+        rt(1:N) = rtype
+    end subroutine ref_get_types
+
+    subroutine ref_get_bonds(bi, bj, blen) bind(C, name="ref_get_bonds")
+        integer(c_int), intent(out) :: bi(*), bj(*)
+        real(c_double), intent(out) :: blen(*)
+        if (NC > 0) then
+            bi(1:NC) = BBI(1:NC); bj(1:NC) = BBJ(1:NC); blen(1:NC) = BB_len(1:NC)
+        endif
+    end subroutine ref_get_bonds
+
+    subroutine ref_get_box(b) bind(C, name="ref_get_box")
+        real(c_double), intent(out) :: b(3)
+        b = box
+    end subroutine ref_get_box
 
 end module ref_harness

@@ -33,6 +33,7 @@ LJ, LJ12, LJW, WCA, MORSE, GB, GBR, ECP, ECPR, DH, GBW, GBC, GBWC, ECPW, ECPC, E
 ECP_BRENT_REFERENCE, ECP_CONVERGED, ECP_LEARNED = 0, 1, 2
 REBUILD_REFERENCE, REBUILD_HALF_SKIN = 0, 1
 INT_VV, INT_NHT, INT_LGV, INT_LGVC = 1, 4, 6, 7
+REPORT_STATE_DATA, REPORT_TRAJECTORY, REPORT_CONFIG, REPORT_CHECKPOINT = 1, 2, 3, 4
 
 ERRORS = {-1: "SX_ERR_ARG", -2: "SX_ERR_HIP", -3: "SX_ERR_STATE", -4: "SX_ERR_CAPACITY",
           -5: "SX_ERR_CONSTRAINT", -6: "SX_ERR_CUTOFF", -7: "SX_ERR_NOTEMP"}
@@ -130,10 +131,21 @@ def lib():
             "sx_batch_thermo": [vp, _PD],
             "sx_batch_autotune": [vp, C.c_double, C.c_double, C.c_int, C.c_int, _PI, _PD],
             "sx_batch_set_shape": [vp, C.c_int],
+            "sx_set_atom_types": [vp, _PI], "sx_get_atom_types": [vp, _PI],
+            "sx_reporter_open": [vp, C.c_int, C.c_char_p, C.POINTER(C.c_void_p)],
+            "sx_report": [vp, C.c_int],
+            "sx_report_host": [vp, C.c_int, C.c_int, C.POINTER(_PD), _PI, _PD, _PD, C.c_int, _PI, _PI, _PD],
+            "sx_reporter_set_config_index": [vp, C.c_int], "sx_reporter_flush": [vp],
+            "sx_reporter_close": [vp],
+            "sx_load_checkpoint": [vp, C.c_char_p, _PI],
+            "sx_read_checkpoint_host": [C.c_char_p, _PI, _PI, _PD, _PI, C.c_int, C.c_int, C.POINTER(_PD), _PI,
+                                        _PI, _PI, _PD],
         }.items():
             getattr(L, name).argtypes = args
             getattr(L, name).restype = C.c_int
         L.sx_mlp_num_params.restype = C.c_int
+        L.sx_reporter_error.argtypes = [vp]
+        L.sx_reporter_error.restype = C.c_char_p
         _lib = L
     return _lib
 
@@ -514,3 +526,119 @@ class System:
 
     def state(self, fields=FIELDS):
         return {f: self.download(f) for f in fields}
+
+    # ---- atom types and checkpoints ----
+    def set_atom_types(self, rtype):
+        a = np.ascontiguousarray(rtype, dtype=np.int32)
+        if a.shape != (self.n,):
+            raise ValueError("one type per atom")
+        self._ok(self.L.sx_set_atom_types(self.h, a.ctypes.data_as(_PI)))
+
+    def get_atom_types(self):
+        a = np.zeros(self.n, np.int32)
+        self._ok(self.L.sx_get_atom_types(self.h, a.ctypes.data_as(_PI)))
+        return a
+
+    def load_checkpoint(self, filename):
+        """load_checkpoint (reporters.f90:188-221); returns config_index."""
+        ci = C.c_int()
+        n = C.c_int()
+        self.L.sx_read_checkpoint_host(os.fsencode(filename), C.byref(n), None, None, None, 0, 0, None, None, None,
+                                       None, None)
+        self._ok(self.L.sx_load_checkpoint(self.h, os.fsencode(filename), C.byref(ci)))
+        self.n = n.value
+        return ci.value
+
+    def reporter(self, kind, filename):
+        """A reporter of this system's state (reporters.f90)."""
+        return Reporter(kind, filename, self)
+
+
+class Reporter:
+    """One open reporter file: state data, trajectory, configuration or
+    checkpoint frames in the reference's text formats."""
+
+    def __init__(self, kind, filename, system=None):
+        self.L = lib()
+        self.kind = int(kind)
+        h = C.c_void_p()
+        ctx = system.h if system is not None else None
+        r = self.L.sx_reporter_open(ctx, self.kind, os.fsencode(filename), C.byref(h))
+        if r != 0:
+            msg = self.L.sx_last_error(ctx).decode() if ctx else "cannot open reporter"
+            raise SucroseError(r, msg)
+        self.h = h
+        self.system = system
+
+    def _ok(self, r):
+        if r != 0:
+            raise SucroseError(r, self.L.sx_reporter_error(self.h).decode())
+
+    def report(self, step=0):
+        self._ok(self.L.sx_report(self.h, int(step)))
+
+    def report_host(self, step, fields: dict, scal, box, rtype=None, bonds=None):
+        """A frame from host arrays: fields maps FIELDS names to arrays;
+        scal is the SX_NSCALARS vector; bonds = (bbi1, bbj1, length)."""
+        n = len(next(iter(fields.values())))
+        keep = {FIELD[k.upper()]: np.ascontiguousarray(v, dtype=np.float64) for k, v in fields.items()}
+        ptrs = (_PD * len(FIELDS))()
+        for k, v in keep.items():
+            ptrs[k] = _dp(v)
+        sc = np.ascontiguousarray(scal, dtype=np.float64)
+        bx = np.ascontiguousarray(box, dtype=np.float64)
+        rt = np.ascontiguousarray(rtype if rtype is not None else np.full(n, -1), dtype=np.int32)
+        if bonds is not None and len(bonds[0]):
+            bi = np.ascontiguousarray(bonds[0], dtype=np.int32)
+            bj = np.ascontiguousarray(bonds[1], dtype=np.int32)
+            bl = np.ascontiguousarray(bonds[2], dtype=np.float64)
+            nc, pbi, pbj, pbl = len(bi), bi.ctypes.data_as(_PI), bj.ctypes.data_as(_PI), _dp(bl)
+        else:
+            nc, pbi, pbj, pbl = 0, None, None, None
+        self._ok(self.L.sx_report_host(self.h, int(step), n, ptrs, rt.ctypes.data_as(_PI), _dp(sc), _dp(bx), nc,
+                                       pbi, pbj, pbl))
+
+    def set_config_index(self, k):
+        self._ok(self.L.sx_reporter_set_config_index(self.h, int(k)))
+
+    def flush(self):
+        self._ok(self.L.sx_reporter_flush(self.h))
+
+    def close(self):
+        if self.h:
+            r = self.L.sx_reporter_close(self.h)
+            self.h = None
+            if r != 0:
+                raise SucroseError(r, "reporter close failed")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def read_checkpoint(filename):
+    """The host parse of load_checkpoint: dict with n, nc, box, config_index,
+    fields (FIELDS name -> array, the checkpoint's columns and the defaults
+    of set_num_atoms elsewhere), rtype and the 1-based bonds."""
+    L = lib()
+    n, nc, ci = C.c_int(), C.c_int(), C.c_int()
+    box = np.zeros(3)
+    r = L.sx_read_checkpoint_host(os.fsencode(filename), C.byref(n), C.byref(nc), _dp(box), C.byref(ci), 0, 0,
+                                  None, None, None, None, None)
+    if r not in (0, -4):
+        raise SucroseError(r, f"cannot read checkpoint {filename}")
+    N, NC = n.value, nc.value
+    arrays = {f: np.zeros(N) for f in FIELDS}
+    ptrs = (_PD * len(FIELDS))()
+    for k, f in enumerate(FIELDS):
+        ptrs[k] = _dp(arrays[f])
+    rt = np.zeros(N, np.int32)
+    bi = np.zeros(max(NC, 1), np.int32); bj = np.zeros(max(NC, 1), np.int32); bl = np.zeros(max(NC, 1))
+    r = L.sx_read_checkpoint_host(os.fsencode(filename), C.byref(n), C.byref(nc), _dp(box), C.byref(ci), N, NC, ptrs,
+                                  rt.ctypes.data_as(_PI), bi.ctypes.data_as(_PI), bj.ctypes.data_as(_PI), _dp(bl))
+    if r != 0:
+        raise SucroseError(r, f"cannot read checkpoint {filename}")
+    return {"n": N, "nc": NC, "box": box, "config_index": ci.value, "fields": arrays, "rtype": rt,
+            "bonds": (bi[:NC], bj[:NC], bl[:NC])}

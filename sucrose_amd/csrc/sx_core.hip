@@ -390,6 +390,8 @@ int sx_set_num_atoms(sx_ctx *c, int num_atoms, int rigid_atoms)
         c->nrigid = rigid_atoms;
         c->n = num_atoms * rigid_atoms;
         c->nc = 0;
+        c->h_rtype.assign(c->n, -1);
+        c->h_bbi.clear(); c->h_bbj.clear(); c->h_bblen.clear();
         c->ld = ((long)c->n + 63) / 64 * 64;
         HIPCHK(hipMalloc(&c->d_fields, sizeof(double) * SX_NFIELDS * c->ld));
         HIPCHK(hipMalloc(&c->d_pe_acc, sizeof(double) * c->ld));
@@ -426,6 +428,24 @@ int sx_set_box(sx_ctx *c, const double box[3], int periodic)
         c->h_scal[SX_DENSITY] = (double)(c->n / c->nrigid) / (box[0] * box[1] * box[2]);
         sx_sync_scalars_to_device(c);
         c->have_graph = false;
+    });
+}
+
+int sx_set_atom_types(sx_ctx *c, const int *rtype)
+{
+    return guard(c, [&] {
+        need_system(c);
+        if (!rtype) throw sx_error{SX_ERR_ARG, "null types"};
+        c->h_rtype.assign(rtype, rtype + c->n);
+    });
+}
+
+int sx_get_atom_types(sx_ctx *c, int *rtype)
+{
+    return guard(c, [&] {
+        need_system(c);
+        if (!rtype) throw sx_error{SX_ERR_ARG, "null types"};
+        std::copy(c->h_rtype.begin(), c->h_rtype.end(), rtype);
     });
 }
 
@@ -494,6 +514,8 @@ int sx_initialize_system(sx_ctx *c, int remove_com)
         if (c->box[0] <= 0 || c->box[1] <= 0 || c->box[2] <= 0)
             throw sx_error{SX_ERR_STATE, "Error: Box dimensions not set. Set box dimensions before initializing system."};
         hipLaunchKernelGGL(k_init_defaults, dim3(sx_blocks(c->n, 256)), dim3(256), 0, c->stream, c->dstate());
+        for (int &t : c->h_rtype)   // system.f90:590-593
+            if (t < 0) t = 0;
         sx_sync_scalars_to_host(c);
         int dof = 6 * c->n - c->nc;
         if (remove_com) dof -= 3;

@@ -1374,6 +1374,9 @@ int sx_set_constraints(sx_ctx *c, int nc, const int *bbi1, const int *bbj1, cons
         up(bl, &m.bond_len);
         up(gidx, &m.bond_gidx);
         c->nc = nc;
+        c->h_bbi.assign(bbi1, bbi1 + nc);
+        c->h_bbj.assign(bbj1, bbj1 + nc);
+        c->h_bblen.assign(len, len + nc);
         c->have_graph = false;
     });
 }

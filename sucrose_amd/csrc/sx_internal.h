@@ -112,6 +112,12 @@ struct sx_ctx {
     int part_cap = 0;
     int *d_err = nullptr;
     unsigned *d_ticket = nullptr; // last-block reduction counters (self re-arming)
+    // host-side metadata the kernels never read: atom types (rtype of
+    // module system, -1 until set) and the constraint table as given
+    // (1-based), for the reporters
+    std::vector<int> h_rtype;
+    std::vector<int> h_bbi, h_bbj;
+    std::vector<double> h_bblen;
     // constraints
     int *d_bbi = nullptr, *d_bbj = nullptr;
     double *d_bblen = nullptr;

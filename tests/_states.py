@@ -95,3 +95,29 @@ def rel_err(a, b, scale=None):
         scale = np.sqrt(np.mean(b * b)) if b.size else 1.0
     scale = scale if scale > 0 else 1.0
     return float(np.max(np.abs(a - b)) / scale) if b.size else 0.0
+
+
+def report_state(rng, n, L=7.0, nbonds=0):
+    """A state for the reporter formats: ordinary values plus the cases the
+    F0.d descriptor treats specially (exact ties at 3 and 10 decimals,
+    negative values that round to zero, -0.0, |x| < 1, large magnitudes),
+    integer types of several widths and signs, and bonds."""
+    st = base_state(rng, n, L, sig=(1.0, 1.5, 3.0), eps=(5.0, 4.0, 1.0), charges=True)
+    special = np.array([1.0625, -1.0625, 2.0625, -0.0625, 0.0004, -0.0004, -0.0, 0.0, 0.9995, -0.99951,
+                        123456.789, -98765.4321, 1e-11, -1e-11, 0.5, -0.5, 2.5e-11, 0.00000000005,
+                        -0.00000000005, 1.00000000005])
+    fields = ["rx", "ry", "rz", "vx", "vy", "vz", "lx", "ly", "lz", "fx", "fy", "fz", "qx", "charge", "mass"]
+    for k, f in enumerate(fields):
+        v = st[f].copy()
+        m = min(n, len(special))
+        v[:m] = np.roll(special, k)[:m] if f != "mass" else np.abs(np.roll(special, k)[:m]) + 0.5
+        st[f] = v
+    rtype = rng.integers(-3, 2000, n).astype(np.int32)
+    rtype[0] = 0
+    if nbonds:
+        st.bbi = np.arange(nbonds, dtype=np.int64) % (n - 1)
+        st.bbj = st.bbi + 1
+        st.bb_len = rng.uniform(0.5, 1.5, nbonds)
+    st["pot_eng"], st["kin_eng"], st["kin_rot"] = -1234.5625 * n / 8, 0.0625 * n, 0.125 * n
+    st["temp"], st["pressure"], st["density"] = 1.0625, -0.00000000005, n / L ** 3
+    return st, rtype
