@@ -217,6 +217,26 @@ typedef struct sx_step_spec {
 int sx_run(sx_ctx *ctx, const sx_step_spec *spec, int nsteps);
 
 /* ------------------------------------------------------------------ */
+/* Monte Carlo barostat (module monte_carlo_barostat, integrators.f90:859-947) */
+/* ------------------------------------------------------------------ */
+int sx_set_target_pressure(sx_ctx *ctx, double pres);   /* set_target_pressure system.f90:224 */
+/* mc_barostat (integrators.f90:864-945): an isotropic volume move of at most
+ * volume_change * V (volume_change < 0: the reference's default 0.01), the
+ * positions scaled with the box, accepted on log(u[1]) < -beta dU - beta P dV
+ * + N log(V_new/V_old) with beta = 1/temp.  compute_potential is the
+ * neighbour list rebuilt at the current box, initialize_forces and the pair
+ * loops (and, with spec->bonded, the bonded terms) of spec (its integrator,
+ * dt, damping and first_step are not used).  u[0], u[1] are the two numbers
+ * the reference draws with random_number.  An accepted move leaves the
+ * forces, pot_eng and the list of the new state, sets density = N / V and,
+ * as compute_potential's initialize_forces does in the reference, zero
+ * kin_eng, kin_rot, temp, pressure and the virials; a rejected one (or
+ * V_new <= 0) restores the box, the positions, the list, the forces, torques
+ * and every scalar the move started from.  RX_old.. hold the positions
+ * before the move, as in the reference.  accept may be NULL. */
+int sx_mc_barostat(sx_ctx *ctx, const sx_step_spec *spec, double volume_change, const double u[2], int *accept);
+
+/* ------------------------------------------------------------------ */
 /* reporters (reporters.f90): text files in the reference's formats     */
 /* ------------------------------------------------------------------ */
 enum sx_report_kind {

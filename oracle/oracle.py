@@ -357,6 +357,41 @@ class Oracle:
         o = self._nl(st, nl)
         return bool(self.lib.or_nlist_needs_rebuild(C.byref(s), C.byref(o)))
 
+    # -- Monte Carlo barostat --
+    def mc_barostat(self, st: State, potential, target_pres, u, volume_change=None):
+        """mc_barostat (integrators.f90:864-945) step by step, with the
+        reference's expressions in its order; potential(st) is its
+        compute_potential (it sets st's forces and returns pot_eng); u are the
+        two numbers the move draws with random_number.  Returns accept
+        (None where the reference returns early without setting it)."""
+        if target_pres <= 1e-3:
+            raise ValueError("Error: Target pressure too small.")
+        max_dV = 0.01 if volume_change is None else volume_change
+        beta = 1.0 / st["temp"]
+        V_old = st.box[0] * st.box[1] * st.box[2]
+        L_old = st.box.copy()
+        for a, b in (("rx_old", "rx"), ("ry_old", "ry"), ("rz_old", "rz")):
+            st[a] = st[b].copy()
+        U_old = potential(st)
+        V_new = V_old + max_dV * (2.0 * u[0] - 1.0) * V_old
+        if V_new <= 0.0:
+            return None
+        scale = (V_new / V_old) ** (1.0 / 3.0)
+        st.box[:] = L_old * scale
+        for a, b in (("rx", "rx_old"), ("ry", "ry_old"), ("rz", "rz_old")):
+            st[a] = st[b] * scale
+        U_new = potential(st)
+        dU = U_new - U_old
+        exponent = -beta * dU - beta * target_pres * (V_new - V_old) + float(st.n) * np.log(V_new / V_old)
+        if np.log(u[1]) < exponent:
+            volume = st.box[0] * st.box[1] * st.box[2]
+            st["density"] = float(st.n) / volume
+            return True
+        st.box[:] = L_old
+        for a, b in (("rx", "rx_old"), ("ry", "ry_old"), ("rz", "rz_old")):
+            st[a] = st[b].copy()
+        return False
+
     # -- integrators --
     def integrate(self, kind, phase, st: State, dt, step=0, damping=1.0, coupling=None):
         d = C.c_double
@@ -497,6 +532,16 @@ class Reference:
         self.lib.ref_get_box(_dp(box))
         return n.value, nc.value, cfg.value, np.ascontiguousarray(buf), rt, (bi[:nc.value], bj[:nc.value],
                                                                              bl[:nc.value]), box
+
+    def mc_barostat(self, seed, target_pres, volume_change, sigma, eps, rc):
+        """mc_barostat with the harness's all-pairs Lennard-Jones potential;
+        returns the two random numbers it drew and accept."""
+        u = np.zeros(2)
+        acc = C.c_int()
+        d = C.c_double
+        self.lib.ref_mc_barostat(C.c_int(seed), d(target_pres), d(volume_change), d(sigma), d(eps), d(rc), _dp(u),
+                                 C.byref(acc))
+        return u, bool(acc.value)
 
     def bonded(self, kind, idx, coeff, angle=0.0):
         idx = list(idx) + [0] * (4 - len(idx))

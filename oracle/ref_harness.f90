@@ -27,6 +27,8 @@ module ref_harness
     implicit none
 
     integer, parameter :: NFIELD = 37
+    ! Lennard-Jones parameters of harness_potential (mc_barostat's compute_potential)
+    real(real64) :: hp_sigma = 1.0, hp_eps = 1.0, hp_rc = 2.5
 
 contains
 
@@ -490,5 +492,49 @@ contains
         real(c_double), intent(out) :: b(3)
         b = box
     end subroutine ref_get_box
+
+    ! compute_potential for mc_barostat: initialize_forces, then the
+    ! Lennard-Jones pair loop over all pairs I < J; returns pot_eng
+    real(real64) function harness_potential()
+        integer :: I, J
+        real(real64) :: pe, fij(3)
+        call initialize_forces
+        do I = 1, N - 1
+            do J = I + 1, N
+                call lj_calculate_forces(I, J, hp_sigma, hp_eps, hp_rc, pe, fij)
+                pot_eng = pot_eng + pe
+                FX(I) = FX(I) + fij(1); FY(I) = FY(I) + fij(2); FZ(I) = FZ(I) + fij(3)
+                FX(J) = FX(J) - fij(1); FY(J) = FY(J) - fij(2); FZ(J) = FZ(J) - fij(3)
+            enddo
+        enddo
+        harness_potential = pot_eng
+    end function harness_potential
+
+    ! mc_barostat (integrators.f90:864-945) with harness_potential; the
+    ! generator of random_number is seeded from `seed` and the two numbers
+    ! the move draws are returned in u (drawn ahead from the same seed)
+    subroutine ref_mc_barostat(seed, pres, volume_change, sigma, eps, rc, u, acc) bind(C, name="ref_mc_barostat")
+        integer(c_int), value :: seed
+        real(c_double), value :: pres, volume_change, sigma, eps, rc
+        real(c_double), intent(out) :: u(2)
+        integer(c_int), intent(out) :: acc
+        integer :: ns, k
+        integer, allocatable :: sd(:)
+        logical :: accept
+        hp_sigma = sigma; hp_eps = eps; hp_rc = rc
+        call random_seed(size=ns)
+        allocate(sd(ns))
+        do k = 1, ns
+            sd(k) = seed + 7919 * k
+        enddo
+        call random_seed(put=sd)
+        call random_number(u(1))
+        call random_number(u(2))
+        call random_seed(put=sd)
+        call set_target_pressure(pres)
+        accept = .false.
+        call mc_barostat(harness_potential, accept, volume_change)
+        acc = merge(1, 0, accept)
+    end subroutine ref_mc_barostat
 
 end module ref_harness

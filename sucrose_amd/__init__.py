@@ -132,6 +132,8 @@ def lib():
             "sx_batch_autotune": [vp, C.c_double, C.c_double, C.c_int, C.c_int, _PI, _PD],
             "sx_batch_set_shape": [vp, C.c_int],
             "sx_set_atom_types": [vp, _PI], "sx_get_atom_types": [vp, _PI],
+            "sx_set_target_pressure": [vp, C.c_double],
+            "sx_mc_barostat": [vp, C.POINTER(StepSpec), C.c_double, _PD, _PI],
             "sx_reporter_open": [vp, C.c_int, C.c_char_p, C.POINTER(C.c_void_p)],
             "sx_report": [vp, C.c_int],
             "sx_report_host": [vp, C.c_int, C.c_int, C.POINTER(_PD), _PI, _PD, _PD, C.c_int, _PI, _PI, _PD],
@@ -526,6 +528,28 @@ class System:
 
     def state(self, fields=FIELDS):
         return {f: self.download(f) for f in fields}
+
+    # ---- Monte Carlo barostat ----
+    def set_target_pressure(self, p):
+        self._ok(self.L.sx_set_target_pressure(self.h, float(p)))
+
+    def mc_barostat(self, forces, u, volume_change=None, bonded=False):
+        """mc_barostat (integrators.f90:864-945): compute_potential is the
+        list rebuilt at the current box plus the pair loops `forces` (list of
+        (kind, params...)) and the bonded terms; u are the move's two uniform
+        numbers.  Returns True when the volume move is accepted."""
+        sp = StepSpec()
+        sp.nforces = len(forces)
+        for k, fp in enumerate(forces):
+            sp.kinds[k] = fp[0]
+            for q, v in enumerate(fp[1:]):
+                sp.params[k][q] = v
+        sp.bonded = 1 if bonded else 0
+        uu = np.ascontiguousarray(u, dtype=np.float64)
+        acc = C.c_int()
+        dv = -1.0 if volume_change is None else float(volume_change)
+        self._ok(self.L.sx_mc_barostat(self.h, C.byref(sp), dv, _dp(uu), C.byref(acc)))
+        return bool(acc.value)
 
     # ---- atom types and checkpoints ----
     def set_atom_types(self, rtype):

@@ -107,6 +107,7 @@ struct sx_ctx {
     int *d_segcount = nullptr;   // ECP work-list segment counts
     double *d_scal = nullptr;    // SX_NSCALARS doubles (device copy of the scalars)
     double h_scal[SX_NSCALARS] = {0};
+    double target_pres = 0.0;    // set_target_pressure (system.f90:224)
     bool scal_dirty_host = false; // host copy newer than device
     double *d_part = nullptr;    // reduction partials
     int part_cap = 0;
@@ -202,6 +203,7 @@ void sx_sync_scalars_to_device(sx_ctx *c);
 void sx_sync_scalars_to_host(sx_ctx *c);
 void sx_nlist_build_impl(sx_ctx *c, bool conditional);
 void sx_nlist_flag_impl(sx_ctx *c);
+void sx_nlist_rebox_impl(sx_ctx *c);
 void sx_nlist_want_half(sx_ctx *c);
 void sx_forces_impl(sx_ctx *c, int kind, const double *p, int np);
 void sx_bonded_impl(sx_ctx *c);

@@ -718,6 +718,25 @@ static void need_nlist(sx_ctx *c)
     if (!c->nl.init) throw sx_error{SX_ERR_STATE, "neighbour list not initialised (sx_nlist_init)"};
 }
 
+// the list after a change of box (the barostat): the cell grid is derived
+// again when its cells would no longer be at least rc+skin wide, then the
+// list is built
+void sx_nlist_rebox_impl(sx_ctx *c)
+{
+    NList &nl = c->nl;
+    for (int d = 0; d < 3; ++d)
+        if (nl.cutoff > c->box[d] / 2.0) throw sx_error{SX_ERR_CUTOFF, "A cutoff is too large for the box dimensions"};
+    double rskin = nl.cutoff + nl.skin;
+    bool same = nl.use_cells == (c->periodic != 0);
+    for (int d = 0; d < 3 && same; ++d) {
+        int want = (int)std::floor(c->box[d] / rskin);
+        if (nl.use_cells ? (c->box[d] / nl.ncell[d] < rskin) : (want >= 3)) same = false;
+    }
+    if (!same) nl_setup(c);
+    sx_nlist_build_impl(c, false);
+    c->have_graph = false;
+}
+
 extern "C" {
 
 int sx_nlist_init(sx_ctx *c, double skin, double cutoff)

@@ -4,8 +4,11 @@
 // round trip.
 #include "sx_internal.h"
 
+#include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 __global__ void k_step_set(int *dstep, int v) { *dstep = v; }
 
@@ -14,6 +17,23 @@ struct DeferPe {
     explicit DeferPe(sx_ctx *c_) : c(c_) { c->defer_pe = true; }
     ~DeferPe() { c->defer_pe = false; c->ecp_pp_by_step = false; }
 };
+
+// the parameter count of each pair kind in a step spec
+static int nparams_of(int kind)
+{
+    switch (kind) {
+    case SX_LJ: case SX_LJ12: case SX_DH: return 3;
+    case SX_LJW: case SX_MORSE: return 4;
+    case SX_WCA: return 2;
+    case SX_GB: case SX_GBR: return 3;
+    case SX_GBW: case SX_GBC: return 4;
+    case SX_GBWC: return 5;
+    case SX_ECP: return 1;
+    case SX_ECPW: case SX_ECPC: return 2;
+    case SX_ECPWC: return 3;
+    default: return 0;
+    }
+}
 
 static bool is_ecp(int kind)
 {
@@ -44,22 +64,7 @@ static void enqueue_step(sx_ctx *c, const sx_step_spec &sp)
         sx_nlist_build_impl(c, true);
         sx_init_forces_impl(c);
     }
-    for (int f = 0; f < sp.nforces; ++f) {
-        int np = 0;
-        switch (sp.kinds[f]) {
-        case SX_LJ: case SX_LJ12: case SX_DH: np = 3; break;
-        case SX_LJW: case SX_MORSE: np = 4; break;
-        case SX_WCA: np = 2; break;
-        case SX_GB: case SX_GBR: np = 3; break;
-        case SX_GBW: case SX_GBC: np = 4; break;
-        case SX_GBWC: np = 5; break;
-        case SX_ECP: np = 1; break;
-        case SX_ECPW: case SX_ECPC: np = 2; break;
-        case SX_ECPWC: np = 3; break;
-        default: np = 0;
-        }
-        sx_forces_impl(c, sp.kinds[f], sp.params[f], np);
-    }
+    for (int f = 0; f < sp.nforces; ++f) sx_forces_impl(c, sp.kinds[f], sp.params[f], nparams_of(sp.kinds[f]));
     if (sp.bonded) sx_bonded_impl(c);
     // final step with calculate_thermo folded in (and the Langevin counter advance)
     switch (sp.integrator) {
@@ -126,5 +131,110 @@ extern "C" int sx_run(sx_ctx *c, const sx_step_spec *spec, int nsteps)
         }
         for (int t = 0; t < nsteps; ++t) HIPCHK(hipGraphLaunch(c->graph_exec, c->stream));
         sx_check_device_error(c);
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Monte Carlo barostat (integrators.f90:864-945)
+// ---------------------------------------------------------------------------
+__global__ void k_scale_positions(DState s, double scale, int save)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    if (save) {   // RX_old = RX (integrators.f90:898-900)
+        s.f[SX_RX_OLD][i] = s.f[SX_RX][i];
+        s.f[SX_RY_OLD][i] = s.f[SX_RY][i];
+        s.f[SX_RZ_OLD][i] = s.f[SX_RZ][i];
+    } else {      // RX = RX_old * scale (:917-919; scale 1 restores, :937-939)
+        s.f[SX_RX][i] = s.f[SX_RX_OLD][i] * scale;
+        s.f[SX_RY][i] = s.f[SX_RY_OLD][i] * scale;
+        s.f[SX_RZ][i] = s.f[SX_RZ_OLD][i] * scale;
+    }
+}
+
+// compute_potential: the list rebuilt at the current box, initialize_forces,
+// the pair loops (and bonded terms) of the spec; returns pot_eng
+static double mc_potential(sx_ctx *c, const sx_step_spec &sp)
+{
+    sx_nlist_rebox_impl(c);
+    sx_init_forces_impl(c);
+    for (int f = 0; f < sp.nforces; ++f) sx_forces_impl(c, sp.kinds[f], sp.params[f], nparams_of(sp.kinds[f]));
+    if (sp.bonded) sx_bonded_impl(c);
+    sx_sync_scalars_to_host(c);
+    return c->h_scal[SX_POT_ENG];
+}
+
+extern "C" int sx_set_target_pressure(sx_ctx *c, double pres)
+{
+    return guard(c, [&] { c->target_pres = pres; });
+}
+
+extern "C" int sx_mc_barostat(sx_ctx *c, const sx_step_spec *spec, double volume_change, const double u[2],
+                              int *accept)
+{
+    return guard(c, [&] {
+        need_system(c);
+        if (!spec || !u || spec->nforces < 0 || spec->nforces > 4) throw sx_error{SX_ERR_ARG, "bad arguments"};
+        if (!c->nl.init) throw sx_error{SX_ERR_STATE, "neighbour list not initialised (sx_nlist_init)"};
+        if (c->target_pres <= 1e-3) throw sx_error{SX_ERR_ARG, "Error: Target pressure too small."};
+        if (accept) *accept = 0;
+        double max_dV = volume_change < 0.0 ? 0.01 : volume_change;
+        sx_sync_scalars_to_host(c);
+        double beta = 1.0 / c->h_scal[SX_TEMP];
+        double V_old = c->box[0] * c->box[1] * c->box[2];
+        double L_old[3] = {c->box[0], c->box[1], c->box[2]};
+        DState s = c->dstate();
+        int nb = sx_blocks(c->n, 256);
+        hipLaunchKernelGGL(k_scale_positions, dim3(nb), dim3(256), 0, c->stream, s, 1.0, 1);
+        // forces and torques of the state the move starts from, restored on
+        // a rejected move (with the list and pot_eng of that state)
+        long n = c->n;
+        std::vector<double> keep(6 * n);
+        const int fk[6] = {SX_FX, SX_FY, SX_FZ, SX_TX, SX_TY, SX_TZ};
+        for (int k = 0; k < 6; ++k)
+            HIPCHK(hipMemcpyAsync(keep.data() + k * n, c->d_fields + (long)fk[k] * c->ld, sizeof(double) * n,
+                                  hipMemcpyDeviceToHost, c->stream));
+        double scal_keep[SX_NSCALARS];
+        std::copy(c->h_scal, c->h_scal + SX_NSCALARS, scal_keep);
+        auto restore = [&]() {
+            for (int d = 0; d < 3; ++d) c->box[d] = L_old[d];
+            DState s0 = c->dstate();
+            hipLaunchKernelGGL(k_scale_positions, dim3(nb), dim3(256), 0, c->stream, s0, 1.0, 0);
+            sx_nlist_rebox_impl(c);
+            for (int k = 0; k < 6; ++k)
+                HIPCHK(hipMemcpyAsync(c->d_fields + (long)fk[k] * c->ld, keep.data() + k * n, sizeof(double) * n,
+                                      hipMemcpyHostToDevice, c->stream));
+            std::copy(scal_keep, scal_keep + SX_NSCALARS, c->h_scal);
+            sx_sync_scalars_to_device(c);
+            HIPCHK(hipStreamSynchronize(c->stream));
+        };
+        double U_old = mc_potential(c, *spec);
+        double V_new = V_old + max_dV * (2.0 * u[0] - 1.0) * V_old;
+        if (V_new <= 0.0) {   // unphysical: no move (integrators.f90:907-909)
+            restore();
+            return;
+        }
+        double scale = std::pow(V_new / V_old, 1.0 / 3.0);
+        for (int d = 0; d < 3; ++d) c->box[d] = L_old[d] * scale;
+        hipLaunchKernelGGL(k_scale_positions, dim3(nb), dim3(256), 0, c->stream, c->dstate(), scale, 0);
+        double U_new;
+        try {
+            U_new = mc_potential(c, *spec);
+        } catch (const sx_error &e) {
+            restore();
+            throw;
+        }
+        double dU = U_new - U_old;
+        double exponent = -beta * dU - beta * c->target_pres * (V_new - V_old) + (double)c->n * std::log(V_new / V_old);
+        if (std::log(u[1]) < exponent) {
+            double volume = c->box[0] * c->box[1] * c->box[2];
+            c->h_scal[SX_DENSITY] = (double)c->n / volume;
+            sx_sync_scalars_to_device(c);
+            HIPCHK(hipStreamSynchronize(c->stream));
+            if (accept) *accept = 1;
+        } else {
+            restore();
+        }
+        c->have_graph = false;
     });
 }

@@ -121,3 +121,22 @@ def report_state(rng, n, L=7.0, nbonds=0):
     st["pot_eng"], st["kin_eng"], st["kin_rot"] = -1234.5625 * n / 8, 0.0625 * n, 0.125 * n
     st["temp"], st["pressure"], st["density"] = 1.0625, -0.00000000005, n / L ** 3
     return st, rtype
+
+
+def all_pairs(n):
+    """A neighbour-list dict holding every pair I < J in the reference's order."""
+    return {"offset": np.concatenate([[0], np.cumsum(np.arange(n - 1, -1, -1))]).astype(np.int64),
+            "nbr": np.concatenate([np.arange(i + 1, n) for i in range(n)] + [np.zeros(1)]).astype(np.int32),
+            "x_last": np.zeros(n), "y_last": np.zeros(n), "z_last": np.zeros(n),
+            "cutoff": 0.0, "skin": 0.0, "nnz": n * (n - 1) // 2}
+
+
+def lattice_lj(rng, n, L, jitter=0.05, T=1.0):
+    """n Lennard-Jones particles on a jittered cubic lattice in a box of edge L."""
+    m = int(np.ceil(n ** (1 / 3)))
+    st = base_state(rng, n, L, forces=False, T=T)
+    g = np.arange(m) * (L / m) - L / 2
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)[:n] + rng.normal(size=(n, 3)) * jitter
+    st["rx"], st["ry"], st["rz"] = P[:, 0], P[:, 1], P[:, 2]
+    return st

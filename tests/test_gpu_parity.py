@@ -486,3 +486,58 @@ def test_run_graph_langevin(dev, oracle, kind, params, rc):
         assert rel_err(got[f], ref[f]) < 1e-9, (f, rel_err(got[f], ref[f]))
     for s in ("pot_eng", "kin_eng", "kin_rot", "temp"):
         assert abs(got[s] - ref[s]) <= 1e-9 * max(1.0, abs(ref[s])), (s, got[s], ref[s])
+
+
+def test_mc_barostat(dev, oracle):
+    """sx_mc_barostat vs the oracle's mc_barostat (integrators.f90:864-945)
+    with the same two uniform numbers; compute_potential is the Lennard-Jones
+    loop over a list rebuilt at the current box on both sides.  Accepted
+    moves: box and positions bit for bit, forces and pot_eng to 1e-10.
+    Rejected moves: box and positions restored bit for bit, and the forces
+    and pot_eng the move started from."""
+    import sucrose_amd as sx
+    from tests._states import lattice_lj
+    say("mc barostat: accepted and rejected volume moves vs oracle")
+    seen = set()
+    cases = [(50.0, 0.002, (0.2, 0.7)), (50.0, 0.002, (0.9, 0.7)), (0.5, 0.01, (0.8, 0.9)),
+             (3.0, 0.002, (0.45, 0.3)), (0.5, 0.002, (0.3, 0.999)), (50.0, 0.02, (0.1, 0.5))]
+    for c, (pres, dv, u) in enumerate(cases):
+        rng = np.random.default_rng(300 + c)
+        st = lattice_lj(rng, 512, 8.6)
+        st["temp"] = 1.3
+        to_device(dev, st)
+        dev.init_nlist(0.3, 2.5)
+        dev.set_target_pressure(pres)
+        dev.initialize_forces()   # zeroes temp, as the reference's does
+        dev.forces(LJ, 1.0, 1.0, 2.5)
+        dev.set_scalar("TEMP", 1.3)
+        before = from_device(dev, st)
+        acc = dev.mc_barostat([(LJ, 1.0, 1.0, 2.5)], u, volume_change=dv)
+
+        def potential(s):
+            nl = oracle.nlist_new(s, 2.5, 0.3)
+            oracle.nlist_build(s, nl)
+            oracle.call("or_initialize_forces", s)
+            oracle.pair_loop(s, nl, LJ, [1.0, 1.0, 2.5])
+            return s["pot_eng"]
+
+        mine = st.copy()
+        a = oracle.mc_barostat(mine, potential, pres, np.asarray(u), dv)
+        got = from_device(dev, st)
+        seen.add(acc)
+        assert acc == a, (c, acc, a)
+        assert np.array_equal(dev.get_box(), mine.box), c
+        for f in ("rx", "ry", "rz", "rx_old", "ry_old", "rz_old"):
+            assert np.array_equal(got[f], mine[f]), (c, f)
+        if acc:
+            for f in ("fx", "fy", "fz"):
+                assert rel_err(got[f], mine[f]) < TOL, (c, f)
+            assert abs(got["pot_eng"] - mine["pot_eng"]) <= TOL * abs(mine["pot_eng"])
+            assert dev.get_scalar("DENSITY") == mine["density"]
+        else:
+            for f in ("fx", "fy", "fz"):
+                assert np.array_equal(got[f], before[f]), (c, f)
+            for f in ("pot_eng", "temp"):
+                assert got[f] == before[f], (c, f)
+    assert seen == {True, False}
+    dev.set_box(9.0)

@@ -5,6 +5,8 @@ Both sides run on identical FP64 inputs; every comparison is bit-for-bit
 in the reference's evaluation order without contraction.  These tests skip
 when ``oracle/_ref`` has not been built (``make -C oracle ref``).
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -287,3 +289,41 @@ def test_driver_loop_bit_exact(integ, kind, p, n, box, sig, eps, rc, skin, nstep
                                     sig, eps, rc, skin, T)
     assert np.isfinite(pe) and np.isfinite(temp)
     assert same, diff
+
+
+def test_mc_barostat_bit_exact(oracle):
+    """mc_barostat (integrators.f90:864-945) of the reference, driven through
+    the harness with an all-pairs Lennard-Jones compute_potential and its
+    random_number seeded, against the oracle's restatement with the same two
+    numbers: accept, box, positions, forces and pot_eng bit for bit."""
+    from oracle.oracle import Reference
+    from tests._states import all_pairs, lattice_lj
+    R = Reference(private=True)
+    seen = set()
+    for seed, (pres, dv) in enumerate([(0.5, 0.01), (50.0, 0.02), (0.01, 0.05), (3.0, 0.002), (50.0, 0.002),
+                                       (1.0, 0.001), (0.5, 0.002), (3.0, 0.002)] * 2):
+        rng = np.random.default_rng(200 + seed % 8)
+        st = lattice_lj(rng, 108, 5.2)
+        st["temp"] = 1.3
+        R.load(st)
+        u, acc = R.mc_barostat(seed + 1, pres, dv, 1.0, 1.0, 2.5)
+        ref = st.copy()
+        R.store(ref)
+        box = np.zeros(3)
+        R.lib.ref_get_box(box.ctypes.data_as(C.POINTER(C.c_double)))
+        nl = all_pairs(st.n)
+
+        def potential(s):
+            oracle.call("or_initialize_forces", s)
+            oracle.pair_loop(s, nl, LJ, [1.0, 1.0, 2.5])
+            return s["pot_eng"]
+
+        mine = st.copy()
+        a = oracle.mc_barostat(mine, potential, pres, u, dv)
+        assert a == acc, (seed, a, acc)
+        seen.add(acc)
+        assert np.array_equal(mine.box, box)
+        for f in ("rx", "ry", "rz", "rx_old", "fx", "fz"):
+            assert np.array_equal(mine[f], ref[f]), (seed, f)
+        assert mine["pot_eng"] == ref["pot_eng"] and mine["density"] == ref["density"]
+    assert seen == {True, False}
