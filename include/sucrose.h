@@ -78,10 +78,13 @@ enum sx_scalar {
  *   SX_ECPW  cutoff, wf                             force.f90:1553 (welded)
  *   SX_ECPC  cutoff, chirality                      force.f90:1744 (chiral)
  *   SX_ECPWC cutoff, wf, chirality                  force.f90:1986
+ *   SX_DIPOLE cutoff, dielectric                    force.f90:2687 (point dipoles:
+ *            the body z axis times the charge; sx_forces also applies the
+ *            routine's reaction-field side effects, see sx_init_reaction_field)
  */
 enum sx_kind {
     SX_LJ = 1, SX_LJ12, SX_LJW, SX_WCA, SX_MORSE, SX_GB, SX_GBR, SX_ECP, SX_ECPR, SX_DH,
-    SX_GBW, SX_GBC, SX_GBWC, SX_ECPW, SX_ECPC, SX_ECPWC
+    SX_GBW, SX_GBC, SX_GBWC, SX_ECPW, SX_ECPC, SX_ECPWC, SX_DIPOLE
 };
 
 /* ECP contact-parameter solver (sx_set_ecp_solver) */
@@ -212,9 +215,54 @@ typedef struct sx_step_spec {
     double dt, damping;
     int first_step;        /* Langevin counter of the first replayed step         */
 } sx_step_spec;
-/* Runs nsteps of: initial step, nlist update, initialize_forces, pair loops,
- * bonded, final step, calculate_thermo.  The first call captures the graph. */
+/* Runs nsteps of: initial step, placement of registered composite-body
+ * satellites, nlist update, initialize_forces, pair loops (then
+ * eval_reaction_field when a kind is SX_DIPOLE), bonded, transfer of the
+ * satellites' forces to their centres, final step, calculate_thermo.  The
+ * first call captures the graph. */
 int sx_run(sx_ctx *ctx, const sx_step_spec *spec, int nsteps);
+
+/* ------------------------------------------------------------------ */
+/* composite rigid bodies (system.f90:743-881)                          */
+/* ------------------------------------------------------------------ */
+/* copy_transform (:743-842) for count (from, to) pairs, 1-based, as if called
+ * in order: `to` is placed at `from` plus offsets[3t..3t+2] rotated by from's
+ * orientation and takes from's quaternion; set_inertia -1 (argument absent),
+ * 0 or 1 (the inertia bookkeeping, with the reference's three errors).  The
+ * `to` atoms must be distinct and none may be a `from` of the same call. */
+int sx_copy_transform(sx_ctx *ctx, int count, const int *from1, const int *to1, const double *offsets,
+                      int set_inertia);
+/* transfer_forces (:844-881) for count (from, to) pairs, 1-based, as if
+ * called in order: from's force and torque and the torque of that force about
+ * `to` are added to `to`, from's are zeroed.  The `from` atoms must be
+ * distinct and none may be a `to`. */
+int sx_transfer_forces(sx_ctx *ctx, int count, const int *from1, const int *to1);
+/* Composite bodies held on the device for repeated use: satellite t sits at
+ * offsets[3t..] from centre centre1[t].  sx_composite_place runs the
+ * copy_transform of every satellite, sx_composite_transfer the
+ * transfer_forces of every satellite onto its centre; sx_run calls them
+ * after the initial step and after the pair loops and bonded terms. */
+int sx_set_composites(sx_ctx *ctx, int count, const int *centre1, const int *sat1, const double *offsets);
+int sx_composite_place(sx_ctx *ctx);
+int sx_composite_transfer(sx_ctx *ctx);
+
+/* ------------------------------------------------------------------ */
+/* reaction field of the point dipoles (module dipole, force.f90:2669-2767) */
+/* ------------------------------------------------------------------ */
+/* init_reaction_field (:2677): the dipole table DIP (3 x n) and the reaction
+ * field RFL set to zero.  sx_forces(SX_DIPOLE) then does what the reference's
+ * pair loop over dipole_calculate_forces does besides forces and torques: a
+ * pair (i < j) within the cutoff sets DIP(i) to particle i's scaled axis and
+ * adds 2(eps-1)/(2 eps+1)/rc^3 times particle j's scaled axis to the field of
+ * EVERY particle (the whole-array update, :2745-2747), so the field is one
+ * vector shared by all particles, accumulated over calls until the next
+ * sx_init_reaction_field.  sx_pair_eval has no such side effects. */
+int sx_init_reaction_field(sx_ctx *ctx);
+/* eval_reaction_field (:2753-2766): for each charged particle pot_eng +=
+ * DIP.RFL / 2 and T -= DIP x RFL */
+int sx_eval_reaction_field(sx_ctx *ctx);
+/* dip[3*n] (x then y then z rows), rfl[3] */
+int sx_get_reaction_field(sx_ctx *ctx, double *dip, double rfl[3]);
 
 /* ------------------------------------------------------------------ */
 /* Monte Carlo barostat (module monte_carlo_barostat, integrators.f90:859-947) */

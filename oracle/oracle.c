@@ -502,6 +502,189 @@ void or_dh(const or_sys *s, int i, int j, double cutoff, double db_len, double d
 }
 
 /* ------------------------------------------------------------------ */
+/* composite rigid bodies (system.f90:743-881)                          */
+/* ------------------------------------------------------------------ */
+static void inertia_from_shape(or_sys *s, int i)
+{
+    s->ixx[i] = (pow(s->sig_y[i], 2.0) + pow(s->sig_z[i], 2.0)) / 20.0;
+    s->iyy[i] = (pow(s->sig_x[i], 2.0) + pow(s->sig_z[i], 2.0)) / 20.0;
+    s->izz[i] = (pow(s->sig_x[i], 2.0) + pow(s->sig_y[i], 2.0)) / 20.0;
+}
+
+static void inertia_add(or_sys *s, int from, int to, double xo, double yo, double zo)
+{
+    s->ixx[from] = s->ixx[from] + s->ixx[to] + (pow(yo, 2.0) + pow(zo, 2.0));
+    s->iyy[from] = s->iyy[from] + s->iyy[to] + (pow(xo, 2.0) + pow(zo, 2.0));
+    s->izz[from] = s->izz[from] + s->izz[to] + (pow(xo, 2.0) + pow(yo, 2.0));
+}
+
+/* copy_transform (system.f90:743-842): places `to` at `from` plus the offset
+   rotated by from's orientation and gives it from's quaternion; set_inertia
+   -1 (absent), 0 or 1.  Returns 0, or 1/2/3 for the three stops (second
+   particle without shape, first particle without shape, second particle
+   with mass) */
+int or_copy_transform(or_sys *s, int from, int to, double xo, double yo, double zo, int set_inertia)
+{
+    double w = s->qw[from], x = s->qx[from], y = s->qy[from], z = s->qz[from];
+    double rot[3][3];
+    rot[0][0] = w * w + x * x - y * y - z * z;
+    rot[0][1] = 2.0 * (x * y - w * z);
+    rot[0][2] = 2.0 * (x * z + w * y);
+    rot[1][0] = 2.0 * (x * y + w * z);
+    rot[1][1] = w * w - x * x + y * y - z * z;
+    rot[1][2] = 2.0 * (y * z - w * x);
+    rot[2][0] = 2.0 * (x * z - w * y);
+    rot[2][1] = 2.0 * (y * z + w * x);
+    rot[2][2] = w * w - x * x - y * y + z * z;
+    double ro[3];
+    for (int k = 0; k < 3; ++k) ro[k] = rot[k][0] * xo + rot[k][1] * yo + rot[k][2] * zo;
+    s->rx[to] = s->rx[from] + ro[0];
+    s->ry[to] = s->ry[from] + ro[1];
+    s->rz[to] = s->rz[from] + ro[2];
+    s->qw[to] = s->qw[from];
+    s->qx[to] = s->qx[from];
+    s->qy[to] = s->qy[from];
+    s->qz[to] = s->qz[from];
+    if (set_inertia != 1) return 0;
+    if (!(s->mass[to] <= 0.0)) return 3;
+    int to_has_i = s->ixx[to] > 0.0 && s->iyy[to] > 0.0 && s->izz[to] > 0.0;
+    int to_has_shape = s->sig_x[to] > 0.0 && s->sig_y[to] > 0.0 && s->sig_z[to] > 0.0;
+    if (s->ixx[from] > 0.0 && s->iyy[from] > 0.0 && s->izz[from] > 0.0) {
+    } else if (s->sig_x[from] > 0.0 && s->sig_y[from] > 0.0 && s->sig_z[from] > 0.0) {
+        inertia_from_shape(s, from);
+    } else {
+        return 2;
+    }
+    if (to_has_i) {
+    } else if (to_has_shape) {
+        inertia_from_shape(s, to);
+    } else {
+        return 1;
+    }
+    inertia_add(s, from, to, xo, yo, zo);
+    return 0;
+}
+
+/* transfer_forces (system.f90:844-881): from's force and torque, plus the
+   torque of its force about `to`, move to `to` */
+void or_transfer_forces(or_sys *s, int from, int to)
+{
+    double r[3] = {s->rx[from] - s->rx[to], s->ry[from] - s->ry[to], s->rz[from] - s->rz[to]};
+    if (s->periodic)
+        for (int k = 0; k < 3; ++k) r[k] = mimg(r[k], s->box[k]);
+    double f[3] = {s->fx[from], s->fy[from], s->fz[from]};
+    double t[3];
+    t[0] = r[1] * f[2] - r[2] * f[1];
+    t[1] = r[2] * f[0] - r[0] * f[2];
+    t[2] = r[0] * f[1] - r[1] * f[0];
+    s->fx[to] = s->fx[to] + s->fx[from];
+    s->fy[to] = s->fy[to] + s->fy[from];
+    s->fz[to] = s->fz[to] + s->fz[from];
+    s->tx[to] = s->tx[to] + s->tx[from] + t[0];
+    s->ty[to] = s->ty[to] + s->ty[from] + t[1];
+    s->tz[to] = s->tz[to] + s->tz[from] + t[2];
+    s->fx[from] = 0.0; s->fy[from] = 0.0; s->fz[from] = 0.0;
+    s->tx[from] = 0.0; s->ty[from] = 0.0; s->tz[from] = 0.0;
+}
+
+/* ------------------------------------------------------------------ */
+/* point dipoles with a reaction field (module dipole, force.f90:2669-2767) */
+/* ------------------------------------------------------------------ */
+/* the body z axis scaled by the particle's charge (force.f90:2709-2718) */
+static void dip_axis(const or_sys *s, int i, double u[3])
+{
+    double w = s->qw[i], x = s->qx[i], y = s->qy[i], z = s->qz[i];
+    u[0] = 2.0 * (x * z + w * y);
+    u[1] = 2.0 * (y * z - w * x);
+    u[2] = w * w - x * x - y * y + z * z;
+    for (int k = 0; k < 3; ++k) u[k] = u[k] * s->charge[i];
+}
+
+/* dipole_calculate_forces (force.f90:2687-2751); dip and rfl are the
+   module's DIPX..DIPZ and RFLX..RFLZ (3 arrays of n each).  A pair within
+   the cutoff sets particle i's dipole and adds particle j's scaled dipole to
+   the reaction field of EVERY particle (the whole-array update :2745-2747). */
+void or_dipole(const or_sys *s, int i, int j, double cutoff, double dielectric, double *pe, double f[3],
+               double ti[3], double tj[3], double *dip, double *rfl)
+{
+    int n = s->n;
+    *pe = 0.0;
+    for (int k = 0; k < 3; ++k) f[k] = ti[k] = tj[k] = 0.0;
+    double dip_coeff = (2.0 * (dielectric - 1.0)) / (2.0 * dielectric + 1.0);
+    dip_coeff = dip_coeff / (cutoff * cutoff * cutoff);
+    double r[3];
+    r[0] = s->rx[i] - s->rx[j];
+    r[1] = s->ry[i] - s->ry[j];
+    r[2] = s->rz[i] - s->rz[j];
+    if (s->periodic)
+        for (int k = 0; k < 3; ++k) r[k] = mimg(r[k], s->box[k]);
+    double rij_sq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    double rij_mag = sqrt(rij_sq);
+    double u1[3], u2[3];
+    dip_axis(s, i, u1);
+    dip_axis(s, j, u2);
+    if (!(rij_mag < cutoff)) return;
+    double ci = u1[0] * r[0] + u1[1] * r[1] + u1[2] * r[2];
+    double cj = u2[0] * r[0] + u2[1] * r[1] + u2[2] * r[2];
+    double cij = u1[0] * u2[0] + u1[1] * u2[1] + u1[2] * u2[2];
+    double t1[3], t2[3], t3[3];
+    for (int k = 0; k < 3; ++k) {
+        t1[k] = (cij - (5.0 * ci * cj / rij_sq)) * r[k] + cj * u1[k] + ci * u2[k];
+        t2[k] = 3.0 * cj * (r[k] / rij_sq) - u2[k];
+        t3[k] = 3.0 * ci * (r[k] / rij_sq) - u1[k];
+    }
+    double rij_cube = rij_sq * rij_mag;
+    double rij_pow5 = rij_cube * rij_sq;
+    *pe = (cij / rij_cube) - ((3.0 * ci * cj) / rij_pow5);
+    double c1[3], c2[3];
+    cross(u1, t2, c1);
+    cross(u2, t3, c2);
+    for (int k = 0; k < 3; ++k) {
+        f[k] = (3.0 / rij_pow5) * t1[k];
+        ti[k] = (1.0 / rij_cube) * c1[k];
+        tj[k] = (1.0 / rij_cube) * c2[k];
+    }
+    for (int k = 0; k < 3; ++k) dip[k * n + i] = u1[k];
+    for (int k = 0; k < 3; ++k) {
+        double add = u2[k] * dip_coeff;
+        for (int q = 0; q < n; ++q) rfl[k * n + q] = rfl[k * n + q] + add;
+    }
+}
+
+/* eval_reaction_field (force.f90:2753-2766) */
+void or_eval_reaction_field(or_sys *s, const double *dip, const double *rfl)
+{
+    int n = s->n;
+    for (int i = 0; i < n; ++i) {
+        if (s->charge[i] != 0.0) {
+            double dx = dip[i], dy = dip[n + i], dz = dip[2 * n + i];
+            double rx = rfl[i], ry = rfl[n + i], rz = rfl[2 * n + i];
+            s->pot_eng = s->pot_eng + 0.5 * (dx * rx + dy * ry + dz * rz);
+            s->tx[i] = s->tx[i] - (dy * rz - dz * ry);
+            s->ty[i] = s->ty[i] - (dz * rx - dx * rz);
+            s->tz[i] = s->tz[i] - (dx * ry - dy * rx);
+        }
+    }
+}
+
+/* the caller's pair loop (README.md:86-103) over dipole_calculate_forces */
+void or_dipole_loop(or_sys *s, const or_nlist *nl, double cutoff, double dielectric, double *dip, double *rfl)
+{
+    for (int i = 0; i < s->n - 1; ++i) {
+        for (long k = nl->offset[i]; k < nl->offset[i + 1]; ++k) {
+            int j = nl->nbr[k];
+            double pe, f[3], ti[3], tj[3];
+            or_dipole(s, i, j, cutoff, dielectric, &pe, f, ti, tj, dip, rfl);
+            s->pot_eng = s->pot_eng + pe;
+            s->fx[i] = s->fx[i] + f[0]; s->fy[i] = s->fy[i] + f[1]; s->fz[i] = s->fz[i] + f[2];
+            s->fx[j] = s->fx[j] - f[0]; s->fy[j] = s->fy[j] - f[1]; s->fz[j] = s->fz[j] - f[2];
+            s->tx[i] = s->tx[i] + ti[0]; s->ty[i] = s->ty[i] + ti[1]; s->tz[i] = s->tz[i] + ti[2];
+            s->tx[j] = s->tx[j] + tj[0]; s->ty[j] = s->ty[j] + tj[1]; s->tz[j] = s->tz[j] + tj[2];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* Gay-Berne (force.f90:237-533)                                       */
 /* ------------------------------------------------------------------ */
 typedef struct { double ru1, ru2, uu; } gb_par;

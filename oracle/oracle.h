@@ -89,6 +89,12 @@ void or_morse(const or_sys *s, int i, int j, double dissoc, double width, double
               double cutoff, double *pe, double f[3]);
 void or_dh(const or_sys *s, int i, int j, double cutoff, double db_len, double dielectric,
            double *pe, double f[3]);
+int or_copy_transform(or_sys *s, int from, int to, double xo, double yo, double zo, int set_inertia);
+void or_transfer_forces(or_sys *s, int from, int to);
+void or_dipole(const or_sys *s, int i, int j, double cutoff, double dielectric, double *pe, double f[3],
+               double ti[3], double tj[3], double *dip, double *rfl);
+void or_eval_reaction_field(or_sys *s, const double *dip, const double *rfl);
+void or_dipole_loop(or_sys *s, const or_nlist *nl, double cutoff, double dielectric, double *dip, double *rfl);
 /* Gay-Berne; mu, nu are the exponents the reference fixes to (1, 2)
  * (force.f90:240).  variant: 0 gb (cutoff + smooth switch), 1 gbr (repulsive). */
 /* variant 0 gb_, 1 gbr_, 2 gbw_, 3 gbc_, 4 gbwc_ */

@@ -39,7 +39,7 @@ SCALARS = [
 SIDX = {s: k for k, s in enumerate(SCALARS)}
 
 # force kinds (include/sucrose.h SX_KIND_*)
-LJ, LJ12, LJW, WCA, MORSE, GB, GBR, ECP, ECPR, DH, GBW, GBC, GBWC, ECPW, ECPC, ECPWC = range(1, 17)
+LJ, LJ12, LJW, WCA, MORSE, GB, GBR, ECP, ECPR, DH, GBW, GBC, GBWC, ECPW, ECPC, ECPWC, DIPOLE = range(1, 18)
 # integrator kinds used by ref_integrate
 VV, VVC, QQ, NHT, NHTC, LGV, LGVC = range(1, 8)
 
@@ -357,6 +357,43 @@ class Oracle:
         o = self._nl(st, nl)
         return bool(self.lib.or_nlist_needs_rebuild(C.byref(s), C.byref(o)))
 
+    # -- composite rigid bodies --
+    def copy_transform(self, st: State, frm, to, offset, set_inertia=None):
+        s = self._sys(st)
+        d = C.c_double
+        si = -1 if set_inertia is None else int(bool(set_inertia))
+        r = self.lib.or_copy_transform(C.byref(s), int(frm), int(to), d(offset[0]), d(offset[1]), d(offset[2]), si)
+        self._back(s, st)
+        return r
+
+    def transfer_forces(self, st: State, frm, to):
+        s = self._sys(st)
+        self.lib.or_transfer_forces(C.byref(s), int(frm), int(to))
+        self._back(s, st)
+
+    # -- point dipoles with a reaction field (module dipole) --
+    def dipole_pair(self, st: State, i, j, cutoff, dielectric, dip, rfl):
+        """dipole_calculate_forces for one pair; dip and rfl are (3, n)
+        arrays updated in place (the module's DIP* and RFL*)."""
+        s = self._sys(st)
+        pe = C.c_double()
+        f = np.zeros(3); ti = np.zeros(3); tj = np.zeros(3)
+        self.lib.or_dipole(C.byref(s), int(i), int(j), C.c_double(cutoff), C.c_double(dielectric), C.byref(pe),
+                           _dp(f), _dp(ti), _dp(tj), _dp(dip), _dp(rfl))
+        return pe.value, f, ti, tj
+
+    def dipole_loop(self, st: State, nl: dict, cutoff, dielectric, dip, rfl):
+        s = self._sys(st)
+        nls = self._nl(st, nl)
+        self.lib.or_dipole_loop(C.byref(s), C.byref(nls), C.c_double(cutoff), C.c_double(dielectric), _dp(dip),
+                                _dp(rfl))
+        self._back(s, st)
+
+    def eval_reaction_field(self, st: State, dip, rfl):
+        s = self._sys(st)
+        self.lib.or_eval_reaction_field(C.byref(s), _dp(dip), _dp(rfl))
+        self._back(s, st)
+
     # -- Monte Carlo barostat --
     def mc_barostat(self, st: State, potential, target_pres, u, volume_change=None):
         """mc_barostat (integrators.f90:864-945) step by step, with the
@@ -532,6 +569,34 @@ class Reference:
         self.lib.ref_get_box(_dp(box))
         return n.value, nc.value, cfg.value, np.ascontiguousarray(buf), rt, (bi[:nc.value], bj[:nc.value],
                                                                              bl[:nc.value]), box
+
+    def copy_transform(self, frm, to, offset, set_inertia=None):
+        d = C.c_double
+        si = -1 if set_inertia is None else int(bool(set_inertia))
+        self.lib.ref_copy_transform(C.c_int(frm + 1), C.c_int(to + 1), d(offset[0]), d(offset[1]), d(offset[2]),
+                                    C.c_int(si))
+
+    def transfer_forces(self, frm, to):
+        self.lib.ref_transfer_forces(C.c_int(frm + 1), C.c_int(to + 1))
+
+    def init_reaction_field(self):
+        self.lib.ref_init_reaction_field()
+
+    def dipole_pair(self, i, j, cutoff, dielectric):
+        pe = C.c_double()
+        f = np.zeros(3); ti = np.zeros(3); tj = np.zeros(3)
+        self.lib.ref_dipole_pair(C.c_int(i + 1), C.c_int(j + 1), C.c_double(cutoff), C.c_double(dielectric),
+                                 C.byref(pe), _dp(f), _dp(ti), _dp(tj))
+        return pe.value, f, ti, tj
+
+    def eval_reaction_field(self):
+        self.lib.ref_eval_reaction_field()
+
+    def get_dipole(self):
+        dip = np.zeros((3, self.n)); rfl = np.zeros((3, self.n))
+        d = np.zeros((self.n, 3), order="F"); r = np.zeros((self.n, 3), order="F")
+        self.lib.ref_get_dipole(_dp(d), _dp(r))
+        return np.ascontiguousarray(d.T), np.ascontiguousarray(r.T)
 
     def mc_barostat(self, seed, target_pres, volume_change, sigma, eps, rc):
         """mc_barostat with the harness's all-pairs Lennard-Jones potential;

@@ -23,6 +23,7 @@ module ref_harness
     use nose_hoover
     use langevin
     use monte_carlo_barostat
+    use dipole
     use reporters
     implicit none
 
@@ -536,5 +537,44 @@ contains
         call mc_barostat(harness_potential, accept, volume_change)
         acc = merge(1, 0, accept)
     end subroutine ref_mc_barostat
+
+    ! ---- module dipole (force.f90:2669-2767) ----
+    subroutine ref_init_reaction_field() bind(C, name="ref_init_reaction_field")
+        if (allocated(DIPX)) deallocate(DIPX, DIPY, DIPZ, RFLX, RFLY, RFLZ)
+        call init_reaction_field
+    end subroutine ref_init_reaction_field
+
+    subroutine ref_dipole_pair(i1, j1, cutoff, dielectric, pe, f, ti, tj) bind(C, name="ref_dipole_pair")
+        integer(c_int), value :: i1, j1
+        real(c_double), value :: cutoff, dielectric
+        real(c_double), intent(out) :: pe, f(3), ti(3), tj(3)
+        call dipole_calculate_forces(i1, j1, cutoff, dielectric, pe, f, ti, tj)
+    end subroutine ref_dipole_pair
+
+    subroutine ref_eval_reaction_field() bind(C, name="ref_eval_reaction_field")
+        call eval_reaction_field
+    end subroutine ref_eval_reaction_field
+
+    subroutine ref_get_dipole(dip, rfl) bind(C, name="ref_get_dipole")
+        real(c_double), intent(out) :: dip(N, 3), rfl(N, 3)
+        dip(:, 1) = DIPX; dip(:, 2) = DIPY; dip(:, 3) = DIPZ
+        rfl(:, 1) = RFLX; rfl(:, 2) = RFLY; rfl(:, 3) = RFLZ
+    end subroutine ref_get_dipole
+
+    ! ---- composite rigid bodies (system.f90:743-881) ----
+    subroutine ref_copy_transform(from1, to1, xo, yo, zo, set_inertia) bind(C, name="ref_copy_transform")
+        integer(c_int), value :: from1, to1, set_inertia
+        real(c_double), value :: xo, yo, zo
+        if (set_inertia < 0) then
+            call copy_transform(from1, to1, xo, yo, zo)
+        else
+            call copy_transform(from1, to1, xo, yo, zo, set_inertia /= 0)
+        endif
+    end subroutine ref_copy_transform
+
+    subroutine ref_transfer_forces(from1, to1) bind(C, name="ref_transfer_forces")
+        integer(c_int), value :: from1, to1
+        call transfer_forces(from1, to1)
+    end subroutine ref_transfer_forces
 
 end module ref_harness

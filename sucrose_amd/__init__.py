@@ -29,7 +29,7 @@ SCALARS = ["POT_ENG", "KIN_ENG", "KIN_ROT", "TEMP", "PRESSURE", "DENSITY", "VIRI
            "VIRIAL_BONDED", "TARGET_TEMP", "DOF", "HB0", "HB1", "HB2", "ETA0", "ETA1", "ETA2",
            "PETA0", "PETA1", "PETA2"]
 SCALAR = {s: k for k, s in enumerate(SCALARS)}
-LJ, LJ12, LJW, WCA, MORSE, GB, GBR, ECP, ECPR, DH, GBW, GBC, GBWC, ECPW, ECPC, ECPWC = range(1, 17)
+LJ, LJ12, LJW, WCA, MORSE, GB, GBR, ECP, ECPR, DH, GBW, GBC, GBWC, ECPW, ECPC, ECPWC, DIPOLE = range(1, 18)
 ECP_BRENT_REFERENCE, ECP_CONVERGED, ECP_LEARNED = 0, 1, 2
 REBUILD_REFERENCE, REBUILD_HALF_SKIN = 0, 1
 INT_VV, INT_NHT, INT_LGV, INT_LGVC = 1, 4, 6, 7
@@ -133,6 +133,12 @@ def lib():
             "sx_batch_set_shape": [vp, C.c_int],
             "sx_set_atom_types": [vp, _PI], "sx_get_atom_types": [vp, _PI],
             "sx_set_target_pressure": [vp, C.c_double],
+            "sx_copy_transform": [vp, C.c_int, _PI, _PI, _PD, C.c_int],
+            "sx_transfer_forces": [vp, C.c_int, _PI, _PI],
+            "sx_set_composites": [vp, C.c_int, _PI, _PI, _PD],
+            "sx_composite_place": [vp], "sx_composite_transfer": [vp],
+            "sx_init_reaction_field": [vp], "sx_eval_reaction_field": [vp],
+            "sx_get_reaction_field": [vp, _PD, _PD],
             "sx_mc_barostat": [vp, C.POINTER(StepSpec), C.c_double, _PD, _PI],
             "sx_reporter_open": [vp, C.c_int, C.c_char_p, C.POINTER(C.c_void_p)],
             "sx_report": [vp, C.c_int],
@@ -528,6 +534,45 @@ class System:
 
     def state(self, fields=FIELDS):
         return {f: self.download(f) for f in fields}
+
+    # ---- composite rigid bodies (system.f90:743-881) ----
+    def copy_transform(self, from1, to1, offsets, set_inertia=None):
+        a = np.ascontiguousarray(np.atleast_1d(from1), dtype=np.int32)
+        b = np.ascontiguousarray(np.atleast_1d(to1), dtype=np.int32)
+        off = np.ascontiguousarray(np.reshape(offsets, (-1, 3)), dtype=np.float64)
+        si = -1 if set_inertia is None else int(bool(set_inertia))
+        self._ok(self.L.sx_copy_transform(self.h, len(a), a.ctypes.data_as(_PI), b.ctypes.data_as(_PI), _dp(off), si))
+
+    def transfer_forces(self, from1, to1):
+        a = np.ascontiguousarray(np.atleast_1d(from1), dtype=np.int32)
+        b = np.ascontiguousarray(np.atleast_1d(to1), dtype=np.int32)
+        self._ok(self.L.sx_transfer_forces(self.h, len(a), a.ctypes.data_as(_PI), b.ctypes.data_as(_PI)))
+
+    def set_composites(self, centre1, sat1, offsets):
+        a = np.ascontiguousarray(centre1, dtype=np.int32)
+        b = np.ascontiguousarray(sat1, dtype=np.int32)
+        off = np.ascontiguousarray(np.reshape(offsets, (-1, 3)), dtype=np.float64)
+        self._ok(self.L.sx_set_composites(self.h, len(a), a.ctypes.data_as(_PI), b.ctypes.data_as(_PI), _dp(off)))
+
+    def composite_place(self):
+        self._ok(self.L.sx_composite_place(self.h))
+
+    def composite_transfer(self):
+        self._ok(self.L.sx_composite_transfer(self.h))
+
+    # ---- reaction field of the point dipoles (module dipole) ----
+    def init_reaction_field(self):
+        self._ok(self.L.sx_init_reaction_field(self.h))
+
+    def eval_reaction_field(self):
+        self._ok(self.L.sx_eval_reaction_field(self.h))
+
+    def get_reaction_field(self):
+        """(DIP as a (3, n) array, RFL as a 3-vector shared by all particles)."""
+        dip = np.zeros(3 * self.n)
+        rfl = np.zeros(3)
+        self._ok(self.L.sx_get_reaction_field(self.h, _dp(dip), _dp(rfl)))
+        return dip.reshape(3, self.n), rfl
 
     # ---- Monte Carlo barostat ----
     def set_target_pressure(self, p):

@@ -19,7 +19,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["sx_core.hip", "sx_nlist.hip", "sx_pair.hip", "sx_integrate.hip", "sx_run.hip", "sx_mlp.hip",
-           "sx_batch.hip", "sx_report.hip", "sx_workloads.cpp"]
+           "sx_batch.hip", "sx_report.hip", "sx_bodies.hip",
+           "sx_workloads.cpp"]
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -90,6 +90,16 @@ struct Molecules {
     int max_atoms = 0;
 };
 
+// composite bodies registered for the step driver: placements (centre ->
+// satellite, offsets) and the transfer tables (each centre's satellites in
+// the order given)
+struct Composites {
+    int count = 0, ncentres = 0;
+    int *centre = nullptr, *sat = nullptr;
+    double *off = nullptr;
+    int *t_centres = nullptr, *t_start = nullptr, *t_sats = nullptr;
+};
+
 struct sx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -108,6 +118,10 @@ struct sx_ctx {
     double *d_scal = nullptr;    // SX_NSCALARS doubles (device copy of the scalars)
     double h_scal[SX_NSCALARS] = {0};
     double target_pres = 0.0;    // set_target_pressure (system.f90:224)
+    Composites comp;
+    double *d_dip = nullptr;     // dipole table DIP (3 * n) of module dipole
+    double *d_rfl = nullptr;     // its reaction field (one vector, shared by all particles)
+    int dip_cap = 0;
     bool scal_dirty_host = false; // host copy newer than device
     double *d_part = nullptr;    // reduction partials
     int part_cap = 0;
@@ -204,6 +218,10 @@ void sx_sync_scalars_to_host(sx_ctx *c);
 void sx_nlist_build_impl(sx_ctx *c, bool conditional);
 void sx_nlist_flag_impl(sx_ctx *c);
 void sx_nlist_rebox_impl(sx_ctx *c);
+void sx_eval_reaction_field_impl(sx_ctx *c);
+void sx_composite_place_impl(sx_ctx *c);
+void sx_composite_transfer_impl(sx_ctx *c);
+void sx_composite_free(sx_ctx *c);
 void sx_nlist_want_half(sx_ctx *c);
 void sx_forces_impl(sx_ctx *c, int kind, const double *p, int np);
 void sx_bonded_impl(sx_ctx *c);

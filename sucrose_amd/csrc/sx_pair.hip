@@ -343,6 +343,7 @@ static void check_params(int kind, int np)
     case SX_GBWC: need = 5; break;
     case SX_ECPW: case SX_ECPC: need = 2; break;
     case SX_ECPWC: need = 3; break;
+    case SX_DIPOLE: need = 2; break;   // cutoff, dielectric
     default: throw sx_error{SX_ERR_ARG, "unknown force kind"};
     }
     if (np < need) throw sx_error{SX_ERR_ARG, "too few parameters for this force kind"};
@@ -598,6 +599,83 @@ void sx_ecp_prepare(sx_ctx *c, int kind, const double *p, int np)
     ecp_stage(c, 1, kind, sc, L);
 }
 
+// ---------------------------------------------------------------------------
+// reaction field of the point dipoles (module dipole, force.f90:2669-2767)
+// ---------------------------------------------------------------------------
+// per particle i, over its list entries j > i within the cutoff (the pairs
+// the reference's loop hands to dipole_calculate_forces with i first): DIP(i)
+// = i's scaled axis when there is one; the field increments of those pairs
+// are summed per particle, then over particles by the last workgroup
+__global__ void __launch_bounds__(256) k_dipole_rf(DState s, const int *row_start, const int *nbr, double cutoff,
+                                                   double dip_coeff, double *dip, double *part, unsigned *ticket,
+                                                   double *rfl)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (i < s.n) {
+        bool any = false;
+        for (int k = row_start[i]; k < row_start[i + 1]; ++k) {
+            int j = nbr[k];
+            if (j <= i) continue;
+            double r[3], rsq, rmag;
+            if (!dip_in_range(s, i, j, cutoff, r, rsq, rmag)) continue;
+            any = true;
+            double u2[3];
+            dip_axis(s, j, u2);
+            for (int q = 0; q < 3; ++q) v[q] = v[q] + u2[q] * dip_coeff;
+        }
+        if (any) {
+            double u1[3];
+            dip_axis(s, i, u1);
+            for (int q = 0; q < 3; ++q) dip[(long)q * s.n + i] = u1[q];
+        }
+    }
+    SxRedIds ids;
+    ids.id[0] = 0; ids.id[1] = 1; ids.id[2] = 2; ids.n = 3; ids.accumulate = 1;
+    sx_reduce_last_block<3>(v, part, ticket, rfl, ids);
+}
+
+// eval_reaction_field (force.f90:2753-2766) per particle: the energy term
+// goes to the particle's energy slot
+__global__ void k_eval_rf(DState s, const double *dip, const double *rfl, double *pe_acc)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n || s.f[SX_CHARGE][i] == 0.0) return;
+    long n = s.n;
+    double dx = dip[i], dy = dip[n + i], dz = dip[2 * n + i];
+    double rx = rfl[0], ry = rfl[1], rz = rfl[2];
+    pe_acc[i] = pe_acc[i] + 0.5 * (dx * rx + dy * ry + dz * rz);
+    s.f[SX_TX][i] = s.f[SX_TX][i] - (dy * rz - dz * ry);
+    s.f[SX_TY][i] = s.f[SX_TY][i] - (dz * rx - dx * rz);
+    s.f[SX_TZ][i] = s.f[SX_TZ][i] - (dx * ry - dy * rx);
+}
+
+static void need_rf(sx_ctx *c)
+{
+    if (!c->d_dip || c->dip_cap != c->n)
+        throw sx_error{SX_ERR_STATE, "reaction field not initialised (sx_init_reaction_field)"};
+}
+
+static void dipole_side_effects(sx_ctx *c, const Params &P)
+{
+    need_rf(c);
+    double cutoff = P.p[0], dielectric = P.p[1];
+    double dip_coeff = (2.0 * (dielectric - 1.0)) / (2.0 * dielectric + 1.0);
+    dip_coeff = dip_coeff / (cutoff * cutoff * cutoff);
+    int nb = sx_blocks(c->n, 256);
+    sx_ensure_part(c, nb, 3);
+    hipLaunchKernelGGL(k_dipole_rf, dim3(nb), dim3(256), 0, c->stream, c->dstate(), c->nl.row_start, c->nl.nbr,
+                       cutoff, dip_coeff, c->d_dip, c->d_part, c->d_ticket, c->d_rfl);
+}
+
+void sx_eval_reaction_field_impl(sx_ctx *c)
+{
+    need_rf(c);
+    hipLaunchKernelGGL(k_eval_rf, dim3(sx_blocks(c->n, 256)), dim3(256), 0, c->stream, c->dstate(), c->d_dip,
+                       c->d_rfl, c->d_pe_acc);
+    if (!c->defer_pe) sx_pe_total_impl(c);
+}
+
 static void ecp_forces_impl(sx_ctx *c, int kind, const Params &P)
 {
     ensure_pair_buffers(c);
@@ -636,6 +714,10 @@ void sx_forces_impl(sx_ctx *c, int kind, const double *p, int np)
         case SX_GBW: launch_gather<SX_GBW, true>(c, P, lpp); break;
         case SX_GBC: launch_gather<SX_GBC, true>(c, P, lpp); break;
         case SX_GBWC: launch_gather<SX_GBWC, true>(c, P, lpp); break;
+        case SX_DIPOLE:
+            launch_gather<SX_DIPOLE, true>(c, P, lpp);
+            dipole_side_effects(c, P);
+            break;
         }
     }
     // the step driver folds this reduction into the final-step kernel
@@ -643,6 +725,40 @@ void sx_forces_impl(sx_ctx *c, int kind, const double *p, int np)
 }
 
 extern "C" {
+
+int sx_init_reaction_field(sx_ctx *c)
+{
+    return guard(c, [&] {
+        need_system(c);
+        if (c->dip_cap != c->n) {
+            if (c->d_dip) HIPCHK(hipFree(c->d_dip));
+            HIPCHK(hipMalloc(&c->d_dip, sizeof(double) * 3 * (size_t)c->n));
+            if (!c->d_rfl) HIPCHK(hipMalloc(&c->d_rfl, sizeof(double) * 3));
+            c->dip_cap = c->n;
+            c->have_graph = false;
+        }
+        HIPCHK(hipMemsetAsync(c->d_dip, 0, sizeof(double) * 3 * (size_t)c->n, c->stream));
+        HIPCHK(hipMemsetAsync(c->d_rfl, 0, sizeof(double) * 3, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int sx_eval_reaction_field(sx_ctx *c)
+{
+    return guard(c, [&] { need_system(c); sx_eval_reaction_field_impl(c); });
+}
+
+int sx_get_reaction_field(sx_ctx *c, double *dip, double rfl[3])
+{
+    return guard(c, [&] {
+        need_system(c);
+        need_rf(c);
+        if (dip)
+            HIPCHK(hipMemcpyAsync(dip, c->d_dip, sizeof(double) * 3 * (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
+        if (rfl) HIPCHK(hipMemcpyAsync(rfl, c->d_rfl, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
 
 int sx_forces(sx_ctx *c, int kind, const double *params, int nparams)
 {
@@ -771,6 +887,7 @@ int sx_pair_eval(sx_ctx *c, int kind, const double *params, int nparams, long np
         case SX_ECPW: hipLaunchKernelGGL(k_pair_eval<SX_ECPW>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
         case SX_ECPC: hipLaunchKernelGGL(k_pair_eval<SX_ECPC>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
         case SX_ECPWC: hipLaunchKernelGGL(k_pair_eval<SX_ECPWC>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
+        case SX_DIPOLE: hipLaunchKernelGGL(k_pair_eval<SX_DIPOLE>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
         }
         std::vector<double> h(11 * np);
         HIPCHK(hipMemcpyAsync(h.data(), dout, sizeof(double) * 11 * np, hipMemcpyDeviceToHost, c->stream));

@@ -392,6 +392,63 @@ __device__ __forceinline__ void d_gb(const DState &s, int a, int b, const double
     gb_core<VARIANT>(r, A, u2, p[0], mu, nu, o, wf, chir);
 }
 
+// ---------------------------------------------------------------------------
+// point dipoles (dipole_calculate_forces, force.f90:2687-2751): the body z
+// axis of each particle scaled by its charge; the reaction-field side effects
+// of the reference routine are applied by a separate kernel
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void dip_axis(const DState &s, int i, double u[3])
+{
+    double w = s.f[SX_QW][i], x = s.f[SX_QX][i], y = s.f[SX_QY][i], z = s.f[SX_QZ][i];
+    u[0] = 2.0 * (x * z + w * y);
+    u[1] = 2.0 * (y * z - w * x);
+    u[2] = w * w - x * x - y * y + z * z;
+    double q = s.f[SX_CHARGE][i];
+    u[0] = u[0] * q; u[1] = u[1] * q; u[2] = u[2] * q;
+}
+
+// true when the pair lies within the cutoff (the test of force.f90:2720)
+__device__ __forceinline__ bool dip_in_range(const DState &s, int a, int b, double cutoff, double r[3],
+                                             double &rij_sq, double &rij_mag)
+{
+    pair_rij(s, a, b, r);
+    rij_sq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    rij_mag = sqrt(rij_sq);
+    return rij_mag < cutoff;
+}
+
+__device__ __forceinline__ void d_dipole(const DState &s, int a, int b, const double *p, PairOut &o)
+{
+    double cutoff = p[0];
+    double r[3], rij_sq, rij_mag;
+    o.pe = 0.0;
+    for (int k = 0; k < 3; ++k) o.f[k] = o.ti[k] = o.tj[k] = 0.0;
+    if (!dip_in_range(s, a, b, cutoff, r, rij_sq, rij_mag)) return;
+    double u1[3], u2[3];
+    dip_axis(s, a, u1);
+    dip_axis(s, b, u2);
+    double ci = u1[0] * r[0] + u1[1] * r[1] + u1[2] * r[2];
+    double cj = u2[0] * r[0] + u2[1] * r[1] + u2[2] * r[2];
+    double cij = u1[0] * u2[0] + u1[1] * u2[1] + u1[2] * u2[2];
+    double t1[3], t2[3], t3[3];
+    for (int k = 0; k < 3; ++k) {
+        t1[k] = (cij - (5.0 * ci * cj / rij_sq)) * r[k] + cj * u1[k] + ci * u2[k];
+        t2[k] = 3.0 * cj * (r[k] / rij_sq) - u2[k];
+        t3[k] = 3.0 * ci * (r[k] / rij_sq) - u1[k];
+    }
+    double rij_cube = rij_sq * rij_mag;
+    double rij_pow5 = rij_cube * rij_sq;
+    o.pe = (cij / rij_cube) - ((3.0 * ci * cj) / rij_pow5);
+    double c1[3], c2[3];
+    cross3(u1, t2, c1);
+    cross3(u2, t3, c2);
+    for (int k = 0; k < 3; ++k) {
+        o.f[k] = (3.0 / rij_pow5) * t1[k];
+        o.ti[k] = (1.0 / rij_cube) * c1[k];
+        o.tj[k] = (1.0 / rij_cube) * c2[k];
+    }
+}
+
 // isotropic dispatch
 template <int KIND>
 __device__ __forceinline__ void d_pair(const DState &s, int a, int b, const double *p, PairOut &o)
@@ -407,4 +464,5 @@ __device__ __forceinline__ void d_pair(const DState &s, int a, int b, const doub
     else if (KIND == SX_GBW) d_gb<2>(s, a, b, p, o);
     else if (KIND == SX_GBC) d_gb<3>(s, a, b, p, o);
     else if (KIND == SX_GBWC) d_gb<4>(s, a, b, p, o);
+    else if (KIND == SX_DIPOLE) d_dipole(s, a, b, p, o);
 }

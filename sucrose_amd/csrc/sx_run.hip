@@ -31,6 +31,7 @@ static int nparams_of(int kind)
     case SX_ECP: return 1;
     case SX_ECPW: case SX_ECPC: return 2;
     case SX_ECPWC: return 3;
+    case SX_DIPOLE: return 2;
     default: return 0;
     }
 }
@@ -43,7 +44,7 @@ static bool is_ecp(int kind)
 static void enqueue_step(sx_ctx *c, const sx_step_spec &sp)
 {
     DeferPe defer(c);   // pot_eng is reduced by the final-step kernel
-    if (sp.integrator == SX_INT_LGV) {
+    if (sp.integrator == SX_INT_LGV && c->comp.count == 0) {
         // initial step, rebuild decision, initialize_forces (and the ECP
         // per-particle records) in one kernel, then the conditional rebuild
         bool ecp = false;
@@ -58,14 +59,22 @@ static void enqueue_step(sx_ctx *c, const sx_step_spec &sp)
             sx_nht_impl(c, sp.dt, nullptr, 0, false);
             sx_qq_initial_impl(c, sp.dt);
             break;
+        case SX_INT_LGV: sx_lgv_initial_impl(c, sp.dt, 0, c->d_step, sp.damping, false); break;
         case SX_INT_LGVC: sx_lgv_initial_impl(c, sp.dt, 0, c->d_step, sp.damping, true); break;
         }
+        sx_composite_place_impl(c);   // satellites follow their centres
         sx_nlist_flag_impl(c);
         sx_nlist_build_impl(c, true);
         sx_init_forces_impl(c);
     }
-    for (int f = 0; f < sp.nforces; ++f) sx_forces_impl(c, sp.kinds[f], sp.params[f], nparams_of(sp.kinds[f]));
+    bool dipoles = false;
+    for (int f = 0; f < sp.nforces; ++f) {
+        sx_forces_impl(c, sp.kinds[f], sp.params[f], nparams_of(sp.kinds[f]));
+        dipoles = dipoles || sp.kinds[f] == SX_DIPOLE;
+    }
+    if (dipoles) sx_eval_reaction_field_impl(c);
     if (sp.bonded) sx_bonded_impl(c);
+    sx_composite_transfer_impl(c);   // satellite forces onto their centres
     // final step with calculate_thermo folded in (and the Langevin counter advance)
     switch (sp.integrator) {
     case SX_INT_VV: sx_vv_final_impl(c, sp.dt, true); break;

@@ -541,3 +541,119 @@ def test_mc_barostat(dev, oracle):
                 assert got[f] == before[f], (c, f)
     assert seen == {True, False}
     dev.set_box(9.0)
+
+
+def test_dipole_pairs_forces_and_reaction_field(dev, oracle):
+    """Point dipoles (module dipole): per-pair terms through sx_pair_eval,
+    then sx_forces over the list with its reaction-field side effects
+    (twice, so that the field accumulates as the reference's does) and
+    eval_reaction_field, against the oracle's loop over
+    dipole_calculate_forces and eval_reaction_field."""
+    from oracle.oracle import DIPOLE
+    say("dipoles: pair terms, forces, dipole table, reaction field")
+    rng = np.random.default_rng(91)
+    n, L, rc, eps_r = 1200, 14.0, 2.6, 78.5
+    st = base_state(rng, n, L, forces=False, charges=True)
+    st["charge"] = st["charge"] * rng.uniform(0.5, 1.5, n)
+    st["charge"][::5] = 0.0
+    to_device(dev, st)
+    i = rng.integers(0, n, 3000)
+    j = rng.integers(0, n, 3000)
+    m = i != j
+    i, j = i[m], j[m]
+    pe, f, ti, tj, _ = dev.pair_eval(DIPOLE, i + 1, j + 1, rc, eps_r)
+    dummy = (np.zeros((3, n)), np.zeros((3, n)))
+    for t in range(len(i)):
+        r = oracle.dipole_pair(st, i[t], j[t], rc, eps_r, *dummy)
+        for a, b in ((pe[t], r[0]), (f[t], r[1]), (ti[t], r[2]), (tj[t], r[3])):
+            assert np.abs(np.asarray(a) - np.asarray(b)).max() <= TOL * max(1.0, np.abs(b).max()), t
+    dev.init_nlist(0.3, rc)
+    dev.init_reaction_field()
+    nl = oracle.nlist_new(st, rc, 0.3)
+    oracle.nlist_build(st, nl)
+    ref = st.copy()
+    dip = np.zeros((3, n)); rfl = np.zeros((3, n))
+    for rep in range(2):
+        dev.initialize_forces()
+        dev.forces(DIPOLE, rc, eps_r)
+        oracle.call("or_initialize_forces", ref)
+        oracle.dipole_loop(ref, nl, rc, eps_r, dip, rfl)
+    gd, gr = dev.get_reaction_field()
+    assert np.array_equal(gd, dip)
+    assert np.all(rfl == rfl[:, :1])     # one field shared by every particle
+    scale = np.abs(dip).max() * 2 * (eps_r - 1) / (2 * eps_r + 1) / rc ** 3 * nl["nnz"]
+    assert np.abs(gr - rfl[:, 0]).max() <= 1e-12 * scale
+    dev.eval_reaction_field()
+    oracle.eval_reaction_field(ref, dip, rfl)
+    got = from_device(dev, st)
+    for fl in ("fx", "fy", "fz", "tx", "ty", "tz"):
+        assert rel_err(got[fl], ref[fl]) < TOL, fl
+    assert abs(got["pot_eng"] - ref["pot_eng"]) <= TOL * max(1.0, abs(ref["pot_eng"]))
+
+
+def test_composite_bodies(dev, oracle):
+    """copy_transform (with the inertia bookkeeping), transfer_forces and the
+    registered composites inside graph-replayed steps, against the oracle
+    called in the same order."""
+    import sucrose_amd as sx
+    from tests._states import lattice_lj
+    say("composite bodies: copy_transform, transfer_forces, sx_run with satellites")
+    rng = np.random.default_rng(95)
+    n = 300
+    st = base_state(rng, n, 12.0, sig=(1.0, 1.2, 1.6))
+    st["mass"][100:] = 0.0
+    st["ixx"][:40] = -1.0
+    to_device(dev, st)
+    ref = st.copy()
+    centres = np.repeat(np.arange(100), 2)
+    sats = np.arange(100, 300)
+    offs = rng.normal(size=(200, 3)) * 0.6
+    dev.copy_transform(centres + 1, sats + 1, offs, set_inertia=True)
+    for t in range(200):
+        assert oracle.copy_transform(ref, centres[t], sats[t], offs[t], True) == 0
+    got = from_device(dev, st)
+    for f in ("rx", "ry", "rz", "qw", "qx", "ixx", "iyy", "izz"):
+        assert np.array_equal(got[f], ref[f]), f
+    with pytest.raises(sx.SucroseError):   # second particle must have mass zero
+        dev.copy_transform([1], [2], [[0.1, 0.2, 0.3]], set_inertia=True)
+    assert oracle.copy_transform(ref, 0, 1, [0.1, 0.2, 0.3], True) == 3   # the placement precedes the stop
+    dev.transfer_forces(sats + 1, centres + 1)
+    for t in range(200):
+        oracle.transfer_forces(ref, sats[t], centres[t])
+    got = from_device(dev, st)
+    for f in ("fx", "fy", "fz", "tx", "ty", "tz"):
+        assert np.array_equal(got[f], ref[f]), f
+    # graph-replayed velocity-Verlet steps with the satellites registered
+    # (bodies of radius 0.9 at least 3.75 apart)
+    st = lattice_lj(rng, 512, 30.0, jitter=0.02)
+    cen = np.arange(0, 512, 4)
+    sat = np.setdiff1d(np.arange(512), cen)
+    cen_of = np.repeat(cen, 3)
+    off = np.tile([[0.0, 0.0, 0.9], [0.0, 0.9, 0.0], [0.9, 0.0, 0.0]], (128, 1))
+    to_device(dev, st)
+    dev.set_composites(cen_of + 1, sat + 1, off)
+    dev.composite_place()
+    dev.init_nlist(0.3, 2.5)
+    dev.initialize_forces()
+    dev.forces(LJ, 1.0, 1.0, 2.5)
+    dev.composite_transfer()
+    ref = from_device(dev, st)
+    nl = oracle.nlist_new(ref, 2.5, 0.3)
+    oracle.nlist_build(ref, nl)
+    dev.run(1, [(LJ, 1.0, 1.0, 2.5)], 10, 0.002)
+    for t in range(10):
+        oracle.integrate(VV, 0, ref, 0.002)
+        for k in range(len(sat)):
+            oracle.copy_transform(ref, cen_of[k], sat[k], off[k])
+        if oracle.needs_rebuild(ref, nl):
+            oracle.nlist_build(ref, nl)
+        oracle.call("or_initialize_forces", ref)
+        oracle.pair_loop(ref, nl, LJ, [1.0, 1.0, 2.5])
+        for k in range(len(sat)):
+            oracle.transfer_forces(ref, sat[k], cen_of[k])
+        oracle.integrate(VV, 1, ref, 0.002)
+        oracle.call("or_calculate_thermo", ref)
+    got = from_device(dev, st)
+    dev.set_composites([], [], np.zeros((0, 3)))
+    for f in ("rx", "ry", "vx", "qw", "fx", "tz"):
+        assert rel_err(got[f], ref[f]) < 1e-9, (f, rel_err(got[f], ref[f]))

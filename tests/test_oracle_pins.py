@@ -487,6 +487,59 @@ def test_bonded_closed_forms_and_gradients(oracle, kind, nat, coeff, angle, form
                 assert abs(fd - f[a][k]) <= 2e-5 * scale, (kind, a, k, fd, f[a][k])
 
 
+def test_dipole_forces_torques_are_minus_gradient(oracle):
+    """dipole_calculate_forces (force.f90:2687-2751): U = u1.u2/r^3 -
+    3 (u1.r)(u2.r)/r^5 for the charge-scaled body z axes; force = -grad U,
+    torques = minus the derivative under rotations of each particle; two
+    parallel dipoles on their common axis give U = -2 q1 q2 / r^3."""
+    rng = np.random.default_rng(31)
+    st = base_state(rng, 2, 30.0, charges=True)
+    st["charge"] = [1.3, -0.7]
+    dip = np.zeros((3, 2)); rfl = np.zeros((3, 2))
+    for _ in range(30):
+        d = rng.normal(size=3)
+        d *= rng.uniform(0.8, 2.3) / np.linalg.norm(d)
+        st["rx"], st["ry"], st["rz"] = [0, d[0]], [0, d[1]], [0, d[2]]
+        q = rand_quats(rng, 2)
+        for k, f in enumerate(["qw", "qx", "qy", "qz"]):
+            st[f] = q[:, k]
+        pe, f, ti, tj = oracle.dipole_pair(st, 0, 1, 2.5, 10.0, dip, rfl)
+        _fd_check(lambda s: oracle.dipole_pair(s, 0, 1, 2.5, 10.0, dip.copy(), rfl.copy())[0], st, True,
+                  force=f, torques=(ti, tj))
+    st["qw"], st["qx"], st["qy"], st["qz"] = [1, 1], [0, 0], [0, 0], [0, 0]
+    st["rx"], st["ry"], st["rz"] = [0, 0], [0, 0], [0, 1.7]
+    pe = oracle.dipole_pair(st, 0, 1, 2.5, 10.0, dip, rfl)[0]
+    assert abs(pe - (-2.0 * 1.3 * -0.7 / 1.7 ** 3)) < 1e-14
+
+
+def test_composite_bodies_invariants(oracle):
+    """copy_transform puts `to` at |offset| from `from` (a rotation), at the
+    offset itself for the identity quaternion and at R(q) offset in general
+    (R the body-to-space matrix of the quaternion); transfer_forces keeps the
+    total force and the total angular momentum source sum(r x F + T) (open
+    box) and leaves the satellite with none."""
+    rng = np.random.default_rng(8)
+    st = base_state(rng, 12, 40.0)
+    st.periodic = False
+    for t in range(6):
+        off = rng.normal(size=3)
+        assert oracle.copy_transform(st, t, 6 + t, off) == 0
+        d = np.array([st[f][6 + t] - st[f][t] for f in ("rx", "ry", "rz")])
+        w, x, y, z = (st[f][t] for f in ("qw", "qx", "qy", "qz"))
+        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                      [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                      [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+        assert np.allclose(d, R @ off, atol=1e-13) and abs(np.linalg.norm(d) - np.linalg.norm(off)) < 1e-13
+    pos = lambda: np.stack([st["rx"], st["ry"], st["rz"]], 1)
+    F = lambda: np.stack([st["fx"], st["fy"], st["fz"]], 1)
+    T = lambda: np.stack([st["tx"], st["ty"], st["tz"]], 1)
+    f0, l0 = F().sum(0), (np.cross(pos(), F()) + T()).sum(0)
+    for t in range(6):
+        oracle.transfer_forces(st, 6 + t, t % 3)
+    assert np.allclose(F().sum(0), f0, atol=1e-12) and np.allclose((np.cross(pos(), F()) + T()).sum(0), l0, atol=1e-11)
+    assert np.all(F()[6:] == 0) and np.all(T()[6:] == 0)
+
+
 def test_rattle_satisfies_constraints(oracle):
     """apply_constraints_a/b (system.f90:1060-1211): bond lengths to 2e-5 L^2
     after the position stage, r.v near zero after the velocity stage."""

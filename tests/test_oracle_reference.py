@@ -327,3 +327,70 @@ def test_mc_barostat_bit_exact(oracle):
             assert np.array_equal(mine[f], ref[f]), (seed, f)
         assert mine["pot_eng"] == ref["pot_eng"] and mine["density"] == ref["density"]
     assert seen == {True, False}
+
+
+def test_dipole_reaction_field_bit_exact(oracle, reference):
+    """dipole_calculate_forces, its side effects on the module's dipole and
+    reaction-field arrays (the whole-array update of every particle's field,
+    force.f90:2745-2747) and eval_reaction_field, bit for bit."""
+    from tests._states import base_state
+    rng = np.random.default_rng(77)
+    n = 60
+    st = base_state(rng, n, 6.0, charges=True)
+    st["charge"] = st["charge"] * rng.uniform(0.5, 1.5, n)
+    st["charge"][::7] = 0.0
+    reference.load(st)
+    reference.init_reaction_field()
+    dip = np.zeros((3, n)); rfl = np.zeros((3, n))
+    for t in range(600):
+        i, j = rng.integers(0, n, 2)
+        if i == j:
+            continue
+        a = oracle.dipole_pair(st, i, j, 2.4, 78.5, dip, rfl)
+        b = reference.dipole_pair(i, j, 2.4, 78.5)
+        assert a[0] == b[0] and all(np.array_equal(x, y) for x, y in zip(a[1:], b[1:])), t
+    d, r = reference.get_dipole()
+    assert np.array_equal(dip, d) and np.array_equal(rfl, r)
+    assert np.count_nonzero(dip[0]) > n // 2
+    ref = st.copy()
+    reference.eval_reaction_field()
+    reference.store(ref)
+    mine = st.copy()
+    oracle.eval_reaction_field(mine, dip, rfl)
+    assert mine["pot_eng"] == ref["pot_eng"]
+    for f in ("tx", "ty", "tz"):
+        assert np.array_equal(mine[f], ref[f]), f
+
+
+def test_composite_bodies_bit_exact(oracle, reference):
+    """copy_transform (with and without the inertia bookkeeping) and
+    transfer_forces (system.f90:743-881), bit for bit; satellites straddle
+    the periodic boundary."""
+    from tests._states import base_state
+    rng = np.random.default_rng(5)
+    n = 40
+    st = base_state(rng, n, 5.0, sig=(1.0, 1.5, 2.0))
+    st["mass"][20:] = 0.0
+    st["ixx"][:5] = -1.0
+    st["ixx"][25:30] = -1.0
+    st["rx"][:10] = 2.45
+    reference.load(st)
+    mine = st.copy()
+    ops = []
+    for t in range(20):
+        frm, to = t % 20, 20 + t
+        off = rng.normal(size=3) * 0.8
+        ops.append(("copy", frm, to, off, [None, False, True][t % 3]))
+    for t in range(20):
+        ops.append(("transfer", 20 + t, t % 7, None, None))
+    for op, frm, to, off, si in ops:
+        if op == "copy":
+            assert oracle.copy_transform(mine, frm, to, off, si) == 0
+            reference.copy_transform(frm, to, off, si)
+        else:
+            oracle.transfer_forces(mine, frm, to)
+            reference.transfer_forces(frm, to)
+    ref = st.copy()
+    reference.store(ref)
+    for f in ("rx", "ry", "rz", "qw", "qz", "ixx", "iyy", "izz", "fx", "fy", "fz", "tx", "ty", "tz"):
+        assert np.array_equal(mine[f], ref[f]), f
