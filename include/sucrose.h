@@ -81,10 +81,12 @@ enum sx_scalar {
  *   SX_DIPOLE cutoff, dielectric                    force.f90:2687 (point dipoles:
  *            the body z axis times the charge; sx_forces also applies the
  *            routine's reaction-field side effects, see sx_init_reaction_field)
+ *   SX_COULOMB cutoff, dielectric                   force.f90:2627 (switched over
+ *            the last 2 units before the cutoff)
  */
 enum sx_kind {
     SX_LJ = 1, SX_LJ12, SX_LJW, SX_WCA, SX_MORSE, SX_GB, SX_GBR, SX_ECP, SX_ECPR, SX_DH,
-    SX_GBW, SX_GBC, SX_GBWC, SX_ECPW, SX_ECPC, SX_ECPWC, SX_DIPOLE
+    SX_GBW, SX_GBC, SX_GBWC, SX_ECPW, SX_ECPC, SX_ECPWC, SX_DIPOLE, SX_COULOMB
 };
 
 /* ECP contact-parameter solver (sx_set_ecp_solver) */

@@ -501,6 +501,37 @@ void or_dh(const or_sys *s, int i, int j, double cutoff, double db_len, double d
     for (int k = 0; k < 3; ++k) f[k] = cmask ? (fac * r[k] / rij_sq) : 0.0;
 }
 
+/* coulombic_calculate_forces (force.f90:2627-2665): q_i q_j / (4 pi eps r),
+   switched off over the last 2 units before the cutoff; the switched force
+   subtracts sdx * PE * factor (PE already switched) from every component */
+void or_coulomb(const or_sys *s, int i, int j, double cutoff, double dielectric, double *pe, double f[3])
+{
+    *pe = 0.0;
+    f[0] = f[1] = f[2] = 0.0;
+    double r[3];
+    r[0] = s->rx[i] - s->rx[j];
+    r[1] = s->ry[i] - s->ry[j];
+    r[2] = s->rz[i] - s->rz[j];
+    if (s->periodic)
+        for (int k = 0; k < 3; ++k) r[k] = mimg(r[k], s->box[k]);
+    double rij_sq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    double rij_mag = sqrt(rij_sq);
+    double sw = cutoff - 2.0;
+    double factor = 1.0 / (cutoff - sw);
+    double xx = (rij_mag - sw) * factor;
+    double sx = 1.0 - 6.0 * xx * xx * xx * xx * xx + 15.0 * xx * xx * xx * xx - 10.0 * xx * xx * xx;
+    double sdx = -30.0 * xx * xx * xx * xx + 60.0 * xx * xx * xx - 30.0 * xx * xx;
+    if (rij_mag < cutoff) {
+        double p = s->charge[i] * s->charge[j] / (4.0 * PI_REF * dielectric * rij_mag);
+        for (int k = 0; k < 3; ++k) f[k] = p * r[k] / rij_sq;
+        if (rij_mag >= sw) {
+            p = sx * p;
+            for (int k = 0; k < 3; ++k) f[k] = sx * f[k] - sdx * p * factor;
+        }
+        *pe = p;
+    }
+}
+
 /* ------------------------------------------------------------------ */
 /* composite rigid bodies (system.f90:743-881)                          */
 /* ------------------------------------------------------------------ */
@@ -1873,6 +1904,7 @@ void or_pair_loop(or_sys *s, const or_nlist *nl, int kind, const double *p)
             case 14: ecp_eval(s, i, j, 2, p[0], p[1], 0.0, -1.0, &pe, f, ti, tj, NULL, NULL); torque = 1; break;
             case 15: ecp_eval(s, i, j, 3, p[0], 0.0, p[1], -1.0, &pe, f, ti, tj, NULL, NULL); torque = 1; break;
             case 16: ecp_eval(s, i, j, 4, p[0], p[1], p[2], -1.0, &pe, f, ti, tj, NULL, NULL); torque = 1; break;
+            case 18: or_coulomb(s, i, j, p[0], p[1], &pe, f); break;
             default: return;
             }
             s->pot_eng = s->pot_eng + pe;

@@ -89,6 +89,7 @@ void or_morse(const or_sys *s, int i, int j, double dissoc, double width, double
               double cutoff, double *pe, double f[3]);
 void or_dh(const or_sys *s, int i, int j, double cutoff, double db_len, double dielectric,
            double *pe, double f[3]);
+void or_coulomb(const or_sys *s, int i, int j, double cutoff, double dielectric, double *pe, double f[3]);
 int or_copy_transform(or_sys *s, int from, int to, double xo, double yo, double zo, int set_inertia);
 void or_transfer_forces(or_sys *s, int from, int to);
 void or_dipole(const or_sys *s, int i, int j, double cutoff, double dielectric, double *pe, double f[3],

@@ -39,7 +39,7 @@ SCALARS = [
 SIDX = {s: k for k, s in enumerate(SCALARS)}
 
 # force kinds (include/sucrose.h SX_KIND_*)
-LJ, LJ12, LJW, WCA, MORSE, GB, GBR, ECP, ECPR, DH, GBW, GBC, GBWC, ECPW, ECPC, ECPWC, DIPOLE = range(1, 18)
+LJ, LJ12, LJW, WCA, MORSE, GB, GBR, ECP, ECPR, DH, GBW, GBC, GBWC, ECPW, ECPC, ECPWC, DIPOLE, COULOMB = range(1, 19)
 # integrator kinds used by ref_integrate
 VV, VVC, QQ, NHT, NHTC, LGV, LGVC = range(1, 8)
 
@@ -221,6 +221,8 @@ class Oracle:
             L.or_morse(C.byref(s), i, j, d(p[0]), d(p[1]), d(p[2]), d(p[3]), C.byref(pe), fp)
         elif kind == DH:
             L.or_dh(C.byref(s), i, j, d(p[0]), d(p[1]), d(p[2]), C.byref(pe), fp)
+        elif kind == COULOMB:
+            L.or_coulomb(C.byref(s), i, j, d(p[0]), d(p[1]), C.byref(pe), fp)
         elif kind in (GB, GBR):
             cut = p[0] if kind == GB else 0.0
             mu = int(p[1]) if len(p) > 1 else 1

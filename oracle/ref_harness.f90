@@ -18,6 +18,7 @@ module ref_harness
     use ecp
     use bonded
     use debye_huckel
+    use coulumbic
     use verlet
     use quaternion
     use nose_hoover
@@ -236,6 +237,8 @@ contains
             call ecpc_calculate_forces(i1, j1, p(1), pe, f, ti, tj, p(2))
         case (16)   ! p = cutoff, wf, chirality
             call ecpwc_calculate_forces(i1, j1, p(1), pe, f, ti, tj, p(2), p(3))
+        case (18)   ! p = cutoff, dielectric
+            call coulombic_calculate_forces(i1, j1, p(1), p(2), pe, f)
         end select
     end subroutine ref_pair
 

@@ -29,7 +29,7 @@ SCALARS = ["POT_ENG", "KIN_ENG", "KIN_ROT", "TEMP", "PRESSURE", "DENSITY", "VIRI
            "VIRIAL_BONDED", "TARGET_TEMP", "DOF", "HB0", "HB1", "HB2", "ETA0", "ETA1", "ETA2",
            "PETA0", "PETA1", "PETA2"]
 SCALAR = {s: k for k, s in enumerate(SCALARS)}
-LJ, LJ12, LJW, WCA, MORSE, GB, GBR, ECP, ECPR, DH, GBW, GBC, GBWC, ECPW, ECPC, ECPWC, DIPOLE = range(1, 18)
+LJ, LJ12, LJW, WCA, MORSE, GB, GBR, ECP, ECPR, DH, GBW, GBC, GBWC, ECPW, ECPC, ECPWC, DIPOLE, COULOMB = range(1, 19)
 ECP_BRENT_REFERENCE, ECP_CONVERGED, ECP_LEARNED = 0, 1, 2
 REBUILD_REFERENCE, REBUILD_HALF_SKIN = 0, 1
 INT_VV, INT_NHT, INT_LGV, INT_LGVC = 1, 4, 6, 7

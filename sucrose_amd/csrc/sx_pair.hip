@@ -343,7 +343,7 @@ static void check_params(int kind, int np)
     case SX_GBWC: need = 5; break;
     case SX_ECPW: case SX_ECPC: need = 2; break;
     case SX_ECPWC: need = 3; break;
-    case SX_DIPOLE: need = 2; break;   // cutoff, dielectric
+    case SX_DIPOLE: case SX_COULOMB: need = 2; break;   // cutoff, dielectric
     default: throw sx_error{SX_ERR_ARG, "unknown force kind"};
     }
     if (np < need) throw sx_error{SX_ERR_ARG, "too few parameters for this force kind"};
@@ -709,6 +709,7 @@ void sx_forces_impl(sx_ctx *c, int kind, const double *p, int np)
         case SX_WCA: launch_gather<SX_WCA, false>(c, P, lpp); break;
         case SX_MORSE: launch_gather<SX_MORSE, false>(c, P, lpp); break;
         case SX_DH: launch_gather<SX_DH, false>(c, P, lpp); break;
+        case SX_COULOMB: launch_gather<SX_COULOMB, false>(c, P, lpp); break;
         case SX_GB: launch_gather<SX_GB, true>(c, P, lpp); break;
         case SX_GBR: launch_gather<SX_GBR, true>(c, P, lpp); break;
         case SX_GBW: launch_gather<SX_GBW, true>(c, P, lpp); break;
@@ -887,6 +888,7 @@ int sx_pair_eval(sx_ctx *c, int kind, const double *params, int nparams, long np
         case SX_ECPW: hipLaunchKernelGGL(k_pair_eval<SX_ECPW>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
         case SX_ECPC: hipLaunchKernelGGL(k_pair_eval<SX_ECPC>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
         case SX_ECPWC: hipLaunchKernelGGL(k_pair_eval<SX_ECPWC>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
+        case SX_COULOMB: hipLaunchKernelGGL(k_pair_eval<SX_COULOMB>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
         case SX_DIPOLE: hipLaunchKernelGGL(k_pair_eval<SX_DIPOLE>, g, b, 0, c->stream, s, np, di, dj, P, c->d_ecp_pp, ld, solver, dout); break;
         }
         std::vector<double> h(11 * np);

@@ -144,6 +144,34 @@ __device__ __forceinline__ void d_dh(const DState &s, int a, int b, const double
     for (int k = 0; k < 3; ++k) o.f[k] = cmask ? (fac * r[k] / rij_sq) : 0.0;
 }
 
+// coulombic_calculate_forces (force.f90:2627-2665): switched off over the
+// last 2 units before the cutoff, the switched force subtracting
+// sdx * PE * factor (PE already switched) from every component
+__device__ __forceinline__ void d_coulomb(const DState &s, int a, int b, const double *p, PairOut &o)
+{
+    const double PI_REF = 3.1415926535897932384626433832795;
+    double cutoff = p[0], dielectric = p[1];
+    double r[3];
+    pair_rij(s, a, b, r);
+    double rij_sq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    double rij_mag = sqrt(rij_sq);
+    o.pe = 0.0;
+    o.f[0] = o.f[1] = o.f[2] = 0.0;
+    if (!(rij_mag < cutoff)) return;
+    double sw = cutoff - 2.0;
+    double factor = 1.0 / (cutoff - sw);
+    double pe = s.f[SX_CHARGE][a] * s.f[SX_CHARGE][b] / (4.0 * PI_REF * dielectric * rij_mag);
+    for (int k = 0; k < 3; ++k) o.f[k] = pe * r[k] / rij_sq;
+    if (rij_mag >= sw) {
+        double xx = (rij_mag - sw) * factor;
+        double sx = 1.0 - 6.0 * xx * xx * xx * xx * xx + 15.0 * xx * xx * xx * xx - 10.0 * xx * xx * xx;
+        double sdx = -30.0 * xx * xx * xx * xx + 60.0 * xx * xx * xx - 30.0 * xx * xx;
+        pe = sx * pe;
+        for (int k = 0; k < 3; ++k) o.f[k] = sx * o.f[k] - sdx * pe * factor;
+    }
+    o.pe = pe;
+}
+
 // ---------------------------------------------------------------------------
 // Gay-Berne (force.f90:252-533)
 // ---------------------------------------------------------------------------
@@ -465,4 +493,5 @@ __device__ __forceinline__ void d_pair(const DState &s, int a, int b, const doub
     else if (KIND == SX_GBC) d_gb<3>(s, a, b, p, o);
     else if (KIND == SX_GBWC) d_gb<4>(s, a, b, p, o);
     else if (KIND == SX_DIPOLE) d_dipole(s, a, b, p, o);
+    else if (KIND == SX_COULOMB) d_coulomb(s, a, b, p, o);
 }

@@ -31,7 +31,7 @@ static int nparams_of(int kind)
     case SX_ECP: return 1;
     case SX_ECPW: case SX_ECPC: return 2;
     case SX_ECPWC: return 3;
-    case SX_DIPOLE: return 2;
+    case SX_DIPOLE: case SX_COULOMB: return 2;
     default: return 0;
     }
 }

@@ -11,7 +11,7 @@ list and the Philox words are compared for exact equality.
 import numpy as np
 import pytest
 
-from oracle.oracle import (DH, ECP, ECPC, ECPR, ECPW, ECPWC, GB, GBC, GBR, GBW, GBWC, LGV, LGVC, LJ, LJ12, LJW, MORSE,
+from oracle.oracle import (COULOMB, DH, ECP, ECPC, ECPR, ECPW, ECPWC, GB, GBC, GBR, GBW, GBWC, LGV, LGVC, LJ, LJ12, LJW, MORSE,
                            NHT, QQ, VV, VVC, WCA)
 
 ECP_KINDS = (ECP, ECPR, ECPW, ECPC, ECPWC)
@@ -63,6 +63,7 @@ PAIR_CASES = [
     (ECPC, (1.5, 0.7), dict(sig=(1, 1, 3))),
     (ECPWC, (1.5, 0.3, 0.7), dict(sig=(1, 1, 3))),
     (ECPWC, (1.2, 0.5, -1.3), dict(sig=(1.0, 2.0, 3.0), eps=(1.0, 2.0, 3.0))),
+    (COULOMB, (2.9, 2.0), dict(charges=True)),
 ]
 
 
@@ -147,6 +148,7 @@ FORCE_CASES = [
     (ECPW, (1.2, 0.3), 3.6, dict(sig=(1, 1, 3)), 2000, 24.0),
     (ECPC, (1.2, 0.7), 3.3, dict(sig=(1, 1, 3)), 2000, 24.0),
     (ECPWC, (1.2, 0.3, 0.7), 3.6, dict(sig=(1, 1, 3)), 2000, 24.0),
+    (COULOMB, (5.0, 2.0), 5.0, dict(charges=True), 2000, 16.0),
 ]
 
 

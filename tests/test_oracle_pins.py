@@ -13,7 +13,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from oracle.oracle import (DH, ECP, ECPC, ECPR, ECPW, ECPWC, GB, GBC, GBR, GBW, GBWC, LGV, LJ, LJ12, LJW, MORSE, QQ,
+from oracle.oracle import (COULOMB, DH, ECP, ECPC, ECPR, ECPW, ECPWC, GB, GBC, GBR, GBW, GBWC, LGV, LJ, LJ12, LJW, MORSE, QQ,
                            VV, VVC, WCA, State)
 from tests._states import base_state, chains, rand_quats
 
@@ -196,7 +196,10 @@ def _fd_check(energy, st, rotations, h=1e-6, rtol=2e-5, force=None, torques=None
 
 ISO = [(LJ, [1.0, 1.0, 2.5], 0.9, 2.4), (LJ12, [1.0, 1.0, 2.5], 0.9, 2.4),
        (LJW, [1.0, 1.0, 2.5, 0.3], 0.9, 2.4), (WCA, [1.0, 1.0], 0.9, 1.1),
-       (MORSE, [1.0, 2.0, 1.1, 2.5], 0.7, 2.4), (DH, [5.0, 1.0, 1.0], 0.6, 4.9)]
+       (MORSE, [1.0, 2.0, 1.1, 2.5], 0.7, 2.4), (DH, [5.0, 1.0, 1.0], 0.6, 4.9),
+       # Coulomb inside the switch band only: within the band the reference
+       # subtracts a scalar from every force component (not a gradient)
+       (COULOMB, [5.0, 2.0], 0.6, 2.95)]
 
 
 @pytest.mark.parametrize("kind,p,rmin,rmax", ISO)
@@ -210,6 +213,18 @@ def test_isotropic_forces_are_minus_gradient(oracle, kind, p, rmin, rmax):
         st["rx"], st["ry"], st["rz"] = [0, d[0]], [0, d[1]], [0, d[2]]
         pe, f, _, _ = oracle.pair(kind, st, 0, 1, p)
         _fd_check(lambda s: oracle.pair(kind, s, 0, 1, p)[0], st, False, force=f)
+
+
+def test_coulomb_closed_form(oracle):
+    """Below the switch band coulombic_calculate_forces is q_i q_j / (4 pi
+    eps r) exactly, and zero from the cutoff on."""
+    st = base_state(np.random.default_rng(3), 2, 30.0, charges=True)
+    st["charge"] = [1.5, -0.5]
+    for r in (0.7, 1.3, 2.9, 5.0, 6.0):
+        st["rx"], st["ry"], st["rz"] = [0.0, r], [0.0, 0.0], [0.0, 0.0]
+        pe = oracle.pair(COULOMB, st, 0, 1, [5.0, 2.0])[0]
+        want = 1.5 * -0.5 / (4 * np.pi * 2.0 * r) if r < 3.0 else (0.0 if r >= 5.0 else pe)
+        assert abs(pe - want) <= 1e-15 * max(1.0, abs(want)), r
 
 
 def test_lj_special_values(oracle):

@@ -10,7 +10,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from oracle.oracle import (DH, ECP, ECPC, ECPR, ECPW, ECPWC, GB, GBC, GBR, GBW, GBWC, LGV, LGVC, LJ, LJ12, LJW, MORSE,
+from oracle.oracle import (COULOMB, DH, ECP, ECPC, ECPR, ECPW, ECPWC, GB, GBC, GBR, GBW, GBWC, LGV, LGVC, LJ, LJ12, LJW, MORSE,
                            NHT, NHTC, QQ, VV, VVC, WCA, State, reference_available)
 from tests._states import base_state, chains
 
@@ -93,6 +93,7 @@ PAIRS = [
     (ECPW, [1.5, 0.3], dict(sig=(1, 1, 3)), 0.6, 3.7),
     (ECPC, [1.5, 0.6], dict(sig=(1, 1, 3)), 0.6, 3.4),
     (ECPWC, [1.5, 0.3, -0.6], dict(sig=(1.0, 2.0, 3.0)), 0.6, 3.7),
+    (COULOMB, [5.0, 2.0], dict(charges=True), 0.5, 5.5),
 ]
 
 
