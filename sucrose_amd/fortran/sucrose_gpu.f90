@@ -19,7 +19,11 @@ module sucrose_gpu
 
     ! pair kinds (sx_kind) and integrators (sx_integrator)
     integer(c_int), parameter, public :: SX_LJ = 1, SX_LJ12 = 2, SX_LJW = 3, SX_WCA = 4, SX_MORSE = 5, &
-        SX_GB = 6, SX_GBR = 7, SX_ECP = 8, SX_ECPR = 9, SX_DH = 10
+        SX_GB = 6, SX_GBR = 7, SX_ECP = 8, SX_ECPR = 9, SX_DH = 10, SX_GBW = 11, SX_GBC = 12, SX_GBWC = 13, &
+        SX_ECPW = 14, SX_ECPC = 15, SX_ECPWC = 16, SX_DIPOLE = 17, SX_COULOMB = 18
+    ! reporters (sx_report_kind)
+    integer(c_int), parameter, public :: SX_REPORT_STATE_DATA = 1, SX_REPORT_TRAJECTORY = 2, &
+        SX_REPORT_CONFIG = 3, SX_REPORT_CHECKPOINT = 4
     integer(c_int), parameter, public :: SX_INT_VV = 1, SX_INT_NHT = 4, SX_INT_LGV = 6, SX_INT_LGVC = 7
     ! scalar ids (sx_scalar)
     integer(c_int), parameter :: SX_POT_ENG = 0, SX_KIN_ENG = 1, SX_KIN_ROT = 2, SX_TEMP = 3, &
@@ -156,6 +160,83 @@ module sucrose_gpu
             type(c_ptr), value :: c
             real(c_double), value :: dt, damping
         end function
+        integer(c_int) function sx_get_box(c, b) bind(C, name="sx_get_box")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: c
+            real(c_double), intent(out) :: b(3)
+        end function
+        integer(c_int) function sx_set_atom_types(c, rt) bind(C, name="sx_set_atom_types")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: c
+            integer(c_int), intent(in) :: rt(*)
+        end function
+        integer(c_int) function sx_set_target_pressure(c, p) bind(C, name="sx_set_target_pressure")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: c
+            real(c_double), value :: p
+        end function
+        integer(c_int) function sx_mc_barostat(c, spec, dv, u, acc) bind(C, name="sx_mc_barostat")
+            import :: c_int, c_ptr, c_double, sx_step_spec
+            type(c_ptr), value :: c
+            type(sx_step_spec), intent(in) :: spec
+            real(c_double), value :: dv
+            real(c_double), intent(in) :: u(2)
+            integer(c_int), intent(out) :: acc
+        end function
+        integer(c_int) function sx_reporter_open(c, kind, fname, r) bind(C, name="sx_reporter_open")
+            import :: c_int, c_ptr, c_char
+            type(c_ptr), value :: c
+            integer(c_int), value :: kind
+            character(kind=c_char), intent(in) :: fname(*)
+            type(c_ptr), intent(out) :: r
+        end function
+        integer(c_int) function sx_report(r, step) bind(C, name="sx_report")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: r
+            integer(c_int), value :: step
+        end function
+        integer(c_int) function sx_reporter_close(r) bind(C, name="sx_reporter_close")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: r
+        end function
+        integer(c_int) function sx_load_checkpoint(c, fname, cfg) bind(C, name="sx_load_checkpoint")
+            import :: c_int, c_ptr, c_char
+            type(c_ptr), value :: c
+            character(kind=c_char), intent(in) :: fname(*)
+            integer(c_int), intent(out) :: cfg
+        end function
+        integer(c_int) function sx_copy_transform(c, cnt, f1, t1, off, si) bind(C, name="sx_copy_transform")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: c
+            integer(c_int), value :: cnt, si
+            integer(c_int), intent(in) :: f1(*), t1(*)
+            real(c_double), intent(in) :: off(*)
+        end function
+        integer(c_int) function sx_transfer_forces(c, cnt, f1, t1) bind(C, name="sx_transfer_forces")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: c
+            integer(c_int), value :: cnt
+            integer(c_int), intent(in) :: f1(*), t1(*)
+        end function
+        integer(c_int) function sx_init_reaction_field(c) bind(C, name="sx_init_reaction_field")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: c
+        end function
+        integer(c_int) function sx_eval_reaction_field(c) bind(C, name="sx_eval_reaction_field")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: c
+        end function
+        integer(c_int) function sx_set_bonded(c, kind, cnt, idx1, coeff, angle) bind(C, name="sx_set_bonded")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: c
+            integer(c_int), value :: kind, cnt
+            integer(c_int), intent(in) :: idx1(*)
+            real(c_double), value :: coeff, angle
+        end function
+        integer(c_int) function sx_bonded_forces(c) bind(C, name="sx_bonded_forces")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: c
+        end function
         integer(c_int) function sx_run(c, spec, nsteps) bind(C, name="sx_run")
             import :: c_int, c_ptr, sx_step_spec
             type(c_ptr), value :: c
@@ -167,7 +248,10 @@ module sucrose_gpu
     public :: gpu_init, gpu_finalize, gpu_push_system, gpu_pull_system, gpu_init_nlist, gpu_update_nlist, &
         gpu_initialize_forces, gpu_calculate_forces, gpu_calculate_thermo, gpu_vv_initial_step, &
         gpu_vv_final_step, gpu_qq_initial_step, gpu_qq_final_step, gpu_nht_initial_step, &
-        gpu_nht_final_step, gpu_lgv_initial_step, gpu_lgv_final_step, gpu_run
+        gpu_nht_final_step, gpu_lgv_initial_step, gpu_lgv_final_step, gpu_run, gpu_set_target_pressure, &
+        gpu_mc_barostat, gpu_reporter_open, gpu_report, gpu_reporter_close, gpu_load_checkpoint, &
+        gpu_copy_transform, gpu_transfer_forces, gpu_init_reaction_field, gpu_eval_reaction_field, &
+        gpu_set_bonded, gpu_bonded_forces
 
 contains
 
@@ -219,9 +303,12 @@ contains
         call check(sx_set_scalar(ctx, SX_POT_ENG, pot_eng), 'sx_set_scalar')
         call check(sx_set_scalar(ctx, SX_KIN_ENG, kin_eng), 'sx_set_scalar')
         call check(sx_set_scalar(ctx, SX_KIN_ROT, kin_rot), 'sx_set_scalar')
+        call check(sx_set_scalar(ctx, SX_TEMP, temp), 'sx_set_scalar')
+        call check(sx_set_scalar(ctx, SX_PRESSURE, pressure), 'sx_set_scalar')
         call check(sx_set_scalar(ctx, SX_DENSITY, density), 'sx_set_scalar')
         call check(sx_set_scalar(ctx, SX_TARGET_TEMP, target_temp), 'sx_set_scalar')
         call check(sx_set_scalar(ctx, SX_DOF, real(dof, c_double)), 'sx_set_scalar')
+        call check(sx_set_atom_types(ctx, int(rtype, c_int)), 'sx_set_atom_types')
         if (NC > 0) then
             bi = int(BBI(1:NC), c_int)
             bj = int(BBJ(1:NC), c_int)
@@ -252,6 +339,8 @@ contains
         call check(sx_get_scalar(ctx, SX_PRESSURE, pressure), 'sx_get_scalar')
         call check(sx_get_scalar(ctx, SX_VIRIAL, virial), 'sx_get_scalar')
         call check(sx_get_scalar(ctx, SX_VIRIAL_BONDED, virial_bonded), 'sx_get_scalar')
+        call check(sx_get_scalar(ctx, SX_DENSITY, density), 'sx_get_scalar')
+        call check(sx_get_box(ctx, box), 'sx_get_box')
     contains
         subroutine get(field, a)
             integer, intent(in) :: field
@@ -341,5 +430,100 @@ contains
         integer, intent(in) :: nsteps
         call check(sx_run(ctx, spec, int(nsteps, c_int)), 'sx_run')
     end subroutine gpu_run
+
+    subroutine gpu_set_target_pressure(pres)         ! set_target_pressure, system.f90:224
+        real(real64), intent(in) :: pres
+        call check(sx_set_target_pressure(ctx, pres), 'sx_set_target_pressure')
+    end subroutine gpu_set_target_pressure
+
+    ! mc_barostat (integrators.f90:864-945): the two uniform numbers come from
+    ! random_number in the order the reference draws them (the volume move,
+    ! then the acceptance test); compute_potential is the list rebuilt at the
+    ! current box plus the pair loops (and bonded terms) of spec
+    subroutine gpu_mc_barostat(spec, accept, volume_change)
+        type(sx_step_spec), intent(in) :: spec
+        logical, intent(out) :: accept
+        real(real64), intent(in), optional :: volume_change
+        real(c_double) :: u(2), dv
+        integer(c_int) :: acc
+        call random_number(u(1))
+        call random_number(u(2))
+        dv = -1.0_c_double
+        if (present(volume_change)) dv = volume_change
+        call check(sx_mc_barostat(ctx, spec, dv, u, acc), 'sx_mc_barostat')
+        accept = acc /= 0
+    end subroutine gpu_mc_barostat
+
+    ! open_*_file / report_* / close_*_file (reporters.f90) on the device
+    ! state: a handle per open file
+    function gpu_reporter_open(kind, filename) result(r)
+        integer, intent(in) :: kind
+        character(*), intent(in) :: filename
+        type(c_ptr) :: r
+        call check(sx_reporter_open(ctx, int(kind, c_int), trim(filename) // c_null_char, r), 'sx_reporter_open')
+    end function gpu_reporter_open
+
+    subroutine gpu_report(r, step)
+        type(c_ptr), intent(in) :: r
+        integer, intent(in) :: step
+        call check(sx_report(r, int(step, c_int)), 'sx_report')
+    end subroutine gpu_report
+
+    subroutine gpu_reporter_close(r)
+        type(c_ptr), intent(inout) :: r
+        call check(sx_reporter_close(r), 'sx_reporter_close')
+        r = c_null_ptr
+    end subroutine gpu_reporter_close
+
+    subroutine gpu_load_checkpoint(filename, config_index)   ! load_checkpoint, reporters.f90:188
+        character(*), intent(in) :: filename
+        integer, intent(out), optional :: config_index
+        integer(c_int) :: cfg
+        call check(sx_load_checkpoint(ctx, trim(filename) // c_null_char, cfg), 'sx_load_checkpoint')
+        if (present(config_index)) config_index = cfg
+    end subroutine gpu_load_checkpoint
+
+    subroutine gpu_copy_transform(from, to, xoffset, yoffset, zoffset, set_inertia)   ! system.f90:743
+        integer, intent(in) :: from, to
+        real(real64), intent(in) :: xoffset, yoffset, zoffset
+        logical, intent(in), optional :: set_inertia
+        integer(c_int) :: si
+        si = -1
+        if (present(set_inertia)) si = merge(1_c_int, 0_c_int, set_inertia)
+        call check(sx_copy_transform(ctx, 1_c_int, [int(from, c_int)], [int(to, c_int)], &
+                                     [xoffset, yoffset, zoffset], si), 'sx_copy_transform')
+    end subroutine gpu_copy_transform
+
+    subroutine gpu_transfer_forces(from, to)         ! transfer_forces, system.f90:844
+        integer, intent(in) :: from, to
+        call check(sx_transfer_forces(ctx, 1_c_int, [int(from, c_int)], [int(to, c_int)]), 'sx_transfer_forces')
+    end subroutine gpu_transfer_forces
+
+    subroutine gpu_init_reaction_field()             ! init_reaction_field, force.f90:2677
+        call check(sx_init_reaction_field(ctx), 'sx_init_reaction_field')
+    end subroutine gpu_init_reaction_field
+
+    subroutine gpu_eval_reaction_field()             ! eval_reaction_field, force.f90:2753
+        call check(sx_eval_reaction_field(ctx), 'sx_eval_reaction_field')
+    end subroutine gpu_eval_reaction_field
+
+    ! bonded terms registered once (kind as sx_set_bonded: 1 bend_, 2
+    ! cosine_bend_, 3 dih_, 4 cosine_dih_, 5 angle_bend_, 6 angle_dih_);
+    ! idx(:, t) holds term t's atoms
+    subroutine gpu_set_bonded(kind, idx, coeff, angle)
+        integer, intent(in) :: kind
+        integer, intent(in) :: idx(:, :)
+        real(real64), intent(in) :: coeff
+        real(real64), intent(in), optional :: angle
+        real(c_double) :: a
+        a = 0.0_c_double
+        if (present(angle)) a = angle
+        call check(sx_set_bonded(ctx, int(kind, c_int), int(size(idx, 2), c_int), int(reshape(idx, [size(idx)]), c_int), &
+                                 coeff, a), 'sx_set_bonded')
+    end subroutine gpu_set_bonded
+
+    subroutine gpu_bonded_forces()
+        call check(sx_bonded_forces(ctx), 'sx_bonded_forces')
+    end subroutine gpu_bonded_forces
 
 end module sucrose_gpu

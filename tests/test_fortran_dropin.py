@@ -31,3 +31,23 @@ def test_lj_nve_dropin_matches_reference_loop():
     r = subprocess.run([EXE], capture_output=True, text=True, timeout=300)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0 and "DROPIN OK" in r.stdout
+
+
+NPT = os.path.join(ROOT, "tests", "fortran", "_build", "npt_dropin")
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(NPT), reason="drop-in driver not built (make -C tests/fortran)")
+def test_barostat_and_reporters_dropin(tmp_path):
+    """gpu_mc_barostat with random_number reseeded agrees move by move with
+    the reference's mc_barostat (accept, box, positions exactly), and the
+    device reporters write the reference's trajectory and checkpoint frames
+    byte for byte (tests/fortran/npt_dropin.f90)."""
+    print("npt_dropin: 108 LJ atoms, 6 barostat moves, reference vs GPU binding; reporter frames", flush=True)
+    r = subprocess.run([NPT, str(tmp_path)], capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0 and "NPT DROPIN OK" in r.stdout
+    for name in ("traj", "chk"):
+        a = (tmp_path / f"{name}_ref.txt").read_bytes()
+        b = (tmp_path / f"{name}_gpu.txt").read_bytes()
+        assert a == b, name
