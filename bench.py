@@ -36,10 +36,10 @@ FLOP_APPLY_ENTRY = 8        # per full-list entry of the per-particle apply
 # copy measured the same way (FETCH x2.000, WRITE x1.000); the runs and the
 # arithmetic are in profiles/ecp_langevin_32768_fetch_write.txt
 TRAFFIC_BYTES = {
-    "ecp_precompute": (1694.5 * 2.0 + 8192.0) * 1024,
-    "ecp_screen": (10238.6 * 2.0 + 914.2) * 1024,
-    "ecp_solve": (10438.2 * 2.0 + 3885.0) * 1024,
-    "ecp_apply": (14661.7 * 2.0 + 1792.0) * 1024,
+    "ecp_precompute": (1687.0 * 2.0 + 8192.0) * 1024,
+    "ecp_screen": (10242.1 * 2.0 + 917.6) * 1024,
+    "ecp_solve": (12637.2 * 2.0 + 6209.3) * 1024,
+    "ecp_apply": (12423.7 * 2.0 + 3977.4) * 1024,
 }
 
 

@@ -363,8 +363,9 @@ int sx_batch_set_shape(sx_ctx *ctx, int choice);
 /* measurement and synthetic set-up (no counterpart in the reference:   */
 /* benchmark and test harness support)                                 */
 /* ------------------------------------------------------------------ */
-/* HIP-event timing of the ECP pipeline on the current state, averaged over
- * nrep runs: ms = {per-particle precompute, screen, solve, apply};
+/* HIP-event timing of the ECP pipeline on the current state: after three
+ * untimed passes, the median over nrep timed passes of each stage's
+ * duration, ms = {per-particle precompute, screen, solve, apply};
  * stats = {half pairs screened, pairs solved, solver iterations} per run.
  * Forces accumulate; re-initialise them afterwards. */
 int sx_profile_ecp(sx_ctx *ctx, int kind, const double *params, int nparams, int nrep, double ms[4],

@@ -785,21 +785,22 @@ int sx_profile_ecp(sx_ctx *c, int kind, const double *params, int nparams, int n
         // extra pass with the solve counters on for the pair statistics
         EcpLaunch L = ecp_launch_shape(c, false);
         EcpLaunch Ls = ecp_launch_shape(c, true);
+        // three untimed passes, then the median of nrep timed passes per stage
         hipEvent_t ev[5];
         for (auto &e : ev) HIPCHK(hipEventCreate(&e));
-        double acc[4] = {0, 0, 0, 0};
+        std::vector<double> times[4];
         long dev0[8], dev1[8];
-        for (int r = 0; r < nrep; ++r) {
+        for (int r = -3; r < nrep; ++r) {
             for (int stage = 0; stage < 4; ++stage) {
                 HIPCHK(hipEventRecord(ev[stage], c->stream));
                 ecp_stage(c, stage, kind, sc, L);
             }
             HIPCHK(hipEventRecord(ev[4], c->stream));
             HIPCHK(hipEventSynchronize(ev[4]));
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < 4 && r >= 0; ++q) {
                 float t = 0;
                 HIPCHK(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
-                acc[q] += t;
+                times[q].push_back(t);
             }
         }
         HIPCHK(hipMemcpyAsync(dev0, c->d_counts, sizeof(long) * 8, hipMemcpyDeviceToHost, c->stream));
@@ -809,7 +810,12 @@ int sx_profile_ecp(sx_ctx *c, int kind, const double *params, int nparams, int n
         HIPCHK(hipMemcpyAsync(sizes, c->d_flag, sizeof(int) * 8, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         for (auto &e : ev) hipEventDestroy(e);
-        for (int q = 0; q < 4; ++q) ms[q] = acc[q] / std::max(1, nrep);
+        for (int q = 0; q < 4; ++q) {
+            std::vector<double> &v = times[q];
+            if (v.empty()) { ms[q] = 0.0; continue; }
+            std::sort(v.begin(), v.end());
+            ms[q] = v.size() % 2 ? v[v.size() / 2] : 0.5 * (v[v.size() / 2 - 1] + v[v.size() / 2]);
+        }
         stats[0] = sizes[2];
         stats[1] = (double)(dev1[0] - dev0[0]);
         stats[2] = (double)(dev1[1] - dev0[1]);
