@@ -269,16 +269,16 @@ __device__ __forceinline__ double sx_mimg_fast(double d, double L, double invL) 
 // blockDim.x must be 256.
 // Returns true in the last block, where tot[] (valid in thread 0) holds the
 // grid totals.
-template <int W>
+template <int W, int BS = 256>
 __device__ __forceinline__ bool sx_block_reduce_last(double (&v)[W], double *part, unsigned *ticket, double (&tot)[W])
 {
-    __shared__ double sh[W][256];
+    __shared__ double sh[W][BS];
     __shared__ bool last;
     int t = threadIdx.x;
 #pragma unroll
     for (int q = 0; q < W; ++q) sh[q][t] = v[q];
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int s = BS / 2; s > 0; s >>= 1) {
         if (t < s) {
 #pragma unroll
             for (int q = 0; q < W; ++q) sh[q][t] = sh[q][t] + sh[q][t + s];
@@ -299,12 +299,12 @@ __device__ __forceinline__ bool sx_block_reduce_last(double (&v)[W], double *par
 #pragma unroll
     for (int q = 0; q < W; ++q) {
         double a = 0.0;
-        for (int b = t; b < nb; b += 256)
+        for (int b = t; b < nb; b += BS)
             a = a + __hip_atomic_load(&part[b * W + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         sh[q][t] = a;
     }
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int s = BS / 2; s > 0; s >>= 1) {
         if (t < s) {
 #pragma unroll
             for (int q = 0; q < W; ++q) sh[q][t] = sh[q][t] + sh[q][t + s];
