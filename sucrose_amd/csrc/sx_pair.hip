@@ -185,6 +185,7 @@ __global__ void __launch_bounds__(256) k_ecp_screen(const double4 *pos4, const f
 }
 
 #define ECP_SOLVE_THREADS 64
+#define ECP_SOLVE_WG 64    // workgroup of the solve stage (256, pairs spread over 4 wavefronts, measured 2x slower)
 
 // one block per segment of the work list; stats (may be NULL) accumulates
 // solved pairs, solver iterations, fallbacks and, for the learned solver,
@@ -194,7 +195,7 @@ __global__ void __launch_bounds__(256) k_ecp_screen(const double4 *pos4, const f
 // get 1 + k, so the apply stage finds the solved pairs of a row without
 // reading the others
 template <int VARIANT>
-__global__ void __launch_bounds__(ECP_SOLVE_THREADS) k_ecp_solve(DState s, const double *pp, long ld, const int *half_i,
+__global__ void __launch_bounds__(ECP_SOLVE_WG) k_ecp_solve(DState s, const double *pp, long ld, const int *half_i,
                                                                  const int *half_j, const int *work, const int *segcount,
                                                                  const int *nnz_half, EcpScreen sc, int solver,
                                                                  const float *lam0, double *res, unsigned char *live,
@@ -205,7 +206,10 @@ __global__ void __launch_bounds__(ECP_SOLVE_THREADS) k_ecp_solve(DState s, const
     ecp_segment(*nnz_half, blockIdx.x, start, end);
     long cnt = segcount[blockIdx.x];
     long my_iters = 0, my_solved = 0, my_fb = 0, my_skip = 0;
-    for (long u = threadIdx.x; u < cnt; u += blockDim.x) {
+    // the segment's pairs dealt round-robin over the workgroup's wavefronts,
+    // so each wavefront carries a few pairs and the CU interleaves them
+    const int nw = blockDim.x / SX_WAVE, lane = threadIdx.x % SX_WAVE, wave = threadIdx.x / SX_WAVE;
+    for (long u = (long)lane * nw + wave; u < cnt; u += blockDim.x) {
         long t = start + u;
         int k = work[t];
         int iters = 0, fb = 0;
@@ -485,7 +489,7 @@ static void ecp_stage(sx_ctx *c, int stage, int kind, const EcpScreen &sc, const
         }
         const float *lam0 = nullptr;
 #define SX_ECP_SOLVE(V)                                                                                          \
-    hipLaunchKernelGGL(k_ecp_solve<V>, dim3(ECP_NSEG), dim3(ECP_SOLVE_THREADS), 0, c->stream, s, c->d_ecp_pp, ld, \
+    hipLaunchKernelGGL(k_ecp_solve<V>, dim3(ECP_NSEG), dim3(ECP_SOLVE_WG), 0, c->stream, s, c->d_ecp_pp, ld, \
                        nl.half_i, nl.half_j, c->d_work, c->d_segcount, nnz_half, sc, c->ecp_solver, lam0,         \
                        c->d_pairbuf, c->d_live, stats, nl.fwdpos, nl.revpos, c->d_idx_full)
         switch (sc.fvar) {
@@ -984,7 +988,8 @@ extern "C" int sx_ecp_compare_solvers(sx_ctx *c, int kind, const double *params,
         });
         auto full = [&](int solver, const float *l0) {
 #define SX_ECP_FULL(V)                                                                                            \
-    hipLaunchKernelGGL(k_ecp_solve<V>, g, b, 0, c->stream, s, c->d_ecp_pp, 0L, nl.half_i, nl.half_j, c->d_work,  \
+    hipLaunchKernelGGL(k_ecp_solve<V>, g, dim3(ECP_SOLVE_WG), 0, c->stream, s, c->d_ecp_pp, 0L, nl.half_i,      \
+                       nl.half_j, c->d_work,                                                                     \
                        c->d_segcount, nnz_half, sc, solver, l0, c->d_pairbuf, c->d_live, (long *)nullptr,        \
                        (const int *)nullptr, (const int *)nullptr, (int *)nullptr)
             switch (sc.fvar) {
