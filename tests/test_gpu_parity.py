@@ -659,3 +659,51 @@ def test_composite_bodies(dev, oracle):
     dev.set_composites([], [], np.zeros((0, 3)))
     for f in ("rx", "ry", "vx", "qw", "fx", "tz"):
         assert rel_err(got[f], ref[f]) < 1e-9, (f, rel_err(got[f], ref[f]))
+
+
+def test_run_graph_dipoles_and_coulomb(dev, oracle):
+    """sx_run with the point dipoles and the switched Coulomb kind in one
+    step (eval_reaction_field applied after the pair loops, the field
+    accumulating across steps as the reference's does) vs the oracle's
+    README loop, 8 velocity-Verlet steps."""
+    from oracle.oracle import DIPOLE
+    say("run graph: dipoles + Coulomb, reaction field accumulating")
+    rng = np.random.default_rng(97)
+    n, L = 800, 16.0
+    st = base_state(rng, n, L, forces=False, charges=True, mass=(1.0, 2.0))
+    g = np.arange(10) * (L / 10) - L / 2
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)[:n] + rng.normal(size=(n, 3)) * 0.05
+    st["rx"], st["ry"], st["rz"] = P[:, 0], P[:, 1], P[:, 2]
+    st["vx"] *= 0.3; st["vy"] *= 0.3; st["vz"] *= 0.3
+    to_device(dev, st)
+    dev.init_nlist(0.3, 5.0)
+    dev.init_reaction_field()
+    forces = [(DIPOLE, 3.0, 10.0), (COULOMB, 5.0, 10.0)]
+    dev.initialize_forces()
+    for k in forces:
+        dev.forces(*k)
+    dev.eval_reaction_field()
+    ref = from_device(dev, st)
+    _, rfl0 = dev.get_reaction_field()
+    dip = dev.get_reaction_field()[0].copy()
+    rfl = np.repeat(rfl0[:, None], n, axis=1)
+    nl = oracle.nlist_new(ref, 5.0, 0.3)
+    oracle.nlist_build(ref, nl)
+    dev.run(1, forces, 8, 0.002)
+    for t in range(8):
+        oracle.integrate(VV, 0, ref, 0.002)
+        if oracle.needs_rebuild(ref, nl):
+            oracle.nlist_build(ref, nl)
+        oracle.call("or_initialize_forces", ref)
+        oracle.dipole_loop(ref, nl, 3.0, 10.0, dip, rfl)
+        oracle.pair_loop(ref, nl, COULOMB, [5.0, 10.0])
+        oracle.eval_reaction_field(ref, dip, rfl)
+        oracle.integrate(VV, 1, ref, 0.002)
+        oracle.call("or_calculate_thermo", ref)
+    got = from_device(dev, st)
+    for f in ("rx", "vx", "fx", "fz", "tx", "tz"):
+        assert rel_err(got[f], ref[f]) < 1e-9, (f, rel_err(got[f], ref[f]))
+    assert abs(got["pot_eng"] - ref["pot_eng"]) <= 1e-9 * max(1.0, abs(ref["pot_eng"]))
+    gd, gr = dev.get_reaction_field()
+    assert np.abs(gr - rfl[:, 0]).max() <= 1e-10 * max(1.0, np.abs(rfl[:, 0]).max())
