@@ -120,6 +120,17 @@ int sx_set_num_atoms(sx_ctx *ctx, int num_atoms, int rigid_atoms);
 /* set_box (system.f90:185) + set_periodic_boundary (:209) */
 int sx_set_box(sx_ctx *ctx, const double box[3], int periodic);
 int sx_get_box(sx_ctx *ctx, double box[3]);
+/* set_density (system.f90:167-183): a cubic box of edge ((N/Nrigid)/dens)^(1/3) */
+int sx_set_density(sx_ctx *ctx, double dens);
+/* generate_fcc_lattice (system.f90:438-523): 4 unit_cells^3 sites of the four
+ * body-diagonal orientations, the box from set_density, positions shifted by
+ * -box/2; sets the number of atoms (rigid_atoms per site when > 0) if none
+ * is set yet, as the reference does; the periodic flag is left as it is */
+int sx_generate_fcc_lattice(sx_ctx *ctx, int unit_cells, double dens, int rigid_atoms);
+/* count values of one field from 1-based atom first1 on: the per-atom setters
+ * set_atom_mass/charge/shape/energy/position/velocity/inertia (system.f90:
+ * 129-343) are one or three such calls */
+int sx_set_atom_values(sx_ctx *ctx, int field, int first1, int count, const double *values);
 /* set_atom_type (system.f90:109) for all n atoms at once; rtype is host
  * metadata (-1 until set; initialize_system turns negatives into 0), read by
  * the trajectory and checkpoint reporters */

@@ -132,6 +132,8 @@ def lib():
             "sx_batch_autotune": [vp, C.c_double, C.c_double, C.c_int, C.c_int, _PI, _PD],
             "sx_batch_set_shape": [vp, C.c_int],
             "sx_set_atom_types": [vp, _PI], "sx_get_atom_types": [vp, _PI],
+            "sx_set_density": [vp, C.c_double], "sx_generate_fcc_lattice": [vp, C.c_int, C.c_double, C.c_int],
+            "sx_set_atom_values": [vp, C.c_int, C.c_int, C.c_int, _PD],
             "sx_set_target_pressure": [vp, C.c_double],
             "sx_copy_transform": [vp, C.c_int, _PI, _PI, _PD, C.c_int],
             "sx_transfer_forces": [vp, C.c_int, _PI, _PI],
@@ -595,6 +597,54 @@ class System:
         dv = -1.0 if volume_change is None else float(volume_change)
         self._ok(self.L.sx_mc_barostat(self.h, C.byref(sp), dv, _dp(uu), C.byref(acc)))
         return bool(acc.value)
+
+    # ---- set-up routines of module system (system.f90:43-360) ----
+    def set_density(self, dens):
+        self._ok(self.L.sx_set_density(self.h, float(dens)))
+
+    def set_periodic_boundary(self):
+        b = self.get_box()
+        self._ok(self.L.sx_set_box(self.h, _dp(b), 1))
+
+    def generate_fcc_lattice(self, unit_cells, dens, rigid_atoms=None):
+        self._ok(self.L.sx_generate_fcc_lattice(self.h, int(unit_cells), float(dens), int(rigid_atoms or 0)))
+        if self.n == 0:
+            self.n = 4 * int(unit_cells) ** 3 * int(rigid_atoms or 1)
+
+    def _set_atom(self, fields, id1, values):
+        for f, v in zip(fields, values):
+            a = np.array([float(v)])
+            self._ok(self.L.sx_set_atom_values(self.h, FIELD[f], int(id1), 1, _dp(a)))
+
+    def set_atom_mass(self, id1, m):
+        self._set_atom(["MASS"], id1, [m])
+
+    def set_atom_charge(self, id1, q):
+        self._set_atom(["CHARGE"], id1, [q])
+
+    def set_atom_shape(self, id1, length, width, height):
+        if length == 0.0 or width == 0.0 or height == 0.0:
+            raise SucroseError(-1, "Error: Shape dimensions cannot be zero.")
+        self._set_atom(["SIG_X", "SIG_Y", "SIG_Z"], id1, [length, width, height])
+
+    def set_atom_energy(self, id1, ex, ey, ez):
+        self._set_atom(["EPS_X", "EPS_Y", "EPS_Z"], id1, [ex, ey, ez])
+
+    def set_atom_position(self, id1, x, y, z):
+        self._set_atom(["RX", "RY", "RZ"], id1, [x, y, z])
+
+    def set_atom_velocity(self, id1, vx, vy, vz):
+        self._set_atom(["VX", "VY", "VZ"], id1, [vx, vy, vz])
+
+    def set_atom_inertia(self, id1, ix, iy, iz):
+        self._set_atom(["IXX", "IYY", "IZZ"], id1, [ix, iy, iz])
+
+    def set_atom_type(self, id1, t):
+        if not 1 <= int(id1) <= self.n:
+            raise SucroseError(-1, f"Error: Atom index {id1} does not exist in system of {self.n} atoms.")
+        rt = self.get_atom_types()
+        rt[int(id1) - 1] = int(t)
+        self.set_atom_types(rt)
 
     # ---- atom types and checkpoints ----
     def set_atom_types(self, rtype):

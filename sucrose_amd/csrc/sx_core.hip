@@ -5,6 +5,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <string>
+#include <vector>
 
 static thread_local std::string g_create_error;
 
@@ -429,6 +431,95 @@ int sx_set_box(sx_ctx *c, const double box[3], int periodic)
         c->h_scal[SX_DENSITY] = (double)(c->n / c->nrigid) / (box[0] * box[1] * box[2]);
         sx_sync_scalars_to_device(c);
         c->have_graph = false;
+    });
+}
+
+// set_density (system.f90:167-183): a cubic box of (N / Nrigid) / dens
+static void set_density_impl(sx_ctx *c, double dens)
+{
+    if (dens == 0.0) throw sx_error{SX_ERR_ARG, "Error: Density cannot be zero."};
+    double vol = (double)(c->n / c->nrigid) / dens;
+    double box_len = std::pow(vol, 1.0 / 3.0);
+    c->box[0] = c->box[1] = c->box[2] = box_len;
+    sx_sync_scalars_to_host(c);
+    c->h_scal[SX_DENSITY] = dens;
+    sx_sync_scalars_to_device(c);
+    c->have_graph = false;
+}
+
+int sx_set_density(sx_ctx *c, double dens)
+{
+    return guard(c, [&] { need_system(c); set_density_impl(c, dens); });
+}
+
+int sx_generate_fcc_lattice(sx_ctx *c, int unit_cells, double dens, int rigid_atoms)
+{
+    return guard(c, [&] {
+        if (unit_cells <= 0) throw sx_error{SX_ERR_ARG, "unit_cells must be > 0"};
+        int num_atoms = 4 * unit_cells * unit_cells * unit_cells;
+        if (c->n <= 0) {
+            int rc = sx_set_num_atoms(c, num_atoms, rigid_atoms > 0 ? rigid_atoms : 1);
+            if (rc != SX_OK) throw sx_error{rc, c->err};
+        } else if (c->n < num_atoms) {
+            throw sx_error{SX_ERR_ARG, "Error: FCC lattice of " + std::to_string(num_atoms) +
+                                           " atoms cannot be generated for system of " + std::to_string(c->n) +
+                                           " atoms."};
+        }
+        set_density_impl(c, dens);
+        double cell = c->box[0] / (double)unit_cells;
+        double half_cell = cell / 2.0;
+        double rroot3 = 1.0 / std::sqrt(3.0);
+        double den = std::sqrt(1.0 - rroot3 * rroot3);
+        double sp = std::sqrt((1.0 + rroot3) / 2.0), sm = std::sqrt((1.0 - rroot3) / 2.0);
+        // the four basis sites and their orientations (system.f90:466-492)
+        const double bx[4] = {0.0, half_cell, 0.0, half_cell}, by[4] = {0.0, half_cell, half_cell, 0.0},
+                     bz[4] = {0.0, 0.0, half_cell, half_cell};
+        const double qw[4] = {sp, sm, sm, sp};
+        const double qx[4] = {sm * (rroot3 / den), sp * (-rroot3 / den), sp * (rroot3 / den), sm * (-rroot3 / den)};
+        const double qy[4] = {sm * (-rroot3 / den), sp * (-rroot3 / den), sp * (rroot3 / den), sm * (rroot3 / den)};
+        std::vector<double> f[7];
+        for (auto &v : f) v.resize(num_atoms);
+        int m = 0;
+        for (int I = 1; I <= unit_cells; ++I)
+            for (int J = 1; J <= unit_cells; ++J)
+                for (int K = 1; K <= unit_cells; ++K) {
+                    for (int r = 0; r < 4; ++r) {
+                        f[0][m + r] = bx[r] + cell * (double)(I - 1);
+                        f[1][m + r] = by[r] + cell * (double)(J - 1);
+                        f[2][m + r] = bz[r] + cell * (double)(K - 1);
+                        f[3][m + r] = qw[r];
+                        f[4][m + r] = qx[r];
+                        f[5][m + r] = qy[r];
+                        f[6][m + r] = 0.0;
+                    }
+                    m += 4;
+                }
+        for (int i = 0; i < num_atoms; ++i) {   // shifted by -box/2 (:518-520)
+            f[0][i] = f[0][i] - c->box[0] / 2.0;
+            f[1][i] = f[1][i] - c->box[1] / 2.0;
+            f[2][i] = f[2][i] - c->box[2] / 2.0;
+        }
+        const int fl[7] = {SX_RX, SX_RY, SX_RZ, SX_QW, SX_QX, SX_QY, SX_QZ};
+        for (int k = 0; k < 7; ++k)
+            HIPCHK(hipMemcpyAsync(c->d_fields + (long)fl[k] * c->ld, f[k].data(), sizeof(double) * num_atoms,
+                                  hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int sx_set_atom_values(sx_ctx *c, int field, int first1, int count, const double *values)
+{
+    return guard(c, [&] {
+        need_system(c);
+        if (field < 0 || field >= SX_NFIELDS || count < 0 || (count > 0 && !values))
+            throw sx_error{SX_ERR_ARG, "bad field values"};
+        if (first1 < 1 || first1 - 1 + count > c->n)
+            throw sx_error{SX_ERR_ARG, "Error: Atom index " + std::to_string(first1) + " does not exist in system of " +
+                                           std::to_string(c->n) + " atoms."};
+        if (count == 0) return;
+        HIPCHK(hipMemcpyAsync(c->d_fields + (long)field * c->ld + (first1 - 1), values, sizeof(double) * count,
+                              hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
     });
 }
 

@@ -707,3 +707,31 @@ def test_run_graph_dipoles_and_coulomb(dev, oracle):
     assert abs(got["pot_eng"] - ref["pot_eng"]) <= 1e-9 * max(1.0, abs(ref["pot_eng"]))
     gd, gr = dev.get_reaction_field()
     assert np.abs(gr - rfl[:, 0]).max() <= 1e-10 * max(1.0, np.abs(rfl[:, 0]).max())
+
+
+def test_fcc_lattice_and_atom_setters(oracle):
+    """generate_fcc_lattice and set_density on the library (host set-up,
+    uploaded) against the oracle, bit for bit; the per-atom setters and their
+    errors."""
+    import ctypes as C
+    import sucrose_amd as sx
+    from oracle.oracle import State
+    say("fcc lattice and per-atom setters")
+    d = sx.System(0)
+    d.generate_fcc_lattice(6, 0.8442)
+    assert d.n == 864
+    a = State(864, [1.0, 1.0, 1.0])
+    oracle.call("or_generate_fcc_lattice", a, C.c_int(6), C.c_double(0.8442))
+    assert np.array_equal(d.get_box(), a.box)
+    for f in ("rx", "ry", "rz", "qw", "qx", "qy", "qz"):
+        assert np.array_equal(d.download(f.upper()), a[f]), f
+    assert d.get_scalar("DENSITY") == a["density"]
+    d.set_atom_position(5, 1.5, -2.0, 0.25)
+    d.set_atom_shape(7, 1.0, 1.0, 3.0)
+    d.set_atom_mass(864, 2.5)
+    assert d.download("RX")[4] == 1.5 and d.download("SIG_Z")[6] == 3.0 and d.download("MASS")[863] == 2.5
+    with pytest.raises(sx.SucroseError):
+        d.set_atom_mass(865, 1.0)
+    with pytest.raises(sx.SucroseError):
+        d.set_atom_shape(1, 0.0, 1.0, 1.0)
+    d.close()

@@ -253,6 +253,10 @@ def test_fcc_lattice(oracle, reference):
     a = st.copy()
     oracle.call("or_generate_fcc_lattice", a, C.c_int(6), C.c_double(0.8442))
     assert np.array_equal(a.buf[:, :3], b.buf[:, :3])
+    assert np.array_equal(a.buf[:, 9:13], b.buf[:, 9:13])   # quaternions
+    box = np.zeros(3)
+    reference.lib.ref_get_box(box.ctypes.data_as(C.POINTER(C.c_double)))
+    assert np.array_equal(a.box, box) and a["density"] == b["density"]
 
 
 NLIST_CASES = [
