@@ -332,6 +332,9 @@ int sx_reporter_set_config_index(sx_reporter *r, int config_index);
 int sx_reporter_flush(sx_reporter *r);        /* waits until every frame is written */
 const char *sx_reporter_error(const sx_reporter *r);
 int sx_reporter_close(sx_reporter *r);        /* close_*_file: flush, close, free   */
+/* A reporter fed by a context must be closed before that context is
+ * destroyed or given a different number of atoms by sx_load_checkpoint or
+ * sx_set_num_atoms while frames are in flight (sx_reporter_flush first). */
 /* load_checkpoint (reporters.f90:188-221): set_num_atoms, set_num_constraints,
  * set_box, then the atom and bond lines (the mass of a line goes to the atom
  * the line names, the other columns to the line's position; inv_mass and the
