@@ -131,6 +131,7 @@ int sx_generate_fcc_lattice(sx_ctx *ctx, int unit_cells, double dens, int rigid_
  * set_atom_mass/charge/shape/energy/position/velocity/inertia (system.f90:
  * 129-343) are one or three such calls */
 int sx_set_atom_values(sx_ctx *ctx, int field, int first1, int count, const double *values);
+int sx_get_atom_values(sx_ctx *ctx, int field, int first1, int count, double *values);
 /* set_atom_type (system.f90:109) for all n atoms at once; rtype is host
  * metadata (-1 until set; initialize_system turns negatives into 0), read by
  * the trajectory and checkpoint reporters */

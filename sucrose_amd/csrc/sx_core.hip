@@ -523,6 +523,22 @@ int sx_set_atom_values(sx_ctx *c, int field, int first1, int count, const double
     });
 }
 
+int sx_get_atom_values(sx_ctx *c, int field, int first1, int count, double *values)
+{
+    return guard(c, [&] {
+        need_system(c);
+        if (field < 0 || field >= SX_NFIELDS || count < 0 || (count > 0 && !values))
+            throw sx_error{SX_ERR_ARG, "bad field values"};
+        if (first1 < 1 || first1 - 1 + count > c->n)
+            throw sx_error{SX_ERR_ARG, "Error: Atom index " + std::to_string(first1) + " does not exist in system of " +
+                                           std::to_string(c->n) + " atoms."};
+        if (count == 0) return;
+        HIPCHK(hipMemcpyAsync(values, c->d_fields + (long)field * c->ld + (first1 - 1), sizeof(double) * count,
+                              hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
 int sx_set_atom_types(sx_ctx *c, const int *rtype)
 {
     return guard(c, [&] {
