@@ -617,9 +617,10 @@ __global__ void __launch_bounds__(NHT_ST) k_nhtp_scan(NhtBuf B, const double *sc
 {
     if (*B.conv) return;
     const int PER = NHT_C / NHT_ST;
-    __shared__ AffMap tot[NHT_ST];
+    const int NW = NHT_ST / SX_WAVE;
+    __shared__ AffMap wtot[NHT_ST / SX_WAVE];
     __shared__ double red[NHT_ST];
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t % SX_WAVE, w = t / SX_WAVE;
     AffMap loc[PER];
     for (int q = 0; q < PER; ++q) {
         const double *A = B.maps + (size_t)(t * PER + q) * 20;
@@ -628,22 +629,34 @@ __global__ void __launch_bounds__(NHT_ST) k_nhtp_scan(NhtBuf B, const double *sc
     }
     AffMap P = loc[0];
     for (int q = 1; q < PER; ++q) P = aff_compose(loc[q], P);
-    tot[t] = P;
-    __syncthreads();
-    // inclusive prefix over the threads (Hillis-Steele, later after earlier)
-    for (int o = 1; o < NHT_ST; o <<= 1) {
-        AffMap m2 = tot[t];
-        if (t >= o) m2 = aff_compose(m2, tot[t - o]);
-        __syncthreads();
-        tot[t] = m2;
-        __syncthreads();
+    // inclusive prefix (later after earlier): within each wavefront by
+    // shuffles, then over the wavefront totals through LDS
+    AffMap x = P;
+    for (int o = 1; o < SX_WAVE; o <<= 1) {
+        AffMap y;
+        for (int k = 0; k < 16; ++k) y.M[k] = __shfl_up(x.M[k], o, SX_WAVE);
+        for (int k = 0; k < 4; ++k) y.c[k] = __shfl_up(x.c[k], o, SX_WAVE);
+        if (lane >= o) x = aff_compose(x, y);
     }
+    if (lane == SX_WAVE - 1) wtot[w] = x;
+    __syncthreads();
+    AffMap pre;   // all wavefronts before this one, composed (identity for the first)
+    for (int k = 0; k < 16; ++k) pre.M[k] = (k % 5 == 0) ? 1.0 : 0.0;
+    for (int k = 0; k < 4; ++k) pre.c[k] = 0.0;
+    for (int q = 0; q < w && q < NW; ++q) pre = aff_compose(wtot[q], pre);
+    AffMap incl = aff_compose(x, pre);
+    // exclusive prefix of this thread: the previous lane's inclusive one, or
+    // for a wavefront's first lane the wavefronts before it
+    AffMap excl;
+    for (int k = 0; k < 16; ++k) excl.M[k] = __shfl_up(incl.M[k], 1, SX_WAVE);
+    for (int k = 0; k < 4; ++k) excl.c[k] = __shfl_up(incl.c[k], 1, SX_WAVE);
+    if (lane == 0) excl = pre;
     double s0[4] = {B.s0[0], B.s0[1], B.s0[2], B.s0[3]};
     double e[4];
     if (t == 0) {
         for (int k = 0; k < 4; ++k) e[k] = s0[k];
     } else {
-        aff_apply(tot[t - 1], s0, e);
+        aff_apply(excl, s0, e);
     }
     const double T = scal[SX_TARGET_TEMP];
     const double scale[4] = {sqrt(fabs(scal[SX_HB0] * T)), sqrt(fabs(scal[SX_HB1] * T)), sqrt(fabs(scal[SX_HB2] * T)),
