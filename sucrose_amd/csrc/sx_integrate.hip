@@ -362,20 +362,22 @@ __global__ void __launch_bounds__(NHT_CHUNK) k_nht(DState s, double dt, int phas
 
 // ---------------------------------------------------------------------------
 // The same chain recurrence evaluated in parallel (DEER: Newton iteration on
-// the whole sequence).  The particles are cut into NHT_C consecutive chunks,
-// one per thread, spread over many workgroups.  A chunk maps the chain state
-// entering it, s = (p_eta(1:3), sum of m v^2), to the state leaving it; a pass
-// with forward-mode dual numbers gives that map's value and 4x4 Jacobian at a
-// guessed entry state.  One workgroup composes the affine models of all chunks
-// with a prefix scan, which yields new entry states; the iteration converges
-// quadratically to the states the serial walk produces (three iterations at
-// equilibrium), and a last pass applies the updates.  The results equal the
-// serial recurrence to rounding (the particle updates are the same expressions
-// in the same order; only each chunk's entry state is reached through the
-// scan).  Each call is a fixed sequence of launches (graph-capturable):
-// prep, NHT_MAXIT iterations that return at once after convergence, final
-// pass, finish; without convergence the finish kernel walks the recurrence
-// serially.
+// the whole sequence).  The particles are cut into at most NHT_CMAX
+// consecutive chunks, one per lane, NHT_BLK chunks per workgroup.  A chunk
+// maps the chain state entering it, s = (p_eta(1:3), sum of m v^2), to the
+// state leaving it; a pass with forward-mode dual numbers gives that map's
+// value and 4x4 Jacobian at a guessed entry state.  The affine models of the
+// chunks are composed by a two-level prefix scan (a shuffle scan within each
+// workgroup, stored; the workgroup totals scanned at the start of the next
+// pass), which yields the next Newton iterate of every entry state; the
+// iteration converges quadratically to the states the serial walk produces
+// (a few passes at equilibrium), and a last pass applies the updates.  The
+// results equal the serial recurrence to rounding (the particle updates are
+// the same expressions in the same order; only each chunk's entry state is
+// reached through the scan).  Each call is a fixed sequence of launches
+// (graph-capturable): prep, NHT_MAXIT passes that return at once after
+// convergence, final; without convergence the final kernel walks the
+// recurrence serially.
 // ---------------------------------------------------------------------------
 struct D4 {
     double v, d[4];
@@ -480,9 +482,10 @@ __device__ __forceinline__ void nht_turn(int phase, double dt, double tq, double
     }
 }
 
-#define NHT_C 2048       // chunks
-#define NHT_ST 512       // threads of the scan and the reductions (NHT_C / NHT_ST chunks each)
-#define NHT_MAXIT 12
+#define NHT_CMAX 4096    // most chunks per call
+#define NHT_BLK 64       // chunks per workgroup (one wavefront), so at most 64 workgroups
+#define NHT_MAXIT 12     // passes per call (the first at the flat guess)
+#define NHT_TOL 1e-12
 struct AffMap {
     double M[16];   // row-major 4x4
     double c[4];
@@ -514,16 +517,62 @@ __device__ __forceinline__ void aff_apply(const AffMap &A, const double x[4], do
     }
 }
 
-// device scratch of the parallel evaluation: maps (20 per chunk), guesses
-// (4 per chunk), per-chunk sums (4), then s0 (4) and the end state (3)
+__device__ __forceinline__ AffMap aff_identity()
+{
+    AffMap r;
+    for (int k = 0; k < 16; ++k) r.M[k] = (k % 5 == 0) ? 1.0 : 0.0;
+    for (int k = 0; k < 4; ++k) r.c[k] = 0.0;
+    return r;
+}
+
+__device__ __forceinline__ AffMap aff_load(const double *A)
+{
+    AffMap r;
+    for (int k = 0; k < 16; ++k) r.M[k] = A[k];
+    for (int k = 0; k < 4; ++k) r.c[k] = A[16 + k];
+    return r;
+}
+
+__device__ __forceinline__ void aff_store(const AffMap &a, double *A)
+{
+    for (int k = 0; k < 16; ++k) A[k] = a.M[k];
+    for (int k = 0; k < 4; ++k) A[16 + k] = a.c[k];
+}
+
+// inclusive prefix over the lanes of a wavefront, later lanes after earlier
+__device__ __forceinline__ AffMap aff_wave_scan(AffMap x, int lane)
+{
+    for (int o = 1; o < SX_WAVE; o <<= 1) {
+        AffMap y;
+        for (int k = 0; k < 16; ++k) y.M[k] = __shfl_up(x.M[k], o, SX_WAVE);
+        for (int k = 0; k < 4; ++k) y.c[k] = __shfl_up(x.c[k], o, SX_WAVE);
+        if (lane >= o) x = aff_compose(x, y);
+    }
+    return x;
+}
+
+__device__ __forceinline__ AffMap aff_shfl(const AffMap &x, int src)
+{
+    AffMap r;
+    for (int k = 0; k < 16; ++k) r.M[k] = __shfl(x.M[k], src, SX_WAVE);
+    for (int k = 0; k < 4; ++k) r.c[k] = __shfl(x.c[k], src, SX_WAVE);
+    return r;
+}
+
+// device scratch of the parallel evaluation
 struct NhtBuf {
-    double *maps, *g, *part, *s0, *pend;
-    int *conv;
+    double *P[2];    // per chunk: composed models of the chunks before it in its workgroup (20), by pass parity
+    double *T[2];    // per workgroup: composed model of all its chunks (20), by pass parity
+    double *g;       // per chunk: entry state of the latest iterate (4)
+    double *part;    // per workgroup partial sums (4)
+    double *s0;      // chain state entering the call (4), then the state after the last particle (3)
+    unsigned long long *dev;   // per pass: largest scaled change of an entry state (bits of a double >= 0)
+    unsigned *ticket;          // last-workgroup ticket (resets itself)
+    int chunk, nb;             // particles per chunk, workgroups (chunks = nb * NHT_BLK)
 };
 
-__device__ __forceinline__ void nht_chunk(int n, int c, int &i0, int &i1)
+__device__ __forceinline__ void nht_chunk(int n, int chunk, int c, int &i0, int &i1)
 {
-    const int chunk = (n + NHT_C - 1) / NHT_C;
     i0 = min(n, c * chunk);
     i1 = min(n, i0 + chunk);
 }
@@ -543,49 +592,98 @@ __device__ __forceinline__ NhtArgs nht_args(const double *scal, double dt, doubl
     return a;
 }
 
-// sum of m |v|^2 before the call (strided + tree order), the chain state
-// entering the call, flat first guesses
-__global__ void __launch_bounds__(NHT_ST) k_nhtp_prep(DState s, const double *scal, NhtBuf B,
-                                                      unsigned long long *stats)
+// converged by pass k: some earlier pass j >= 1 moved no entry state by more
+// than the tolerance (a pass that returned early leaves its slot at zero,
+// which only happens after an earlier one converged)
+__device__ __forceinline__ bool nht_converged(const NhtBuf &B, int k)
 {
-    __shared__ double red[NHT_ST];
-    const int t = threadIdx.x, n = s.n;
-    double acc = 0.0;
-    for (int i = t; i < n; i += NHT_ST) {
-        double vx = s.f[SX_VX][i], vy = s.f[SX_VY][i], vz = s.f[SX_VZ][i];
-        acc += s.f[SX_MASS][i] * (vx * vx + vy * vy + vz * vz);
+    for (int j = 1; j < k; ++j) {
+        unsigned long long u = __hip_atomic_load(&B.dev[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__longlong_as_double((long long)u) < NHT_TOL) return true;
     }
-    red[t] = acc;
-    __syncthreads();
-    for (int o = NHT_ST / 2; o > 0; o >>= 1) {
-        if (t < o) red[t] = red[t] + red[t + o];
-        __syncthreads();
-    }
-    const double s0[4] = {scal[SX_PETA0], scal[SX_PETA1], scal[SX_PETA2], red[0]};
-    for (int c = t; c < NHT_C; c += NHT_ST)
-        for (int k = 0; k < 4; ++k) B.g[c * 4 + k] = s0[k];
-    if (t == 0) {
-        for (int k = 0; k < 4; ++k) B.s0[k] = s0[k];
-        *B.conv = 0;
-        if (stats) atomicAdd(&stats[1], 1ull);
-    }
+    return false;
 }
 
-// one chunk per thread: dual-number pass from the guessed entry state ->
-// the chunk's affine model x -> J x + (E - J g)
-__global__ void __launch_bounds__(64) k_nhtp_pass(DState s, double dt, int phase, const double *scal, double ndof,
-                                                  NhtBuf B)
+// sum of m |v|^2 before the call (each workgroup over its chunks' particles,
+// then the workgroup sums in a fixed order), the chain state entering the
+// call; clears the per-pass change slots
+__global__ void __launch_bounds__(NHT_BLK) k_nhtp_prep(DState s, const double *scal, NhtBuf B,
+                                                       unsigned long long *stats)
 {
-    if (*B.conv) return;
-    const int c = blockIdx.x * 64 + threadIdx.x;
-    if (c >= NHT_C) return;
+    const int b = blockIdx.x, t = threadIdx.x, n = s.n;
+    const int i0 = min(n, b * NHT_BLK * B.chunk), i1 = min(n, i0 + NHT_BLK * B.chunk);
+    double v[1] = {0.0};
+    for (int i = i0 + t; i < i1; i += NHT_BLK) {
+        double vx = s.f[SX_VX][i], vy = s.f[SX_VY][i], vz = s.f[SX_VZ][i];
+        v[0] += s.f[SX_MASS][i] * (vx * vx + vy * vy + vz * vz);
+    }
+    double tot[1];
+    if (!sx_block_reduce_last<1, NHT_BLK>(v, B.part, B.ticket, tot)) return;
+    if (t == 0) {
+        B.s0[0] = scal[SX_PETA0]; B.s0[1] = scal[SX_PETA1]; B.s0[2] = scal[SX_PETA2];
+        B.s0[3] = tot[0];
+        if (stats) atomicAdd(&stats[1], 1ull);
+    }
+    if (t < NHT_MAXIT) B.dev[t] = 0ull;
+}
+
+// pass k, one chunk per lane.  For k >= 1 the chunk's entry state is first
+// advanced to the Newton iterate given by the chunk models of pass k - 1:
+// the workgroup totals are composed by a shuffle scan (every workgroup
+// redundantly), the result applied to the state entering the call gives this
+// workgroup's entry state, and the stored within-workgroup prefix gives the
+// chunk's; the largest scaled change of an entry state over the grid goes to
+// the pass's slot (converged when below 1e-12 of the magnitude plus the
+// thermal scale, sqrt(Q_k T) for p_eta(k) and n_dof T for sum m v^2; a NaN
+// never converges).  Then a dual-number walk over the chunk's particles gives
+// its affine model x -> J x + (E - J g), and a shuffle scan over the
+// workgroup stores each chunk's prefix and the workgroup total.
+__global__ void __launch_bounds__(NHT_BLK) k_nhtp_pass(DState s, double dt, int phase, const double *scal,
+                                                       double ndof, NhtBuf B, int k, unsigned long long *stats)
+{
+    if (nht_converged(B, k)) return;
+    const int b = blockIdx.x, lane = threadIdx.x, c = b * NHT_BLK + lane;
+    const double s0[4] = {B.s0[0], B.s0[1], B.s0[2], B.s0[3]};
+    double g[4];
+    if (k == 0) {
+        for (int q = 0; q < 4; ++q) g[q] = s0[q];
+    } else {
+        const int pk = (k - 1) & 1;
+        AffMap x = lane < B.nb ? aff_load(B.T[pk] + (size_t)lane * 20) : aff_identity();
+        x = aff_wave_scan(x, lane);
+        double E[4];
+        if (b == 0) {
+            for (int q = 0; q < 4; ++q) E[q] = s0[q];
+        } else {
+            AffMap pre = aff_shfl(x, b - 1);
+            aff_apply(pre, s0, E);
+        }
+        AffMap Pc = aff_load(B.P[pk] + (size_t)c * 20);
+        aff_apply(Pc, E, g);
+        const double T = scal[SX_TARGET_TEMP];
+        const double scale[4] = {sqrt(fabs(scal[SX_HB0] * T)), sqrt(fabs(scal[SX_HB1] * T)),
+                                 sqrt(fabs(scal[SX_HB2] * T)), fabs(ndof * T)};
+        double dv = 0.0;
+        for (int q = 0; q < 4; ++q) {
+            double old = (k == 1) ? s0[q] : B.g[c * 4 + q];
+            double d = fabs(g[q] - old) / (fabs(g[q]) + scale[q] + 1e-300);
+            dv = (d == d && dv == dv) ? fmax(dv, d) : INFINITY;
+            B.g[c * 4 + q] = g[q];
+        }
+        for (int o = SX_WAVE / 2; o > 0; o >>= 1) {
+            double y = __shfl_xor(dv, o, SX_WAVE);
+            dv = (y == y && dv == dv) ? fmax(dv, y) : INFINITY;
+        }
+        if (lane == 0) {
+            atomicMax(&B.dev[k], (unsigned long long)__double_as_longlong(dv));
+            if (b == 0 && stats) atomicAdd(&stats[0], 1ull);
+        }
+    }
     const NhtArgs a = nht_args(scal, dt, ndof);
     int i0, i1;
-    nht_chunk(s.n, c, i0, i1);
-    double g[4];
-    for (int k = 0; k < 4; ++k) g[k] = B.g[c * 4 + k];
+    nht_chunk(s.n, B.chunk, c, i0, i1);
     D4 p[3], sv;
-    for (int k = 0; k < 3; ++k) { p[k] = dd(g[k]); p[k].d[k] = 1.0; }
+    for (int q = 0; q < 3; ++q) { p[q] = dd(g[q]); p[q].d[q] = 1.0; }
     sv = dd(g[3]);
     sv.d[3] = 1.0;
     for (int i = i0; i < i1; ++i) {
@@ -597,108 +695,64 @@ __global__ void __launch_bounds__(64) k_nhtp_pass(DState s, double dt, int phase
         nht_turn(phase, dt, a.tq, a.ndofT, a.Temp, a.hb, m, acc3, p, sv, v, ea);
     }
     const D4 *out[4] = {&p[0], &p[1], &p[2], &sv};
-    double *A = B.maps + (size_t)c * 20;
+    AffMap A;
     for (int r = 0; r < 4; ++r) {
         double cr = out[r]->v;
-        for (int k = 0; k < 4; ++k) {
-            A[r * 4 + k] = out[r]->d[k];
-            cr -= out[r]->d[k] * g[k];
+        for (int q = 0; q < 4; ++q) {
+            A.M[r * 4 + q] = out[r]->d[q];
+            cr -= out[r]->d[q] * g[q];
         }
-        A[16 + r] = cr;
+        A.c[r] = cr;
     }
+    AffMap incl = aff_wave_scan(A, lane);
+    AffMap excl = aff_shfl(incl, lane > 0 ? lane - 1 : 0);
+    if (lane == 0) excl = aff_identity();
+    aff_store(excl, B.P[k & 1] + (size_t)c * 20);
+    if (lane == NHT_BLK - 1) aff_store(incl, B.T[k & 1] + (size_t)b * 20);
 }
 
-// prefix of the chunk models -> new entry states; converged when no entry
-// state component moved by more than 1e-12 of its magnitude plus its thermal
-// scale (sqrt(Q_k T) for p_eta(k), n_dof T for sum m v^2), so a p_eta passing
-// through zero does not stall the test; a NaN never counts as converged
-__global__ void __launch_bounds__(NHT_ST) k_nhtp_scan(NhtBuf B, const double *scal, double ndof,
-                                                      unsigned long long *stats)
+// converged: the particle updates of every chunk from the converged entry
+// state, then the sums of the eta increments and kinetic energy (fixed
+// order) and the chain state after the last particle.  Not converged: one
+// lane walks the whole recurrence from the state entering the call.
+__global__ void __launch_bounds__(NHT_BLK) k_nhtp_final(DState s, double dt, int phase, int save, double *scal,
+                                                        double ndof, NhtBuf B, unsigned long long *stats)
 {
-    if (*B.conv) return;
-    const int PER = NHT_C / NHT_ST;
-    const int NW = NHT_ST / SX_WAVE;
-    __shared__ AffMap wtot[NHT_ST / SX_WAVE];
-    __shared__ double red[NHT_ST];
-    const int t = threadIdx.x, lane = t % SX_WAVE, w = t / SX_WAVE;
-    AffMap loc[PER];
-    for (int q = 0; q < PER; ++q) {
-        const double *A = B.maps + (size_t)(t * PER + q) * 20;
-        for (int k = 0; k < 16; ++k) loc[q].M[k] = A[k];
-        for (int k = 0; k < 4; ++k) loc[q].c[k] = A[16 + k];
-    }
-    AffMap P = loc[0];
-    for (int q = 1; q < PER; ++q) P = aff_compose(loc[q], P);
-    // inclusive prefix (later after earlier): within each wavefront by
-    // shuffles, then over the wavefront totals through LDS
-    AffMap x = P;
-    for (int o = 1; o < SX_WAVE; o <<= 1) {
-        AffMap y;
-        for (int k = 0; k < 16; ++k) y.M[k] = __shfl_up(x.M[k], o, SX_WAVE);
-        for (int k = 0; k < 4; ++k) y.c[k] = __shfl_up(x.c[k], o, SX_WAVE);
-        if (lane >= o) x = aff_compose(x, y);
-    }
-    if (lane == SX_WAVE - 1) wtot[w] = x;
-    __syncthreads();
-    AffMap pre;   // all wavefronts before this one, composed (identity for the first)
-    for (int k = 0; k < 16; ++k) pre.M[k] = (k % 5 == 0) ? 1.0 : 0.0;
-    for (int k = 0; k < 4; ++k) pre.c[k] = 0.0;
-    for (int q = 0; q < w && q < NW; ++q) pre = aff_compose(wtot[q], pre);
-    AffMap incl = aff_compose(x, pre);
-    // exclusive prefix of this thread: the previous lane's inclusive one, or
-    // for a wavefront's first lane the wavefronts before it
-    AffMap excl;
-    for (int k = 0; k < 16; ++k) excl.M[k] = __shfl_up(incl.M[k], 1, SX_WAVE);
-    for (int k = 0; k < 4; ++k) excl.c[k] = __shfl_up(incl.c[k], 1, SX_WAVE);
-    if (lane == 0) excl = pre;
-    double s0[4] = {B.s0[0], B.s0[1], B.s0[2], B.s0[3]};
-    double e[4];
-    if (t == 0) {
-        for (int k = 0; k < 4; ++k) e[k] = s0[k];
-    } else {
-        aff_apply(excl, s0, e);
-    }
-    const double T = scal[SX_TARGET_TEMP];
-    const double scale[4] = {sqrt(fabs(scal[SX_HB0] * T)), sqrt(fabs(scal[SX_HB1] * T)), sqrt(fabs(scal[SX_HB2] * T)),
-                             fabs(ndof * T)};
-    double dev = 0.0;
-    for (int q = 0; q < PER; ++q) {
-        const int c = t * PER + q;
-        for (int k = 0; k < 4; ++k) {
-            double d = fabs(e[k] - B.g[c * 4 + k]) / (fabs(e[k]) + scale[k] + 1e-300);
-            dev = (d == d && dev == dev) ? fmax(dev, d) : INFINITY;
-            B.g[c * 4 + k] = e[k];
-        }
-        if (q + 1 < PER) {
-            double nx[4];
-            aff_apply(loc[q], e, nx);
-            for (int k = 0; k < 4; ++k) e[k] = nx[k];
-        }
-    }
-    red[t] = dev;
-    __syncthreads();
-    for (int o = NHT_ST / 2; o > 0; o >>= 1) {
-        if (t < o) red[t] = fmax(red[t], red[t + o]);
-        __syncthreads();
-    }
-    if (t == 0) {
-        if (stats) atomicAdd(&stats[0], 1ull);
-        if (red[0] < 1e-12) *B.conv = 1;
-    }
-}
-
-// the particle updates of every chunk from its converged entry state
-__global__ void __launch_bounds__(64) k_nhtp_final(DState s, double dt, int phase, int save, const double *scal,
-                                                   double ndof, NhtBuf B)
-{
-    if (!*B.conv) return;
-    const int c = blockIdx.x * 64 + threadIdx.x;
-    if (c >= NHT_C) return;
+    const int b = blockIdx.x, t = threadIdx.x, c = b * NHT_BLK + t;
     const NhtArgs a = nht_args(scal, dt, ndof);
+    if (!nht_converged(B, NHT_MAXIT)) {
+        if (b != 0 || t != 0) return;
+        if (stats) atomicAdd(&stats[2], 1ull);
+        double p[3] = {B.s0[0], B.s0[1], B.s0[2]}, sv = B.s0[3];
+        double eta_loc[3] = {0.0, 0.0, 0.0}, ke = 0.0;
+        for (int i = 0; i < s.n; ++i) {
+            double m = s.f[SX_MASS][i], im = s.f[SX_INV_MASS][i];
+            double acc3[3] = {0.5 * dt * (s.f[SX_FX][i] * im), 0.5 * dt * (s.f[SX_FY][i] * im),
+                              0.5 * dt * (s.f[SX_FZ][i] * im)};
+            double v[3] = {s.f[SX_VX][i], s.f[SX_VY][i], s.f[SX_VZ][i]};
+            double ea[3];
+            nht_turn(phase, dt, a.tq, a.ndofT, a.Temp, a.hb, m, acc3, p, sv, v, ea);
+            for (int q = 0; q < 3; ++q) eta_loc[q] += ea[q];
+            s.f[SX_VX][i] = v[0]; s.f[SX_VY][i] = v[1]; s.f[SX_VZ][i] = v[2];
+            if (phase == 0) {
+                if (save) save_old(s, i);
+                s.f[SX_RX][i] = s.f[SX_RX][i] + dt * v[0];
+                s.f[SX_RY][i] = s.f[SX_RY][i] + dt * v[1];
+                s.f[SX_RZ][i] = s.f[SX_RZ][i] + dt * v[2];
+                wrap_pos(s, i);
+            } else {
+                ke += 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            }
+        }
+        for (int q = 0; q < 3; ++q) scal[SX_ETA0 + q] = scal[SX_ETA0 + q] + eta_loc[q];
+        if (phase == 1) scal[SX_KIN_ENG] = ke;
+        scal[SX_PETA0] = p[0]; scal[SX_PETA1] = p[1]; scal[SX_PETA2] = p[2];
+        return;
+    }
     int i0, i1;
-    nht_chunk(s.n, c, i0, i1);
+    nht_chunk(s.n, B.chunk, c, i0, i1);
     double p[3] = {B.g[c * 4 + 0], B.g[c * 4 + 1], B.g[c * 4 + 2]}, sv = B.g[c * 4 + 3];
-    double eta_loc[3] = {0.0, 0.0, 0.0}, ke = 0.0;
+    double v4[4] = {0.0, 0.0, 0.0, 0.0};   // eta increments, kinetic energy
     for (int i = i0; i < i1; ++i) {
         double m = s.f[SX_MASS][i], im = s.f[SX_INV_MASS][i];
         double acc3[3] = {0.5 * dt * (s.f[SX_FX][i] * im), 0.5 * dt * (s.f[SX_FY][i] * im),
@@ -706,7 +760,7 @@ __global__ void __launch_bounds__(64) k_nhtp_final(DState s, double dt, int phas
         double v[3] = {s.f[SX_VX][i], s.f[SX_VY][i], s.f[SX_VZ][i]};
         double ea[3];
         nht_turn(phase, dt, a.tq, a.ndofT, a.Temp, a.hb, m, acc3, p, sv, v, ea);
-        for (int k = 0; k < 3; ++k) eta_loc[k] += ea[k];
+        for (int q = 0; q < 3; ++q) v4[q] += ea[q];
         s.f[SX_VX][i] = v[0]; s.f[SX_VY][i] = v[1]; s.f[SX_VZ][i] = v[2];
         if (phase == 0) {
             if (save) save_old(s, i);
@@ -715,72 +769,22 @@ __global__ void __launch_bounds__(64) k_nhtp_final(DState s, double dt, int phas
             s.f[SX_RZ][i] = s.f[SX_RZ][i] + dt * v[2];
             wrap_pos(s, i);
         } else {
-            ke += 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            v4[3] += 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
         }
     }
-    for (int k = 0; k < 3; ++k) B.part[c * 4 + k] = eta_loc[k];
-    B.part[c * 4 + 3] = ke;
     // the chain state after the last particle belongs to the last non-empty chunk
-    if (i1 == s.n && i0 < s.n)
-        for (int k = 0; k < 3; ++k) B.pend[k] = p[k];
-}
-
-// converged: sums of the chunks' eta increments and kinetic energy (fixed
-// order), the chain state after the last particle.  Not converged: one lane
-// walks the whole recurrence from the state entering the call.
-__global__ void __launch_bounds__(NHT_ST) k_nhtp_finish(DState s, double dt, int phase, int save, double *scal,
-                                                        double ndof, NhtBuf B, unsigned long long *stats)
-{
-    __shared__ double red[NHT_ST];
-    __shared__ double tot[4];
-    const int t = threadIdx.x;
-    if (*B.conv) {
-        for (int k = 0; k < 4; ++k) {
-            double acc = 0.0;
-            for (int c = t; c < NHT_C; c += NHT_ST) acc += B.part[c * 4 + k];
-            red[t] = acc;
-            __syncthreads();
-            for (int o = NHT_ST / 2; o > 0; o >>= 1) {
-                if (t < o) red[t] = red[t] + red[t + o];
-                __syncthreads();
-            }
-            if (t == 0) tot[k] = red[0];
-            __syncthreads();
-        }
-        if (t == 0) {
-            for (int k = 0; k < 3; ++k) scal[SX_ETA0 + k] = scal[SX_ETA0 + k] + tot[k];
-            if (phase == 1) scal[SX_KIN_ENG] = tot[3];
-            scal[SX_PETA0] = B.pend[0]; scal[SX_PETA1] = B.pend[1]; scal[SX_PETA2] = B.pend[2];
-        }
-        return;
+    if (i1 == s.n && i0 < s.n) {
+        for (int q = 0; q < 3; ++q) B.s0[4 + q] = p[q];
+        __threadfence();
     }
-    if (t != 0) return;
-    if (stats) atomicAdd(&stats[2], 1ull);
-    const NhtArgs a = nht_args(scal, dt, ndof);
-    double p[3] = {B.s0[0], B.s0[1], B.s0[2]}, sv = B.s0[3];
-    double eta_loc[3] = {0.0, 0.0, 0.0}, ke = 0.0;
-    for (int i = 0; i < s.n; ++i) {
-        double m = s.f[SX_MASS][i], im = s.f[SX_INV_MASS][i];
-        double acc3[3] = {0.5 * dt * (s.f[SX_FX][i] * im), 0.5 * dt * (s.f[SX_FY][i] * im),
-                          0.5 * dt * (s.f[SX_FZ][i] * im)};
-        double v[3] = {s.f[SX_VX][i], s.f[SX_VY][i], s.f[SX_VZ][i]};
-        double ea[3];
-        nht_turn(phase, dt, a.tq, a.ndofT, a.Temp, a.hb, m, acc3, p, sv, v, ea);
-        for (int k = 0; k < 3; ++k) eta_loc[k] += ea[k];
-        s.f[SX_VX][i] = v[0]; s.f[SX_VY][i] = v[1]; s.f[SX_VZ][i] = v[2];
-        if (phase == 0) {
-            if (save) save_old(s, i);
-            s.f[SX_RX][i] = s.f[SX_RX][i] + dt * v[0];
-            s.f[SX_RY][i] = s.f[SX_RY][i] + dt * v[1];
-            s.f[SX_RZ][i] = s.f[SX_RZ][i] + dt * v[2];
-            wrap_pos(s, i);
-        } else {
-            ke += 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-        }
+    double tot[4];
+    if (!sx_block_reduce_last<4, NHT_BLK>(v4, B.part, B.ticket, tot)) return;
+    if (t == 0) {
+        for (int q = 0; q < 3; ++q) scal[SX_ETA0 + q] = scal[SX_ETA0 + q] + tot[q];
+        if (phase == 1) scal[SX_KIN_ENG] = tot[3];
+        for (int q = 0; q < 3; ++q)
+            scal[SX_PETA0 + q] = __hip_atomic_load(&B.s0[4 + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    for (int k = 0; k < 3; ++k) scal[SX_ETA0 + k] = scal[SX_ETA0 + k] + eta_loc[k];
-    if (phase == 1) scal[SX_KIN_ENG] = ke;
-    scal[SX_PETA0] = p[0]; scal[SX_PETA1] = p[1]; scal[SX_PETA2] = p[2];
 }
 
 __global__ void k_nht_hb(double *scal, double ndof, double coupling, int has_coupling)
@@ -1139,28 +1143,34 @@ void sx_nht_impl(sx_ctx *c, double dt, const double *coupling, int phase, bool s
         hipLaunchKernelGGL(k_nht, dim3(1), dim3(NHT_CHUNK), 0, c->stream, c->dstate(), dt, phase, save ? 1 : 0,
                            c->d_scal, ndof, c->d_part);
     } else {
+        // 2 x chunk prefixes, 2 x workgroup totals, entry states, partial
+        // sums, s0 + end state, per-pass change slots, then the ticket
+        const size_t nP = (size_t)NHT_CMAX * 20, nT = (size_t)(NHT_CMAX / NHT_BLK) * 20;
+        const size_t nd = 2 * nP + 2 * nT + (size_t)NHT_CMAX * 4 + (NHT_CMAX / NHT_BLK) * 4 + 8 + 16;
         if (!c->d_nht) {
-            const size_t nd = (size_t)NHT_C * (20 + 4 + 4) + 8;
-            HIPCHK(hipMalloc(&c->d_nht, sizeof(double) * nd + sizeof(int) * 4));
+            HIPCHK(hipMalloc(&c->d_nht, sizeof(double) * nd + sizeof(unsigned) * 4));
+            HIPCHK(hipMemsetAsync(c->d_nht + nd, 0, sizeof(unsigned) * 4, c->stream));
         }
         NhtBuf B;
-        B.maps = c->d_nht;
-        B.g = B.maps + (size_t)NHT_C * 20;
-        B.part = B.g + (size_t)NHT_C * 4;
-        B.s0 = B.part + (size_t)NHT_C * 4;
-        B.pend = B.s0 + 4;
-        B.conv = (int *)(B.s0 + 8);
+        B.P[0] = c->d_nht;
+        B.P[1] = B.P[0] + nP;
+        B.T[0] = B.P[1] + nP;
+        B.T[1] = B.T[0] + nT;
+        B.g = B.T[1] + nT;
+        B.part = B.g + (size_t)NHT_CMAX * 4;
+        B.s0 = B.part + (NHT_CMAX / NHT_BLK) * 4;
+        B.dev = (unsigned long long *)(B.s0 + 8);
+        B.ticket = (unsigned *)(c->d_nht + nd);
+        B.chunk = (c->n + NHT_CMAX - 1) / NHT_CMAX;
+        const int nchunks = (c->n + B.chunk - 1) / B.chunk;
+        B.nb = (nchunks + NHT_BLK - 1) / NHT_BLK;
         unsigned long long *st = (unsigned long long *)(c->d_counts + 4);
         DState s = c->dstate();
-        const int nb = NHT_C / 64;
-        hipLaunchKernelGGL(k_nhtp_prep, dim3(1), dim3(NHT_ST), 0, c->stream, s, c->d_scal, B, st);
-        for (int it = 0; it < NHT_MAXIT; ++it) {
-            hipLaunchKernelGGL(k_nhtp_pass, dim3(nb), dim3(64), 0, c->stream, s, dt, phase, c->d_scal, ndof, B);
-            hipLaunchKernelGGL(k_nhtp_scan, dim3(1), dim3(NHT_ST), 0, c->stream, B, c->d_scal, ndof, st);
-        }
-        hipLaunchKernelGGL(k_nhtp_final, dim3(nb), dim3(64), 0, c->stream, s, dt, phase, save ? 1 : 0, c->d_scal,
-                           ndof, B);
-        hipLaunchKernelGGL(k_nhtp_finish, dim3(1), dim3(NHT_ST), 0, c->stream, s, dt, phase, save ? 1 : 0,
+        hipLaunchKernelGGL(k_nhtp_prep, dim3(B.nb), dim3(NHT_BLK), 0, c->stream, s, c->d_scal, B, st);
+        for (int k = 0; k < NHT_MAXIT; ++k)
+            hipLaunchKernelGGL(k_nhtp_pass, dim3(B.nb), dim3(NHT_BLK), 0, c->stream, s, dt, phase, c->d_scal, ndof,
+                               B, k, st);
+        hipLaunchKernelGGL(k_nhtp_final, dim3(B.nb), dim3(NHT_BLK), 0, c->stream, s, dt, phase, save ? 1 : 0,
                            c->d_scal, ndof, B, st);
     }
 }

@@ -490,6 +490,57 @@ def test_run_graph_langevin(dev, oracle, kind, params, rc):
         assert abs(got[s] - ref[s]) <= 1e-9 * max(1.0, abs(ref[s])), (s, got[s], ref[s])
 
 
+def test_run_graph_nose_hoover(dev, oracle):
+    """sx_run (graph replay) with the Nose-Hoover chain + quaternion steps on
+    2744 particles, so that both chain calls of every step take the parallel
+    (Newton-scan) kernels inside the captured graph, vs the oracle's README
+    loop (serial chain walk), 10 steps, no serial fallback."""
+    import sucrose_amd as sx
+    say("run graph Nose-Hoover n=2744")
+    rng = np.random.default_rng(13)
+    m, L = 14, 15.08
+    n = m ** 3
+    st = base_state(rng, n, L, forces=False, T=1.0)
+    g = np.arange(m) * (L / m) - L / 2
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1) + rng.normal(size=(n, 3)) * 0.03
+    st["rx"], st["ry"], st["rz"] = P[:, 0], P[:, 1], P[:, 2]
+    s2 = np.sum(st["mass"] * (st["vx"] ** 2 + st["vy"] ** 2 + st["vz"] ** 2))
+    f = np.sqrt((3 * n - 3) * 1.0 / s2)
+    for c in ("vx", "vy", "vz"):
+        st[c] = st[c] * f
+    to_device(dev, st)
+    dev.init_nlist(0.3, 2.5)
+    dev.initialize_forces()
+    dev.forces(LJ, 1.0, 1.0, 2.5)
+    ref = from_device(dev, st)
+    nl = oracle.nlist_new(ref, 2.5, 0.3)
+    oracle.nlist_build(ref, nl)
+    nsteps, dt = 10, 0.002
+    it0, calls0, fb0 = dev.nht_stats()
+    dev.run(sx.INT_NHT, [(LJ, 1.0, 1.0, 2.5)], nsteps, dt, first_step=1)
+    it1, calls1, fb1 = dev.nht_stats()
+    for t in range(nsteps):
+        oracle.integrate(NHT, 0, ref, dt)
+        oracle.integrate(QQ, 0, ref, dt)
+        if oracle.needs_rebuild(ref, nl):
+            oracle.nlist_build(ref, nl)
+        oracle.call("or_initialize_forces", ref)
+        oracle.pair_loop(ref, nl, LJ, [1.0, 1.0, 2.5])
+        oracle.integrate(NHT, 1, ref, dt)
+        oracle.integrate(QQ, 1, ref, dt)
+        oracle.call("or_calculate_thermo", ref)
+    got = from_device(dev, st)
+    say(f"  {calls1 - calls0} chain calls, {it1 - it0} Newton passes, {fb1 - fb0} serial fallbacks")
+    assert calls1 - calls0 == 2 * nsteps and fb1 == fb0
+    for f in ("rx", "vx", "vz", "qw", "lx", "fx"):
+        assert rel_err(got[f], ref[f]) < 1e-9, (f, rel_err(got[f], ref[f]))
+    for s in ("peta0", "peta1", "peta2", "eta0", "eta2"):
+        assert abs(got[s] - ref[s]) <= TOL_CHAIN * max(1.0, abs(ref[s])), (s, got[s], ref[s])
+    for s in ("pot_eng", "kin_eng", "kin_rot"):
+        assert abs(got[s] - ref[s]) <= 1e-9 * max(1.0, abs(ref[s])), (s, got[s], ref[s])
+
+
 def test_mc_barostat(dev, oracle):
     """sx_mc_barostat vs the oracle's mc_barostat (integrators.f90:864-945)
     with the same two uniform numbers; compute_potential is the Lennard-Jones
