@@ -105,9 +105,9 @@ __device__ __forceinline__ void batch_rij(const double *tab, int ld, int jq, boo
                                           double r[3])
 {
     const double xj = tab[TB_X * ld + jq], yj = tab[TB_Y * ld + jq], zj = tab[TB_Z * ld + jq];
-    r[0] = sx_mimg(fwd ? me.x - xj : xj - me.x, L);
-    r[1] = sx_mimg(fwd ? me.y - yj : yj - me.y, L);
-    r[2] = sx_mimg(fwd ? me.z - zj : zj - me.z, L);
+    r[0] = sx_mimg_div(fwd ? me.x - xj : xj - me.x, L);
+    r[1] = sx_mimg_div(fwd ? me.y - yj : yj - me.y, L);
+    r[2] = sx_mimg_div(fwd ? me.z - zj : zj - me.z, L);
 }
 
 // false only where the pair's force, torques and energy are exactly zero:

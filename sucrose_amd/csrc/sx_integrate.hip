@@ -897,6 +897,118 @@ __global__ void __launch_bounds__(64) k_rattle(DState s, double dt, int nmol, co
     if (lane == 0) vir_part[blockIdx.x] = vir;
 }
 
+// The same sweeps for molecules of at most 64 bonds, with the serial bond
+// loop kept off the LDS crossbar: lane q holds bond q's atom lanes, length,
+// inverse masses and the bond vector that is fixed during the phase (old
+// positions in phase 0, positions in phase 1), the moved flags are one
+// wave-uniform 64-bit mask, and a bond's values are read from their lanes
+// with readlane at a uniform index.  The same expressions in the same order
+// as k_rattle, so the same results.
+__device__ __forceinline__ double rl_d(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <int PHASE>
+__global__ void __launch_bounds__(64) k_rattle64(DState s, double dt, int nmol, const int *atom_start,
+                                                const int *atoms, const int *bond_start, const int *bond_a,
+                                                const int *bond_b, const double *bond_len, double *vir_part,
+                                                int *err)
+{
+    const int m = blockIdx.x, lane = threadIdx.x;
+    double vir = 0.0;
+    if (m < nmol) {
+        const int a0 = atom_start[m], na = atom_start[m + 1] - a0;
+        const int b0 = bond_start[m], nb = bond_start[m + 1] - b0;
+        const int g = lane < na ? atoms[a0 + lane] : 0;
+        double rx = 0, ry = 0, rz = 0, vx = 0, vy = 0, vz = 0, ox = 0, oy = 0, oz = 0, im = 0;
+        if (lane < na) {
+            rx = s.f[SX_RX][g]; ry = s.f[SX_RY][g]; rz = s.f[SX_RZ][g];
+            vx = s.f[SX_VX][g]; vy = s.f[SX_VY][g]; vz = s.f[SX_VZ][g];
+            ox = s.f[SX_RX_OLD][g]; oy = s.f[SX_RY_OLD][g]; oz = s.f[SX_RZ_OLD][g];
+            im = s.f[SX_INV_MASS][g];
+        }
+        int qa = 0, qb = 0;
+        double qlen = 0.0;
+        if (lane < nb) { qa = bond_a[b0 + lane]; qb = bond_b[b0 + lane]; qlen = bond_len[b0 + lane]; }
+        const double qima = shfl_d(im, qa), qimb = shfl_d(im, qb);
+        double Q[3] = {PHASE == 0 ? shfl_d(ox, qa) - shfl_d(ox, qb) : shfl_d(rx, qa) - shfl_d(rx, qb),
+                       PHASE == 0 ? shfl_d(oy, qa) - shfl_d(oy, qb) : shfl_d(ry, qa) - shfl_d(ry, qb),
+                       PHASE == 0 ? shfl_d(oz, qa) - shfl_d(oz, qb) : shfl_d(rz, qa) - shfl_d(rz, qb)};
+        if (s.periodic)
+            for (int k = 0; k < 3; ++k) Q[k] = sx_mimg(Q[k], s.box[k]);
+        unsigned long long mv = __ballot(lane < na);
+        const double r_tol = 1.0e-5;
+        int failed = 0;
+        for (int rattle = 0; rattle < 100; ++rattle) {
+            if (mv == 0ull) break;
+            for (int q = 0; q < nb; ++q) {
+                const int la = __builtin_amdgcn_readlane(qa, q), lb = __builtin_amdgcn_readlane(qb, q);
+                const unsigned long long bits = (1ull << la) | (1ull << lb);
+                if (!(mv & bits)) continue;
+                const double len = rl_d(qlen, q), ima = rl_d(qima, q), imb = rl_d(qimb, q);
+                if (PHASE == 0) {
+                    double R[3] = {rl_d(rx, la) - rl_d(rx, lb), rl_d(ry, la) - rl_d(ry, lb),
+                                   rl_d(rz, la) - rl_d(rz, lb)};
+                    if (s.periodic)
+                        for (int k = 0; k < 3; ++k) R[k] = sx_mimg(R[k], s.box[k]);
+                    double chi = len * len - (R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+                    if (fabs(chi) > 2 * r_tol * (len * len)) {
+                        const double Ro[3] = {rl_d(Q[0], q), rl_d(Q[1], q), rl_d(Q[2], q)};
+                        double vdot = R[0] * Ro[0] + R[1] * Ro[1] + R[2] * Ro[2];
+                        if (vdot < r_tol * (len * len)) { failed = 1; break; }
+                        double L = chi / (2.0 * vdot * (ima + imb));
+                        vir = vir + (L * (len * len)) / (dt * dt);
+                        if (lane == la) {
+                            double c = L * ima;
+                            rx = rx + c * Ro[0]; ry = ry + c * Ro[1]; rz = rz + c * Ro[2];
+                            vx = vx + c * Ro[0] / dt; vy = vy + c * Ro[1] / dt; vz = vz + c * Ro[2] / dt;
+                        }
+                        if (lane == lb) {
+                            double c = L * imb;
+                            rx = rx - c * Ro[0]; ry = ry - c * Ro[1]; rz = rz - c * Ro[2];
+                            vx = vx - c * Ro[0] / dt; vy = vy - c * Ro[1] / dt; vz = vz - c * Ro[2] / dt;
+                        }
+                        mv |= bits;
+                    } else {
+                        mv &= ~bits;
+                    }
+                } else {
+                    const double R[3] = {rl_d(Q[0], q), rl_d(Q[1], q), rl_d(Q[2], q)};
+                    double V[3] = {rl_d(vx, la) - rl_d(vx, lb), rl_d(vy, la) - rl_d(vy, lb),
+                                   rl_d(vz, la) - rl_d(vz, lb)};
+                    double vdot = R[0] * V[0] + R[1] * V[1] + R[2] * V[2];
+                    double L = -vdot / ((ima + imb) * (len * len));
+                    if (fabs(L) > r_tol) {
+                        vir = vir + (L * (len * len)) / dt;
+                        if (lane == la) {
+                            double c = L * ima;
+                            vx = vx + c * R[0]; vy = vy + c * R[1]; vz = vz + c * R[2];
+                        }
+                        if (lane == lb) {
+                            double c = L * imb;
+                            vx = vx - c * R[0]; vy = vy - c * R[1]; vz = vz - c * R[2];
+                        }
+                        mv |= bits;
+                    } else {
+                        mv &= ~bits;
+                    }
+                }
+            }
+            if (failed) break;
+        }
+        if (failed && lane == 0) atomicCAS(err, 0, DERR_CONSTRAINT);
+        if (lane < na) {
+            s.f[SX_RX][g] = rx; s.f[SX_RY][g] = ry; s.f[SX_RZ][g] = rz;
+            s.f[SX_VX][g] = vx; s.f[SX_VY][g] = vy; s.f[SX_VZ][g] = vz;
+        }
+    }
+    if (lane == 0) vir_part[blockIdx.x] = vir;
+}
+
 // ---------------------------------------------------------------------------
 // bonded: per-term evaluation into a buffer, per-atom gather
 // ---------------------------------------------------------------------------
@@ -1192,7 +1304,16 @@ void sx_rattle_impl(sx_ctx *c, double dt, int phase)
     Molecules &m = c->mol;
     if (m.nmol == 0) return;
     sx_ensure_part(c, m.nmol, 1);
-    if (phase == 0)
+    if (m.max_bonds <= SX_WAVE) {
+        if (phase == 0)
+            hipLaunchKernelGGL(k_rattle64<0>, dim3(m.nmol), dim3(64), 0, c->stream, c->dstate(), dt, m.nmol,
+                               m.atom_start, m.atoms, m.bond_start, m.bond_a, m.bond_b, m.bond_len, c->d_part,
+                               c->d_err);
+        else
+            hipLaunchKernelGGL(k_rattle64<1>, dim3(m.nmol), dim3(64), 0, c->stream, c->dstate(), dt, m.nmol,
+                               m.atom_start, m.atoms, m.bond_start, m.bond_a, m.bond_b, m.bond_len, c->d_part,
+                               c->d_err);
+    } else if (phase == 0)
         hipLaunchKernelGGL(k_rattle<0>, dim3(m.nmol), dim3(64), 0, c->stream, c->dstate(), dt, m.nmol, m.atom_start,
                            m.atoms, m.bond_start, m.bond_a, m.bond_b, m.bond_len, c->d_part, c->d_err);
     else
@@ -1359,7 +1480,7 @@ int sx_set_constraints(sx_ctx *c, int nc, const int *bbi1, const int *bbj1, cons
         }
         std::vector<int> atom_start{0}, atoms, bond_start{0}, ba, bb, gidx;
         std::vector<double> bl;
-        int max_atoms = 0;
+        int max_atoms = 0, max_bonds = 0;
         for (auto &bonds : mb) {
             std::vector<int> loc;  // atoms of the molecule in first-appearance order
             auto local = [&](int g) {
@@ -1375,6 +1496,7 @@ int sx_set_constraints(sx_ctx *c, int nc, const int *bbi1, const int *bbj1, cons
             if ((int)loc.size() > SX_WAVE)
                 throw sx_error{SX_ERR_ARG, "a constrained molecule has more than 64 atoms"};
             max_atoms = std::max(max_atoms, (int)loc.size());
+            max_bonds = std::max(max_bonds, (int)bonds.size());
             for (int g : loc) atoms.push_back(g);
             atom_start.push_back((int)atoms.size());
             bond_start.push_back((int)ba.size());
@@ -1386,6 +1508,7 @@ int sx_set_constraints(sx_ctx *c, int nc, const int *bbi1, const int *bbj1, cons
         m = Molecules();
         m.nmol = (int)mb.size();
         m.max_atoms = max_atoms;
+        m.max_bonds = max_bonds;
         auto up = [&](auto &vec, auto **dst) {
             using T = typename std::decay<decltype(vec[0])>::type;
             HIPCHK(hipMalloc(dst, sizeof(T) * std::max<size_t>(1, vec.size())));

@@ -87,7 +87,7 @@ struct Molecules {
     int *bond_start = nullptr, *bond_a = nullptr, *bond_b = nullptr;  // local atom ids
     double *bond_len = nullptr;
     int *bond_gidx = nullptr;
-    int max_atoms = 0;
+    int max_atoms = 0, max_bonds = 0;
 };
 
 // composite bodies registered for the step driver: placements (centre ->
@@ -253,7 +253,20 @@ static inline int sx_blocks(long n, int bs) { return (int)((n + bs - 1) / bs); }
 // ---------------------------------------------------------------------------
 // device helpers (FP64, reference evaluation order)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double sx_mimg(double d, double L) { return d - round(d / L) * L; }
+// minimum image, d - anint(d/L)*L.  For |d| < L/2 the correctly rounded
+// quotient is at most the double just below 1/2, so anint gives a signed zero
+// and the result is d + 0.0 (which also turns -0.0 into +0.0 as the full
+// expression does): the same bits without the division.
+__device__ __forceinline__ double sx_mimg(double d, double L)
+{
+    if (fabs(d) < 0.5 * L) return d + 0.0;
+    return d - round(d / L) * L;
+}
+
+// the same value always through the division, for callers (all pairs of a
+// small box) where half the components lie beyond L/2 and a wavefront would
+// take both paths
+__device__ __forceinline__ double sx_mimg_div(double d, double L) { return d - round(d / L) * L; }
 
 // Same value as sx_mimg whenever |d| is not within a few ulp of an odd
 // multiple of L/2: round(d*invL) == round(d/L) there, and the subtraction

@@ -7,9 +7,9 @@ __device__ __forceinline__ void wrap_pos(const DState &s, int i)
 {
     if (s.periodic) {
         double x = s.f[SX_RX][i], y = s.f[SX_RY][i], z = s.f[SX_RZ][i];
-        s.f[SX_RX][i] = x - round(x / s.box[0]) * s.box[0];
-        s.f[SX_RY][i] = y - round(y / s.box[1]) * s.box[1];
-        s.f[SX_RZ][i] = z - round(z / s.box[2]) * s.box[2];
+        s.f[SX_RX][i] = sx_mimg(x, s.box[0]);
+        s.f[SX_RY][i] = sx_mimg(y, s.box[1]);
+        s.f[SX_RZ][i] = sx_mimg(z, s.box[2]);
     }
 }
 
