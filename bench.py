@@ -140,17 +140,22 @@ def other_workloads(sx, args):
         d.run(sx.INT_NHT, gb, 100, p["dt"], first_step=step)
         d.synchronize()
         rb0 = d.rebuild_count()
+        it0, calls0, fb0 = d.nht_stats()
         t0 = time.perf_counter()
         d.run(sx.INT_NHT, gb, args.gb_steps, p["dt"], first_step=step + 100)
         d.synchronize()
         el = time.perf_counter() - t0
+        it1, calls1, fb1 = d.nht_stats()
         rec = {"value": p["n"] * args.gb_steps / el, "unit": "particle-steps/s", "steps": args.gb_steps,
                "ms_per_step": el / args.gb_steps * 1e3, "rebuilds": d.rebuild_count() - rb0,
                "T": d.get_scalar("TEMP"), "pe_per_particle": d.get_scalar("POT_ENG") / p["n"],
+               "chain_newton_updates_per_call": (it1 - it0) / max(1, calls1 - calls0),
+               "chain_serial_fallbacks": fb1 - fb0,
                "config": {"workload": "gb_nhc_8000", "n": p["n"], "rho": p["rho"], "gb": "GB(3,5,2,1)",
                           "sig": [1, 1, 3], "eps": [5, 5, 1], "T": p["T"], "dt": p["dt"], "cutoff": p["cutoff"],
                           "skin": p["skin"], "integrator": "nht (chain 3, tau 2) + quaternion", "dtype": "f64"}}
-        log(f"[gb] {args.gb_steps} steps in {el:.3f}s -> {rec['value']:.4e} particle-steps/s")
+        log(f"[gb] {args.gb_steps} steps in {el:.3f}s -> {rec['value']:.4e} particle-steps/s; "
+            f"{rec['chain_newton_updates_per_call']:.2f} Newton updates per chain call, {fb1 - fb0} serial fallbacks")
         if not args.no_cpu:
             cpu = gb_cpu_baseline(d.state(), p["box"], p, 10)
             if cpu:
