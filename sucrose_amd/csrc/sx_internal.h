@@ -224,6 +224,7 @@ void sx_composite_transfer_impl(sx_ctx *c);
 void sx_composite_free(sx_ctx *c);
 void sx_nlist_want_half(sx_ctx *c);
 void sx_forces_impl(sx_ctx *c, int kind, const double *p, int np);
+bool sx_forces_two_impl(sx_ctx *c, int k1, const double *p1, int np1, int k2, const double *p2, int np2);
 void sx_bonded_impl(sx_ctx *c);
 void sx_lgv_initial_impl(sx_ctx *c, double dt, int step, const int *dstep, double damping, bool constrained);
 // lgv_initial_step, the rebuild decision and initialize_forces in one launch

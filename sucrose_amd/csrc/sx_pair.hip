@@ -69,6 +69,52 @@ __global__ void __launch_bounds__(256) k_gather(DState s, const int *row_start, 
     }
 }
 
+// two isotropic kinds over one list in one pass (one read of each row and its
+// neighbours): each kind's sums are kept apart, reduced as k_gather reduces
+// them and added in the order of the two loops, so forces and energies equal
+// those of two k_gather launches with the same lanes per particle
+template <int K1, int K2, int LPP>
+__global__ void __launch_bounds__(256) k_gather2(DState s, const int *row_start, const int *nbr, Params P1,
+                                                 Params P2, double *pe_acc)
+{
+    int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = gid / LPP, l = gid % LPP;
+    double a1[4] = {0, 0, 0, 0}, a2[4] = {0, 0, 0, 0};  // fx fy fz pe of each kind
+    if (i < s.n) {
+        int b = row_start[i], e = row_start[i + 1];
+        for (int q = b + l; q < e; q += LPP) {
+            int j = nbr[q];
+            PairOut o1, o2;
+            if (j > i) {
+                d_pair<K1>(s, i, j, P1.p, o1);
+                d_pair<K2>(s, i, j, P2.p, o2);
+                a1[0] = a1[0] + o1.f[0]; a1[1] = a1[1] + o1.f[1]; a1[2] = a1[2] + o1.f[2];
+                a1[3] = a1[3] + o1.pe;
+                a2[0] = a2[0] + o2.f[0]; a2[1] = a2[1] + o2.f[1]; a2[2] = a2[2] + o2.f[2];
+                a2[3] = a2[3] + o2.pe;
+            } else {
+                d_pair<K1>(s, j, i, P1.p, o1);
+                d_pair<K2>(s, j, i, P2.p, o2);
+                a1[0] = a1[0] - o1.f[0]; a1[1] = a1[1] - o1.f[1]; a1[2] = a1[2] - o1.f[2];
+                a2[0] = a2[0] - o2.f[0]; a2[1] = a2[1] - o2.f[1]; a2[2] = a2[2] - o2.f[2];
+            }
+        }
+    }
+#pragma unroll
+    for (int off = LPP / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            a1[c] = a1[c] + __shfl_xor(a1[c], off, LPP);
+            a2[c] = a2[c] + __shfl_xor(a2[c], off, LPP);
+        }
+    if (l == 0 && i < s.n) {
+        s.f[SX_FX][i] = (s.f[SX_FX][i] + a1[0]) + a2[0];
+        s.f[SX_FY][i] = (s.f[SX_FY][i] + a1[1]) + a2[1];
+        s.f[SX_FZ][i] = (s.f[SX_FZ][i] + a1[2]) + a2[2];
+        pe_acc[i] = (pe_acc[i] + a1[3]) + a2[3];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // ECP
 // ---------------------------------------------------------------------------
@@ -695,6 +741,36 @@ void sx_ecp_records(sx_ctx *c, double **pp, float **scr, double **pos4)
     *pp = c->d_ecp_pp;
     *scr = c->d_ecp_scr;
     *pos4 = c->d_pos4;
+}
+
+template <int K1, int K2>
+static void launch_gather2(sx_ctx *c, const Params &P1, const Params &P2, int lpp)
+{
+    const int nb = sx_blocks((long)c->n * lpp, 256);
+    DState s = c->dstate();
+    switch (lpp) {
+    case 1: hipLaunchKernelGGL((k_gather2<K1, K2, 1>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P1, P2, c->d_pe_acc); break;
+    case 4: hipLaunchKernelGGL((k_gather2<K1, K2, 4>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P1, P2, c->d_pe_acc); break;
+    case 8: hipLaunchKernelGGL((k_gather2<K1, K2, 8>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P1, P2, c->d_pe_acc); break;
+    default: hipLaunchKernelGGL((k_gather2<K1, K2, 16>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P1, P2, c->d_pe_acc); break;
+    }
+}
+
+// two pair loops of the step driver in one pass where both kinds are WCA and
+// Debye-Hueckel (the charged-chain step); false: not such a pair
+bool sx_forces_two_impl(sx_ctx *c, int k1, const double *p1, int np1, int k2, const double *p2, int np2)
+{
+    const bool wd = k1 == SX_WCA && k2 == SX_DH, dw = k1 == SX_DH && k2 == SX_WCA;
+    if (!wd && !dw) return false;
+    check_params(k1, np1);
+    check_params(k2, np2);
+    if (!c->nl.init) throw sx_error{SX_ERR_STATE, "neighbour list not initialised (sx_nlist_init)"};
+    const Params P1 = make_params(c, k1, p1, np1), P2 = make_params(c, k2, p2, np2);
+    const int lpp = choose_lpp(c, k1);   // the same for every kind
+    if (wd) launch_gather2<SX_WCA, SX_DH>(c, P1, P2, lpp);
+    else launch_gather2<SX_DH, SX_WCA>(c, P1, P2, lpp);
+    if (!c->defer_pe) sx_pe_total_impl(c);
+    return true;
 }
 
 void sx_forces_impl(sx_ctx *c, int kind, const double *p, int np)

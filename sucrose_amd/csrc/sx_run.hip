@@ -68,7 +68,10 @@ static void enqueue_step(sx_ctx *c, const sx_step_spec &sp)
         sx_init_forces_impl(c);
     }
     bool dipoles = false;
-    for (int f = 0; f < sp.nforces; ++f) {
+    const bool two = sp.nforces == 2 &&
+                     sx_forces_two_impl(c, sp.kinds[0], sp.params[0], nparams_of(sp.kinds[0]), sp.kinds[1],
+                                        sp.params[1], nparams_of(sp.kinds[1]));
+    for (int f = 0; f < sp.nforces && !two; ++f) {
         sx_forces_impl(c, sp.kinds[f], sp.params[f], nparams_of(sp.kinds[f]));
         dipoles = dipoles || sp.kinds[f] == SX_DIPOLE;
     }

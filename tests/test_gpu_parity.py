@@ -490,6 +490,52 @@ def test_run_graph_langevin(dev, oracle, kind, params, rc):
         assert abs(got[s] - ref[s]) <= 1e-9 * max(1.0, abs(ref[s])), (s, got[s], ref[s])
 
 
+@pytest.mark.parametrize("order", [0, 1])
+def test_run_graph_two_pair_loops(dev, oracle, order):
+    """sx_run (graph replay) with two pair loops over one list, WCA and
+    Debye-Hueckel on charged particles (the charged-chain step's pair
+    kernels, evaluated in one pass), in either order, vs the oracle's README
+    loop running the two loops one after the other, 10 Langevin steps."""
+    import sucrose_amd as sx
+    say(f"run graph WCA + DH order={order}")
+    rng = np.random.default_rng(21 + order)
+    m, L = 12, 12.0
+    n = m ** 3
+    st = base_state(rng, n, L, forces=False, T=1.0, charges=True)
+    g = np.arange(m) * (L / m) - L / 2
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1) + rng.normal(size=(n, 3)) * 0.05
+    st["rx"], st["ry"], st["rz"] = P[:, 0], P[:, 1], P[:, 2]
+    loops = [(WCA, (1.0, 1.0)), (DH, (5.0, 1.0, 1.0))]
+    if order:
+        loops.reverse()
+    to_device(dev, st)
+    dev.init_nlist(0.3, 5.0)
+    dev.initialize_forces()
+    for kind, params in loops:
+        dev.forces(kind, *params)
+    ref = from_device(dev, st)
+    nl = oracle.nlist_new(ref, 5.0, 0.3)
+    oracle.nlist_build(ref, nl)
+    nsteps, dt, first = 10, 0.004, 3
+    dev.run(sx.INT_LGV, [(k,) + tuple(p) for k, p in loops], nsteps, dt, damping=1.0, first_step=first)
+    for t in range(nsteps):
+        oracle.integrate(LGV, 0, ref, dt, step=first + t, damping=1.0)
+        if oracle.needs_rebuild(ref, nl):
+            oracle.nlist_build(ref, nl)
+        oracle.call("or_initialize_forces", ref)
+        for kind, params in loops:
+            oracle.pair_loop(ref, nl, kind, list(params))
+        oracle.integrate(LGV, 1, ref, dt, damping=1.0)
+        oracle.call("or_calculate_thermo", ref)
+    got = from_device(dev, st)
+    assert np.any(ref["fx"] != 0)
+    for f in ("rx", "ry", "vx", "vz", "qw", "lx", "fx", "fy"):
+        assert rel_err(got[f], ref[f]) < 1e-9, (f, rel_err(got[f], ref[f]))
+    for s in ("pot_eng", "kin_eng", "kin_rot", "temp"):
+        assert abs(got[s] - ref[s]) <= 1e-9 * max(1.0, abs(ref[s])), (s, got[s], ref[s])
+
+
 def test_run_graph_nose_hoover(dev, oracle):
     """sx_run (graph replay) with the Nose-Hoover chain + quaternion steps on
     2744 particles, so that both chain calls of every step take the parallel
