@@ -328,10 +328,36 @@ __global__ void k_rows_brute(DState s, double rsq_cut, int *row_len, int *lower,
 }
 
 // one wavefront per row: the row goes to LDS, each lane takes up to 8 of its
-// entries and ranks them against the whole row (the entries of a row are
+// entries (as few as the row's length allows) and ranks them against the whole row (the entries of a row are
 // distinct, so rank = number of smaller entries is the sorted position), then
 // writes them back in ascending j
 #define SORT_PER_LANE 8
+
+// PER entries of the row per lane from q0 on, ranked against the whole row
+template <int PER>
+__device__ __forceinline__ void rank_row_entries(const int *sh, int len, int len4, int lane, int q0, int *row)
+{
+    int mine[PER], rank[PER];
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+        const int q = q0 + t * SX_WAVE + lane;
+        mine[t] = q < len ? sh[q] : 0x7fffffff;
+        rank[t] = 0;
+    }
+    const int4 *sh4 = (const int4 *)sh;
+    for (int m = 0; m < len4 / 4; ++m) {
+        const int4 v = sh4[m];   // the same address in every lane: a broadcast read
+#pragma unroll
+        for (int t = 0; t < PER; ++t)
+            rank[t] += (v.x < mine[t]) + (v.y < mine[t]) + (v.z < mine[t]) + (v.w < mine[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+        const int q = q0 + t * SX_WAVE + lane;
+        if (q < len) row[rank[t]] = mine[t];
+    }
+}
+
 __global__ void __launch_bounds__(256) k_sort_rows(int n, const int *row_start, int *nbr, const int *flag, int *err)
 {
     if (flag && *flag == 0) return;
@@ -350,25 +376,18 @@ __global__ void __launch_bounds__(256) k_sort_rows(int n, const int *row_start, 
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (int q0 = 0; q0 < len; q0 += SX_WAVE * SORT_PER_LANE) {
-            int mine[SORT_PER_LANE], rank[SORT_PER_LANE];
-#pragma unroll
-            for (int t = 0; t < SORT_PER_LANE; ++t) {
-                const int q = q0 + t * SX_WAVE + lane;
-                mine[t] = q < len ? sh[q] : 0x7fffffff;
-                rank[t] = 0;
-            }
-            const int4 *sh4 = (const int4 *)sh;
-            for (int m = 0; m < len4 / 4; ++m) {
-                const int4 v = sh4[m];   // the same address in every lane: a broadcast read
-#pragma unroll
-                for (int t = 0; t < SORT_PER_LANE; ++t)
-                    rank[t] += (v.x < mine[t]) + (v.y < mine[t]) + (v.z < mine[t]) + (v.w < mine[t]);
-            }
-#pragma unroll
-            for (int t = 0; t < SORT_PER_LANE; ++t) {
-                const int q = q0 + t * SX_WAVE + lane;
-                if (q < len) nbr[b + rank[t]] = mine[t];
+        // as many entries per lane as the row needs (up to SORT_PER_LANE per pass)
+        const int per = min(SORT_PER_LANE, (len + SX_WAVE - 1) / SX_WAVE);
+        for (int q0 = 0; q0 < len; q0 += SX_WAVE * per) {
+            switch (per) {
+            case 1: rank_row_entries<1>(sh, len, len4, lane, q0, nbr + b); break;
+            case 2: rank_row_entries<2>(sh, len, len4, lane, q0, nbr + b); break;
+            case 3: rank_row_entries<3>(sh, len, len4, lane, q0, nbr + b); break;
+            case 4: rank_row_entries<4>(sh, len, len4, lane, q0, nbr + b); break;
+            case 5: rank_row_entries<5>(sh, len, len4, lane, q0, nbr + b); break;
+            case 6: rank_row_entries<6>(sh, len, len4, lane, q0, nbr + b); break;
+            case 7: rank_row_entries<7>(sh, len, len4, lane, q0, nbr + b); break;
+            default: rank_row_entries<8>(sh, len, len4, lane, q0, nbr + b); break;
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
