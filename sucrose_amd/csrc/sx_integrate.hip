@@ -604,11 +604,24 @@ __device__ __forceinline__ bool nht_converged(const NhtBuf &B, int k)
     return false;
 }
 
+// the chain masses of the call: the coupling given, else n_dof T tau^2 for
+// the first link and T tau^2 for the others (tau = 2)
+__device__ __forceinline__ void nht_set_hb(double *scal, double ndof, double coupling, int has_coupling)
+{
+    double T = scal[SX_TARGET_TEMP];
+    if (has_coupling) {
+        scal[SX_HB0] = scal[SX_HB1] = scal[SX_HB2] = coupling;
+    } else {
+        scal[SX_HB0] = scal[SX_HB1] = scal[SX_HB2] = T * (2.0 * 2.0);
+        scal[SX_HB0] = ndof * T * (2.0 * 2.0);
+    }
+}
+
 // sum of m |v|^2 before the call (each workgroup over its chunks' particles,
 // then the workgroup sums in a fixed order), the chain state entering the
 // call; clears the per-pass change slots
-__global__ void __launch_bounds__(NHT_BLK) k_nhtp_prep(DState s, const double *scal, NhtBuf B,
-                                                       unsigned long long *stats)
+__global__ void __launch_bounds__(NHT_BLK) k_nhtp_prep(DState s, double *scal, NhtBuf B, unsigned long long *stats,
+                                                       double ndof, double coupling, int has_coupling)
 {
     const int b = blockIdx.x, t = threadIdx.x, n = s.n;
     const int i0 = min(n, b * NHT_BLK * B.chunk), i1 = min(n, i0 + NHT_BLK * B.chunk);
@@ -623,6 +636,7 @@ __global__ void __launch_bounds__(NHT_BLK) k_nhtp_prep(DState s, const double *s
         B.s0[0] = scal[SX_PETA0]; B.s0[1] = scal[SX_PETA1]; B.s0[2] = scal[SX_PETA2];
         B.s0[3] = tot[0];
         if (stats) atomicAdd(&stats[1], 1ull);
+        nht_set_hb(scal, ndof, coupling, has_coupling);   // read by the passes that follow
     }
     if (t < NHT_MAXIT) B.dev[t] = 0ull;
 }
@@ -790,13 +804,7 @@ __global__ void __launch_bounds__(NHT_BLK) k_nhtp_final(DState s, double dt, int
 __global__ void k_nht_hb(double *scal, double ndof, double coupling, int has_coupling)
 {
     if (threadIdx.x != 0) return;
-    double T = scal[SX_TARGET_TEMP];
-    if (has_coupling) {
-        scal[SX_HB0] = scal[SX_HB1] = scal[SX_HB2] = coupling;
-    } else {
-        scal[SX_HB0] = scal[SX_HB1] = scal[SX_HB2] = T * (2.0 * 2.0);
-        scal[SX_HB0] = ndof * T * (2.0 * 2.0);
-    }
+    nht_set_hb(scal, ndof, coupling, has_coupling);
 }
 
 // ---------------------------------------------------------------------------
@@ -1246,11 +1254,11 @@ void sx_lgv_final_impl(sx_ctx *c, double dt, double damping, bool constrained, b
 void sx_nht_impl(sx_ctx *c, double dt, const double *coupling, int phase, bool save)
 {
     double ndof = (double)(3 * (c->n / c->nrigid) - 3);
-    hipLaunchKernelGGL(k_nht_hb, dim3(1), dim3(64), 0, c->stream, c->d_scal, ndof, coupling ? *coupling : 0.0,
-                       coupling ? 1 : 0);
     // the serial walk for small systems, the parallel (DEER) evaluation above
-    // a few thousand particles
+    // a few thousand particles (whose prep kernel also sets the chain masses)
     if (c->n < 2048) {
+        hipLaunchKernelGGL(k_nht_hb, dim3(1), dim3(64), 0, c->stream, c->d_scal, ndof, coupling ? *coupling : 0.0,
+                           coupling ? 1 : 0);
         sx_ensure_part(c, 1, 1);
         hipLaunchKernelGGL(k_nht, dim3(1), dim3(NHT_CHUNK), 0, c->stream, c->dstate(), dt, phase, save ? 1 : 0,
                            c->d_scal, ndof, c->d_part);
@@ -1278,7 +1286,8 @@ void sx_nht_impl(sx_ctx *c, double dt, const double *coupling, int phase, bool s
         B.nb = (nchunks + NHT_BLK - 1) / NHT_BLK;
         unsigned long long *st = (unsigned long long *)(c->d_counts + 4);
         DState s = c->dstate();
-        hipLaunchKernelGGL(k_nhtp_prep, dim3(B.nb), dim3(NHT_BLK), 0, c->stream, s, c->d_scal, B, st);
+        hipLaunchKernelGGL(k_nhtp_prep, dim3(B.nb), dim3(NHT_BLK), 0, c->stream, s, c->d_scal, B, st, ndof,
+                           coupling ? *coupling : 0.0, coupling ? 1 : 0);
         for (int k = 0; k < NHT_MAXIT; ++k)
             hipLaunchKernelGGL(k_nhtp_pass, dim3(B.nb), dim3(NHT_BLK), 0, c->stream, s, dt, phase, c->d_scal, ndof,
                                B, k, st);
