@@ -587,6 +587,62 @@ def test_run_graph_nose_hoover(dev, oracle):
         assert abs(got[s] - ref[s]) <= 1e-9 * max(1.0, abs(ref[s])), (s, got[s], ref[s])
 
 
+def test_run_graph_gay_berne_nose_hoover(dev, oracle):
+    """The Gay-Berne + Nose-Hoover step (GB(3,5,2,1), cutoff 4, mu = 2,
+    nu = 1, chain + quaternion steps) through sx_run on 2197 ellipsoids, so
+    that the chain takes the parallel kernels inside the captured graph, vs
+    the oracle's README loop, 8 steps."""
+    import sucrose_amd as sx
+    say("run graph Gay-Berne + Nose-Hoover n=2197")
+    rng = np.random.default_rng(17)
+    m, a = 13, 3.2
+    n, L = m ** 3, m * a
+    st = base_state(rng, n, L, forces=False, T=3.0, sig=(1, 1, 3), eps=(5, 5, 1))
+    g = np.arange(m) * a - L / 2
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    st["rx"], st["ry"], st["rz"] = X.ravel(), Y.ravel(), Z.ravel()
+    s2 = np.sum(st["mass"] * (st["vx"] ** 2 + st["vy"] ** 2 + st["vz"] ** 2))
+    f = np.sqrt((3 * n - 3) * 3.0 / s2)
+    for c in ("vx", "vy", "vz"):
+        st[c] = st[c] * f
+    params = (4.0, 2, 1)
+    to_device(dev, st)
+    dev.init_nlist(0.4, 6.0)
+    dev.initialize_forces()
+    dev.forces(GB, *params)
+    ref = from_device(dev, st)
+    nl = oracle.nlist_new(ref, 6.0, 0.4)
+    oracle.nlist_build(ref, nl)
+    nsteps, dt = 8, 0.0015
+    it0, calls0, fb0 = dev.nht_stats()
+    dev.run(sx.INT_NHT, [(GB,) + params], nsteps, dt, first_step=1)
+    it1, calls1, fb1 = dev.nht_stats()
+    for t in range(nsteps):
+        oracle.integrate(NHT, 0, ref, dt)
+        oracle.integrate(QQ, 0, ref, dt)
+        if oracle.needs_rebuild(ref, nl):
+            oracle.nlist_build(ref, nl)
+        oracle.call("or_initialize_forces", ref)
+        oracle.pair_loop(ref, nl, GB, list(params))
+        oracle.integrate(NHT, 1, ref, dt)
+        oracle.integrate(QQ, 1, ref, dt)
+        oracle.call("or_calculate_thermo", ref)
+    got = from_device(dev, st)
+    say(f"  {calls1 - calls0} chain calls, {it1 - it0} Newton passes, {fb1 - fb0} serial fallbacks")
+    assert calls1 - calls0 == 2 * nsteps and fb1 == fb0
+    assert np.any(ref["tx"] != 0)
+    for f in ("rx", "vy", "qw", "qz", "lx", "lz", "fx", "tz"):
+        assert rel_err(got[f], ref[f]) < 1e-9, (f, rel_err(got[f], ref[f]))
+    # from the lattice start the chain is driven far from its thermal scale
+    # (p_eta(1) near 200 after 8 steps) and each call integrates it for
+    # N dt / 2 time units, so it carries the summation-order rounding of
+    # sum m v^2 forward: 1e-7 relative here, the particle states above 1e-9
+    for s in ("peta0", "peta1", "peta2", "eta0"):
+        assert abs(got[s] - ref[s]) <= 1e-7 * max(1.0, abs(ref[s])), (s, got[s], ref[s])
+    for s in ("pot_eng", "kin_eng", "kin_rot"):
+        assert abs(got[s] - ref[s]) <= 1e-9 * max(1.0, abs(ref[s])), (s, got[s], ref[s])
+
+
 def test_mc_barostat(dev, oracle):
     """sx_mc_barostat vs the oracle's mc_barostat (integrators.f90:864-945)
     with the same two uniform numbers; compute_potential is the Lennard-Jones
