@@ -35,7 +35,7 @@ typedef struct sx_ctx sx_ctx;
 #define SX_ERR_ARG        -1   /* bad argument (index out of range, n <= 0, ...) */
 #define SX_ERR_HIP        -2   /* HIP runtime error                                */
 #define SX_ERR_STATE      -3   /* call order (e.g. forces before nlist_init)      */
-#define SX_ERR_CAPACITY   -4   /* device capacity exceeded (re-grown on next call) */
+#define SX_ERR_CAPACITY   -4   /* device capacity exceeded (grown for the next call) */
 #define SX_ERR_CONSTRAINT -5   /* RATTLE failure: system.f90:1101-1105            */
 #define SX_ERR_CUTOFF     -6   /* cutoff > box/2: system.f90:902-906              */
 #define SX_ERR_NOTEMP     -7   /* target temperature unset: integrators.f90:311   */
@@ -115,7 +115,8 @@ int  sx_version(void);
 /* system state (module system, system.f90)                            */
 /* ------------------------------------------------------------------ */
 /* set_num_atoms (system.f90:43): allocates n = num_atoms*rigid_atoms
- * particles with the reference's sentinel values (-1 params, q = (1,0,0,0)). */
+ * particles with the reference's sentinel values (-1 params, q = (1,0,0,0));
+ * bonded terms registered for the previous system are dropped. */
 int sx_set_num_atoms(sx_ctx *ctx, int num_atoms, int rigid_atoms);
 /* set_box (system.f90:185) + set_periodic_boundary (:209) */
 int sx_set_box(sx_ctx *ctx, const double box[3], int periodic);
@@ -191,7 +192,8 @@ int sx_nht_stats(sx_ctx *ctx, long out[3]);
 /* bonded terms (module bonded, force.f90:2244-2579); kind 1 bend_, 2 cosine_bend_,
  * 3 dih_, 4 cosine_dih_, 5 angle_bend_ (:2464, harmonic in the angle),
  * 6 angle_dih_ (:2508); idx1 holds 3 (bends: 1, 2, 5) or 4 (dihedrals: 3, 4, 6)
- * 1-based atoms per term.  Registered terms are evaluated by sx_bonded_forces. */
+ * 1-based atoms per term.  Each call adds its terms to those registered;
+ * sx_bonded_forces evaluates them all (sx_set_num_atoms drops them). */
 int sx_set_bonded(sx_ctx *ctx, int kind, int count, const int *idx1, double coeff, double angle);
 int sx_bonded_forces(sx_ctx *ctx);
 
@@ -227,13 +229,21 @@ typedef struct sx_step_spec {
     double params[4][8];
     int bonded;            /* evaluate registered bonded terms                    */
     double dt, damping;
-    int first_step;        /* Langevin counter of the first replayed step         */
+    int first_step;        /* Langevin counter of the first replayed step; < 0:
+                              continue from the device's counter (the step after
+                              the last one run, or the one sx_load_checkpoint
+                              restored)                                           */
 } sx_step_spec;
 /* Runs nsteps of: initial step, placement of registered composite-body
  * satellites, nlist update, initialize_forces, pair loops (then
  * eval_reaction_field when a kind is SX_DIPOLE), bonded, transfer of the
  * satellites' forces to their centres, final step, calculate_thermo.  The
- * first call captures the graph. */
+ * first call captures the graph (again after a new target temperature or a
+ * change of the list's storage).  Before replaying, the neighbour list is
+ * given room for half as many entries again as its last build.  A rebuild
+ * inside the replays that still does not fit empties the list from then on
+ * (no pair forces, no out-of-range access); the call then grows the list,
+ * rebuilds it at the current positions and returns SX_ERR_CAPACITY. */
 int sx_run(sx_ctx *ctx, const sx_step_spec *spec, int nsteps);
 
 /* ------------------------------------------------------------------ */

@@ -390,6 +390,13 @@ int sx_set_num_atoms(sx_ctx *c, int num_atoms, int rigid_atoms)
     return guard(c, [&] {
         if (num_atoms <= 0 || rigid_atoms <= 0) throw sx_error{SX_ERR_ARG, "num_atoms and rigid_atoms must be > 0"};
         free_system(c);
+        // registered bonded terms name atoms of the previous system
+        for (auto &b : c->bonded) {
+            if (b.idx) hipFree(b.idx);
+            if (b.out) hipFree(b.out);
+        }
+        c->bonded.clear();
+        c->have_graph = false;
         c->nrigid = rigid_atoms;
         c->n = num_atoms * rigid_atoms;
         c->nc = 0;
@@ -598,6 +605,9 @@ int sx_set_scalar(sx_ctx *c, int id, double v)
     return guard(c, [&] {
         if (id < 0 || id >= SX_NSCALARS) throw sx_error{SX_ERR_ARG, "bad scalar id"};
         sx_sync_scalars_to_host(c);
+        // captured Langevin steps hold T and its noise counter int(1000 T)
+        // as launch constants: a new target temperature re-captures them
+        if (id == SX_TARGET_TEMP && (!c->have_target_temp || c->h_scal[id] != v)) c->have_graph = false;
         c->h_scal[id] = v;
         if (id == SX_TARGET_TEMP) c->have_target_temp = true;
         sx_sync_scalars_to_device(c);
@@ -628,7 +638,10 @@ int sx_initialize_system(sx_ctx *c, int remove_com)
         int dof = 6 * c->n - c->nc;
         if (remove_com) dof -= 3;
         c->h_scal[SX_DOF] = dof;
-        if (c->nrigid > 1) c->h_scal[SX_TARGET_TEMP] = c->h_scal[SX_TARGET_TEMP] * c->nrigid;
+        if (c->nrigid > 1) {
+            c->h_scal[SX_TARGET_TEMP] = c->h_scal[SX_TARGET_TEMP] * c->nrigid;
+            c->have_graph = false;
+        }
         sx_sync_scalars_to_device(c);
         sx_init_forces_impl(c);
         HIPCHK(hipStreamSynchronize(c->stream));

@@ -218,6 +218,13 @@ void sx_sync_scalars_to_host(sx_ctx *c);
 void sx_nlist_build_impl(sx_ctx *c, bool conditional);
 void sx_nlist_flag_impl(sx_ctx *c);
 void sx_nlist_rebox_impl(sx_ctx *c);
+// grows the list (contents kept) so that it can take half as many entries
+// again as the last build before graph replays
+void sx_nlist_headroom(sx_ctx *c);
+// device error check after replays; a list that overflowed inside them is
+// grown and rebuilt before SX_ERR_CAPACITY is thrown; headroom: then make
+// room for the next replays (sx_nlist_headroom)
+void sx_nlist_check_after_replay(sx_ctx *c, bool headroom);
 void sx_eval_reaction_field_impl(sx_ctx *c);
 void sx_composite_place_impl(sx_ctx *c);
 void sx_composite_transfer_impl(sx_ctx *c);

@@ -88,8 +88,12 @@ __global__ void k_cell_assign(DState s, CellGrid g, int *cell_of, int *cell_coun
 }
 
 // exclusive scan of n ints by one work-group; out has n+1 entries; optional
-// conditional flag (skip when *flag == 0)
-__global__ void __launch_bounds__(1024) k_scan(const int *in, int *out, int n, const int *flag, int *total)
+// conditional flag (skip when *flag == 0).  With cap > 0 a total beyond cap
+// leaves an empty list instead: out[0..n] (and zero_also[0..n-1]) all zero,
+// *total the count that did not fit (for the host to grow the list) and the
+// capacity error raised, so that no later kernel indexes past the list.
+__global__ void __launch_bounds__(1024) k_scan(const int *in, int *out, int n, const int *flag, int *total,
+                                               long cap, int *err, int *zero_also)
 {
     if (flag && *flag == 0) return;
     // tiles of 4096: 4 consecutive elements per thread, wavefront shuffle
@@ -126,6 +130,18 @@ __global__ void __launch_bounds__(1024) k_scan(const int *in, int *out, int n, c
         long tile = wsum[15];
         __syncthreads();
         carry += tile;
+    }
+    // carry is the same in every thread, so the branch is uniform
+    if (cap > 0 && carry > cap) {
+        for (long i = t; i <= n; i += blockDim.x) {
+            out[i] = 0;
+            if (zero_also && i < n) zero_also[i] = 0;
+        }
+        if (t == 0) {
+            if (total) *total = carry > 0x7fffffffL ? 0x7fffffff : (int)carry;
+            atomicCAS(err, 0, DERR_NLIST_CAP);
+        }
+        return;
     }
     if (t == 0) {
         out[n] = (int)carry;
@@ -165,6 +181,9 @@ __global__ void k_rows_cells(DState s, CellGrid g, const int *cell_of, const int
         atomicCAS(err, 0, DERR_NLIST_CAP);
         return;
     }
+    // a slot that does not match the counted length (the list was emptied
+    // because it did not fit) is not filled
+    if (PASS && row_start[i + 1] - row_start[i] != row_len[i]) return;
     for (int dx = -1; dx <= 1; ++dx)
         for (int dy = -1; dy <= 1; ++dy)
             for (int dz = -1; dz <= 1; ++dz) {
@@ -227,6 +246,9 @@ __global__ void __launch_bounds__(256) k_rows_cells_g(DState s, CellGrid g, cons
             if (lane == 0) atomicCAS(err, 0, DERR_NLIST_CAP);
             return;
         }
+        // a slot that does not match the counted length (the list was
+        // emptied because it did not fit) is not filled; uniform per group
+        if (row_start[i + 1] - row_start[i] != row_len[i]) return;
     }
     for (int dx = -1; dx <= 1; ++dx)
         for (int dy = -1; dy <= 1; ++dy) {
@@ -312,6 +334,9 @@ __global__ void k_rows_brute(DState s, double rsq_cut, int *row_len, int *lower,
         atomicCAS(err, 0, DERR_NLIST_CAP);
         return;
     }
+    // a slot that does not match the counted length (the list was emptied
+    // because it did not fit) is not filled
+    if (PASS && row_start[i + 1] - row_start[i] != row_len[i]) return;
     for (int j = 0; j < s.n; ++j) {
         if (j == i) continue;
         if (in_list(s, i, j, rsq_cut)) {
@@ -573,6 +598,82 @@ static void nl_alloc_lists(sx_ctx *c, long cap_full)
     c->have_graph = false;
 }
 
+// grows the list arrays to cap_full entries keeping their contents (the list
+// stays valid, the rebuild decision is not disturbed)
+static void nl_grow_keep(sx_ctx *c, long cap_full)
+{
+    NList &nl = c->nl;
+    if (cap_full <= nl.cap_full) return;
+    if (cap_full > 0x7fffffffL) throw sx_error{SX_ERR_CAPACITY, "neighbour list larger than 2^31 entries"};
+    const long cap_half = cap_full / 2 + 1;
+    int **full[] = {&nl.nbr, &nl.pairk};
+    int **half[] = {&nl.half_i, &nl.half_j, &nl.fwdpos, &nl.revpos};
+    for (int **a : full) {
+        int *p = nullptr;
+        HIPCHK(hipMalloc(&p, sizeof(int) * cap_full));
+        HIPCHK(hipMemcpyAsync(p, *a, sizeof(int) * nl.cap_full, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(*a));
+        *a = p;
+    }
+    for (int **a : half) {
+        int *p = nullptr;
+        HIPCHK(hipMalloc(&p, sizeof(int) * cap_half));
+        HIPCHK(hipMemcpyAsync(p, *a, sizeof(int) * nl.cap_half, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(*a));
+        *a = p;
+    }
+    nl.cap_full = cap_full;
+    nl.cap_half = cap_half;
+    c->have_graph = false;
+}
+
+// entries of the last build (d_flag[1]; a build that did not fit leaves the
+// count it needed there)
+static long nl_last_total(sx_ctx *c)
+{
+    int total = 0;
+    HIPCHK(hipMemcpyAsync(&total, c->d_flag + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return total;
+}
+
+// Room for the list to grow by half before graph replays, whose rebuilds
+// cannot allocate.
+void sx_nlist_headroom(sx_ctx *c)
+{
+    NList &nl = c->nl;
+    if (!nl.init) return;
+    const long total = nl_last_total(c);
+    if (3 * total / 2 + 1024 > nl.cap_full) nl_grow_keep(c, 2 * total + 2048);
+}
+
+// After graph replays: a rebuild that did not fit emptied the list (k_scan)
+// and raised DERR_NLIST_CAP.  The list is then grown to the size it needed,
+// rebuilt at the current positions, and the capacity error reported; any
+// other device error is reported as it is.
+void sx_nlist_check_after_replay(sx_ctx *c, bool headroom)
+{
+    int e = 0;
+    HIPCHK(hipMemcpyAsync(&e, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    NList &nl = c->nl;
+    if (e == DERR_NLIST_CAP && nl.init) {
+        const long need = nl_last_total(c);
+        const long had = nl.cap_full;
+        HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+        nl_alloc_lists(c, std::max(had, 3 * need / 2 + 1024));
+        sx_nlist_build_impl(c, false);
+        throw sx_error{SX_ERR_CAPACITY,
+                       "neighbour list capacity exceeded during the run (" + std::to_string(need) + " entries, room for " +
+                           std::to_string(had) + "): from that rebuild on the list was empty and the steps ran "
+                           "without pair forces; the list has been grown and rebuilt at the current positions"};
+    }
+    sx_check_device_error(c);
+    if (headroom) sx_nlist_headroom(c);
+}
+
 static void nl_setup(sx_ctx *c)
 {
     NList &nl = c->nl;
@@ -635,7 +736,7 @@ static void half_index_launch(sx_ctx *c, const int *flag)
 {
     NList &nl = c->nl;
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.half_len, nl.half_start, c->n, flag,
-                       c->d_flag + 2);
+                       c->d_flag + 2, 0L, (int *)nullptr, (int *)nullptr);
     launch_half_index_g(c, flag);
 }
 
@@ -678,18 +779,18 @@ void sx_nlist_build_impl(sx_ctx *c, bool conditional)
             hipLaunchKernelGGL(k_cell_assign, dim3(nb), dim3(256), 0, c->stream, s, g, nl.cell_of, nl.cell_count,
                                nl.x_last, nl.y_last, nl.z_last, flag, conditional ? c->d_counts : (long *)nullptr);
             hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.cell_count, nl.cell_start, nl.ncells,
-                               flag, (int *)nullptr);
+                               flag, (int *)nullptr, 0L, (int *)nullptr, (int *)nullptr);
             hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(256), 0, c->stream, s, nl.cell_of, nl.cell_start,
                                nl.cell_count, nl.cell_items, (double4 *)nl.cell_pos, flag);
             launch_rows_g(c, 0, g, flag);
             hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.row_len, nl.row_start, n, flag,
-                               c->d_flag + 1);
+                               c->d_flag + 1, nl.cap_full, c->d_err, nl.half_len);
             launch_rows_g(c, 1, g, flag);
         } else {
             hipLaunchKernelGGL(k_rows_brute<0>, dim3(nb), dim3(256), 0, c->stream, s, nl.r_skin_sq, nl.row_len,
                                nl.lower, nl.half_len, nl.row_start, nl.nbr, flag, nl.cap_full, c->d_err);
             hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.row_len, nl.row_start, n, flag,
-                               c->d_flag + 1);
+                               c->d_flag + 1, nl.cap_full, c->d_err, nl.half_len);
             hipLaunchKernelGGL(k_rows_brute<1>, dim3(nb), dim3(256), 0, c->stream, s, nl.r_skin_sq, nl.row_len,
                                nl.lower, nl.half_len, nl.row_start, nl.nbr, flag, nl.cap_full, c->d_err);
         }

@@ -105,14 +105,20 @@ extern "C" int sx_run(sx_ctx *c, const sx_step_spec *spec, int nsteps)
             throw sx_error{SX_ERR_NOTEMP, "Error: Target temperature not set."};
         for (int f = 0; f < spec->nforces; ++f)
             if (is_ecp(spec->kinds[f])) sx_nlist_want_half(c);
+        // first_step < 0: continue from the device's Langevin counter (the
+        // step after the last one run, or the one a checkpoint restored)
+        const bool resume = spec->first_step < 0;
+        if (resume && !c->d_step)
+            throw sx_error{SX_ERR_STATE, "no Langevin step counter to continue from (first_step < 0)"};
         if (!c->d_step) HIPCHK(hipMalloc(&c->d_step, sizeof(int)));
-        hipLaunchKernelGGL(k_step_set, dim3(1), dim3(1), 0, c->stream, c->d_step, spec->first_step);
+        if (!resume) hipLaunchKernelGGL(k_step_set, dim3(1), dim3(1), 0, c->stream, c->d_step, spec->first_step);
         if (nsteps == 0) return;
         // SUCROSE_NO_GRAPH=1 enqueues the same launches eagerly (per-kernel tracing tools)
         const char *ng = getenv("SUCROSE_NO_GRAPH");
         if (ng && ng[0] == '1') {
+            sx_nlist_headroom(c);
             for (int t = 0; t < nsteps; ++t) enqueue_step(c, *spec);
-            sx_check_device_error(c);
+            sx_nlist_check_after_replay(c, true);
             return;
         }
         sx_step_spec key = *spec;
@@ -121,11 +127,14 @@ extern "C" int sx_run(sx_ctx *c, const sx_step_spec *spec, int nsteps)
             HIPCHK(hipStreamSynchronize(c->stream));
             if (c->graph_exec) { hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
             if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
-            // one eager step sizes every lazily allocated buffer
+            // room for the list to grow inside the replays, then one eager
+            // step sizes every lazily allocated buffer
+            sx_nlist_headroom(c);
             sx_step_spec warm = *spec;
             enqueue_step(c, warm);
-            sx_check_device_error(c);
-            hipLaunchKernelGGL(k_step_set, dim3(1), dim3(1), 0, c->stream, c->d_step, spec->first_step + 1);
+            sx_nlist_check_after_replay(c, false);
+            if (!resume)
+                hipLaunchKernelGGL(k_step_set, dim3(1), dim3(1), 0, c->stream, c->d_step, spec->first_step + 1);
             --nsteps;
             HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             try {
@@ -142,7 +151,7 @@ extern "C" int sx_run(sx_ctx *c, const sx_step_spec *spec, int nsteps)
             c->have_graph = true;
         }
         for (int t = 0; t < nsteps; ++t) HIPCHK(hipGraphLaunch(c->graph_exec, c->stream));
-        sx_check_device_error(c);
+        sx_nlist_check_after_replay(c, true);
     });
 }
 

@@ -133,11 +133,13 @@ def test_checkpoint_load_restores_state(tmp_path):
     for f, g in (("INV_MASS", "MASS"), ("IXX_INV", "IXX"), ("IYY_INV", "IYY"), ("IZZ_INV", "IZZ")):
         fresh.upload(f, 1.0 / fresh.download(g))
         dev.upload(f, 1.0 / dev.download(g))
+    # both continue their Langevin counters (first_step -1): the original
+    # from its own, the fresh context from the one the checkpoint restored
     for d in (dev, fresh):
         d.init_nlist(0.3, 2.5)
         d.initialize_forces()
         d.forces(sx.LJ, 1.0, 1.0, 2.5)
-        d.run(sx.INT_LGV, [(sx.LJ, 1.0, 1.0, 2.5)], 4, 0.004, damping=1.0, first_step=46)
+        d.run(sx.INT_LGV, [(sx.LJ, 1.0, 1.0, 2.5)], 4, 0.004, damping=1.0, first_step=-1)
     for f in ("RX", "VX", "QW", "LZ"):
         assert np.array_equal(dev.download(f), fresh.download(f)), f
     dev.close()
