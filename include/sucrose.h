@@ -457,6 +457,14 @@ int sx_ecp_compare_solvers(sx_ctx *ctx, int kind, const double *params, int npar
 void sx_philox4x32_10(const uint32_t ctr[4], uint32_t out[4]);
 /* device batch of philox_normal_3seq(i, c2, c3, c4) for i = 1..n -> out[3*n] */
 int sx_philox_normals(sx_ctx *ctx, int n, int c2, int c3, int c4, double *out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): sx_philox_normals with the same results, staged in a
+ * fixed per-GPU scratch array (4 MiB, set up by sx_create) instead of a buffer
+ * allocated per call.  n <= 174762 (SX_ERR_CAPACITY beyond).  It calls no
+ * hipMalloc or hipFree.  Calls on one GPU share its scratch: calls that
+ * overlap (other host threads, other contexts on the same GPU) race. */
+int sx_philox_normals_noalloc(sx_ctx *ctx, int n, int c2, int c3, int c4, double *out);
 
 #ifdef __cplusplus
 }

@@ -99,6 +99,7 @@ def lib():
             "sx_apply_constraints_a": [vp, C.c_double], "sx_apply_constraints_b": [vp, C.c_double],
             "sx_run": [vp, C.POINTER(StepSpec), C.c_int],
             "sx_philox_normals": [vp, C.c_int, C.c_int, C.c_int, C.c_int, _PD],
+            "sx_philox_normals_noalloc": [vp, C.c_int, C.c_int, C.c_int, C.c_int, _PD],
         }.items():
             getattr(L, name).argtypes = args
             getattr(L, name).restype = C.c_int
@@ -527,6 +528,12 @@ class System:
     def philox_normals(self, n, c2, c3, c4):
         out = np.zeros(3 * n)
         self._ok(self.L.sx_philox_normals(self.h, n, c2, c3, c4, _dp(out)))
+        return out.reshape(n, 3)
+
+    def philox_normals_noalloc(self, n, c2, c3, c4):
+        """philox_normals staged in the GPU's fixed scratch (no allocation)."""
+        out = np.zeros(3 * n)
+        self._ok(self.L.sx_philox_normals_noalloc(self.h, n, c2, c3, c4, _dp(out)))
         return out.reshape(n, 3)
 
     # ---- whole-state helpers ----

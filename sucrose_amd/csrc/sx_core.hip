@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -276,6 +277,42 @@ __global__ void k_philox_normals(int n, int c2, int c3, int c4, double *out)
     out[3 * i] = a; out[3 * i + 1] = b; out[3 * i + 2] = c;
 }
 
+// Device scratch of the allocation-free entry point sx_philox_normals_noalloc:
+// SX_SCRATCH_DOUBLES doubles (4 MiB) on every GPU the library runs on, held
+// until the process ends.  Host code never uses the array's name as an
+// address: sx_create (with its device current) asks hipGetSymbolAddress for
+// that device's copy once, and every use goes through g_scratch_dev[device].
+#define SX_SCRATCH_DOUBLES (1L << 19)
+#define SX_SCRATCH_DEVICES 64
+static __device__ __align__(256) double scratch[SX_SCRATCH_DOUBLES];
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[SX_SCRATCH_DEVICES];
+static std::string g_scratch_err[SX_SCRATCH_DEVICES];
+
+// the calling thread's current device must be `device`
+static void scratch_setup(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= SX_SCRATCH_DEVICES || g_scratch_dev[device]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return;
+    }
+    g_scratch_dev[device] = static_cast<double *>(p);
+}
+
+static double *scratch_address(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= SX_SCRATCH_DEVICES)
+        throw sx_error{SX_ERR_ARG, "no allocation-free scratch for device indices beyond 63"};
+    if (!g_scratch_dev[device])
+        throw sx_error{SX_ERR_HIP, "allocation-free scratch unavailable on this device: " + g_scratch_err[device]};
+    return g_scratch_dev[device];
+}
+
 // ---------------------------------------------------------------------------
 // host implementations
 // ---------------------------------------------------------------------------
@@ -325,6 +362,7 @@ int sx_create(int device, sx_ctx **out)
         HIPCHK(hipGetDeviceCount(&ndev));
         if (device < 0 || device >= ndev) throw sx_error{SX_ERR_ARG, "no such device"};
         HIPCHK(hipSetDevice(device));
+        scratch_setup(device);   // once per device: the address of its scratch array
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIPCHK(hipMalloc(&c->d_scal, sizeof(double) * SX_NSCALARS));
         HIPCHK(hipMalloc(&c->d_err, sizeof(int) * 4));
@@ -707,6 +745,24 @@ int sx_philox_normals(sx_ctx *c, int n, int c2, int c3, int c4, double *out)
         HIPCHK(hipMemcpyAsync(out, d, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipFree(d));
+    });
+}
+
+// sx_philox_normals with the device's scratch array as the staging buffer:
+// no allocation.  The kernel writes all 3 n values it stages, so nothing is
+// cleared first.  Calls on one GPU share its scratch: overlapping calls race.
+int sx_philox_normals_noalloc(sx_ctx *c, int n, int c2, int c3, int c4, double *out)
+{
+    return guard(c, [&] {
+        if (n <= 0 || !out) throw sx_error{SX_ERR_ARG, "bad philox request"};
+        if (3L * n > SX_SCRATCH_DOUBLES)
+            throw sx_error{SX_ERR_CAPACITY, "at most " + std::to_string(SX_SCRATCH_DOUBLES / 3) +
+                                                " normal triples per call without allocation"};
+        double *d = scratch_address(c->device);
+        hipLaunchKernelGGL(k_philox_normals, dim3(sx_blocks(n, 256)), dim3(256), 0, c->stream, n, c2, c3, c4, d);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, d, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
     });
 }
 
