@@ -409,8 +409,10 @@ int sx_destroy(sx_ctx *c)
     sx_ecp_mlp_free(c);
     sx_batch_free(c);
     sx_composite_free(c);
-    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
-    if (c->graph) hipGraphDestroy(c->graph);
+    for (int k = 0; k < 4; ++k) {
+        if (c->graph_exec[k]) hipGraphExecDestroy(c->graph_exec[k]);
+        if (c->graph[k]) hipGraphDestroy(c->graph[k]);
+    }
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return SX_OK;

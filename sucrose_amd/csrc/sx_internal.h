@@ -170,9 +170,11 @@ struct sx_ctx {
     void *mlp = nullptr;
     // batched replicas (sx_batch_set)
     void *batch = nullptr;
-    // graph
-    hipGraphExec_t graph_exec = nullptr;
-    hipGraph_t graph = nullptr;
+    // step graphs of SX_GRAPH_CHUNK[k] whole steps each (captured on demand;
+    // a replay launch has a fixed cost of ~13 us, so long runs replay the
+    // longest chunks); all belong to graph_spec while have_graph holds
+    hipGraphExec_t graph_exec[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
     sx_step_spec graph_spec;
     bool have_graph = false;
     DState dstate() const;

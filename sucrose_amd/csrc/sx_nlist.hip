@@ -87,18 +87,12 @@ __global__ void k_cell_assign(DState s, CellGrid g, int *cell_of, int *cell_coun
     atomicAdd(&cell_count[c], 1);
 }
 
-// exclusive scan of n ints by one work-group; out has n+1 entries; optional
-// conditional flag (skip when *flag == 0).  With cap > 0 a total beyond cap
-// leaves an empty list instead: out[0..n] (and zero_also[0..n-1]) all zero,
-// *total the count that did not fit (for the host to grow the list) and the
-// capacity error raised, so that no later kernel indexes past the list.
-__global__ void __launch_bounds__(1024) k_scan(const int *in, int *out, int n, const int *flag, int *total,
-                                               long cap, int *err, int *zero_also)
+// exclusive scan of n ints by one work-group into out[0..n-1]; returns the
+// total (the same in every thread).  Tiles of 4096: 4 consecutive elements
+// per thread, wavefront shuffle scans, one LDS pass over the 16 wavefront
+// totals.
+__device__ long scan_block(const int *in, int *out, int n, int *wsum)
 {
-    if (flag && *flag == 0) return;
-    // tiles of 4096: 4 consecutive elements per thread, wavefront shuffle
-    // scans, one LDS pass over the 16 wavefront totals
-    __shared__ int wsum[16];
     int t = threadIdx.x, lane = t & 63, w = t >> 6;
     long carry = 0;
     for (long base = 0; base < n; base += 4096) {
@@ -131,6 +125,24 @@ __global__ void __launch_bounds__(1024) k_scan(const int *in, int *out, int n, c
         __syncthreads();
         carry += tile;
     }
+    return carry;
+}
+
+// exclusive scan of n ints by one work-group; out has n+1 entries; optional
+// conditional flag (skip when *flag == 0).  With cap > 0 a total beyond cap
+// leaves an empty list instead: out[0..n] (and zero_also[0..n-1]) all zero,
+// *total the count that did not fit (for the host to grow the list) and the
+// capacity error raised, so that no later kernel indexes past the list.
+// in2 (may be NULL): a second array of n ints scanned the same way into
+// out2[0..n] after the first (and after its zeroing), its total in *total2.
+__global__ void __launch_bounds__(1024) k_scan(const int *in, int *out, int n, const int *flag, int *total,
+                                               long cap, int *err, int *zero_also, const int *in2, int *out2,
+                                               int *total2)
+{
+    if (flag && *flag == 0) return;
+    __shared__ int wsum[16];
+    const int t = threadIdx.x;
+    const long carry = scan_block(in, out, n, wsum);
     // carry is the same in every thread, so the branch is uniform
     if (cap > 0 && carry > cap) {
         for (long i = t; i <= n; i += blockDim.x) {
@@ -141,11 +153,16 @@ __global__ void __launch_bounds__(1024) k_scan(const int *in, int *out, int n, c
             if (total) *total = carry > 0x7fffffffL ? 0x7fffffff : (int)carry;
             atomicCAS(err, 0, DERR_NLIST_CAP);
         }
-        return;
-    }
-    if (t == 0) {
+        __syncthreads();   // the zeros are what the second scan reads
+    } else if (t == 0) {
         out[n] = (int)carry;
         if (total) *total = (int)carry;
+    }
+    if (!in2) return;
+    const long carry2 = scan_block(in2, out2, n, wsum);
+    if (t == 0) {
+        out2[n] = (int)carry2;
+        if (total2) *total2 = (int)carry2;
     }
 }
 
@@ -736,7 +753,8 @@ static void half_index_launch(sx_ctx *c, const int *flag)
 {
     NList &nl = c->nl;
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.half_len, nl.half_start, c->n, flag,
-                       c->d_flag + 2, 0L, (int *)nullptr, (int *)nullptr);
+                       c->d_flag + 2, 0L, (int *)nullptr, (int *)nullptr, (const int *)nullptr, (int *)nullptr,
+                       (int *)nullptr);
     launch_half_index_g(c, flag);
 }
 
@@ -757,6 +775,20 @@ void sx_nlist_want_half(sx_ctx *c)
 // Builds the list.  conditional: every kernel reads the device rebuild flag
 // (graph replay); otherwise the build is unconditional and capacity grows as
 // needed.
+static CellGrid cell_grid(const sx_ctx *c)
+{
+    const NList &nl = c->nl;
+    CellGrid g;
+    g.fast = 1;
+    for (int d = 0; d < 3; ++d) {
+        g.nc[d] = nl.ncell[d];
+        g.w[d] = c->box[d] / nl.ncell[d];
+        g.inv[d] = 1.0 / c->box[d];
+        if (!(nl.cutoff + nl.skin < 0.5 * c->box[d] * (1.0 - 1e-12))) g.fast = 0;
+    }
+    return g;
+}
+
 void sx_nlist_build_impl(sx_ctx *c, bool conditional)
 {
     NList &nl = c->nl;
@@ -766,31 +798,31 @@ void sx_nlist_build_impl(sx_ctx *c, bool conditional)
     int nb = sx_blocks(n, 256);
     for (int attempt = 0; attempt < 4; ++attempt) {
         if (nl.use_cells) {
-            CellGrid g;
-            g.fast = 1;
-            for (int d = 0; d < 3; ++d) {
-                g.nc[d] = nl.ncell[d];
-                g.w[d] = c->box[d] / nl.ncell[d];
-                g.inv[d] = 1.0 / c->box[d];
-                if (!(nl.cutoff + nl.skin < 0.5 * c->box[d] * (1.0 - 1e-12))) g.fast = 0;
-            }
+            CellGrid g = cell_grid(c);
             // the cell counts are zero between builds (k_cell_scatter counts
             // them back down)
             hipLaunchKernelGGL(k_cell_assign, dim3(nb), dim3(256), 0, c->stream, s, g, nl.cell_of, nl.cell_count,
                                nl.x_last, nl.y_last, nl.z_last, flag, conditional ? c->d_counts : (long *)nullptr);
             hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.cell_count, nl.cell_start, nl.ncells,
-                               flag, (int *)nullptr, 0L, (int *)nullptr, (int *)nullptr);
+                               flag, (int *)nullptr, 0L, (int *)nullptr, (int *)nullptr, (const int *)nullptr,
+                               (int *)nullptr, (int *)nullptr);
             hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(256), 0, c->stream, s, nl.cell_of, nl.cell_start,
                                nl.cell_count, nl.cell_items, (double4 *)nl.cell_pos, flag);
             launch_rows_g(c, 0, g, flag);
-            hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.row_len, nl.row_start, n, flag,
-                               c->d_flag + 1, nl.cap_full, c->d_err, nl.half_len);
-            launch_rows_g(c, 1, g, flag);
         } else {
             hipLaunchKernelGGL(k_rows_brute<0>, dim3(nb), dim3(256), 0, c->stream, s, nl.r_skin_sq, nl.row_len,
                                nl.lower, nl.half_len, nl.row_start, nl.nbr, flag, nl.cap_full, c->d_err);
-            hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.row_len, nl.row_start, n, flag,
-                               c->d_flag + 1, nl.cap_full, c->d_err, nl.half_len);
+        }
+        // row starts, and in the same launch the half-list starts (d_flag[2]
+        // = the half list's length) when the half list is kept
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, nl.row_len, nl.row_start, n, flag,
+                           c->d_flag + 1, nl.cap_full, c->d_err, nl.half_len,
+                           nl.need_half ? (const int *)nl.half_len : (const int *)nullptr, nl.half_start,
+                           c->d_flag + 2);
+        if (nl.use_cells) {
+            CellGrid g = cell_grid(c);
+            launch_rows_g(c, 1, g, flag);
+        } else {
             hipLaunchKernelGGL(k_rows_brute<1>, dim3(nb), dim3(256), 0, c->stream, s, nl.r_skin_sq, nl.row_len,
                                nl.lower, nl.half_len, nl.row_start, nl.nbr, flag, nl.cap_full, c->d_err);
         }
@@ -808,7 +840,7 @@ void sx_nlist_build_impl(sx_ctx *c, bool conditional)
     }
     hipLaunchKernelGGL(k_sort_rows, dim3(std::min(sx_blocks(n, 4), 2048)), dim3(256), 0, c->stream, n, nl.row_start,
                        nl.nbr, flag, c->d_err);
-    if (nl.need_half) half_index_launch(c, flag);
+    if (nl.need_half) launch_half_index_g(c, flag);   // half_start came with the row scan
     if (!nl.use_cells)
         hipLaunchKernelGGL(k_save_last, dim3(nb), dim3(256), 0, c->stream, s, nl.x_last, nl.y_last, nl.z_last, flag,
                            conditional ? c->d_counts : (long *)nullptr);

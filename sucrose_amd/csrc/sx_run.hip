@@ -95,6 +95,35 @@ static bool same_spec(const sx_step_spec &a, const sx_step_spec &b)
     return memcmp(&a, &b, sizeof(sx_step_spec)) == 0;
 }
 
+// steps per captured graph: a replay launch costs ~13 us on its own and each
+// kernel node ~1 us more, so a run replays the longest chunks that fit
+static const int SX_GRAPH_CHUNK[4] = {1, 4, 16, 64};
+
+static void drop_graphs(sx_ctx *c)
+{
+    for (int k = 0; k < 4; ++k) {
+        if (c->graph_exec[k]) { hipGraphExecDestroy(c->graph_exec[k]); c->graph_exec[k] = nullptr; }
+        if (c->graph[k]) { hipGraphDestroy(c->graph[k]); c->graph[k] = nullptr; }
+    }
+    c->have_graph = false;
+}
+
+// captures SX_GRAPH_CHUNK[k] consecutive steps as one graph
+static void capture_chunk(sx_ctx *c, const sx_step_spec &sp, int k)
+{
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    try {
+        for (int t = 0; t < SX_GRAPH_CHUNK[k]; ++t) enqueue_step(c, sp);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        hipStreamEndCapture(c->stream, &g);
+        if (g) hipGraphDestroy(g);
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(c->stream, &c->graph[k]));
+    HIPCHK(hipGraphInstantiate(&c->graph_exec[k], c->graph[k], nullptr, nullptr, 0));
+}
+
 extern "C" int sx_run(sx_ctx *c, const sx_step_spec *spec, int nsteps)
 {
     return guard(c, [&] {
@@ -125,8 +154,7 @@ extern "C" int sx_run(sx_ctx *c, const sx_step_spec *spec, int nsteps)
         key.first_step = 0;
         if (!c->have_graph || !same_spec(key, c->graph_spec)) {
             HIPCHK(hipStreamSynchronize(c->stream));
-            if (c->graph_exec) { hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
-            if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+            drop_graphs(c);
             // room for the list to grow inside the replays, then one eager
             // step sizes every lazily allocated buffer
             sx_nlist_headroom(c);
@@ -136,21 +164,17 @@ extern "C" int sx_run(sx_ctx *c, const sx_step_spec *spec, int nsteps)
             if (!resume)
                 hipLaunchKernelGGL(k_step_set, dim3(1), dim3(1), 0, c->stream, c->d_step, spec->first_step + 1);
             --nsteps;
-            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            try {
-                enqueue_step(c, *spec);
-            } catch (...) {
-                hipGraph_t g;
-                hipStreamEndCapture(c->stream, &g);
-                if (g) hipGraphDestroy(g);
-                throw;
-            }
-            HIPCHK(hipStreamEndCapture(c->stream, &c->graph));
-            HIPCHK(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
             c->graph_spec = key;
             c->have_graph = true;
         }
-        for (int t = 0; t < nsteps; ++t) HIPCHK(hipGraphLaunch(c->graph_exec, c->stream));
+        // the remaining steps as the longest captured chunks that fit
+        for (int k = 3; k >= 0; --k) {
+            while (nsteps >= SX_GRAPH_CHUNK[k]) {
+                if (!c->graph_exec[k]) capture_chunk(c, *spec, k);
+                HIPCHK(hipGraphLaunch(c->graph_exec[k], c->stream));
+                nsteps -= SX_GRAPH_CHUNK[k];
+            }
+        }
         sx_nlist_check_after_replay(c, true);
     });
 }
