@@ -41,10 +41,121 @@ TRAFFIC_BYTES = {
     "ecp_solve": (12637.2 * 2.0 + 6209.3) * 1024,
     "ecp_apply": (12423.7 * 2.0 + 3977.4) * 1024,
 }
+# FP64 operations issued per launch of each ECP stage kernel at this workload:
+# rocprofv3 --pmc SQ_INSTS_VALU_FLOPS_FP64 (counted per wavefront instruction,
+# FMA = 2, transcendental included) times the 64 lanes of a wavefront; the run
+# is in profiles/ecp_langevin_32768_fp64_counters.txt
+ISSUED_FP64 = {
+    "ecp_precompute": 1.224e5 * 64,
+    "ecp_screen": 1.657e5 * 64,
+    "ecp_solve": 2.565e6 * 64,
+    "ecp_apply": 2.670e5 * 64,
+}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def host_cpu():
+    """The host CPU's model name and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        ncpu = os.cpu_count() or 1
+    return model, ncpu
+
+
+def cpu_all_cores(state_arrays, box, params, seconds):
+    """The C restatement of the same loop with its pair terms and list build
+    spread over all host threads (liboracle_omp.so; results equal the serial
+    loop's bit for bit): the all-cores CPU figure, kind "port"."""
+    from oracle.oracle import ECPR, FIELDS, LGV, Oracle, State
+    path = os.path.join(ROOT, "oracle", "liboracle_omp.so")
+    if not os.path.exists(path):
+        return None
+    n = params["n"]
+    st = State(n, [box, box, box])
+    for k, f in enumerate(FIELDS):
+        st.buf[:, k] = state_arrays[f.upper()]
+    st["target_temp"] = params["T"]
+    st["dof"] = 6 * n - 3
+    st["density"] = n / box ** 3
+    T = Oracle(path)
+    nl = T.nlist_new(st, params["cutoff"], params["skin"])
+    T.nlist_build_threads(st, nl)
+    dt, gamma, step0 = params["dt"], params["gamma"], params["next_step"]
+    steps = 0
+    t0 = time.time()
+    while time.time() - t0 < seconds or steps == 0:
+        T.integrate(LGV, 0, st, dt, step=step0 + steps, damping=gamma)
+        if T.needs_rebuild(st, nl):
+            T.nlist_build_threads(st, nl)
+        T.call("or_initialize_forces", st)
+        T.pair_loop_threads(st, nl, ECPR, [0.0])
+        T.integrate(LGV, 1, st, dt, damping=gamma)
+        T.call("or_calculate_thermo", st)
+        steps += 1
+    el = time.time() - t0
+    return {"value": n * steps / el, "unit": "particle-steps/s", "cores": T.threads(), "kind": "port",
+            "sample": f"{steps} steps of the C restatement of the reference loop (lgv + ecpr pair loop over its "
+                      f"Verlet list, reference rebuild rule) on the same {n}-particle state, pair terms and list "
+                      f"build over {T.threads()} OpenMP threads, accumulation in the loop's order",
+            "seconds": el}
+
+
+def chains_cpu_baseline(state, p, seconds):
+    """The reference's own charged-chain loop (oracle/ref_harness.f90
+    ref_run_chain_steps: lgvc_initial_step with RATTLE, its Verlet list, WCA
+    and Debye-Hueckel over it, cosine_bend_ and dih_ terms, lgvc_final_step
+    with RATTLE, calculate_thermo) on the device's state, in a private copy of
+    the reference library."""
+    import numpy as np
+    from oracle.oracle import FIELDS, Reference, State, reference_available
+    if not reference_available(fast=True):
+        return None
+    n, L = p["n"], p["box"]
+    st = State(n, [L, L, L])
+    for k, f in enumerate(FIELDS):
+        st.buf[:, k] = state[f.upper()]
+    ids = np.arange(n).reshape(p["nchains"], p["nbeads"])
+    st.bbi = ids[:, :-1].ravel().astype(np.int32)
+    st.bbj = ids[:, 1:].ravel().astype(np.int32)
+    st.bb_len = np.full(len(st.bbi), 1.0)
+    st["target_temp"] = p["T"]
+    st["dof"] = 6 * n - st.nc - 3
+    st["density"] = n / L ** 3
+    bends = np.stack([ids[:, :-2], ids[:, 1:-1], ids[:, 2:]], -1).reshape(-1, 3)
+    dihs = np.stack([ids[:, :-3], ids[:, 1:-2], ids[:, 2:-1], ids[:, 3:]], -1).reshape(-1, 4)
+    wca = p["forces"][0][1:3]
+    dh = p["forces"][1][1:4]
+    R = Reference(fast=True, private=True)
+    R.load(st)
+    t0 = time.time()
+    R.nlist_init(p["skin"], dh[0])
+    log(f"[chains cpu] reference init_nlist {time.time() - t0:.1f}s")
+    step = p["next_step"]
+    t0 = time.time()
+    kb, ab, kd = p["k_bend"], p["bend_angle"], p["k_dih"]
+    R.run_chain_steps(1, step, p["dt"], p["gamma"], wca, dh, bends, kb, ab, dihs, kd)
+    t1 = max(time.time() - t0, 1e-3)
+    k = int(max(1, min(30, seconds / t1)))
+    t0 = time.time()
+    R.run_chain_steps(k, step + 1, p["dt"], p["gamma"], wca, dh, bends, kb, ab, dihs, kd)
+    el = time.time() - t0
+    return {"value": n * k / el, "unit": "particle-steps/s", "cores": 1, "kind": "reference",
+            "sample": f"{k} steps of the reference's charged-chain loop (lgvc + RATTLE, wca and debye_huckel over "
+                      f"its Verlet list, cosine_bend_ and dih_ terms) on the same {n}-bead state after one untimed "
+                      f"step", "seconds": el}
 
 
 def cpu_baseline(state_arrays, box, params, seconds):
@@ -180,6 +291,11 @@ def other_workloads(sx, args):
                           "bonded": "cosine_bend k=20 about straight + dih k(1+cos) k=1", "constraints": "rattle",
                           "integrator": "lgvc", "dtype": "f64"}}
         log(f"[chains] {args.chain_steps} steps in {el:.3f}s -> {rec['value']:.4e} particle-steps/s")
+        if not args.no_cpu:
+            cpu = chains_cpu_baseline(d.state(), p, 15.0)
+            if cpu:
+                rec["cpu_baseline"] = cpu
+                log(f"[chains cpu] reference {cpu['value']:.4e} particle-steps/s ({cpu['seconds']:.1f}s)")
         out["chains_100000"] = rec
         d.close()
     if args.batch_steps > 0:
@@ -440,10 +556,15 @@ def main():
     names = list(flops)
     k = max(range(4), key=lambda q: ms[q])
     ach = flops[names[k]] / (ms[k] * 1e-3) / 1e12
-    roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+    issued = ISSUED_FP64.get(names[k])
+    roof = {"bound": "fp64-valu", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "frac": ach / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_BYTES.get(names[k]), "kernel": names[k],
             "traffic_unit": "bytes per launch (L2 misses: rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
-            "peak_kind": "fp64 (vector ALU; MI355X fp64 dense figure)",
+            "peak_kind": "fp64 vector ALU (MI355X fp64 dense figure); no matrix-core work in this kernel",
+            "issued_fp64_flops_per_launch": issued,
+            "issued_achieved": issued / (ms[k] * 1e-3) / 1e12 if issued else None,
+            "issued_note": "rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes: every issued FP64 operation of the "
+                           "kernel, inactive lanes and the division/square-root refinement steps included",
             "stage_ms": {names[q]: ms[q] for q in range(4)},
             "pairs_screened": screened, "pairs_solved": solved, "brent_iters_per_solved_pair": iters / max(solved, 1)}
     log(f"[gpu] stages ms {roof['stage_ms']} screened={screened:.0f} solved={solved:.0f} "
@@ -463,7 +584,12 @@ def main():
         p["next_step"] = step
         cpu = cpu_baseline(arrays, p["box"], p, args.cpu_seconds)
         cpu["cores"] = 1
-        log(f"[cpu] {cpu['kind']} {cpu['value']:.4e} particle-steps/s ({cpu['seconds']:.1f}s)")
+        cpu["cpu_model"], cpu["host_cpus"] = host_cpu()
+        log(f"[cpu] {cpu['kind']} {cpu['value']:.4e} particle-steps/s ({cpu['seconds']:.1f}s) on {cpu['cpu_model']}")
+        allc = cpu_all_cores(arrays, p["box"], p, min(10.0, args.cpu_seconds))
+        if allc:
+            cpu["all_cores"] = allc
+            log(f"[cpu] all cores ({allc['cores']} threads) {allc['value']:.4e} particle-steps/s")
 
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,

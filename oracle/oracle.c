@@ -10,6 +10,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define PI_REF 3.1415926535897932384626433832795 /* xmath.f90:120 (-r8) */
 
@@ -1880,14 +1883,17 @@ int or_apply_constraints_b(or_sys *s, double dt)
 /* ------------------------------------------------------------------ */
 /* README pair loop (README.md:86-103) over the half list               */
 /* ------------------------------------------------------------------ */
-void or_pair_loop(or_sys *s, const or_nlist *nl, int kind, const double *p)
+/* one pair (i, j) of kind's routine; returns 1 for the kinds with torques,
+   -1 for an unknown kind */
+static int pair_terms(const or_sys *s, int i, int j, int kind, const double *p, double *pe_out, double f[3],
+                      double ti[3], double tj[3])
 {
-    for (int i = 0; i < s->n - 1; ++i) {
-        for (long k = nl->offset[i]; k < nl->offset[i + 1]; ++k) {
-            int j = nl->nbr[k];
-            double pe = 0.0, f[3] = {0, 0, 0}, ti[3] = {0, 0, 0}, tj[3] = {0, 0, 0};
-            int torque = 0;
-            switch (kind) {
+    double pe = 0.0;
+    int torque = 0;
+    f[0] = f[1] = f[2] = 0.0;
+    ti[0] = ti[1] = ti[2] = 0.0;
+    tj[0] = tj[1] = tj[2] = 0.0;
+    switch (kind) {
             case 1: or_lj(s, i, j, p[0], p[1], p[2], &pe, f); break;
             case 2: or_lj12(s, i, j, p[0], p[1], p[2], &pe, f); break;
             case 3: or_ljw(s, i, j, p[0], p[1], p[2], p[3], &pe, f); break;
@@ -1905,8 +1911,21 @@ void or_pair_loop(or_sys *s, const or_nlist *nl, int kind, const double *p)
             case 15: ecp_eval(s, i, j, 3, p[0], 0.0, p[1], -1.0, &pe, f, ti, tj, NULL, NULL); torque = 1; break;
             case 16: ecp_eval(s, i, j, 4, p[0], p[1], p[2], -1.0, &pe, f, ti, tj, NULL, NULL); torque = 1; break;
             case 18: or_coulomb(s, i, j, p[0], p[1], &pe, f); break;
-            default: return;
+            default: return -1;
             }
+    *pe_out = pe;
+    return torque;
+}
+
+/* the caller's pair loop (README.md:86-103) over the half list */
+void or_pair_loop(or_sys *s, const or_nlist *nl, int kind, const double *p)
+{
+    for (int i = 0; i < s->n - 1; ++i) {
+        for (long k = nl->offset[i]; k < nl->offset[i + 1]; ++k) {
+            int j = nl->nbr[k];
+            double pe, f[3], ti[3], tj[3];
+            const int torque = pair_terms(s, i, j, kind, p, &pe, f, ti, tj);
+            if (torque < 0) return;
             s->pot_eng = s->pot_eng + pe;
             s->fx[i] = s->fx[i] + f[0]; s->fy[i] = s->fy[i] + f[1]; s->fz[i] = s->fz[i] + f[2];
             s->fx[j] = s->fx[j] - f[0]; s->fy[j] = s->fy[j] - f[1]; s->fz[j] = s->fz[j] - f[2];
@@ -1916,4 +1935,90 @@ void or_pair_loop(or_sys *s, const or_nlist *nl, int kind, const double *p)
             }
         }
     }
+}
+
+/* ------------------------------------------------------------------ */
+/* all-cores variants for the CPU baseline (built with -fopenmp into    */
+/* liboracle_omp.so; without OpenMP they run on one thread)             */
+/* ------------------------------------------------------------------ */
+/* or_pair_loop with the pair terms evaluated by all threads into buf
+   (10 doubles per half-list entry, buf sized 10 nnz) and accumulated in the
+   loop's order afterwards: the same bits as or_pair_loop */
+void or_pair_loop_threads(or_sys *s, const or_nlist *nl, int kind, const double *p, double *buf)
+{
+    const int n = s->n;
+    int torque = 0;
+    /* every row's entries are independent: rows are dealt to threads */
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : torque)
+    for (int i = 0; i < n - 1; ++i) {
+        for (long k = nl->offset[i]; k < nl->offset[i + 1]; ++k) {
+            double *r = buf + 10 * k;
+            int t = pair_terms(s, i, nl->nbr[k], kind, p, &r[0], &r[1], &r[4], &r[7]);
+            if (t > 0) torque = 1;
+        }
+    }
+    for (int i = 0; i < n - 1; ++i) {
+        for (long k = nl->offset[i]; k < nl->offset[i + 1]; ++k) {
+            const int j = nl->nbr[k];
+            const double *r = buf + 10 * k;
+            s->pot_eng = s->pot_eng + r[0];
+            s->fx[i] = s->fx[i] + r[1]; s->fy[i] = s->fy[i] + r[2]; s->fz[i] = s->fz[i] + r[3];
+            s->fx[j] = s->fx[j] - r[1]; s->fy[j] = s->fy[j] - r[2]; s->fz[j] = s->fz[j] - r[3];
+            if (torque) {
+                s->tx[i] = s->tx[i] + r[4]; s->ty[i] = s->ty[i] + r[5]; s->tz[i] = s->tz[i] + r[6];
+                s->tx[j] = s->tx[j] + r[7]; s->ty[j] = s->ty[j] + r[8]; s->tz[j] = s->tz[j] + r[9];
+            }
+        }
+    }
+}
+
+/* or_nlist_build with the rows counted and filled by all threads (row counts
+   in cnt[n]): the same list */
+int or_nlist_build_threads(or_sys *s, or_nlist *nl, long *cnt)
+{
+    double rskin = nl->cutoff + nl->skin;
+    nl->r_skin_sq = rskin * rskin;
+    const int n = s->n;
+    for (int pass = 0; pass < 2; ++pass) {
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int i = 0; i < n - 1; ++i) {
+            long K = pass ? nl->offset[i] : 0;
+            for (int j = i + 1; j < n; ++j) {
+                double d0 = s->rx[i] - s->rx[j], d1 = s->ry[i] - s->ry[j], d2 = s->rz[i] - s->rz[j];
+                if (s->periodic) {
+                    d0 = mimg(d0, s->box[0]); d1 = mimg(d1, s->box[1]); d2 = mimg(d2, s->box[2]);
+                }
+                double rsq = d0 * d0 + d1 * d1 + d2 * d2;
+                if (rsq < nl->r_skin_sq) {
+                    if (pass) nl->nbr[K] = j;
+                    ++K;
+                }
+            }
+            if (!pass) cnt[i] = K;
+        }
+        if (!pass) {
+            long K = 0;
+            for (int i = 0; i < n - 1; ++i) {
+                nl->offset[i] = K;
+                K += cnt[i];
+            }
+            if (K > nl->cap) return -1;
+            if (n >= 1) nl->offset[n - 1] = K;
+            nl->offset[n] = K;
+            nl->nnz = K;
+        }
+    }
+    memcpy(nl->x_last, s->rx, sizeof(double) * n);
+    memcpy(nl->y_last, s->ry, sizeof(double) * n);
+    memcpy(nl->z_last, s->rz, sizeof(double) * n);
+    return 0;
+}
+
+int or_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }

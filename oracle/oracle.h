@@ -165,6 +165,11 @@ double or_exprel(double x);
 /* kind codes as in include/sucrose.h: 1 LJ, 2 LJ12, 3 LJW, 4 WCA, 5 MORSE, 6 GB, 7 GBR,
  * 8 ECP, 9 ECPR, 10 DH.  p = potential parameters (see sx_forces_* in sucrose.h). */
 void or_pair_loop(or_sys *s, const or_nlist *nl, int kind, const double *p);
+/* all-cores variants (CPU baseline): the same results as or_pair_loop and
+   or_nlist_build; buf holds 10 doubles per half-list entry, cnt n longs */
+void or_pair_loop_threads(or_sys *s, const or_nlist *nl, int kind, const double *p, double *buf);
+int or_nlist_build_threads(or_sys *s, or_nlist *nl, long *cnt);
+int or_threads(void);
 
 #ifdef __cplusplus
 }

@@ -325,6 +325,32 @@ class Oracle:
         self.lib.or_pair_loop(C.byref(s), C.byref(nls), kind, _dp(pa))
         self._back(s, st)
 
+    # -- all-cores variants (liboracle_omp.so): the same results --
+    def threads(self):
+        return int(self.lib.or_threads())
+
+    def pair_loop_threads(self, st: State, nl: dict, kind, p):
+        s = self._sys(st)
+        nls = self._nl(st, nl)
+        pa = np.asarray(p, dtype=np.float64)
+        buf = nl.get("_buf")
+        if buf is None or len(buf) < 10 * max(nl["nnz"], 1):
+            buf = nl["_buf"] = np.zeros(10 * max(nl["nnz"], 1) + 1024)
+        self.lib.or_pair_loop_threads(C.byref(s), C.byref(nls), kind, _dp(pa), _dp(buf))
+        self._back(s, st)
+
+    def nlist_build_threads(self, st: State, nl: dict):
+        cnt = np.zeros(st.n + 1, np.int64)
+        while True:
+            s = self._sys(st)
+            o = self._nl(st, nl)
+            r = self.lib.or_nlist_build_threads(C.byref(s), C.byref(o), cnt.ctypes.data_as(_PL))
+            if r == 0:
+                nl["nnz"] = o.nnz
+                nl["r_skin_sq"] = o.r_skin_sq
+                return nl
+            nl["nbr"] = np.zeros(2 * len(nl["nbr"]), np.int32)
+
     # -- neighbour list --
     def _nl(self, st, nl):
         o = OrNlist()
@@ -661,3 +687,16 @@ class Reference:
         pa = np.zeros(8)
         pa[:len(p)] = p
         self.lib.ref_run_steps(integ, fkind, _dp(pa), nsteps, step0, dt, damping)
+
+    def run_chain_steps(self, nsteps, step0, dt, damping, wca, dh, bends0, kbend, abend, dihs0, kdih):
+        """The charged-chain loop of ref_harness.f90 (lgvc + WCA and DH over
+        the list + cosine_bend_ and dih_ terms); bends0 (m, 3) and dihs0 (m, 4)
+        hold 0-based atoms."""
+        pw = np.asarray(wca, dtype=np.float64)
+        pd = np.asarray(dh, dtype=np.float64)
+        b = np.ascontiguousarray(np.asarray(bends0) + 1, dtype=np.int32)
+        d = np.ascontiguousarray(np.asarray(dihs0) + 1, dtype=np.int32)
+        self.lib.ref_run_chain_steps(C.c_int(nsteps), C.c_int(step0), C.c_double(dt), C.c_double(damping),
+                                     _dp(pw), _dp(pd), C.c_int(len(b)), b.ctypes.data_as(_PI),
+                                     C.c_double(kbend), C.c_double(abend), C.c_int(len(d)),
+                                     d.ctypes.data_as(_PI), C.c_double(kdih))

@@ -410,6 +410,65 @@ contains
         enddo
     end subroutine ref_run_steps
 
+    ! ---- the charged-chain loop: lgvc_initial_step (with apply_constraints_a),
+    !      the rebuild check, one pass over the list calling
+    !      wca_calculate_forces and dh_calculate_forces for each pair, the
+    !      cosine_bend_ and dih_ terms, lgvc_final_step (with
+    !      apply_constraints_b), calculate_thermo.  pw = (sigma, epsilon),
+    !      pd = (cutoff, db_len, dielectric); bend(3, nbend) and dih(4, ndih)
+    !      hold 1-based atoms. ----
+    subroutine ref_run_chain_steps(nsteps, step0, dt, damping, pw, pd, nbend, bend, kbend, abend, ndih, dih, kdih) &
+            bind(C, name="ref_run_chain_steps")
+        integer(c_int), value :: nsteps, step0, nbend, ndih
+        real(c_double), value :: dt, damping, kbend, abend, kdih
+        real(c_double), intent(in) :: pw(2), pd(3)
+        integer(c_int), intent(in) :: bend(3, nbend), dih(4, ndih)
+        integer :: t, I, K, J, m
+        real(real64) :: pe, f(3), f4(3, 4)
+        do t = 1, nsteps
+            call lgvc_initial_step(dt, step0 + t - 1, damping)
+            if (needs_rebuild()) call build_nlist
+            call initialize_forces
+            do I = 1, N - 1
+                do K = offset(I), offset(I + 1) - 1
+                    J = nlist(K)
+                    call wca_calculate_forces(I, J, pw(1), pw(2), pe, f)
+                    pot_eng = pot_eng + pe
+                    FX(I) = FX(I) + f(1); FY(I) = FY(I) + f(2); FZ(I) = FZ(I) + f(3)
+                    FX(J) = FX(J) - f(1); FY(J) = FY(J) - f(2); FZ(J) = FZ(J) - f(3)
+                    call dh_calculate_forces(I, J, pd(1), pd(2), pd(3), pe, f)
+                    pot_eng = pot_eng + pe
+                    FX(I) = FX(I) + f(1); FY(I) = FY(I) + f(2); FZ(I) = FZ(I) + f(3)
+                    FX(J) = FX(J) - f(1); FY(J) = FY(J) - f(2); FZ(J) = FZ(J) - f(3)
+                enddo
+            enddo
+            do m = 1, nbend
+                f4 = 0.0
+                call cosine_bend_calculate_forces(bend(1, m), bend(2, m), bend(3, m), pe, f4(:, 1), f4(:, 2), &
+                                                  f4(:, 3), kbend, abend)
+                pot_eng = pot_eng + pe
+                do K = 1, 3
+                    FX(bend(K, m)) = FX(bend(K, m)) + f4(1, K)
+                    FY(bend(K, m)) = FY(bend(K, m)) + f4(2, K)
+                    FZ(bend(K, m)) = FZ(bend(K, m)) + f4(3, K)
+                enddo
+            enddo
+            do m = 1, ndih
+                f4 = 0.0
+                call dih_calculate_forces(dih(1, m), dih(2, m), dih(3, m), dih(4, m), pe, f4(:, 1), f4(:, 2), &
+                                          f4(:, 3), f4(:, 4), kdih)
+                pot_eng = pot_eng + pe
+                do K = 1, 4
+                    FX(dih(K, m)) = FX(dih(K, m)) + f4(1, K)
+                    FY(dih(K, m)) = FY(dih(K, m)) + f4(2, K)
+                    FZ(dih(K, m)) = FZ(dih(K, m)) + f4(3, K)
+                enddo
+            enddo
+            call lgvc_final_step(dt, damping)
+            call calculate_thermo
+        enddo
+    end subroutine ref_run_chain_steps
+
     ! ---- reporters (reporters.f90): kind 1 state data, 2 trajectory,
     !      3 config, 4 checkpoint; one frame written to a new file ----
     function cstr(c, n) result(f)

@@ -185,7 +185,7 @@ def chains_state(dev: System, nchains=2000, nbeads=50, rho=0.5, seed=7, T=1.0, d
     log(f"[workload] chains {nchains}x{nbeads} (L={L:.4f}) equilibrated {equil_steps} steps, PE/N={pe / n:.4f} "
         f"T={dev.get_scalar('TEMP'):.4f}, {time.time() - t0:.1f}s")
     return dict(n=n, box=L, rho=rho, T=T, dt=dt, gamma=gamma, skin=skin, next_step=1 + equil_steps, seed=seed,
-                forces=forces, nchains=nchains, nbeads=nbeads)
+                forces=forces, nchains=nchains, nbeads=nbeads, k_bend=k_bend, bend_angle=bend_angle, k_dih=k_dih)
 
 
 def batched_layout(nsys=1024, nmin=64, nmax=512, seed=31):
