@@ -406,6 +406,7 @@ int sx_destroy(sx_ctx *c)
         if (b.idx) hipFree(b.idx);
         if (b.out) hipFree(b.out);
     }
+    if (c->d_gb_tab) hipFree(c->d_gb_tab);
     sx_ecp_mlp_free(c);
     sx_batch_free(c);
     sx_composite_free(c);

@@ -164,6 +164,9 @@ struct sx_ctx {
     float *d_ecp_scr = nullptr;   // FP32 screen records, 8 floats per particle
     double *d_pos4 = nullptr;     // positions packed x y z pad
     int ecp_pp_cap = 0;
+    // per-particle Gay-Berne factors (GB_TAB doubles per particle)
+    double *d_gb_tab = nullptr;
+    int gb_tab_cap = 0;
     int ecp_solver = SX_ECP_BRENT_REFERENCE;
     long ecp_stats[4] = {0, 0, 0, 0};
     // MLP (learned warm start)

@@ -69,6 +69,57 @@ __global__ void __launch_bounds__(256) k_gather(DState s, const int *row_start, 
     }
 }
 
+// GbSelf records of every particle (d_gb_tab)
+__global__ void k_gb_self(DState s, int mu, double *tab)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    GbSelf g;
+    gb_self(s, i, mu, g);
+    gb_self_store(g, tab + (long)GB_TAB * i);
+}
+
+// k_gather for the Gay-Berne family with the per-particle factors read from
+// the table: the same pairs in the same order, the same values
+template <int VARIANT, int LPP>
+__global__ void __launch_bounds__(256) k_gather_gb(DState s, const int *row_start, const int *nbr, Params P,
+                                                   const double *tab, double *pe_acc)
+{
+    int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = gid / LPP, l = gid % LPP;
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};  // fx fy fz tx ty tz pe
+    if (i < s.n) {
+        int b = row_start[i], e = row_start[i + 1];
+        for (int q = b + l; q < e; q += LPP) {
+            int j = nbr[q];
+            PairOut o;
+            if (j > i) {
+                d_gb_tab<VARIANT>(s, i, j, P.p, tab, o);
+                acc[0] = acc[0] + o.f[0]; acc[1] = acc[1] + o.f[1]; acc[2] = acc[2] + o.f[2];
+                acc[3] = acc[3] + o.ti[0]; acc[4] = acc[4] + o.ti[1]; acc[5] = acc[5] + o.ti[2];
+                acc[6] = acc[6] + o.pe;
+            } else {
+                d_gb_tab<VARIANT>(s, j, i, P.p, tab, o);
+                acc[0] = acc[0] - o.f[0]; acc[1] = acc[1] - o.f[1]; acc[2] = acc[2] - o.f[2];
+                acc[3] = acc[3] + o.tj[0]; acc[4] = acc[4] + o.tj[1]; acc[5] = acc[5] + o.tj[2];
+            }
+        }
+    }
+#pragma unroll
+    for (int off = LPP / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int c = 0; c < 7; ++c) acc[c] = acc[c] + __shfl_xor(acc[c], off, LPP);
+    if (l == 0 && i < s.n) {
+        s.f[SX_FX][i] = s.f[SX_FX][i] + acc[0];
+        s.f[SX_FY][i] = s.f[SX_FY][i] + acc[1];
+        s.f[SX_FZ][i] = s.f[SX_FZ][i] + acc[2];
+        s.f[SX_TX][i] = s.f[SX_TX][i] + acc[3];
+        s.f[SX_TY][i] = s.f[SX_TY][i] + acc[4];
+        s.f[SX_TZ][i] = s.f[SX_TZ][i] + acc[5];
+        pe_acc[i] = pe_acc[i] + acc[6];
+    }
+}
+
 // two isotropic kinds over one list in one pass (one read of each row and its
 // neighbours): each kind's sums are kept apart, reduced as k_gather reduces
 // them and added in the order of the two loops, so forces and energies equal
@@ -412,6 +463,28 @@ static void launch_gather(sx_ctx *c, const Params &P, int lpp)
     case 4: hipLaunchKernelGGL((k_gather<KIND, 4, TORQUE>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_pe_acc); break;
     case 8: hipLaunchKernelGGL((k_gather<KIND, 8, TORQUE>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_pe_acc); break;
     default: hipLaunchKernelGGL((k_gather<KIND, 16, TORQUE>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_pe_acc); break;
+    }
+}
+
+// the Gay-Berne family: the per-particle table, then the gather over it
+template <int VARIANT>
+static void launch_gather_gb(sx_ctx *c, const Params &P, int lpp)
+{
+    int n = c->n;
+    if (c->gb_tab_cap < n) {
+        if (c->d_gb_tab) HIPCHK(hipFree(c->d_gb_tab));
+        HIPCHK(hipMalloc(&c->d_gb_tab, sizeof(double) * GB_TAB * (size_t)n));
+        c->gb_tab_cap = n;
+        c->have_graph = false;
+    }
+    DState s = c->dstate();
+    hipLaunchKernelGGL(k_gb_self, dim3(sx_blocks(n, 256)), dim3(256), 0, c->stream, s, (int)P.p[1], c->d_gb_tab);
+    int nb = sx_blocks((long)n * lpp, 256);
+    switch (lpp) {
+    case 1: hipLaunchKernelGGL((k_gather_gb<VARIANT, 1>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_gb_tab, c->d_pe_acc); break;
+    case 4: hipLaunchKernelGGL((k_gather_gb<VARIANT, 4>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_gb_tab, c->d_pe_acc); break;
+    case 8: hipLaunchKernelGGL((k_gather_gb<VARIANT, 8>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_gb_tab, c->d_pe_acc); break;
+    default: hipLaunchKernelGGL((k_gather_gb<VARIANT, 16>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_gb_tab, c->d_pe_acc); break;
     }
 }
 
@@ -790,11 +863,11 @@ void sx_forces_impl(sx_ctx *c, int kind, const double *p, int np)
         case SX_MORSE: launch_gather<SX_MORSE, false>(c, P, lpp); break;
         case SX_DH: launch_gather<SX_DH, false>(c, P, lpp); break;
         case SX_COULOMB: launch_gather<SX_COULOMB, false>(c, P, lpp); break;
-        case SX_GB: launch_gather<SX_GB, true>(c, P, lpp); break;
-        case SX_GBR: launch_gather<SX_GBR, true>(c, P, lpp); break;
-        case SX_GBW: launch_gather<SX_GBW, true>(c, P, lpp); break;
-        case SX_GBC: launch_gather<SX_GBC, true>(c, P, lpp); break;
-        case SX_GBWC: launch_gather<SX_GBWC, true>(c, P, lpp); break;
+        case SX_GB: launch_gather_gb<0>(c, P, lpp); break;
+        case SX_GBR: launch_gather_gb<1>(c, P, lpp); break;
+        case SX_GBW: launch_gather_gb<2>(c, P, lpp); break;
+        case SX_GBC: launch_gather_gb<3>(c, P, lpp); break;
+        case SX_GBWC: launch_gather_gb<4>(c, P, lpp); break;
         case SX_DIPOLE:
             launch_gather<SX_DIPOLE, true>(c, P, lpp);
             dipole_side_effects(c, P);

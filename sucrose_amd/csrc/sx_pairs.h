@@ -420,6 +420,34 @@ __device__ __forceinline__ void d_gb(const DState &s, int a, int b, const double
     gb_core<VARIANT>(r, A, u2, p[0], mu, nu, o, wf, chir);
 }
 
+// The same pair with particle a's factors and particle b's axis read from a
+// per-particle table of GbSelf records (GB_TAB doubles each: u, chi, chi',
+// sig_x, eps_z), filled once per force evaluation by k_gb_self: the values
+// d_gb computes per pair, without the two pow calls per pair when mu != 1.
+#define GB_TAB 8
+__device__ __forceinline__ void gb_self_store(const GbSelf &g, double *t)
+{
+    t[0] = g.u[0]; t[1] = g.u[1]; t[2] = g.u[2];
+    t[3] = g.chi; t[4] = g.xhi; t[5] = g.lprp; t[6] = g.epsz; t[7] = 0.0;
+}
+
+template <int VARIANT>
+__device__ __forceinline__ void d_gb_tab(const DState &s, int a, int b, const double *p, const double *tab,
+                                         PairOut &o)
+{
+    int mu = (int)p[1], nu = (int)p[2];
+    const double wf = (VARIANT == 2 || VARIANT == 4) ? p[3] : 0.0;
+    const double chir = VARIANT == 3 ? p[3] : (VARIANT == 4 ? p[4] : 0.0);
+    double r[3];
+    pair_rij(s, a, b, r);
+    const double *ta = tab + (long)GB_TAB * a, *tb = tab + (long)GB_TAB * b;
+    GbSelf A;
+    A.u[0] = ta[0]; A.u[1] = ta[1]; A.u[2] = ta[2];
+    A.chi = ta[3]; A.xhi = ta[4]; A.lprp = ta[5]; A.epsz = ta[6];
+    const double u2[3] = {tb[0], tb[1], tb[2]};
+    gb_core<VARIANT>(r, A, u2, p[0], mu, nu, o, wf, chir);
+}
+
 // ---------------------------------------------------------------------------
 // point dipoles (dipole_calculate_forces, force.f90:2687-2751): the body z
 // axis of each particle scaled by its charge; the reaction-field side effects
