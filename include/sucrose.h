@@ -358,8 +358,8 @@ int sx_read_checkpoint_host(const char *filename, int *n, int *nc, double box[3]
                             int cap_nc, double *const *fields, int *rtype, int *bbi1, int *bbj1, double *bblen);
 
 /* ------------------------------------------------------------------ */
-/* batched replicas: independent small systems advanced together, three     */
-/* launches per step (the README loop of each system), graph-replayed      */
+/* batched replicas: independent small systems advanced together, one       */
+/* launch per step (or three, by autotuning), graph-replayed               */
 /* ------------------------------------------------------------------ */
 /* The context's particles are split into nsys contiguous systems of counts[s]
  * particles, each in its own cubic periodic box of edge box[s], with pair
@@ -374,14 +374,18 @@ int sx_batch_set(sx_ctx *ctx, int nsys, const int *counts, const double *box, co
 int sx_batch_run(sx_ctx *ctx, int nsteps, double dt, double damping, int first_step);
 /* per system: KE, KErot, PE, T = 2 (KE + KErot) / (6 N - 3); out[4 nsys] */
 int sx_batch_thermo(sx_ctx *ctx, double *out);
-/* times nrep steps of each of the 8 launch shapes from the same state,
- * restores it, keeps the fastest; ms[8] (may be NULL) gets ms per step.
- * Shapes (particles per force workgroup, lanes per particle): 0 (64,1)
+/* times nrep steps of each of the 10 launch shapes from the same state,
+ * restores it, keeps the fastest; ms[10] (may be NULL) gets ms per step.
+ * Shapes 0-7 take three launches per step (initial step, forces, final
+ * step); as (particles per force workgroup, lanes per particle): 0 (64,1)
  * 1 (16,4) 2 (32,4) 3 (32,8) 4 (64,8) 5 (16,64) 6 (32,64) 7 (64,64); 64 lanes
  * means one wavefront per particle that queues the pairs in range and
  * evaluates them 64 at a time.  Shape 0 adds each particle's pair terms in
- * the README loop's order; the others in a fixed tree order. */
-int sx_batch_autotune(sx_ctx *ctx, double dt, double damping, int first_step, int nrep, int *choice, double ms[8]);
+ * the README loop's order; the others in a fixed tree order.  Shapes 8 and
+ * 9 run the whole step in one launch: workgroups of 256 or 512 threads take
+ * whole systems from a queue, most expensive first, one compacting
+ * wavefront per particle. */
+int sx_batch_autotune(sx_ctx *ctx, double dt, double damping, int first_step, int nrep, int *choice, double ms[10]);
 int sx_batch_set_shape(sx_ctx *ctx, int choice);
 
 /* ------------------------------------------------------------------ */

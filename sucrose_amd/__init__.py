@@ -439,7 +439,7 @@ class System:
 
     # ---- batched replicas ----
     # (particles per force workgroup, lanes per particle)
-    BATCH_SHAPES = ((64, 1), (16, 4), (32, 4), (32, 8), (64, 8), (16, 64), (32, 64), (64, 64))
+    BATCH_SHAPES = ((64, 1), (16, 4), (32, 4), (32, 8), (64, 8), (16, 64), (32, 64), (64, 64), (0, 256), (0, 512))
 
     def batch_set(self, counts, boxes, kinds, params, temps):
         """Splits the particles into len(counts) independent systems (contiguous

@@ -52,6 +52,9 @@ struct BatchState {
     int choice = 6;
     int cap = 0;                   // largest system
     double dt = -1, damp = -1;     // constants currently uploaded
+    int *d_order = nullptr;        // systems, most expensive first (one-launch step)
+    unsigned *d_qctr = nullptr;    // its queue and done counters
+    int ngrid = 0;                 // its workgroups: two per CU
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_choice = -1;
@@ -60,16 +63,15 @@ struct BatchState {
     const double *fields = nullptr;
 };
 
-// (particles per force workgroup, lanes per particle); 64 lanes = one
-// compacting wavefront per particle.  Shape 0 adds each particle's pair terms
-// in the README loop's order.
-static const int BATCH_SHAPES[][2] = {{64, 1}, {16, 4}, {32, 4}, {32, 8}, {64, 8}, {16, 64}, {32, 64}, {64, 64}};
-static const int BATCH_NSHAPES = 8;
-static int shape_threads(int choice)
-{
-    const int P = BATCH_SHAPES[choice][0], LPP = BATCH_SHAPES[choice][1];
-    return LPP == 64 ? 256 : std::min(256, P * LPP);
-}
+// Three-launch shapes 0-7: (particles per force workgroup, lanes per
+// particle); 64 lanes = one compacting wavefront per particle.  Shape 0 adds
+// each particle's pair terms in the README loop's order.  Shapes 8 and 9:
+// the whole step in one launch (k_batch_step) with workgroups of 256 and 512
+// threads taking whole systems (P = 0 marks them).
+static const int BATCH_SHAPES[][2] = {{64, 1},  {16, 4},  {32, 4}, {32, 8}, {64, 8},
+                                      {16, 64}, {32, 64}, {64, 64}, {0, 256}, {0, 512}};
+static const int BATCH_NSHAPES = 10;
+static bool shape_fused(int choice) { return BATCH_SHAPES[choice][0] == 0; }
 
 // Per-particle table of every system, written right after lgv_initial_step
 // (one SoA block of TB_N rows per system, row length N_s): position, and for
@@ -338,6 +340,148 @@ __global__ void __launch_bounds__(256) k_batch_final(DState s, const BatchSys *s
     }
 }
 
+// The whole step in one launch: a grid of workgroups takes whole systems
+// from a queue counter, the most expensive first (order), and runs each
+// system's lgv_initial_step with its table (in LDS when it fits), its pair
+// loop (one compacting wavefront per particle, as k_batch_forces with 64
+// lanes), lgv_final_step and its thermo.  A system never spans workgroups,
+// so only workgroup barriers occur and every workgroup leaves once the queue
+// is empty; the last one to leave resets the queue and advances the step
+// counter.  qctr: [0] queue, [1] workgroups done.
+template <int BT>
+__global__ void __launch_bounds__(BT) k_batch_step(DState s, const BatchSys *sys, const int *order, int nsys,
+                                                   double dt, double damp, int *dstep, double *gtab, double *pe,
+                                                   double *out, unsigned *qctr, int lds_ok)
+{
+    extern __shared__ double lds_tab[];
+    __shared__ int queue[BT / 64][128];
+    __shared__ double red[BT][3];
+    __shared__ int cur;
+    const int step = *dstep;   // no workgroup advances it before every one has left
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (;;) {
+        if (threadIdx.x == 0) cur = (int)atomicAdd(&qctr[0], 1u);
+        __syncthreads();
+        const int k = cur;
+        __syncthreads();
+        if (k >= nsys) break;
+        const int sid = order[k];
+        const BatchSys S = sys[sid];
+        DState ls = s;
+        ls.box[0] = ls.box[1] = ls.box[2] = S.L;
+        ls.periodic = 1;
+        const bool gb = S.kind == SX_GB;
+        const int mu = (int)S.p[1], nu = (int)S.p[2];
+        const int ld = S.n;
+        double *tab = lds_ok ? lds_tab : gtab + (size_t)TB_N * S.off;
+        // lgv_initial_step of each particle (counter: its 1-based index in
+        // the system) and its table row
+        LgvConst kc;
+        kc.dt = dt; kc.damp = damp; kc.T = S.T; kc.e1 = S.e1; kc.e2 = S.e2; kc.kk1 = S.kk1; kc.kk2 = S.kk2;
+        kc.temp_int = S.temp_int; kc.save = 0;
+        for (int q = threadIdx.x; q < S.n; q += BT) {
+            const int i = S.off + q;
+            lgv_initial_particle(ls, i, q + 1, kc, step);
+            tab[TB_X * ld + q] = s.f[SX_RX][i];
+            tab[TB_Y * ld + q] = s.f[SX_RY][i];
+            tab[TB_Z * ld + q] = s.f[SX_RZ][i];
+            if (gb) {
+                GbSelf g;
+                gb_self(ls, i, mu, g);
+                tab[TB_U0 * ld + q] = g.u[0];
+                tab[TB_U1 * ld + q] = g.u[1];
+                tab[TB_U2 * ld + q] = g.u[2];
+                tab[TB_CHI * ld + q] = g.chi;
+                tab[TB_XHI * ld + q] = g.xhi;
+                tab[TB_LPRP * ld + q] = g.lprp;
+                tab[TB_EPSZ * ld + q] = g.epsz;
+                tab[TB_REACH * ld + q] = (fmax(s.f[SX_SIG_X][i], s.f[SX_SIG_Z][i]) + S.p[0] - 1.0) * (1.0 + 1e-9);
+            }
+        }
+        __syncthreads();
+        // initialize_forces + the pair loop: one compacting wavefront per particle
+        int *jq_buf = queue[wv];
+        for (int q = wv; q < S.n; q += BT / 64) {
+            const int i = S.off + q;
+            BatchSelf me;
+            batch_self(tab, ld, q, gb, S.p[2], me);
+            double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+            int cnt = 0;
+            for (int base = 0; base < S.n; base += 64) {
+                const int jq = base + lane;
+                const bool in = jq < S.n && jq != q && batch_in_reach(tab, ld, q, jq, me, gb, S.L);
+                const unsigned long long m = __ballot(in);
+                const int pre = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                if (in) jq_buf[cnt + pre] = jq;
+                cnt += __popcll(m);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (cnt >= 64) {
+                    const int j = jq_buf[lane];
+                    const int carry = cnt - 64 > lane ? jq_buf[64 + lane] : 0;
+                    batch_eval(tab, ld, q, j, me, gb, S.p, mu, nu, S.L, acc);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    if (cnt - 64 > lane) jq_buf[lane] = carry;
+                    cnt -= 64;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+            }
+            if (lane < cnt) batch_eval(tab, ld, q, jq_buf[lane], me, gb, S.p, mu, nu, S.L, acc);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+                for (int c = 0; c < 7; ++c) acc[c] = acc[c] + __shfl_xor(acc[c], off, 64);
+            if (lane == 0) {
+                s.f[SX_FX][i] = acc[0]; s.f[SX_FY][i] = acc[1]; s.f[SX_FZ][i] = acc[2];
+                s.f[SX_TX][i] = acc[3]; s.f[SX_TY][i] = acc[4]; s.f[SX_TZ][i] = acc[5];
+                pe[i] = acc[6];
+            }
+        }
+        __syncthreads();
+        // lgv_final_step and the system's thermo (as k_batch_final)
+        double ke = 0.0, kr = 0.0, pl = 0.0;
+        for (int q = threadIdx.x; q < S.n; q += BT) {
+            const int i = S.off + q;
+            half_kick(ls, i, dt);
+            ang_kick(ls, i, dt);
+            ke += kin_trans(ls, i);
+            kr += kin_rot(ls, i);
+            pl += pe[i];
+        }
+        red[threadIdx.x][0] = ke; red[threadIdx.x][1] = kr; red[threadIdx.x][2] = pl;
+        __syncthreads();
+        for (int o = BT / 2; o > 0; o >>= 1) {
+            if (threadIdx.x < o)
+                for (int c = 0; c < 3; ++c) red[threadIdx.x][c] = red[threadIdx.x][c] + red[threadIdx.x + o][c];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            out[4 * sid + 0] = red[0][0];
+            out[4 * sid + 1] = red[0][1];
+            out[4 * sid + 2] = red[0][2];
+            out[4 * sid + 3] = 2.0 * (red[0][0] + red[0][1]) / (double)(6 * S.n - 3);
+        }
+        __syncthreads();   // the table and red are reused by the next system
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&qctr[1], 1u) == gridDim.x - 1) {
+            atomicExch(&qctr[0], 0u);
+            atomicExch(&qctr[1], 0u);
+            *dstep = step + 1;
+        }
+    }
+}
+
 __global__ void k_batch_step_set(int *d, int v) { *d = v; }
 
 static BatchState *batch_of(sx_ctx *c)
@@ -356,7 +500,8 @@ void sx_batch_free(sx_ctx *c)
     if (B->exec) hipGraphExecDestroy(B->exec);
     if (B->graph) hipGraphDestroy(B->graph);
     for (void *q : {(void *)B->d_sys, (void *)B->d_sys_of, (void *)B->d_tab, (void *)B->d_pe, (void *)B->d_blocks,
-                    (void *)B->d_out, (void *)B->d_ticket, (void *)B->d_step})
+                    (void *)B->d_out, (void *)B->d_ticket, (void *)B->d_step, (void *)B->d_order,
+                    (void *)B->d_qctr})
         if (q) hipFree(q);
     delete B;
     c->batch = nullptr;
@@ -380,7 +525,7 @@ static void batch_constants(sx_ctx *c, BatchState *B, double dt, double damp)
 // first (work ~ slice length x N_s, Gay-Berne weighted 4x)
 static void batch_blocks(sx_ctx *c, BatchState *B, int choice)
 {
-    if (B->blocks_choice == choice) return;
+    if (B->blocks_choice == choice || shape_fused(choice)) return;
     const int P = BATCH_SHAPES[choice][0];
     std::vector<int4> blk;
     std::vector<double> cost;
@@ -413,6 +558,24 @@ static void batch_blocks(sx_ctx *c, BatchState *B, int choice)
 static void batch_launch(sx_ctx *c, BatchState *B, int choice, double dt, double damp)
 {
     DState s = c->dstate();
+    if (shape_fused(choice)) {
+        // the table in LDS when it fits beside the kernel's own arrays (queue,
+        // reduction) in 64 KiB per workgroup; else each system's global table
+        const int BT = BATCH_SHAPES[choice][1];
+        const size_t fixed = (size_t)(BT / 64) * 128 * sizeof(int) + (size_t)BT * 3 * sizeof(double) + 64;
+        const size_t tbytes = sizeof(double) * TB_N * (size_t)B->cap;
+        const bool lds = tbytes + fixed <= (size_t)BATCH_LDS_BYTES;
+        const size_t lbytes = lds ? tbytes : 0;
+        const dim3 g(std::min(B->ngrid, B->nsys));
+        if (BATCH_SHAPES[choice][1] == 256)
+            hipLaunchKernelGGL(k_batch_step<256>, g, dim3(256), lbytes, c->stream, s, B->d_sys, B->d_order, B->nsys,
+                               dt, damp, B->d_step, B->d_tab, B->d_pe, B->d_out, B->d_qctr, lds ? 1 : 0);
+        else
+            hipLaunchKernelGGL(k_batch_step<512>, g, dim3(512), lbytes, c->stream, s, B->d_sys, B->d_order, B->nsys,
+                               dt, damp, B->d_step, B->d_tab, B->d_pe, B->d_out, B->d_qctr, lds ? 1 : 0);
+        HIPCHK(hipGetLastError());
+        return;
+    }
     hipLaunchKernelGGL(k_batch_initial, dim3(sx_blocks(c->n, 256)), dim3(256), 0, c->stream, s, B->d_sys,
                        B->d_sys_of, dt, damp, B->d_step, B->d_tab);
     const bool lds = sizeof(double) * TB_N * (size_t)B->cap <= (size_t)BATCH_LDS_BYTES;
@@ -517,6 +680,23 @@ int sx_batch_set(sx_ctx *c, int nsys, const int *counts, const double *box, cons
         HIPCHK(hipMalloc(&B->d_step, sizeof(int)));
         HIPCHK(hipMemsetAsync(B->d_out, 0, sizeof(double) * 4 * nsys, c->stream));
         HIPCHK(hipMemsetAsync(B->d_ticket, 0, sizeof(unsigned), c->stream));
+        // the one-launch step takes systems in this order: pair work ~ N^2,
+        // Gay-Berne weighted 4x, largest first
+        std::vector<int> order(nsys);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            auto w = [&](int q) {
+                return (double)B->h_sys[q].n * B->h_sys[q].n * (B->h_sys[q].kind == SX_GB ? 4.0 : 1.0);
+            };
+            return w(a) > w(b);
+        });
+        HIPCHK(hipMalloc(&B->d_order, sizeof(int) * nsys));
+        HIPCHK(hipMemcpyAsync(B->d_order, order.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMalloc(&B->d_qctr, sizeof(unsigned) * 2));
+        HIPCHK(hipMemsetAsync(B->d_qctr, 0, sizeof(unsigned) * 2, c->stream));
+        int ncu = 0;
+        HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+        B->ngrid = std::max(1, 2 * ncu);
         HIPCHK(hipStreamSynchronize(c->stream));
         c->batch = owner.release();
     });

@@ -970,15 +970,13 @@ __global__ void __launch_bounds__(64) k_rattle64(DState s, double dt, int nmol, 
                         if (vdot < r_tol * (len * len)) { failed = 1; break; }
                         double L = chi / (2.0 * vdot * (ima + imb));
                         vir = vir + (L * (len * len)) / (dt * dt);
-                        if (lane == la) {
-                            double c = L * ima;
+                        // atom a gains c = L ima along Ro, atom b loses L imb:
+                        // b's update as the addition of the negated product
+                        // (negation is exact), so both lanes share one path
+                        if (lane == la || lane == lb) {
+                            const double c = lane == la ? L * ima : -(L * imb);
                             rx = rx + c * Ro[0]; ry = ry + c * Ro[1]; rz = rz + c * Ro[2];
                             vx = vx + c * Ro[0] / dt; vy = vy + c * Ro[1] / dt; vz = vz + c * Ro[2] / dt;
-                        }
-                        if (lane == lb) {
-                            double c = L * imb;
-                            rx = rx - c * Ro[0]; ry = ry - c * Ro[1]; rz = rz - c * Ro[2];
-                            vx = vx - c * Ro[0] / dt; vy = vy - c * Ro[1] / dt; vz = vz - c * Ro[2] / dt;
                         }
                         mv |= bits;
                     } else {
@@ -992,13 +990,9 @@ __global__ void __launch_bounds__(64) k_rattle64(DState s, double dt, int nmol, 
                     double L = -vdot / ((ima + imb) * (len * len));
                     if (fabs(L) > r_tol) {
                         vir = vir + (L * (len * len)) / dt;
-                        if (lane == la) {
-                            double c = L * ima;
+                        if (lane == la || lane == lb) {   // as in phase a: b's update negated
+                            const double c = lane == la ? L * ima : -(L * imb);
                             vx = vx + c * R[0]; vy = vy + c * R[1]; vz = vz + c * R[2];
-                        }
-                        if (lane == lb) {
-                            double c = L * imb;
-                            vx = vx - c * R[0]; vy = vy - c * R[1]; vz = vz - c * R[2];
                         }
                         mv |= bits;
                     } else {

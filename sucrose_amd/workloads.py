@@ -152,7 +152,7 @@ def gb_nhc_state(dev: System, n_side=20, rho=0.30, seed=15, T=3.0, dt=0.0015, cu
 
 
 def chains_state(dev: System, nchains=2000, nbeads=50, rho=0.5, seed=7, T=1.0, dt=0.005, gamma=1.0, bond=1.0,
-                 min_dist=0.95, k_bend=20.0, bend_angle=180.0, k_dih=1.0, dh=(5.0, 1.0, 1.0), skin=0.3,
+                 min_dist=0.95, k_bend=20.0, bend_angle=180.0, k_dih=1.0, dh=(5.0, 1.0, 1.0), skin=0.8,
                  equil_steps=200, log=print):
     """Builds the charged-chain workload on dev; returns its parameters and the force list."""
     t0 = time.time()

@@ -134,7 +134,7 @@ def test_batched_steps_match_oracle(oracle):
     dev.close()
 
 
-@pytest.mark.parametrize("shape", range(8))
+@pytest.mark.parametrize("shape", range(10))
 def test_batched_launch_shapes(oracle, shape):
     import sucrose_amd as sx
     systems = _systems(seed=5 + shape)
@@ -155,7 +155,7 @@ def test_batched_autotune_restores_state(oracle):
     _load(dev, systems)
     choice, ms = dev.batch_autotune(0.002, damping=1.0, first_step=7, nrep=2)
     say(f"  chose {choice}: " + " ".join(f"{m:.3f}" for m in ms))
-    assert 0 <= choice < 8 and np.all(ms > 0)
+    assert 0 <= choice < len(sx.System.BATCH_SHAPES) and len(ms) == 10 and np.all(ms > 0)
     dev.batch_run(2, 0.002, damping=1.0, first_step=7)
     _compare(dev, oracle, systems, 2, 7, 0.002, 1.0, "autotuned")
     dev.close()
