@@ -451,6 +451,20 @@ def learned_vs_plain(dev, sx, p, step, nsteps):
     dev.build_nlist()
     lam_plain = cmp["ms_newton_plain"]
     lam_learned = cmp["ms_network"] + cmp["ms_newton_warm"]
+    # Amdahl bound of any warm start on this stage, from the same timings: the
+    # time of one Newton round of a wavefront (the difference between the
+    # plain and warm-started lambda solves over the difference of their
+    # per-wavefront maximum iterations), the part of the plain stage those
+    # rounds take, and the stage time left with a free network and (a) no
+    # iterations at all, (b) the warm start's own rounds
+    d_it = cmp["wave_iters_plain"] - cmp["wave_iters_warm"]
+    t_round = (lam_plain - cmp["ms_newton_warm"]) / d_it if d_it > 0 else float("nan")
+    removable = t_round * cmp["wave_iters_plain"]
+    floor = cmp["ms_solve_plain"] - removable
+    amdahl = {"ms_per_newton_round": t_round, "ms_iterations_in_plain_stage": removable,
+              "ms_stage_without_iterations": floor,
+              "max_speedup_free_network_no_iterations": cmp["ms_solve_plain"] / floor if floor > 0 else None,
+              "max_speedup_free_network_warm_rounds": cmp["ms_solve_plain"] / (floor + t_round * cmp["wave_iters_warm"])}
     out = {
         "training": {"samples": samples, "adam_steps": train_steps, "batch": 4096, "seconds": t_train,
                      "loss_first": losses[0][0], "loss_last": losses[-1][1], "dtype": "bf16 mfma, fp32 accumulate"},
@@ -466,6 +480,7 @@ def learned_vs_plain(dev, sx, p, step, nsteps):
         "speedup_vs_brent_1e-10": cmp["ms_brent_tight"] / lam_learned,
         "solve_stage_ms": {"plain": cmp["ms_solve_plain"], "learned": cmp["ms_solve_learned"]},
         "solve_stage_speedup_vs_plain": cmp["ms_solve_plain"] / cmp["ms_solve_learned"],
+        "amdahl_bound": amdahl,
         "force_rel_err_learned_vs_plain": err_learned,
         "force_rel_err_brent_reference_vs_converged": err_brent,
         "step_particle_steps_per_s": rates,

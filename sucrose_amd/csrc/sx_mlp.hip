@@ -498,6 +498,81 @@ __global__ void __launch_bounds__(256) k_mlp_lambda0(DState s, const float *p, c
     (void)end;
 }
 
+// forward_regs for NG independent groups of 16 pairs at once: each weight
+// tile is loaded once and its NG products issue back to back, so the matrix
+// cores work on NG independent chains.  Each group's operations are those of
+// forward_regs, so its outputs are the same.
+template <int NG>
+__device__ __forceinline__ void forward_regs_n(const bf16x4 *w1, const bf16x8 *wi, const float *bias,
+                                               const bf16x4 (&xb)[NG], int lane, f32x4 (&y)[NG])
+{
+    const int g = lane >> 4;
+    bf16x4 h[NG][4], h2[NG][4];
+    auto operand = [](const bf16x4 &lo, const bf16x4 &hi) {
+        short8 s8 = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+        return __builtin_bit_cast(bf16x8, s8);
+    };
+#pragma unroll
+    for (int mo = 0; mo < 4; ++mo) {
+        const bf16x4 wt = w1[mo * 64 + lane];
+        f32x4 acc[NG];
+#pragma unroll
+        for (int n = 0; n < NG; ++n) {
+            acc[n] = f32x4{0.f, 0.f, 0.f, 0.f};
+            acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wt, xb[n], acc[n], 0, 0, 0);
+        }
+#pragma unroll
+        for (int n = 0; n < NG; ++n)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) h[n][mo][j] = to_bf16(silu(acc[n][j] + bias[16 * mo + 4 * g + j]));
+    }
+#pragma unroll
+    for (int layer = 0; layer < 2; ++layer) {
+        const bf16x8 *tiles = wi + (layer == 0 ? I_L2 : I_L3) * 64;
+        const float *b = bias + 64 * (layer + 1);
+        bf16x8 in0[NG], in1[NG];
+#pragma unroll
+        for (int n = 0; n < NG; ++n) {
+            in0[n] = operand(h[n][0], h[n][1]);
+            in1[n] = operand(h[n][2], h[n][3]);
+        }
+#pragma unroll
+        for (int mo = 0; mo < 4; ++mo) {
+            const bf16x8 t0 = tiles[(mo * 2 + 0) * 64 + lane], t1 = tiles[(mo * 2 + 1) * 64 + lane];
+            f32x4 acc[NG];
+#pragma unroll
+            for (int n = 0; n < NG; ++n) {
+                acc[n] = f32x4{0.f, 0.f, 0.f, 0.f};
+                acc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(t0, in0[n], acc[n], 0, 0, 0);
+            }
+#pragma unroll
+            for (int n = 0; n < NG; ++n) acc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(t1, in1[n], acc[n], 0, 0, 0);
+#pragma unroll
+            for (int n = 0; n < NG; ++n)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) h2[n][mo][j] = to_bf16(silu(acc[n][j] + b[16 * mo + 4 * g + j]));
+        }
+#pragma unroll
+        for (int n = 0; n < NG; ++n)
+#pragma unroll
+            for (int mo = 0; mo < 4; ++mo) h[n][mo] = h2[n][mo];
+    }
+    const bf16x8 o0 = wi[(I_L4 + 0) * 64 + lane], o1 = wi[(I_L4 + 1) * 64 + lane];
+#pragma unroll
+    for (int n = 0; n < NG; ++n) {
+        y[n] = f32x4{0.f, 0.f, 0.f, 0.f};
+        y[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(o0, operand(h[n][0], h[n][1]), y[n], 0, 0, 0);
+    }
+#pragma unroll
+    for (int n = 0; n < NG; ++n) {
+        y[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(o1, operand(h[n][2], h[n][3]), y[n], 0, 0, 0);
+        if (g == 0) {
+            y[n][0] += bias[192];
+            y[n][1] += bias[193];
+        }
+    }
+}
+
 // The learned solve stage in one kernel: one wavefront per work-list segment
 // (as the plain solve stage);
 // for each round of 64 pairs the network runs on four groups of 16 (its
@@ -552,13 +627,20 @@ __global__ void __launch_bounds__(64) k_ecp_solve_learned(DState s, const double
             xs[lane * 8 + 7] = 0.f;
         }
         __syncthreads();
-        for (int q = 0; q < 4 && base + 16 * q < cnt; ++q) {
-            bf16x4 xb;
+        {
+            // the round's four groups of 16 pairs through the network together
+            bf16x4 xb[4];
+            f32x4 y[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) xb[j] = to_bf16(xs[(16 * q + r) * 8 + 4 * g + j]);
-            if (g >= 2) xb = bf16x4{0, 0, 0, 0};
-            f32x4 y = forward_regs(s1, si, sp, xb, lane);
-            if (g == 0) l0s[16 * q + r] = fminf(fmaxf(y[0], 0.02f), 0.98f);
+            for (int q = 0; q < 4; ++q) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) xb[q][j] = to_bf16(xs[(16 * q + r) * 8 + 4 * g + j]);
+                if (g >= 2) xb[q] = bf16x4{0, 0, 0, 0};
+            }
+            forward_regs_n<4>(s1, si, sp, xb, lane, y);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (g == 0 && base + 16 * q < cnt) l0s[16 * q + r] = fminf(fmaxf(y[q][0], 0.02f), 0.98f);
         }
         __syncthreads();
         if (u < cnt) {
