@@ -237,10 +237,59 @@ def gb_cpu_baseline(state, box, params, nsteps):
                       f"the same {n}-particle state after one untimed step", "seconds": el}
 
 
+def elem_err(a, b):
+    """Worst |a - b| / (|b| + median of the non-zero |b|) over the elements."""
+    import numpy as np
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nz = np.abs(b[b != 0])
+    med = float(np.median(nz)) if nz.size else 1.0
+    return float(np.max(np.abs(a - b) / (np.abs(b) + med)))
+
+
+FORCE_FIELDS = ("FX", "FY", "FZ", "TX", "TY", "TZ")
+
+
+def f32_error(d, snap, evaluate, fields=FORCE_FIELDS, groups=None):
+    """The FP32 pair terms' error on one state: evaluate() at 64 and at 32
+    bits from the same snapshot (the 64-bit terms are the reference's
+    arithmetic); worst force/torque element error (within each group of
+    particles, e.g. each batched system, when groups are given) and relative
+    energy error.  Leaves the snapshot loaded and the precision at 32 bits."""
+    import numpy as np
+    got = {}
+    for bits in (64, 32):
+        d.load_arrays(snap)
+        d.set_pair_precision(bits)
+        pe = evaluate()
+        got[bits] = (np.stack([d.download(f) for f in fields]), pe)
+    d.load_arrays(snap)
+    f64, pe64 = got[64]
+    f32, pe32 = got[32]
+    groups = groups or [slice(None)]
+    worst = max(elem_err(f32[k][g], f64[k][g]) for g in groups for k in range(len(fields)) if np.any(f64[k][g]))
+    return {"force_elem_err": worst,
+            "pe_rel_err": float(np.max(np.abs(np.asarray(pe32) - np.asarray(pe64)) /
+                                       np.maximum(1.0, np.abs(np.asarray(pe64))))),
+            "measure": "FP32 vs FP64 pair terms on one state; |a-b|/(|b|+median|b|) per force/torque element"
+                       + (", per system" if len(groups) > 1 else "")}
+
+
+def f32_row(rec, n, steps, el, err, rebuilds, extra=None):
+    """The FP32 companion of a workload's FP64 record."""
+    row = {"value": n * steps / el, "unit": "particle-steps/s", "steps": steps, "ms_per_step": el / steps * 1e3,
+           "speedup_vs_f64": rec["ms_per_step"] / (el / steps * 1e3), "rebuilds": rebuilds, "error": err,
+           "config": dict(rec["config"], dtype="f32 pair terms (f64 positions, sums, integrator)")}
+    if extra:
+        row.update(extra)
+    return row
+
+
 def other_workloads(sx, args):
     """The Gay-Berne + Nose-Hoover, the charged-chain and the batched-replica
     configurations, each prepared on the device and timed over whole
-    graph-replayed steps (a section with 0 steps is skipped)."""
+    graph-replayed steps (a section with 0 steps is skipped); each is timed
+    again with FP32 pair terms from the same state."""
     from sucrose_amd.workloads import chains_state, gb_nhc_state
     out = {}
     if args.gb_steps > 0:
@@ -267,12 +316,31 @@ def other_workloads(sx, args):
                           "skin": p["skin"], "integrator": "nht (chain 3, tau 2) + quaternion", "dtype": "f64"}}
         log(f"[gb] {args.gb_steps} steps in {el:.3f}s -> {rec['value']:.4e} particle-steps/s; "
             f"{rec['chain_newton_updates_per_call']:.2f} Newton updates per chain call, {fb1 - fb0} serial fallbacks")
+        snap = d.state()
         if not args.no_cpu:
-            cpu = gb_cpu_baseline(d.state(), p["box"], p, 10)
+            cpu = gb_cpu_baseline(snap, p["box"], p, 10)
             if cpu:
                 rec["cpu_baseline"] = cpu
                 log(f"[gb cpu] reference {cpu['value']:.4e} particle-steps/s ({cpu['seconds']:.1f}s)")
         out["gb_nhc_8000"] = rec
+
+        def gb_forces():
+            d.initialize_forces()
+            d.forces(*gb[0])
+            return d.get_scalar("POT_ENG")
+
+        err = f32_error(d, snap, gb_forces)
+        step += 100 + args.gb_steps
+        d.run(sx.INT_NHT, gb, 100, p["dt"], first_step=step)
+        d.synchronize()
+        rb0 = d.rebuild_count()
+        t0 = time.perf_counter()
+        d.run(sx.INT_NHT, gb, args.gb_steps, p["dt"], first_step=step + 100)
+        d.synchronize()
+        el = time.perf_counter() - t0
+        out["gb_nhc_8000_f32"] = f32_row(rec, p["n"], args.gb_steps, el, err, d.rebuild_count() - rb0,
+                                         {"T": d.get_scalar("TEMP")})
+        log(f"[gb f32] {el / args.gb_steps * 1e3:.4f} ms/step; error {err['force_elem_err']:.2e}")
         d.close()
     if args.chain_steps > 0:
         d = sx.System(0)
@@ -291,15 +359,34 @@ def other_workloads(sx, args):
                           "bonded": "cosine_bend k=20 about straight + dih k(1+cos) k=1", "constraints": "rattle",
                           "integrator": "lgvc", "dtype": "f64"}}
         log(f"[chains] {args.chain_steps} steps in {el:.3f}s -> {rec['value']:.4e} particle-steps/s")
+        snap = d.state()
         if not args.no_cpu:
-            cpu = chains_cpu_baseline(d.state(), p, 15.0)
+            cpu = chains_cpu_baseline(snap, p, 15.0)
             if cpu:
                 rec["cpu_baseline"] = cpu
                 log(f"[chains cpu] reference {cpu['value']:.4e} particle-steps/s ({cpu['seconds']:.1f}s)")
         out["chains_100000"] = rec
+
+        def chain_forces():
+            d.initialize_forces()
+            for f in p["forces"]:
+                d.forces(*f)
+            return d.get_scalar("POT_ENG")
+
+        err = f32_error(d, snap, chain_forces, FORCE_FIELDS[:3])
+        first = p["next_step"] + args.chain_steps
+        rb0 = d.rebuild_count()
+        t0 = time.perf_counter()
+        d.run(sx.INT_LGVC, p["forces"], args.chain_steps, p["dt"], damping=p["gamma"], first_step=first,
+              bonded=True)
+        d.synchronize()
+        el = time.perf_counter() - t0
+        out["chains_100000_f32"] = f32_row(rec, p["n"], args.chain_steps, el, err, d.rebuild_count() - rb0,
+                                           {"T": d.get_scalar("TEMP")})
+        log(f"[chains f32] {el / args.chain_steps * 1e3:.4f} ms/step; error {err['force_elem_err']:.2e}")
         d.close()
     if args.batch_steps > 0:
-        out["batched_1024"] = batched_workload(sx, args)
+        out["batched_1024"], out["batched_1024_f32"] = batched_workload(sx, args)
     return out
 
 
@@ -343,8 +430,33 @@ def batched_workload(sx, args):
         if cpu:
             rec["cpu_baseline"] = cpu
             log(f"[batch cpu] reference {cpu['value']:.4e} particle-steps/s ({cpu['seconds']:.1f}s)")
+    # FP32 pair terms: the error of one step's forces from one state, then
+    # the autotuner and the timed run again
+    step += args.batch_steps
+    snap = d.state()
+
+    def batch_step():
+        d.batch_run(1, p["dt"], damping=p["gamma"], first_step=step)
+        return d.batch_thermo()[:, 2].copy()
+
+    ends = np.cumsum([0] + list(p["counts"]))
+    err = f32_error(d, snap, batch_step, groups=[slice(a, b) for a, b in zip(ends[:-1], ends[1:])])
+    choice, ms = d.batch_autotune(p["dt"], damping=p["gamma"], first_step=step, nrep=5)
+    shape = sx.System.BATCH_SHAPES[choice]
+    d.batch_run(50, p["dt"], damping=p["gamma"], first_step=step)
+    step += 50
+    d.synchronize()
+    t0 = time.perf_counter()
+    d.batch_run(args.batch_steps, p["dt"], damping=p["gamma"], first_step=step)
+    d.synchronize()
+    el = time.perf_counter() - t0
+    row = f32_row(rec, p["n"], args.batch_steps, el, err, 0,
+                  {"launch_shape": {"particles_per_workgroup": shape[0], "lanes_per_particle": shape[1]},
+                   "autotune_ms": [round(float(m), 4) for m in ms]})
+    del row["rebuilds"]
+    log(f"[batch f32] {el / args.batch_steps * 1e3:.4f} ms/step (shape {shape}); error {err['force_elem_err']:.2e}")
     d.close()
-    return rec
+    return rec, row
 
 
 def batch_cpu_baseline(state, p, nsample=8, seconds_each=1.5):

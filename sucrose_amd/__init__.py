@@ -84,6 +84,7 @@ def lib():
             "sx_forces": [vp, C.c_int, _PD, C.c_int],
             "sx_pair_eval": [vp, C.c_int, _PD, C.c_int, C.c_long, _PI, _PI, _PD, _PD, _PD, _PD, _PD],
             "sx_set_ecp_solver": [vp, C.c_int], "sx_ecp_stats": [vp, C.POINTER(C.c_long)],
+            "sx_set_pair_precision": [vp, C.c_int], "sx_get_pair_precision": [vp, C.POINTER(C.c_int)],
             "sx_nht_stats": [vp, C.POINTER(C.c_long)],
             "sx_set_bonded": [vp, C.c_int, C.c_int, _PI, C.c_double, C.c_double],
             "sx_bonded_forces": [vp],
@@ -353,6 +354,16 @@ class System:
 
     def set_ecp_solver(self, solver):
         self._ok(self.L.sx_set_ecp_solver(self.h, int(solver)))
+
+    def set_pair_precision(self, bits):
+        """32: FP32 pair terms for LJ, WCA, Debye-Hueckel and GB (variant 0);
+        64: FP64 throughout (the default)."""
+        self._ok(self.L.sx_set_pair_precision(self.h, int(bits)))
+
+    def pair_precision(self):
+        b = C.c_int()
+        self._ok(self.L.sx_get_pair_precision(self.h, C.byref(b)))
+        return b.value
 
     def ecp_stats(self):
         o = (C.c_long * 4)()

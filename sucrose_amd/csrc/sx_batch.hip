@@ -59,6 +59,7 @@ struct BatchState {
     hipGraphExec_t exec = nullptr;
     int graph_choice = -1;
     double graph_dt = -1, graph_damp = -1;   // kernel arguments baked into the graph
+    bool graph_f32 = false;                  // and the pair terms' precision
     int n_total = 0;                         // particle count and field storage at sx_batch_set
     const double *fields = nullptr;
 };
@@ -89,15 +90,21 @@ struct BatchSelf {
     GbSelf g;
 };
 
+// Gay-Berne factors of table particle q
+__device__ __forceinline__ void batch_gb_self(const double *tab, int ld, int q, GbSelf &g)
+{
+    g.u[0] = tab[TB_U0 * ld + q]; g.u[1] = tab[TB_U1 * ld + q]; g.u[2] = tab[TB_U2 * ld + q];
+    g.chi = tab[TB_CHI * ld + q]; g.xhi = tab[TB_XHI * ld + q];
+    g.lprp = tab[TB_LPRP * ld + q]; g.epsz = tab[TB_EPSZ * ld + q];
+}
+
 // (an LJ particle's reach is the cutoff)
 __device__ __forceinline__ void batch_self(const double *tab, int ld, int q, bool gb, double rc_lj, BatchSelf &me)
 {
     me.x = tab[TB_X * ld + q]; me.y = tab[TB_Y * ld + q]; me.z = tab[TB_Z * ld + q];
     me.reach = rc_lj;
     if (gb) {
-        me.g.u[0] = tab[TB_U0 * ld + q]; me.g.u[1] = tab[TB_U1 * ld + q]; me.g.u[2] = tab[TB_U2 * ld + q];
-        me.g.chi = tab[TB_CHI * ld + q]; me.g.xhi = tab[TB_XHI * ld + q];
-        me.g.lprp = tab[TB_LPRP * ld + q]; me.g.epsz = tab[TB_EPSZ * ld + q];
+        batch_gb_self(tab, ld, q, me.g);
         me.reach = tab[TB_REACH * ld + q];
     }
 }
@@ -113,9 +120,10 @@ __device__ __forceinline__ void batch_rij(const double *tab, int ld, int jq, boo
 }
 
 // false only where the pair's force, torques and energy are exactly zero:
-// beyond the LJ cutoff, or beyond the Gay-Berne reach of the pair's I
-__device__ __forceinline__ bool batch_in_reach(const double *tab, int ld, int q, int jq, const BatchSelf &me, bool gb,
-                                               double L)
+// beyond the LJ cutoff, or beyond the Gay-Berne reach of the pair's I (an
+// FP64 test in both precisions of the pair terms)
+__device__ __forceinline__ bool batch_in_reach(const double *tab, int ld, int q, int jq, const BatchSelf &me,
+                                               bool gb, double L)
 {
     const bool fwd = jq > q;
     double r[3];
@@ -126,16 +134,26 @@ __device__ __forceinline__ bool batch_in_reach(const double *tab, int ld, int q,
 }
 
 // adds pair (q, jq)'s contribution to particle q: force, torque, and the
-// pair energy once (on the pair's I)
+// pair energy once (on the pair's I); T = float: the FP32 pair mode (LJ in
+// FP32, gb_ in its mixed form), the sums in FP64 either way
+template <typename T>
+__device__ __forceinline__ void batch_gb(const double r[3], const GbSelf &A, const double u2[3], const double *p,
+                                         int mu, int nu, PairOutT<T> &o)
+{
+    if constexpr (sizeof(T) == sizeof(double)) gb_core<0>(r, A, u2, p[0], mu, nu, o);
+    else gb_core_f32(r, A, u2, p[0], mu, nu, o);
+}
+
+template <typename T>
 __device__ __forceinline__ void batch_eval(const double *tab, int ld, int q, int jq, const BatchSelf &me, bool gb,
                                            const double *p, int mu, int nu, double L, double acc[7])
 {
     const bool fwd = jq > q;
     double r[3];
     batch_rij(tab, ld, jq, fwd, me, L, r);
-    PairOut o;
+    PairOutT<T> o;
     if (!gb) {
-        lj_core(r, p, o);
+        lj_core<T>(r, p, o);
         if (fwd) {
             acc[0] = acc[0] + o.f[0]; acc[1] = acc[1] + o.f[1]; acc[2] = acc[2] + o.f[2];
             acc[6] = acc[6] + o.pe;
@@ -146,16 +164,14 @@ __device__ __forceinline__ void batch_eval(const double *tab, int ld, int q, int
     }
     const double uj[3] = {tab[TB_U0 * ld + jq], tab[TB_U1 * ld + jq], tab[TB_U2 * ld + jq]};
     if (fwd) {
-        gb_core<0>(r, me.g, uj, p[0], mu, nu, o);
+        batch_gb(r, me.g, uj, p, mu, nu, o);
         acc[0] = acc[0] + o.f[0]; acc[1] = acc[1] + o.f[1]; acc[2] = acc[2] + o.f[2];
         acc[3] = acc[3] + o.ti[0]; acc[4] = acc[4] + o.ti[1]; acc[5] = acc[5] + o.ti[2];
         acc[6] = acc[6] + o.pe;
     } else {
         GbSelf gj;
-        gj.u[0] = uj[0]; gj.u[1] = uj[1]; gj.u[2] = uj[2];
-        gj.chi = tab[TB_CHI * ld + jq]; gj.xhi = tab[TB_XHI * ld + jq];
-        gj.lprp = tab[TB_LPRP * ld + jq]; gj.epsz = tab[TB_EPSZ * ld + jq];
-        gb_core<0>(r, gj, me.g.u, p[0], mu, nu, o);
+        batch_gb_self(tab, ld, jq, gj);
+        batch_gb(r, gj, me.g.u, p, mu, nu, o);
         acc[0] = acc[0] - o.f[0]; acc[1] = acc[1] - o.f[1]; acc[2] = acc[2] - o.f[2];
         acc[3] = acc[3] + o.tj[0]; acc[4] = acc[4] + o.tj[1]; acc[5] = acc[5] + o.tj[2];
     }
@@ -198,7 +214,7 @@ __global__ void __launch_bounds__(256) k_batch_initial(DState s, const BatchSys 
 }
 
 // initialize_forces + the pair loop for particles [q0, q1) of one system
-template <int BT, int LPP, bool LDS>
+template <int BT, int LPP, bool LDS, typename T>
 __global__ void __launch_bounds__(BT) k_batch_forces(DState s, const BatchSys *sys, const int4 *blocks,
                                                      const double *gtab, double *pe)
 {
@@ -230,7 +246,7 @@ __global__ void __launch_bounds__(BT) k_batch_forces(DState s, const BatchSys *s
             double acc[7] = {0, 0, 0, 0, 0, 0, 0};
             for (int jq = lane; jq < S.n; jq += LPP) {
                 if (jq == q || !batch_in_reach(tab, ld, q, jq, me, gb, L)) continue;
-                batch_eval(tab, ld, q, jq, me, gb, S.p, mu, nu, L, acc);
+                batch_eval<T>(tab, ld, q, jq, me, gb, S.p, mu, nu, L, acc);
             }
 #pragma unroll
             for (int off = LPP / 2; off > 0; off >>= 1)
@@ -270,7 +286,7 @@ __global__ void __launch_bounds__(BT) k_batch_forces(DState s, const BatchSys *s
                 if (cnt >= 64) {
                     const int j = jq_buf[lane];
                     const int carry = cnt - 64 > lane ? jq_buf[64 + lane] : 0;
-                    batch_eval(tab, ld, q, j, me, gb, S.p, mu, nu, L, acc);
+                    batch_eval<T>(tab, ld, q, j, me, gb, S.p, mu, nu, L, acc);
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -281,7 +297,7 @@ __global__ void __launch_bounds__(BT) k_batch_forces(DState s, const BatchSys *s
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
             }
-            if (lane < cnt) batch_eval(tab, ld, q, jq_buf[lane], me, gb, S.p, mu, nu, L, acc);
+            if (lane < cnt) batch_eval<T>(tab, ld, q, jq_buf[lane], me, gb, S.p, mu, nu, L, acc);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -348,7 +364,7 @@ __global__ void __launch_bounds__(256) k_batch_final(DState s, const BatchSys *s
 // so only workgroup barriers occur and every workgroup leaves once the queue
 // is empty; the last one to leave resets the queue and advances the step
 // counter.  qctr: [0] queue, [1] workgroups done.
-template <int BT>
+template <int BT, typename T>
 __global__ void __launch_bounds__(BT) k_batch_step(DState s, const BatchSys *sys, const int *order, int nsys,
                                                    double dt, double damp, int *dstep, double *gtab, double *pe,
                                                    double *out, unsigned *qctr, int lds_ok)
@@ -421,7 +437,7 @@ __global__ void __launch_bounds__(BT) k_batch_step(DState s, const BatchSys *sys
                 if (cnt >= 64) {
                     const int j = jq_buf[lane];
                     const int carry = cnt - 64 > lane ? jq_buf[64 + lane] : 0;
-                    batch_eval(tab, ld, q, j, me, gb, S.p, mu, nu, S.L, acc);
+                    batch_eval<T>(tab, ld, q, j, me, gb, S.p, mu, nu, S.L, acc);
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -432,7 +448,7 @@ __global__ void __launch_bounds__(BT) k_batch_step(DState s, const BatchSys *sys
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
             }
-            if (lane < cnt) batch_eval(tab, ld, q, jq_buf[lane], me, gb, S.p, mu, nu, S.L, acc);
+            if (lane < cnt) batch_eval<T>(tab, ld, q, jq_buf[lane], me, gb, S.p, mu, nu, S.L, acc);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -555,7 +571,8 @@ static void batch_blocks(sx_ctx *c, BatchState *B, int choice)
     B->blocks_choice = choice;
 }
 
-static void batch_launch(sx_ctx *c, BatchState *B, int choice, double dt, double damp)
+template <typename T>
+static void batch_launch_t(sx_ctx *c, BatchState *B, int choice, double dt, double damp)
 {
     DState s = c->dstate();
     if (shape_fused(choice)) {
@@ -568,10 +585,10 @@ static void batch_launch(sx_ctx *c, BatchState *B, int choice, double dt, double
         const size_t lbytes = lds ? tbytes : 0;
         const dim3 g(std::min(B->ngrid, B->nsys));
         if (BATCH_SHAPES[choice][1] == 256)
-            hipLaunchKernelGGL(k_batch_step<256>, g, dim3(256), lbytes, c->stream, s, B->d_sys, B->d_order, B->nsys,
+            hipLaunchKernelGGL((k_batch_step<256, T>), g, dim3(256), lbytes, c->stream, s, B->d_sys, B->d_order, B->nsys,
                                dt, damp, B->d_step, B->d_tab, B->d_pe, B->d_out, B->d_qctr, lds ? 1 : 0);
         else
-            hipLaunchKernelGGL(k_batch_step<512>, g, dim3(512), lbytes, c->stream, s, B->d_sys, B->d_order, B->nsys,
+            hipLaunchKernelGGL((k_batch_step<512, T>), g, dim3(512), lbytes, c->stream, s, B->d_sys, B->d_order, B->nsys,
                                dt, damp, B->d_step, B->d_tab, B->d_pe, B->d_out, B->d_qctr, lds ? 1 : 0);
         HIPCHK(hipGetLastError());
         return;
@@ -584,10 +601,10 @@ static void batch_launch(sx_ctx *c, BatchState *B, int choice, double dt, double
 #define SX_BATCH_CASE(K, BT, LPP)                                                                                  \
     case K:                                                                                                        \
         if (lds)                                                                                                   \
-            hipLaunchKernelGGL((k_batch_forces<BT, LPP, true>), g, dim3(BT), lbytes, c->stream, s, B->d_sys,       \
+            hipLaunchKernelGGL((k_batch_forces<BT, LPP, true, T>), g, dim3(BT), lbytes, c->stream, s, B->d_sys,       \
                                B->d_blocks, B->d_tab, B->d_pe);                                                    \
         else                                                                                                       \
-            hipLaunchKernelGGL((k_batch_forces<BT, LPP, false>), g, dim3(BT), 0, c->stream, s, B->d_sys,           \
+            hipLaunchKernelGGL((k_batch_forces<BT, LPP, false, T>), g, dim3(BT), 0, c->stream, s, B->d_sys,           \
                                B->d_blocks, B->d_tab, B->d_pe);                                                    \
         break;
     switch (choice) {
@@ -604,6 +621,13 @@ static void batch_launch(sx_ctx *c, BatchState *B, int choice, double dt, double
 #undef SX_BATCH_CASE
     hipLaunchKernelGGL(k_batch_final, dim3(B->nsys), dim3(256), 0, c->stream, s, B->d_sys, dt, B->d_pe, B->d_out,
                        B->d_ticket, B->d_step, B->nsys);
+}
+
+// the pair terms in FP32 when the context's pair precision is 32 bits
+static void batch_launch(sx_ctx *c, BatchState *B, int choice, double dt, double damp)
+{
+    if (c->pair_f32) batch_launch_t<float>(c, B, choice, dt, damp);
+    else batch_launch_t<double>(c, B, choice, dt, damp);
 }
 
 extern "C" {
@@ -718,7 +742,8 @@ int sx_batch_run(sx_ctx *c, int nsteps, double dt, double damping, int first_ste
             sx_check_device_error(c);
             return;
         }
-        if (!B->exec || B->graph_choice != B->choice || B->graph_dt != dt || B->graph_damp != damping) {
+        if (!B->exec || B->graph_choice != B->choice || B->graph_dt != dt || B->graph_damp != damping ||
+            B->graph_f32 != c->pair_f32) {
             if (B->exec) { hipGraphExecDestroy(B->exec); B->exec = nullptr; }
             if (B->graph) { hipGraphDestroy(B->graph); B->graph = nullptr; }
             HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -728,6 +753,7 @@ int sx_batch_run(sx_ctx *c, int nsteps, double dt, double damping, int first_ste
             B->graph_choice = B->choice;
             B->graph_dt = dt;
             B->graph_damp = damping;
+            B->graph_f32 = c->pair_f32;
         }
         for (int t = 0; t < nsteps; ++t) HIPCHK(hipGraphLaunch(B->exec, c->stream));
         sx_check_device_error(c);

@@ -167,6 +167,8 @@ struct sx_ctx {
     // per-particle Gay-Berne factors (GB_TAB doubles per particle)
     double *d_gb_tab = nullptr;
     int gb_tab_cap = 0;
+    // FP32 pair terms for LJ, WCA, Debye-Hueckel and gb_ (sx_set_pair_precision)
+    bool pair_f32 = false;
     int ecp_solver = SX_ECP_BRENT_REFERENCE;
     long ecp_stats[4] = {0, 0, 0, 0};
     // MLP (learned warm start)

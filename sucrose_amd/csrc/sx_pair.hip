@@ -28,7 +28,7 @@ struct Params {
 // ---------------------------------------------------------------------------
 // isotropic / GB gather
 // ---------------------------------------------------------------------------
-template <int KIND, int LPP, bool TORQUE>
+template <int KIND, int LPP, bool TORQUE, typename T>
 __global__ void __launch_bounds__(256) k_gather(DState s, const int *row_start, const int *nbr, Params P,
                                                 double *pe_acc)
 {
@@ -39,14 +39,14 @@ __global__ void __launch_bounds__(256) k_gather(DState s, const int *row_start, 
         int b = row_start[i], e = row_start[i + 1];
         for (int q = b + l; q < e; q += LPP) {
             int j = nbr[q];
-            PairOut o;
+            PairOutT<T> o;
             if (j > i) {
-                d_pair<KIND>(s, i, j, P.p, o);
+                d_pair_t<KIND>(s, i, j, P.p, o);
                 acc[0] = acc[0] + o.f[0]; acc[1] = acc[1] + o.f[1]; acc[2] = acc[2] + o.f[2];
                 if (TORQUE) { acc[3] = acc[3] + o.ti[0]; acc[4] = acc[4] + o.ti[1]; acc[5] = acc[5] + o.ti[2]; }
                 acc[6] = acc[6] + o.pe;
             } else {
-                d_pair<KIND>(s, j, i, P.p, o);
+                d_pair_t<KIND>(s, j, i, P.p, o);
                 acc[0] = acc[0] - o.f[0]; acc[1] = acc[1] - o.f[1]; acc[2] = acc[2] - o.f[2];
                 if (TORQUE) { acc[3] = acc[3] + o.tj[0]; acc[4] = acc[4] + o.tj[1]; acc[5] = acc[5] + o.tj[2]; }
             }
@@ -79,9 +79,22 @@ __global__ void k_gb_self(DState s, int mu, double *tab)
     gb_self_store(g, tab + (long)GB_TAB * i);
 }
 
+template <int VARIANT, typename T>
+__device__ __forceinline__ void gb_tab_t(const DState &s, int a, int b, const double *p, const double *tab,
+                                         PairOutT<T> &o)
+{
+    if constexpr (sizeof(T) == sizeof(double)) {
+        d_gb_tab<VARIANT>(s, a, b, p, tab, o);
+    } else {
+        static_assert(VARIANT == 0, "the FP32 pair mode covers gb_ only");
+        d_gb_tab_f32(s, a, b, p, tab, o);
+    }
+}
+
 // k_gather for the Gay-Berne family with the per-particle factors read from
-// the table: the same pairs in the same order, the same values
-template <int VARIANT, int LPP>
+// the table: the same pairs in the same order, the same values (T = float:
+// gb_ in the FP32 pair mode's mixed form)
+template <int VARIANT, int LPP, typename T>
 __global__ void __launch_bounds__(256) k_gather_gb(DState s, const int *row_start, const int *nbr, Params P,
                                                    const double *tab, double *pe_acc)
 {
@@ -92,14 +105,14 @@ __global__ void __launch_bounds__(256) k_gather_gb(DState s, const int *row_star
         int b = row_start[i], e = row_start[i + 1];
         for (int q = b + l; q < e; q += LPP) {
             int j = nbr[q];
-            PairOut o;
+            PairOutT<T> o;
             if (j > i) {
-                d_gb_tab<VARIANT>(s, i, j, P.p, tab, o);
+                gb_tab_t<VARIANT>(s, i, j, P.p, tab, o);
                 acc[0] = acc[0] + o.f[0]; acc[1] = acc[1] + o.f[1]; acc[2] = acc[2] + o.f[2];
                 acc[3] = acc[3] + o.ti[0]; acc[4] = acc[4] + o.ti[1]; acc[5] = acc[5] + o.ti[2];
                 acc[6] = acc[6] + o.pe;
             } else {
-                d_gb_tab<VARIANT>(s, j, i, P.p, tab, o);
+                gb_tab_t<VARIANT>(s, j, i, P.p, tab, o);
                 acc[0] = acc[0] - o.f[0]; acc[1] = acc[1] - o.f[1]; acc[2] = acc[2] - o.f[2];
                 acc[3] = acc[3] + o.tj[0]; acc[4] = acc[4] + o.tj[1]; acc[5] = acc[5] + o.tj[2];
             }
@@ -124,7 +137,7 @@ __global__ void __launch_bounds__(256) k_gather_gb(DState s, const int *row_star
 // neighbours): each kind's sums are kept apart, reduced as k_gather reduces
 // them and added in the order of the two loops, so forces and energies equal
 // those of two k_gather launches with the same lanes per particle
-template <int K1, int K2, int LPP>
+template <int K1, int K2, int LPP, typename T>
 __global__ void __launch_bounds__(256) k_gather2(DState s, const int *row_start, const int *nbr, Params P1,
                                                  Params P2, double *pe_acc)
 {
@@ -135,17 +148,17 @@ __global__ void __launch_bounds__(256) k_gather2(DState s, const int *row_start,
         int b = row_start[i], e = row_start[i + 1];
         for (int q = b + l; q < e; q += LPP) {
             int j = nbr[q];
-            PairOut o1, o2;
+            PairOutT<T> o1, o2;
             if (j > i) {
-                d_pair<K1>(s, i, j, P1.p, o1);
-                d_pair<K2>(s, i, j, P2.p, o2);
+                d_pair_t<K1>(s, i, j, P1.p, o1);
+                d_pair_t<K2>(s, i, j, P2.p, o2);
                 a1[0] = a1[0] + o1.f[0]; a1[1] = a1[1] + o1.f[1]; a1[2] = a1[2] + o1.f[2];
                 a1[3] = a1[3] + o1.pe;
                 a2[0] = a2[0] + o2.f[0]; a2[1] = a2[1] + o2.f[1]; a2[2] = a2[2] + o2.f[2];
                 a2[3] = a2[3] + o2.pe;
             } else {
-                d_pair<K1>(s, j, i, P1.p, o1);
-                d_pair<K2>(s, j, i, P2.p, o2);
+                d_pair_t<K1>(s, j, i, P1.p, o1);
+                d_pair_t<K2>(s, j, i, P2.p, o2);
                 a1[0] = a1[0] - o1.f[0]; a1[1] = a1[1] - o1.f[1]; a1[2] = a1[2] - o1.f[2];
                 a2[0] = a2[0] - o2.f[0]; a2[1] = a2[1] - o2.f[1]; a2[2] = a2[2] - o2.f[2];
             }
@@ -450,8 +463,8 @@ static void check_params(int kind, int np)
     if (np < need) throw sx_error{SX_ERR_ARG, "too few parameters for this force kind"};
 }
 
-template <int KIND, bool TORQUE>
-static void launch_gather(sx_ctx *c, const Params &P, int lpp)
+template <int KIND, bool TORQUE, typename T>
+static void launch_gather_t(sx_ctx *c, const Params &P, int lpp)
 {
     int n = c->n;
     long threads = (long)n * lpp;
@@ -459,16 +472,29 @@ static void launch_gather(sx_ctx *c, const Params &P, int lpp)
     sx_ensure_part(c, nb, 1);
     DState s = c->dstate();
     switch (lpp) {
-    case 1: hipLaunchKernelGGL((k_gather<KIND, 1, TORQUE>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_pe_acc); break;
-    case 4: hipLaunchKernelGGL((k_gather<KIND, 4, TORQUE>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_pe_acc); break;
-    case 8: hipLaunchKernelGGL((k_gather<KIND, 8, TORQUE>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_pe_acc); break;
-    default: hipLaunchKernelGGL((k_gather<KIND, 16, TORQUE>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_pe_acc); break;
+    case 1: hipLaunchKernelGGL((k_gather<KIND, 1, TORQUE, T>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_pe_acc); break;
+    case 4: hipLaunchKernelGGL((k_gather<KIND, 4, TORQUE, T>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_pe_acc); break;
+    case 8: hipLaunchKernelGGL((k_gather<KIND, 8, TORQUE, T>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_pe_acc); break;
+    default: hipLaunchKernelGGL((k_gather<KIND, 16, TORQUE, T>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_pe_acc); break;
     }
 }
 
+// FP32 pair terms where the context asks for them and the kind has them
+template <int KIND, bool TORQUE>
+static void launch_gather(sx_ctx *c, const Params &P, int lpp)
+{
+    if constexpr (sx_has_f32<KIND>()) {
+        if (c->pair_f32) {
+            launch_gather_t<KIND, TORQUE, float>(c, P, lpp);
+            return;
+        }
+    }
+    launch_gather_t<KIND, TORQUE, double>(c, P, lpp);
+}
+
 // the Gay-Berne family: the per-particle table, then the gather over it
-template <int VARIANT>
-static void launch_gather_gb(sx_ctx *c, const Params &P, int lpp)
+template <int VARIANT, typename T>
+static void launch_gather_gb_t(sx_ctx *c, const Params &P, int lpp)
 {
     int n = c->n;
     if (c->gb_tab_cap < n) {
@@ -478,14 +504,28 @@ static void launch_gather_gb(sx_ctx *c, const Params &P, int lpp)
         c->have_graph = false;
     }
     DState s = c->dstate();
+    const double *tab = c->d_gb_tab;
     hipLaunchKernelGGL(k_gb_self, dim3(sx_blocks(n, 256)), dim3(256), 0, c->stream, s, (int)P.p[1], c->d_gb_tab);
     int nb = sx_blocks((long)n * lpp, 256);
     switch (lpp) {
-    case 1: hipLaunchKernelGGL((k_gather_gb<VARIANT, 1>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_gb_tab, c->d_pe_acc); break;
-    case 4: hipLaunchKernelGGL((k_gather_gb<VARIANT, 4>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_gb_tab, c->d_pe_acc); break;
-    case 8: hipLaunchKernelGGL((k_gather_gb<VARIANT, 8>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_gb_tab, c->d_pe_acc); break;
-    default: hipLaunchKernelGGL((k_gather_gb<VARIANT, 16>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, c->d_gb_tab, c->d_pe_acc); break;
+    case 1: hipLaunchKernelGGL((k_gather_gb<VARIANT, 1, T>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, tab, c->d_pe_acc); break;
+    case 4: hipLaunchKernelGGL((k_gather_gb<VARIANT, 4, T>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, tab, c->d_pe_acc); break;
+    case 8: hipLaunchKernelGGL((k_gather_gb<VARIANT, 8, T>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, tab, c->d_pe_acc); break;
+    default: hipLaunchKernelGGL((k_gather_gb<VARIANT, 16, T>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P, tab, c->d_pe_acc); break;
     }
+}
+
+// FP32 for gb_ (variant 0) where the context asks for it
+template <int VARIANT>
+static void launch_gather_gb(sx_ctx *c, const Params &P, int lpp)
+{
+    if constexpr (VARIANT == 0) {
+        if (c->pair_f32) {
+            launch_gather_gb_t<0, float>(c, P, lpp);
+            return;
+        }
+    }
+    launch_gather_gb_t<VARIANT, double>(c, P, lpp);
 }
 
 static int choose_lpp(sx_ctx *c, int kind)
@@ -816,17 +856,24 @@ void sx_ecp_records(sx_ctx *c, double **pp, float **scr, double **pos4)
     *pos4 = c->d_pos4;
 }
 
-template <int K1, int K2>
-static void launch_gather2(sx_ctx *c, const Params &P1, const Params &P2, int lpp)
+template <int K1, int K2, typename T>
+static void launch_gather2_t(sx_ctx *c, const Params &P1, const Params &P2, int lpp)
 {
     const int nb = sx_blocks((long)c->n * lpp, 256);
     DState s = c->dstate();
     switch (lpp) {
-    case 1: hipLaunchKernelGGL((k_gather2<K1, K2, 1>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P1, P2, c->d_pe_acc); break;
-    case 4: hipLaunchKernelGGL((k_gather2<K1, K2, 4>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P1, P2, c->d_pe_acc); break;
-    case 8: hipLaunchKernelGGL((k_gather2<K1, K2, 8>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P1, P2, c->d_pe_acc); break;
-    default: hipLaunchKernelGGL((k_gather2<K1, K2, 16>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P1, P2, c->d_pe_acc); break;
+    case 1: hipLaunchKernelGGL((k_gather2<K1, K2, 1, T>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P1, P2, c->d_pe_acc); break;
+    case 4: hipLaunchKernelGGL((k_gather2<K1, K2, 4, T>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P1, P2, c->d_pe_acc); break;
+    case 8: hipLaunchKernelGGL((k_gather2<K1, K2, 8, T>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P1, P2, c->d_pe_acc); break;
+    default: hipLaunchKernelGGL((k_gather2<K1, K2, 16, T>), dim3(nb), dim3(256), 0, c->stream, s, c->nl.row_start, c->nl.nbr, P1, P2, c->d_pe_acc); break;
     }
+}
+
+template <int K1, int K2>
+static void launch_gather2(sx_ctx *c, const Params &P1, const Params &P2, int lpp)
+{
+    if (c->pair_f32) launch_gather2_t<K1, K2, float>(c, P1, P2, lpp);
+    else launch_gather2_t<K1, K2, double>(c, P1, P2, lpp);
 }
 
 // two pair loops of the step driver in one pass where both kinds are WCA and
@@ -984,6 +1031,20 @@ int sx_set_ecp_solver(sx_ctx *c, int solver)
         c->ecp_solver = solver;
         c->have_graph = false;
     });
+}
+
+int sx_set_pair_precision(sx_ctx *c, int bits)
+{
+    return guard(c, [&] {
+        if (bits != 32 && bits != 64) throw sx_error{SX_ERR_ARG, "pair precision must be 32 or 64 bits"};
+        if (c->pair_f32 != (bits == 32)) c->have_graph = false;
+        c->pair_f32 = bits == 32;
+    });
+}
+
+int sx_get_pair_precision(sx_ctx *c, int *bits)
+{
+    return guard(c, [&] { *bits = c->pair_f32 ? 32 : 64; });
 }
 
 int sx_ecp_stats(sx_ctx *c, long out[4])

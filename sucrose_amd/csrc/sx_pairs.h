@@ -3,15 +3,32 @@
 // evaluation order (the translation unit is built with -ffp-contract=off), so
 // a pair's PE, FXIJ, TXI, TXJ equal the reference's to the last bit for the
 // same FP64 inputs, transcendental functions aside.
+//
+// The FP32 pair mode.  The cores of LJ, WCA and Debye-Hueckel are templates
+// on the arithmetic type T: T = double is the form above, T = float the same
+// expressions in FP32 with the hardware reciprocal, square root and exp2
+// (about 1 ulp each).  gb_ has a mixed form of its own (gb_core_f32).  In
+// both the separation, the cutoff test that decides whether a pair interacts
+// and every sum over pairs stay FP64.
 #pragma once
 #include "sx_internal.h"
 
-struct PairOut {
-    double pe;
-    double f[3];
-    double ti[3];
-    double tj[3];
+template <typename T>
+struct PairOutT {
+    T pe;
+    T f[3];
+    T ti[3];
+    T tj[3];
 };
+using PairOut = PairOutT<double>;
+
+__device__ __forceinline__ double sx_dv(double a, double b) { return a / b; }
+__device__ __forceinline__ float sx_dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double sx_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float sx_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double sx_exp(double x) { return exp(x); }
+// e^x = 2^(x log2 e)
+__device__ __forceinline__ float sx_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
 
 __device__ __forceinline__ void pair_rij(const DState &s, int a, int b, double r[3])
 {
@@ -26,19 +43,21 @@ __device__ __forceinline__ void pair_rij(const DState &s, int a, int b, double r
 }
 
 // lj_calculate_forces (force.f90:6-49) on a minimum-image separation r = R(a) - R(b)
-__device__ __forceinline__ void lj_core(const double r[3], const double *p, PairOut &o)
+template <typename T = double>
+__device__ __forceinline__ void lj_core(const double r[3], const double *p, PairOutT<T> &o)
 {
-    double sigma = p[0], epsilon = p[1], cutoff = p[2];
-    double rij_sq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-    double rcut_sq = cutoff * cutoff;
-    bool cmask = rij_sq < rcut_sq;
-    double sr2 = (sigma * sigma) / rij_sq, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
-    double sc2 = (sigma * sigma) / rcut_sq, sc6 = sc2 * sc2 * sc2, sc12 = sc6 * sc6;
-    double pot = 4.0 * epsilon * (sr12 - sr6);
-    double pot_cut = 4.0 * epsilon * (sc12 - sc6);
-    double coeff = 24.0 * epsilon * (2.0 * sr12 - sr6);
-    o.pe = cmask ? (pot - pot_cut) : 0.0;
-    for (int k = 0; k < 3; ++k) o.f[k] = cmask ? (coeff * r[k] / rij_sq) : 0.0;
+    const double rsq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    const bool cmask = rsq < p[2] * p[2];
+    T sigma = T(p[0]), epsilon = T(p[1]), cutoff = T(p[2]);
+    T rij_sq = T(rsq);
+    T rcut_sq = cutoff * cutoff;
+    T sr2 = sx_dv(sigma * sigma, rij_sq), sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
+    T sc2 = sx_dv(sigma * sigma, rcut_sq), sc6 = sc2 * sc2 * sc2, sc12 = sc6 * sc6;
+    T pot = T(4) * epsilon * (sr12 - sr6);
+    T pot_cut = T(4) * epsilon * (sc12 - sc6);
+    T coeff = T(24) * epsilon * (T(2) * sr12 - sr6);
+    o.pe = cmask ? (pot - pot_cut) : T(0);
+    for (int k = 0; k < 3; ++k) o.f[k] = cmask ? sx_dv(coeff * T(r[k]), rij_sq) : T(0);
 }
 
 __device__ __forceinline__ void d_lj(const DState &s, int a, int b, const double *p, PairOut &o)
@@ -93,20 +112,27 @@ __device__ __forceinline__ void d_ljw(const DState &s, int a, int b, const doubl
 }
 
 // wca_calculate_forces (force.f90:155-190)
+template <typename T = double>
+__device__ __forceinline__ void wca_core(const double r[3], const double *p, PairOutT<T> &o)
+{
+    o.pe = T(0); o.f[0] = o.f[1] = o.f[2] = T(0);
+    const double rsq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    if (rsq <= p[0] * p[0]) {
+        T sigma = T(p[0]), epsilon = T(p[1]);
+        T rij_sq = T(rsq);
+        T sigma_sq = sigma * sigma;
+        T sr2 = sx_dv(sigma_sq, rij_sq), sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
+        o.pe = epsilon * (sr12 - T(2) * sr6) + epsilon;
+        T coeff = T(12) * epsilon * (sr12 - sr6);
+        for (int k = 0; k < 3; ++k) o.f[k] = sx_dv(coeff * T(r[k]), rij_sq);
+    }
+}
+
 __device__ __forceinline__ void d_wca(const DState &s, int a, int b, const double *p, PairOut &o)
 {
-    double sigma = p[0], epsilon = p[1];
     double r[3];
     pair_rij(s, a, b, r);
-    o.pe = 0.0; o.f[0] = o.f[1] = o.f[2] = 0.0;
-    double rij_sq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-    double sigma_sq = sigma * sigma;
-    if (rij_sq <= sigma_sq) {
-        double sr2 = sigma_sq / rij_sq, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
-        o.pe = epsilon * (sr12 - 2.0 * sr6) + epsilon;
-        double coeff = 12.0 * epsilon * (sr12 - sr6);
-        for (int k = 0; k < 3; ++k) o.f[k] = coeff * r[k] / rij_sq;
-    }
+    wca_core(r, p, o);
 }
 
 // morse_calculate_forces (force.f90:199-233)
@@ -128,20 +154,30 @@ __device__ __forceinline__ void d_morse(const DState &s, int a, int b, const dou
     }
 }
 
-// dh_calculate_forces (force.f90:2589-2617)
+// dh_calculate_forces (force.f90:2589-2617) for charges qa, qb; in FP32 the
+// cutoff test compares the FP64 squared distance with the squared cutoff
+template <typename T = double>
+__device__ __forceinline__ void dh_core(const double r[3], double qa, double qb, const double *p, PairOutT<T> &o)
+{
+    const T PI_REF = T(3.1415926535897932384626433832795);
+    T cutoff = T(p[0]), db_len = T(p[1]), dielectric = T(p[2]);
+    const double rsq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    T rij_sq = T(rsq);
+    T rij_mag = sx_sqrt(rij_sq);
+    bool cmask;
+    if constexpr (sizeof(T) == sizeof(double)) cmask = rij_mag < cutoff;
+    else cmask = rsq < p[0] * p[0];
+    T pe = sx_dv(T(qa) * T(qb) * sx_exp(sx_dv(-rij_mag, db_len)), T(4) * PI_REF * dielectric * rij_mag);
+    T fac = pe * (T(1) + sx_dv(rij_mag, db_len));
+    o.pe = cmask ? pe : T(0);
+    for (int k = 0; k < 3; ++k) o.f[k] = cmask ? sx_dv(fac * T(r[k]), rij_sq) : T(0);
+}
+
 __device__ __forceinline__ void d_dh(const DState &s, int a, int b, const double *p, PairOut &o)
 {
-    const double PI_REF = 3.1415926535897932384626433832795;
-    double cutoff = p[0], db_len = p[1], dielectric = p[2];
     double r[3];
     pair_rij(s, a, b, r);
-    double rij_sq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-    double rij_mag = sqrt(rij_sq);
-    bool cmask = rij_mag < cutoff;
-    double pe = (s.f[SX_CHARGE][a] * s.f[SX_CHARGE][b] * exp(-rij_mag / db_len)) / (4.0 * PI_REF * dielectric * rij_mag);
-    double fac = pe * (1.0 + rij_mag / db_len);
-    o.pe = cmask ? pe : 0.0;
-    for (int k = 0; k < 3; ++k) o.f[k] = cmask ? (fac * r[k] / rij_sq) : 0.0;
+    dh_core(r, s.f[SX_CHARGE][a], s.f[SX_CHARGE][b], p, o);
 }
 
 // coulombic_calculate_forces (force.f90:2627-2665): switched off over the
@@ -175,11 +211,12 @@ __device__ __forceinline__ void d_coulomb(const DState &s, int a, int b, const d
 // ---------------------------------------------------------------------------
 // Gay-Berne (force.f90:252-533)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double d_ipow(double x, int n)
+template <typename T>
+__device__ __forceinline__ T d_ipow(T x, int n)
 {
     // binary powering, the lowering of the reference's constant integer powers
-    if (n == 0) return 1.0;
-    double result = 1.0, base = x;
+    if (n == 0) return T(1);
+    T result = T(1), base = x;
     bool first = true;
     while (n > 0) {
         if (n & 1) {
@@ -449,6 +486,112 @@ __device__ __forceinline__ void d_gb_tab(const DState &s, int a, int b, const do
 }
 
 // ---------------------------------------------------------------------------
+// gb_ (variant 0) in the FP32 pair mode.  gb_ is ill-conditioned for FP32:
+// for close pairs with aligned axes g(chi) cancels towards (1-chi)/(1+chi),
+// sigma = sig_x / sqrt(g) carries that error into r - sigma + sigma_0, and the
+// 12th power of sr amplifies it again, so rounding the inputs alone to FP32
+// moves such a pair's force by ~3e-5 of its size; the torques' cross products
+// with the axes cancel for the same pairs.  The mixed form keeps in FP64 the
+// orientation function g of chi and chi' with its gradients in r, u1 and u2
+// (gb_g, gb_dgdr, gb_dgdu, each division by 1 +- chi u1.u2 done once), sigma,
+// sr and its powers, dR/dr and the torque sums; the strength factors
+// eps1^nu eps2^mu and their derivatives, the force sums and the switch are
+// FP32.  It needs 7 FP64 divisions and 2 square roots per pair, where the
+// reference-order FP64 form needs about 60 divisions.
+// ---------------------------------------------------------------------------
+struct GbOri {
+    double g, dgr[3], dgu1[3], dgu2[3];
+};
+
+__device__ __forceinline__ void gb_ori(double c, const GbGeo &q, double inv_rsq, GbOri &o)
+{
+    const double a = (q.ru1 + q.ru2) / (1.0 + c * q.uu), b = (q.ru1 - q.ru2) / (1.0 - c * q.uu);
+    o.g = 1.0 - 0.5 * c * inv_rsq * (a * (q.ru1 + q.ru2) + b * (q.ru1 - q.ru2));
+    const double h = 0.5 * c * c * inv_rsq * (a * a - b * b);
+    for (int k = 0; k < 3; ++k) {
+        o.dgr[k] = -c * inv_rsq * (a * (q.u1[k] + q.u2[k]) + b * (q.u1[k] - q.u2[k])) +
+                   2.0 * q.r[k] * inv_rsq * (1.0 - o.g);
+        o.dgu1[k] = -c * q.r[k] * inv_rsq * (a + b) + h * q.u2[k];
+        o.dgu2[k] = -c * q.r[k] * inv_rsq * (a - b) + h * q.u1[k];
+    }
+}
+
+__device__ __forceinline__ void gb_core_f32(const double r[3], const GbSelf &A, const double u2[3], double cutoff,
+                                            int mu, int nu, PairOutT<float> &o)
+{
+    o.pe = 0.0f;
+    for (int k = 0; k < 3; ++k) { o.f[k] = 0.0f; o.ti[k] = 0.0f; o.tj[k] = 0.0f; }
+    GbGeo q;
+    for (int k = 0; k < 3; ++k) { q.r[k] = r[k]; q.u1[k] = A.u[k]; q.u2[k] = u2[k]; }
+    q.rsq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    q.ru1 = r[0] * q.u1[0] + r[1] * q.u1[1] + r[2] * q.u1[2];
+    q.ru2 = r[0] * q.u2[0] + r[1] * q.u2[1] + r[2] * q.u2[2];
+    q.uu = q.u1[0] * q.u2[0] + q.u1[1] * q.u2[1] + q.u1[2] * q.u2[2];
+    const double rij_mag = sqrt(q.rsq), inv_mag = 1.0 / rij_mag, inv_rsq = inv_mag * inv_mag;
+    GbOri C, X;
+    gb_ori(A.chi, q, inv_rsq, C);
+    const double rs = 1.0 / sqrt(C.g);
+    const double sigma = A.lprp * rs;            // sigma_0 = 1
+    const double rlim = sigma + cutoff - 1.0;
+    if (!(rij_mag < rlim)) return;
+    gb_ori(A.xhi, q, inv_rsq, X);
+    const double sr = 1.0 / (rij_mag - sigma + 1.0);
+    const double sr2 = sr * sr, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
+    const double srx = sr12 - 2.0 * sr6, srd = 12.0 * (sr6 - sr12) * sr;
+    const double g32 = sigma * rs * rs;          // sigma / g
+    // strength factors
+    const float chu = (float)(A.chi * q.uu);
+    const float eps1 = __builtin_amdgcn_rsqf(1.0f - chu * chu);
+    const float eps2 = sx_dv((float)X.g, (float)A.epsz);
+    const float e1n = d_ipow(eps1, nu), e2m = d_ipow(eps2, mu);
+    const float e2m1 = d_ipow(eps2, mu - 1), e1n2 = d_ipow(eps1, nu + 2);
+    const float e12 = e1n * e2m, de2 = e1n * ((float)mu * e2m1);
+    const float fsrx = (float)srx, fsrd = (float)srd;
+    float pot = e12 * fsrx;
+    for (int k = 0; k < 3; ++k) {
+        const float dR_dr = (float)(r[k] * inv_mag + 0.5 * C.dgr[k] * g32);
+        o.f[k] = -(e12 * fsrd * dR_dr + de2 * (float)X.dgr[k] * fsrx);
+    }
+    const double E1 = (double)(e2m * ((float)nu * e1n2)) * (A.chi * A.chi * q.uu), E2 = de2, E12 = e12;
+    double TI[3], TJ[3], ti[3], tj[3];
+    for (int k = 0; k < 3; ++k) {
+        TI[k] = -(srx * (E1 * q.u2[k] + E2 * X.dgu1[k]) + E12 * (srd * 0.5 * C.dgu1[k] * g32));
+        TJ[k] = -(srx * (E1 * q.u1[k] + E2 * X.dgu2[k]) + E12 * (srd * 0.5 * C.dgu2[k] * g32));
+    }
+    cross3(q.u1, TI, ti);
+    cross3(q.u2, TJ, tj);
+    for (int k = 0; k < 3; ++k) { o.ti[k] = (float)ti[k]; o.tj[k] = (float)tj[k]; }
+    const double sw = 0.9 * rlim;
+    if (rij_mag >= sw) {
+        const float factor = __builtin_amdgcn_rcpf((float)(rlim - sw));
+        const float xx = (float)(rij_mag - sw) * factor;
+        const float sx = 1.0f - 6.0f * xx * xx * xx * xx * xx + 15.0f * xx * xx * xx * xx - 10.0f * xx * xx * xx;
+        const float sdx = -30.0f * xx * xx * xx * xx + 60.0f * xx * xx * xx - 30.0f * xx * xx;
+        for (int k = 0; k < 3; ++k) {
+            o.f[k] = sx * o.f[k] - sdx * pot * factor;
+            o.ti[k] = sx * o.ti[k] - sdx * pot * factor;
+            o.tj[k] = sx * o.tj[k] - sdx * pot * factor;
+        }
+        pot = sx * pot;
+    }
+    o.pe = pot;
+}
+
+// d_gb_tab<0> in the FP32 pair mode
+__device__ __forceinline__ void d_gb_tab_f32(const DState &s, int a, int b, const double *p, const double *tab,
+                                             PairOutT<float> &o)
+{
+    double r[3];
+    pair_rij(s, a, b, r);
+    const double *ta = tab + (long)GB_TAB * a, *tb = tab + (long)GB_TAB * b;
+    GbSelf A;
+    A.u[0] = ta[0]; A.u[1] = ta[1]; A.u[2] = ta[2];
+    A.chi = ta[3]; A.xhi = ta[4]; A.lprp = ta[5]; A.epsz = ta[6];
+    const double u2[3] = {tb[0], tb[1], tb[2]};
+    gb_core_f32(r, A, u2, p[0], (int)p[1], (int)p[2], o);
+}
+
+// ---------------------------------------------------------------------------
 // point dipoles (dipole_calculate_forces, force.f90:2687-2751): the body z
 // axis of each particle scaled by its charge; the reaction-field side effects
 // of the reference routine are applied by a separate kernel
@@ -522,4 +665,27 @@ __device__ __forceinline__ void d_pair(const DState &s, int a, int b, const doub
     else if (KIND == SX_GBWC) d_gb<4>(s, a, b, p, o);
     else if (KIND == SX_DIPOLE) d_dipole(s, a, b, p, o);
     else if (KIND == SX_COULOMB) d_coulomb(s, a, b, p, o);
+}
+
+// the isotropic kinds of the FP32 pair mode
+template <int KIND>
+constexpr bool sx_has_f32()
+{
+    return KIND == SX_LJ || KIND == SX_WCA || KIND == SX_DH;
+}
+
+// d_pair in the arithmetic type T (float: the kinds of sx_has_f32 only)
+template <int KIND, typename T>
+__device__ __forceinline__ void d_pair_t(const DState &s, int a, int b, const double *p, PairOutT<T> &o)
+{
+    if constexpr (sizeof(T) == sizeof(double)) {
+        d_pair<KIND>(s, a, b, p, o);
+    } else {
+        static_assert(sx_has_f32<KIND>(), "no FP32 form of this pair kind");
+        double r[3];
+        pair_rij(s, a, b, r);
+        if constexpr (KIND == SX_LJ) lj_core<float>(r, p, o);
+        else if constexpr (KIND == SX_WCA) wca_core<float>(r, p, o);
+        else dh_core<float>(r, s.f[SX_CHARGE][a], s.f[SX_CHARGE][b], p, o);
+    }
 }
