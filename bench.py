@@ -299,10 +299,14 @@ def other_workloads(sx, args):
         step = p["next_step"]
         d.run(sx.INT_NHT, gb, 100, p["dt"], first_step=step)
         d.synchronize()
+        step += 100
+        # the timed segment's starting point, so that the FP32 run can retrace it
+        start = d.state()
+        start_scal = {k: d.get_scalar(k) for k in sx.SCALARS}
         rb0 = d.rebuild_count()
         it0, calls0, fb0 = d.nht_stats()
         t0 = time.perf_counter()
-        d.run(sx.INT_NHT, gb, args.gb_steps, p["dt"], first_step=step + 100)
+        d.run(sx.INT_NHT, gb, args.gb_steps, p["dt"], first_step=step)
         d.synchronize()
         el = time.perf_counter() - t0
         it1, calls1, fb1 = d.nht_stats()
@@ -330,17 +334,32 @@ def other_workloads(sx, args):
             return d.get_scalar("POT_ENG")
 
         err = f32_error(d, snap, gb_forces)
-        step += 100 + args.gb_steps
+
+        def back_to_start():
+            d.load_arrays(start)
+            for k, v in start_scal.items():
+                d.set_scalar(k, v)
+            d.build_nlist()
+            d.synchronize()
+
+        # FP32: the same segment from the same state (after a warm-up run that
+        # captures the FP32 step graphs)
+        back_to_start()
         d.run(sx.INT_NHT, gb, 100, p["dt"], first_step=step)
-        d.synchronize()
+        back_to_start()
         rb0 = d.rebuild_count()
+        it0, calls0, fb0 = d.nht_stats()
         t0 = time.perf_counter()
-        d.run(sx.INT_NHT, gb, args.gb_steps, p["dt"], first_step=step + 100)
+        d.run(sx.INT_NHT, gb, args.gb_steps, p["dt"], first_step=step)
         d.synchronize()
         el = time.perf_counter() - t0
+        it1, calls1, fb1 = d.nht_stats()
         out["gb_nhc_8000_f32"] = f32_row(rec, p["n"], args.gb_steps, el, err, d.rebuild_count() - rb0,
-                                         {"T": d.get_scalar("TEMP")})
-        log(f"[gb f32] {el / args.gb_steps * 1e3:.4f} ms/step; error {err['force_elem_err']:.2e}")
+                                         {"T": d.get_scalar("TEMP"),
+                                          "chain_newton_updates_per_call": (it1 - it0) / max(1, calls1 - calls0),
+                                          "chain_serial_fallbacks": fb1 - fb0})
+        log(f"[gb f32] {el / args.gb_steps * 1e3:.4f} ms/step; error {err['force_elem_err']:.2e}; "
+            f"{(it1 - it0) / max(1, calls1 - calls0):.2f} Newton updates per chain call, {fb1 - fb0} serial fallbacks")
         d.close()
     if args.chain_steps > 0:
         d = sx.System(0)

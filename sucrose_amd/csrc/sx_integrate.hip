@@ -375,9 +375,9 @@ __global__ void __launch_bounds__(NHT_CHUNK) k_nht(DState s, double dt, int phas
 // results equal the serial recurrence to rounding (the particle updates are
 // the same expressions in the same order; only each chunk's entry state is
 // reached through the scan).  Each call is a fixed sequence of launches
-// (graph-capturable): prep, NHT_MAXIT passes that return at once after
-// convergence, final; without convergence the final kernel walks the
-// recurrence serially.
+// (graph-capturable): NHT_MAXIT passes that return at once after
+// convergence (the first also sets up the call), final; without convergence
+// the final kernel walks the recurrence serially.
 // ---------------------------------------------------------------------------
 struct D4 {
     double v, d[4];
@@ -486,23 +486,26 @@ __device__ __forceinline__ void nht_turn(int phase, double dt, double tq, double
 #define NHT_BLK 64       // chunks per workgroup (one wavefront), so at most 64 workgroups
 #define NHT_MAXIT 12     // passes per call (the first at the flat guess)
 #define NHT_TOL 1e-12
+#define NHT_FLOOR 1e-10
 struct AffMap {
     double M[16];   // row-major 4x4
     double c[4];
 };
 
-// (b after a): x -> Mb (Ma x + ca) + cb
+// (b after a): x -> Mb (Ma x + ca) + cb.  The models only steer the Newton
+// iterate, whose converged entry states the particle updates then start
+// from, so their sums use fused multiply-adds.
 __device__ __forceinline__ AffMap aff_compose(const AffMap &b, const AffMap &a)
 {
     AffMap r;
     for (int i = 0; i < 4; ++i) {
         for (int j = 0; j < 4; ++j) {
-            double acc = 0.0;
-            for (int k = 0; k < 4; ++k) acc += b.M[i * 4 + k] * a.M[k * 4 + j];
+            double acc = b.M[i * 4] * a.M[j];
+            for (int k = 1; k < 4; ++k) acc = fma(b.M[i * 4 + k], a.M[k * 4 + j], acc);
             r.M[i * 4 + j] = acc;
         }
         double acc = b.c[i];
-        for (int k = 0; k < 4; ++k) acc += b.M[i * 4 + k] * a.c[k];
+        for (int k = 0; k < 4; ++k) acc = fma(b.M[i * 4 + k], a.c[k], acc);
         r.c[i] = acc;
     }
     return r;
@@ -512,7 +515,7 @@ __device__ __forceinline__ void aff_apply(const AffMap &A, const double x[4], do
 {
     for (int r = 0; r < 4; ++r) {
         double v = A.c[r];
-        for (int k = 0; k < 4; ++k) v += A.M[r * 4 + k] * x[k];
+        for (int k = 0; k < 4; ++k) v = fma(A.M[r * 4 + k], x[k], v);
         y[r] = v;
     }
 }
@@ -594,54 +597,48 @@ __device__ __forceinline__ NhtArgs nht_args(const double *scal, double dt, doubl
 
 // converged by pass k: some earlier pass j >= 1 moved no entry state by more
 // than the tolerance (a pass that returned early leaves its slot at zero,
-// which only happens after an earlier one converged)
+// which only happens after an earlier one converged), or moved them by less
+// than NHT_FLOOR without shrinking to a quarter of the change before it: the
+// iteration converges quadratically until the rounding of the composed
+// models stops it, and where the chain amplifies that rounding the floor can
+// lie above NHT_TOL
 __device__ __forceinline__ bool nht_converged(const NhtBuf &B, int k)
 {
+    double prev = INFINITY;
     for (int j = 1; j < k; ++j) {
         unsigned long long u = __hip_atomic_load(&B.dev[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (__longlong_as_double((long long)u) < NHT_TOL) return true;
+        const double d = __longlong_as_double((long long)u);
+        if (d < NHT_TOL || (d < NHT_FLOOR && d > 0.25 * prev)) return true;
+        prev = d;
     }
     return false;
 }
 
 // the chain masses of the call: the coupling given, else n_dof T tau^2 for
 // the first link and T tau^2 for the others (tau = 2)
+__device__ __forceinline__ void nht_hb(double hb[3], double T, double ndof, double coupling, int has_coupling)
+{
+    if (has_coupling) {
+        hb[0] = hb[1] = hb[2] = coupling;
+    } else {
+        hb[0] = hb[1] = hb[2] = T * (2.0 * 2.0);
+        hb[0] = ndof * T * (2.0 * 2.0);
+    }
+}
+
 __device__ __forceinline__ void nht_set_hb(double *scal, double ndof, double coupling, int has_coupling)
 {
-    double T = scal[SX_TARGET_TEMP];
-    if (has_coupling) {
-        scal[SX_HB0] = scal[SX_HB1] = scal[SX_HB2] = coupling;
-    } else {
-        scal[SX_HB0] = scal[SX_HB1] = scal[SX_HB2] = T * (2.0 * 2.0);
-        scal[SX_HB0] = ndof * T * (2.0 * 2.0);
-    }
+    double hb[3];
+    nht_hb(hb, scal[SX_TARGET_TEMP], ndof, coupling, has_coupling);
+    scal[SX_HB0] = hb[0]; scal[SX_HB1] = hb[1]; scal[SX_HB2] = hb[2];
 }
 
-// sum of m |v|^2 before the call (each workgroup over its chunks' particles,
-// then the workgroup sums in a fixed order), the chain state entering the
-// call; clears the per-pass change slots
-__global__ void __launch_bounds__(NHT_BLK) k_nhtp_prep(DState s, double *scal, NhtBuf B, unsigned long long *stats,
-                                                       double ndof, double coupling, int has_coupling)
-{
-    const int b = blockIdx.x, t = threadIdx.x, n = s.n;
-    const int i0 = min(n, b * NHT_BLK * B.chunk), i1 = min(n, i0 + NHT_BLK * B.chunk);
-    double v[1] = {0.0};
-    for (int i = i0 + t; i < i1; i += NHT_BLK) {
-        double vx = s.f[SX_VX][i], vy = s.f[SX_VY][i], vz = s.f[SX_VZ][i];
-        v[0] += s.f[SX_MASS][i] * (vx * vx + vy * vy + vz * vz);
-    }
-    double tot[1];
-    if (!sx_block_reduce_last<1, NHT_BLK>(v, B.part, B.ticket, tot)) return;
-    if (t == 0) {
-        B.s0[0] = scal[SX_PETA0]; B.s0[1] = scal[SX_PETA1]; B.s0[2] = scal[SX_PETA2];
-        B.s0[3] = tot[0];
-        if (stats) atomicAdd(&stats[1], 1ull);
-        nht_set_hb(scal, ndof, coupling, has_coupling);   // read by the passes that follow
-    }
-    if (t < NHT_MAXIT) B.dev[t] = 0ull;
-}
-
-// pass k, one chunk per lane.  For k >= 1 the chunk's entry state is first
+// pass k, one chunk per lane.  Pass 0 linearises every chunk at a flat guess
+// (the chain momenta entering the call, sum m v^2 = n_dof T) and meanwhile
+// sets up the call: it clears the per-pass change slots, and its last
+// workgroup stores the state entering the call (the chain momenta and
+// sum m v^2, summed per workgroup in a fixed order) and the chain masses.
+// For k >= 1 the chunk's entry state is first
 // advanced to the Newton iterate given by the chunk models of pass k - 1:
 // the workgroup totals are composed by a shuffle scan (every workgroup
 // redundantly), the result applied to the state entering the call gives this
@@ -652,16 +649,38 @@ __global__ void __launch_bounds__(NHT_BLK) k_nhtp_prep(DState s, double *scal, N
 // never converges).  Then a dual-number walk over the chunk's particles gives
 // its affine model x -> J x + (E - J g), and a shuffle scan over the
 // workgroup stores each chunk's prefix and the workgroup total.
-__global__ void __launch_bounds__(NHT_BLK) k_nhtp_pass(DState s, double dt, int phase, const double *scal,
-                                                       double ndof, NhtBuf B, int k, unsigned long long *stats)
+__global__ void __launch_bounds__(NHT_BLK) k_nhtp_pass(DState s, double dt, int phase, double *scal, double ndof,
+                                                       NhtBuf B, int k, unsigned long long *stats, double coupling,
+                                                       int has_coupling)
 {
     if (nht_converged(B, k)) return;
     const int b = blockIdx.x, lane = threadIdx.x, c = b * NHT_BLK + lane;
-    const double s0[4] = {B.s0[0], B.s0[1], B.s0[2], B.s0[3]};
+    NhtArgs a = nht_args(scal, dt, ndof);
+    int i0, i1;
+    nht_chunk(s.n, B.chunk, c, i0, i1);
     double g[4];
     if (k == 0) {
-        for (int q = 0; q < 4; ++q) g[q] = s0[q];
+        double hb[3] = {0.0, 0.0, 0.0};
+        nht_hb(hb, a.Temp, ndof, coupling, has_coupling);
+        for (int q = 0; q < 3; ++q) a.hb[q] = hb[q];
+        g[0] = scal[SX_PETA0]; g[1] = scal[SX_PETA1]; g[2] = scal[SX_PETA2];
+        g[3] = a.ndofT;
+        for (int q = 0; q < 4; ++q) B.g[c * 4 + q] = g[q];
+        if (b == 0 && lane > 0 && lane < NHT_MAXIT) B.dev[lane] = 0ull;
+        double v2[1] = {0.0};
+        for (int i = i0; i < i1; ++i) {
+            const double vx = s.f[SX_VX][i], vy = s.f[SX_VY][i], vz = s.f[SX_VZ][i];
+            v2[0] += s.f[SX_MASS][i] * (vx * vx + vy * vy + vz * vz);
+        }
+        double tot[1];
+        if (sx_block_reduce_last<1, NHT_BLK>(v2, B.part, B.ticket, tot) && lane == 0) {
+            B.s0[0] = g[0]; B.s0[1] = g[1]; B.s0[2] = g[2];
+            B.s0[3] = tot[0];
+            scal[SX_HB0] = hb[0]; scal[SX_HB1] = hb[1]; scal[SX_HB2] = hb[2];
+            if (stats) atomicAdd(&stats[1], 1ull);
+        }
     } else {
+        const double s0[4] = {B.s0[0], B.s0[1], B.s0[2], B.s0[3]};
         const int pk = (k - 1) & 1;
         AffMap x = lane < B.nb ? aff_load(B.T[pk] + (size_t)lane * 20) : aff_identity();
         x = aff_wave_scan(x, lane);
@@ -679,7 +698,7 @@ __global__ void __launch_bounds__(NHT_BLK) k_nhtp_pass(DState s, double dt, int 
                                  sqrt(fabs(scal[SX_HB2] * T)), fabs(ndof * T)};
         double dv = 0.0;
         for (int q = 0; q < 4; ++q) {
-            double old = (k == 1) ? s0[q] : B.g[c * 4 + q];
+            double old = B.g[c * 4 + q];
             double d = fabs(g[q] - old) / (fabs(g[q]) + scale[q] + 1e-300);
             dv = (d == d && dv == dv) ? fmax(dv, d) : INFINITY;
             B.g[c * 4 + q] = g[q];
@@ -692,10 +711,20 @@ __global__ void __launch_bounds__(NHT_BLK) k_nhtp_pass(DState s, double dt, int 
             atomicMax(&B.dev[k], (unsigned long long)__double_as_longlong(dv));
             if (b == 0 && stats) atomicAdd(&stats[0], 1ull);
         }
+        if (dv < NHT_TOL) {
+            // no entry state of this workgroup moved: the models of pass k - 1,
+            // linearised at the same states, stand for this pass's
+            const double *Pp = B.P[pk] + (size_t)c * 20;
+            double *Pn = B.P[k & 1] + (size_t)c * 20;
+            for (int q = 0; q < 20; ++q) Pn[q] = Pp[q];
+            if (lane == NHT_BLK - 1) {
+                const double *Tp = B.T[pk] + (size_t)b * 20;
+                double *Tn = B.T[k & 1] + (size_t)b * 20;
+                for (int q = 0; q < 20; ++q) Tn[q] = Tp[q];
+            }
+            return;
+        }
     }
-    const NhtArgs a = nht_args(scal, dt, ndof);
-    int i0, i1;
-    nht_chunk(s.n, B.chunk, c, i0, i1);
     D4 p[3], sv;
     for (int q = 0; q < 3; ++q) { p[q] = dd(g[q]); p[q].d[q] = 1.0; }
     sv = dd(g[3]);
@@ -1249,7 +1278,7 @@ void sx_nht_impl(sx_ctx *c, double dt, const double *coupling, int phase, bool s
 {
     double ndof = (double)(3 * (c->n / c->nrigid) - 3);
     // the serial walk for small systems, the parallel (DEER) evaluation above
-    // a few thousand particles (whose prep kernel also sets the chain masses)
+    // a few thousand particles (whose first pass also sets the chain masses)
     if (c->n < 2048) {
         hipLaunchKernelGGL(k_nht_hb, dim3(1), dim3(64), 0, c->stream, c->d_scal, ndof, coupling ? *coupling : 0.0,
                            coupling ? 1 : 0);
@@ -1280,11 +1309,9 @@ void sx_nht_impl(sx_ctx *c, double dt, const double *coupling, int phase, bool s
         B.nb = (nchunks + NHT_BLK - 1) / NHT_BLK;
         unsigned long long *st = (unsigned long long *)(c->d_counts + 4);
         DState s = c->dstate();
-        hipLaunchKernelGGL(k_nhtp_prep, dim3(B.nb), dim3(NHT_BLK), 0, c->stream, s, c->d_scal, B, st, ndof,
-                           coupling ? *coupling : 0.0, coupling ? 1 : 0);
         for (int k = 0; k < NHT_MAXIT; ++k)
             hipLaunchKernelGGL(k_nhtp_pass, dim3(B.nb), dim3(NHT_BLK), 0, c->stream, s, dt, phase, c->d_scal, ndof,
-                               B, k, st);
+                               B, k, st, coupling ? *coupling : 0.0, coupling ? 1 : 0);
         hipLaunchKernelGGL(k_nhtp_final, dim3(B.nb), dim3(NHT_BLK), 0, c->stream, s, dt, phase, save ? 1 : 0,
                            c->d_scal, ndof, B, st);
     }
