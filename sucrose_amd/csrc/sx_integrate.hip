@@ -135,33 +135,26 @@ __global__ void k_lgv_initial(DState s, LgvConst k, int step, const int *dstep)
 // the last workgroup to finish) and initialize_forces (system.f90:423-436:
 // each particle's F, T and energy slot after its half kick has read F); with
 // PP also the ECP per-particle records of the new orientations.
-template <bool PP, int BS>
-__global__ void __launch_bounds__(BS) k_lgv_step_begin(DState s, LgvConst k, const int *dstep, const double *xl,
-                                                        const double *yl, const double *zl, double *part,
-                                                        unsigned *ticket, double thresh, int rule, int *flag,
-                                                        double *scal, double *pe_acc, double *pp, float *scr,
-                                                        double *pos4)
+// the particles' moves since the list was built, |mimg(r - r_last)|^2
+__device__ __forceinline__ double moved_sq(const DState &s, int i, const double *xl, const double *yl,
+                                           const double *zl)
+{
+    double dx = s.f[SX_RX][i] - xl[i], dy = s.f[SX_RY][i] - yl[i], dz = s.f[SX_RZ][i] - zl[i];
+    if (s.periodic) {
+        dx = sx_mimg(dx, s.box[0]); dy = sx_mimg(dy, s.box[1]); dz = sx_mimg(dz, s.box[2]);
+    }
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// needs_rebuild (system.f90:1005-1039) from each thread's move m: per-block
+// maxima, the last block to finish takes the maximum over blocks and writes
+// the rebuild flag (max is order-free); every thread of the block calls it
+template <int BS>
+__device__ __forceinline__ void rebuild_flag_from(double m, double *part, unsigned *ticket, double thresh, int rule,
+                                                  int *flag)
 {
     __shared__ double sh[BS];
     __shared__ bool last;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double m = 0.0;
-    if (i == 0) {
-        scal[SX_POT_ENG] = 0.0; scal[SX_KIN_ENG] = 0.0; scal[SX_KIN_ROT] = 0.0; scal[SX_TEMP] = 0.0;
-        scal[SX_PRESSURE] = 0.0; scal[SX_VIRIAL] = 0.0; scal[SX_VIRIAL_BONDED] = 0.0;
-    }
-    if (i < s.n) {
-        lgv_initial_particle(s, i, i + 1, k, *dstep);
-        double dx = s.f[SX_RX][i] - xl[i], dy = s.f[SX_RY][i] - yl[i], dz = s.f[SX_RZ][i] - zl[i];
-        if (s.periodic) {
-            dx = sx_mimg(dx, s.box[0]); dy = sx_mimg(dy, s.box[1]); dz = sx_mimg(dz, s.box[2]);
-        }
-        m = dx * dx + dy * dy + dz * dz;
-        s.f[SX_FX][i] = 0.0; s.f[SX_FY][i] = 0.0; s.f[SX_FZ][i] = 0.0;
-        s.f[SX_TX][i] = 0.0; s.f[SX_TY][i] = 0.0; s.f[SX_TZ][i] = 0.0;
-        pe_acc[i] = 0.0;
-        if (PP) ecp_pp_particle(s, i, pp, scr, pos4);
-    }
     sh[threadIdx.x] = m;
     __syncthreads();
     for (int o = BS / 2; o > 0; o >>= 1) {
@@ -190,6 +183,58 @@ __global__ void __launch_bounds__(BS) k_lgv_step_begin(DState s, LgvConst k, con
         flag[0] = rule == SX_REBUILD_REFERENCE ? (4.0 * mx > thresh) : (mx > thresh);
         *ticket = 0u;
     }
+}
+
+template <bool PP, int BS>
+__global__ void __launch_bounds__(BS) k_lgv_step_begin(DState s, LgvConst k, const int *dstep, const double *xl,
+                                                        const double *yl, const double *zl, double *part,
+                                                        unsigned *ticket, double thresh, int rule, int *flag,
+                                                        double *scal, double *pe_acc, double *pp, float *scr,
+                                                        double *pos4)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if (i == 0) {
+        scal[SX_POT_ENG] = 0.0; scal[SX_KIN_ENG] = 0.0; scal[SX_KIN_ROT] = 0.0; scal[SX_TEMP] = 0.0;
+        scal[SX_PRESSURE] = 0.0; scal[SX_VIRIAL] = 0.0; scal[SX_VIRIAL_BONDED] = 0.0;
+    }
+    if (i < s.n) {
+        lgv_initial_particle(s, i, i + 1, k, *dstep);
+        m = moved_sq(s, i, xl, yl, zl);
+        s.f[SX_FX][i] = 0.0; s.f[SX_FY][i] = 0.0; s.f[SX_FZ][i] = 0.0;
+        s.f[SX_TX][i] = 0.0; s.f[SX_TY][i] = 0.0; s.f[SX_TZ][i] = 0.0;
+        pe_acc[i] = 0.0;
+        if (PP) ecp_pp_particle(s, i, pp, scr, pos4);
+    }
+    rebuild_flag_from<BS>(m, part, ticket, thresh, rule, flag);
+}
+
+// qq_initial_step (integrators.f90:123-171) of each particle, the rebuild
+// decision and initialize_forces in one launch: the Nose-Hoover step of the
+// step driver, whose chain kernels have already kicked and moved the
+// particles (F and T are cleared after the angular kick has read T)
+__global__ void __launch_bounds__(256) k_qq_step_begin(DState s, double dt, const double *xl, const double *yl,
+                                                       const double *zl, double *part, unsigned *ticket,
+                                                       double thresh, int rule, int *flag, double *scal,
+                                                       double *pe_acc)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if (i == 0) {
+        scal[SX_POT_ENG] = 0.0; scal[SX_KIN_ENG] = 0.0; scal[SX_KIN_ROT] = 0.0; scal[SX_TEMP] = 0.0;
+        scal[SX_PRESSURE] = 0.0; scal[SX_VIRIAL] = 0.0; scal[SX_VIRIAL_BONDED] = 0.0;
+    }
+    if (i < s.n) {
+        ang_kick(s, i, dt);
+        double qdw, qdx, qdy, qdz, RM[9], bx, by, bz;
+        quat_midpoint(s, i, dt, qdw, qdx, qdy, qdz, RM, bx, by, bz, true);
+        quat_advance(s, i, dt, qdw, qdx, qdy, qdz);
+        m = moved_sq(s, i, xl, yl, zl);
+        s.f[SX_FX][i] = 0.0; s.f[SX_FY][i] = 0.0; s.f[SX_FZ][i] = 0.0;
+        s.f[SX_TX][i] = 0.0; s.f[SX_TY][i] = 0.0; s.f[SX_TZ][i] = 0.0;
+        pe_acc[i] = 0.0;
+    }
+    rebuild_flag_from<256>(m, part, ticket, thresh, rule, flag);
 }
 
 template <int BS>
@@ -379,50 +424,35 @@ __global__ void __launch_bounds__(NHT_CHUNK) k_nht(DState s, double dt, int phas
 // convergence (the first also sets up the call), final; without convergence
 // the final kernel walks the recurrence serially.
 // ---------------------------------------------------------------------------
-struct D4 {
-    double v, d[4];
-};
-__device__ __forceinline__ D4 dd(double c) { return D4{c, {0, 0, 0, 0}}; }
-__device__ __forceinline__ D4 operator+(const D4 &a, const D4 &b)
-{
-    return D4{a.v + b.v, {a.d[0] + b.d[0], a.d[1] + b.d[1], a.d[2] + b.d[2], a.d[3] + b.d[3]}};
-}
-__device__ __forceinline__ D4 operator-(const D4 &a, const D4 &b)
-{
-    return D4{a.v - b.v, {a.d[0] - b.d[0], a.d[1] - b.d[1], a.d[2] - b.d[2], a.d[3] - b.d[3]}};
-}
-__device__ __forceinline__ D4 operator*(const D4 &a, const D4 &b)
-{
-    D4 r;
-    r.v = a.v * b.v;
-    for (int k = 0; k < 4; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
-    return r;
-}
-__device__ __forceinline__ D4 operator/(const D4 &a, const D4 &b)
-{
-    D4 r;
-    r.v = a.v / b.v;
-    double ib = 1.0 / b.v;
-    for (int k = 0; k < 4; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) * ib;
-    return r;
-}
-__device__ __forceinline__ D4 operator+(const D4 &a, double b) { return a + dd(b); }
-__device__ __forceinline__ D4 operator-(const D4 &a, double b) { return a - dd(b); }
-__device__ __forceinline__ D4 operator*(const D4 &a, double b)
-{
-    return D4{a.v * b, {a.d[0] * b, a.d[1] * b, a.d[2] * b, a.d[3] * b}};
-}
-__device__ __forceinline__ D4 operator/(const D4 &a, double b) { return a / dd(b); }
-__device__ __forceinline__ D4 operator*(double a, const D4 &b) { return b * a; }
-__device__ __forceinline__ D4 operator-(double a, const D4 &b) { return dd(a) - b; }
-__device__ __forceinline__ D4 operator-(const D4 &a) { return dd(0.0) - a; }
 __device__ __forceinline__ double val(double x) { return x; }
-__device__ __forceinline__ double val(const D4 &x) { return x.v; }
 __device__ __forceinline__ double texp(double x) { return exp(x); }
-__device__ __forceinline__ D4 texp(const D4 &x)
+
+// value and one directional derivative
+struct D1 {
+    double v, d;
+};
+__device__ __forceinline__ D1 operator+(const D1 &a, const D1 &b) { return D1{a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ D1 operator-(const D1 &a, const D1 &b) { return D1{a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ D1 operator*(const D1 &a, const D1 &b) { return D1{a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ D1 operator/(const D1 &a, const D1 &b)
+{
+    D1 r;
+    r.v = a.v / b.v;
+    r.d = (a.d - r.v * b.d) * (1.0 / b.v);
+    return r;
+}
+__device__ __forceinline__ D1 operator+(const D1 &a, double b) { return D1{a.v + b, a.d}; }
+__device__ __forceinline__ D1 operator-(const D1 &a, double b) { return D1{a.v - b, a.d}; }
+__device__ __forceinline__ D1 operator*(const D1 &a, double b) { return D1{a.v * b, a.d * b}; }
+__device__ __forceinline__ D1 operator/(const D1 &a, double b) { return a / D1{b, 0.0}; }
+__device__ __forceinline__ D1 operator*(double a, const D1 &b) { return b * a; }
+__device__ __forceinline__ D1 operator-(double a, const D1 &b) { return D1{a - b.v, -b.d}; }
+__device__ __forceinline__ D1 operator-(const D1 &a) { return D1{0.0 - a.v, 0.0 - a.d}; }
+__device__ __forceinline__ double val(const D1 &x) { return x.v; }
+__device__ __forceinline__ D1 texp(const D1 &x)
 {
     double e = exp(x.v);
-    return D4{e, {e * x.d[0], e * x.d[1], e * x.d[2], e * x.d[3]}};
+    return D1{e, e * x.d};
 }
 
 // exprel and u4 of the serial kernel above, for doubles or dual numbers
@@ -483,7 +513,8 @@ __device__ __forceinline__ void nht_turn(int phase, double dt, double tq, double
 }
 
 #define NHT_CMAX 4096    // most chunks per call
-#define NHT_BLK 64       // chunks per workgroup (one wavefront), so at most 64 workgroups
+#define NHT_BLK 64       // chunks per workgroup, so at most 64 workgroups
+#define NHT_DIRS 4       // wavefronts per workgroup in a pass: one per derivative direction
 #define NHT_MAXIT 12     // passes per call (the first at the flat guess)
 #define NHT_TOL 1e-12
 #define NHT_FLOOR 1e-10
@@ -646,15 +677,21 @@ __device__ __forceinline__ void nht_set_hb(double *scal, double ndof, double cou
 // chunk's; the largest scaled change of an entry state over the grid goes to
 // the pass's slot (converged when below 1e-12 of the magnitude plus the
 // thermal scale, sqrt(Q_k T) for p_eta(k) and n_dof T for sum m v^2; a NaN
-// never converges).  Then a dual-number walk over the chunk's particles gives
-// its affine model x -> J x + (E - J g), and a shuffle scan over the
-// workgroup stores each chunk's prefix and the workgroup total.
-__global__ void __launch_bounds__(NHT_BLK) k_nhtp_pass(DState s, double dt, int phase, double *scal, double ndof,
-                                                       NhtBuf B, int k, unsigned long long *stats, double coupling,
-                                                       int has_coupling)
+// never converges).  Then four wavefronts walk the chunk's particles with
+// dual numbers, each carrying the derivative along one component of the
+// entry state, which gives the chunk's affine model x -> J x + (E - J g); a
+// shuffle scan over the workgroup's chunks stores each chunk's prefix and the
+// workgroup total.
+__global__ void __launch_bounds__(NHT_BLK * NHT_DIRS) k_nhtp_pass(DState s, double dt, int phase, double *scal,
+                                                                  double ndof, NhtBuf B, int k,
+                                                                  unsigned long long *stats, double coupling,
+                                                                  int has_coupling)
 {
+    __shared__ double gsh[NHT_BLK][4], col[NHT_DIRS][NHT_BLK][4], val[NHT_BLK][4];
+    __shared__ int skip;
     if (nht_converged(B, k)) return;
-    const int b = blockIdx.x, lane = threadIdx.x, c = b * NHT_BLK + lane;
+    const int b = blockIdx.x, lane = threadIdx.x % NHT_BLK, dir = threadIdx.x / NHT_BLK;
+    const int c = b * NHT_BLK + lane;
     NhtArgs a = nht_args(scal, dt, ndof);
     int i0, i1;
     nht_chunk(s.n, B.chunk, c, i0, i1);
@@ -665,85 +702,99 @@ __global__ void __launch_bounds__(NHT_BLK) k_nhtp_pass(DState s, double dt, int 
         for (int q = 0; q < 3; ++q) a.hb[q] = hb[q];
         g[0] = scal[SX_PETA0]; g[1] = scal[SX_PETA1]; g[2] = scal[SX_PETA2];
         g[3] = a.ndofT;
-        for (int q = 0; q < 4; ++q) B.g[c * 4 + q] = g[q];
-        if (b == 0 && lane > 0 && lane < NHT_MAXIT) B.dev[lane] = 0ull;
         double v2[1] = {0.0};
-        for (int i = i0; i < i1; ++i) {
-            const double vx = s.f[SX_VX][i], vy = s.f[SX_VY][i], vz = s.f[SX_VZ][i];
-            v2[0] += s.f[SX_MASS][i] * (vx * vx + vy * vy + vz * vz);
+        if (dir == 0) {
+            for (int q = 0; q < 4; ++q) B.g[c * 4 + q] = g[q];
+            if (b == 0 && lane > 0 && lane < NHT_MAXIT) B.dev[lane] = 0ull;
+            for (int i = i0; i < i1; ++i) {
+                const double vx = s.f[SX_VX][i], vy = s.f[SX_VY][i], vz = s.f[SX_VZ][i];
+                v2[0] += s.f[SX_MASS][i] * (vx * vx + vy * vy + vz * vz);
+            }
         }
         double tot[1];
-        if (sx_block_reduce_last<1, NHT_BLK>(v2, B.part, B.ticket, tot) && lane == 0) {
+        if (sx_block_reduce_last<1, NHT_BLK * NHT_DIRS>(v2, B.part, B.ticket, tot) && threadIdx.x == 0) {
             B.s0[0] = g[0]; B.s0[1] = g[1]; B.s0[2] = g[2];
             B.s0[3] = tot[0];
             scal[SX_HB0] = hb[0]; scal[SX_HB1] = hb[1]; scal[SX_HB2] = hb[2];
             if (stats) atomicAdd(&stats[1], 1ull);
         }
     } else {
-        const double s0[4] = {B.s0[0], B.s0[1], B.s0[2], B.s0[3]};
         const int pk = (k - 1) & 1;
-        AffMap x = lane < B.nb ? aff_load(B.T[pk] + (size_t)lane * 20) : aff_identity();
-        x = aff_wave_scan(x, lane);
-        double E[4];
-        if (b == 0) {
-            for (int q = 0; q < 4; ++q) E[q] = s0[q];
-        } else {
-            AffMap pre = aff_shfl(x, b - 1);
-            aff_apply(pre, s0, E);
-        }
-        AffMap Pc = aff_load(B.P[pk] + (size_t)c * 20);
-        aff_apply(Pc, E, g);
-        const double T = scal[SX_TARGET_TEMP];
-        const double scale[4] = {sqrt(fabs(scal[SX_HB0] * T)), sqrt(fabs(scal[SX_HB1] * T)),
-                                 sqrt(fabs(scal[SX_HB2] * T)), fabs(ndof * T)};
-        double dv = 0.0;
-        for (int q = 0; q < 4; ++q) {
-            double old = B.g[c * 4 + q];
-            double d = fabs(g[q] - old) / (fabs(g[q]) + scale[q] + 1e-300);
-            dv = (d == d && dv == dv) ? fmax(dv, d) : INFINITY;
-            B.g[c * 4 + q] = g[q];
-        }
-        for (int o = SX_WAVE / 2; o > 0; o >>= 1) {
-            double y = __shfl_xor(dv, o, SX_WAVE);
-            dv = (y == y && dv == dv) ? fmax(dv, y) : INFINITY;
-        }
-        if (lane == 0) {
-            atomicMax(&B.dev[k], (unsigned long long)__double_as_longlong(dv));
-            if (b == 0 && stats) atomicAdd(&stats[0], 1ull);
-        }
-        if (dv < NHT_TOL) {
-            // no entry state of this workgroup moved: the models of pass k - 1,
-            // linearised at the same states, stand for this pass's
-            const double *Pp = B.P[pk] + (size_t)c * 20;
-            double *Pn = B.P[k & 1] + (size_t)c * 20;
-            for (int q = 0; q < 20; ++q) Pn[q] = Pp[q];
-            if (lane == NHT_BLK - 1) {
-                const double *Tp = B.T[pk] + (size_t)b * 20;
-                double *Tn = B.T[k & 1] + (size_t)b * 20;
-                for (int q = 0; q < 20; ++q) Tn[q] = Tp[q];
+        if (dir == 0) {
+            const double s0[4] = {B.s0[0], B.s0[1], B.s0[2], B.s0[3]};
+            AffMap x = lane < B.nb ? aff_load(B.T[pk] + (size_t)lane * 20) : aff_identity();
+            x = aff_wave_scan(x, lane);
+            double E[4];
+            if (b == 0) {
+                for (int q = 0; q < 4; ++q) E[q] = s0[q];
+            } else {
+                AffMap pre = aff_shfl(x, b - 1);
+                aff_apply(pre, s0, E);
             }
-            return;
+            AffMap Pc = aff_load(B.P[pk] + (size_t)c * 20);
+            aff_apply(Pc, E, g);
+            const double T = scal[SX_TARGET_TEMP];
+            const double scale[4] = {sqrt(fabs(scal[SX_HB0] * T)), sqrt(fabs(scal[SX_HB1] * T)),
+                                     sqrt(fabs(scal[SX_HB2] * T)), fabs(ndof * T)};
+            double dv = 0.0;
+            for (int q = 0; q < 4; ++q) {
+                double old = B.g[c * 4 + q];
+                double d = fabs(g[q] - old) / (fabs(g[q]) + scale[q] + 1e-300);
+                dv = (d == d && dv == dv) ? fmax(dv, d) : INFINITY;
+                B.g[c * 4 + q] = g[q];
+                gsh[lane][q] = g[q];
+            }
+            for (int o = SX_WAVE / 2; o > 0; o >>= 1) {
+                double y = __shfl_xor(dv, o, SX_WAVE);
+                dv = (y == y && dv == dv) ? fmax(dv, y) : INFINITY;
+            }
+            if (lane == 0) {
+                atomicMax(&B.dev[k], (unsigned long long)__double_as_longlong(dv));
+                if (b == 0 && stats) atomicAdd(&stats[0], 1ull);
+                skip = dv < NHT_TOL;
+            }
+            if (dv < NHT_TOL) {
+                // no entry state of this workgroup moved: the models of pass
+                // k - 1, linearised at the same states, stand for this pass's
+                const double *Pp = B.P[pk] + (size_t)c * 20;
+                double *Pn = B.P[k & 1] + (size_t)c * 20;
+                for (int q = 0; q < 20; ++q) Pn[q] = Pp[q];
+                if (lane == NHT_BLK - 1) {
+                    const double *Tp = B.T[pk] + (size_t)b * 20;
+                    double *Tn = B.T[k & 1] + (size_t)b * 20;
+                    for (int q = 0; q < 20; ++q) Tn[q] = Tp[q];
+                }
+            }
         }
+        __syncthreads();
+        if (skip) return;
+        for (int q = 0; q < 4; ++q) g[q] = gsh[lane][q];
     }
-    D4 p[3], sv;
-    for (int q = 0; q < 3; ++q) { p[q] = dd(g[q]); p[q].d[q] = 1.0; }
-    sv = dd(g[3]);
-    sv.d[3] = 1.0;
+    // the walk: wavefront dir carries the derivative along entry component dir
+    D1 p[3], sv;
+    for (int q = 0; q < 3; ++q) p[q] = D1{g[q], q == dir ? 1.0 : 0.0};
+    sv = D1{g[3], dir == 3 ? 1.0 : 0.0};
     for (int i = i0; i < i1; ++i) {
         double m = s.f[SX_MASS][i], im = s.f[SX_INV_MASS][i];
         double acc3[3] = {0.5 * dt * (s.f[SX_FX][i] * im), 0.5 * dt * (s.f[SX_FY][i] * im),
                           0.5 * dt * (s.f[SX_FZ][i] * im)};
-        D4 v[3] = {dd(s.f[SX_VX][i]), dd(s.f[SX_VY][i]), dd(s.f[SX_VZ][i])};
-        D4 ea[3];
+        D1 v[3] = {D1{s.f[SX_VX][i], 0.0}, D1{s.f[SX_VY][i], 0.0}, D1{s.f[SX_VZ][i], 0.0}};
+        D1 ea[3];
         nht_turn(phase, dt, a.tq, a.ndofT, a.Temp, a.hb, m, acc3, p, sv, v, ea);
     }
-    const D4 *out[4] = {&p[0], &p[1], &p[2], &sv};
+    const D1 out[4] = {p[0], p[1], p[2], sv};
+    for (int r = 0; r < 4; ++r) {
+        col[dir][lane][r] = out[r].d;
+        if (dir == 0) val[lane][r] = out[r].v;
+    }
+    __syncthreads();
+    if (dir != 0) return;
     AffMap A;
     for (int r = 0; r < 4; ++r) {
-        double cr = out[r]->v;
+        double cr = val[lane][r];
         for (int q = 0; q < 4; ++q) {
-            A.M[r * 4 + q] = out[r]->d[q];
-            cr -= out[r]->d[q] * g[q];
+            A.M[r * 4 + q] = col[q][lane][r];
+            cr -= col[q][lane][r] * g[q];
         }
         A.c[r] = cr;
     }
@@ -1206,6 +1257,16 @@ void sx_qq_initial_impl(sx_ctx *c, double dt)
     hipLaunchKernelGGL(k_qq_initial, dim3(sx_blocks(c->n, 256)), dim3(256), 0, c->stream, c->dstate(), dt);
 }
 
+void sx_qq_step_begin_impl(sx_ctx *c, double dt)
+{
+    NList &nl = c->nl;
+    int nb = sx_blocks(c->n, 256);
+    sx_ensure_part(c, nb, 1);
+    double thresh = nl.rule == SX_REBUILD_REFERENCE ? nl.r_skin_sq : (nl.skin / 2.0) * (nl.skin / 2.0);
+    hipLaunchKernelGGL(k_qq_step_begin, dim3(nb), dim3(256), 0, c->stream, c->dstate(), dt, nl.x_last, nl.y_last,
+                       nl.z_last, c->d_part, c->d_ticket, thresh, nl.rule, c->d_flag, c->d_scal, c->d_pe_acc);
+}
+
 void sx_qq_final_impl(sx_ctx *c, double dt, bool thermo)
 {
     int nb = sx_blocks(c->n, 256);
@@ -1310,7 +1371,7 @@ void sx_nht_impl(sx_ctx *c, double dt, const double *coupling, int phase, bool s
         unsigned long long *st = (unsigned long long *)(c->d_counts + 4);
         DState s = c->dstate();
         for (int k = 0; k < NHT_MAXIT; ++k)
-            hipLaunchKernelGGL(k_nhtp_pass, dim3(B.nb), dim3(NHT_BLK), 0, c->stream, s, dt, phase, c->d_scal, ndof,
+            hipLaunchKernelGGL(k_nhtp_pass, dim3(B.nb), dim3(NHT_BLK * NHT_DIRS), 0, c->stream, s, dt, phase, c->d_scal, ndof,
                                B, k, st, coupling ? *coupling : 0.0, coupling ? 1 : 0);
         hipLaunchKernelGGL(k_nhtp_final, dim3(B.nb), dim3(NHT_BLK), 0, c->stream, s, dt, phase, save ? 1 : 0,
                            c->d_scal, ndof, B, st);

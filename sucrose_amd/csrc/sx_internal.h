@@ -253,6 +253,7 @@ void sx_lgv_final_impl(sx_ctx *c, double dt, double damping, bool constrained, b
 void sx_vv_initial_impl(sx_ctx *c, double dt, bool save);
 void sx_vv_final_impl(sx_ctx *c, double dt, bool thermo = false);
 void sx_qq_initial_impl(sx_ctx *c, double dt);
+void sx_qq_step_begin_impl(sx_ctx *c, double dt);   // qq_initial + rebuild flag + initialize_forces
 void sx_qq_final_impl(sx_ctx *c, double dt, bool thermo = false);
 void sx_nht_impl(sx_ctx *c, double dt, const double *coupling, int phase, bool save);
 void sx_rattle_impl(sx_ctx *c, double dt, int phase);

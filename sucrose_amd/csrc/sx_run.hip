@@ -53,19 +53,25 @@ static void enqueue_step(sx_ctx *c, const sx_step_spec &sp)
         c->ecp_pp_by_step = ecp;
         sx_nlist_build_impl(c, true);
     } else {
+        // without composite bodies the Nose-Hoover step's quaternion half,
+        // the rebuild decision and initialize_forces are one kernel
+        const bool fused = sp.integrator == SX_INT_NHT && c->comp.count == 0;
         switch (sp.integrator) {
         case SX_INT_VV: sx_vv_initial_impl(c, sp.dt, false); break;
         case SX_INT_NHT:
             sx_nht_impl(c, sp.dt, nullptr, 0, false);
-            sx_qq_initial_impl(c, sp.dt);
+            if (fused) sx_qq_step_begin_impl(c, sp.dt);
+            else sx_qq_initial_impl(c, sp.dt);
             break;
         case SX_INT_LGV: sx_lgv_initial_impl(c, sp.dt, 0, c->d_step, sp.damping, false); break;
         case SX_INT_LGVC: sx_lgv_initial_impl(c, sp.dt, 0, c->d_step, sp.damping, true); break;
         }
-        sx_composite_place_impl(c);   // satellites follow their centres
-        sx_nlist_flag_impl(c);
+        if (!fused) {
+            sx_composite_place_impl(c);   // satellites follow their centres
+            sx_nlist_flag_impl(c);
+        }
         sx_nlist_build_impl(c, true);
-        sx_init_forces_impl(c);
+        if (!fused) sx_init_forces_impl(c);
     }
     bool dipoles = false;
     const bool two = sp.nforces == 2 &&
