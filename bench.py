@@ -317,7 +317,8 @@ def other_workloads(sx, args):
                "chain_serial_fallbacks": fb1 - fb0,
                "config": {"workload": "gb_nhc_8000", "n": p["n"], "rho": p["rho"], "gb": "GB(3,5,2,1)",
                           "sig": [1, 1, 3], "eps": [5, 5, 1], "T": p["T"], "dt": p["dt"], "cutoff": p["cutoff"],
-                          "skin": p["skin"], "integrator": "nht (chain 3, tau 2) + quaternion", "dtype": "f64"}}
+                          "skin": p["skin"], "list_radius": p["cutoff"] + p["skin"],
+                          "integrator": "nht (chain 3, tau 2) + quaternion", "dtype": "f64"}}
         log(f"[gb] {args.gb_steps} steps in {el:.3f}s -> {rec['value']:.4e} particle-steps/s; "
             f"{rec['chain_newton_updates_per_call']:.2f} Newton updates per chain call, {fb1 - fb0} serial fallbacks")
         snap = d.state()
@@ -342,24 +343,43 @@ def other_workloads(sx, args):
             d.build_nlist()
             d.synchronize()
 
-        # FP32: the same segment from the same state (after a warm-up run that
-        # captures the FP32 step graphs)
-        back_to_start()
-        d.run(sx.INT_NHT, gb, 100, p["dt"], first_step=step)
-        back_to_start()
-        rb0 = d.rebuild_count()
-        it0, calls0, fb0 = d.nht_stats()
-        t0 = time.perf_counter()
-        d.run(sx.INT_NHT, gb, args.gb_steps, p["dt"], first_step=step)
-        d.synchronize()
-        el = time.perf_counter() - t0
-        it1, calls1, fb1 = d.nht_stats()
-        out["gb_nhc_8000_f32"] = f32_row(rec, p["n"], args.gb_steps, el, err, d.rebuild_count() - rb0,
-                                         {"T": d.get_scalar("TEMP"),
-                                          "chain_newton_updates_per_call": (it1 - it0) / max(1, calls1 - calls0),
-                                          "chain_serial_fallbacks": fb1 - fb0})
+        def segment(bits, list_cutoff):
+            """The timed segment again from its starting state, with the pair
+            terms at `bits` and a list of radius list_cutoff + skin, after a
+            warm-up run from the same state that captures the step graphs."""
+            d.set_pair_precision(bits)
+            d.init_nlist(p["skin"], list_cutoff)
+            back_to_start()
+            d.run(sx.INT_NHT, gb, 100, p["dt"], first_step=step)
+            back_to_start()
+            rb0 = d.rebuild_count()
+            it0, calls0, fb0 = d.nht_stats()
+            t0 = time.perf_counter()
+            d.run(sx.INT_NHT, gb, args.gb_steps, p["dt"], first_step=step)
+            d.synchronize()
+            el = time.perf_counter() - t0
+            it1, calls1, fb1 = d.nht_stats()
+            return el, {"rebuilds": d.rebuild_count() - rb0, "T": d.get_scalar("TEMP"),
+                        "chain_newton_updates_per_call": (it1 - it0) / max(1, calls1 - calls0),
+                        "chain_serial_fallbacks": fb1 - fb0}
+
+        el, x = segment(32, p["cutoff"])
+        out["gb_nhc_8000_f32"] = f32_row(rec, p["n"], args.gb_steps, el, err, x.pop("rebuilds"), x)
         log(f"[gb f32] {el / args.gb_steps * 1e3:.4f} ms/step; error {err['force_elem_err']:.2e}; "
-            f"{(it1 - it0) / max(1, calls1 - calls0):.2f} Newton updates per chain call, {fb1 - fb0} serial fallbacks")
+            f"{x['chain_newton_updates_per_call']:.2f} Newton updates per chain call, "
+            f"{x['chain_serial_fallbacks']} serial fallbacks")
+        # the list radius that covers gb_'s whole range, sigma_max + cutoff -
+        # sigma_0 = 6.0, plus the skin (the headline rows use cutoff + skin)
+        full = 6.0
+        for bits in (64, 32):
+            el, x = segment(bits, full)
+            key = f"gb_nhc_8000_list{full + p['skin']:.1f}" + ("_f32" if bits == 32 else "")
+            out[key] = dict(value=p["n"] * args.gb_steps / el, unit="particle-steps/s", steps=args.gb_steps,
+                            ms_per_step=el / args.gb_steps * 1e3, **x,
+                            config=dict(rec["config"], list_radius=full + p["skin"],
+                                        dtype="f64" if bits == 64 else
+                                        "f32 pair terms (f64 positions, sums, integrator)"))
+            log(f"[gb list {full + p['skin']:.1f} {bits}-bit] {el / args.gb_steps * 1e3:.4f} ms/step")
         d.close()
     if args.chain_steps > 0:
         d = sx.System(0)
