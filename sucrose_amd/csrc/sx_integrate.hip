@@ -626,23 +626,22 @@ __device__ __forceinline__ NhtArgs nht_args(const double *scal, double dt, doubl
     return a;
 }
 
-// converged by pass k: some earlier pass j >= 1 moved no entry state by more
-// than the tolerance (a pass that returned early leaves its slot at zero,
-// which only happens after an earlier one converged), or moved them by less
-// than NHT_FLOOR without shrinking to a quarter of the change before it: the
-// iteration converges quadratically until the rounding of the composed
-// models stops it, and where the chain amplifies that rounding the floor can
-// lie above NHT_TOL
+// converged by pass k: pass k - 1 moved no entry state by more than the
+// tolerance, or by less than NHT_FLOOR without shrinking to a quarter of the
+// change of pass k - 2 (the iteration converges quadratically until the
+// rounding of the composed models stops it, and where the chain amplifies
+// that rounding the floor can lie above NHT_TOL).  A pass that returns early
+// leaves its slot at zero, so once a pass has converged every later one
+// finds its predecessor's slot at zero: two loads decide, for any k.
 __device__ __forceinline__ bool nht_converged(const NhtBuf &B, int k)
 {
-    double prev = INFINITY;
-    for (int j = 1; j < k; ++j) {
-        unsigned long long u = __hip_atomic_load(&B.dev[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const double d = __longlong_as_double((long long)u);
-        if (d < NHT_TOL || (d < NHT_FLOOR && d > 0.25 * prev)) return true;
-        prev = d;
-    }
-    return false;
+    if (k < 2) return false;
+    const double d = __longlong_as_double(
+        (long long)__hip_atomic_load(&B.dev[k - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    const double prev = k >= 3 ? __longlong_as_double((long long)__hip_atomic_load(
+                                     &B.dev[k - 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                               : INFINITY;
+    return d < NHT_TOL || (d < NHT_FLOOR && d > 0.25 * prev);
 }
 
 // the chain masses of the call: the coupling given, else n_dof T tau^2 for
@@ -721,8 +720,11 @@ __global__ void __launch_bounds__(NHT_BLK * NHT_DIRS) k_nhtp_pass(DState s, doub
     } else {
         const int pk = (k - 1) & 1;
         if (dir == 0) {
+            // every load first, so that one round trip serves them all
             const double s0[4] = {B.s0[0], B.s0[1], B.s0[2], B.s0[3]};
             AffMap x = lane < B.nb ? aff_load(B.T[pk] + (size_t)lane * 20) : aff_identity();
+            const AffMap Pc = aff_load(B.P[pk] + (size_t)c * 20);
+            const double gold[4] = {B.g[c * 4 + 0], B.g[c * 4 + 1], B.g[c * 4 + 2], B.g[c * 4 + 3]};
             x = aff_wave_scan(x, lane);
             double E[4];
             if (b == 0) {
@@ -731,15 +733,13 @@ __global__ void __launch_bounds__(NHT_BLK * NHT_DIRS) k_nhtp_pass(DState s, doub
                 AffMap pre = aff_shfl(x, b - 1);
                 aff_apply(pre, s0, E);
             }
-            AffMap Pc = aff_load(B.P[pk] + (size_t)c * 20);
             aff_apply(Pc, E, g);
             const double T = scal[SX_TARGET_TEMP];
             const double scale[4] = {sqrt(fabs(scal[SX_HB0] * T)), sqrt(fabs(scal[SX_HB1] * T)),
                                      sqrt(fabs(scal[SX_HB2] * T)), fabs(ndof * T)};
             double dv = 0.0;
             for (int q = 0; q < 4; ++q) {
-                double old = B.g[c * 4 + q];
-                double d = fabs(g[q] - old) / (fabs(g[q]) + scale[q] + 1e-300);
+                double d = fabs(g[q] - gold[q]) / (fabs(g[q]) + scale[q] + 1e-300);
                 dv = (d == d && dv == dv) ? fmax(dv, d) : INFINITY;
                 B.g[c * 4 + q] = g[q];
                 gsh[lane][q] = g[q];
