@@ -267,8 +267,10 @@ def f32_error(d, snap, evaluate, fields=FORCE_FIELDS, groups=None):
     f64, pe64 = got[64]
     f32, pe32 = got[32]
     groups = groups or [slice(None)]
-    worst = max(elem_err(f32[k][g], f64[k][g]) for g in groups for k in range(len(fields)) if np.any(f64[k][g]))
-    return {"force_elem_err": worst,
+    per = [max([elem_err(f32[k][g], f64[k][g]) for k in range(len(fields)) if np.any(f64[k][g])] or [0.0])
+           for g in groups]
+    worst = max(per)
+    return {"force_elem_err": worst, "per_group": per,
             "pe_rel_err": float(np.max(np.abs(np.asarray(pe32) - np.asarray(pe64)) /
                                        np.maximum(1.0, np.abs(np.asarray(pe64))))),
             "measure": "FP32 vs FP64 pair terms on one state; |a-b|/(|b|+median|b|) per force/torque element"
@@ -335,6 +337,7 @@ def other_workloads(sx, args):
             return d.get_scalar("POT_ENG")
 
         err = f32_error(d, snap, gb_forces)
+        err.pop("per_group")
 
         def back_to_start():
             d.load_arrays(start)
@@ -413,6 +416,7 @@ def other_workloads(sx, args):
             return d.get_scalar("POT_ENG")
 
         err = f32_error(d, snap, chain_forces, FORCE_FIELDS[:3])
+        err.pop("per_group")
         first = p["next_step"] + args.chain_steps
         rb0 = d.rebuild_count()
         t0 = time.perf_counter()
@@ -480,6 +484,11 @@ def batched_workload(sx, args):
 
     ends = np.cumsum([0] + list(p["counts"]))
     err = f32_error(d, snap, batch_step, groups=[slice(a, b) for a, b in zip(ends[:-1], ends[1:])])
+    per = np.asarray(err.pop("per_group"))
+    kinds = np.asarray(p["kinds"])
+    err["force_elem_err_lj"] = float(per[kinds == sx.LJ].max())
+    err["force_elem_err_gb"] = float(per[kinds == sx.GB].max())
+    err["systems_over_1e-5"] = int(np.sum(per > 1e-5))
     choice, ms = d.batch_autotune(p["dt"], damping=p["gamma"], first_step=step, nrep=5)
     shape = sx.System.BATCH_SHAPES[choice]
     d.batch_run(50, p["dt"], damping=p["gamma"], first_step=step)
@@ -493,7 +502,9 @@ def batched_workload(sx, args):
                   {"launch_shape": {"particles_per_workgroup": shape[0], "lanes_per_particle": shape[1]},
                    "autotune_ms": [round(float(m), 4) for m in ms]})
     del row["rebuilds"]
-    log(f"[batch f32] {el / args.batch_steps * 1e3:.4f} ms/step (shape {shape}); error {err['force_elem_err']:.2e}")
+    log(f"[batch f32] {el / args.batch_steps * 1e3:.4f} ms/step (shape {shape}); error {err['force_elem_err']:.2e} "
+        f"(LJ {err['force_elem_err_lj']:.2e}, GB {err['force_elem_err_gb']:.2e}, "
+        f"{err['systems_over_1e-5']} systems over 1e-5)")
     d.close()
     return rec, row
 

@@ -6,8 +6,8 @@
 //
 // The FP32 pair mode.  The cores of LJ, WCA and Debye-Hueckel are templates
 // on the arithmetic type T: T = double is the form above, T = float the same
-// expressions in FP32 with the hardware reciprocal, square root and exp2
-// (about 1 ulp each).  gb_ has a mixed form of its own (gb_core_f32).  In
+// expressions in FP32 with the hardware square root and exp2 (about 1 ulp
+// each) and the hardware reciprocal refined by a Newton step.  gb_ has a mixed form of its own (gb_core_f32).  In
 // both the separation, the cutoff test that decides whether a pair interacts
 // and every sum over pairs stay FP64.
 #pragma once
@@ -23,7 +23,12 @@ struct PairOutT {
 using PairOut = PairOutT<double>;
 
 __device__ __forceinline__ double sx_dv(double a, double b) { return a / b; }
-__device__ __forceinline__ float sx_dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+// the hardware reciprocal refined by one Newton step (x + x (1 - b x))
+__device__ __forceinline__ float sx_dv(float a, float b)
+{
+    const float x = __builtin_amdgcn_rcpf(b);
+    return a * __builtin_fmaf(__builtin_fmaf(-b, x, 1.0f), x, x);
+}
 __device__ __forceinline__ double sx_sqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float sx_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double sx_exp(double x) { return exp(x); }
@@ -494,10 +499,11 @@ __device__ __forceinline__ void d_gb_tab(const DState &s, int a, int b, const do
 // with the axes cancel for the same pairs.  The mixed form keeps in FP64 the
 // orientation function g of chi and chi' with its gradients in r, u1 and u2
 // (gb_g, gb_dgdr, gb_dgdu, each division by 1 +- chi u1.u2 done once), sigma,
-// sr and its powers, dR/dr and the torque sums; the strength factors
-// eps1^nu eps2^mu and their derivatives, the force sums and the switch are
-// FP32.  It needs 7 FP64 divisions and 2 square roots per pair, where the
-// reference-order FP64 form needs about 60 divisions.
+// sr and its powers, and the sums that form forces, torques, energy and the
+// switch; the strength factors eps1^nu eps2^mu and their derivatives are
+// FP32.  It needs 7 FP64 divisions and 2 square roots per pair (and one more
+// division in the switch band), where the reference-order FP64 form needs
+// about 60 divisions.
 // ---------------------------------------------------------------------------
 struct GbOri {
     double g, dgr[3], dgu1[3], dgu2[3];
@@ -545,36 +551,34 @@ __device__ __forceinline__ void gb_core_f32(const double r[3], const GbSelf &A, 
     const float eps2 = sx_dv((float)X.g, (float)A.epsz);
     const float e1n = d_ipow(eps1, nu), e2m = d_ipow(eps2, mu);
     const float e2m1 = d_ipow(eps2, mu - 1), e1n2 = d_ipow(eps1, nu + 2);
-    const float e12 = e1n * e2m, de2 = e1n * ((float)mu * e2m1);
-    const float fsrx = (float)srx, fsrd = (float)srd;
-    float pot = e12 * fsrx;
+    // the sums in FP64 from the FP32 strengths: forces and torques each add
+    // terms that cancel for some pairs
+    const double E12 = (double)(e1n * e2m), DE2 = (double)(e1n * ((float)mu * e2m1));
+    const double E1 = (double)(e2m * ((float)nu * e1n2)) * (A.chi * A.chi * q.uu);
+    double pot = E12 * srx, f[3], TI[3], TJ[3], ti[3], tj[3];
     for (int k = 0; k < 3; ++k) {
-        const float dR_dr = (float)(r[k] * inv_mag + 0.5 * C.dgr[k] * g32);
-        o.f[k] = -(e12 * fsrd * dR_dr + de2 * (float)X.dgr[k] * fsrx);
-    }
-    const double E1 = (double)(e2m * ((float)nu * e1n2)) * (A.chi * A.chi * q.uu), E2 = de2, E12 = e12;
-    double TI[3], TJ[3], ti[3], tj[3];
-    for (int k = 0; k < 3; ++k) {
-        TI[k] = -(srx * (E1 * q.u2[k] + E2 * X.dgu1[k]) + E12 * (srd * 0.5 * C.dgu1[k] * g32));
-        TJ[k] = -(srx * (E1 * q.u1[k] + E2 * X.dgu2[k]) + E12 * (srd * 0.5 * C.dgu2[k] * g32));
+        const double dR_dr = r[k] * inv_mag + 0.5 * C.dgr[k] * g32;
+        f[k] = -(E12 * srd * dR_dr + DE2 * X.dgr[k] * srx);
+        TI[k] = -(srx * (E1 * q.u2[k] + DE2 * X.dgu1[k]) + E12 * (srd * 0.5 * C.dgu1[k] * g32));
+        TJ[k] = -(srx * (E1 * q.u1[k] + DE2 * X.dgu2[k]) + E12 * (srd * 0.5 * C.dgu2[k] * g32));
     }
     cross3(q.u1, TI, ti);
     cross3(q.u2, TJ, tj);
-    for (int k = 0; k < 3; ++k) { o.ti[k] = (float)ti[k]; o.tj[k] = (float)tj[k]; }
     const double sw = 0.9 * rlim;
     if (rij_mag >= sw) {
-        const float factor = __builtin_amdgcn_rcpf((float)(rlim - sw));
-        const float xx = (float)(rij_mag - sw) * factor;
-        const float sx = 1.0f - 6.0f * xx * xx * xx * xx * xx + 15.0f * xx * xx * xx * xx - 10.0f * xx * xx * xx;
-        const float sdx = -30.0f * xx * xx * xx * xx + 60.0f * xx * xx * xx - 30.0f * xx * xx;
+        const double factor = 1.0 / (rlim - sw);
+        const double xx = (rij_mag - sw) * factor;
+        const double sx = 1.0 - 6.0 * xx * xx * xx * xx * xx + 15.0 * xx * xx * xx * xx - 10.0 * xx * xx * xx;
+        const double sdx = -30.0 * xx * xx * xx * xx + 60.0 * xx * xx * xx - 30.0 * xx * xx;
         for (int k = 0; k < 3; ++k) {
-            o.f[k] = sx * o.f[k] - sdx * pot * factor;
-            o.ti[k] = sx * o.ti[k] - sdx * pot * factor;
-            o.tj[k] = sx * o.tj[k] - sdx * pot * factor;
+            f[k] = sx * f[k] - sdx * pot * factor;
+            ti[k] = sx * ti[k] - sdx * pot * factor;
+            tj[k] = sx * tj[k] - sdx * pot * factor;
         }
         pot = sx * pot;
     }
-    o.pe = pot;
+    for (int k = 0; k < 3; ++k) { o.f[k] = (float)f[k]; o.ti[k] = (float)ti[k]; o.tj[k] = (float)tj[k]; }
+    o.pe = (float)pot;
 }
 
 // d_gb_tab<0> in the FP32 pair mode
