@@ -184,11 +184,13 @@ int sx_set_ecp_solver(sx_ctx *ctx, int solver);
 /* Arithmetic of the pair terms, 64 (default) or 32 bits.  No counterpart in
  * the reference, whose force.f90 (6-49, 155-190, 308-428, 2589-2617) is FP64
  * throughout; the workloads quote FP32 pair terms.  At 32 bits the terms of
- * SX_LJ, SX_WCA, SX_DH and SX_GB (sx_forces, the WCA + Debye-Hueckel pass of
- * the step driver and the batched replicas) are evaluated in FP32 with
- * hardware reciprocal, square root and exp2; positions, the separation, the
- * cutoff / neighbour decisions, every sum over pairs, the integrators and
- * all other kinds (ECP included) stay FP64.  Invalidates captured steps. */
+ * SX_LJ, SX_WCA and SX_DH (sx_forces, the WCA + Debye-Hueckel pass of the
+ * step driver and the batched replicas) are evaluated in FP32 with hardware
+ * reciprocal, square root and exp2, and SX_GB in an FP64 form with shared
+ * divisions instead of the reference's expression order (FP32 is too coarse
+ * for gb_ near contact); positions, the separation, the cutoff / neighbour
+ * decisions, every sum over pairs, the integrators and all other kinds (ECP
+ * included) stay FP64.  Invalidates captured steps. */
 int sx_set_pair_precision(sx_ctx *ctx, int bits);
 int sx_get_pair_precision(sx_ctx *ctx, int *bits);
 /* ECP diagnostics of the last sx_forces(SX_ECP*): pairs screened, pairs solved,

@@ -356,8 +356,9 @@ class System:
         self._ok(self.L.sx_set_ecp_solver(self.h, int(solver)))
 
     def set_pair_precision(self, bits):
-        """32: FP32 pair terms for LJ, WCA, Debye-Hueckel and GB (variant 0);
-        64: FP64 throughout (the default)."""
+        """32: FP32 pair terms for LJ, WCA and Debye-Hueckel, and GB (variant 0)
+        in its form with fewer divisions; 64: the reference's FP64 expressions
+        throughout (the default)."""
         self._ok(self.L.sx_set_pair_precision(self.h, int(bits)))
 
     def pair_precision(self):

@@ -7,9 +7,10 @@
 // The FP32 pair mode.  The cores of LJ, WCA and Debye-Hueckel are templates
 // on the arithmetic type T: T = double is the form above, T = float the same
 // expressions in FP32 with the hardware square root and exp2 (about 1 ulp
-// each) and the hardware reciprocal refined by a Newton step.  gb_ has a mixed form of its own (gb_core_f32).  In
-// both the separation, the cutoff test that decides whether a pair interacts
-// and every sum over pairs stay FP64.
+// each) and the hardware reciprocal refined by a Newton step.  gb_, too
+// ill-conditioned for FP32, has a form of its own with fewer divisions
+// (gb_core_f32).  In all of them the separation, the cutoff test that decides
+// whether a pair interacts and every sum over pairs stay FP64.
 #pragma once
 #include "sx_internal.h"
 
@@ -496,14 +497,18 @@ __device__ __forceinline__ void d_gb_tab(const DState &s, int a, int b, const do
 // sigma = sig_x / sqrt(g) carries that error into r - sigma + sigma_0, and the
 // 12th power of sr amplifies it again, so rounding the inputs alone to FP32
 // moves such a pair's force by ~3e-5 of its size; the torques' cross products
-// with the axes cancel for the same pairs.  The mixed form keeps in FP64 the
-// orientation function g of chi and chi' with its gradients in r, u1 and u2
-// (gb_g, gb_dgdr, gb_dgdu, each division by 1 +- chi u1.u2 done once), sigma,
-// sr and its powers, and the sums that form forces, torques, energy and the
-// switch; the strength factors eps1^nu eps2^mu and their derivatives are
-// FP32.  It needs 7 FP64 divisions and 2 square roots per pair (and one more
-// division in the switch band), where the reference-order FP64 form needs
-// about 60 divisions.
+// with the axes cancel for the same pairs, and FP32 strength factors
+// eps1^nu eps2^mu (a common factor of every term of a pair) leave ~2e-7 of
+// each pair's force in particles whose pair forces cancel, up to 1.6e-5 of
+// the particle's force in the batched workload.  So this form computes in
+// FP64 throughout and saves what the FP32 mode is for, the long division and
+// square-root sequences: g of chi and chi' with its gradients in r, u1 and u2
+// (gb_g, gb_dgdr, gb_dgdu) share each division by r^2 and by 1 +- chi u1.u2,
+// sigma and g^(-3/2) share one reciprocal square root, and eps1 is an FP32
+// reciprocal square root refined by a Newton step: 8 FP64 divisions and 2
+// square roots per pair (one more division in the switch band), where the
+// reference-order form needs about 60 divisions.  The results are rounded to
+// FP32 on output.
 // ---------------------------------------------------------------------------
 struct GbOri {
     double g, dgr[3], dgu1[3], dgu2[3];
@@ -545,16 +550,16 @@ __device__ __forceinline__ void gb_core_f32(const double r[3], const GbSelf &A, 
     const double sr2 = sr * sr, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
     const double srx = sr12 - 2.0 * sr6, srd = 12.0 * (sr6 - sr12) * sr;
     const double g32 = sigma * rs * rs;          // sigma / g
-    // strength factors
-    const float chu = (float)(A.chi * q.uu);
-    const float eps1 = __builtin_amdgcn_rsqf(1.0f - chu * chu);
-    const float eps2 = sx_dv((float)X.g, (float)A.epsz);
-    const float e1n = d_ipow(eps1, nu), e2m = d_ipow(eps2, mu);
-    const float e2m1 = d_ipow(eps2, mu - 1), e1n2 = d_ipow(eps1, nu + 2);
-    // the sums in FP64 from the FP32 strengths: forces and torques each add
-    // terms that cancel for some pairs
-    const double E12 = (double)(e1n * e2m), DE2 = (double)(e1n * ((float)mu * e2m1));
-    const double E1 = (double)(e2m * ((float)nu * e1n2)) * (A.chi * A.chi * q.uu);
+    // strength factors: eps1 = (1 - (chi u1.u2)^2)^(-1/2) from the FP32
+    // reciprocal square root and one Newton step in FP64, eps2 = g(chi') / eps_z
+    const double x1 = 1.0 - (A.chi * q.uu) * (A.chi * q.uu);
+    const double y0 = (double)__builtin_amdgcn_rsqf((float)x1);
+    const double eps1 = y0 * (1.5 - 0.5 * x1 * y0 * y0);
+    const double eps2 = X.g / A.epsz;
+    const double e1n = d_ipow(eps1, nu), e2m = d_ipow(eps2, mu);
+    const double e2m1 = d_ipow(eps2, mu - 1), e1n2 = d_ipow(eps1, nu + 2);
+    const double E12 = e1n * e2m, DE2 = e1n * ((double)mu * e2m1);
+    const double E1 = e2m * ((double)nu * e1n2) * (A.chi * A.chi * q.uu);
     double pot = E12 * srx, f[3], TI[3], TJ[3], ti[3], tj[3];
     for (int k = 0; k < 3; ++k) {
         const double dR_dr = r[k] * inv_mag + 0.5 * C.dgr[k] * g32;

@@ -1,8 +1,9 @@
 """FP32 pair mode (sx_set_pair_precision(32)) vs the CPU oracle.
 
 The workloads quote FP32 pair terms.  In that mode the device evaluates the
-LJ, WCA, Debye-Hueckel and gb_ (variant 0) pair terms in FP32 (hardware
-reciprocal, square root and exp2) while positions, separations, the cutoff
+LJ, WCA and Debye-Hueckel pair terms in FP32 (hardware reciprocal, square
+root and exp2) and gb_ (variant 0) in a form with fewer FP64 divisions (FP32
+is too coarse for gb_ near contact), while positions, separations, the cutoff
 and neighbour decisions, every sum over pairs and the integrators stay FP64.
 The bar: every force and torque component within 1e-5 of its own magnitude
 plus the median magnitude, and the potential energy within 1e-5 relative,
