@@ -387,9 +387,16 @@ def other_workloads(sx, args):
     if args.chain_steps > 0:
         d = sx.System(0)
         p = chains_state(d, log=log)
+        warm = 64 + 16 + 4   # captures every step-graph length a long run replays
+        step = p["next_step"]
+        d.run(sx.INT_LGVC, p["forces"], warm, p["dt"], damping=p["gamma"], first_step=step, bonded=True)
+        step += warm
+        d.synchronize()
+        start = d.state()
+        start_scal = {k: d.get_scalar(k) for k in sx.SCALARS}
         rb0 = d.rebuild_count()
         t0 = time.perf_counter()
-        d.run(sx.INT_LGVC, p["forces"], args.chain_steps, p["dt"], damping=p["gamma"], first_step=p["next_step"],
+        d.run(sx.INT_LGVC, p["forces"], args.chain_steps, p["dt"], damping=p["gamma"], first_step=step,
               bonded=True)
         d.synchronize()
         el = time.perf_counter() - t0
@@ -417,10 +424,22 @@ def other_workloads(sx, args):
 
         err = f32_error(d, snap, chain_forces, FORCE_FIELDS[:3])
         err.pop("per_group")
-        first = p["next_step"] + args.chain_steps
+
+        def back_to_start():
+            d.load_arrays(start)
+            for k, v in start_scal.items():
+                d.set_scalar(k, v)
+            d.build_nlist()
+            d.synchronize()
+
+        # FP32: the same segment (same state, same Langevin counters) after a
+        # warm-up that captures the FP32 step graphs
+        back_to_start()
+        d.run(sx.INT_LGVC, p["forces"], warm, p["dt"], damping=p["gamma"], first_step=step, bonded=True)
+        back_to_start()
         rb0 = d.rebuild_count()
         t0 = time.perf_counter()
-        d.run(sx.INT_LGVC, p["forces"], args.chain_steps, p["dt"], damping=p["gamma"], first_step=first,
+        d.run(sx.INT_LGVC, p["forces"], args.chain_steps, p["dt"], damping=p["gamma"], first_step=step,
               bonded=True)
         d.synchronize()
         el = time.perf_counter() - t0
@@ -447,7 +466,8 @@ def batched_workload(sx, args):
         + f" -> {shape[0]} particles per force workgroup, {shape[1]} lanes/particle")
     d.batch_run(50, p["dt"], damping=p["gamma"], first_step=step)
     step += 50
-    state = d.state() if not args.no_cpu else None
+    start, start_step = d.state(), step
+    state = start if not args.no_cpu else None
     d.synchronize()
     t0 = time.perf_counter()
     d.batch_run(args.batch_steps, p["dt"], damping=p["gamma"], first_step=step)
@@ -474,7 +494,7 @@ def batched_workload(sx, args):
             rec["cpu_baseline"] = cpu
             log(f"[batch cpu] reference {cpu['value']:.4e} particle-steps/s ({cpu['seconds']:.1f}s)")
     # FP32 pair terms: the error of one step's forces from one state, then
-    # the autotuner and the timed run again
+    # the autotuner, and the timed segment again from its starting state
     step += args.batch_steps
     snap = d.state()
 
@@ -491,11 +511,12 @@ def batched_workload(sx, args):
     err["systems_over_1e-5"] = int(np.sum(per > 1e-5))
     choice, ms = d.batch_autotune(p["dt"], damping=p["gamma"], first_step=step, nrep=5)
     shape = sx.System.BATCH_SHAPES[choice]
-    d.batch_run(50, p["dt"], damping=p["gamma"], first_step=step)
-    step += 50
+    d.load_arrays(start)
+    d.batch_run(50, p["dt"], damping=p["gamma"], first_step=start_step)   # captures the step graph
+    d.load_arrays(start)
     d.synchronize()
     t0 = time.perf_counter()
-    d.batch_run(args.batch_steps, p["dt"], damping=p["gamma"], first_step=step)
+    d.batch_run(args.batch_steps, p["dt"], damping=p["gamma"], first_step=start_step)
     d.synchronize()
     el = time.perf_counter() - t0
     row = f32_row(rec, p["n"], args.batch_steps, el, err, 0,
