@@ -198,7 +198,8 @@ int sx_get_pair_precision(sx_ctx *ctx, int *bits);
 int sx_ecp_stats(sx_ctx *ctx, long out[4]);
 /* Nose-Hoover diagnostics (cumulative over sx_nht_* calls of 2048 particles or
  * more, which evaluate the chain recurrence in parallel by Newton iteration):
- * Newton iterations, calls, calls that fell back to the serial walk. */
+ * Newton iterations, calls, calls whose passes did not converge (evaluated
+ * window by window instead). */
 int sx_nht_stats(sx_ctx *ctx, long out[3]);
 
 /* bonded terms (module bonded, force.f90:2244-2579); kind 1 bend_, 2 cosine_bend_,

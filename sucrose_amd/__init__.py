@@ -373,7 +373,8 @@ class System:
 
     def nht_stats(self):
         """Cumulative Nose-Hoover (parallel evaluation) counts: Newton
-        iterations, calls, serial fallbacks."""
+        iterations, calls, calls evaluated window by window because their
+        passes did not converge."""
         o = (C.c_long * 3)()
         self._ok(self.L.sx_nht_stats(self.h, o))
         return list(o)

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <numeric>
 
 template <int W>
@@ -422,7 +423,8 @@ __global__ void __launch_bounds__(NHT_CHUNK) k_nht(DState s, double dt, int phas
 // reached through the scan).  Each call is a fixed sequence of launches
 // (graph-capturable): NHT_MAXIT passes that return at once after
 // convergence (the first also sets up the call), final; without convergence
-// the final kernel walks the recurrence serially.
+// the final kernel takes the chunks in windows, one window after another
+// (nht_windows).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double val(double x) { return x; }
 __device__ __forceinline__ double texp(double x) { return exp(x); }
@@ -453,6 +455,52 @@ __device__ __forceinline__ D1 texp(const D1 &x)
 {
     double e = exp(x.v);
     return D1{e, e * x.d};
+}
+
+// value and all four directional derivatives (the one-wavefront window walks
+// of the final kernel)
+struct D4 {
+    double v, d[4];
+};
+__device__ __forceinline__ D4 dd(double c) { return D4{c, {0, 0, 0, 0}}; }
+__device__ __forceinline__ D4 operator+(const D4 &a, const D4 &b)
+{
+    return D4{a.v + b.v, {a.d[0] + b.d[0], a.d[1] + b.d[1], a.d[2] + b.d[2], a.d[3] + b.d[3]}};
+}
+__device__ __forceinline__ D4 operator-(const D4 &a, const D4 &b)
+{
+    return D4{a.v - b.v, {a.d[0] - b.d[0], a.d[1] - b.d[1], a.d[2] - b.d[2], a.d[3] - b.d[3]}};
+}
+__device__ __forceinline__ D4 operator*(const D4 &a, const D4 &b)
+{
+    D4 r;
+    r.v = a.v * b.v;
+    for (int k = 0; k < 4; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+}
+__device__ __forceinline__ D4 operator/(const D4 &a, const D4 &b)
+{
+    D4 r;
+    r.v = a.v / b.v;
+    const double ib = 1.0 / b.v;
+    for (int k = 0; k < 4; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) * ib;
+    return r;
+}
+__device__ __forceinline__ D4 operator+(const D4 &a, double b) { return a + dd(b); }
+__device__ __forceinline__ D4 operator-(const D4 &a, double b) { return a - dd(b); }
+__device__ __forceinline__ D4 operator*(const D4 &a, double b)
+{
+    return D4{a.v * b, {a.d[0] * b, a.d[1] * b, a.d[2] * b, a.d[3] * b}};
+}
+__device__ __forceinline__ D4 operator/(const D4 &a, double b) { return a / dd(b); }
+__device__ __forceinline__ D4 operator*(double a, const D4 &b) { return b * a; }
+__device__ __forceinline__ D4 operator-(double a, const D4 &b) { return dd(a) - b; }
+__device__ __forceinline__ D4 operator-(const D4 &a) { return dd(0.0) - a; }
+__device__ __forceinline__ double val(const D4 &x) { return x.v; }
+__device__ __forceinline__ D4 texp(const D4 &x)
+{
+    const double e = exp(x.v);
+    return D4{e, {e * x.d[0], e * x.d[1], e * x.d[2], e * x.d[3]}};
 }
 
 // exprel and u4 of the serial kernel above, for doubles or dual numbers
@@ -805,67 +853,148 @@ __global__ void __launch_bounds__(NHT_BLK * NHT_DIRS) k_nhtp_pass(DState s, doub
     if (lane == NHT_BLK - 1) aff_store(incl, B.T[k & 1] + (size_t)b * 20);
 }
 
-// converged: the particle updates of every chunk from the converged entry
-// state, then the sums of the eta increments and kinetic energy (fixed
-// order) and the chain state after the last particle.  Not converged: one
-// lane walks the whole recurrence from the state entering the call.
+// particle i's turn of the chain call from chain state (p, sv), with its
+// velocity update, its drift in phase 0 and its kinetic energy in phase 1
+__device__ __forceinline__ void nht_particle(const DState &s, int i, int phase, int save, const NhtArgs &a,
+                                             double p[3], double &sv, double v4[4])
+{
+    const double dt = a.dt;
+    double m = s.f[SX_MASS][i], im = s.f[SX_INV_MASS][i];
+    double acc3[3] = {0.5 * dt * (s.f[SX_FX][i] * im), 0.5 * dt * (s.f[SX_FY][i] * im),
+                      0.5 * dt * (s.f[SX_FZ][i] * im)};
+    double v[3] = {s.f[SX_VX][i], s.f[SX_VY][i], s.f[SX_VZ][i]};
+    double ea[3];
+    nht_turn(phase, dt, a.tq, a.ndofT, a.Temp, a.hb, m, acc3, p, sv, v, ea);
+    for (int q = 0; q < 3; ++q) v4[q] += ea[q];
+    s.f[SX_VX][i] = v[0]; s.f[SX_VY][i] = v[1]; s.f[SX_VZ][i] = v[2];
+    if (phase == 0) {
+        if (save) save_old(s, i);
+        s.f[SX_RX][i] = s.f[SX_RX][i] + dt * v[0];
+        s.f[SX_RY][i] = s.f[SX_RY][i] + dt * v[1];
+        s.f[SX_RZ][i] = s.f[SX_RZ][i] + dt * v[2];
+        wrap_pos(s, i);
+    } else {
+        v4[3] += 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+}
+
+#define NHT_WIN_IT 16   // Newton iterations of one window before its serial walk
+
+// Without convergence of the passes (a call whose chain swings far within
+// it): the first workgroup takes the chunks in windows of NHT_BLK, one window
+// after another from the exact state entering it.  Within a window it runs
+// Newton from the flat guess over the window's chunks, one per lane, with
+// dual numbers carrying all four derivatives and one shuffle scan per
+// iteration (a window spans a short stretch of the chain's evolution, so a
+// few iterations converge); a window that has not converged after NHT_WIN_IT
+// iterations is walked serially by one lane.  Then the window's particles
+// are updated from the converged entry states and the state after its last
+// chunk enters the next window.
+__device__ void nht_windows(const DState &s, int phase, int save, double *scal, double ndof, const NhtBuf &B,
+                            const NhtArgs &a)
+{
+    const int lane = threadIdx.x;
+    const int nchunks = (s.n + B.chunk - 1) / B.chunk;
+    const double T = a.Temp;
+    const double scale[4] = {sqrt(fabs(a.hb[0] * T)), sqrt(fabs(a.hb[1] * T)), sqrt(fabs(a.hb[2] * T)),
+                             fabs(ndof * T)};
+    double e[4] = {B.s0[0], B.s0[1], B.s0[2], B.s0[3]};
+    double v4[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int w0 = 0; w0 < nchunks; w0 += NHT_BLK) {
+        int i0, i1;
+        nht_chunk(s.n, B.chunk, w0 + lane, i0, i1);
+        double g[4] = {e[0], e[1], e[2], e[3]};
+        bool ok = false;
+        for (int it = 0; it < NHT_WIN_IT && !ok; ++it) {
+            D4 p[3], sv;
+            for (int q = 0; q < 3; ++q) { p[q] = dd(g[q]); p[q].d[q] = 1.0; }
+            sv = dd(g[3]);
+            sv.d[3] = 1.0;
+            for (int i = i0; i < i1; ++i) {
+                double m = s.f[SX_MASS][i], im = s.f[SX_INV_MASS][i];
+                double acc3[3] = {0.5 * a.dt * (s.f[SX_FX][i] * im), 0.5 * a.dt * (s.f[SX_FY][i] * im),
+                                  0.5 * a.dt * (s.f[SX_FZ][i] * im)};
+                D4 v[3] = {dd(s.f[SX_VX][i]), dd(s.f[SX_VY][i]), dd(s.f[SX_VZ][i])};
+                D4 ea[3];
+                nht_turn(phase, a.dt, a.tq, a.ndofT, a.Temp, a.hb, m, acc3, p, sv, v, ea);
+            }
+            const D4 *out[4] = {&p[0], &p[1], &p[2], &sv};
+            AffMap A;
+            for (int r = 0; r < 4; ++r) {
+                double cr = out[r]->v;
+                for (int q = 0; q < 4; ++q) {
+                    A.M[r * 4 + q] = out[r]->d[q];
+                    cr -= out[r]->d[q] * g[q];
+                }
+                A.c[r] = cr;
+            }
+            AffMap incl = aff_wave_scan(A, lane);
+            AffMap excl = aff_shfl(incl, lane > 0 ? lane - 1 : 0);
+            if (lane == 0) excl = aff_identity();
+            double gn[4];
+            aff_apply(excl, e, gn);
+            double dv = 0.0;
+            for (int q = 0; q < 4; ++q) {
+                const double d = fabs(gn[q] - g[q]) / (fabs(gn[q]) + scale[q] + 1e-300);
+                dv = (d == d && dv == dv) ? fmax(dv, d) : INFINITY;
+                g[q] = gn[q];
+            }
+            for (int o = SX_WAVE / 2; o > 0; o >>= 1) {
+                const double y = __shfl_xor(dv, o, SX_WAVE);
+                dv = (y == y && dv == dv) ? fmax(dv, y) : INFINITY;
+            }
+            ok = dv < NHT_TOL;
+        }
+        const int last = min(nchunks, w0 + NHT_BLK) - 1 - w0;   // lane of the window's last chunk
+        double x[4] = {0.0, 0.0, 0.0, 0.0};
+        if (ok) {
+            double p[3] = {g[0], g[1], g[2]}, sv = g[3];
+            for (int i = i0; i < i1; ++i) nht_particle(s, i, phase, save, a, p, sv, v4);
+            x[0] = p[0]; x[1] = p[1]; x[2] = p[2]; x[3] = sv;
+            for (int q = 0; q < 4; ++q) e[q] = __shfl(x[q], last, SX_WAVE);
+        } else {
+            int j0, j1, k0, k1;
+            nht_chunk(s.n, B.chunk, w0, j0, k0);
+            nht_chunk(s.n, B.chunk, w0 + last, k1, j1);
+            (void)k0; (void)k1;
+            if (lane == 0) {
+                double p[3] = {e[0], e[1], e[2]}, sv = e[3];
+                for (int i = j0; i < j1; ++i) nht_particle(s, i, phase, save, a, p, sv, v4);
+                x[0] = p[0]; x[1] = p[1]; x[2] = p[2]; x[3] = sv;
+            }
+            for (int q = 0; q < 4; ++q) e[q] = __shfl(x[q], 0, SX_WAVE);
+        }
+    }
+    for (int o = SX_WAVE / 2; o > 0; o >>= 1)
+        for (int q = 0; q < 4; ++q) v4[q] = v4[q] + __shfl_xor(v4[q], o, SX_WAVE);
+    if (lane == 0) {
+        for (int q = 0; q < 3; ++q) scal[SX_ETA0 + q] = scal[SX_ETA0 + q] + v4[q];
+        if (phase == 1) scal[SX_KIN_ENG] = v4[3];
+        scal[SX_PETA0] = e[0]; scal[SX_PETA1] = e[1]; scal[SX_PETA2] = e[2];
+    }
+}
+
+// converged by pass `passes`: the particle updates of every chunk from the
+// converged entry state, then the sums of the eta increments and kinetic
+// energy (fixed order) and the chain state after the last particle.  Not
+// converged: the windows above.
 __global__ void __launch_bounds__(NHT_BLK) k_nhtp_final(DState s, double dt, int phase, int save, double *scal,
-                                                        double ndof, NhtBuf B, unsigned long long *stats)
+                                                        double ndof, NhtBuf B, unsigned long long *stats,
+                                                        int passes)
 {
     const int b = blockIdx.x, t = threadIdx.x, c = b * NHT_BLK + t;
     const NhtArgs a = nht_args(scal, dt, ndof);
-    if (!nht_converged(B, NHT_MAXIT)) {
-        if (b != 0 || t != 0) return;
-        if (stats) atomicAdd(&stats[2], 1ull);
-        double p[3] = {B.s0[0], B.s0[1], B.s0[2]}, sv = B.s0[3];
-        double eta_loc[3] = {0.0, 0.0, 0.0}, ke = 0.0;
-        for (int i = 0; i < s.n; ++i) {
-            double m = s.f[SX_MASS][i], im = s.f[SX_INV_MASS][i];
-            double acc3[3] = {0.5 * dt * (s.f[SX_FX][i] * im), 0.5 * dt * (s.f[SX_FY][i] * im),
-                              0.5 * dt * (s.f[SX_FZ][i] * im)};
-            double v[3] = {s.f[SX_VX][i], s.f[SX_VY][i], s.f[SX_VZ][i]};
-            double ea[3];
-            nht_turn(phase, dt, a.tq, a.ndofT, a.Temp, a.hb, m, acc3, p, sv, v, ea);
-            for (int q = 0; q < 3; ++q) eta_loc[q] += ea[q];
-            s.f[SX_VX][i] = v[0]; s.f[SX_VY][i] = v[1]; s.f[SX_VZ][i] = v[2];
-            if (phase == 0) {
-                if (save) save_old(s, i);
-                s.f[SX_RX][i] = s.f[SX_RX][i] + dt * v[0];
-                s.f[SX_RY][i] = s.f[SX_RY][i] + dt * v[1];
-                s.f[SX_RZ][i] = s.f[SX_RZ][i] + dt * v[2];
-                wrap_pos(s, i);
-            } else {
-                ke += 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-            }
-        }
-        for (int q = 0; q < 3; ++q) scal[SX_ETA0 + q] = scal[SX_ETA0 + q] + eta_loc[q];
-        if (phase == 1) scal[SX_KIN_ENG] = ke;
-        scal[SX_PETA0] = p[0]; scal[SX_PETA1] = p[1]; scal[SX_PETA2] = p[2];
+    if (!nht_converged(B, passes)) {
+        if (b != 0) return;
+        if (t == 0 && stats) atomicAdd(&stats[2], 1ull);
+        nht_windows(s, phase, save, scal, ndof, B, a);
         return;
     }
     int i0, i1;
     nht_chunk(s.n, B.chunk, c, i0, i1);
     double p[3] = {B.g[c * 4 + 0], B.g[c * 4 + 1], B.g[c * 4 + 2]}, sv = B.g[c * 4 + 3];
     double v4[4] = {0.0, 0.0, 0.0, 0.0};   // eta increments, kinetic energy
-    for (int i = i0; i < i1; ++i) {
-        double m = s.f[SX_MASS][i], im = s.f[SX_INV_MASS][i];
-        double acc3[3] = {0.5 * dt * (s.f[SX_FX][i] * im), 0.5 * dt * (s.f[SX_FY][i] * im),
-                          0.5 * dt * (s.f[SX_FZ][i] * im)};
-        double v[3] = {s.f[SX_VX][i], s.f[SX_VY][i], s.f[SX_VZ][i]};
-        double ea[3];
-        nht_turn(phase, dt, a.tq, a.ndofT, a.Temp, a.hb, m, acc3, p, sv, v, ea);
-        for (int q = 0; q < 3; ++q) v4[q] += ea[q];
-        s.f[SX_VX][i] = v[0]; s.f[SX_VY][i] = v[1]; s.f[SX_VZ][i] = v[2];
-        if (phase == 0) {
-            if (save) save_old(s, i);
-            s.f[SX_RX][i] = s.f[SX_RX][i] + dt * v[0];
-            s.f[SX_RY][i] = s.f[SX_RY][i] + dt * v[1];
-            s.f[SX_RZ][i] = s.f[SX_RZ][i] + dt * v[2];
-            wrap_pos(s, i);
-        } else {
-            v4[3] += 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-        }
-    }
+    for (int i = i0; i < i1; ++i) nht_particle(s, i, phase, save, a, p, sv, v4);
     // the chain state after the last particle belongs to the last non-empty chunk
     if (i1 == s.n && i0 < s.n) {
         for (int q = 0; q < 3; ++q) B.s0[4 + q] = p[q];
@@ -1370,11 +1499,15 @@ void sx_nht_impl(sx_ctx *c, double dt, const double *coupling, int phase, bool s
         B.nb = (nchunks + NHT_BLK - 1) / NHT_BLK;
         unsigned long long *st = (unsigned long long *)(c->d_counts + 4);
         DState s = c->dstate();
-        for (int k = 0; k < NHT_MAXIT; ++k)
+        // SUCROSE_NHT_PASSES (1..NHT_MAXIT) caps the passes, so that tests can
+        // drive the final kernel's windows from the host
+        int passes = NHT_MAXIT;
+        if (const char *e = getenv("SUCROSE_NHT_PASSES")) passes = std::max(1, std::min(NHT_MAXIT, atoi(e)));
+        for (int k = 0; k < passes; ++k)
             hipLaunchKernelGGL(k_nhtp_pass, dim3(B.nb), dim3(NHT_BLK * NHT_DIRS), 0, c->stream, s, dt, phase, c->d_scal, ndof,
                                B, k, st, coupling ? *coupling : 0.0, coupling ? 1 : 0);
         hipLaunchKernelGGL(k_nhtp_final, dim3(B.nb), dim3(NHT_BLK), 0, c->stream, s, dt, phase, save ? 1 : 0,
-                           c->d_scal, ndof, B, st);
+                           c->d_scal, ndof, B, st, passes);
     }
 }
 

@@ -389,6 +389,32 @@ def test_nose_hoover_far_from_equilibrium(dev, oracle):
         st = _nht_call_matches(dev, oracle, st, phase, dt=0.004)
 
 
+@pytest.mark.parametrize("phase", [0, 1])
+@pytest.mark.parametrize("far", [False, True])
+def test_nose_hoover_windows(dev, oracle, phase, far, monkeypatch):
+    """The final kernel's evaluation of a call whose Newton passes did not
+    converge: the chunks in windows of 64, each window from the exact state
+    entering it, Newton within the window (or its serial walk).  The passes
+    are capped at one from the host (SUCROSE_NHT_PASSES), so every call takes
+    this path; it must reproduce the oracle's serial walk, near equilibrium
+    and far from it."""
+    say(f"nose-hoover windows n=6000 phase={phase} far={far}")
+    monkeypatch.setenv("SUCROSE_NHT_PASSES", "1")
+    if far:
+        rng = np.random.default_rng(6 + phase)
+        st = base_state(rng, 6000, 30.0)
+        st["peta0"], st["peta1"], st["peta2"] = 0.3, -0.2, 0.1
+        dt = 0.004
+    else:
+        st = _thermal_chain_state(oracle, 41 + phase)
+        dt = CHAIN_DT
+    to_device(dev, st)
+    it0, calls0, fb0 = dev.nht_stats()
+    _nht_call_matches(dev, oracle, st, phase, dt=dt)
+    it1, calls1, fb1 = dev.nht_stats()
+    assert calls1 - calls0 == 1 and fb1 - fb0 == 1
+
+
 @pytest.mark.parametrize("kind", [VVC, LGVC])
 @pytest.mark.parametrize("phase", [0, 1])
 def test_rattle_one_step(dev, oracle, kind, phase):
