@@ -362,19 +362,30 @@ __global__ void __launch_bounds__(256) k_aniso_apply(DState s, const int *row_st
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     if (i < s.n) {
         int b = row_start[i], e = row_start[i + 1];
-        for (int q = b + l; q < e; q += LPP) {
-            const int v = idx_full[q];
-            if (v == 0) continue;
-            idx_full[q] = 0;
-            const int k = v - 1;
-            const double *r = res + (long)k * 10;
-            if (nbr[q] > i) {
-                acc[0] = acc[0] + r[1]; acc[1] = acc[1] + r[2]; acc[2] = acc[2] + r[3];
-                acc[3] = acc[3] + r[4]; acc[4] = acc[4] + r[5]; acc[5] = acc[5] + r[6];
-                acc[6] = acc[6] + r[0];
-            } else {
-                acc[0] = acc[0] - r[1]; acc[1] = acc[1] - r[2]; acc[2] = acc[2] - r[3];
-                acc[3] = acc[3] + r[7]; acc[4] = acc[4] + r[8]; acc[5] = acc[5] + r[9];
+        // the lane's marks of up to APPLY_BATCH entries are loaded together
+        // (one round trip instead of one per entry), then applied in row order
+        constexpr int APPLY_BATCH = 8;
+        for (int q0 = b + l; q0 < e; q0 += APPLY_BATCH * LPP) {
+            int mark[APPLY_BATCH];
+#pragma unroll
+            for (int t = 0; t < APPLY_BATCH; ++t) {
+                const int q = q0 + t * LPP;
+                mark[t] = q < e ? idx_full[q] : 0;
+            }
+#pragma unroll
+            for (int t = 0; t < APPLY_BATCH; ++t) {
+                if (mark[t] == 0) continue;
+                const int q = q0 + t * LPP;
+                idx_full[q] = 0;
+                const double *r = res + (long)(mark[t] - 1) * 10;
+                if (nbr[q] > i) {
+                    acc[0] = acc[0] + r[1]; acc[1] = acc[1] + r[2]; acc[2] = acc[2] + r[3];
+                    acc[3] = acc[3] + r[4]; acc[4] = acc[4] + r[5]; acc[5] = acc[5] + r[6];
+                    acc[6] = acc[6] + r[0];
+                } else {
+                    acc[0] = acc[0] - r[1]; acc[1] = acc[1] - r[2]; acc[2] = acc[2] - r[3];
+                    acc[3] = acc[3] + r[7]; acc[4] = acc[4] + r[8]; acc[5] = acc[5] + r[9];
+                }
             }
         }
     }
