@@ -411,20 +411,19 @@ __global__ void __launch_bounds__(NHT_CHUNK) k_nht(DState s, double dt, int phas
 // the whole sequence).  The particles are cut into at most NHT_CMAX
 // consecutive chunks, one per lane, NHT_BLK chunks per workgroup.  A chunk
 // maps the chain state entering it, s = (p_eta(1:3), sum of m v^2), to the
-// state leaving it; a pass with forward-mode dual numbers gives that map's
+// state leaving it; a walk with forward-mode dual numbers gives that map's
 // value and 4x4 Jacobian at a guessed entry state.  The affine models of the
 // chunks are composed by a two-level prefix scan (a shuffle scan within each
-// workgroup, stored; the workgroup totals scanned at the start of the next
-// pass), which yields the next Newton iterate of every entry state; the
-// iteration converges quadratically to the states the serial walk produces
-// (a few passes at equilibrium), and a last pass applies the updates.  The
-// results equal the serial recurrence to rounding (the particle updates are
-// the same expressions in the same order; only each chunk's entry state is
-// reached through the scan).  Each call is a fixed sequence of launches
-// (graph-capturable): NHT_MAXIT passes that return at once after
-// convergence (the first also sets up the call), final; without convergence
-// the final kernel takes the chunks in windows, one window after another
-// (nht_windows).
+// workgroup, kept; the workgroup totals, published, scanned at the start of
+// the next pass), which yields the next Newton iterate of every entry state;
+// the iteration converges quadratically to the states the serial walk
+// produces (a few passes at equilibrium), and the particle updates follow.
+// The results equal the serial recurrence to rounding (the particle updates
+// are the same expressions in the same order; only each chunk's entry state
+// is reached through the scan).  Each call is one launch (k_nhtp_solve),
+// graph-capturable, whose workgroups meet between passes through flags;
+// without convergence one workgroup takes the chunks in windows, one window
+// after another (nht_windows).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double val(double x) { return x; }
 __device__ __forceinline__ double texp(double x) { return exp(x); }
@@ -560,12 +559,14 @@ __device__ __forceinline__ void nht_turn(int phase, double dt, double tq, double
     }
 }
 
-#define NHT_CMAX 4096    // most chunks per call
-#define NHT_BLK 64       // chunks per workgroup, so at most 64 workgroups
-#define NHT_DIRS 4       // wavefronts per workgroup in a pass: one per derivative direction
-#define NHT_MAXIT 12     // passes per call (the first at the flat guess)
+#define NHT_CMAX 8192    // most chunks per call
+#define NHT_BLK 64       // chunks per workgroup
+#define NHT_WGMAX (NHT_CMAX / NHT_BLK)   // 128 workgroups: two workgroup totals per lane in the scans
+#define NHT_DIRS 4       // wavefronts per workgroup: one per derivative direction
+#define NHT_MAXIT 24     // passes per call (the first at the flat guess); fewer than 63
 #define NHT_TOL 1e-12
 #define NHT_FLOOR 1e-10
+#define NHT_SPIN (1 << 14)   // polls of one wait for the other workgroups before DERR_NHT_SYNC
 struct AffMap {
     double M[16];   // row-major 4x4
     double c[4];
@@ -621,36 +622,80 @@ __device__ __forceinline__ void aff_store(const AffMap &a, double *A)
     for (int k = 0; k < 4; ++k) A[16 + k] = a.c[k];
 }
 
-// inclusive prefix over the lanes of a wavefront, later lanes after earlier
-__device__ __forceinline__ AffMap aff_wave_scan(AffMap x, int lane)
+// A double moved across lanes by a DPP control (gfx9 encodings: row_shr:n
+// 0x110 + n, row_bcast:15 0x142, row_bcast:31 0x143, wave_shr:1 0x138);
+// lanes outside ROWMASK, or whose source lies outside the row or wave,
+// keep `old`.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_d(double v, double old)
 {
-    for (int o = 1; o < SX_WAVE; o <<= 1) {
-        AffMap y;
-        for (int k = 0; k < 16; ++k) y.M[k] = __shfl_up(x.M[k], o, SX_WAVE);
-        for (int k = 0; k < 4; ++k) y.c[k] = __shfl_up(x.c[k], o, SX_WAVE);
-        if (lane >= o) x = aff_compose(x, y);
-    }
+    const long long bv = __double_as_longlong(v), bo = __double_as_longlong(old);
+    const int lo = __builtin_amdgcn_update_dpp((int)bo, (int)bv, CTRL, ROWMASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(bo >> 32), (int)(bv >> 32), CTRL, ROWMASK, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// lane src's double, src wave-uniform
+__device__ __forceinline__ double lane_d(double v, int src)
+{
+    const long long bv = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)bv, src), hi = __builtin_amdgcn_readlane((int)(bv >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// x after the map from the lane the DPP control names (identity where there
+// is none; composing with the identity leaves x exactly as it is)
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ AffMap aff_dpp_step(const AffMap &x)
+{
+    AffMap y;
+    for (int k = 0; k < 16; ++k) y.M[k] = dpp_d<CTRL, ROWMASK>(x.M[k], (k % 5 == 0) ? 1.0 : 0.0);
+    for (int k = 0; k < 4; ++k) y.c[k] = dpp_d<CTRL, ROWMASK>(x.c[k], 0.0);
+    return aff_compose(x, y);
+}
+
+// inclusive prefix over the lanes of a wavefront, later lanes after earlier:
+// within each row of 16 lanes by shifts of 1, 2, 4 and 8, then rows 1 and 3
+// after the last lane of the row before, then rows 2 and 3 after lane 31
+__device__ __forceinline__ AffMap aff_wave_scan(AffMap x)
+{
+    x = aff_dpp_step<0x111, 0xf>(x);
+    x = aff_dpp_step<0x112, 0xf>(x);
+    x = aff_dpp_step<0x114, 0xf>(x);
+    x = aff_dpp_step<0x118, 0xf>(x);
+    x = aff_dpp_step<0x142, 0xa>(x);
+    x = aff_dpp_step<0x143, 0xc>(x);
     return x;
 }
 
-__device__ __forceinline__ AffMap aff_shfl(const AffMap &x, int src)
+// the previous lane's map (the identity in lane 0)
+__device__ __forceinline__ AffMap aff_prev_lane(const AffMap &x)
 {
     AffMap r;
-    for (int k = 0; k < 16; ++k) r.M[k] = __shfl(x.M[k], src, SX_WAVE);
-    for (int k = 0; k < 4; ++k) r.c[k] = __shfl(x.c[k], src, SX_WAVE);
+    for (int k = 0; k < 16; ++k) r.M[k] = dpp_d<0x138, 0xf>(x.M[k], (k % 5 == 0) ? 1.0 : 0.0);
+    for (int k = 0; k < 4; ++k) r.c[k] = dpp_d<0x138, 0xf>(x.c[k], 0.0);
     return r;
 }
 
-// device scratch of the parallel evaluation
+// device scratch of the parallel evaluation.  A workgroup publishes pass k
+// of a call by writing its records and then its flag word for the pass's
+// parity: the tag epoch * 64 + k + 1 (32 bits, compared modulo 2^32) above
+// the largest scaled change of its chunks' entry states in that pass (an
+// FP32 value, which is all the convergence test needs), where epoch counts
+// the calls made with this buffer.  Every workgroup reads the epoch when it
+// starts, and workgroup 0 advances it once every workgroup has published
+// pass 0 (so has started), so flags never need clearing.  A workgroup
+// publishes pass k + 2 only after every workgroup has published pass k + 1,
+// that is after every workgroup has read its pass-k flag.
 struct NhtBuf {
-    double *P[2];    // per chunk: composed models of the chunks before it in its workgroup (20), by pass parity
-    double *T[2];    // per workgroup: composed model of all its chunks (20), by pass parity
-    double *g;       // per chunk: entry state of the latest iterate (4)
-    double *part;    // per workgroup partial sums (4)
-    double *s0;      // chain state entering the call (4), then the state after the last particle (3)
-    unsigned long long *dev;   // per pass: largest scaled change of an entry state (bits of a double >= 0)
-    unsigned *ticket;          // last-workgroup ticket (resets itself)
-    int chunk, nb;             // particles per chunk, workgroups (chunks = nb * NHT_BLK)
+    double *T[2];    // by pass parity: element q of workgroup w's composed model at q * NHT_WGMAX + w
+    double *msum;    // per workgroup: sum m v^2 over its particles (pass 0)
+    double *part;    // per workgroup partial sums of the final reduction (4)
+    double *s0;      // [4..6]: chain state after the last particle
+    unsigned long long *flag;    // [2][NHT_WGMAX]: per parity and workgroup, tag and change
+    unsigned long long *epoch;   // calls evaluated with this buffer
+    unsigned *ticket;            // last-workgroup ticket (resets itself)
+    int chunk, nb;               // particles per chunk, workgroups (chunks = nb * NHT_BLK)
 };
 
 __device__ __forceinline__ void nht_chunk(int n, int chunk, int c, int &i0, int &i1)
@@ -662,35 +707,6 @@ __device__ __forceinline__ void nht_chunk(int n, int chunk, int c, int &i0, int 
 struct NhtArgs {
     double dt, ndofT, Temp, tq, hb[3];
 };
-
-__device__ __forceinline__ NhtArgs nht_args(const double *scal, double dt, double ndof)
-{
-    NhtArgs a;
-    a.dt = dt;
-    a.Temp = scal[SX_TARGET_TEMP];
-    a.hb[0] = scal[SX_HB0]; a.hb[1] = scal[SX_HB1]; a.hb[2] = scal[SX_HB2];
-    a.ndofT = ndof * a.Temp;
-    a.tq = dt / 4.0;
-    return a;
-}
-
-// converged by pass k: pass k - 1 moved no entry state by more than the
-// tolerance, or by less than NHT_FLOOR without shrinking to a quarter of the
-// change of pass k - 2 (the iteration converges quadratically until the
-// rounding of the composed models stops it, and where the chain amplifies
-// that rounding the floor can lie above NHT_TOL).  A pass that returns early
-// leaves its slot at zero, so once a pass has converged every later one
-// finds its predecessor's slot at zero: two loads decide, for any k.
-__device__ __forceinline__ bool nht_converged(const NhtBuf &B, int k)
-{
-    if (k < 2) return false;
-    const double d = __longlong_as_double(
-        (long long)__hip_atomic_load(&B.dev[k - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    const double prev = k >= 3 ? __longlong_as_double((long long)__hip_atomic_load(
-                                     &B.dev[k - 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                               : INFINITY;
-    return d < NHT_TOL || (d < NHT_FLOOR && d > 0.25 * prev);
-}
 
 // the chain masses of the call: the coupling given, else n_dof T tau^2 for
 // the first link and T tau^2 for the others (tau = 2)
@@ -711,146 +727,84 @@ __device__ __forceinline__ void nht_set_hb(double *scal, double ndof, double cou
     scal[SX_HB0] = hb[0]; scal[SX_HB1] = hb[1]; scal[SX_HB2] = hb[2];
 }
 
-// pass k, one chunk per lane.  Pass 0 linearises every chunk at a flat guess
-// (the chain momenta entering the call, sum m v^2 = n_dof T) and meanwhile
-// sets up the call: it clears the per-pass change slots, and its last
-// workgroup stores the state entering the call (the chain momenta and
-// sum m v^2, summed per workgroup in a fixed order) and the chain masses.
-// For k >= 1 the chunk's entry state is first
-// advanced to the Newton iterate given by the chunk models of pass k - 1:
-// the workgroup totals are composed by a shuffle scan (every workgroup
-// redundantly), the result applied to the state entering the call gives this
-// workgroup's entry state, and the stored within-workgroup prefix gives the
-// chunk's; the largest scaled change of an entry state over the grid goes to
-// the pass's slot (converged when below 1e-12 of the magnitude plus the
-// thermal scale, sqrt(Q_k T) for p_eta(k) and n_dof T for sum m v^2; a NaN
-// never converges).  Then four wavefronts walk the chunk's particles with
-// dual numbers, each carrying the derivative along one component of the
-// entry state, which gives the chunk's affine model x -> J x + (E - J g); a
-// shuffle scan over the workgroup's chunks stores each chunk's prefix and the
-// workgroup total.
-__global__ void __launch_bounds__(NHT_BLK * NHT_DIRS) k_nhtp_pass(DState s, double dt, int phase, double *scal,
-                                                                  double ndof, NhtBuf B, int k,
-                                                                  unsigned long long *stats, double coupling,
-                                                                  int has_coupling)
+// largest of two scaled changes; a NaN counts as infinite, so it never converges
+__device__ __forceinline__ double nht_dmax(double x, double y)
 {
-    __shared__ double gsh[NHT_BLK][4], col[NHT_DIRS][NHT_BLK][4], val[NHT_BLK][4];
-    __shared__ int skip;
-    if (nht_converged(B, k)) return;
-    const int b = blockIdx.x, lane = threadIdx.x % NHT_BLK, dir = threadIdx.x / NHT_BLK;
-    const int c = b * NHT_BLK + lane;
-    NhtArgs a = nht_args(scal, dt, ndof);
-    int i0, i1;
-    nht_chunk(s.n, B.chunk, c, i0, i1);
-    double g[4];
-    if (k == 0) {
-        double hb[3] = {0.0, 0.0, 0.0};
-        nht_hb(hb, a.Temp, ndof, coupling, has_coupling);
-        for (int q = 0; q < 3; ++q) a.hb[q] = hb[q];
-        g[0] = scal[SX_PETA0]; g[1] = scal[SX_PETA1]; g[2] = scal[SX_PETA2];
-        g[3] = a.ndofT;
-        double v2[1] = {0.0};
-        if (dir == 0) {
-            for (int q = 0; q < 4; ++q) B.g[c * 4 + q] = g[q];
-            if (b == 0 && lane > 0 && lane < NHT_MAXIT) B.dev[lane] = 0ull;
-            for (int i = i0; i < i1; ++i) {
-                const double vx = s.f[SX_VX][i], vy = s.f[SX_VY][i], vz = s.f[SX_VZ][i];
-                v2[0] += s.f[SX_MASS][i] * (vx * vx + vy * vy + vz * vz);
-            }
+    return (x == x && y == y) ? fmax(x, y) : INFINITY;
+}
+
+__device__ __forceinline__ double nht_wave_dmax(double x)
+{
+    for (int o = SX_WAVE / 2; o > 0; o >>= 1) x = nht_dmax(x, __shfl_xor(x, o, SX_WAVE));
+    return x;
+}
+
+// The records workgroups exchange between passes (totals, changes, sums of
+// m v^2, flags) are written and read only with agent-scope atomic stores
+// and loads, which are coherent across the XCDs' L2 caches by themselves (no
+// L2 write-back or invalidation, which would cost microseconds per pass and
+// evict the particle data).  A workgroup stores its flag only after the
+// stores of its record have completed (s_waitcnt), and a reader issues its
+// record loads only after its flag loads have returned ready, so a reader
+// that sees a flag sees the record published with it.
+__device__ __forceinline__ double nht_ld(const double *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void nht_st(double *p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// workgroup w's model from the element-major totals (a wavefront reading
+// workgroups 2l and 2l + 1 per lane reads 1 KB in a row per element)
+__device__ __forceinline__ AffMap aff_load_shared(const double *T, int w)
+{
+    AffMap r;
+    for (int k = 0; k < 16; ++k) r.M[k] = nht_ld(T + k * NHT_WGMAX + w);
+    for (int k = 0; k < 4; ++k) r.c[k] = nht_ld(T + (16 + k) * NHT_WGMAX + w);
+    return r;
+}
+
+// Waits until every workgroup has published pass `k` (tag >= target modulo
+// 2^32; lane l watches workgroups 2l and 2l + 1, both flags loaded before
+// either is tested) and returns in d the largest change they published;
+// false after NHT_SPIN polls.
+__device__ __forceinline__ bool nht_wait(const NhtBuf &B, int k, unsigned target, int lane, double &d)
+{
+    const int w0 = 2 * lane, w1 = w0 + 1;
+    const unsigned long long *F = B.flag + (k & 1) * NHT_WGMAX;
+    const unsigned long long *f0 = F + min(w0, B.nb - 1), *f1 = F + min(w1, B.nb - 1);
+    for (int it = 0; it < NHT_SPIN; ++it) {
+        const unsigned long long a0 = __hip_atomic_load(f0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long a1 = __hip_atomic_load(f1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const bool ok = (int)((unsigned)(a0 >> 32) - target) >= 0 && (int)((unsigned)(a1 >> 32) - target) >= 0;
+        if (__all(ok)) {
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            d = nht_wave_dmax(nht_dmax((double)__uint_as_float((unsigned)a0), (double)__uint_as_float((unsigned)a1)));
+            return true;
         }
-        double tot[1];
-        if (sx_block_reduce_last<1, NHT_BLK * NHT_DIRS>(v2, B.part, B.ticket, tot) && threadIdx.x == 0) {
-            B.s0[0] = g[0]; B.s0[1] = g[1]; B.s0[2] = g[2];
-            B.s0[3] = tot[0];
-            scal[SX_HB0] = hb[0]; scal[SX_HB1] = hb[1]; scal[SX_HB2] = hb[2];
-            if (stats) atomicAdd(&stats[1], 1ull);
-        }
-    } else {
-        const int pk = (k - 1) & 1;
-        if (dir == 0) {
-            // every load first, so that one round trip serves them all
-            const double s0[4] = {B.s0[0], B.s0[1], B.s0[2], B.s0[3]};
-            AffMap x = lane < B.nb ? aff_load(B.T[pk] + (size_t)lane * 20) : aff_identity();
-            const AffMap Pc = aff_load(B.P[pk] + (size_t)c * 20);
-            const double gold[4] = {B.g[c * 4 + 0], B.g[c * 4 + 1], B.g[c * 4 + 2], B.g[c * 4 + 3]};
-            x = aff_wave_scan(x, lane);
-            double E[4];
-            if (b == 0) {
-                for (int q = 0; q < 4; ++q) E[q] = s0[q];
-            } else {
-                AffMap pre = aff_shfl(x, b - 1);
-                aff_apply(pre, s0, E);
-            }
-            aff_apply(Pc, E, g);
-            const double T = scal[SX_TARGET_TEMP];
-            const double scale[4] = {sqrt(fabs(scal[SX_HB0] * T)), sqrt(fabs(scal[SX_HB1] * T)),
-                                     sqrt(fabs(scal[SX_HB2] * T)), fabs(ndof * T)};
-            double dv = 0.0;
-            for (int q = 0; q < 4; ++q) {
-                double d = fabs(g[q] - gold[q]) / (fabs(g[q]) + scale[q] + 1e-300);
-                dv = (d == d && dv == dv) ? fmax(dv, d) : INFINITY;
-                B.g[c * 4 + q] = g[q];
-                gsh[lane][q] = g[q];
-            }
-            for (int o = SX_WAVE / 2; o > 0; o >>= 1) {
-                double y = __shfl_xor(dv, o, SX_WAVE);
-                dv = (y == y && dv == dv) ? fmax(dv, y) : INFINITY;
-            }
-            if (lane == 0) {
-                atomicMax(&B.dev[k], (unsigned long long)__double_as_longlong(dv));
-                if (b == 0 && stats) atomicAdd(&stats[0], 1ull);
-                skip = dv < NHT_TOL;
-            }
-            if (dv < NHT_TOL) {
-                // no entry state of this workgroup moved: the models of pass
-                // k - 1, linearised at the same states, stand for this pass's
-                const double *Pp = B.P[pk] + (size_t)c * 20;
-                double *Pn = B.P[k & 1] + (size_t)c * 20;
-                for (int q = 0; q < 20; ++q) Pn[q] = Pp[q];
-                if (lane == NHT_BLK - 1) {
-                    const double *Tp = B.T[pk] + (size_t)b * 20;
-                    double *Tn = B.T[k & 1] + (size_t)b * 20;
-                    for (int q = 0; q < 20; ++q) Tn[q] = Tp[q];
-                }
-            }
-        }
-        __syncthreads();
-        if (skip) return;
-        for (int q = 0; q < 4; ++q) g[q] = gsh[lane][q];
+        __builtin_amdgcn_s_sleep(1);
     }
-    // the walk: wavefront dir carries the derivative along entry component dir
-    D1 p[3], sv;
-    for (int q = 0; q < 3; ++q) p[q] = D1{g[q], q == dir ? 1.0 : 0.0};
-    sv = D1{g[3], dir == 3 ? 1.0 : 0.0};
-    for (int i = i0; i < i1; ++i) {
-        double m = s.f[SX_MASS][i], im = s.f[SX_INV_MASS][i];
-        double acc3[3] = {0.5 * dt * (s.f[SX_FX][i] * im), 0.5 * dt * (s.f[SX_FY][i] * im),
-                          0.5 * dt * (s.f[SX_FZ][i] * im)};
-        D1 v[3] = {D1{s.f[SX_VX][i], 0.0}, D1{s.f[SX_VY][i], 0.0}, D1{s.f[SX_VZ][i], 0.0}};
-        D1 ea[3];
-        nht_turn(phase, dt, a.tq, a.ndofT, a.Temp, a.hb, m, acc3, p, sv, v, ea);
-    }
-    const D1 out[4] = {p[0], p[1], p[2], sv};
-    for (int r = 0; r < 4; ++r) {
-        col[dir][lane][r] = out[r].d;
-        if (dir == 0) val[lane][r] = out[r].v;
-    }
-    __syncthreads();
-    if (dir != 0) return;
-    AffMap A;
-    for (int r = 0; r < 4; ++r) {
-        double cr = val[lane][r];
-        for (int q = 0; q < 4; ++q) {
-            A.M[r * 4 + q] = col[q][lane][r];
-            cr -= col[q][lane][r] * g[q];
-        }
-        A.c[r] = cr;
-    }
-    AffMap incl = aff_wave_scan(A, lane);
-    AffMap excl = aff_shfl(incl, lane > 0 ? lane - 1 : 0);
-    if (lane == 0) excl = aff_identity();
-    aff_store(excl, B.P[k & 1] + (size_t)c * 20);
-    if (lane == NHT_BLK - 1) aff_store(incl, B.T[k & 1] + (size_t)b * 20);
+    return false;
+}
+
+// workgroup b's pass k: its total model, (pass 0) its sum of m v^2; then,
+// once those stores have completed, the flag word with its largest change
+__device__ __forceinline__ void nht_publish(const NhtBuf &B, int b, int k, unsigned ep, const double *Tw,
+                                            double dv, double msum)
+{
+    // the parity slot picked by a select: an indexed read of the argument
+    // struct would be a memory load waited for before the stores
+    double *T = ((k & 1) ? B.T[1] : B.T[0]) + b;
+    for (int q = 0; q < 20; ++q) nht_st(T + q * NHT_WGMAX, Tw[q]);
+    if (k == 0) nht_st(B.msum + b, msum);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    const unsigned long long word = ((unsigned long long)(ep + (unsigned)(k + 1)) << 32) | __float_as_uint((float)dv);
+    __hip_atomic_store(B.flag + (k & 1) * NHT_WGMAX + b, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // particle i's turn of the chain call from chain state (p, sv), with its
@@ -881,28 +835,28 @@ __device__ __forceinline__ void nht_particle(const DState &s, int i, int phase, 
 #define NHT_WIN_IT 16   // Newton iterations of one window before its serial walk
 
 // Without convergence of the passes (a call whose chain swings far within
-// it): the first workgroup takes the chunks in windows of NHT_BLK, one window
-// after another from the exact state entering it.  Within a window it runs
-// Newton from the flat guess over the window's chunks, one per lane, with
-// dual numbers carrying all four derivatives and one shuffle scan per
-// iteration (a window spans a short stretch of the chain's evolution, so a
-// few iterations converge); a window that has not converged after NHT_WIN_IT
-// iterations is walked serially by one lane.  Then the window's particles
-// are updated from the converged entry states and the state after its last
-// chunk enters the next window.
-__device__ void nht_windows(const DState &s, int phase, int save, double *scal, double ndof, const NhtBuf &B,
-                            const NhtArgs &a)
+// it): one wavefront takes the chunks in windows of NHT_BLK, one window
+// after another from the exact state entering it (s0 for the first).
+// Within a window it runs Newton from the flat guess over the window's
+// chunks, one per lane, with dual numbers carrying all four derivatives and
+// one shuffle scan per iteration (a window spans a short stretch of the
+// chain's evolution, so a few iterations converge); a window that has not
+// converged after NHT_WIN_IT iterations is walked serially by one lane.
+// Then the window's particles are updated from the converged entry states
+// and the state after its last chunk enters the next window.
+__device__ void nht_windows(const DState &s, int phase, int save, double *scal, double ndof, int chunk,
+                            const NhtArgs &a, const double s0[4])
 {
     const int lane = threadIdx.x;
-    const int nchunks = (s.n + B.chunk - 1) / B.chunk;
+    const int nchunks = (s.n + chunk - 1) / chunk;
     const double T = a.Temp;
     const double scale[4] = {sqrt(fabs(a.hb[0] * T)), sqrt(fabs(a.hb[1] * T)), sqrt(fabs(a.hb[2] * T)),
                              fabs(ndof * T)};
-    double e[4] = {B.s0[0], B.s0[1], B.s0[2], B.s0[3]};
+    double e[4] = {s0[0], s0[1], s0[2], s0[3]};
     double v4[4] = {0.0, 0.0, 0.0, 0.0};
     for (int w0 = 0; w0 < nchunks; w0 += NHT_BLK) {
         int i0, i1;
-        nht_chunk(s.n, B.chunk, w0 + lane, i0, i1);
+        nht_chunk(s.n, chunk, w0 + lane, i0, i1);
         double g[4] = {e[0], e[1], e[2], e[3]};
         bool ok = false;
         for (int it = 0; it < NHT_WIN_IT && !ok; ++it) {
@@ -928,22 +882,16 @@ __device__ void nht_windows(const DState &s, int phase, int save, double *scal, 
                 }
                 A.c[r] = cr;
             }
-            AffMap incl = aff_wave_scan(A, lane);
-            AffMap excl = aff_shfl(incl, lane > 0 ? lane - 1 : 0);
-            if (lane == 0) excl = aff_identity();
+            const AffMap incl = aff_wave_scan(A);
+            const AffMap excl = aff_prev_lane(incl);
             double gn[4];
             aff_apply(excl, e, gn);
             double dv = 0.0;
             for (int q = 0; q < 4; ++q) {
-                const double d = fabs(gn[q] - g[q]) / (fabs(gn[q]) + scale[q] + 1e-300);
-                dv = (d == d && dv == dv) ? fmax(dv, d) : INFINITY;
+                dv = nht_dmax(dv, fabs(gn[q] - g[q]) / (fabs(gn[q]) + scale[q] + 1e-300));
                 g[q] = gn[q];
             }
-            for (int o = SX_WAVE / 2; o > 0; o >>= 1) {
-                const double y = __shfl_xor(dv, o, SX_WAVE);
-                dv = (y == y && dv == dv) ? fmax(dv, y) : INFINITY;
-            }
-            ok = dv < NHT_TOL;
+            ok = nht_wave_dmax(dv) < NHT_TOL;
         }
         const int last = min(nchunks, w0 + NHT_BLK) - 1 - w0;   // lane of the window's last chunk
         double x[4] = {0.0, 0.0, 0.0, 0.0};
@@ -951,18 +899,18 @@ __device__ void nht_windows(const DState &s, int phase, int save, double *scal, 
             double p[3] = {g[0], g[1], g[2]}, sv = g[3];
             for (int i = i0; i < i1; ++i) nht_particle(s, i, phase, save, a, p, sv, v4);
             x[0] = p[0]; x[1] = p[1]; x[2] = p[2]; x[3] = sv;
-            for (int q = 0; q < 4; ++q) e[q] = __shfl(x[q], last, SX_WAVE);
+            for (int q = 0; q < 4; ++q) e[q] = lane_d(x[q], last);
         } else {
             int j0, j1, k0, k1;
-            nht_chunk(s.n, B.chunk, w0, j0, k0);
-            nht_chunk(s.n, B.chunk, w0 + last, k1, j1);
+            nht_chunk(s.n, chunk, w0, j0, k0);
+            nht_chunk(s.n, chunk, w0 + last, k1, j1);
             (void)k0; (void)k1;
             if (lane == 0) {
                 double p[3] = {e[0], e[1], e[2]}, sv = e[3];
                 for (int i = j0; i < j1; ++i) nht_particle(s, i, phase, save, a, p, sv, v4);
                 x[0] = p[0]; x[1] = p[1]; x[2] = p[2]; x[3] = sv;
             }
-            for (int q = 0; q < 4; ++q) e[q] = __shfl(x[q], 0, SX_WAVE);
+            for (int q = 0; q < 4; ++q) e[q] = lane_d(x[q], 0);
         }
     }
     for (int o = SX_WAVE / 2; o > 0; o >>= 1)
@@ -974,39 +922,208 @@ __device__ void nht_windows(const DState &s, int phase, int save, double *scal, 
     }
 }
 
-// converged by pass `passes`: the particle updates of every chunk from the
-// converged entry state, then the sums of the eta increments and kinetic
-// energy (fixed order) and the chain state after the last particle.  Not
-// converged: the windows above.
-__global__ void __launch_bounds__(NHT_BLK) k_nhtp_final(DState s, double dt, int phase, int save, double *scal,
-                                                        double ndof, NhtBuf B, unsigned long long *stats,
-                                                        int passes)
+// One chain call in one launch of nb <= NHT_WGMAX workgroups, which the
+// launch needs resident together (at most 128 of the 256 CUs).  Pass k:
+// - wavefront 0 waits until every workgroup has published pass k - 1;
+// - for k >= 2 it decides convergence from the largest changes published in
+//   passes k - 1 and k - 2: pass k - 1 moved no entry state by more than the
+//   tolerance, or by less than NHT_FLOOR without shrinking to a quarter of
+//   the change of pass k - 2 (the iteration converges quadratically until
+//   the rounding of the composed models stops it, and where the chain
+//   amplifies that rounding the floor can lie above NHT_TOL).  Every
+//   workgroup reads the same published values, so all decide alike;
+// - otherwise it composes the workgroup totals of pass k - 1 by a shuffle
+//   scan (two per lane), applies the prefix to the state entering the call
+//   and then the chunk's own prefix (kept in registers): the next Newton
+//   iterate of every entry state.  Pass 0 starts from the flat guess (the
+//   chain momenta entering the call, sum m v^2 = n_dof T) and sums m v^2 per
+//   workgroup, which pass 1 adds in a fixed order into the state entering
+//   the call;
+// - a workgroup none of whose entry states moved by more than the tolerance
+//   keeps its models; the others walk their chunks with dual numbers, four
+//   wavefronts each carrying the derivative along one component of the entry
+//   state, which gives every chunk's affine model x -> J x + (E - J g), and a
+//   shuffle scan over the workgroup's chunks gives each chunk's prefix and
+//   the workgroup total, which is published.
+// Converged: the particle updates of every chunk from its entry state with
+// the serial recurrence's expressions in its order, then the sums of the eta
+// increments and kinetic energy (fixed order) and the chain state after the
+// last particle.  Not converged after `maxit` passes: workgroup 0 takes the
+// chunks in windows (nht_windows).  A wait that outlasts NHT_SPIN polls (a
+// workgroup that never became resident) raises DERR_NHT_SYNC and ends the
+// workgroup; every wait is bounded, so the launch always ends.
+__global__ void __launch_bounds__(NHT_BLK * NHT_DIRS) k_nhtp_solve(DState s, double dt, int phase, int save,
+                                                                   double *scal, double ndof, NhtBuf B, int maxit,
+                                                                   unsigned long long *stats, double coupling,
+                                                                   int has_coupling, int *err)
 {
-    const int b = blockIdx.x, t = threadIdx.x, c = b * NHT_BLK + t;
-    const NhtArgs a = nht_args(scal, dt, ndof);
-    if (!nht_converged(B, passes)) {
-        if (b != 0) return;
-        if (t == 0 && stats) atomicAdd(&stats[2], 1ull);
-        nht_windows(s, phase, save, scal, ndof, B, a);
-        return;
-    }
+    __shared__ double gsh[NHT_BLK][4], col[NHT_DIRS][NHT_BLK][4], val[NHT_BLK][4];
+    __shared__ double psh[NHT_BLK][20], tsh[20];   // each chunk's prefix within the workgroup, the total
+    __shared__ int ctl_sh[2];
+    const int b = blockIdx.x, lane = threadIdx.x % NHT_BLK, dir = threadIdx.x / NHT_BLK;
+    const int c = b * NHT_BLK + lane;
+    NhtArgs a;
+    a.dt = dt;
+    a.Temp = scal[SX_TARGET_TEMP];
+    a.ndofT = ndof * a.Temp;
+    a.tq = dt / 4.0;
+    nht_hb(a.hb, a.Temp, ndof, coupling, has_coupling);
     int i0, i1;
     nht_chunk(s.n, B.chunk, c, i0, i1);
-    double p[3] = {B.g[c * 4 + 0], B.g[c * 4 + 1], B.g[c * 4 + 2]}, sv = B.g[c * 4 + 3];
+    const unsigned long long epoch = B.epoch[0];
+    const unsigned ep = (unsigned)(epoch * 64ull);
+    // wavefront 0: the chunk's entry state, the state entering the call
+    double g[4] = {scal[SX_PETA0], scal[SX_PETA1], scal[SX_PETA2], a.ndofT};
+    double s0[4] = {g[0], g[1], g[2], 0.0};
+    double dprev = INFINITY, msum = 0.0, dvk = 0.0;
+    const double scale[4] = {sqrt(fabs(a.hb[0] * a.Temp)), sqrt(fabs(a.hb[1] * a.Temp)),
+                             sqrt(fabs(a.hb[2] * a.Temp)), fabs(ndof * a.Temp)};
+    int ctl;   // 0 walk, 1 keep the models, 2 converged, 3 not converged, 4 waited in vain
+    for (int k = 0;; ++k) {
+        if (dir == 0) {
+            ctl = 0;
+            dvk = 0.0;
+            double d = INFINITY;
+            if (k == 0) {
+                for (int i = i0; i < i1; ++i) {
+                    const double vx = s.f[SX_VX][i], vy = s.f[SX_VY][i], vz = s.f[SX_VZ][i];
+                    msum += s.f[SX_MASS][i] * (vx * vx + vy * vy + vz * vz);
+                }
+                for (int o = SX_WAVE / 2; o > 0; o >>= 1) msum = msum + __shfl_xor(msum, o, SX_WAVE);
+                if (b == 0 && lane == 0 && stats) atomicAdd(&stats[1], 1ull);
+            } else if (!nht_wait(B, k - 1, ep + (unsigned)k, lane, d)) {
+                if (lane == 0) atomicCAS(err, DERR_NONE, DERR_NHT_SYNC);
+                ctl = 4;
+            } else {
+                if (k == 1) d = INFINITY;   // pass 0 has no previous iterate
+                // converged (see above), or the observed contraction makes the
+                // next step smaller than the tolerance: d * (d / dprev), the
+                // estimate of a linearly converging iteration, which bounds a
+                // quadratically converging one (k >= 3: both changes measured)
+                const bool conv = k >= 2 && (d < NHT_TOL || (d < NHT_FLOOR && d > 0.25 * dprev) ||
+                                             (k >= 3 && d < 1e-3 * dprev && d * (d / dprev) < NHT_TOL));
+                dprev = d;
+                if (conv) ctl = 2;
+                else if (k == maxit) ctl = 3;
+            }
+            // the models of pass k - 1 for the next iterate; in pass 1 also the
+            // sums of m v^2 (needed by the windows too, when one pass is all)
+            if (k >= 1 && (ctl == 0 || (k == 1 && ctl == 3))) {
+                const int w0 = 2 * lane, w1 = w0 + 1;
+                const double *Tp = ((k - 1) & 1) ? B.T[1] : B.T[0];
+                const bool ld = ctl == 0;
+                const AffMap t0 = ld && w0 < B.nb ? aff_load_shared(Tp, w0) : aff_identity();
+                const AffMap t1 = ld && w1 < B.nb ? aff_load_shared(Tp, w1) : aff_identity();
+                if (k == 1) {
+                    double m2 = (w0 < B.nb ? nht_ld(B.msum + w0) : 0.0) + (w1 < B.nb ? nht_ld(B.msum + w1) : 0.0);
+                    for (int o = SX_WAVE / 2; o > 0; o >>= 1) m2 = m2 + __shfl_xor(m2, o, SX_WAVE);
+                    s0[3] = m2;
+                    if (b == 0 && lane == 0) {
+                        scal[SX_HB0] = a.hb[0]; scal[SX_HB1] = a.hb[1]; scal[SX_HB2] = a.hb[2];
+                        B.epoch[0] = epoch + 1ull;   // every workgroup has started
+                    }
+                }
+                if (ctl == 0) {
+                    const AffMap x = aff_wave_scan(aff_compose(t1, t0));
+                    double e[4], eprev[4], f[4], E[4], gn[4];
+                    aff_apply(x, s0, e);   // state after workgroups 0 .. 2 lane + 1
+                    for (int q = 0; q < 4; ++q) eprev[q] = dpp_d<0x138, 0xf>(e[q], s0[q]);
+                    aff_apply(t0, eprev, f);   // state after workgroups 0 .. 2 lane
+                    for (int q = 0; q < 4; ++q) E[q] = lane_d((b & 1) ? f[q] : eprev[q], b >> 1);
+                    aff_apply(aff_load(psh[lane]), E, gn);
+                    double dv = 0.0;
+                    for (int q = 0; q < 4; ++q) {
+                        dv = nht_dmax(dv, fabs(gn[q] - g[q]) / (fabs(gn[q]) + scale[q] + 1e-300));
+                        g[q] = gn[q];
+                    }
+                    dvk = nht_wave_dmax(dv);
+                    ctl = dvk < NHT_TOL ? 1 : 0;
+                    if (b == 0 && lane == 0 && stats) atomicAdd(&stats[0], 1ull);
+                }
+            }
+            for (int q = 0; q < 4; ++q) gsh[lane][q] = g[q];
+            if (lane == 0) ctl_sh[k & 1] = ctl;
+        }
+        __syncthreads();
+        ctl = ctl_sh[k & 1];
+        if (ctl >= 2) break;
+        if (ctl == 0) {
+            // the walk: wavefront dir carries the derivative along entry component dir
+            D1 p[3], sv;
+            for (int q = 0; q < 3; ++q) p[q] = D1{gsh[lane][q], q == dir ? 1.0 : 0.0};
+            sv = D1{gsh[lane][3], dir == 3 ? 1.0 : 0.0};
+            for (int i = i0; i < i1; ++i) {
+                double m = s.f[SX_MASS][i], im = s.f[SX_INV_MASS][i];
+                double acc3[3] = {0.5 * dt * (s.f[SX_FX][i] * im), 0.5 * dt * (s.f[SX_FY][i] * im),
+                                  0.5 * dt * (s.f[SX_FZ][i] * im)};
+                D1 v[3] = {D1{s.f[SX_VX][i], 0.0}, D1{s.f[SX_VY][i], 0.0}, D1{s.f[SX_VZ][i], 0.0}};
+                D1 ea[3];
+                nht_turn(phase, dt, a.tq, a.ndofT, a.Temp, a.hb, m, acc3, p, sv, v, ea);
+            }
+            const D1 out[4] = {p[0], p[1], p[2], sv};
+            for (int r = 0; r < 4; ++r) {
+                col[dir][lane][r] = out[r].d;
+                if (dir == 0) val[lane][r] = out[r].v;
+            }
+            __syncthreads();
+            if (dir == 0) {
+                AffMap A;
+                for (int r = 0; r < 4; ++r) {
+                    double cr = val[lane][r];
+                    for (int q = 0; q < 4; ++q) {
+                        A.M[r * 4 + q] = col[q][lane][r];
+                        cr -= col[q][lane][r] * g[q];
+                    }
+                    A.c[r] = cr;
+                }
+                const AffMap incl = aff_wave_scan(A);
+                aff_store(aff_prev_lane(incl), psh[lane]);
+                if (lane == NHT_BLK - 1) aff_store(incl, tsh);
+            }
+        }
+        // published by the lane that stored the total (its own LDS writes)
+        if (dir == 0 && lane == NHT_BLK - 1) nht_publish(B, b, k, ep, tsh, dvk, msum);
+    }
+    if (ctl == 4) return;
+    if (ctl == 3) {
+        if (b == 0 && dir == 0) {
+            if (lane == 0 && stats) atomicAdd(&stats[2], 1ull);
+            nht_windows(s, phase, save, scal, ndof, B.chunk, a, s0);
+        }
+        return;
+    }
+    if (dir != 0) return;
     double v4[4] = {0.0, 0.0, 0.0, 0.0};   // eta increments, kinetic energy
+    double p[3] = {g[0], g[1], g[2]}, sv = g[3];
     for (int i = i0; i < i1; ++i) nht_particle(s, i, phase, save, a, p, sv, v4);
     // the chain state after the last particle belongs to the last non-empty chunk
-    if (i1 == s.n && i0 < s.n) {
-        for (int q = 0; q < 3; ++q) B.s0[4 + q] = p[q];
-        __threadfence();
-    }
+    if (i1 == s.n && i0 < s.n)
+        for (int q = 0; q < 3; ++q) nht_st(B.s0 + 4 + q, p[q]);
+    for (int o = SX_WAVE / 2; o > 0; o >>= 1)
+        for (int q = 0; q < 4; ++q) v4[q] = v4[q] + __shfl_xor(v4[q], o, SX_WAVE);
+    if (lane == 0)
+        for (int q = 0; q < 4; ++q) nht_st(B.part + b * 4 + q, v4[q]);
+    // the sums by the last workgroup to arrive (same protocol as the passes:
+    // the stores complete before the ticket, the loads follow it)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    unsigned t = 0;
+    if (lane == 0) t = __hip_atomic_fetch_add(B.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    t = __shfl(t, 0, SX_WAVE);
+    if (t != (unsigned)(B.nb - 1)) return;
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    const int w0 = 2 * lane, w1 = w0 + 1;
     double tot[4];
-    if (!sx_block_reduce_last<4, NHT_BLK>(v4, B.part, B.ticket, tot)) return;
-    if (t == 0) {
+    for (int q = 0; q < 4; ++q) {
+        tot[q] = (w0 < B.nb ? nht_ld(B.part + w0 * 4 + q) : 0.0) + (w1 < B.nb ? nht_ld(B.part + w1 * 4 + q) : 0.0);
+        for (int o = SX_WAVE / 2; o > 0; o >>= 1) tot[q] = tot[q] + __shfl_xor(tot[q], o, SX_WAVE);
+    }
+    if (lane == 0) {
         for (int q = 0; q < 3; ++q) scal[SX_ETA0 + q] = scal[SX_ETA0 + q] + tot[q];
         if (phase == 1) scal[SX_KIN_ENG] = tot[3];
-        for (int q = 0; q < 3; ++q)
-            scal[SX_PETA0 + q] = __hip_atomic_load(&B.s0[4 + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int q = 0; q < 3; ++q) scal[SX_PETA0 + q] = nht_ld(B.s0 + 4 + q);
+        __hip_atomic_store(B.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -1464,6 +1581,38 @@ void sx_lgv_final_impl(sx_ctx *c, double dt, double damping, bool constrained, b
     }
 }
 
+// the chain's device scratch: doubles (2 x workgroup totals, sums of m v^2,
+// partial sums, end state), then the flags (2 x) and the epoch (u64), then
+// the ticket; the flags, epoch and ticket start at zero
+static const size_t NHT_ND = 2 * NHT_WGMAX * 20 + NHT_WGMAX + NHT_WGMAX * 4 + 8;
+static const size_t NHT_SYNC_BYTES = sizeof(unsigned long long) * (2 * NHT_WGMAX + 1) + sizeof(unsigned) * 4;
+
+static NhtBuf nht_buffers(sx_ctx *c)
+{
+    if (!c->d_nht) {
+        HIPCHK(hipMalloc(&c->d_nht, sizeof(double) * NHT_ND + NHT_SYNC_BYTES));
+        HIPCHK(hipMemsetAsync(c->d_nht + NHT_ND, 0, NHT_SYNC_BYTES, c->stream));
+    }
+    NhtBuf B;
+    B.T[0] = c->d_nht;
+    B.T[1] = B.T[0] + NHT_WGMAX * 20;
+    B.msum = B.T[1] + NHT_WGMAX * 20;
+    B.part = B.msum + NHT_WGMAX;
+    B.s0 = B.part + NHT_WGMAX * 4;
+    B.flag = (unsigned long long *)(c->d_nht + NHT_ND);
+    B.epoch = B.flag + 2 * NHT_WGMAX;
+    B.ticket = (unsigned *)(B.epoch + 1);
+    B.chunk = B.nb = 0;
+    return B;
+}
+
+// after DERR_NHT_SYNC: a call whose workgroups gave up waiting left flags,
+// epoch and ticket mid-call
+void sx_nht_reset_sync(sx_ctx *c)
+{
+    if (c->d_nht) HIPCHK(hipMemsetAsync(c->d_nht + NHT_ND, 0, NHT_SYNC_BYTES, c->stream));
+}
+
 void sx_nht_impl(sx_ctx *c, double dt, const double *coupling, int phase, bool save)
 {
     double ndof = (double)(3 * (c->n / c->nrigid) - 3);
@@ -1476,38 +1625,18 @@ void sx_nht_impl(sx_ctx *c, double dt, const double *coupling, int phase, bool s
         hipLaunchKernelGGL(k_nht, dim3(1), dim3(NHT_CHUNK), 0, c->stream, c->dstate(), dt, phase, save ? 1 : 0,
                            c->d_scal, ndof, c->d_part);
     } else {
-        // 2 x chunk prefixes, 2 x workgroup totals, entry states, partial
-        // sums, s0 + end state, per-pass change slots, then the ticket
-        const size_t nP = (size_t)NHT_CMAX * 20, nT = (size_t)(NHT_CMAX / NHT_BLK) * 20;
-        const size_t nd = 2 * nP + 2 * nT + (size_t)NHT_CMAX * 4 + (NHT_CMAX / NHT_BLK) * 4 + 8 + 16;
-        if (!c->d_nht) {
-            HIPCHK(hipMalloc(&c->d_nht, sizeof(double) * nd + sizeof(unsigned) * 4));
-            HIPCHK(hipMemsetAsync(c->d_nht + nd, 0, sizeof(unsigned) * 4, c->stream));
-        }
-        NhtBuf B;
-        B.P[0] = c->d_nht;
-        B.P[1] = B.P[0] + nP;
-        B.T[0] = B.P[1] + nP;
-        B.T[1] = B.T[0] + nT;
-        B.g = B.T[1] + nT;
-        B.part = B.g + (size_t)NHT_CMAX * 4;
-        B.s0 = B.part + (NHT_CMAX / NHT_BLK) * 4;
-        B.dev = (unsigned long long *)(B.s0 + 8);
-        B.ticket = (unsigned *)(c->d_nht + nd);
+        NhtBuf B = nht_buffers(c);
         B.chunk = (c->n + NHT_CMAX - 1) / NHT_CMAX;
         const int nchunks = (c->n + B.chunk - 1) / B.chunk;
         B.nb = (nchunks + NHT_BLK - 1) / NHT_BLK;
         unsigned long long *st = (unsigned long long *)(c->d_counts + 4);
-        DState s = c->dstate();
         // SUCROSE_NHT_PASSES (1..NHT_MAXIT) caps the passes, so that tests can
-        // drive the final kernel's windows from the host
+        // drive the windows from the host
         int passes = NHT_MAXIT;
         if (const char *e = getenv("SUCROSE_NHT_PASSES")) passes = std::max(1, std::min(NHT_MAXIT, atoi(e)));
-        for (int k = 0; k < passes; ++k)
-            hipLaunchKernelGGL(k_nhtp_pass, dim3(B.nb), dim3(NHT_BLK * NHT_DIRS), 0, c->stream, s, dt, phase, c->d_scal, ndof,
-                               B, k, st, coupling ? *coupling : 0.0, coupling ? 1 : 0);
-        hipLaunchKernelGGL(k_nhtp_final, dim3(B.nb), dim3(NHT_BLK), 0, c->stream, s, dt, phase, save ? 1 : 0,
-                           c->d_scal, ndof, B, st, passes);
+        hipLaunchKernelGGL(k_nhtp_solve, dim3(B.nb), dim3(NHT_BLK * NHT_DIRS), 0, c->stream, c->dstate(), dt, phase,
+                           save ? 1 : 0, c->d_scal, ndof, B, passes, st, coupling ? *coupling : 0.0, coupling ? 1 : 0,
+                           c->d_err);
     }
 }
 

@@ -36,7 +36,7 @@ struct sx_error {
     } while (0)
 
 // device error word codes (first error wins)
-enum { DERR_NONE = 0, DERR_NLIST_CAP = 1, DERR_ROW_CAP = 2, DERR_CONSTRAINT = 3, DERR_ECP_CAP = 4 };
+enum { DERR_NONE = 0, DERR_NLIST_CAP = 1, DERR_ROW_CAP = 2, DERR_CONSTRAINT = 3, DERR_ECP_CAP = 4, DERR_NHT_SYNC = 5 };
 
 // ---------------------------------------------------------------------------
 // state passed to kernels by value
@@ -254,6 +254,7 @@ void sx_vv_initial_impl(sx_ctx *c, double dt, bool save);
 void sx_vv_final_impl(sx_ctx *c, double dt, bool thermo = false);
 void sx_qq_initial_impl(sx_ctx *c, double dt);
 void sx_qq_step_begin_impl(sx_ctx *c, double dt);   // qq_initial + rebuild flag + initialize_forces
+void sx_nht_reset_sync(sx_ctx *c);   // clears the chain's call flags after DERR_NHT_SYNC
 void sx_qq_final_impl(sx_ctx *c, double dt, bool thermo = false);
 void sx_nht_impl(sx_ctx *c, double dt, const double *coupling, int phase, bool save);
 void sx_rattle_impl(sx_ctx *c, double dt, int phase);
