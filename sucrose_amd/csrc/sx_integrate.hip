@@ -999,9 +999,12 @@ __global__ void __launch_bounds__(NHT_BLK * NHT_DIRS) k_nhtp_solve(DState s, dou
                 // converged (see above), or the observed contraction makes the
                 // next step smaller than the tolerance: d * (d / dprev), the
                 // estimate of a linearly converging iteration, which bounds a
-                // quadratically converging one (k >= 3: both changes measured)
+                // quadratically converging one.  Only once both changes are
+                // measured (k >= 3) and the previous one was already below the
+                // thermal scale: after a diverging pass (dprev infinite) the
+                // estimate says nothing.
                 const bool conv = k >= 2 && (d < NHT_TOL || (d < NHT_FLOOR && d > 0.25 * dprev) ||
-                                             (k >= 3 && d < 1e-3 * dprev && d * (d / dprev) < NHT_TOL));
+                                             (k >= 3 && dprev < 1.0 && d < 1e-3 * dprev && d * (d / dprev) < NHT_TOL));
                 dprev = d;
                 if (conv) ctl = 2;
                 else if (k == maxit) ctl = 3;

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from oracle.oracle import (COULOMB, DH, ECP, ECPC, ECPR, ECPW, ECPWC, GB, GBC, GBR, GBW, GBWC, LGV, LGVC, LJ, LJ12, LJW, MORSE,
-                           NHT, QQ, VV, VVC, WCA)
+                           NHT, QQ, VV, VVC, WCA, State)
 
 ECP_KINDS = (ECP, ECPR, ECPW, ECPC, ECPWC)
 GB_ANISO = (GB, GBR, GBW, GBC, GBWC) + ECP_KINDS   # kinds with torques
@@ -392,12 +392,12 @@ def test_nose_hoover_far_from_equilibrium(dev, oracle):
 @pytest.mark.parametrize("phase", [0, 1])
 @pytest.mark.parametrize("far", [False, True])
 def test_nose_hoover_windows(dev, oracle, phase, far, monkeypatch):
-    """The final kernel's evaluation of a call whose Newton passes did not
-    converge: the chunks in windows of 64, each window from the exact state
-    entering it, Newton within the window (or its serial walk).  The passes
-    are capped at one from the host (SUCROSE_NHT_PASSES), so every call takes
-    this path; it must reproduce the oracle's serial walk, near equilibrium
-    and far from it."""
+    """The evaluation of a call whose Newton passes did not converge: the
+    chunks in windows of 64, each window from the exact state entering it,
+    Newton within the window (or its serial walk).  The passes are capped at
+    one from the host (SUCROSE_NHT_PASSES), so every call takes this path; it
+    must reproduce the oracle's serial walk, near equilibrium and far from
+    it."""
     say(f"nose-hoover windows n=6000 phase={phase} far={far}")
     monkeypatch.setenv("SUCROSE_NHT_PASSES", "1")
     if far:
@@ -413,6 +413,55 @@ def test_nose_hoover_windows(dev, oracle, phase, far, monkeypatch):
     _nht_call_matches(dev, oracle, st, phase, dt=dt)
     it1, calls1, fb1 = dev.nht_stats()
     assert calls1 - calls0 == 1 and fb1 - fb0 == 1
+
+
+def test_nose_hoover_thermal_regime(dev, oracle):
+    """The Gay-Berne + Nose-Hoover workload (8000 ellipsoids, dt 0.0015) run
+    into the chain's thermal regime: from step ~4000 p_eta(1) is near its
+    thermal scale and one call swings it by its own size, so calls need many
+    Newton updates, some diverge and the windows take over.  Every chain call
+    of 110 steps there, each from the device's own state, must reproduce the
+    oracle's serial walk (a call accepted right after a diverging pass once
+    came out NaN in this stretch)."""
+    import sucrose_amd as sx
+    from sucrose_amd.workloads import gb_nhc_state
+    say("nose-hoover thermal regime, gb_nhc_8000 steps 5150-5260")
+    p = gb_nhc_state(dev, log=lambda *a: None)
+    gb = (sx.GB, p["cutoff"], p["mu"], p["nu"])
+    dt, step = p["dt"], p["next_step"]
+    dev.run(sx.INT_NHT, [gb], 5150, dt, first_step=step)
+    n = dev.download("rx").size
+    tmpl = State(n, np.array(dev.get_box()[:3]))
+    tmpl["target_temp"] = dev.get_scalar("TARGET_TEMP")
+    tmpl["dof"] = dev.get_scalar("DOF")
+    it0, calls0, fb0 = dev.nht_stats()
+    worst_v = worst_chain = 0.0
+    for t in range(110):
+        for phase in (0, 1):
+            if phase == 1:
+                dev.qq_initial_step(dt)
+                dev.update_nlist()
+                dev.initialize_forces()
+                dev.forces(*gb)
+            st = from_device(dev, tmpl)
+            st["target_temp"], st["dof"] = tmpl["target_temp"], tmpl["dof"]
+            ref = st.copy()
+            oracle.integrate(NHT, phase, ref, dt)
+            (dev.nht_initial_step if phase == 0 else dev.nht_final_step)(dt)
+            got = from_device(dev, tmpl)
+            for f in ("rx", "vx", "vy", "vz"):
+                e = rel_err(got[f], ref[f])
+                assert e < TOL, (t, phase, f, e)
+                worst_v = max(worst_v, e)
+            for sname in ("peta0", "peta1", "peta2", "eta0"):
+                e = abs(got[sname] - ref[sname]) / max(1.0, abs(ref[sname]))
+                assert e <= TOL_CHAIN, (t, phase, sname, got[sname], ref[sname])
+                worst_chain = max(worst_chain, e)
+        dev.qq_final_step(dt)
+    it1, calls1, fb1 = dev.nht_stats()
+    say(f"  {calls1 - calls0} calls, {(it1 - it0) / (calls1 - calls0):.1f} Newton updates per call, "
+        f"{fb1 - fb0} by windows; worst {worst_v:.1e} (particles), {worst_chain:.1e} (chain)")
+    assert calls1 - calls0 == 220
 
 
 @pytest.mark.parametrize("kind", [VVC, LGVC])
