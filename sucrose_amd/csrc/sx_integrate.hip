@@ -163,13 +163,11 @@ __device__ __forceinline__ void rebuild_flag_from(double m, double *part, unsign
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        part[blockIdx.x] = sh[0];
-        __threadfence();
-        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+        const double p[1] = {sh[0]};
+        last = sx_publish_take_ticket<1>(part, p, ticket);
     }
     __syncthreads();
     if (!last) return;
-    __threadfence();
     double v = 0.0;
     for (int b = threadIdx.x; b < (int)gridDim.x; b += BS)
         v = fmax(v, __hip_atomic_load(&part[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
@@ -182,7 +180,7 @@ __device__ __forceinline__ void rebuild_flag_from(double m, double *part, unsign
     if (threadIdx.x == 0) {
         double mx = sh[0];
         flag[0] = rule == SX_REBUILD_REFERENCE ? (4.0 * mx > thresh) : (mx > thresh);
-        *ticket = 0u;
+        sx_ticket_reset(ticket);
     }
 }
 

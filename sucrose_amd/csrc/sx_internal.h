@@ -299,6 +299,29 @@ __device__ __forceinline__ double sx_mimg_fast(double d, double L, double invL) 
 // blockDim.x must be 256.
 // Returns true in the last block, where tot[] (valid in thread 0) holds the
 // grid totals.
+// Thread 0 of a workgroup publishes its W partial values and takes a ticket;
+// true in the last workgroup to do so.  The partials are agent-scope atomic
+// stores, completed (s_waitcnt) before the ticket increment, and the last
+// workgroup reads them with agent-scope atomic loads issued after its ticket
+// returned: coherent across the XCDs' L2 caches without the L2 write-back and
+// invalidation of __threadfence().
+template <int W>
+__device__ __forceinline__ bool sx_publish_take_ticket(double *part, const double (&v)[W], unsigned *ticket)
+{
+#pragma unroll
+    for (int q = 0; q < W; ++q) __hip_atomic_store(&part[blockIdx.x * W + q], v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    return __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+}
+
+// re-arms a ticket for the next launch (in the last workgroup)
+__device__ __forceinline__ void sx_ticket_reset(unsigned *ticket)
+{
+    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <int W, int BS = 256>
 __device__ __forceinline__ bool sx_block_reduce_last(double (&v)[W], double *part, unsigned *ticket, double (&tot)[W])
 {
@@ -316,15 +339,13 @@ __device__ __forceinline__ bool sx_block_reduce_last(double (&v)[W], double *par
         __syncthreads();
     }
     if (t == 0) {
+        double p[W];
 #pragma unroll
-        for (int q = 0; q < W; ++q) part[blockIdx.x * W + q] = sh[q][0];
-        __threadfence();
-        unsigned k = atomicAdd(ticket, 1u);
-        last = (k == gridDim.x - 1);
+        for (int q = 0; q < W; ++q) p[q] = sh[q][0];
+        last = sx_publish_take_ticket<W>(part, p, ticket);
     }
     __syncthreads();
     if (!last) return false;
-    __threadfence();
     int nb = gridDim.x;
 #pragma unroll
     for (int q = 0; q < W; ++q) {
@@ -344,7 +365,7 @@ __device__ __forceinline__ bool sx_block_reduce_last(double (&v)[W], double *par
     if (t == 0) {
 #pragma unroll
         for (int q = 0; q < W; ++q) tot[q] = sh[q][0];
-        *ticket = 0u;
+        sx_ticket_reset(ticket);
     }
     return true;
 }

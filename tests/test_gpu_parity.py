@@ -415,7 +415,7 @@ def test_nose_hoover_windows(dev, oracle, phase, far, monkeypatch):
     assert calls1 - calls0 == 1 and fb1 - fb0 == 1
 
 
-def test_nose_hoover_thermal_regime(dev, oracle):
+def test_nose_hoover_thermal_regime(oracle):
     """The Gay-Berne + Nose-Hoover workload (8000 ellipsoids, dt 0.0015) run
     into the chain's thermal regime: from step ~4000 p_eta(1) is near its
     thermal scale and one call swings it by its own size, so calls need many
@@ -426,6 +426,8 @@ def test_nose_hoover_thermal_regime(dev, oracle):
     import sucrose_amd as sx
     from sucrose_amd.workloads import gb_nhc_state
     say("nose-hoover thermal regime, gb_nhc_8000 steps 5150-5260")
+    # a context of its own: the workload sets the half-skin rebuild rule
+    dev = sx.System(0)
     p = gb_nhc_state(dev, log=lambda *a: None)
     gb = (sx.GB, p["cutoff"], p["mu"], p["nu"])
     dt, step = p["dt"], p["next_step"]
@@ -461,6 +463,7 @@ def test_nose_hoover_thermal_regime(dev, oracle):
     it1, calls1, fb1 = dev.nht_stats()
     say(f"  {calls1 - calls0} calls, {(it1 - it0) / (calls1 - calls0):.1f} Newton updates per call, "
         f"{fb1 - fb0} by windows; worst {worst_v:.1e} (particles), {worst_chain:.1e} (chain)")
+    dev.close()
     assert calls1 - calls0 == 220
 
 
