@@ -91,6 +91,10 @@ void sx_check_device_error(sx_ctx *c)
         case DERR_ROW_CAP: throw sx_error{SX_ERR_CAPACITY, "neighbour row longer than the sort buffer"};
         case DERR_CONSTRAINT: throw sx_error{SX_ERR_CONSTRAINT, "Constraint failure (RATTLE)"};
         case DERR_ECP_CAP: throw sx_error{SX_ERR_CAPACITY, "ECP work list capacity exceeded"};
+        case DERR_LIST_SYNC:
+            sx_nlist_reset_sync(c);
+            throw sx_error{SX_ERR_HIP, "neighbour list rebuild: the workgroups of one build were not resident "
+                                       "together (the list is invalid; rebuild it)"};
         case DERR_NHT_SYNC:
             sx_nht_reset_sync(c);
             throw sx_error{SX_ERR_HIP, "Nose-Hoover chain: the workgroups of one call were not resident together "

@@ -36,7 +36,7 @@ struct sx_error {
     } while (0)
 
 // device error word codes (first error wins)
-enum { DERR_NONE = 0, DERR_NLIST_CAP = 1, DERR_ROW_CAP = 2, DERR_CONSTRAINT = 3, DERR_ECP_CAP = 4, DERR_NHT_SYNC = 5 };
+enum { DERR_NONE = 0, DERR_NLIST_CAP = 1, DERR_ROW_CAP = 2, DERR_CONSTRAINT = 3, DERR_ECP_CAP = 4, DERR_NHT_SYNC = 5, DERR_LIST_SYNC = 6 };
 
 // ---------------------------------------------------------------------------
 // state passed to kernels by value
@@ -67,6 +67,8 @@ struct NList {
     long cap_full = 0, cap_half = 0;
     int max_row = 0;
     int lpr_rows = 16, lpr_half = 16;   // lanes per row of the row and half-index kernels
+    int lb_grid = 0;                     // workgroups of the one-launch conditional rebuild (0: stage kernels)
+    unsigned long long *lb_sync = nullptr;   // its barrier flags and epoch
     bool need_half = false;             // half list and pair index kept (ECP paths, export)
     long rebuilds = 0;
     // host copies of the sizes of the last build
@@ -255,6 +257,7 @@ void sx_vv_final_impl(sx_ctx *c, double dt, bool thermo = false);
 void sx_qq_initial_impl(sx_ctx *c, double dt);
 void sx_qq_step_begin_impl(sx_ctx *c, double dt);   // qq_initial + rebuild flag + initialize_forces
 void sx_nht_reset_sync(sx_ctx *c);   // clears the chain's call flags after DERR_NHT_SYNC
+void sx_nlist_reset_sync(sx_ctx *c);   // clears the one-launch rebuild's flags after DERR_LIST_SYNC
 void sx_qq_final_impl(sx_ctx *c, double dt, bool thermo = false);
 void sx_nht_impl(sx_ctx *c, double dt, const double *coupling, int phase, bool save);
 void sx_rattle_impl(sx_ctx *c, double dt, int phase);

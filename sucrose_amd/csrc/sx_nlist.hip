@@ -70,13 +70,9 @@ __device__ __forceinline__ int cell_coord(double x, double L, int nc, double w)
 // bins the particles and records the positions the list is built from
 // (x_last of build_nlist, system.f90:1001); count[7] counts the builds
 // done inside graph replays
-__global__ void k_cell_assign(DState s, CellGrid g, int *cell_of, int *cell_count, double *xl, double *yl,
-                              double *zl, const int *flag, long *count)
+__device__ __forceinline__ void cell_assign_one(const DState &s, const CellGrid &g, int i, int *cell_of,
+                                                int *cell_count, double *xl, double *yl, double *zl)
 {
-    if (flag && *flag == 0) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0 && count) count[7] += 1;
-    if (i >= s.n) return;
     double x = s.f[SX_RX][i], y = s.f[SX_RY][i], z = s.f[SX_RZ][i];
     xl[i] = x; yl[i] = y; zl[i] = z;
     int cx = cell_coord(x, s.box[0], g.nc[0], g.w[0]);
@@ -85,6 +81,16 @@ __global__ void k_cell_assign(DState s, CellGrid g, int *cell_of, int *cell_coun
     int c = (cx * g.nc[1] + cy) * g.nc[2] + cz;
     cell_of[i] = c;
     atomicAdd(&cell_count[c], 1);
+}
+
+__global__ void k_cell_assign(DState s, CellGrid g, int *cell_of, int *cell_count, double *xl, double *yl,
+                              double *zl, const int *flag, long *count)
+{
+    if (flag && *flag == 0) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && count) count[7] += 1;
+    if (i >= s.n) return;
+    cell_assign_one(s, g, i, cell_of, cell_count, xl, yl, zl);
 }
 
 // exclusive scan of n ints by one work-group into out[0..n-1]; returns the
@@ -135,12 +141,10 @@ __device__ long scan_block(const int *in, int *out, int n, int *wsum)
 // capacity error raised, so that no later kernel indexes past the list.
 // in2 (may be NULL): a second array of n ints scanned the same way into
 // out2[0..n] after the first (and after its zeroing), its total in *total2.
-__global__ void __launch_bounds__(1024) k_scan(const int *in, int *out, int n, const int *flag, int *total,
-                                               long cap, int *err, int *zero_also, const int *in2, int *out2,
-                                               int *total2)
+// the scan of k_scan below by one workgroup of 1024 threads
+__device__ void scan_body(const int *in, int *out, int n, int *total, long cap, int *err, int *zero_also,
+                          const int *in2, int *out2, int *total2, int *wsum)
 {
-    if (flag && *flag == 0) return;
-    __shared__ int wsum[16];
     const int t = threadIdx.x;
     const long carry = scan_block(in, out, n, wsum);
     // carry is the same in every thread, so the branch is uniform
@@ -166,18 +170,33 @@ __global__ void __launch_bounds__(1024) k_scan(const int *in, int *out, int n, c
     }
 }
 
+__global__ void __launch_bounds__(1024) k_scan(const int *in, int *out, int n, const int *flag, int *total,
+                                               long cap, int *err, int *zero_also, const int *in2, int *out2,
+                                               int *total2)
+{
+    if (flag && *flag == 0) return;
+    __shared__ int wsum[16];
+    scan_body(in, out, n, total, cap, err, zero_also, in2, out2, total2, wsum);
+}
+
 // places each particle in its cell's slot range by counting the cell's
 // count back down, which leaves every count at zero for the next build
+__device__ __forceinline__ void cell_scatter_one(const DState &s, int i, const int *cell_of, const int *cell_start,
+                                                 int *cell_count, int *items, double4 *cell_pos)
+{
+    int c = cell_of[i];
+    int p = cell_start[c + 1] - atomicSub(&cell_count[c], 1);
+    items[p] = i;
+    cell_pos[p] = make_double4(s.f[SX_RX][i], s.f[SX_RY][i], s.f[SX_RZ][i], 0.0);
+}
+
 __global__ void k_cell_scatter(DState s, const int *cell_of, const int *cell_start, int *cell_count, int *items,
                                double4 *cell_pos, const int *flag)
 {
     if (flag && *flag == 0) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.n) return;
-    int c = cell_of[i];
-    int p = cell_start[c + 1] - atomicSub(&cell_count[c], 1);
-    items[p] = i;
-    cell_pos[p] = make_double4(s.f[SX_RX][i], s.f[SX_RY][i], s.f[SX_RZ][i], 0.0);
+    cell_scatter_one(s, i, cell_of, cell_start, cell_count, items, cell_pos);
 }
 
 // count (pass 0) or fill (pass 1) the full rows using the cell grid
@@ -237,17 +256,15 @@ __device__ __forceinline__ unsigned long long group_mask(int LPR)
     return LPR == 64 ? ~0ull : (((1ull << LPR) - 1ull) << (l64 - l64 % LPR));
 }
 
+// the row of the particle in cell-order slot `slot`, by the LPR lanes of a
+// group (all of them with the same slot)
 template <int PASS, int LPR>
-__global__ void __launch_bounds__(256) k_rows_cells_g(DState s, CellGrid g, const int *cell_of, const int *cell_start,
-                                                      const int *items, const double4 *cell_pos, double rsq_cut,
-                                                      int *row_len, int *lower, int *half_len, const int *row_start,
-                                                      int *nbr, const int *flag, long cap, int *err)
+__device__ __forceinline__ void rows_cells_group(const DState &s, const CellGrid &g, const int *cell_of,
+                                                 const int *cell_start, const int *items, const double4 *cell_pos,
+                                                 double rsq_cut, int *row_len, int *lower, int *half_len,
+                                                 const int *row_start, int *nbr, long cap, int *err, int slot)
 {
-    if (flag && *flag == 0) return;
     const int lane = threadIdx.x % LPR;
-    // rows are taken in cell order, so neighbouring groups read the same cells
-    const int slot = blockIdx.x * (256 / LPR) + threadIdx.x / LPR;
-    if (slot >= s.n) return;   // whole groups leave together
     const int i = items[slot];
     const unsigned long long gm = group_mask(LPR);
     const unsigned long long below = gm & ((1ull << (threadIdx.x & 63)) - 1ull);
@@ -299,23 +316,32 @@ __global__ void __launch_bounds__(256) k_rows_cells_g(DState s, CellGrid g, cons
     }
 }
 
-// half-pair index with LPR lanes per row (entries of a row are independent)
-template <int LPR>
-__global__ void __launch_bounds__(256) k_half_index_g(int n, const int *row_start, const int *lower,
-                                                      const int *half_start, const int *nbr, int *pairk, int *half_i,
-                                                      int *half_j, int *fwdpos, int *revpos, const int *flag,
-                                                      long cap_half, int *err)
+template <int PASS, int LPR>
+__global__ void __launch_bounds__(256) k_rows_cells_g(DState s, CellGrid g, const int *cell_of, const int *cell_start,
+                                                      const int *items, const double4 *cell_pos, double rsq_cut,
+                                                      int *row_len, int *lower, int *half_len, const int *row_start,
+                                                      int *nbr, const int *flag, long cap, int *err)
 {
     if (flag && *flag == 0) return;
-    const int lane = threadIdx.x % LPR;
-    const int i = blockIdx.x * (256 / LPR) + threadIdx.x / LPR;
-    if (i >= n) return;
+    // rows are taken in cell order, so neighbouring groups read the same cells
+    const int slot = blockIdx.x * (256 / LPR) + threadIdx.x / LPR;
+    if (slot >= s.n) return;   // whole groups leave together
+    rows_cells_group<PASS, LPR>(s, g, cell_of, cell_start, items, cell_pos, rsq_cut, row_len, lower, half_len,
+                                row_start, nbr, cap, err, slot);
+}
+
+// half-pair index with LPR lanes per row (entries of a row are independent)
+// row i's entries by lane `lane` of the row's L lanes
+__device__ __forceinline__ void half_index_row(int n, int i, int lane, int L, const int *row_start, const int *lower,
+                                               const int *half_start, const int *nbr, int *pairk, int *half_i,
+                                               int *half_j, int *fwdpos, int *revpos, long cap_half, int *err)
+{
     if ((long)half_start[n] > cap_half) {
         if (lane == 0) atomicCAS(err, 0, DERR_NLIST_CAP);
         return;
     }
     const int b = row_start[i], e = row_start[i + 1], lo = lower[i];
-    for (int p = b + lane; p < e; p += LPR) {
+    for (int p = b + lane; p < e; p += L) {
         const int j = nbr[p];
         if (j > i) {
             const int k = half_start[i] + (p - b - lo);
@@ -335,6 +361,19 @@ __global__ void __launch_bounds__(256) k_half_index_g(int n, const int *row_star
             revpos[k] = p;
         }
     }
+}
+
+template <int LPR>
+__global__ void __launch_bounds__(256) k_half_index_g(int n, const int *row_start, const int *lower,
+                                                      const int *half_start, const int *nbr, int *pairk, int *half_i,
+                                                      int *half_j, int *fwdpos, int *revpos, const int *flag,
+                                                      long cap_half, int *err)
+{
+    if (flag && *flag == 0) return;
+    const int i = blockIdx.x * (256 / LPR) + threadIdx.x / LPR;
+    if (i >= n) return;
+    half_index_row(n, i, threadIdx.x % LPR, LPR, row_start, lower, half_start, nbr, pairk, half_i, half_j, fwdpos,
+                   revpos, cap_half, err);
 }
 
 // brute force rows (small or non-periodic boxes)
@@ -400,18 +439,15 @@ __device__ __forceinline__ void rank_row_entries(const int *sh, int len, int len
     }
 }
 
-__global__ void __launch_bounds__(256) k_sort_rows(int n, const int *row_start, int *nbr, const int *flag, int *err)
+// row i sorted in place by one wavefront through its LDS buffer sh
+__device__ __forceinline__ void sort_row(int i, const int *row_start, int *nbr, int *sh, int lane, int *err)
 {
-    if (flag && *flag == 0) return;
-    __shared__ __align__(16) int buf[4][SORT_CAP];
-    const int w = threadIdx.x / SX_WAVE, lane = threadIdx.x % SX_WAVE;
-    int *sh = buf[w];
-    for (int i = blockIdx.x * 4 + w; i < n; i += gridDim.x * 4) {
+    {
         const int b = row_start[i], len = row_start[i + 1] - b;
-        if (len <= 1) continue;
+        if (len <= 1) return;
         if (len > SORT_CAP) {
             if (lane == 0) atomicCAS(err, 0, DERR_ROW_CAP);
-            continue;
+            return;
         }
         const int len4 = (len + 3) & ~3;
         for (int q = lane; q < len4; q += SX_WAVE) sh[q] = q < len ? nbr[b + q] : 0x7fffffff;
@@ -436,6 +472,14 @@ __global__ void __launch_bounds__(256) k_sort_rows(int n, const int *row_start, 
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+}
+
+__global__ void __launch_bounds__(256) k_sort_rows(int n, const int *row_start, int *nbr, const int *flag, int *err)
+{
+    if (flag && *flag == 0) return;
+    __shared__ __align__(16) int buf[4][SORT_CAP];
+    const int w = threadIdx.x / SX_WAVE, lane = threadIdx.x % SX_WAVE;
+    for (int i = blockIdx.x * 4 + w; i < n; i += gridDim.x * 4) sort_row(i, row_start, nbr, buf[w], lane, err);
 }
 
 // half-pair index of every full-list entry, and the half list (i, j) pairs
@@ -523,6 +567,197 @@ __global__ void __launch_bounds__(256) k_disp_max(DState s, const double *xl, co
     }
 }
 
+static CellGrid cell_grid(const sx_ctx *c);
+
+// ---------------------------------------------------------------------------
+// The conditional rebuild of a replayed step in one launch.  Between builds a
+// step pays one launch that reads the rebuild flag and returns (eight flag-
+// gated launches of the stage kernels above cost about 2 us each inside a
+// replayed graph).  When the flag is set, the same stages run one after the
+// other in LB_GRID resident workgroups of 1024 threads (one per CU), which
+// meet at device-wide barriers between stages: bin, cell starts (workgroup
+// 0), scatter, count rows, row and half-list starts (workgroup 0), fill rows,
+// sort rows, half index.  The stage code is the kernels' own.
+// ---------------------------------------------------------------------------
+#define LB_THREADS 1024
+#define LB_WAVES (LB_THREADS / SX_WAVE)
+#define LB_GMAX 256          // most workgroups (four flags per lane of the polling wavefront)
+#define LB_SPIN (1 << 16)    // polls of one barrier before DERR_LIST_SYNC
+
+struct ListArgs {
+    CellGrid g;
+    int *cell_of, *cell_count, *cell_start, *cell_items;
+    double4 *cell_pos;
+    double *xl, *yl, *zl;
+    long *count;
+    int ncells;
+    double rsq_cut;
+    int *row_len, *lower, *half_len, *row_start, *nbr;
+    long cap_full;
+    int need_half, lpr_half;
+    int *half_start, *pairk, *half_i, *half_j, *fwdpos, *revpos;
+    long cap_half;
+    int *total, *total_half;     // lengths of the full and half lists
+    int *err;
+    unsigned long long *sync;    // per workgroup: epoch * 16 + the last stage finished; [LB_GMAX]: the epoch
+};
+
+// Every workgroup has finished the stage that `target` names: each wavefront
+// waits for its own stores, one L2 write-back per workgroup makes them
+// visible to the other XCDs, then the flag; the first wavefront polls every
+// flag (lane l: workgroups l, l + 64, l + 128, l + 192, loaded before any is
+// tested), then one L2 invalidation before the workgroup reads what the
+// others wrote.  false after LB_SPIN polls (DERR_LIST_SYNC raised), so that
+// every wait ends.
+__device__ bool lb_barrier(const ListArgs &a, unsigned long long target)
+{
+    __shared__ int ok_sh;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(a.sync + blockIdx.x, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (threadIdx.x < SX_WAVE) {
+        const int G = gridDim.x;
+        bool done = false;
+        for (int it = 0; it < LB_SPIN && !done; ++it) {
+            unsigned long long f[LB_GMAX / SX_WAVE];
+#pragma unroll
+            for (int k = 0; k < LB_GMAX / SX_WAVE; ++k) {
+                const int w = threadIdx.x + k * SX_WAVE;
+                f[k] = w < G ? __hip_atomic_load(a.sync + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : target;
+            }
+            bool ok = true;
+#pragma unroll
+            for (int k = 0; k < LB_GMAX / SX_WAVE; ++k) ok = ok && f[k] >= target;
+            done = __all(ok);
+            if (!done) __builtin_amdgcn_s_sleep(1);
+        }
+        if (threadIdx.x == 0) {
+            ok_sh = done;
+            if (!done) atomicCAS(a.err, DERR_NONE, DERR_LIST_SYNC);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    return ok_sh != 0;
+}
+
+template <int LPR>
+__global__ void __launch_bounds__(LB_THREADS) k_list_rebuild(DState s, ListArgs a, const int *flag)
+{
+    if (*flag == 0) return;
+    __shared__ int wsum[16];
+    __shared__ __align__(16) int sortbuf[LB_WAVES][SORT_CAP];
+    const int n = s.n, G = gridDim.x, lane = threadIdx.x % SX_WAVE, w = threadIdx.x / SX_WAVE;
+    const int tid = blockIdx.x * LB_THREADS + threadIdx.x, nth = G * LB_THREADS;
+    const unsigned long long ep =
+        __hip_atomic_load(a.sync + LB_GMAX, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) * 16ull;
+    // bin the particles, recording the positions the list is built from
+    if (tid == 0 && a.count) a.count[7] += 1;
+    for (int i = tid; i < n; i += nth) cell_assign_one(s, a.g, i, a.cell_of, a.cell_count, a.xl, a.yl, a.zl);
+    if (!lb_barrier(a, ep + 1)) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0)   // every workgroup has started: the next build's epoch
+        __hip_atomic_store(a.sync + LB_GMAX, ep / 16ull + 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // cell starts
+    if (blockIdx.x == 0)
+        scan_body(a.cell_count, a.cell_start, a.ncells, nullptr, 0L, nullptr, nullptr, nullptr, nullptr, nullptr,
+                  wsum);
+    if (!lb_barrier(a, ep + 2)) return;
+    for (int i = tid; i < n; i += nth)
+        cell_scatter_one(s, i, a.cell_of, a.cell_start, a.cell_count, a.cell_items, a.cell_pos);
+    if (!lb_barrier(a, ep + 3)) return;
+    // rows in cell order, LPR lanes per row
+    for (int slot = tid / LPR; slot < n; slot += nth / LPR)
+        rows_cells_group<0, LPR>(s, a.g, a.cell_of, a.cell_start, a.cell_items, a.cell_pos, a.rsq_cut, a.row_len,
+                                 a.lower, a.half_len, a.row_start, a.nbr, a.cap_full, a.err, slot);
+    if (!lb_barrier(a, ep + 4)) return;
+    // row starts, and the half-list starts when the half list is kept
+    if (blockIdx.x == 0)
+        scan_body(a.row_len, a.row_start, n, a.total, a.cap_full, a.err, a.half_len,
+                  a.need_half ? (const int *)a.half_len : nullptr, a.half_start, a.total_half, wsum);
+    if (!lb_barrier(a, ep + 5)) return;
+    for (int slot = tid / LPR; slot < n; slot += nth / LPR)
+        rows_cells_group<1, LPR>(s, a.g, a.cell_of, a.cell_start, a.cell_items, a.cell_pos, a.rsq_cut, a.row_len,
+                                 a.lower, a.half_len, a.row_start, a.nbr, a.cap_full, a.err, slot);
+    if (!lb_barrier(a, ep + 6)) return;
+    for (int i = blockIdx.x * LB_WAVES + w; i < n; i += G * LB_WAVES) sort_row(i, a.row_start, a.nbr, sortbuf[w], lane, a.err);
+    if (!a.need_half) return;
+    if (!lb_barrier(a, ep + 7)) return;
+    const int L = a.lpr_half;
+    for (int i = tid / L; i < n; i += nth / L)
+        half_index_row(n, i, tid % L, L, a.row_start, a.lower, a.half_start, a.nbr, a.pairk, a.half_i, a.half_j,
+                       a.fwdpos, a.revpos, a.cap_half, a.err);
+}
+
+template <int LPR>
+static int lb_blocks_per_cu()
+{
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_list_rebuild<LPR>, LB_THREADS, 0) != hipSuccess) return 0;
+    return per;
+}
+
+// workgroups of the one-launch rebuild for this device and row width, 0 when
+// a workgroup of it does not fit a CU
+static int lb_grid_for(int lpr)
+{
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    int per = 0;
+    switch (lpr) {
+    case 4: per = lb_blocks_per_cu<4>(); break;
+    case 8: per = lb_blocks_per_cu<8>(); break;
+    case 16: per = lb_blocks_per_cu<16>(); break;
+    case 32: per = lb_blocks_per_cu<32>(); break;
+    default: per = lb_blocks_per_cu<64>(); break;
+    }
+    return per >= 1 ? std::min(cus, LB_GMAX) : 0;
+}
+
+static void launch_list_rebuild(sx_ctx *c, const int *flag)
+{
+    NList &nl = c->nl;
+    ListArgs a;
+    a.g = cell_grid(c);
+    a.cell_of = nl.cell_of; a.cell_count = nl.cell_count; a.cell_start = nl.cell_start; a.cell_items = nl.cell_items;
+    a.cell_pos = (double4 *)nl.cell_pos;
+    a.xl = nl.x_last; a.yl = nl.y_last; a.zl = nl.z_last;
+    a.count = c->d_counts;
+    a.ncells = nl.ncells;
+    a.rsq_cut = nl.r_skin_sq;
+    a.row_len = nl.row_len; a.lower = nl.lower; a.half_len = nl.half_len; a.row_start = nl.row_start; a.nbr = nl.nbr;
+    a.cap_full = nl.cap_full;
+    a.need_half = nl.need_half ? 1 : 0;
+    a.lpr_half = nl.lpr_half;
+    a.half_start = nl.half_start; a.pairk = nl.pairk; a.half_i = nl.half_i; a.half_j = nl.half_j;
+    a.fwdpos = nl.fwdpos; a.revpos = nl.revpos;
+    a.cap_half = nl.cap_half;
+    a.total = c->d_flag + 1; a.total_half = c->d_flag + 2;
+    a.err = c->d_err;
+    a.sync = nl.lb_sync;
+    DState s = c->dstate();
+    dim3 grid(nl.lb_grid), blk(LB_THREADS);
+    switch (nl.lpr_rows) {
+    case 4: hipLaunchKernelGGL(k_list_rebuild<4>, grid, blk, 0, c->stream, s, a, flag); break;
+    case 8: hipLaunchKernelGGL(k_list_rebuild<8>, grid, blk, 0, c->stream, s, a, flag); break;
+    case 16: hipLaunchKernelGGL(k_list_rebuild<16>, grid, blk, 0, c->stream, s, a, flag); break;
+    case 32: hipLaunchKernelGGL(k_list_rebuild<32>, grid, blk, 0, c->stream, s, a, flag); break;
+    default: hipLaunchKernelGGL(k_list_rebuild<64>, grid, blk, 0, c->stream, s, a, flag); break;
+    }
+}
+
+// after DERR_LIST_SYNC: a build whose workgroups gave up waiting left its
+// flags mid-build
+void sx_nlist_reset_sync(sx_ctx *c)
+{
+    NList &nl = c->nl;
+    if (nl.lb_sync) HIPCHK(hipMemsetAsync(nl.lb_sync, 0, sizeof(unsigned long long) * (LB_GMAX + 1), c->stream));
+}
+
 // ---------------------------------------------------------------------------
 static void launch_rows_g(sx_ctx *c, int pass, const CellGrid &g, const int *flag)
 {
@@ -590,6 +825,9 @@ static void nl_free(NList &nl)
     nl.cell_of = nl.cell_count = nl.cell_start = nl.cell_items = nl.row_len = nl.row_start = nl.lower = nullptr;
     nl.half_start = nl.half_len = nl.nbr = nl.pairk = nl.half_i = nl.half_j = nl.fwdpos = nl.revpos = nullptr;
     nl.x_last = nl.y_last = nl.z_last = nl.cell_pos = nullptr;
+    if (nl.lb_sync) hipFree(nl.lb_sync);
+    nl.lb_sync = nullptr;
+    nl.lb_grid = 0;
 }
 
 static void nl_alloc_lists(sx_ctx *c, long cap_full)
@@ -742,6 +980,14 @@ static void nl_setup(sx_ctx *c)
             if (b >= 1) nl.lpr_half = pow2_clamp(b, 4, 64);
         }
     }
+    // the one-launch conditional rebuild (cell grids only)
+    if (nl.use_cells) {
+        nl.lb_grid = lb_grid_for(nl.lpr_rows);
+        if (nl.lb_grid > 0) {
+            HIPCHK(hipMalloc(&nl.lb_sync, sizeof(unsigned long long) * (LB_GMAX + 1)));
+            HIPCHK(hipMemsetAsync(nl.lb_sync, 0, sizeof(unsigned long long) * (LB_GMAX + 1), c->stream));
+        }
+    }
     nl.init = true;
 }
 
@@ -794,6 +1040,10 @@ void sx_nlist_build_impl(sx_ctx *c, bool conditional)
     DState s = c->dstate();
     const int *flag = conditional ? c->d_flag : nullptr;
     int nb = sx_blocks(n, 256);
+    if (conditional && nl.use_cells && nl.lb_grid > 0) {
+        launch_list_rebuild(c, flag);
+        return;
+    }
     for (int attempt = 0; attempt < 4; ++attempt) {
         if (nl.use_cells) {
             CellGrid g = cell_grid(c);
