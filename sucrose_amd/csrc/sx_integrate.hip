@@ -1232,110 +1232,139 @@ __global__ void __launch_bounds__(64) k_rattle(DState s, double dt, int nmol, co
     if (lane == 0) vir_part[blockIdx.x] = vir;
 }
 
-// The same sweeps for molecules of at most 64 bonds, with the serial bond
-// loop kept off the LDS crossbar: lane q holds bond q's atom lanes, length,
-// inverse masses and the bond vector that is fixed during the phase (old
-// positions in phase 0, positions in phase 1), the moved flags are one
-// wave-uniform 64-bit mask, and a bond's values are read from their lanes
-// with readlane at a uniform index.  The same expressions in the same order
-// as k_rattle, so the same results.
-__device__ __forceinline__ double rl_d(double v, int l)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)b, l);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-
+// RATTLE for molecules of at most 64 bonds with the sweeps pipelined.  Bond
+// q of sweep s runs at step lvl[q] + P*s (the schedule sx_set_constraints
+// derives from the bond order): the bonds of one step touch disjoint atoms,
+// and every atom sees the updates that touch it in the reference's serial
+// order at strictly increasing steps, so positions, velocities and moved
+// flags take exactly the values of the serial sweeps.  A chain whose bonds
+// are listed along it has lvl[q] = q and P = 2: S sweeps take nb + 2(S-1)
+// steps instead of nb*S dependent bond updates.
+//
+// The reference runs another sweep while some atom is flagged.  A sweep
+// without corrections leaves every flag clear (an atom flagged at its start
+// is evaluated, and cleared, by the first bond of the sweep that touches
+// it), and a sweep that starts with every flag clear skips every bond.  So
+// the wavefront stops at the end of the first sweep without corrections:
+// every sweep the reference runs is run, and any sweep run beyond them (and
+// any step already run for a later sweep) starts from clear flags and
+// changes nothing.  Lane q records in bit s mod 64 of a mask whether bond q
+// was corrected in sweep s; at the step that completes sweep s the wavefront
+// stops if no lane's bit is set.  At most span/P + 1 <= 64 sweeps are in
+// flight, so the bits do not collide.  Atom state lives in LDS, one
+// wavefront per workgroup.
 template <int PHASE>
-__global__ void __launch_bounds__(64) k_rattle64(DState s, double dt, int nmol, const int *atom_start,
-                                                const int *atoms, const int *bond_start, const int *bond_a,
-                                                const int *bond_b, const double *bond_len, double *vir_part,
-                                                int *err)
+__global__ void __launch_bounds__(64) k_rattle_pipe(DState s, double dt, const int *atom_start, const int *atoms,
+                                                   const int *bond_start, const int *bond_a, const int *bond_b,
+                                                   const double *bond_len, const int *bond_lvl, const int *sched,
+                                                   double *vir_part, int *err)
 {
+    __shared__ double sr[3][SX_WAVE], sv[3][SX_WAVE];
+    __shared__ int smv[SX_WAVE];
     const int m = blockIdx.x, lane = threadIdx.x;
+    const int a0 = atom_start[m], na = atom_start[m + 1] - a0;
+    const int b0 = bond_start[m], nb = bond_start[m + 1] - b0;
+    const int g = lane < na ? atoms[a0 + lane] : 0;
+    double rx = 0, ry = 0, rz = 0, ox = 0, oy = 0, oz = 0, im = 0;
+    if (lane < na) {
+        rx = s.f[SX_RX][g]; ry = s.f[SX_RY][g]; rz = s.f[SX_RZ][g];
+        sr[0][lane] = rx; sr[1][lane] = ry; sr[2][lane] = rz;
+        sv[0][lane] = s.f[SX_VX][g]; sv[1][lane] = s.f[SX_VY][g]; sv[2][lane] = s.f[SX_VZ][g];
+        if (PHASE == 0) { ox = s.f[SX_RX_OLD][g]; oy = s.f[SX_RY_OLD][g]; oz = s.f[SX_RZ_OLD][g]; }
+        im = s.f[SX_INV_MASS][g];
+    }
+    smv[lane] = 1;   // moved = .TRUE.
+    int qa = 0, qb = 0, next = INT_MAX;
+    double qlen = 0.0;
+    if (lane < nb) {
+        qa = bond_a[b0 + lane]; qb = bond_b[b0 + lane]; qlen = bond_len[b0 + lane];
+        next = bond_lvl[b0 + lane];
+    }
+    const double ima = shfl_d(im, qa), imb = shfl_d(im, qb), len = qlen;
+    // the bond vector fixed during the phase: old positions in a, positions in b
+    double Q[3] = {PHASE == 0 ? shfl_d(ox, qa) - shfl_d(ox, qb) : shfl_d(rx, qa) - shfl_d(rx, qb),
+                   PHASE == 0 ? shfl_d(oy, qa) - shfl_d(oy, qb) : shfl_d(ry, qa) - shfl_d(ry, qb),
+                   PHASE == 0 ? shfl_d(oz, qa) - shfl_d(oz, qb) : shfl_d(rz, qa) - shfl_d(rz, qb)};
+    if (s.periodic)
+        for (int k = 0; k < 3; ++k) Q[k] = sx_mimg(Q[k], s.box[k]);
+    const int P = sched[2 * m], span = sched[2 * m + 1];
+    const double r_tol = 1.0e-5;
     double vir = 0.0;
-    if (m < nmol) {
-        const int a0 = atom_start[m], na = atom_start[m + 1] - a0;
-        const int b0 = bond_start[m], nb = bond_start[m + 1] - b0;
-        const int g = lane < na ? atoms[a0 + lane] : 0;
-        double rx = 0, ry = 0, rz = 0, vx = 0, vy = 0, vz = 0, ox = 0, oy = 0, oz = 0, im = 0;
-        if (lane < na) {
-            rx = s.f[SX_RX][g]; ry = s.f[SX_RY][g]; rz = s.f[SX_RZ][g];
-            vx = s.f[SX_VX][g]; vy = s.f[SX_VY][g]; vz = s.f[SX_VZ][g];
-            ox = s.f[SX_RX_OLD][g]; oy = s.f[SX_RY_OLD][g]; oz = s.f[SX_RZ_OLD][g];
-            im = s.f[SX_INV_MASS][g];
-        }
-        int qa = 0, qb = 0;
-        double qlen = 0.0;
-        if (lane < nb) { qa = bond_a[b0 + lane]; qb = bond_b[b0 + lane]; qlen = bond_len[b0 + lane]; }
-        const double qima = shfl_d(im, qa), qimb = shfl_d(im, qb);
-        double Q[3] = {PHASE == 0 ? shfl_d(ox, qa) - shfl_d(ox, qb) : shfl_d(rx, qa) - shfl_d(rx, qb),
-                       PHASE == 0 ? shfl_d(oy, qa) - shfl_d(oy, qb) : shfl_d(ry, qa) - shfl_d(ry, qb),
-                       PHASE == 0 ? shfl_d(oz, qa) - shfl_d(oz, qb) : shfl_d(rz, qa) - shfl_d(rz, qb)};
-        if (s.periodic)
-            for (int k = 0; k < 3; ++k) Q[k] = sx_mimg(Q[k], s.box[k]);
-        unsigned long long mv = __ballot(lane < na);
-        const double r_tol = 1.0e-5;
-        int failed = 0;
-        for (int rattle = 0; rattle < 100; ++rattle) {
-            if (mv == 0ull) break;
-            for (int q = 0; q < nb; ++q) {
-                const int la = __builtin_amdgcn_readlane(qa, q), lb = __builtin_amdgcn_readlane(qb, q);
-                const unsigned long long bits = (1ull << la) | (1ull << lb);
-                if (!(mv & bits)) continue;
-                const double len = rl_d(qlen, q), ima = rl_d(qima, q), imb = rl_d(qimb, q);
-                if (PHASE == 0) {
-                    double R[3] = {rl_d(rx, la) - rl_d(rx, lb), rl_d(ry, la) - rl_d(ry, lb),
-                                   rl_d(rz, la) - rl_d(rz, lb)};
+    unsigned long long corr = 0ull;
+    int sw = 0, failed = 0;
+    int chk = span, S = 0;   // the step that completes sweep S
+    __syncthreads();
+    for (int t = 0;; ++t) {
+        if (t == next) {
+            const int ma = smv[qa], mb = smv[qb];
+            const double va[3] = {sv[0][qa], sv[1][qa], sv[2][qa]}, vb[3] = {sv[0][qb], sv[1][qb], sv[2][qb]};
+            if (PHASE == 0) {
+                const double ra[3] = {sr[0][qa], sr[1][qa], sr[2][qa]}, rb[3] = {sr[0][qb], sr[1][qb], sr[2][qb]};
+                if (ma | mb) {
+                    double R[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
                     if (s.periodic)
                         for (int k = 0; k < 3; ++k) R[k] = sx_mimg(R[k], s.box[k]);
-                    double chi = len * len - (R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+                    const double chi = len * len - (R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
                     if (fabs(chi) > 2 * r_tol * (len * len)) {
-                        const double Ro[3] = {rl_d(Q[0], q), rl_d(Q[1], q), rl_d(Q[2], q)};
-                        double vdot = R[0] * Ro[0] + R[1] * Ro[1] + R[2] * Ro[2];
-                        if (vdot < r_tol * (len * len)) { failed = 1; break; }
-                        double L = chi / (2.0 * vdot * (ima + imb));
-                        vir = vir + (L * (len * len)) / (dt * dt);
-                        // atom a gains c = L ima along Ro, atom b loses L imb:
-                        // b's update as the addition of the negated product
-                        // (negation is exact), so both lanes share one path
-                        if (lane == la || lane == lb) {
-                            const double c = lane == la ? L * ima : -(L * imb);
-                            rx = rx + c * Ro[0]; ry = ry + c * Ro[1]; rz = rz + c * Ro[2];
-                            vx = vx + c * Ro[0] / dt; vy = vy + c * Ro[1] / dt; vz = vz + c * Ro[2] / dt;
+                        const double vdot = R[0] * Q[0] + R[1] * Q[1] + R[2] * Q[2];
+                        if (vdot < r_tol * (len * len)) {
+                            failed = 1;
+                        } else {
+                            const double L = chi / (2.0 * vdot * (ima + imb));
+                            vir = vir + (L * (len * len)) / (dt * dt);
+                            // atom b's update as the addition of the negated
+                            // product (negation is exact)
+                            const double ca = L * ima, cb = -(L * imb);
+                            for (int k = 0; k < 3; ++k) {
+                                sr[k][qa] = ra[k] + ca * Q[k];
+                                sr[k][qb] = rb[k] + cb * Q[k];
+                                sv[k][qa] = va[k] + ca * Q[k] / dt;
+                                sv[k][qb] = vb[k] + cb * Q[k] / dt;
+                            }
+                            smv[qa] = 1; smv[qb] = 1;
+                            corr |= 1ull << (sw & 63);
                         }
-                        mv |= bits;
                     } else {
-                        mv &= ~bits;
-                    }
-                } else {
-                    const double R[3] = {rl_d(Q[0], q), rl_d(Q[1], q), rl_d(Q[2], q)};
-                    double V[3] = {rl_d(vx, la) - rl_d(vx, lb), rl_d(vy, la) - rl_d(vy, lb),
-                                   rl_d(vz, la) - rl_d(vz, lb)};
-                    double vdot = R[0] * V[0] + R[1] * V[1] + R[2] * V[2];
-                    double L = -vdot / ((ima + imb) * (len * len));
-                    if (fabs(L) > r_tol) {
-                        vir = vir + (L * (len * len)) / dt;
-                        if (lane == la || lane == lb) {   // as in phase a: b's update negated
-                            const double c = lane == la ? L * ima : -(L * imb);
-                            vx = vx + c * R[0]; vy = vy + c * R[1]; vz = vz + c * R[2];
-                        }
-                        mv |= bits;
-                    } else {
-                        mv &= ~bits;
+                        smv[qa] = 0; smv[qb] = 0;
                     }
                 }
+            } else if (ma | mb) {
+                const double V[3] = {va[0] - vb[0], va[1] - vb[1], va[2] - vb[2]};
+                const double vdot = Q[0] * V[0] + Q[1] * V[1] + Q[2] * V[2];
+                const double L = -vdot / ((ima + imb) * (len * len));
+                if (fabs(L) > r_tol) {
+                    vir = vir + (L * (len * len)) / dt;
+                    const double ca = L * ima, cb = -(L * imb);
+                    for (int k = 0; k < 3; ++k) {
+                        sv[k][qa] = va[k] + ca * Q[k];
+                        sv[k][qb] = vb[k] + cb * Q[k];
+                    }
+                    smv[qa] = 1; smv[qb] = 1;
+                    corr |= 1ull << (sw & 63);
+                } else {
+                    smv[qa] = 0; smv[qb] = 0;
+                }
             }
-            if (failed) break;
+            // the reference's cap: sweeps 0..99
+            next = ++sw < 100 ? next + P : INT_MAX;
         }
-        if (failed && lane == 0) atomicCAS(err, 0, DERR_CONSTRAINT);
-        if (lane < na) {
-            s.f[SX_RX][g] = rx; s.f[SX_RY][g] = ry; s.f[SX_RZ][g] = rz;
-            s.f[SX_VX][g] = vx; s.f[SX_VY][g] = vy; s.f[SX_VZ][g] = vz;
+        __syncthreads();
+        if (__any(failed)) break;
+        if (t == chk) {
+            const bool more = __any((corr >> (S & 63)) & 1ull);
+            corr &= ~(1ull << (S & 63));
+            if (!more || S == 99) break;
+            ++S;
+            chk += P;
         }
     }
-    if (lane == 0) vir_part[blockIdx.x] = vir;
+    if (failed) atomicCAS(err, 0, DERR_CONSTRAINT);
+    if (lane < na) {
+        s.f[SX_RX][g] = sr[0][lane]; s.f[SX_RY][g] = sr[1][lane]; s.f[SX_RZ][g] = sr[2][lane];
+        s.f[SX_VX][g] = sv[0][lane]; s.f[SX_VY][g] = sv[1][lane]; s.f[SX_VZ][g] = sv[2][lane];
+    }
+    for (int o = SX_WAVE / 2; o > 0; o >>= 1) vir = vir + __shfl_xor(vir, o, SX_WAVE);
+    if (lane == 0) vir_part[m] = vir;
 }
 
 // ---------------------------------------------------------------------------
@@ -1660,12 +1689,12 @@ void sx_rattle_impl(sx_ctx *c, double dt, int phase)
     sx_ensure_part(c, m.nmol, 1);
     if (m.max_bonds <= SX_WAVE) {
         if (phase == 0)
-            hipLaunchKernelGGL(k_rattle64<0>, dim3(m.nmol), dim3(64), 0, c->stream, c->dstate(), dt, m.nmol,
-                               m.atom_start, m.atoms, m.bond_start, m.bond_a, m.bond_b, m.bond_len, c->d_part,
+            hipLaunchKernelGGL(k_rattle_pipe<0>, dim3(m.nmol), dim3(64), 0, c->stream, c->dstate(), dt, m.atom_start,
+                               m.atoms, m.bond_start, m.bond_a, m.bond_b, m.bond_len, m.bond_lvl, m.sched, c->d_part,
                                c->d_err);
         else
-            hipLaunchKernelGGL(k_rattle64<1>, dim3(m.nmol), dim3(64), 0, c->stream, c->dstate(), dt, m.nmol,
-                               m.atom_start, m.atoms, m.bond_start, m.bond_a, m.bond_b, m.bond_len, c->d_part,
+            hipLaunchKernelGGL(k_rattle_pipe<1>, dim3(m.nmol), dim3(64), 0, c->stream, c->dstate(), dt, m.atom_start,
+                               m.atoms, m.bond_start, m.bond_a, m.bond_b, m.bond_len, m.bond_lvl, m.sched, c->d_part,
                                c->d_err);
     } else if (phase == 0)
         hipLaunchKernelGGL(k_rattle<0>, dim3(m.nmol), dim3(64), 0, c->stream, c->dstate(), dt, m.nmol, m.atom_start,
@@ -1832,7 +1861,7 @@ int sx_set_constraints(sx_ctx *c, int nc, const int *bbi1, const int *bbj1, cons
             if (mol_of_root[r] < 0) { mol_of_root[r] = (int)mb.size(); mb.emplace_back(); }
             mb[mol_of_root[r]].push_back(b);
         }
-        std::vector<int> atom_start{0}, atoms, bond_start{0}, ba, bb, gidx;
+        std::vector<int> atom_start{0}, atoms, bond_start{0}, ba, bb, gidx, lvl, sched;
         std::vector<double> bl;
         int max_atoms = 0, max_bonds = 0;
         for (auto &bonds : mb) {
@@ -1849,6 +1878,33 @@ int sx_set_constraints(sx_ctx *c, int nc, const int *bbi1, const int *bbj1, cons
             }
             if ((int)loc.size() > SX_WAVE)
                 throw sx_error{SX_ERR_ARG, "a constrained molecule has more than 64 atoms"};
+            // Pipelined sweeps.  Every bond update reads and writes both of
+            // its atoms, so the serial order fixes, for each atom, the order
+            // of the updates that touch it.  Bond q of sweep s is given the
+            // step lvl[q] + P*s: lvl[q] is one past the step of the previous
+            // bond (in the sweep order) that shares an atom with q, and the
+            // period P is one past the widest spread of steps over the bonds
+            // of one atom, so that an atom's first bond of sweep s+1 comes
+            // after its last bond of sweep s.  Each atom then sees its updates
+            // in the serial order at strictly increasing steps, and the bonds
+            // of one step touch disjoint atoms.
+            {
+                std::vector<int> first(loc.size(), -1), last(loc.size(), -1);
+                int span = 0, period = 1;
+                const size_t q0 = ba.size() - bonds.size();
+                for (size_t q = q0; q < ba.size(); ++q) {
+                    const int a = ba[q], b = bb[q];
+                    const int l = std::max(last[a], last[b]) + 1;
+                    lvl.push_back(l);
+                    span = std::max(span, l);
+                    if (first[a] < 0) first[a] = l;
+                    if (first[b] < 0) first[b] = l;
+                    last[a] = last[b] = l;
+                }
+                for (size_t k = 0; k < loc.size(); ++k) period = std::max(period, last[k] - first[k] + 1);
+                sched.push_back(period);
+                sched.push_back(span);
+            }
             max_atoms = std::max(max_atoms, (int)loc.size());
             max_bonds = std::max(max_bonds, (int)bonds.size());
             for (int g : loc) atoms.push_back(g);
@@ -1856,7 +1912,8 @@ int sx_set_constraints(sx_ctx *c, int nc, const int *bbi1, const int *bbj1, cons
             bond_start.push_back((int)ba.size());
         }
         Molecules &m = c->mol;
-        void *old[] = {m.atom_start, m.atoms, m.bond_start, m.bond_a, m.bond_b, m.bond_len, m.bond_gidx};
+        void *old[] = {m.atom_start, m.atoms, m.bond_start, m.bond_a, m.bond_b, m.bond_len, m.bond_gidx,
+                       m.bond_lvl, m.sched};
         for (void *p : old)
             if (p) hipFree(p);
         m = Molecules();
@@ -1875,6 +1932,8 @@ int sx_set_constraints(sx_ctx *c, int nc, const int *bbi1, const int *bbj1, cons
         up(bb, &m.bond_b);
         up(bl, &m.bond_len);
         up(gidx, &m.bond_gidx);
+        up(lvl, &m.bond_lvl);
+        up(sched, &m.sched);
         c->nc = nc;
         c->h_bbi.assign(bbi1, bbi1 + nc);
         c->h_bbj.assign(bbj1, bbj1 + nc);

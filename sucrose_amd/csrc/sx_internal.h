@@ -89,6 +89,9 @@ struct Molecules {
     int *bond_start = nullptr, *bond_a = nullptr, *bond_b = nullptr;  // local atom ids
     double *bond_len = nullptr;
     int *bond_gidx = nullptr;
+    // pipelined sweeps: bond q of sweep s runs at step bond_lvl[q] + P*s,
+    // sched[2m] = P (the period), sched[2m+1] = the largest bond_lvl
+    int *bond_lvl = nullptr, *sched = nullptr;
     int max_atoms = 0, max_bonds = 0;
 };
 

@@ -477,6 +477,89 @@ def test_rattle_one_step(dev, oracle, kind, phase):
     assert abs(got["virial_bonded"] - ref["virial_bonded"]) <= TOL * max(1.0, abs(ref["virial_bonded"]))
 
 
+def _molecule_bonds(kind, n, rng):
+    """Bond lists (local atom ids, in the order given to the constraint list)
+    of a few topologies: the sweep schedule of the device RATTLE is derived
+    from the bond order, so each of these gives it a different shape."""
+    if kind == "chain":
+        return [(i, i + 1) for i in range(n - 1)]
+    if kind == "chain_reversed":
+        return [(i + 1, i) for i in range(n - 1)]
+    if kind == "chain_shuffled":
+        b = [(i, i + 1) for i in range(n - 1)]
+        return [b[k] for k in rng.permutation(len(b))]
+    if kind == "ring":
+        return [(i, (i + 1) % n) for i in range(n)]
+    if kind == "star":
+        return [(0, i) for i in range(1, n)]
+    if kind == "tree":
+        return [(int(rng.integers(i)), i) for i in range(1, n)]
+    h = n // 2   # ladder: two rails and their rungs
+    return [(i, i + 1) for i in range(h - 1)] + [(h + i, h + i + 1) for i in range(h - 1)] + \
+        [(i, h + i) for i in range(h)]
+
+
+@pytest.mark.parametrize("phase", [0, 1])
+def test_rattle_topologies_bit_exact(dev, oracle, phase):
+    """RATTLE a and b alone on molecules of seven topologies (chains with the
+    bonds in order, reversed and shuffled, rings, stars, trees, ladders), up
+    to 64 bonds each, perturbed off their constraints: positions and
+    velocities equal the oracle's serial sweeps bit for bit (the device runs
+    the sweeps pipelined, each atom's updates in the serial order), and some
+    molecules run into the cap of 100 sweeps."""
+    say(f"rattle topologies phase={phase}")
+    import ctypes
+    rng = np.random.default_rng(77 + phase)
+    L = 30.0
+    kinds = ["chain", "chain_reversed", "chain_shuffled", "ring", "star", "tree", "ladder"]
+    old, bi, bj, lens = [], [], [], []
+    base = 0
+    for m in range(70):
+        kind = kinds[m % len(kinds)]
+        n = int(rng.integers(3, 41)) if kind != "chain" else int(rng.integers(30, 65))
+        bonds = _molecule_bonds(kind, n, rng)
+        # atoms on a random walk (stars and trees: around their bonded atom)
+        pts = np.zeros((n, 3))
+        pts[0] = rng.uniform(-L / 2, L / 2, 3)
+        placed = {0}
+        for a, b in sorted(bonds, key=lambda t: max(t)):
+            lo, hi = (a, b) if a in placed else (b, a)
+            if hi in placed:
+                continue
+            d = rng.normal(size=3)
+            pts[hi] = pts[lo] + d / np.linalg.norm(d)
+            placed.add(hi)
+        for a, b in bonds:
+            bi.append(base + a)
+            bj.append(base + b)
+            lens.append(float(np.linalg.norm(pts[a] - pts[b])))
+        old.append(pts)
+        base += n
+    old = np.concatenate(old)
+    old = old - np.round(old / L) * L
+    n = len(old)
+    st = base_state(rng, n, L)
+    # a larger kick on every 5th molecule's atoms drives it into the sweep cap
+    scale = np.where(np.arange(n) % 5 == 0, 0.06, 0.015)
+    pos = old + rng.normal(size=(n, 3)) * scale[:, None]
+    for k, f in enumerate(("rx", "ry", "rz")):
+        st[f] = pos[:, k] if phase == 0 else old[:, k]
+        st[f + "_old"] = old[:, k]
+    st.bbi, st.bbj, st.bb_len = np.array(bi, np.int32), np.array(bj, np.int32), np.array(lens)
+    st["virial_bonded"] = 0.0
+    to_device(dev, st)
+    ref = st.copy()
+    dt = 0.005
+    fn = "or_apply_constraints_a" if phase == 0 else "or_apply_constraints_b"
+    rc = oracle.call(fn, ref, ctypes.c_double(dt))
+    assert rc == 0
+    (dev.apply_constraints_a if phase == 0 else dev.apply_constraints_b)(dt)
+    got = from_device(dev, st)
+    for f in ("rx", "ry", "rz", "vx", "vy", "vz"):
+        assert np.array_equal(got[f], ref[f]), (f, np.abs(got[f] - ref[f]).max())
+    assert abs(got["virial_bonded"] - ref["virial_bonded"]) <= 1e-12 * max(1.0, abs(ref["virial_bonded"]))
+
+
 def test_bonded_terms(dev, oracle):
     say("bonded")
     rng = np.random.default_rng(77)
