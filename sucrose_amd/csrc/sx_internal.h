@@ -59,7 +59,7 @@ struct NList {
     // device arrays
     int *cell_of = nullptr, *cell_count = nullptr, *cell_start = nullptr, *cell_items = nullptr;
     int *row_len = nullptr, *row_start = nullptr, *lower = nullptr, *half_start = nullptr, *half_len = nullptr;
-    int *nbr = nullptr, *pairk = nullptr;
+    int *nbr = nullptr;
     int *fwdpos = nullptr, *revpos = nullptr;   // half pair k -> its entry in row i / in row j
     int *half_i = nullptr, *half_j = nullptr;
     double *x_last = nullptr, *y_last = nullptr, *z_last = nullptr;
@@ -68,7 +68,14 @@ struct NList {
     int max_row = 0;
     int lpr_rows = 16, lpr_half = 16;   // lanes per row of the row and half-index kernels
     int lb_grid = 0;                     // workgroups of the one-launch conditional rebuild (0: stage kernels)
-    unsigned long long *lb_sync = nullptr;   // its barrier flags and epoch
+    unsigned *lb_sync = nullptr;         // its barrier words and build epoch
+    int *lb_rows = nullptr;              // its scratch rows (n x lb_rc)
+    int lb_rc = 0;
+    int *lb_bsum = nullptr;              // its per-workgroup row sums
+    int *lb_bin_items = nullptr;         // its cell bins (ncells x lb_cap)
+    double *lb_bin_pos = nullptr;        // (x y z pad per slot)
+    int lb_cap = 0;
+    int *lb_meta = nullptr;              // its per-particle row records (int4)
     bool need_half = false;             // half list and pair index kept (ECP paths, export)
     long rebuilds = 0;
     // host copies of the sizes of the last build

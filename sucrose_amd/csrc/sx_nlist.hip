@@ -333,8 +333,8 @@ __global__ void __launch_bounds__(256) k_rows_cells_g(DState s, CellGrid g, cons
 // half-pair index with LPR lanes per row (entries of a row are independent)
 // row i's entries by lane `lane` of the row's L lanes
 __device__ __forceinline__ void half_index_row(int n, int i, int lane, int L, const int *row_start, const int *lower,
-                                               const int *half_start, const int *nbr, int *pairk, int *half_i,
-                                               int *half_j, int *fwdpos, int *revpos, long cap_half, int *err)
+                                               const int *half_start, const int *nbr, int *half_i, int *half_j,
+                                               int *fwdpos, int *revpos, long cap_half, int *err)
 {
     if ((long)half_start[n] > cap_half) {
         if (lane == 0) atomicCAS(err, 0, DERR_NLIST_CAP);
@@ -345,7 +345,6 @@ __device__ __forceinline__ void half_index_row(int n, int i, int lane, int L, co
         const int j = nbr[p];
         if (j > i) {
             const int k = half_start[i] + (p - b - lo);
-            pairk[p] = k;
             half_i[k] = i;
             half_j[k] = j;
             fwdpos[k] = p;
@@ -357,7 +356,6 @@ __device__ __forceinline__ void half_index_row(int n, int i, int lane, int L, co
                 if (nbr[m] < i) lb = m + 1; else ub = m;
             }
             const int k = half_start[j] + (lb - row_start[j] - lower[j]);
-            pairk[p] = k;
             revpos[k] = p;
         }
     }
@@ -365,15 +363,15 @@ __device__ __forceinline__ void half_index_row(int n, int i, int lane, int L, co
 
 template <int LPR>
 __global__ void __launch_bounds__(256) k_half_index_g(int n, const int *row_start, const int *lower,
-                                                      const int *half_start, const int *nbr, int *pairk, int *half_i,
-                                                      int *half_j, int *fwdpos, int *revpos, const int *flag,
-                                                      long cap_half, int *err)
+                                                      const int *half_start, const int *nbr, int *half_i, int *half_j,
+                                                      int *fwdpos, int *revpos, const int *flag, long cap_half,
+                                                      int *err)
 {
     if (flag && *flag == 0) return;
     const int i = blockIdx.x * (256 / LPR) + threadIdx.x / LPR;
     if (i >= n) return;
-    half_index_row(n, i, threadIdx.x % LPR, LPR, row_start, lower, half_start, nbr, pairk, half_i, half_j, fwdpos,
-                   revpos, cap_half, err);
+    half_index_row(n, i, threadIdx.x % LPR, LPR, row_start, lower, half_start, nbr, half_i, half_j, fwdpos, revpos,
+                   cap_half, err);
 }
 
 // brute force rows (small or non-periodic boxes)
@@ -482,37 +480,6 @@ __global__ void __launch_bounds__(256) k_sort_rows(int n, const int *row_start, 
     for (int i = blockIdx.x * 4 + w; i < n; i += gridDim.x * 4) sort_row(i, row_start, nbr, buf[w], lane, err);
 }
 
-// half-pair index of every full-list entry, and the half list (i, j) pairs
-__global__ void k_half_index(int n, const int *row_start, const int *lower, const int *half_start, const int *nbr,
-                             int *pairk, int *half_i, int *half_j, const int *flag, long cap_half, int *err)
-{
-    if (flag && *flag == 0) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int b = row_start[i], e = row_start[i + 1], lo = lower[i];
-    if ((long)half_start[n] > cap_half) {
-        atomicCAS(err, 0, DERR_NLIST_CAP);
-        return;
-    }
-    for (int p = b; p < e; ++p) {
-        int j = nbr[p];
-        if (j > i) {
-            int k = half_start[i] + (p - b - lo);
-            pairk[p] = k;
-            half_i[k] = i;
-            half_j[k] = j;
-        } else {
-            // position of i in row j (sorted): binary search
-            int lb = row_start[j], ub = row_start[j + 1];
-            while (lb < ub) {
-                int m = (lb + ub) >> 1;
-                if (nbr[m] < i) lb = m + 1; else ub = m;
-            }
-            pairk[p] = half_start[j] + (lb - row_start[j] - lower[j]);
-        }
-    }
-}
-
 __global__ void k_save_last(DState s, double *xl, double *yl, double *zl, const int *flag, long *count)
 {
     if (flag && *flag == 0) return;
@@ -568,154 +535,607 @@ __global__ void __launch_bounds__(256) k_disp_max(DState s, const double *xl, co
 }
 
 static CellGrid cell_grid(const sx_ctx *c);
+static bool lb_scratch(sx_ctx *c, int rc);
+static void lb_bins(sx_ctx *c, int cap);
+static int nl_max_row(sx_ctx *c);
 
 // ---------------------------------------------------------------------------
 // The conditional rebuild of a replayed step in one launch.  Between builds a
-// step pays one launch that reads the rebuild flag and returns (eight flag-
-// gated launches of the stage kernels above cost about 2 us each inside a
-// replayed graph).  When the flag is set, the same stages run one after the
-// other in LB_GRID resident workgroups of 1024 threads (one per CU), which
-// meet at device-wide barriers between stages: bin, cell starts (workgroup
-// 0), scatter, count rows, row and half-list starts (workgroup 0), fill rows,
-// sort rows, half index.  The stage code is the kernels' own.
+// step pays one launch that reads the rebuild flag and returns.  When the
+// flag is set, one resident workgroup of LB_THREADS threads per CU runs the
+// stages one after the other, meeting at LB_NBAR device-wide barriers:
+//   1. bin: each particle takes a slot of its cell (fixed room lb_cap per
+//      cell, an atomic count), recording the positions the list is built
+//      from;
+//   2. rows, four cells at a time per workgroup (cells handed out in
+//      batches by a counter), each by a quarter of the wavefronts: the
+//      candidates of the cell's 27 neighbour cells are staged in LDS once for
+//      all of the cell's particles; a wavefront takes a particle, tests every
+//      candidate with the reference's pair test, compacts the hits into LDS,
+//      ranks them (the row sorted ascending) and writes the row to a scratch
+//      area of fixed stride lb_rc;
+//   3. the row lengths summed over each workgroup's contiguous range of
+//      particles;
+//   4. each workgroup's offset from the range sums before it, the row (and
+//      half-list) starts of its range and a packed record per particle (row
+//      start, half start, lower count, row length); a total beyond the
+//      list's capacity, a cell beyond lb_cap or a row beyond lb_rc leaves an
+//      empty list for the host to grow (as the stage kernels do);
+//   5. the rows copied from the scratch into the list in particle order, and
+//      the half-pair index: forward entries from the row's own record,
+//      reverse entries by a binary search of the partner's sorted scratch
+//      row, several searches per lane in lockstep.
+// The pair set and the order of every row are those of the stage kernels.
+//
+// Barrier: every workgroup stores its arrival word (agent scope); the first
+// wavefront of workgroup 0 polls all of them, then stores LB_NREL release
+// words, each in its own cache line, which the workgroups poll (a sixteenth
+// of them each).  The bins, rows and row records (stages 1, 2 and 4) are
+// written with plain stores and one L2 write-back per workgroup before its
+// arrival; the range sums of stage 3 with agent-scope stores, written through
+// to memory, so that barrier needs no write-back.  A workgroup invalidates
+// its L2 and L1 once after each barrier before it reads.  Every wait gives up
+// after LB_SPIN polls (DERR_LIST_SYNC, the build abandoned), so that every
+// wait ends.
 // ---------------------------------------------------------------------------
 #define LB_THREADS 1024
 #define LB_WAVES (LB_THREADS / SX_WAVE)
-#define LB_GMAX 256          // most workgroups (four flags per lane of the polling wavefront)
-#define LB_SPIN (1 << 16)    // polls of one barrier before DERR_LIST_SYNC
+#define LB_NQ 4              // cells in flight per workgroup (a quarter of the wavefronts each)
+#define LB_QW (LB_WAVES / LB_NQ)
+#define LB_QT (LB_THREADS / LB_NQ)
+#define LB_TMAX 512          // candidates staged in LDS per cell (more: read from the bins)
+#define LB_RB 512            // longest row (LDS row buffer per wavefront)
+#define LB_NBAR 4            // barriers per build
+#define LB_NREL 16           // release words
+#define LB_SPIN (1 << 20)    // polls of one barrier before DERR_LIST_SYNC
+#define LB_ABORT 0xffffffffu
+// sync words: [0] build epoch, the cell-batch counters of even and odd
+// epochs at LB_CTR0 and LB_CTR0 + 32, release word r at LB_REL0 + 32 r,
+// arrival of workgroup b at LB_ARR0 + b
+#define LB_CTR0 32
+#define LB_REL0 96
+#define LB_ARR0 (LB_REL0 + 32 * LB_NREL)
+#define LB_SYNC_WORDS (LB_ARR0 + 4096)
 
 struct ListArgs {
     CellGrid g;
-    int *cell_of, *cell_count, *cell_start, *cell_items;
-    double4 *cell_pos;
+    int *cell_count;
+    int *bin_items;              // ncells x cap
+    double4 *bin_pos;
+    int cap;
     double *xl, *yl, *zl;
     long *count;
     int ncells;
     double rsq_cut;
     int *row_len, *lower, *half_len, *row_start, *nbr;
     long cap_full;
-    int need_half, lpr_half;
-    int *half_start, *pairk, *half_i, *half_j, *fwdpos, *revpos;
-    long cap_half;
-    int *total, *total_half;     // lengths of the full and half lists
+    int need_half;
+    int *half_start, *half_i, *half_j, *fwdpos, *revpos;
+    int *total, *total_half, *max_row, *max_cell;   // d_flag[1], [2], [3], [5]
+    int *rows;                   // scratch rows, stride rc
+    int rc;
+    int4 *meta;                  // per particle: row start, half start, lower count, row length
+    int *bsum;                   // per workgroup: full and half row sums of its range
     int *err;
-    unsigned long long *sync;    // per workgroup: epoch * 16 + the last stage finished; [LB_GMAX]: the epoch
+    unsigned *sync;
 };
 
-// Every workgroup has finished the stage that `target` names: each wavefront
-// waits for its own stores, one L2 write-back per workgroup makes them
-// visible to the other XCDs, then the flag; the first wavefront polls every
-// flag (lane l: workgroups l, l + 64, l + 128, l + 192, loaded before any is
-// tested), then one L2 invalidation before the workgroup reads what the
-// others wrote.  false after LB_SPIN polls (DERR_LIST_SYNC raised), so that
-// every wait ends.
-__device__ bool lb_barrier(const ListArgs &a, unsigned long long target)
+template <typename T>
+__device__ __forceinline__ void wt_store(T *p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ bool lb_barrier(const ListArgs &a, unsigned target, bool release)
 {
     __shared__ int ok_sh;
+    const int G = gridDim.x;
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_store(a.sync + blockIdx.x, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (release) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        wt_store(a.sync + LB_ARR0 + blockIdx.x, target);
     }
-    if (threadIdx.x < SX_WAVE) {
-        const int G = gridDim.x;
+    if (blockIdx.x == 0 && threadIdx.x < SX_WAVE) {
         bool done = false;
         for (int it = 0; it < LB_SPIN && !done; ++it) {
-            unsigned long long f[LB_GMAX / SX_WAVE];
-#pragma unroll
-            for (int k = 0; k < LB_GMAX / SX_WAVE; ++k) {
-                const int w = threadIdx.x + k * SX_WAVE;
-                f[k] = w < G ? __hip_atomic_load(a.sync + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : target;
-            }
             bool ok = true;
-#pragma unroll
-            for (int k = 0; k < LB_GMAX / SX_WAVE; ++k) ok = ok && f[k] >= target;
+            for (int b = threadIdx.x; b < G; b += SX_WAVE)
+                ok = ok && __hip_atomic_load(a.sync + LB_ARR0 + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target;
             done = __all(ok);
             if (!done) __builtin_amdgcn_s_sleep(1);
         }
-        if (threadIdx.x == 0) {
-            ok_sh = done;
-            if (!done) atomicCAS(a.err, DERR_NONE, DERR_LIST_SYNC);
+        if (!done && threadIdx.x == 0) atomicCAS(a.err, DERR_NONE, DERR_LIST_SYNC);
+        if (threadIdx.x < LB_NREL) wt_store(a.sync + LB_REL0 + 32 * threadIdx.x, done ? target : LB_ABORT);
+    }
+    if (threadIdx.x == 0) {
+        const unsigned *r = a.sync + LB_REL0 + 32 * (blockIdx.x % LB_NREL);
+        unsigned v = 0;
+        int it = 0;
+        for (; it < LB_SPIN; ++it) {
+            v = __hip_atomic_load(r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (v >= target) break;
+            __builtin_amdgcn_s_sleep(1);
         }
+        ok_sh = it < LB_SPIN && v != LB_ABORT;
+        if (it >= LB_SPIN) atomicCAS(a.err, DERR_NONE, DERR_LIST_SYNC);
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
     return ok_sh != 0;
 }
 
-template <int LPR>
-__global__ void __launch_bounds__(LB_THREADS) k_list_rebuild(DState s, ListArgs a, const int *flag)
+__device__ __forceinline__ void lb_range(int n, int b, int G, int &i0, int &i1)
 {
-    if (*flag == 0) return;
-    __shared__ int wsum[16];
-    __shared__ __align__(16) int sortbuf[LB_WAVES][SORT_CAP];
-    const int n = s.n, G = gridDim.x, lane = threadIdx.x % SX_WAVE, w = threadIdx.x / SX_WAVE;
-    const int tid = blockIdx.x * LB_THREADS + threadIdx.x, nth = G * LB_THREADS;
-    const unsigned long long ep =
-        __hip_atomic_load(a.sync + LB_GMAX, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) * 16ull;
-    // bin the particles, recording the positions the list is built from
-    if (tid == 0 && a.count) a.count[7] += 1;
-    for (int i = tid; i < n; i += nth) cell_assign_one(s, a.g, i, a.cell_of, a.cell_count, a.xl, a.yl, a.zl);
-    if (!lb_barrier(a, ep + 1)) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0)   // every workgroup has started: the next build's epoch
-        __hip_atomic_store(a.sync + LB_GMAX, ep / 16ull + 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    // cell starts
-    if (blockIdx.x == 0)
-        scan_body(a.cell_count, a.cell_start, a.ncells, nullptr, 0L, nullptr, nullptr, nullptr, nullptr, nullptr,
-                  wsum);
-    if (!lb_barrier(a, ep + 2)) return;
-    for (int i = tid; i < n; i += nth)
-        cell_scatter_one(s, i, a.cell_of, a.cell_start, a.cell_count, a.cell_items, a.cell_pos);
-    if (!lb_barrier(a, ep + 3)) return;
-    // rows in cell order, LPR lanes per row
-    for (int slot = tid / LPR; slot < n; slot += nth / LPR)
-        rows_cells_group<0, LPR>(s, a.g, a.cell_of, a.cell_start, a.cell_items, a.cell_pos, a.rsq_cut, a.row_len,
-                                 a.lower, a.half_len, a.row_start, a.nbr, a.cap_full, a.err, slot);
-    if (!lb_barrier(a, ep + 4)) return;
-    // row starts, and the half-list starts when the half list is kept
-    if (blockIdx.x == 0)
-        scan_body(a.row_len, a.row_start, n, a.total, a.cap_full, a.err, a.half_len,
-                  a.need_half ? (const int *)a.half_len : nullptr, a.half_start, a.total_half, wsum);
-    if (!lb_barrier(a, ep + 5)) return;
-    for (int slot = tid / LPR; slot < n; slot += nth / LPR)
-        rows_cells_group<1, LPR>(s, a.g, a.cell_of, a.cell_start, a.cell_items, a.cell_pos, a.rsq_cut, a.row_len,
-                                 a.lower, a.half_len, a.row_start, a.nbr, a.cap_full, a.err, slot);
-    if (!lb_barrier(a, ep + 6)) return;
-    for (int i = blockIdx.x * LB_WAVES + w; i < n; i += G * LB_WAVES) sort_row(i, a.row_start, a.nbr, sortbuf[w], lane, a.err);
-    if (!a.need_half) return;
-    if (!lb_barrier(a, ep + 7)) return;
-    const int L = a.lpr_half;
-    for (int i = tid / L; i < n; i += nth / L)
-        half_index_row(n, i, tid % L, L, a.row_start, a.lower, a.half_start, a.nbr, a.pairk, a.half_i, a.half_j,
-                       a.fwdpos, a.revpos, a.cap_half, a.err);
+    i0 = (int)((long)n * b / G);
+    i1 = (int)((long)n * (b + 1) / G);
 }
 
-template <int LPR>
+// the sorted row by ranks (as rank_row_entries), stored through to memory
+template <int PER>
+__device__ __forceinline__ void lb_rank_row(const int *sh, int len, int len4, int lane, int q0, int *row)
+{
+    int mine[PER], rank[PER];
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+        const int q = q0 + t * SX_WAVE + lane;
+        mine[t] = q < len ? sh[q] : 0x7fffffff;
+        rank[t] = 0;
+    }
+    const int4 *sh4 = (const int4 *)sh;
+    for (int m = 0; m < len4 / 4; ++m) {
+        const int4 v = sh4[m];
+#pragma unroll
+        for (int t = 0; t < PER; ++t)
+            rank[t] += (v.x < mine[t]) + (v.y < mine[t]) + (v.z < mine[t]) + (v.w < mine[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+        const int q = q0 + t * SX_WAVE + lane;
+        if (q < len) row[rank[t]] = mine[t];
+    }
+}
+
+// The reference's pair test for the build kernel's cell grids, where the
+// list radius is below half of every box edge.  Bit for bit the same squared
+// distance as in_list_pos without its canonical orientation: IEEE subtraction
+// and multiplication are antisymmetric and round-to-nearest-even is
+// symmetric, so (pi - pj) and (pj - pi) give the same squares.  The image
+// shift rounds to nearest even, which differs from the reference's
+// half-away rounding only at an exact half-box separation, where the pair is
+// beyond the list radius with either shift.
+__device__ __forceinline__ bool lb_pair_in(double4 pi, double4 pj, const DState &s, const CellGrid &g, double rsq_cut)
+{
+    double d0 = pi.x - pj.x, d1 = pi.y - pj.y, d2 = pi.z - pj.z;
+    d0 = d0 - rint(d0 * g.inv[0]) * s.box[0];
+    d1 = d1 - rint(d1 * g.inv[1]) * s.box[1];
+    d2 = d2 - rint(d2 * g.inv[2]) * s.box[2];
+    return d0 * d0 + d1 * d1 + d2 * d2 < rsq_cut;
+}
+
+// The candidates that pass the reference's pair test with the cell's
+// particle u (returned in i), compacted in candidate order into the LDS
+// buffer `row` (the first LB_RB of them); returns their number.  STAGED: the
+// candidates are in LDS (cj, cp), every one is tested first and the hits
+// compacted after (no LDS stores between the loads, so they pipeline);
+// otherwise they are read from the bins through the 27 ranges.
+template <bool STAGED>
+__device__ __forceinline__ int lb_row_hits(const DState &s, const ListArgs &a, const int *cj, const double4 *cp,
+                                           const int *rs, const int *ro, int T, int u, int &i, int *row)
+{
+    const int lane = threadIdx.x % SX_WAVE;
+    // particle u of the cell itself: neighbour range 13 (dx = dy = dz = 0)
+    double4 pi;
+    if (STAGED) {
+        i = cj[ro[13] + u];
+        pi = cp[ro[13] + u];
+    } else {
+        i = a.bin_items[rs[13] + u];
+        pi = a.bin_pos[rs[13] + u];
+    }
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int cnt = 0;
+    if (STAGED) {
+        unsigned long long m[LB_TMAX / SX_WAVE];
+#pragma unroll
+        for (int k = 0; k < LB_TMAX / SX_WAVE; ++k) {
+            m[k] = 0ull;
+            if (k * SX_WAVE < T) {
+                const int q = k * SX_WAVE + lane;
+                bool hit = false;
+                if (q < T) hit = cj[q] != i && lb_pair_in(pi, cp[q], s, a.g, a.rsq_cut);
+                m[k] = __ballot(hit);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < LB_TMAX / SX_WAVE; ++k) {
+            if (k * SX_WAVE >= T) break;
+            const int pos = cnt + __popcll(m[k] & below);
+            if (((m[k] >> lane) & 1ull) && pos < LB_RB) row[pos] = cj[k * SX_WAVE + lane];
+            cnt += __popcll(m[k]);
+        }
+        return cnt;
+    }
+    for (int q0 = 0; q0 < T; q0 += SX_WAVE) {
+        const int q = q0 + lane;
+        int j = -1;
+        bool hit = false;
+        if (q < T) {
+            int r = 0;
+            for (int st = 16; st > 0; st >>= 1)
+                if (r + st < 27 && ro[r + st] <= q) r += st;
+            const long src = rs[r] + q - ro[r];
+            j = a.bin_items[src];
+            hit = j != i && lb_pair_in(pi, a.bin_pos[src], s, a.g, a.rsq_cut);
+        }
+        const unsigned long long m = __ballot(hit);
+        const int pos = cnt + __popcll(m & below);
+        if (hit && pos < LB_RB) row[pos] = j;
+        cnt += __popcll(m);
+    }
+    return cnt;
+}
+
+// A row of at most 64 (distinct) entries in LDS sorted into dst: each lane
+// ranks its entry against the whole row, every load issued before the
+// comparisons
+__device__ __forceinline__ void lb_sort_short(const int *sh, int len, int lane, int *dst)
+{
+    const int mine = lane < len ? sh[lane] : 0x7fffffff;
+    const int4 *sh4 = reinterpret_cast<const int4 *>(sh);
+    const int n4 = (len + 3) >> 2;
+    int4 v[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m)
+        if (m < n4) v[m] = sh4[m];
+    int rank = 0;
+#pragma unroll
+    for (int m = 0; m < 16; ++m)
+        if (m < n4) rank += (v[m].x < mine) + (v[m].y < mine) + (v[m].z < mine) + (v[m].w < mine);
+    if (lane < len) dst[rank] = mine;
+}
+
+// Stage 5 of k_list_build for the entries of the rows i0 .. i0 + nr - 1,
+// whose packed records are ms[0..nr] (in LDS, or in memory); ms[nr] is the
+// next row's, whose start ends the range
+template <typename MS>
+__device__ __forceinline__ void lb_entries(const ListArgs &a, MS ms, int i0, int nr)
+{
+    const int t = threadIdx.x;
+    const int P0 = ms[0].x, P1 = ms[nr].x;
+    constexpr int ILP = 8;
+    for (int b0 = P0 + t; b0 < P1; b0 += ILP * LB_THREADS) {
+        int ie[ILP], je[ILP], tp[ILP], r_[ILP];
+#pragma unroll
+        for (int u = 0; u < ILP; ++u) {
+            const int p = b0 + u * LB_THREADS;
+            ie[u] = je[u] = tp[u] = r_[u] = -1;
+            if (p >= P1) continue;
+            int r = 0, hi = nr;   // the last row start <= p
+            while (hi - r > 1) {
+                const int mid = (r + hi) >> 1;
+                if (ms[mid].x <= p) r = mid; else hi = mid;
+            }
+            r_[u] = r;
+            ie[u] = i0 + r;
+            tp[u] = p - ms[r].x;
+            je[u] = a.rows[(long)ie[u] * a.rc + tp[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < ILP; ++u)
+            if (ie[u] >= 0) a.nbr[b0 + u * LB_THREADS] = je[u];
+        if (!a.need_half) continue;
+        // forward entries (j > i): the half index from row i's own record
+        int4 mj[ILP];
+#pragma unroll
+        for (int u = 0; u < ILP; ++u) {
+            if (ie[u] < 0) continue;
+            const int p = b0 + u * LB_THREADS;
+            if (je[u] > ie[u]) {
+                const int4 mi = ms[r_[u]];
+                const int k = mi.y + (tp[u] - mi.z);
+                a.half_i[k] = ie[u];
+                a.half_j[k] = je[u];
+                a.fwdpos[k] = p;
+            } else {
+                mj[u] = a.meta[je[u]];
+            }
+        }
+        // reverse entries: i's place among the upper entries (> j) of j's
+        // sorted row, by binary searches of [lower[j], row_len[j]) in lockstep
+        int lo[ILP], hi[ILP];
+#pragma unroll
+        for (int u = 0; u < ILP; ++u) {
+            const bool rev = ie[u] >= 0 && je[u] < ie[u];
+            lo[u] = rev ? mj[u].z : 0;
+            hi[u] = rev ? mj[u].w : 0;
+        }
+        for (;;) {
+            int v[ILP];
+            bool any = false;
+#pragma unroll
+            for (int u = 0; u < ILP; ++u)
+                if (lo[u] < hi[u]) v[u] = a.rows[(long)je[u] * a.rc + ((lo[u] + hi[u]) >> 1)];
+#pragma unroll
+            for (int u = 0; u < ILP; ++u)
+                if (lo[u] < hi[u]) {
+                    const int mid = (lo[u] + hi[u]) >> 1;
+                    if (v[u] < ie[u]) lo[u] = mid + 1; else hi[u] = mid;
+                    any = true;
+                }
+            if (!any) break;
+        }
+#pragma unroll
+        for (int u = 0; u < ILP; ++u) {
+            const int p = b0 + u * LB_THREADS;
+            if (ie[u] < 0 || je[u] > ie[u]) continue;
+            const int k = mj[u].y + (lo[u] - mj[u].z);
+            a.revpos[k] = p;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(LB_THREADS) k_list_build(DState s, ListArgs a, const int *flag)
+{
+    if (*flag == 0) return;
+    __shared__ int rng_s[LB_NQ][27], rng_o[LB_NQ][28];
+    __shared__ int cand_j[LB_NQ][LB_TMAX];
+    __shared__ double4 cand_p[LB_NQ][LB_TMAX];
+    __shared__ __align__(16) int rowbuf[LB_WAVES][LB_RB + 4];
+    __shared__ int maxrow_sh, over_sh, batch_sh;
+    __shared__ long red[4][LB_WAVES];
+    const int n = s.n, G = gridDim.x, t = threadIdx.x, lane = t % SX_WAVE, w = t / SX_WAVE;
+    const int tid = blockIdx.x * LB_THREADS + t, nth = G * LB_THREADS;
+    const unsigned ep = __hip_atomic_load(a.sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned base = ep * LB_NBAR;
+    // 1. bin, recording the positions the list is built from
+    if (tid == 0) {
+        if (a.count) a.count[7] += 1;
+        wt_store(a.max_row, 0);
+    }
+    for (int i = tid; i < n; i += nth) {
+        const double x = s.f[SX_RX][i], y = s.f[SX_RY][i], z = s.f[SX_RZ][i];
+        a.xl[i] = x; a.yl[i] = y; a.zl[i] = z;
+        const int cx = cell_coord(x, s.box[0], a.g.nc[0], a.g.w[0]);
+        const int cy = cell_coord(y, s.box[1], a.g.nc[1], a.g.w[1]);
+        const int cz = cell_coord(z, s.box[2], a.g.nc[2], a.g.w[2]);
+        const int c = (cx * a.g.nc[1] + cy) * a.g.nc[2] + cz;
+        const int slot = atomicAdd(&a.cell_count[c], 1);
+        if (slot < a.cap) {
+            const long q = (long)c * a.cap + slot;
+            a.bin_items[q] = i;
+            a.bin_pos[q] = make_double4(x, y, z, 0.0);
+        } else {
+            atomicMax(a.max_cell, slot + 1);
+        }
+    }
+    if (!lb_barrier(a, base + 1, true)) return;
+    if (blockIdx.x == 0 && t == 0) wt_store(a.sync, ep + 1u);   // every workgroup has read the epoch
+    // 2. rows, LB_NQ cells at a time: quarter h of the wavefronts takes cell c0 + h
+    if (t == 0) maxrow_sh = 0;
+    const int nx = a.g.nc[0], ny = a.g.nc[1], nz = a.g.nc[2];
+    const int h = w / LB_QW, ht = t % LB_QT, hw = w % LB_QW;
+    int *myrow = rowbuf[w];
+    unsigned *ctr = a.sync + LB_CTR0 + 32 * (ep & 1u);
+    for (;;) {
+        // the next batch of LB_NQ cells (batches handed out in order)
+        if (t == 0) batch_sh = (int)atomicAdd(ctr, 1u);
+        __syncthreads();
+        const int c0 = LB_NQ * batch_sh;
+        if (c0 >= a.ncells) break;
+        const int c = c0 + h;
+        const bool live = c < a.ncells;
+        int len = 0;
+        if (live && ht < 27) {
+            const int cz = c % nz, cy = (c / nz) % ny, cx = c / (nz * ny);
+            const int x = (cx + ht / 9 - 1 + nx) % nx, y = (cy + (ht / 3) % 3 - 1 + ny) % ny;
+            const int z = (cz + ht % 3 - 1 + nz) % nz;
+            const int cc = (x * ny + y) * nz + z;
+            rng_s[h][ht] = cc * a.cap;
+            len = min(a.cell_count[cc], a.cap);
+        }
+        if (hw == 0) {   // offsets of the 27 ranges
+            int x = len;
+            for (int off = 1; off < 32; off <<= 1) {
+                const int y = __shfl_up(x, off, SX_WAVE);
+                if (lane >= off) x += y;
+            }
+            if (lane < 27) rng_o[h][lane + 1] = x;
+            if (lane == 0) rng_o[h][0] = 0;
+        }
+        __syncthreads();
+        const int T = live ? rng_o[h][27] : 0;
+        const bool staged = T <= LB_TMAX;
+        if (staged)
+            for (int q = ht; q < T; q += LB_QT) {
+                int r = 0;
+                for (int st = 16; st > 0; st >>= 1)
+                    if (r + st < 27 && rng_o[h][r + st] <= q) r += st;
+                const long src = rng_s[h][r] + q - rng_o[h][r];
+                cand_j[h][q] = a.bin_items[src];
+                cand_p[h][q] = a.bin_pos[src];
+            }
+        __syncthreads();
+        const int own = live ? rng_o[h][14] - rng_o[h][13] : 0;
+        for (int u = hw; u < own; u += LB_QW) {
+            int i;
+            const int cnt = staged ? lb_row_hits<true>(s, a, cand_j[h], cand_p[h], rng_s[h], rng_o[h], T, u, i, myrow)
+                                   : lb_row_hits<false>(s, a, cand_j[h], cand_p[h], rng_s[h], rng_o[h], T, u, i, myrow);
+            int low = 0;
+            if (cnt <= LB_RB && cnt <= a.rc) {
+                // each entry's rank among the row's (distinct) entries is its
+                // place in the sorted row (padded to a multiple of 4)
+                const int cnt4 = (cnt + 3) & ~3;
+                for (int e = cnt + lane; e < cnt4; e += SX_WAVE) myrow[e] = 0x7fffffff;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                int *dst = a.rows + (long)i * a.rc;
+                if (cnt <= SX_WAVE) {
+                    lb_sort_short(myrow, cnt, lane, dst);
+                } else {
+                    const int per = min(SORT_PER_LANE, (cnt + SX_WAVE - 1) / SX_WAVE);
+                    for (int q0 = 0; q0 < cnt; q0 += SX_WAVE * per) {
+                        switch (per) {
+                        case 1: lb_rank_row<1>(myrow, cnt, cnt4, lane, q0, dst); break;
+                        case 2: lb_rank_row<2>(myrow, cnt, cnt4, lane, q0, dst); break;
+                        case 3: lb_rank_row<3>(myrow, cnt, cnt4, lane, q0, dst); break;
+                        case 4: lb_rank_row<4>(myrow, cnt, cnt4, lane, q0, dst); break;
+                        case 5: lb_rank_row<5>(myrow, cnt, cnt4, lane, q0, dst); break;
+                        case 6: lb_rank_row<6>(myrow, cnt, cnt4, lane, q0, dst); break;
+                        case 7: lb_rank_row<7>(myrow, cnt, cnt4, lane, q0, dst); break;
+                        default: lb_rank_row<8>(myrow, cnt, cnt4, lane, q0, dst); break;
+                        }
+                    }
+                }
+                for (int e = lane; e < cnt; e += SX_WAVE) low += myrow[e] < i;
+            }
+            for (int off = SX_WAVE / 2; off > 0; off >>= 1) low += __shfl_xor(low, off, SX_WAVE);
+            if (lane == 0) {
+                a.row_len[i] = cnt;
+                a.lower[i] = low;
+                a.half_len[i] = cnt - low;
+                atomicMax(&maxrow_sh, cnt);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        __syncthreads();   // the staged candidates are replaced next
+    }
+    if (t == 0) atomicMax(a.max_row, maxrow_sh);
+    if (!lb_barrier(a, base + 2, true)) return;
+    // 3. sums of the row lengths of this workgroup's particles; the cell
+    // counts and this epoch's batch counter back to zero for later builds
+    for (int c = tid; c < a.ncells; c += nth) a.cell_count[c] = 0;
+    if (tid == 0) wt_store(ctr, 0u);
+    int i0, i1;
+    lb_range(n, blockIdx.x, G, i0, i1);
+    {
+        long f = 0, hh = 0;
+        for (int i = i0 + t; i < i1; i += LB_THREADS) {
+            f += a.row_len[i];
+            hh += a.half_len[i];
+        }
+        for (int off = SX_WAVE / 2; off > 0; off >>= 1) {
+            f += __shfl_xor(f, off, SX_WAVE);
+            hh += __shfl_xor(hh, off, SX_WAVE);
+        }
+        if (lane == 0) { red[0][w] = f; red[1][w] = hh; }
+        __syncthreads();
+        if (t == 0) {
+            long F = 0, H = 0;
+            for (int q = 0; q < LB_WAVES; ++q) { F += red[0][q]; H += red[1][q]; }
+            wt_store(a.bsum + 2 * blockIdx.x, (int)F);
+            wt_store(a.bsum + 2 * blockIdx.x + 1, (int)H);
+        }
+    }
+    if (!lb_barrier(a, base + 3, false)) return;
+    // 4. offsets, totals and the starts of this workgroup's rows
+    {
+        long fb = 0, hb = 0, ft = 0, ht2 = 0;
+        for (int b = t; b < G; b += LB_THREADS) {
+            const long f = a.bsum[2 * b], hh = a.bsum[2 * b + 1];
+            if (b < (int)blockIdx.x) { fb += f; hb += hh; }
+            ft += f;
+            ht2 += hh;
+        }
+        for (int off = SX_WAVE / 2; off > 0; off >>= 1) {
+            fb += __shfl_xor(fb, off, SX_WAVE);
+            hb += __shfl_xor(hb, off, SX_WAVE);
+            ft += __shfl_xor(ft, off, SX_WAVE);
+            ht2 += __shfl_xor(ht2, off, SX_WAVE);
+        }
+        __syncthreads();
+        if (lane == 0) { red[0][w] = fb; red[1][w] = hb; red[2][w] = ft; red[3][w] = ht2; }
+        __syncthreads();
+        fb = hb = ft = ht2 = 0;
+        for (int q = 0; q < LB_WAVES; ++q) { fb += red[0][q]; hb += red[1][q]; ft += red[2][q]; ht2 += red[3][q]; }
+        // a cell beyond its room or a row beyond the scratch stride or the
+        // LDS row buffer leaves the list empty too
+        const int mr = __hip_atomic_load(a.max_row, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int mc = __hip_atomic_load(a.max_cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const bool roomerr = mr > min(LB_RB, a.rc) || mc > a.cap;
+        const bool over = ft > a.cap_full || roomerr;   // uniform: every workgroup sees the same sums
+        if (t == 0) {
+            over_sh = over;
+            if (blockIdx.x == 0) {
+                *a.total = ft > 0x7fffffffL ? 0x7fffffff : (int)ft;
+                *a.total_half = over ? 0 : (int)ht2;
+                if (over) {
+                    atomicCAS(a.err, DERR_NONE, roomerr ? DERR_ROW_CAP : DERR_NLIST_CAP);
+                } else {
+                    a.row_start[n] = (int)ft;
+                    if (a.need_half) a.half_start[n] = (int)ht2;
+                    a.meta[n] = make_int4((int)ft, (int)ht2, 0, 0);
+                }
+            }
+        }
+        if (over) {   // an empty list (as the stage kernels leave one)
+            for (int i = i0 + t; i < i1; i += LB_THREADS) {
+                a.row_start[i] = 0;
+                a.half_len[i] = 0;
+                if (a.need_half) a.half_start[i] = 0;
+            }
+            if (blockIdx.x == 0 && t == 0) {
+                a.row_start[n] = 0;
+                if (a.need_half) a.half_start[n] = 0;
+            }
+        } else if (w == 0) {
+            long cf = fb, ch = hb;
+            for (int b0 = i0; b0 < i1; b0 += SX_WAVE) {
+                const int i = b0 + lane;
+                const int f = i < i1 ? a.row_len[i] : 0, hh = i < i1 ? a.half_len[i] : 0;
+                int xf = f, xh = hh;
+                for (int off = 1; off < SX_WAVE; off <<= 1) {
+                    const int yf = __shfl_up(xf, off, SX_WAVE), yh = __shfl_up(xh, off, SX_WAVE);
+                    if (lane >= off) { xf += yf; xh += yh; }
+                }
+                if (i < i1) {
+                    const int rsi = (int)(cf + xf - f), hsi = (int)(ch + xh - hh);
+                    a.row_start[i] = rsi;
+                    if (a.need_half) a.half_start[i] = hsi;
+                    a.meta[i] = make_int4(rsi, hsi, f - hh, f);
+                }
+                cf += __shfl(xf, SX_WAVE - 1, SX_WAVE);
+                ch += __shfl(xh, SX_WAVE - 1, SX_WAVE);
+            }
+        }
+    }
+    __syncthreads();
+    const bool over = over_sh != 0;
+    if (!lb_barrier(a, base + 4, true)) return;
+    if (over) return;
+    // 5. rows into the list and the half-pair index, over this workgroup's
+    // entries [row_start[i0], row_start[i1]); the row of an entry from a
+    // search of the range's row starts (in LDS when they fit)
+    const int nr = i1 - i0;
+    int4 *ms_sh = reinterpret_cast<int4 *>(&cand_p[0][0]);
+    if (nr + 1 <= LB_NQ * LB_TMAX * 2) {
+        for (int q = t; q <= nr; q += LB_THREADS) ms_sh[q] = a.meta[i0 + q];
+        __syncthreads();
+        lb_entries(a, ms_sh, i0, nr);
+    } else {
+        lb_entries(a, (const int4 *)(a.meta + i0), i0, nr);
+    }
+}
+
 static int lb_blocks_per_cu()
 {
     int per = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_list_rebuild<LPR>, LB_THREADS, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_list_build, LB_THREADS, 0) != hipSuccess) return 0;
     return per;
 }
 
-// workgroups of the one-launch rebuild for this device and row width, 0 when
-// a workgroup of it does not fit a CU
-static int lb_grid_for(int lpr)
+// workgroups of the one-launch rebuild: one per CU (every one of them
+// resident at once), 0 when a workgroup does not fit a CU
+static int lb_grid_for()
 {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 0;
-    int per = 0;
-    switch (lpr) {
-    case 4: per = lb_blocks_per_cu<4>(); break;
-    case 8: per = lb_blocks_per_cu<8>(); break;
-    case 16: per = lb_blocks_per_cu<16>(); break;
-    case 32: per = lb_blocks_per_cu<32>(); break;
-    default: per = lb_blocks_per_cu<64>(); break;
-    }
-    return per >= 1 ? std::min(cus, LB_GMAX) : 0;
+    return lb_blocks_per_cu() >= 1 ? std::min(cus, LB_SYNC_WORDS - LB_ARR0) : 0;
 }
 
 static void launch_list_rebuild(sx_ctx *c, const int *flag)
@@ -723,8 +1143,10 @@ static void launch_list_rebuild(sx_ctx *c, const int *flag)
     NList &nl = c->nl;
     ListArgs a;
     a.g = cell_grid(c);
-    a.cell_of = nl.cell_of; a.cell_count = nl.cell_count; a.cell_start = nl.cell_start; a.cell_items = nl.cell_items;
-    a.cell_pos = (double4 *)nl.cell_pos;
+    a.cell_count = nl.cell_count;
+    a.bin_items = nl.lb_bin_items;
+    a.bin_pos = (double4 *)nl.lb_bin_pos;
+    a.cap = nl.lb_cap;
     a.xl = nl.x_last; a.yl = nl.y_last; a.zl = nl.z_last;
     a.count = c->d_counts;
     a.ncells = nl.ncells;
@@ -732,33 +1154,27 @@ static void launch_list_rebuild(sx_ctx *c, const int *flag)
     a.row_len = nl.row_len; a.lower = nl.lower; a.half_len = nl.half_len; a.row_start = nl.row_start; a.nbr = nl.nbr;
     a.cap_full = nl.cap_full;
     a.need_half = nl.need_half ? 1 : 0;
-    a.lpr_half = nl.lpr_half;
-    a.half_start = nl.half_start; a.pairk = nl.pairk; a.half_i = nl.half_i; a.half_j = nl.half_j;
+    a.half_start = nl.half_start; a.half_i = nl.half_i; a.half_j = nl.half_j;
     a.fwdpos = nl.fwdpos; a.revpos = nl.revpos;
-    a.cap_half = nl.cap_half;
-    a.total = c->d_flag + 1; a.total_half = c->d_flag + 2;
+    a.total = c->d_flag + 1; a.total_half = c->d_flag + 2; a.max_row = c->d_flag + 3; a.max_cell = c->d_flag + 5;
+    a.rows = nl.lb_rows;
+    a.rc = nl.lb_rc;
+    a.meta = (int4 *)nl.lb_meta;
+    a.bsum = nl.lb_bsum;
     a.err = c->d_err;
     a.sync = nl.lb_sync;
-    DState s = c->dstate();
-    dim3 grid(nl.lb_grid), blk(LB_THREADS);
-    switch (nl.lpr_rows) {
-    case 4: hipLaunchKernelGGL(k_list_rebuild<4>, grid, blk, 0, c->stream, s, a, flag); break;
-    case 8: hipLaunchKernelGGL(k_list_rebuild<8>, grid, blk, 0, c->stream, s, a, flag); break;
-    case 16: hipLaunchKernelGGL(k_list_rebuild<16>, grid, blk, 0, c->stream, s, a, flag); break;
-    case 32: hipLaunchKernelGGL(k_list_rebuild<32>, grid, blk, 0, c->stream, s, a, flag); break;
-    default: hipLaunchKernelGGL(k_list_rebuild<64>, grid, blk, 0, c->stream, s, a, flag); break;
-    }
+    hipLaunchKernelGGL(k_list_build, dim3(nl.lb_grid), dim3(LB_THREADS), 0, c->stream, c->dstate(), a, flag);
 }
 
 // after DERR_LIST_SYNC: a build whose workgroups gave up waiting left its
-// flags mid-build
+// arrival count mid-build
 void sx_nlist_reset_sync(sx_ctx *c)
 {
     NList &nl = c->nl;
-    if (nl.lb_sync) HIPCHK(hipMemsetAsync(nl.lb_sync, 0, sizeof(unsigned long long) * (LB_GMAX + 1), c->stream));
+    if (nl.lb_sync) HIPCHK(hipMemsetAsync(nl.lb_sync, 0, sizeof(unsigned) * LB_SYNC_WORDS, c->stream));
+    if (nl.ncells > 0) HIPCHK(hipMemsetAsync(nl.cell_count, 0, sizeof(int) * (nl.ncells + 1), c->stream));
 }
 
-// ---------------------------------------------------------------------------
 static void launch_rows_g(sx_ctx *c, int pass, const CellGrid &g, const int *flag)
 {
     NList &nl = c->nl;
@@ -795,7 +1211,7 @@ static void launch_half_index_g(sx_ctx *c, const int *flag)
     dim3 grid((c->n + 256 / L - 1) / (256 / L)), blk(256);
 #define SX_HALF_G(LL)                                                                                               \
     hipLaunchKernelGGL((k_half_index_g<LL>), grid, blk, 0, c->stream, c->n, nl.row_start, nl.lower, nl.half_start, \
-                       nl.nbr, nl.pairk, nl.half_i, nl.half_j, nl.fwdpos, nl.revpos, flag, nl.cap_half, c->d_err)
+                       nl.nbr, nl.half_i, nl.half_j, nl.fwdpos, nl.revpos, flag, nl.cap_half, c->d_err)
     switch (L) {
     case 4: SX_HALF_G(4); break;
     case 8: SX_HALF_G(8); break;
@@ -816,17 +1232,26 @@ static int pow2_clamp(double x, int lo, int hi)
 static void nl_free(NList &nl)
 {
     int *ip[] = {nl.cell_of, nl.cell_count, nl.cell_start, nl.cell_items, nl.row_len, nl.row_start, nl.lower,
-                 nl.half_start, nl.half_len, nl.nbr, nl.pairk, nl.half_i, nl.half_j, nl.fwdpos, nl.revpos};
+                 nl.half_start, nl.half_len, nl.nbr, nl.half_i, nl.half_j, nl.fwdpos, nl.revpos};
     for (int *p : ip)
         if (p) hipFree(p);
     double *dp[] = {nl.x_last, nl.y_last, nl.z_last, nl.cell_pos};
     for (double *p : dp)
         if (p) hipFree(p);
     nl.cell_of = nl.cell_count = nl.cell_start = nl.cell_items = nl.row_len = nl.row_start = nl.lower = nullptr;
-    nl.half_start = nl.half_len = nl.nbr = nl.pairk = nl.half_i = nl.half_j = nl.fwdpos = nl.revpos = nullptr;
+    nl.half_start = nl.half_len = nl.nbr = nl.half_i = nl.half_j = nl.fwdpos = nl.revpos = nullptr;
     nl.x_last = nl.y_last = nl.z_last = nl.cell_pos = nullptr;
     if (nl.lb_sync) hipFree(nl.lb_sync);
+    if (nl.lb_rows) hipFree(nl.lb_rows);
+    if (nl.lb_bsum) hipFree(nl.lb_bsum);
+    if (nl.lb_bin_items) hipFree(nl.lb_bin_items);
+    if (nl.lb_bin_pos) hipFree(nl.lb_bin_pos);
+    if (nl.lb_meta) hipFree(nl.lb_meta);
+    nl.lb_meta = nullptr;
     nl.lb_sync = nullptr;
+    nl.lb_rows = nl.lb_bsum = nl.lb_bin_items = nullptr;
+    nl.lb_bin_pos = nullptr;
+    nl.lb_rc = nl.lb_cap = 0;
     nl.lb_grid = 0;
 }
 
@@ -834,7 +1259,6 @@ static void nl_alloc_lists(sx_ctx *c, long cap_full)
 {
     NList &nl = c->nl;
     if (nl.nbr) hipFree(nl.nbr);
-    if (nl.pairk) hipFree(nl.pairk);
     if (nl.half_i) hipFree(nl.half_i);
     if (nl.half_j) hipFree(nl.half_j);
     if (nl.fwdpos) hipFree(nl.fwdpos);
@@ -843,7 +1267,6 @@ static void nl_alloc_lists(sx_ctx *c, long cap_full)
     nl.cap_full = cap_full;
     nl.cap_half = cap_full / 2 + 1;
     HIPCHK(hipMalloc(&nl.nbr, sizeof(int) * nl.cap_full));
-    HIPCHK(hipMalloc(&nl.pairk, sizeof(int) * nl.cap_full));
     HIPCHK(hipMalloc(&nl.half_i, sizeof(int) * nl.cap_half));
     HIPCHK(hipMalloc(&nl.half_j, sizeof(int) * nl.cap_half));
     HIPCHK(hipMalloc(&nl.fwdpos, sizeof(int) * nl.cap_half));
@@ -859,7 +1282,7 @@ static void nl_grow_keep(sx_ctx *c, long cap_full)
     if (cap_full <= nl.cap_full) return;
     if (cap_full > 0x7fffffffL) throw sx_error{SX_ERR_CAPACITY, "neighbour list larger than 2^31 entries"};
     const long cap_half = cap_full / 2 + 1;
-    int **full[] = {&nl.nbr, &nl.pairk};
+    int **full[] = {&nl.nbr};
     int **half[] = {&nl.half_i, &nl.half_j, &nl.fwdpos, &nl.revpos};
     for (int **a : full) {
         int *p = nullptr;
@@ -900,6 +1323,11 @@ void sx_nlist_headroom(sx_ctx *c)
     if (!nl.init) return;
     const long total = nl_last_total(c);
     if (3 * total / 2 + 1024 > nl.cap_full) nl_grow_keep(c, 2 * total + 2048);
+    // rows of the one-launch rebuild: room for half as long again
+    if (nl.lb_grid > 0) {
+        const int mr = nl_max_row(c);
+        if (3 * mr / 2 + 32 > nl.lb_rc && !lb_scratch(c, 2 * mr + 64)) nl.lb_grid = 0;
+    }
 }
 
 // After graph replays: a rebuild that did not fit emptied the list (k_scan)
@@ -923,8 +1351,81 @@ void sx_nlist_check_after_replay(sx_ctx *c, bool headroom)
                            std::to_string(had) + "): from that rebuild on the list was empty and the steps ran "
                            "without pair forces; the list has been grown and rebuilt at the current positions"};
     }
+    if (e == DERR_ROW_CAP && nl.init && nl.lb_grid > 0) {
+        // a row (d_flag[3]: the longest) or a cell (d_flag[5]: the fullest,
+        // when one overflowed) of a replayed rebuild outgrew its room; the
+        // list of that rebuild was left empty
+        int f[6] = {0, 0, 0, 0, 0, 0};
+        HIPCHK(hipMemcpyAsync(f, c->d_flag, sizeof(int) * 6, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        const int need_row = f[3], need_cell = f[5], had_row = nl.lb_rc, had_cell = nl.lb_cap;
+        HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+        HIPCHK(hipMemsetAsync(c->d_flag + 5, 0, sizeof(int), c->stream));
+        if (need_row > had_row && !lb_scratch(c, 2 * need_row + 64)) nl.lb_grid = 0;
+        if (need_cell > had_cell) lb_bins(c, 2 * need_cell + 32);
+        sx_nlist_build_impl(c, false);
+        throw sx_error{SX_ERR_CAPACITY,
+                       "neighbour row or cell larger than the rebuild's room during the run (row " +
+                           std::to_string(need_row) + " entries, room for " + std::to_string(had_row) + "; cell " +
+                           std::to_string(need_cell) + " particles, room for " + std::to_string(had_cell) +
+                           "): from that rebuild on the list was empty and the steps ran without pair forces; the "
+                           "room has been grown and the list rebuilt at the current positions"};
+    }
     sx_check_device_error(c);
     if (headroom) sx_nlist_headroom(c);
+}
+
+// scratch rows of the one-launch rebuild with stride rc (at most the LDS row
+// buffer); false when rows that long do not fit it (the stage kernels build
+// then)
+static bool lb_scratch(sx_ctx *c, int rc)
+{
+    NList &nl = c->nl;
+    rc = (std::max(rc, 64) + 3) & ~3;   // rows 16-byte aligned
+    if (rc > LB_RB) return false;
+    if (nl.lb_rows && rc <= nl.lb_rc) return true;
+    if (nl.lb_rows) hipFree(nl.lb_rows);
+    nl.lb_rows = nullptr;
+    HIPCHK(hipMalloc(&nl.lb_rows, sizeof(int) * (size_t)c->n * rc));
+    nl.lb_rc = rc;
+    c->have_graph = false;
+    return true;
+}
+
+// cell bins of the one-launch rebuild with room for cap particles per cell
+static void lb_bins(sx_ctx *c, int cap)
+{
+    NList &nl = c->nl;
+    cap = std::max(cap, 16);
+    if (nl.lb_bin_items && cap <= nl.lb_cap) return;
+    if (nl.lb_bin_items) hipFree(nl.lb_bin_items);
+    if (nl.lb_bin_pos) hipFree(nl.lb_bin_pos);
+    nl.lb_bin_items = nullptr;
+    nl.lb_bin_pos = nullptr;
+    HIPCHK(hipMalloc(&nl.lb_bin_items, sizeof(int) * (size_t)nl.ncells * cap));
+    HIPCHK(hipMalloc(&nl.lb_bin_pos, sizeof(double) * 4 * (size_t)nl.ncells * cap));
+    nl.lb_cap = cap;
+    c->have_graph = false;
+}
+
+__global__ void k_max_row(const int *row_len, int n, int *out)
+{
+    int m = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) m = max(m, row_len[i]);
+    for (int off = SX_WAVE / 2; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, SX_WAVE));
+    if ((threadIdx.x & (SX_WAVE - 1)) == 0) atomicMax(out, m);
+}
+
+// the longest row of the current list (d_flag[3])
+static int nl_max_row(sx_ctx *c)
+{
+    int m = 0;
+    HIPCHK(hipMemsetAsync(c->d_flag + 3, 0, sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_max_row, dim3(std::min(sx_blocks(c->n, 256), 256)), dim3(256), 0, c->stream, c->nl.row_len,
+                       c->n, c->d_flag + 3);
+    HIPCHK(hipMemcpyAsync(&m, c->d_flag + 3, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return m;
 }
 
 static void nl_setup(sx_ctx *c)
@@ -980,19 +1481,24 @@ static void nl_setup(sx_ctx *c)
             if (b >= 1) nl.lpr_half = pow2_clamp(b, 4, 64);
         }
     }
-    // the one-launch conditional rebuild (cell grids only)
+    // the one-launch conditional rebuild (cell grids only), its scratch rows
+    // sized for twice the expected row
     if (nl.use_cells) {
-        nl.lb_grid = lb_grid_for(nl.lpr_rows);
+        nl.lb_grid = lb_grid_for();
+        if (nl.lb_grid > 0 && !lb_scratch(c, (int)std::ceil(2.0 * row_est) + 64)) nl.lb_grid = 0;
         if (nl.lb_grid > 0) {
-            HIPCHK(hipMalloc(&nl.lb_sync, sizeof(unsigned long long) * (LB_GMAX + 1)));
-            HIPCHK(hipMemsetAsync(nl.lb_sync, 0, sizeof(unsigned long long) * (LB_GMAX + 1), c->stream));
+            HIPCHK(hipMalloc(&nl.lb_sync, sizeof(unsigned) * LB_SYNC_WORDS));
+            HIPCHK(hipMemsetAsync(nl.lb_sync, 0, sizeof(unsigned) * LB_SYNC_WORDS, c->stream));
+            HIPCHK(hipMalloc(&nl.lb_bsum, sizeof(int) * 2 * nl.lb_grid));
+            HIPCHK(hipMalloc(&nl.lb_meta, sizeof(int) * 4 * ((size_t)n + 1)));
+            lb_bins(c, (int)std::ceil(2.0 * n / nl.ncells) + 32);
         }
     }
     nl.init = true;
 }
 
 // the reference's half list (half_start, half_i, half_j; d_flag[2] = its
-// length) and pairk, from the sorted rows
+// length) and the pair positions, from the sorted rows
 static void half_index_launch(sx_ctx *c, const int *flag)
 {
     NList &nl = c->nl;
@@ -1040,7 +1546,7 @@ void sx_nlist_build_impl(sx_ctx *c, bool conditional)
     DState s = c->dstate();
     const int *flag = conditional ? c->d_flag : nullptr;
     int nb = sx_blocks(n, 256);
-    if (conditional && nl.use_cells && nl.lb_grid > 0) {
+    if (conditional && nl.use_cells && nl.lb_grid > 0 && cell_grid(c).fast) {
         launch_list_rebuild(c, flag);
         return;
     }

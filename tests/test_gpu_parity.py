@@ -119,6 +119,49 @@ def test_nlist_pair_set_exact(dev, oracle, n, L, rc, skin):
     assert off[-1] == nnz + 1
 
 
+@pytest.mark.parametrize("n,L,rc,skin", [(4096, 30.0, 3.1, 0.3), (8000, 24.0, 2.5, 0.3)])
+def test_nlist_one_launch_rebuild_exact(dev, oracle, n, L, rc, skin):
+    """The conditional rebuild that a step launches (one launch: binning,
+    rows from LDS-staged neighbour cells, scan, compaction and half index)
+    gives the reference's list: after single steps whose moves force a
+    rebuild, the exported half list equals the oracle's list at the same
+    positions and the device's stage-kernel build."""
+    say(f"nlist one-launch rebuild n={n} L={L}")
+    rng = np.random.default_rng(n + 11)
+    st = base_state(rng, n, L, mass=(1.0, 1.0), forces=False)
+    m = round(n ** (1 / 3))
+    g = (np.arange(m) + 0.5) * (L / m) - L / 2
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)[:n] + rng.normal(size=(n, 3)) * 0.05
+    st["rx"], st["ry"], st["rz"] = P[:, 0], P[:, 1], P[:, 2]
+    st["vx"] *= 0.1; st["vy"] *= 0.1; st["vz"] *= 0.1
+    to_device(dev, st)
+    dev.init_nlist(skin, rc)
+    dev.export_nlist()   # keeps the half list from now on
+    for trial in range(3):
+        cur = from_device(dev, st)
+        moved = cur.copy()
+        for f in ("rx", "ry", "rz"):
+            moved[f] = cur[f] + rng.normal(size=n) * 0.2
+        moved["rx"][trial] += 2.0
+        for f in ("rx", "ry", "rz"):
+            dev.upload(f, moved[f])
+        b0 = dev.rebuild_count()
+        dev.run(1, [(LJ, 1.0, 1.0, rc)], 1, 0.001)
+        assert dev.rebuild_count() == b0 + 1
+        off, nb = dev.export_nlist()
+        at = from_device(dev, st)
+        nl = oracle.nlist_new(at, rc, skin)
+        oracle.nlist_build(at, nl)
+        nnz = nl["nnz"]
+        assert len(nb) == nnz, (trial, len(nb), nnz)
+        assert np.array_equal(nb - 1, nl["nbr"][:nnz])
+        assert np.array_equal(off[:-1] - 1, nl["offset"][:n - 1])
+        dev.build_nlist()
+        off2, nb2 = dev.export_nlist()
+        assert np.array_equal(off, off2) and np.array_equal(nb, nb2)
+
+
 def test_needs_rebuild_matches(dev, oracle):
     rng = np.random.default_rng(5)
     n, L = 1000, 12.0
