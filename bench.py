@@ -292,11 +292,24 @@ def other_workloads(sx, args):
     configurations, each prepared on the device and timed over whole
     graph-replayed steps (a section with 0 steps is skipped); each is timed
     again with FP32 pair terms from the same state."""
-    from sucrose_amd.workloads import chains_state, gb_nhc_state
+    from sucrose_amd.workloads import chains_state, gb_nhc_state, load_prepared, save_prepared
+
+    def prepared(d, name, make):
+        # the workload from --load-states, else prepared (and kept with --save-states)
+        if args.load_states:
+            p = load_prepared(d, os.path.join(args.load_states, name + ".npz"))
+            log(f"[workload] loaded {name} from {args.load_states}")
+            return p
+        p = make(d, log=log)
+        if args.save_states:
+            os.makedirs(args.save_states, exist_ok=True)
+            save_prepared(d, p, os.path.join(args.save_states, name + ".npz"))
+        return p
+
     out = {}
     if args.gb_steps > 0:
         d = sx.System(0)
-        p = gb_nhc_state(d, log=log)
+        p = prepared(d, "gb_nhc_8000", gb_nhc_state)
         gb = [(sx.GB, p["cutoff"], p["mu"], p["nu"])]
         step = p["next_step"]
         d.run(sx.INT_NHT, gb, 100, p["dt"], first_step=step)
@@ -331,62 +344,63 @@ def other_workloads(sx, args):
                 log(f"[gb cpu] reference {cpu['value']:.4e} particle-steps/s ({cpu['seconds']:.1f}s)")
         out["gb_nhc_8000"] = rec
 
-        def gb_forces():
-            d.initialize_forces()
-            d.forces(*gb[0])
-            return d.get_scalar("POT_ENG")
+        if not args.no_variants:   # the FP32 rows (and the list-radius variant)
+            def gb_forces():
+                d.initialize_forces()
+                d.forces(*gb[0])
+                return d.get_scalar("POT_ENG")
 
-        err = f32_error(d, snap, gb_forces)
-        err.pop("per_group")
+            err = f32_error(d, snap, gb_forces)
+            err.pop("per_group")
 
-        def back_to_start():
-            d.load_arrays(start)
-            for k, v in start_scal.items():
-                d.set_scalar(k, v)
-            d.build_nlist()
-            d.synchronize()
+            def back_to_start():
+                d.load_arrays(start)
+                for k, v in start_scal.items():
+                    d.set_scalar(k, v)
+                d.build_nlist()
+                d.synchronize()
 
-        def segment(bits, list_cutoff):
-            """The timed segment again from its starting state, with the pair
-            terms at `bits` and a list of radius list_cutoff + skin, after a
-            warm-up run from the same state that captures the step graphs."""
-            d.set_pair_precision(bits)
-            d.init_nlist(p["skin"], list_cutoff)
-            back_to_start()
-            d.run(sx.INT_NHT, gb, 100, p["dt"], first_step=step)
-            back_to_start()
-            rb0 = d.rebuild_count()
-            it0, calls0, fb0 = d.nht_stats()
-            t0 = time.perf_counter()
-            d.run(sx.INT_NHT, gb, args.gb_steps, p["dt"], first_step=step)
-            d.synchronize()
-            el = time.perf_counter() - t0
-            it1, calls1, fb1 = d.nht_stats()
-            return el, {"rebuilds": d.rebuild_count() - rb0, "T": d.get_scalar("TEMP"),
-                        "chain_newton_updates_per_call": (it1 - it0) / max(1, calls1 - calls0),
-                        "chain_serial_fallbacks": fb1 - fb0}
+            def segment(bits, list_cutoff):
+                """The timed segment again from its starting state, with the pair
+                terms at `bits` and a list of radius list_cutoff + skin, after a
+                warm-up run from the same state that captures the step graphs."""
+                d.set_pair_precision(bits)
+                d.init_nlist(p["skin"], list_cutoff)
+                back_to_start()
+                d.run(sx.INT_NHT, gb, 100, p["dt"], first_step=step)
+                back_to_start()
+                rb0 = d.rebuild_count()
+                it0, calls0, fb0 = d.nht_stats()
+                t0 = time.perf_counter()
+                d.run(sx.INT_NHT, gb, args.gb_steps, p["dt"], first_step=step)
+                d.synchronize()
+                el = time.perf_counter() - t0
+                it1, calls1, fb1 = d.nht_stats()
+                return el, {"rebuilds": d.rebuild_count() - rb0, "T": d.get_scalar("TEMP"),
+                            "chain_newton_updates_per_call": (it1 - it0) / max(1, calls1 - calls0),
+                            "chain_serial_fallbacks": fb1 - fb0}
 
-        el, x = segment(32, p["cutoff"])
-        out["gb_nhc_8000_f32"] = f32_row(rec, p["n"], args.gb_steps, el, err, x.pop("rebuilds"), x)
-        log(f"[gb f32] {el / args.gb_steps * 1e3:.4f} ms/step; error {err['force_elem_err']:.2e}; "
-            f"{x['chain_newton_updates_per_call']:.2f} Newton updates per chain call, "
-            f"{x['chain_serial_fallbacks']} serial fallbacks")
-        # the list radius that covers gb_'s whole range, sigma_max + cutoff -
-        # sigma_0 = 6.0, plus the skin (the headline rows use cutoff + skin)
-        full = 6.0
-        for bits in (64, 32):
-            el, x = segment(bits, full)
-            key = f"gb_nhc_8000_list{full + p['skin']:.1f}" + ("_f32" if bits == 32 else "")
-            out[key] = dict(value=p["n"] * args.gb_steps / el, unit="particle-steps/s", steps=args.gb_steps,
-                            ms_per_step=el / args.gb_steps * 1e3, **x,
-                            config=dict(rec["config"], list_radius=full + p["skin"],
-                                        dtype="f64" if bits == 64 else
-                                        "f32 pair terms (f64 positions, sums, integrator)"))
-            log(f"[gb list {full + p['skin']:.1f} {bits}-bit] {el / args.gb_steps * 1e3:.4f} ms/step")
+            el, x = segment(32, p["cutoff"])
+            out["gb_nhc_8000_f32"] = f32_row(rec, p["n"], args.gb_steps, el, err, x.pop("rebuilds"), x)
+            log(f"[gb f32] {el / args.gb_steps * 1e3:.4f} ms/step; error {err['force_elem_err']:.2e}; "
+                f"{x['chain_newton_updates_per_call']:.2f} Newton updates per chain call, "
+                f"{x['chain_serial_fallbacks']} serial fallbacks")
+            # the list radius that covers gb_'s whole range, sigma_max + cutoff -
+            # sigma_0 = 6.0, plus the skin (the headline rows use cutoff + skin)
+            full = 6.0
+            for bits in (64, 32):
+                el, x = segment(bits, full)
+                key = f"gb_nhc_8000_list{full + p['skin']:.1f}" + ("_f32" if bits == 32 else "")
+                out[key] = dict(value=p["n"] * args.gb_steps / el, unit="particle-steps/s", steps=args.gb_steps,
+                                ms_per_step=el / args.gb_steps * 1e3, **x,
+                                config=dict(rec["config"], list_radius=full + p["skin"],
+                                            dtype="f64" if bits == 64 else
+                                            "f32 pair terms (f64 positions, sums, integrator)"))
+                log(f"[gb list {full + p['skin']:.1f} {bits}-bit] {el / args.gb_steps * 1e3:.4f} ms/step")
         d.close()
     if args.chain_steps > 0:
         d = sx.System(0)
-        p = chains_state(d, log=log)
+        p = prepared(d, "chains_100000", chains_state)
         warm = 64 + 16 + 4   # captures every step-graph length a long run replays
         step = p["next_step"]
         d.run(sx.INT_LGVC, p["forces"], warm, p["dt"], damping=p["gamma"], first_step=step, bonded=True)
@@ -416,36 +430,37 @@ def other_workloads(sx, args):
                 log(f"[chains cpu] reference {cpu['value']:.4e} particle-steps/s ({cpu['seconds']:.1f}s)")
         out["chains_100000"] = rec
 
-        def chain_forces():
-            d.initialize_forces()
-            for f in p["forces"]:
-                d.forces(*f)
-            return d.get_scalar("POT_ENG")
+        if not args.no_variants:   # the FP32 row
+            def chain_forces():
+                d.initialize_forces()
+                for f in p["forces"]:
+                    d.forces(*f)
+                return d.get_scalar("POT_ENG")
 
-        err = f32_error(d, snap, chain_forces, FORCE_FIELDS[:3])
-        err.pop("per_group")
+            err = f32_error(d, snap, chain_forces, FORCE_FIELDS[:3])
+            err.pop("per_group")
 
-        def back_to_start():
-            d.load_arrays(start)
-            for k, v in start_scal.items():
-                d.set_scalar(k, v)
-            d.build_nlist()
+            def back_to_start():
+                d.load_arrays(start)
+                for k, v in start_scal.items():
+                    d.set_scalar(k, v)
+                d.build_nlist()
+                d.synchronize()
+
+            # FP32: the same segment (same state, same Langevin counters) after a
+            # warm-up that captures the FP32 step graphs
+            back_to_start()
+            d.run(sx.INT_LGVC, p["forces"], warm, p["dt"], damping=p["gamma"], first_step=step, bonded=True)
+            back_to_start()
+            rb0 = d.rebuild_count()
+            t0 = time.perf_counter()
+            d.run(sx.INT_LGVC, p["forces"], args.chain_steps, p["dt"], damping=p["gamma"], first_step=step,
+                  bonded=True)
             d.synchronize()
-
-        # FP32: the same segment (same state, same Langevin counters) after a
-        # warm-up that captures the FP32 step graphs
-        back_to_start()
-        d.run(sx.INT_LGVC, p["forces"], warm, p["dt"], damping=p["gamma"], first_step=step, bonded=True)
-        back_to_start()
-        rb0 = d.rebuild_count()
-        t0 = time.perf_counter()
-        d.run(sx.INT_LGVC, p["forces"], args.chain_steps, p["dt"], damping=p["gamma"], first_step=step,
-              bonded=True)
-        d.synchronize()
-        el = time.perf_counter() - t0
-        out["chains_100000_f32"] = f32_row(rec, p["n"], args.chain_steps, el, err, d.rebuild_count() - rb0,
-                                           {"T": d.get_scalar("TEMP")})
-        log(f"[chains f32] {el / args.chain_steps * 1e3:.4f} ms/step; error {err['force_elem_err']:.2e}")
+            el = time.perf_counter() - t0
+            out["chains_100000_f32"] = f32_row(rec, p["n"], args.chain_steps, el, err, d.rebuild_count() - rb0,
+                                               {"T": d.get_scalar("TEMP")})
+            log(f"[chains f32] {el / args.chain_steps * 1e3:.4f} ms/step; error {err['force_elem_err']:.2e}")
         d.close()
     if args.batch_steps > 0:
         out["batched_1024"], out["batched_1024_f32"] = batched_workload(sx, args)
@@ -692,6 +707,12 @@ def main():
     ap.add_argument("--learned-steps", type=int, default=1000,
                     help="steps of the whole loop timed for each contact solver")
     ap.add_argument("--save-state", default=None, help="write the prepared workload state to this .npz")
+    ap.add_argument("--save-states", default=None,
+                    help="write the prepared Gay-Berne and charged-chain workloads into this directory")
+    ap.add_argument("--load-states", default=None,
+                    help="start the Gay-Berne and charged-chain workloads from a --save-states directory")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the FP32 rows and the list-radius variant of the other workloads")
     ap.add_argument("--load-state", default=None,
                     help="start from a state written by --save-state instead of preparing the workload")
     ap.add_argument("--calibrate-copy", action="store_true",

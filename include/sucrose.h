@@ -405,9 +405,11 @@ int sx_batch_set_shape(sx_ctx *ctx, int choice);
 /* measurement and synthetic set-up (no counterpart in the reference:   */
 /* benchmark and test harness support)                                 */
 /* ------------------------------------------------------------------ */
-/* HIP-event timing of the ECP pipeline on the current state: after three
- * untimed passes, the median over nrep timed passes of each stage's
- * duration, ms = {per-particle precompute, screen, solve, apply};
+/* HIP-event timing of the ECP pipeline on the current state, each stage
+ * launched nrep times in a captured graph as inside a replayed step: the
+ * median time per launch over three timed replays after an untimed one,
+ * ms = {per-particle precompute, screen, solve, apply} (apply as solve +
+ * apply pairs less the solve alone);
  * stats = {half pairs screened, pairs solved, solver iterations} per run.
  * Forces accumulate; re-initialise them afterwards. */
 int sx_profile_ecp(sx_ctx *ctx, int kind, const double *params, int nparams, int nrep, double ms[4],

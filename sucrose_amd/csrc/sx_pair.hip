@@ -996,30 +996,67 @@ int sx_profile_ecp(sx_ctx *c, int kind, const double *params, int nparams, int n
         // extra pass with the solve counters on for the pair statistics
         EcpLaunch L = ecp_launch_shape(c, false);
         EcpLaunch Ls = ecp_launch_shape(c, true);
-        // three untimed passes, then the median of nrep timed passes per stage
-        hipEvent_t ev[5];
+        // as inside a replayed step: a stage launched nrep times in a
+        // captured graph, replayed once untimed and three times timed, the
+        // time per launch the median replay over nrep (the graph's node
+        // dispatch included, about 1 us).  The apply stage consumes the marks
+        // the solve leaves, so it is timed as solve + apply pairs less the
+        // solve alone.
+        // the apply stage adds into the forces, torques and pair energies:
+        // they are kept aside here and put back after the timed passes
+        const size_t ld = (size_t)c->ld;
+        double *keep = nullptr;
+        HIPCHK(hipMalloc(&keep, sizeof(double) * 7 * ld));
+        double *const fld = c->d_fields;
+        auto hold = [&](bool back) {
+            double *spans[3] = {fld + (size_t)SX_FX * ld, fld + (size_t)SX_TX * ld, c->d_pe_acc};
+            const size_t len[3] = {3 * ld, 3 * ld, ld};
+            size_t off = 0;
+            for (int k = 0; k < 3; ++k) {
+                if (back) HIPCHK(hipMemcpyAsync(spans[k], keep + off, sizeof(double) * len[k], hipMemcpyDeviceToDevice, c->stream));
+                else HIPCHK(hipMemcpyAsync(keep + off, spans[k], sizeof(double) * len[k], hipMemcpyDeviceToDevice, c->stream));
+                off += len[k];
+            }
+        };
+        hold(false);
+        for (int stage = 0; stage < 4; ++stage) ecp_stage(c, stage, kind, sc, L);   // buffers sized
+        HIPCHK(hipStreamSynchronize(c->stream));
+        hipEvent_t ev[2];
         for (auto &e : ev) HIPCHK(hipEventCreate(&e));
-        std::vector<double> times[4];
+        std::vector<double> times[5];
         long dev0[8], dev1[8];
-        for (int r = -3; r < nrep; ++r) {
-            for (int stage = 0; stage < 4; ++stage) {
-                HIPCHK(hipEventRecord(ev[stage], c->stream));
-                ecp_stage(c, stage, kind, sc, L);
+        for (int q = 0; q < 5; ++q) {
+            hipGraph_t g = nullptr;
+            hipGraphExec_t ge = nullptr;
+            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            for (int r = 0; r < nrep; ++r) {
+                if (q < 4) ecp_stage(c, q < 3 ? q : 2, kind, sc, L);
+                if (q == 4) { ecp_stage(c, 2, kind, sc, L); ecp_stage(c, 3, kind, sc, L); }
             }
-            HIPCHK(hipEventRecord(ev[4], c->stream));
-            HIPCHK(hipEventSynchronize(ev[4]));
-            for (int q = 0; q < 4 && r >= 0; ++q) {
+            HIPCHK(hipStreamEndCapture(c->stream, &g));
+            HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            for (int r = -1; r < 3; ++r) {
+                HIPCHK(hipEventRecord(ev[0], c->stream));
+                HIPCHK(hipGraphLaunch(ge, c->stream));
+                HIPCHK(hipEventRecord(ev[1], c->stream));
+                HIPCHK(hipEventSynchronize(ev[1]));
                 float t = 0;
-                HIPCHK(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
-                times[q].push_back(t);
+                HIPCHK(hipEventElapsedTime(&t, ev[0], ev[1]));
+                if (r >= 0) times[q].push_back(t / nrep);
             }
+            HIPCHK(hipGraphExecDestroy(ge));
+            HIPCHK(hipGraphDestroy(g));
         }
+        // q = 3 timed the solve again (alone); apply = (solve + apply) - solve
+        for (int r = 0; r < 3; ++r) times[3][r] = std::max(0.0, times[4][r] - times[3][r]);
         HIPCHK(hipMemcpyAsync(dev0, c->d_counts, sizeof(long) * 8, hipMemcpyDeviceToHost, c->stream));
         for (int stage = 0; stage < 4; ++stage) ecp_stage(c, stage, kind, sc, Ls);
         HIPCHK(hipMemcpyAsync(dev1, c->d_counts, sizeof(long) * 8, hipMemcpyDeviceToHost, c->stream));
         int sizes[8];
         HIPCHK(hipMemcpyAsync(sizes, c->d_flag, sizeof(int) * 8, hipMemcpyDeviceToHost, c->stream));
+        hold(true);
         HIPCHK(hipStreamSynchronize(c->stream));
+        hipFree(keep);
         for (auto &e : ev) hipEventDestroy(e);
         for (int q = 0; q < 4; ++q) {
             std::vector<double> &v = times[q];

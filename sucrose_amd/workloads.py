@@ -46,8 +46,8 @@ import time
 
 import numpy as np
 
-from . import (DH, ECPR, GB, INT_LGV, LJ, INT_LGVC, INT_NHT, REBUILD_HALF_SKIN, WCA, System, count_ellipsoid_overlaps,
-               gen_chains, gen_ellipsoids_rsa, uniform_quaternions)
+from . import (DH, ECPR, GB, INT_LGV, LJ, INT_LGVC, INT_NHT, REBUILD_HALF_SKIN, SCALARS, WCA, System,
+               count_ellipsoid_overlaps, gen_chains, gen_ellipsoids_rsa, uniform_quaternions)
 
 ELLIPSOID_SEMI_AXES = (0.5, 0.5, 1.5)
 
@@ -165,16 +165,11 @@ def chains_state(dev: System, nchains=2000, nbeads=50, rho=0.5, seed=7, T=1.0, d
         dev.upload(f, pos[:, k])
     charge = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
     dev.upload("CHARGE", charge)
-    ids = np.arange(n).reshape(nchains, nbeads) + 1          # 1-based
-    bi = ids[:, :-1].ravel()
-    bj = ids[:, 1:].ravel()
-    dev.set_constraints(bi, bj, np.full(len(bi), bond))
+    chains_topology(dev, nchains, nbeads, bond, k_bend, bend_angle, k_dih, bonded=False)
     dev.set_target_temperature(T)
     dev.initialize_system()
     dev.set_velocity_to_temperature(T, seed)
-    # cosine bend about the straight chain (cosine_bend_, force.f90:2345), dihedral k(1 + cos phi) (dih_, :2286)
-    dev.set_bonded(2, np.stack([ids[:, :-2], ids[:, 1:-1], ids[:, 2:]], -1).reshape(-1), k_bend, bend_angle)
-    dev.set_bonded(3, np.stack([ids[:, :-3], ids[:, 1:-2], ids[:, 2:-1], ids[:, 3:]], -1).reshape(-1), k_dih)
+    chains_topology(dev, nchains, nbeads, bond, k_bend, bend_angle, k_dih, constraints=False)
     dev.set_rebuild_rule(REBUILD_HALF_SKIN)
     dev.init_nlist(skin, dh[0])
     forces = [(WCA, 1.0, 1.0), (DH,) + tuple(dh)]
@@ -184,8 +179,23 @@ def chains_state(dev: System, nchains=2000, nbeads=50, rho=0.5, seed=7, T=1.0, d
         raise RuntimeError("chain workload preparation diverged")
     log(f"[workload] chains {nchains}x{nbeads} (L={L:.4f}) equilibrated {equil_steps} steps, PE/N={pe / n:.4f} "
         f"T={dev.get_scalar('TEMP'):.4f}, {time.time() - t0:.1f}s")
-    return dict(n=n, box=L, rho=rho, T=T, dt=dt, gamma=gamma, skin=skin, next_step=1 + equil_steps, seed=seed,
-                forces=forces, nchains=nchains, nbeads=nbeads, k_bend=k_bend, bend_angle=bend_angle, k_dih=k_dih)
+    return dict(n=n, box=L, rho=rho, T=T, dt=dt, gamma=gamma, skin=skin, cutoff=dh[0], next_step=1 + equil_steps,
+                seed=seed, forces=forces, nchains=nchains, nbeads=nbeads, bond=bond, k_bend=k_bend,
+                bend_angle=bend_angle, k_dih=k_dih)
+
+
+def chains_topology(dev: System, nchains, nbeads, bond, k_bend, bend_angle, k_dih, constraints=True, bonded=True):
+    """The chains' RATTLE bonds (consecutive beads) and bonded terms: cosine
+    bend about the straight chain (cosine_bend_, force.f90:2345) and the
+    dihedral k(1 + cos phi) (dih_, :2286)."""
+    ids = np.arange(nchains * nbeads).reshape(nchains, nbeads) + 1          # 1-based
+    if constraints:
+        bi = ids[:, :-1].ravel()
+        bj = ids[:, 1:].ravel()
+        dev.set_constraints(bi, bj, np.full(len(bi), bond))
+    if bonded:
+        dev.set_bonded(2, np.stack([ids[:, :-2], ids[:, 1:-1], ids[:, 2:]], -1).reshape(-1), k_bend, bend_angle)
+        dev.set_bonded(3, np.stack([ids[:, :-3], ids[:, 1:-2], ids[:, 2:-1], ids[:, 3:]], -1).reshape(-1), k_dih)
 
 
 def batched_layout(nsys=1024, nmin=64, nmax=512, seed=31):
@@ -313,6 +323,43 @@ def load_state(dev: System, path):
             dev.upload(key[2:], z[key])
     dev.set_rebuild_rule(REBUILD_HALF_SKIN)
     dev.init_nlist(params["skin"], params["cutoff"])
+    return params
+
+
+def save_prepared(dev: System, params: dict, path):
+    """Writes a prepared workload (device arrays, scalars, parameters) to an
+    .npz file, so that a later run starts from it instead of preparing it."""
+    arrays = {f"f_{k}": v for k, v in dev.state().items()}
+    scal = {k: dev.get_scalar(k) for k in SCALARS}
+    np.savez(path, params=np.array(json.dumps(params)), scal=np.array(json.dumps(scal)), **arrays)
+
+
+def load_prepared(dev: System, path):
+    """Restores a workload written by save_prepared (for the charged chains
+    also their bonds and bonded terms) and initialises its neighbour list;
+    returns the workload parameters."""
+    z = np.load(path, allow_pickle=False)
+    params = json.loads(str(z["params"]))
+    scal = json.loads(str(z["scal"]))
+    dev.set_num_atoms(params["n"])
+    dev.set_box(params["box"])
+    for key in z.files:
+        if key.startswith("f_"):
+            dev.upload(key[2:], z[key])
+    if "nchains" in params:
+        chains_topology(dev, params["nchains"], params["nbeads"], params["bond"], params["k_bend"],
+                        params["bend_angle"], params["k_dih"])
+    dev.set_target_temperature(params["T"])
+    dev.initialize_system(remove_com=False)
+    for key in z.files:   # initialize_system derives sums only; the fields as saved
+        if key.startswith("f_"):
+            dev.upload(key[2:], z[key])
+    for k, v in scal.items():
+        dev.set_scalar(k, v)
+    dev.set_rebuild_rule(REBUILD_HALF_SKIN)
+    dev.init_nlist(params["skin"], params["cutoff"])
+    if "forces" in params:
+        params["forces"] = [tuple(f) for f in params["forces"]]
     return params
 
 
