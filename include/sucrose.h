@@ -408,10 +408,10 @@ int sx_batch_set_shape(sx_ctx *ctx, int choice);
 /* HIP-event timing of the ECP pipeline on the current state, each stage
  * launched nrep times in a captured graph as inside a replayed step: the
  * median time per launch over three timed replays after an untimed one,
- * ms = {per-particle precompute, screen, solve, apply} (apply as solve +
- * apply pairs less the solve alone);
+ * ms = {per-particle precompute, screen, solve, apply} (apply as screen +
+ * solve + apply less the screen and the solve alone);
  * stats = {half pairs screened, pairs solved, solver iterations} per run.
- * Forces accumulate; re-initialise them afterwards. */
+ * Forces, torques and pair energies are restored afterwards. */
 int sx_profile_ecp(sx_ctx *ctx, int kind, const double *params, int nparams, int nrep, double ms[4],
                    double stats[3]);
 /* random sequential addition of identical ellipsoids (semi-axes a, b, c) with

@@ -7,6 +7,8 @@
 #pragma once
 #include "sx_pairs.h"
 
+#include <climits>
+
 // per-particle record fields (stored per particle, PP_STRIDE doubles each)
 enum {
     PP_AR = 0,     // 6: a11 a21 a31 a22 a32 a33
@@ -109,6 +111,11 @@ __device__ __forceinline__ double ecp_optim_dist(double l, const Sym3 &A, const 
 // segment of the work array and records their count (segcount).  No atomics:
 // the work list is deterministic and later stages walk the segments.
 #define ECP_NSEG 1024
+// the apply stage's slots: per particle one 128-byte line of 16 int2, the
+// count in the first (.x) and up to ECP_SLOTS (partner, half pair) entries
+// after it (a particle with more walks its neighbour row)
+#define ECP_SLOT_LINE 16
+#define ECP_SLOTS (ECP_SLOT_LINE - 1)
 
 __device__ __forceinline__ void ecp_segment(long nnz, int b, long &start, long &end)
 {
@@ -475,35 +482,189 @@ __device__ __forceinline__ double ecp_solve_lambda(int solver, const double *pp,
     return l;
 }
 
+// what the apply stage reads: the slots, and the list for particles whose
+// pairs overflowed them
+struct EcpApplyArgs {
+    int2 *slot;
+    const int *row_start, *nbr, *lower, *half_start;
+    const double *res;
+};
+
+// the current context's apply-stage arguments
+EcpApplyArgs sx_ecp_apply_args(sx_ctx *c);
+
+// compare-and-swap on the partner index
+__device__ __forceinline__ void slot_cas(int2 &a, int2 &b)
+{
+    const bool sw = b.x < a.x;
+    const int2 lo = sw ? b : a, hi = sw ? a : b;
+    a = lo;
+    b = hi;
+}
+
+// the first N entries sorted by partner (bitonic network, N a power of two)
+template <int N>
+__device__ __forceinline__ void slot_sort(int2 (&e)[ECP_SLOT_LINE])
+{
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const int l = i ^ j;
+                if (l > i) {
+                    if ((i & k) == 0) slot_cas(e[i], e[l]);
+                    else slot_cas(e[l], e[i]);
+                }
+            }
+}
+
+// pair k's results added to particle i's sums: i is the pair's first
+// particle (force +F, its torque, the energy) or its second (force -F, the
+// other torque)
+__device__ __forceinline__ void ecp_add(double (&acc)[7], const double *res, long k, bool first)
+{
+    const double *r = res + k * 10;
+    if (first) {
+        acc[0] = acc[0] + r[1]; acc[1] = acc[1] + r[2]; acc[2] = acc[2] + r[3];
+        acc[3] = acc[3] + r[4]; acc[4] = acc[4] + r[5]; acc[5] = acc[5] + r[6];
+        acc[6] = acc[6] + r[0];
+    } else {
+        acc[0] = acc[0] - r[1]; acc[1] = acc[1] - r[2]; acc[2] = acc[2] - r[3];
+        acc[3] = acc[3] + r[7]; acc[4] = acc[4] + r[8]; acc[5] = acc[5] + r[9];
+    }
+}
+
+// Particle i's sums (force, torque, energy) over its solved pairs, from its
+// slots sorted by partner, so that they are added in the order of its
+// neighbour row (ascending partner: the order in which the reference's pair
+// loop reaches them); pairs the learned solve proved out of range (half pair
+// -1) are skipped.  A particle with more than ECP_SLOTS pairs walks its row
+// instead: a forward entry's half pair follows from the row's own start, a
+// reverse entry's from a binary search of the partner's row, and live[k]
+// says whether the pair was solved.  The slot count is cleared for the next
+// solve.
+__device__ __forceinline__ void ecp_particle_sums(int i, const EcpApplyArgs &a, const unsigned char *live,
+                                                  double (&acc)[7])
+{
+    // the particle's whole line in one round trip
+    int4 w[ECP_SLOT_LINE / 2];
+    const int4 *line = reinterpret_cast<const int4 *>(a.slot + (long)i * ECP_SLOT_LINE);
+#pragma unroll
+    for (int u = 0; u < ECP_SLOT_LINE / 2; ++u) w[u] = line[u];
+    const int c = w[0].x;
+    a.slot[(long)i * ECP_SLOT_LINE].x = 0;
+    for (int q = 0; q < 7; ++q) acc[q] = 0.0;
+    if (c <= ECP_SLOTS) {
+        int2 e[ECP_SLOT_LINE];
+#pragma unroll
+        for (int u = 0; u < ECP_SLOT_LINE; ++u) {
+            const int wi = (u + 1) >> 1;   // entry u is word u + 1 of the line
+            const int2 v = (u + 1) & 1 ? make_int2(w[wi].z, w[wi].w) : make_int2(w[wi].x, w[wi].y);
+            e[u] = u < c ? v : make_int2(INT_MAX, -1);
+        }
+        if (c > 8) slot_sort<16>(e);
+        else if (c > 4) slot_sort<8>(e);
+        else if (c > 1) slot_sort<4>(e);
+        // four entries' results requested together (an unused entry reads
+        // pair 0), then added in order
+#pragma unroll
+        for (int b0 = 0; b0 < ECP_SLOT_LINE; b0 += 4) {
+            if (b0 >= c) break;
+            double r[4][10];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const bool use = b0 + u < c && e[b0 + u].y >= 0;
+                const double *p = a.res + (long)(use ? e[b0 + u].y : 0) * 10;
+#pragma unroll
+                for (int q = 0; q < 10; ++q) r[u][q] = p[q];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (!(b0 + u < c && e[b0 + u].y >= 0)) continue;
+                if (e[b0 + u].x > i) {
+                    acc[0] = acc[0] + r[u][1]; acc[1] = acc[1] + r[u][2]; acc[2] = acc[2] + r[u][3];
+                    acc[3] = acc[3] + r[u][4]; acc[4] = acc[4] + r[u][5]; acc[5] = acc[5] + r[u][6];
+                    acc[6] = acc[6] + r[u][0];
+                } else {
+                    acc[0] = acc[0] - r[u][1]; acc[1] = acc[1] - r[u][2]; acc[2] = acc[2] - r[u][3];
+                    acc[3] = acc[3] + r[u][7]; acc[4] = acc[4] + r[u][8]; acc[5] = acc[5] + r[u][9];
+                }
+            }
+        }
+    } else {
+        const int b = a.row_start[i], e = a.row_start[i + 1], lo = a.lower[i], hs = a.half_start[i];
+        for (int q = b; q < e; ++q) {
+            const int j = a.nbr[q];
+            long k;
+            if (j > i) {
+                k = hs + (q - b - lo);
+            } else {
+                const int bj = a.row_start[j], lj = a.lower[j];
+                int l = bj + lj, h = a.row_start[j + 1];
+                while (l < h) {
+                    const int m = (l + h) >> 1;
+                    if (a.nbr[m] < i) l = m + 1; else h = m;
+                }
+                k = a.half_start[j] + (l - bj - lj);
+            }
+            if (live[k]) ecp_add(acc, a.res, k, j > i);
+        }
+    }
+}
+
+// the apply stage's per-particle slots (always taken: a caller with no apply
+// stage after the solve clears the counts itself)
+struct EcpSlots {
+    int2 *slot;
+};
+
 // Solves pair k = (a, b) of the work list and stores its energy, force and
-// torques at res[10 k]; idx_full (may be NULL) gets 1 + k at the pair's two
-// full-list entries.  warm: l0 is a warm start (learned), and a pair whose
+// torques at res[10 k]; the pair takes a slot of a, (b, k), and one of b,
+// (a, k), for the apply stage (k = -1 where the warm start proves the pair
+// out of range; a count beyond ECP_SLOTS keeps rising without a slot, and
+// the apply stage then walks that particle's row).  warm: l0 is a warm start (learned), and a pair whose
 // F(l0) already proves it out of range ends there (returns 1, live[k] = 0).
 template <int VARIANT>
 __device__ __forceinline__ int ecp_pair_solve(const DState &s, const double *pp, long ld, int k, int a, int b,
                                               int solver, bool warm, double l0, const EcpPar &P, double *res,
-                                              unsigned char *live, const int *fwdpos, const int *revpos, int *idx_full,
-                                              int &iters, int &fb)
+                                              unsigned char *live, const EcpSlots &sl, int &iters, int &fb)
 {
     double RIJ[3];
     pair_rij(s, a, b, RIJ);
     iters = 0;
     fb = 0;
+    // the slots are taken once the pair's records are requested (the
+    // counts' round trip then overlaps the solve instead of delaying the
+    // records) and written at the end
+    int sa = ECP_SLOTS, sb = ECP_SLOTS;
+    auto take = [&]() {
+        sa = atomicAdd(&sl.slot[(long)a * ECP_SLOT_LINE].x, 1);
+        sb = atomicAdd(&sl.slot[(long)b * ECP_SLOT_LINE].x, 1);
+    };
+    auto put = [&](int kk) {
+        if (sa < ECP_SLOTS) sl.slot[(long)a * ECP_SLOT_LINE + 1 + sa] = make_int2(b, kk);
+        if (sb < ECP_SLOTS) sl.slot[(long)b * ECP_SLOT_LINE + 1 + sb] = make_int2(a, kk);
+    };
     double lam;
     if (solver == SX_ECP_BRENT_REFERENCE) {
         Sym3 A = load_sym(pp, ld, PP_AR, a), B = load_sym(pp, ld, PP_AR, b);
+        take();
         lam = ecp_brent(A, B, RIJ, ecp_optim_dist(ECP_PSI, A, B, RIJ), iters);
     } else {
         if (warm) {
             // F and its derivatives at the warm start serve both the
             // out-of-range proof and the first Newton step
             Sym3 A = load_sym(pp, ld, PP_AR, a), B = load_sym(pp, ld, PP_AR, b);
+            take();
             double rsq = RIJ[0] * RIJ[0] + RIJ[1] * RIJ[1] + RIJ[2] * RIJ[2];
             l0 = fmin(fmax(l0, 1e-6), 1.0 - 1e-6);
             double d[3];
             ecp_F_derivs(l0, A, B, RIJ, d[0], d[1], d[2]);
             if (ecp_zero_proven(d[0], rsq, VARIANT == 1 ? 1 : 0, P.cutoff + P.wf)) {
                 live[k] = 0;
+                put(-1);
                 return 1;
             }
             int ok = 0;
@@ -515,6 +676,7 @@ __device__ __forceinline__ int ecp_pair_solve(const DState &s, const double *pp,
                 fb = 1;
             }
         } else {
+            take();
             lam = ecp_solve_lambda<VARIANT>(solver, pp, ld, a, b, RIJ, 0.0, ECP_PSI, iters, fb);
         }
     }
@@ -525,9 +687,6 @@ __device__ __forceinline__ int ecp_pair_solve(const DState &s, const double *pp,
     r[1] = o.f[0]; r[2] = o.f[1]; r[3] = o.f[2];
     r[4] = o.ti[0]; r[5] = o.ti[1]; r[6] = o.ti[2];
     r[7] = o.tj[0]; r[8] = o.tj[1]; r[9] = o.tj[2];
-    if (idx_full) {
-        idx_full[fwdpos[k]] = k + 1;
-        idx_full[revpos[k]] = k + 1;
-    }
+    put(k);
     return 0;
 }

@@ -236,19 +236,55 @@ __global__ void __launch_bounds__(256) k_qq_step_begin(DState s, double dt, cons
     rebuild_flag_from<256>(m, part, ticket, thresh, rule, flag);
 }
 
-template <int BS>
+// APPLY: the ECP pair results are added to the particle's forces, torques
+// and energy first (the apply stage, ecp_particle_sums), and the kicks use
+// the sums from registers
+template <int BS, bool APPLY>
 __global__ void __launch_bounds__(BS) k_lgv_final(DState s, double dt, double *part, unsigned *ticket, double *scal,
-                                                  ThermoArgs th, int *dstep)
+                                                  ThermoArgs th, int *dstep, EcpApplyArgs ap,
+                                                  const unsigned char *live, double *pe_acc)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     if (i < s.n) {
-        half_kick(s, i, dt);
-        ang_kick(s, i, dt);
-        v[1] = kin_rot(s, i);
-        v[0] = kin_trans(s, i);
-        if (th.on) v[2] = virial_of(s, i);
-        if (th.pe_acc) v[3] = th.pe_acc[i];
+        if (APPLY) {
+            // the particle's state, requested before the pair sums so that
+            // its latency overlaps theirs
+            double f[7] = {s.f[SX_FX][i], s.f[SX_FY][i], s.f[SX_FZ][i], s.f[SX_TX][i],
+                           s.f[SX_TY][i], s.f[SX_TZ][i], pe_acc[i]};
+            const double im = s.f[SX_INV_MASS][i], mass = s.f[SX_MASS][i];
+            double vx = s.f[SX_VX][i], vy = s.f[SX_VY][i], vz = s.f[SX_VZ][i];
+            double lx = s.f[SX_LX][i], ly = s.f[SX_LY][i], lz = s.f[SX_LZ][i];
+            const double q[4] = {s.f[SX_QW][i], s.f[SX_QX][i], s.f[SX_QY][i], s.f[SX_QZ][i]};
+            const double ii[3] = {s.f[SX_IXX_INV][i], s.f[SX_IYY_INV][i], s.f[SX_IZZ_INV][i]};
+            const double rx = s.f[SX_RX][i], ry = s.f[SX_RY][i], rz = s.f[SX_RZ][i];
+            double acc[7];
+            ecp_particle_sums(i, ap, live, acc);
+            for (int c = 0; c < 7; ++c) f[c] = f[c] + acc[c];
+            // half_kick and ang_kick with the sums in registers
+            vx = vx + 0.5 * dt * (f[0] * im);
+            vy = vy + 0.5 * dt * (f[1] * im);
+            vz = vz + 0.5 * dt * (f[2] * im);
+            lx = lx + 0.5 * dt * f[3];
+            ly = ly + 0.5 * dt * f[4];
+            lz = lz + 0.5 * dt * f[5];
+            s.f[SX_FX][i] = f[0]; s.f[SX_FY][i] = f[1]; s.f[SX_FZ][i] = f[2];
+            s.f[SX_TX][i] = f[3]; s.f[SX_TY][i] = f[4]; s.f[SX_TZ][i] = f[5];
+            pe_acc[i] = f[6];
+            s.f[SX_VX][i] = vx; s.f[SX_VY][i] = vy; s.f[SX_VZ][i] = vz;
+            s.f[SX_LX][i] = lx; s.f[SX_LY][i] = ly; s.f[SX_LZ][i] = lz;
+            v[1] = kin_rot_of(q, lx, ly, lz, ii);
+            v[0] = kin_trans_of(mass, vx, vy, vz);
+            if (th.on) v[2] = rx * f[0] + ry * f[1] + rz * f[2];
+            if (th.pe_acc) v[3] = f[6];
+        } else {
+            half_kick(s, i, dt);
+            ang_kick(s, i, dt);
+            v[1] = kin_rot(s, i);
+            v[0] = kin_trans(s, i);
+            if (th.on) v[2] = virial_of(s, i);
+            if (th.pe_acc) v[3] = th.pe_acc[i];
+        }
     }
     double tot[4];
     if (!sx_block_reduce_last<4, BS>(v, part, ticket, tot) || threadIdx.x != 0) return;
@@ -1600,8 +1636,14 @@ void sx_lgv_final_impl(sx_ctx *c, double dt, double damping, bool constrained, b
     int nb = sx_blocks(c->n, 256);
     sx_ensure_part(c, nb, 4);
     // with constraints the thermo must follow RATTLE (virial_bonded)
-    hipLaunchKernelGGL(k_lgv_final<256>, dim3(nb), dim3(256), 0, c->stream, c->dstate(), dt, c->d_part, c->d_ticket,
-                       c->d_scal, thermo_args(c, thermo && !constrained), dstep);
+    if (c->ecp_apply_in_final && !constrained)
+        hipLaunchKernelGGL((k_lgv_final<256, true>), dim3(nb), dim3(256), 0, c->stream, c->dstate(), dt, c->d_part,
+                           c->d_ticket, c->d_scal, thermo_args(c, thermo), dstep, sx_ecp_apply_args(c),
+                           (const unsigned char *)c->d_live, c->d_pe_acc);
+    else
+        hipLaunchKernelGGL((k_lgv_final<256, false>), dim3(nb), dim3(256), 0, c->stream, c->dstate(), dt, c->d_part,
+                           c->d_ticket, c->d_scal, thermo_args(c, thermo && !constrained), dstep, EcpApplyArgs{},
+                           (const unsigned char *)nullptr, (double *)nullptr);
     if (constrained) {
         sx_rattle_impl(c, dt, 1);
         if (thermo) {

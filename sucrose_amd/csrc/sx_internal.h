@@ -126,6 +126,7 @@ struct sx_ctx {
     double *d_pe_acc = nullptr;  // per-particle potential energy since initialize_forces
     bool defer_pe = false;       // step driver: pot_eng is reduced in the final-step kernel
     bool ecp_pp_by_step = false; // step driver: the initial-step kernel wrote the ECP per-particle records
+    bool ecp_apply_in_final = false; // step driver: the final-step kernel applies the ECP pair results
     int *d_segcount = nullptr;   // ECP work-list segment counts
     double *d_scal = nullptr;    // SX_NSCALARS doubles (device copy of the scalars)
     double h_scal[SX_NSCALARS] = {0};
@@ -165,10 +166,11 @@ struct sx_ctx {
     unsigned char *d_live = nullptr;
     int *d_work = nullptr;        // compacted pair list
     long pair_cap = 0;
-    // per full-list entry: 1 + the half pair whose results the apply stage
-    // adds, or 0; set by the solve stage, cleared again by the apply stage
-    int *d_idx_full = nullptr;
-    long idx_cap = 0;
+    // per particle: a count and the solved pairs it belongs to, as (partner,
+    // half pair), in a line of slots taken by the solve stage; the count is
+    // read and cleared again by the apply stage
+    int2 *d_ecp_slot = nullptr;
+    long ecp_slot_n = 0;
     // parallel Nose-Hoover scratch (maps, guesses, chunk sums, control)
     double *d_nht = nullptr;
     // per-particle ECP precompute

@@ -586,8 +586,7 @@ __global__ void __launch_bounds__(64) k_ecp_solve_learned(DState s, const double
                                                           const bf16x8 *wi, const int *half_i, const int *half_j,
                                                           const int *work, const int *segcount, const int *nnz_half,
                                                           EcpPar par, double *res, unsigned char *live,
-                                                          long *stats, const int *fwdpos, const int *revpos,
-                                                          int *idx_full)
+                                                          long *stats, EcpSlots sl)
 {
     const int cnt = segcount[blockIdx.x];
     if (cnt == 0) return;
@@ -646,7 +645,7 @@ __global__ void __launch_bounds__(64) k_ecp_solve_learned(DState s, const double
         if (u < cnt) {
             int iters = 0, fb = 0;
             if (ecp_pair_solve<VARIANT>(s, pp, 0L, k, a, b, SX_ECP_LEARNED, true, (double)l0s[lane], par, res,
-                                        live, fwdpos, revpos, idx_full, iters, fb)) {
+                                        live, sl, iters, fb)) {
                 ++my_skip;
             } else {
                 my_iters += iters;
@@ -769,16 +768,16 @@ const float *sx_ecp_mlp_predict(sx_ctx *c)
 }
 
 // the learned solve stage over the current work list (results, live flags
-// and full-list marks as the plain solve stage)
-void sx_ecp_solve_learned(sx_ctx *c, int variant, const EcpPar &par, long *stats, bool mark)
+// and the apply stage's slots as the plain solve stage)
+void sx_ecp_solve_learned(sx_ctx *c, int variant, const EcpPar &par, long *stats)
 {
+    const EcpSlots sl{c->d_ecp_slot};
     MlpState *M = mlp_of(c);
     NList &nl = c->nl;
-    int *idx = mark ? c->d_idx_full : nullptr;
 #define SX_LEARNED(V)                                                                                               \
     hipLaunchKernelGGL(k_ecp_solve_learned<V>, dim3(ECP_NSEG), dim3(64), 0, c->stream, c->dstate(), c->d_ecp_pp,  \
                        M->p, M->wf, M->wi, nl.half_i, nl.half_j, c->d_work, c->d_segcount, c->d_flag + 2, par,      \
-                       c->d_pairbuf, c->d_live, stats, nl.fwdpos, nl.revpos, idx)
+                       c->d_pairbuf, c->d_live, stats, sl)
     switch (variant) {
     case 0: SX_LEARNED(0); break;
     case 1: SX_LEARNED(1); break;

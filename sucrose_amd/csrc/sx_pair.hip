@@ -217,10 +217,21 @@ static inline bool is_ecp_kind(int kind)
 // (largest/smallest semi-axis)^2 (checked on the host).  Before the solve,
 // F(psi) >= psi(1-psi) r^2 / lambda_max(G(psi)) with lambda_max(G) <=
 // (1-psi) amax_a^2 + psi amax_b^2 rejects distant pairs without a solve.
-__device__ __forceinline__ bool ecp_screen32(const double4 *pos4, const float *scr, int a, int b,
-                                             const EcpScreen &sc)
+struct ScreenIn {
+    double4 pa, pb;
+    float4 A0, A1, B0, B1;
+};
+
+// the screen's inputs of pair (a, b): positions and the FP32 records
+__device__ __forceinline__ ScreenIn ecp_screen_load(const double4 *pos4, const float *scr, int a, int b)
 {
-    double4 pa = pos4[a], pb = pos4[b];
+    const float4 *s4 = reinterpret_cast<const float4 *>(scr);
+    return ScreenIn{pos4[a], pos4[b], s4[2L * a], s4[2L * a + 1], s4[2L * b], s4[2L * b + 1]};
+}
+
+__device__ __forceinline__ bool ecp_screen32(const ScreenIn &in, const EcpScreen &sc)
+{
+    const double4 pa = in.pa, pb = in.pb;
     double d0 = pa.x - pb.x, d1 = pa.y - pb.y, d2 = pa.z - pb.z;
     if (sc.periodic) {
         d0 = sx_mimg_fast(d0, sc.L[0], sc.invL[0]);
@@ -240,8 +251,7 @@ __device__ __forceinline__ bool ecp_screen32(const double4 *pos4, const float *s
     }
     thr = thr * (1.0f + ECP_SCREEN_MARGIN);
     const float psi = 0.381966011250105f;
-    const float4 *s4 = reinterpret_cast<const float4 *>(scr);
-    float4 A0 = s4[2L * a], A1 = s4[2L * a + 1], B0 = s4[2L * b], B1 = s4[2L * b + 1];
+    const float4 A0 = in.A0, A1 = in.A1, B0 = in.B0, B1 = in.B1;
     // aspect ratios beyond 30 are not screened (condition number of G)
     if (A1.z > 30.0f * A1.w || B1.z > 30.0f * B1.w) return true;
     float lmax = (1.0f - psi) * A1.z * A1.z + psi * B1.z * B1.z;
@@ -263,6 +273,7 @@ __device__ __forceinline__ bool ecp_screen32(const double4 *pos4, const float *s
 }
 
 
+// Compacts the segment's kept pairs in half-list order into the work list
 __global__ void __launch_bounds__(256) k_ecp_screen(const double4 *pos4, const float *scr, const int *half_i,
                                                     const int *half_j, const int *nnz_half, EcpScreen sc,
                                                     unsigned char *live, int *work, int *segcount)
@@ -276,7 +287,8 @@ __global__ void __launch_bounds__(256) k_ecp_screen(const double4 *pos4, const f
         long k = base + t;
         bool keep = false;
         if (k < end) {
-            keep = ecp_screen32(pos4, scr, half_i[k], half_j[k], sc);
+            const int a = half_i[k], b = half_j[k];
+            keep = ecp_screen32(ecp_screen_load(pos4, scr, a, b), sc);
             live[k] = keep ? 1 : 0;
         }
         unsigned long long m = __ballot(keep);
@@ -301,16 +313,12 @@ __global__ void __launch_bounds__(256) k_ecp_screen(const double4 *pos4, const f
 // solved pairs, solver iterations, fallbacks and, for the learned solver,
 // pairs whose warm start already proves them out of range (their live flag
 // is cleared so the apply stage skips them)
-// idx_full (may be NULL): the pair's two full-list entries (fwdpos, revpos)
-// get 1 + k, so the apply stage finds the solved pairs of a row without
-// reading the others
 template <int VARIANT>
 __global__ void __launch_bounds__(ECP_SOLVE_WG) k_ecp_solve(DState s, const double *pp, long ld, const int *half_i,
                                                                  const int *half_j, const int *work, const int *segcount,
                                                                  const int *nnz_half, EcpScreen sc, int solver,
                                                                  const float *lam0, double *res, unsigned char *live,
-                                                                 long *stats, const int *fwdpos, const int *revpos,
-                                                                 int *idx_full)
+                                                                 long *stats, EcpSlots sl)
 {
     long start, end;
     ecp_segment(*nnz_half, blockIdx.x, start, end);
@@ -324,8 +332,7 @@ __global__ void __launch_bounds__(ECP_SOLVE_WG) k_ecp_solve(DState s, const doub
         int k = work[t];
         int iters = 0, fb = 0;
         if (ecp_pair_solve<VARIANT>(s, pp, ld, k, half_i[k], half_j[k], solver, lam0 != nullptr,
-                                    lam0 ? (double)lam0[t] : ECP_PSI, sc.par, res, live, fwdpos, revpos, idx_full,
-                                    iters, fb)) {
+                                    lam0 ? (double)lam0[t] : ECP_PSI, sc.par, res, live, sl, iters, fb)) {
             ++my_skip;
             continue;
         }
@@ -350,58 +357,44 @@ __global__ void __launch_bounds__(ECP_SOLVE_WG) k_ecp_solve(DState s, const doub
     }
 }
 
-// per-particle application of the per-pair results: the row's entries marked
-// by the solve stage, in row order; each mark is cleared after use, so the
-// marks are all zero again when the stage ends
-template <int LPP>
-__global__ void __launch_bounds__(256) k_aniso_apply(DState s, const int *row_start, const int *nbr, int *idx_full,
-                                                     const double *res, double *pe_acc)
+// per-particle application of the per-pair results, one thread per particle
+// (ecp_particle_sums)
+__global__ void __launch_bounds__(256) k_ecp_apply(DState s, EcpApplyArgs a, const unsigned char *live,
+                                                   double *pe_acc)
 {
-    int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    int i = gid / LPP, l = gid % LPP;
-    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (i < s.n) {
-        int b = row_start[i], e = row_start[i + 1];
-        // the lane's marks of up to APPLY_BATCH entries are loaded together
-        // (one round trip instead of one per entry), then applied in row order
-        constexpr int APPLY_BATCH = 8;
-        for (int q0 = b + l; q0 < e; q0 += APPLY_BATCH * LPP) {
-            int mark[APPLY_BATCH];
-#pragma unroll
-            for (int t = 0; t < APPLY_BATCH; ++t) {
-                const int q = q0 + t * LPP;
-                mark[t] = q < e ? idx_full[q] : 0;
-            }
-#pragma unroll
-            for (int t = 0; t < APPLY_BATCH; ++t) {
-                if (mark[t] == 0) continue;
-                const int q = q0 + t * LPP;
-                idx_full[q] = 0;
-                const double *r = res + (long)(mark[t] - 1) * 10;
-                if (nbr[q] > i) {
-                    acc[0] = acc[0] + r[1]; acc[1] = acc[1] + r[2]; acc[2] = acc[2] + r[3];
-                    acc[3] = acc[3] + r[4]; acc[4] = acc[4] + r[5]; acc[5] = acc[5] + r[6];
-                    acc[6] = acc[6] + r[0];
-                } else {
-                    acc[0] = acc[0] - r[1]; acc[1] = acc[1] - r[2]; acc[2] = acc[2] - r[3];
-                    acc[3] = acc[3] + r[7]; acc[4] = acc[4] + r[8]; acc[5] = acc[5] + r[9];
-                }
-            }
-        }
-    }
-#pragma unroll
-    for (int off = LPP / 2; off > 0; off >>= 1)
-#pragma unroll
-        for (int c = 0; c < 7; ++c) acc[c] = acc[c] + __shfl_xor(acc[c], off, LPP);
-    if (l == 0 && i < s.n) {
-        s.f[SX_FX][i] = s.f[SX_FX][i] + acc[0];
-        s.f[SX_FY][i] = s.f[SX_FY][i] + acc[1];
-        s.f[SX_FZ][i] = s.f[SX_FZ][i] + acc[2];
-        s.f[SX_TX][i] = s.f[SX_TX][i] + acc[3];
-        s.f[SX_TY][i] = s.f[SX_TY][i] + acc[4];
-        s.f[SX_TZ][i] = s.f[SX_TZ][i] + acc[5];
-        pe_acc[i] = pe_acc[i] + acc[6];
-    }
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    // the particle's running sums, read before the pairs (independent of
+    // them) so that their latency overlaps
+    const double f0[7] = {s.f[SX_FX][i], s.f[SX_FY][i], s.f[SX_FZ][i], s.f[SX_TX][i],
+                          s.f[SX_TY][i], s.f[SX_TZ][i], pe_acc[i]};
+    double acc[7];
+    ecp_particle_sums(i, a, live, acc);
+    s.f[SX_FX][i] = f0[0] + acc[0];
+    s.f[SX_FY][i] = f0[1] + acc[1];
+    s.f[SX_FZ][i] = f0[2] + acc[2];
+    s.f[SX_TX][i] = f0[3] + acc[3];
+    s.f[SX_TY][i] = f0[4] + acc[4];
+    s.f[SX_TZ][i] = f0[5] + acc[5];
+    pe_acc[i] = f0[6] + acc[6];
+}
+
+__global__ void k_clear_slot_counts(int2 *slot, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) slot[(long)i * ECP_SLOT_LINE].x = 0;
+}
+
+// clears the slot counts that solves with no apply stage after them left
+static void ecp_clear_slots(sx_ctx *c)
+{
+    hipLaunchKernelGGL(k_clear_slot_counts, dim3(sx_blocks(c->n, 256)), dim3(256), 0, c->stream, c->d_ecp_slot, c->n);
+}
+
+EcpApplyArgs sx_ecp_apply_args(sx_ctx *c)
+{
+    const NList &nl = c->nl;
+    return EcpApplyArgs{c->d_ecp_slot, nl.row_start, nl.nbr, nl.lower, nl.half_start, c->d_pairbuf};
 }
 
 // explicit pairs (parity hook)
@@ -579,11 +572,11 @@ static void ensure_pair_buffers(sx_ctx *c)
         c->pair_cap = need;
         c->have_graph = false;
     }
-    if (c->nl.cap_full > c->idx_cap) {
-        if (c->d_idx_full) HIPCHK(hipFree(c->d_idx_full));
-        HIPCHK(hipMalloc(&c->d_idx_full, sizeof(int) * c->nl.cap_full));
-        HIPCHK(hipMemsetAsync(c->d_idx_full, 0, sizeof(int) * c->nl.cap_full, c->stream));
-        c->idx_cap = c->nl.cap_full;
+    if (c->ld > c->ecp_slot_n) {
+        if (c->d_ecp_slot) HIPCHK(hipFree(c->d_ecp_slot));
+        HIPCHK(hipMalloc(&c->d_ecp_slot, sizeof(int2) * ECP_SLOT_LINE * c->ld));
+        HIPCHK(hipMemsetAsync(c->d_ecp_slot, 0, sizeof(int2) * ECP_SLOT_LINE * c->ld, c->stream));
+        c->ecp_slot_n = c->ld;
         c->have_graph = false;
     }
     ensure_pp(c);
@@ -592,7 +585,7 @@ static void ensure_pair_buffers(sx_ctx *c)
 // learned warm start: predictions for the work list of the current call
 const float *sx_ecp_mlp_predict(sx_ctx *c);
 // learned solve stage (network and solve in one kernel)
-void sx_ecp_solve_learned(sx_ctx *c, int variant, const EcpPar &par, long *stats, bool mark);
+void sx_ecp_solve_learned(sx_ctx *c, int variant, const EcpPar &par, long *stats);
 
 static EcpScreen make_screen(sx_ctx *c, int kind, const Params &P)
 {
@@ -613,7 +606,6 @@ static EcpScreen make_screen(sx_ctx *c, int kind, const Params &P)
 
 struct EcpLaunch {
     long cap;
-    int lpp, nb;
     bool stats;   // count solved pairs / iterations (profiling)
 };
 
@@ -621,16 +613,6 @@ static EcpLaunch ecp_launch_shape(sx_ctx *c, bool stats)
 {
     EcpLaunch L;
     L.cap = c->nl.cap_half;
-    // lanes per particle of the apply stage: 4-8 row entries per lane (8 lanes
-    // at 36 entries per row measured fastest)
-    L.lpp = 4;
-    const double row = c->n > 0 ? (double)c->nl.nnz_full / c->n : 0.0;
-    while (L.lpp < 16 && 8.0 * L.lpp < row) L.lpp <<= 1;
-    if (const char *e = getenv("SUCROSE_APPLY_LANES")) {   // tuning override: 4, 8 or 16
-        int v = atoi(e);
-        if (v == 4 || v == 8 || v == 16) L.lpp = v;
-    }
-    L.nb = sx_blocks((long)c->n * L.lpp, 256);
     L.stats = stats;
     return L;
 }
@@ -654,14 +636,14 @@ static void ecp_stage(sx_ctx *c, int stage, int kind, const EcpScreen &sc, const
     case 2: {
         long *stats = L.stats ? c->d_counts : nullptr;
         if (c->ecp_solver == SX_ECP_LEARNED) {
-            sx_ecp_solve_learned(c, sc.fvar, sc.par, stats, true);
+            sx_ecp_solve_learned(c, sc.fvar, sc.par, stats);
             break;
         }
         const float *lam0 = nullptr;
 #define SX_ECP_SOLVE(V)                                                                                          \
     hipLaunchKernelGGL(k_ecp_solve<V>, dim3(ECP_NSEG), dim3(ECP_SOLVE_WG), 0, c->stream, s, c->d_ecp_pp, ld, \
                        nl.half_i, nl.half_j, c->d_work, c->d_segcount, nnz_half, sc, c->ecp_solver, lam0,         \
-                       c->d_pairbuf, c->d_live, stats, nl.fwdpos, nl.revpos, c->d_idx_full)
+                       c->d_pairbuf, c->d_live, stats, EcpSlots{c->d_ecp_slot})
         switch (sc.fvar) {
         case 0: SX_ECP_SOLVE(0); break;
         case 1: SX_ECP_SOLVE(1); break;
@@ -673,15 +655,8 @@ static void ecp_stage(sx_ctx *c, int stage, int kind, const EcpScreen &sc, const
         break;
     }
     case 3:
-        if (L.lpp == 16)
-            hipLaunchKernelGGL(k_aniso_apply<16>, dim3(L.nb), dim3(256), 0, c->stream, s, nl.row_start, nl.nbr,
-                               c->d_idx_full, c->d_pairbuf, c->d_pe_acc);
-        else if (L.lpp == 8)
-            hipLaunchKernelGGL(k_aniso_apply<8>, dim3(L.nb), dim3(256), 0, c->stream, s, nl.row_start, nl.nbr,
-                               c->d_idx_full, c->d_pairbuf, c->d_pe_acc);
-        else
-            hipLaunchKernelGGL(k_aniso_apply<4>, dim3(L.nb), dim3(256), 0, c->stream, s, nl.row_start, nl.nbr,
-                               c->d_idx_full, c->d_pairbuf, c->d_pe_acc);
+        hipLaunchKernelGGL(k_ecp_apply, dim3(sx_blocks(n, 256)), dim3(256), 0, c->stream, s, sx_ecp_apply_args(c),
+                           (const unsigned char *)c->d_live, c->d_pe_acc);
         break;
     }
 }
@@ -855,8 +830,10 @@ static void ecp_forces_impl(sx_ctx *c, int kind, const Params &P)
     ensure_pair_buffers(c);
     EcpScreen sc = make_screen(c, kind, P);
     EcpLaunch L = ecp_launch_shape(c, false);
-    // the whole-step driver writes the records in its initial-step kernel
-    for (int stage = c->ecp_pp_by_step ? 1 : 0; stage < 4; ++stage) ecp_stage(c, stage, kind, sc, L);
+    // the whole-step driver writes the records in its initial-step kernel,
+    // and may leave the apply stage to its final-step kernel
+    for (int stage = c->ecp_pp_by_step ? 1 : 0; stage < (c->ecp_apply_in_final ? 3 : 4); ++stage)
+        ecp_stage(c, stage, kind, sc, L);
 }
 
 void sx_ecp_records(sx_ctx *c, double **pp, float **scr, double **pos4)
@@ -999,9 +976,9 @@ int sx_profile_ecp(sx_ctx *c, int kind, const double *params, int nparams, int n
         // as inside a replayed step: a stage launched nrep times in a
         // captured graph, replayed once untimed and three times timed, the
         // time per launch the median replay over nrep (the graph's node
-        // dispatch included, about 1 us).  The apply stage consumes the marks
-        // the solve leaves, so it is timed as solve + apply pairs less the
-        // solve alone.
+        // dispatch included, about 1 us).  The apply stage consumes the slots
+        // the solve takes, so it is timed as screen + solve + apply less the
+        // screen and the solve alone.
         // the apply stage adds into the forces, torques and pair energies:
         // they are kept aside here and put back after the timed passes
         const size_t ld = (size_t)c->ld;
@@ -1023,15 +1000,16 @@ int sx_profile_ecp(sx_ctx *c, int kind, const double *params, int nparams, int n
         HIPCHK(hipStreamSynchronize(c->stream));
         hipEvent_t ev[2];
         for (auto &e : ev) HIPCHK(hipEventCreate(&e));
-        std::vector<double> times[5];
+        std::vector<double> times[4];
         long dev0[8], dev1[8];
-        for (int q = 0; q < 5; ++q) {
+        for (int q = 0; q < 4; ++q) {
             hipGraph_t g = nullptr;
             hipGraphExec_t ge = nullptr;
             HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             for (int r = 0; r < nrep; ++r) {
-                if (q < 4) ecp_stage(c, q < 3 ? q : 2, kind, sc, L);
-                if (q == 4) { ecp_stage(c, 2, kind, sc, L); ecp_stage(c, 3, kind, sc, L); }
+                if (q < 3) ecp_stage(c, q, kind, sc, L);
+                else
+                    for (int st = 1; st < 4; ++st) ecp_stage(c, st, kind, sc, L);
             }
             HIPCHK(hipStreamEndCapture(c->stream, &g));
             HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
@@ -1046,9 +1024,11 @@ int sx_profile_ecp(sx_ctx *c, int kind, const double *params, int nparams, int n
             }
             HIPCHK(hipGraphExecDestroy(ge));
             HIPCHK(hipGraphDestroy(g));
+            // the solves alone left their slot counts standing
+            if (q == 2) ecp_clear_slots(c);
         }
-        // q = 3 timed the solve again (alone); apply = (solve + apply) - solve
-        for (int r = 0; r < 3; ++r) times[3][r] = std::max(0.0, times[4][r] - times[3][r]);
+        // q = 3 timed screen + solve + apply; apply = that less the two alone
+        for (int r = 0; r < 3; ++r) times[3][r] = std::max(0.0, times[3][r] - times[1][r] - times[2][r]);
         HIPCHK(hipMemcpyAsync(dev0, c->d_counts, sizeof(long) * 8, hipMemcpyDeviceToHost, c->stream));
         for (int stage = 0; stage < 4; ++stage) ecp_stage(c, stage, kind, sc, Ls);
         HIPCHK(hipMemcpyAsync(dev1, c->d_counts, sizeof(long) * 8, hipMemcpyDeviceToHost, c->stream));
@@ -1248,8 +1228,8 @@ extern "C" int sx_ecp_compare_solvers(sx_ctx *c, int kind, const double *params,
 #define SX_ECP_FULL(V)                                                                                            \
     hipLaunchKernelGGL(k_ecp_solve<V>, g, dim3(ECP_SOLVE_WG), 0, c->stream, s, c->d_ecp_pp, 0L, nl.half_i,      \
                        nl.half_j, c->d_work,                                                                     \
-                       c->d_segcount, nnz_half, sc, solver, l0, c->d_pairbuf, c->d_live, (long *)nullptr,        \
-                       (const int *)nullptr, (const int *)nullptr, (int *)nullptr)
+                       c->d_segcount, nnz_half, sc, solver, l0, c->d_pairbuf, c->d_live, (long *)nullptr,    \
+                       EcpSlots{c->d_ecp_slot})
             switch (sc.fvar) {
             case 0: SX_ECP_FULL(0); break;
             case 1: SX_ECP_FULL(1); break;
@@ -1260,7 +1240,7 @@ extern "C" int sx_ecp_compare_solvers(sx_ctx *c, int kind, const double *params,
 #undef SX_ECP_FULL
         };
         double t_full_plain = timed([&] { full(SX_ECP_CONVERGED, nullptr); });
-        double t_full_learned = timed([&] { sx_ecp_solve_learned(c, sc.fvar, sc.par, nullptr, false); });
+        double t_full_learned = timed([&] { sx_ecp_solve_learned(c, sc.fvar, sc.par, nullptr); });
         // iteration counts (4 counters per solver)
         hipLaunchKernelGGL(k_ecp_lambda<3>, g, b, 0, c->stream, s, c->d_ecp_pp, nl.half_i, nl.half_j, c->d_work,
                            c->d_segcount, nnz_half, sc, lam0, lam_b, cnt + 0);
@@ -1279,6 +1259,8 @@ extern "C" int sx_ecp_compare_solvers(sx_ctx *c, int kind, const double *params,
         HIPCHK(hipMemcpyAsync(hp.data(), lam_p, sizeof(double) * cap, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(hw.data(), lam_w, sizeof(double) * cap, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(h0.data(), lam0, sizeof(float) * cap, hipMemcpyDeviceToHost, c->stream));
+        // the solves above took apply-stage slots that no apply stage reads
+        ecp_clear_slots(c);
         HIPCHK(hipStreamSynchronize(c->stream));
         long live = 0;
         double s_err = 0, s_err2 = 0, m_err = 0, m_dw = 0;

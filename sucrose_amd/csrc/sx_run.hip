@@ -15,7 +15,12 @@ __global__ void k_step_set(int *dstep, int v) { *dstep = v; }
 struct DeferPe {
     sx_ctx *c;
     explicit DeferPe(sx_ctx *c_) : c(c_) { c->defer_pe = true; }
-    ~DeferPe() { c->defer_pe = false; c->ecp_pp_by_step = false; }
+    ~DeferPe()
+    {
+        c->defer_pe = false;
+        c->ecp_pp_by_step = false;
+        c->ecp_apply_in_final = false;
+    }
 };
 
 // the parameter count of each pair kind in a step spec
@@ -74,12 +79,18 @@ static void enqueue_step(sx_ctx *c, const sx_step_spec &sp)
         if (!fused) sx_init_forces_impl(c);
     }
     bool dipoles = false;
+    for (int f = 0; f < sp.nforces; ++f) dipoles = dipoles || sp.kinds[f] == SX_DIPOLE;
     const bool two = sp.nforces == 2 &&
                      sx_forces_two_impl(c, sp.kinds[0], sp.params[0], nparams_of(sp.kinds[0]), sp.kinds[1],
                                         sp.params[1], nparams_of(sp.kinds[1]));
+    // an ECP term that adds the last forces of the step (no bonded terms,
+    // reaction field or composite transfers after it) is applied by the
+    // Langevin final-step kernel itself
+    const bool apply_in_final = !two && sp.integrator == SX_INT_LGV && c->comp.count == 0 && sp.nforces > 0 &&
+                                is_ecp(sp.kinds[sp.nforces - 1]) && !sp.bonded && !dipoles;
     for (int f = 0; f < sp.nforces && !two; ++f) {
+        c->ecp_apply_in_final = apply_in_final && f == sp.nforces - 1;
         sx_forces_impl(c, sp.kinds[f], sp.params[f], nparams_of(sp.kinds[f]));
-        dipoles = dipoles || sp.kinds[f] == SX_DIPOLE;
     }
     if (dipoles) sx_eval_reaction_field_impl(c);
     if (sp.bonded) sx_bonded_impl(c);

@@ -28,10 +28,14 @@ __device__ __forceinline__ void save_old(const DState &s, int i)
     s.f[SX_RZ_OLD][i] = s.f[SX_RZ][i];
 }
 
+__device__ __forceinline__ double kin_trans_of(double mass, double vx, double vy, double vz)
+{
+    return 0.5 * mass * (vx * vx + vy * vy + vz * vz);
+}
+
 __device__ __forceinline__ double kin_trans(const DState &s, int i)
 {
-    double vx = s.f[SX_VX][i], vy = s.f[SX_VY][i], vz = s.f[SX_VZ][i];
-    return 0.5 * s.f[SX_MASS][i] * (vx * vx + vy * vy + vz * vz);
+    return kin_trans_of(s.f[SX_MASS][i], s.f[SX_VX][i], s.f[SX_VY][i], s.f[SX_VZ][i]);
 }
 
 // RM = R(q)^T (integrators.f90:144-152)
@@ -48,15 +52,22 @@ __device__ __forceinline__ void rm_of(double w, double x, double y, double z, do
     RM[8] = w * w - x * x - y * y + z * z;   // (3,3)
 }
 
-__device__ __forceinline__ double kin_rot(const DState &s, int i)
+// q = (w, x, y, z), l the angular momentum, ii the inverse principal moments
+__device__ __forceinline__ double kin_rot_of(const double q[4], double lx, double ly, double lz, const double ii[3])
 {
     double RM[9];
-    rm_of(s.f[SX_QW][i], s.f[SX_QX][i], s.f[SX_QY][i], s.f[SX_QZ][i], RM);
-    double lx = s.f[SX_LX][i], ly = s.f[SX_LY][i], lz = s.f[SX_LZ][i];
+    rm_of(q[0], q[1], q[2], q[3], RM);
     double bx = (RM[0] * lx + RM[3] * ly + RM[6] * lz);
     double by = (RM[1] * lx + RM[4] * ly + RM[7] * lz);
     double bz = (RM[2] * lx + RM[5] * ly + RM[8] * lz);
-    return 0.5 * ((bx * bx) * s.f[SX_IXX_INV][i] + (by * by) * s.f[SX_IYY_INV][i] + (bz * bz) * s.f[SX_IZZ_INV][i]);
+    return 0.5 * ((bx * bx) * ii[0] + (by * by) * ii[1] + (bz * bz) * ii[2]);
+}
+
+__device__ __forceinline__ double kin_rot(const DState &s, int i)
+{
+    const double q[4] = {s.f[SX_QW][i], s.f[SX_QX][i], s.f[SX_QY][i], s.f[SX_QZ][i]};
+    const double ii[3] = {s.f[SX_IXX_INV][i], s.f[SX_IYY_INV][i], s.f[SX_IZZ_INV][i]};
+    return kin_rot_of(q, s.f[SX_LX][i], s.f[SX_LY][i], s.f[SX_LZ][i], ii);
 }
 
 __device__ __forceinline__ void quat_midpoint(const DState &s, int i, double dt, double &qdw, double &qdx, double &qdy,
