@@ -256,6 +256,41 @@ def test_forces_pair_loop(dev, oracle, kind, params, rc, kw, n, L):
     assert np.count_nonzero(ref["fx"]) > n // 4
 
 
+def test_ecp_apply_row_walk(dev, oracle):
+    """The apply stage on a state where many particles have more solved pairs
+    than their slot line holds (15) and the others fewer: ECP with cutoff
+    4.0, whose screen keeps every listed pair closer than 3.05, so those
+    particles walk their neighbour rows; forces, torques and energy against
+    the oracle's pair loop as in test_forces_pair_loop."""
+    from tests._pairs import close
+    say("ECP apply with row walks")
+    n, L, rc, cut = 2000, 24.0, 4.3, 4.0
+    rng = np.random.default_rng(91)
+    st = overlap_free_state(rng, ECP, n, L, dict(sig=(1, 1, 3)), seed=91)
+    P = np.stack([st["rx"], st["ry"], st["rz"]], 1)
+    d = P[:, None, :] - P[None, :, :]
+    d -= np.round(d / L) * L
+    r = np.sqrt((d * d).sum(2))
+    np.fill_diagonal(r, np.inf)
+    kept = (r <= cut - 0.95).sum(1)   # a lower bound of the solved pairs per particle
+    say(f"  kept pairs per particle: max {kept.max()}, particles over 15: {(kept > 15).sum()}")
+    assert (kept > 15).sum() > n // 10 and (kept <= 15).sum() > n // 10
+    to_device(dev, st)
+    dev.init_nlist(0.3, rc)
+    dev.initialize_forces()
+    dev.forces(ECP, cut)
+    got = from_device(dev, st)
+    nl = oracle.nlist_new(st, rc, 0.3)
+    oracle.nlist_build(st, nl)
+    ref = st.copy()
+    oracle.pair_loop(ref, nl, ECP, [cut])
+    errs = {f: close(got[f], ref[f]) for f in ("fx", "fy", "fz", "tx", "ty", "tz")}
+    say(f"  worst element errors {errs}")
+    for f, e in errs.items():
+        assert e <= TOL, (f, e)
+    assert abs(got["pot_eng"] - ref["pot_eng"]) <= TOL * abs(ref["pot_eng"])
+
+
 @pytest.mark.parametrize("kind,params,rc", [(ECP, (1.2,), 3.3), (ECPR, (), 3.1), (ECPW, (1.2, 0.3), 3.6),
                                              (ECPC, (1.2, 0.7), 3.3), (ECPWC, (1.2, 0.3, 0.7), 3.6)])
 def test_ecp_converged_and_learned_forces(dev, oracle, kind, params, rc):
@@ -670,7 +705,8 @@ def test_run_graph_lj_nve(dev, oracle):
     assert abs(got["kin_eng"] - ref["kin_eng"]) < 1e-9 * abs(ref["kin_eng"])
 
 
-@pytest.mark.parametrize("kind,params,rc", [(ECP, (1.2,), 3.3), (ECPR, (), 3.1), (LJ, (1.0, 1.0, 2.5), 2.5)])
+@pytest.mark.parametrize("kind,params,rc", [(ECP, (1.2,), 3.3), (ECPR, (), 3.1), (LJ, (1.0, 1.0, 2.5), 2.5),
+                                             (ECP, (4.0,), 4.3)])
 def test_run_graph_langevin(dev, oracle, kind, params, rc):
     """sx_run (graph replay) with the Langevin integrator, whose first kernel
     also takes the rebuild decision, clears the forces and, for ECP kinds,
