@@ -60,7 +60,6 @@ struct NList {
     int *cell_of = nullptr, *cell_count = nullptr, *cell_start = nullptr, *cell_items = nullptr;
     int *row_len = nullptr, *row_start = nullptr, *lower = nullptr, *half_start = nullptr, *half_len = nullptr;
     int *nbr = nullptr;
-    int *fwdpos = nullptr, *revpos = nullptr;   // half pair k -> its entry in row i / in row j
     int *half_i = nullptr, *half_j = nullptr;
     double *x_last = nullptr, *y_last = nullptr, *z_last = nullptr;
     double *cell_pos = nullptr;  // positions in cell order (x y z pad)

@@ -330,48 +330,32 @@ __global__ void __launch_bounds__(256) k_rows_cells_g(DState s, CellGrid g, cons
                                 row_start, nbr, cap, err, slot);
 }
 
-// half-pair index with LPR lanes per row (entries of a row are independent)
-// row i's entries by lane `lane` of the row's L lanes
+// half-pair index with LPR lanes per row: row i's upper entries (j > i, the
+// tail of the sorted row) are half pairs half_start[i] onwards, in order
 __device__ __forceinline__ void half_index_row(int n, int i, int lane, int L, const int *row_start, const int *lower,
                                                const int *half_start, const int *nbr, int *half_i, int *half_j,
-                                               int *fwdpos, int *revpos, long cap_half, int *err)
+                                               long cap_half, int *err)
 {
     if ((long)half_start[n] > cap_half) {
         if (lane == 0) atomicCAS(err, 0, DERR_NLIST_CAP);
         return;
     }
-    const int b = row_start[i], e = row_start[i + 1], lo = lower[i];
+    const int b = row_start[i] + lower[i], e = row_start[i + 1], h = half_start[i];
     for (int p = b + lane; p < e; p += L) {
-        const int j = nbr[p];
-        if (j > i) {
-            const int k = half_start[i] + (p - b - lo);
-            half_i[k] = i;
-            half_j[k] = j;
-            fwdpos[k] = p;
-        } else {
-            // position of i in row j (sorted): binary search
-            int lb = row_start[j], ub = row_start[j + 1];
-            while (lb < ub) {
-                const int m = (lb + ub) >> 1;
-                if (nbr[m] < i) lb = m + 1; else ub = m;
-            }
-            const int k = half_start[j] + (lb - row_start[j] - lower[j]);
-            revpos[k] = p;
-        }
+        half_i[h + (p - b)] = i;
+        half_j[h + (p - b)] = nbr[p];
     }
 }
 
 template <int LPR>
 __global__ void __launch_bounds__(256) k_half_index_g(int n, const int *row_start, const int *lower,
                                                       const int *half_start, const int *nbr, int *half_i, int *half_j,
-                                                      int *fwdpos, int *revpos, const int *flag, long cap_half,
-                                                      int *err)
+                                                      const int *flag, long cap_half, int *err)
 {
     if (flag && *flag == 0) return;
     const int i = blockIdx.x * (256 / LPR) + threadIdx.x / LPR;
     if (i >= n) return;
-    half_index_row(n, i, threadIdx.x % LPR, LPR, row_start, lower, half_start, nbr, half_i, half_j, fwdpos, revpos,
-                   cap_half, err);
+    half_index_row(n, i, threadIdx.x % LPR, LPR, row_start, lower, half_start, nbr, half_i, half_j, cap_half, err);
 }
 
 // brute force rows (small or non-periodic boxes)
@@ -562,9 +546,7 @@ static int nl_max_row(sx_ctx *c);
 //      list's capacity, a cell beyond lb_cap or a row beyond lb_rc leaves an
 //      empty list for the host to grow (as the stage kernels do);
 //   5. the rows copied from the scratch into the list in particle order, and
-//      the half-pair index: forward entries from the row's own record,
-//      reverse entries by a binary search of the partner's sorted scratch
-//      row, several searches per lane in lockstep.
+//      the half pairs (i, j > i) at the places the row's record gives.
 // The pair set and the order of every row are those of the stage kernels.
 //
 // Barrier: every workgroup stores its arrival word (agent scope); the first
@@ -610,7 +592,7 @@ struct ListArgs {
     int *row_len, *lower, *half_len, *row_start, *nbr;
     long cap_full;
     int need_half;
-    int *half_start, *half_i, *half_j, *fwdpos, *revpos;
+    int *half_start, *half_i, *half_j;
     int *total, *total_half, *max_row, *max_cell;   // d_flag[1], [2], [3], [5]
     int *rows;                   // scratch rows, stride rc
     int rc;
@@ -825,52 +807,14 @@ __device__ __forceinline__ void lb_entries(const ListArgs &a, MS ms, int i0, int
         for (int u = 0; u < ILP; ++u)
             if (ie[u] >= 0) a.nbr[b0 + u * LB_THREADS] = je[u];
         if (!a.need_half) continue;
-        // forward entries (j > i): the half index from row i's own record
-        int4 mj[ILP];
+        // upper entries (j > i): the half pair from row i's own record
 #pragma unroll
         for (int u = 0; u < ILP; ++u) {
-            if (ie[u] < 0) continue;
-            const int p = b0 + u * LB_THREADS;
-            if (je[u] > ie[u]) {
-                const int4 mi = ms[r_[u]];
-                const int k = mi.y + (tp[u] - mi.z);
-                a.half_i[k] = ie[u];
-                a.half_j[k] = je[u];
-                a.fwdpos[k] = p;
-            } else {
-                mj[u] = a.meta[je[u]];
-            }
-        }
-        // reverse entries: i's place among the upper entries (> j) of j's
-        // sorted row, by binary searches of [lower[j], row_len[j]) in lockstep
-        int lo[ILP], hi[ILP];
-#pragma unroll
-        for (int u = 0; u < ILP; ++u) {
-            const bool rev = ie[u] >= 0 && je[u] < ie[u];
-            lo[u] = rev ? mj[u].z : 0;
-            hi[u] = rev ? mj[u].w : 0;
-        }
-        for (;;) {
-            int v[ILP];
-            bool any = false;
-#pragma unroll
-            for (int u = 0; u < ILP; ++u)
-                if (lo[u] < hi[u]) v[u] = a.rows[(long)je[u] * a.rc + ((lo[u] + hi[u]) >> 1)];
-#pragma unroll
-            for (int u = 0; u < ILP; ++u)
-                if (lo[u] < hi[u]) {
-                    const int mid = (lo[u] + hi[u]) >> 1;
-                    if (v[u] < ie[u]) lo[u] = mid + 1; else hi[u] = mid;
-                    any = true;
-                }
-            if (!any) break;
-        }
-#pragma unroll
-        for (int u = 0; u < ILP; ++u) {
-            const int p = b0 + u * LB_THREADS;
-            if (ie[u] < 0 || je[u] > ie[u]) continue;
-            const int k = mj[u].y + (lo[u] - mj[u].z);
-            a.revpos[k] = p;
+            if (ie[u] < 0 || je[u] < ie[u]) continue;
+            const int4 mi = ms[r_[u]];
+            const int k = mi.y + (tp[u] - mi.z);
+            a.half_i[k] = ie[u];
+            a.half_j[k] = je[u];
         }
     }
 }
@@ -1155,7 +1099,6 @@ static void launch_list_rebuild(sx_ctx *c, const int *flag)
     a.cap_full = nl.cap_full;
     a.need_half = nl.need_half ? 1 : 0;
     a.half_start = nl.half_start; a.half_i = nl.half_i; a.half_j = nl.half_j;
-    a.fwdpos = nl.fwdpos; a.revpos = nl.revpos;
     a.total = c->d_flag + 1; a.total_half = c->d_flag + 2; a.max_row = c->d_flag + 3; a.max_cell = c->d_flag + 5;
     a.rows = nl.lb_rows;
     a.rc = nl.lb_rc;
@@ -1211,7 +1154,7 @@ static void launch_half_index_g(sx_ctx *c, const int *flag)
     dim3 grid((c->n + 256 / L - 1) / (256 / L)), blk(256);
 #define SX_HALF_G(LL)                                                                                               \
     hipLaunchKernelGGL((k_half_index_g<LL>), grid, blk, 0, c->stream, c->n, nl.row_start, nl.lower, nl.half_start, \
-                       nl.nbr, nl.half_i, nl.half_j, nl.fwdpos, nl.revpos, flag, nl.cap_half, c->d_err)
+                       nl.nbr, nl.half_i, nl.half_j, flag, nl.cap_half, c->d_err)
     switch (L) {
     case 4: SX_HALF_G(4); break;
     case 8: SX_HALF_G(8); break;
@@ -1232,14 +1175,14 @@ static int pow2_clamp(double x, int lo, int hi)
 static void nl_free(NList &nl)
 {
     int *ip[] = {nl.cell_of, nl.cell_count, nl.cell_start, nl.cell_items, nl.row_len, nl.row_start, nl.lower,
-                 nl.half_start, nl.half_len, nl.nbr, nl.half_i, nl.half_j, nl.fwdpos, nl.revpos};
+                 nl.half_start, nl.half_len, nl.nbr, nl.half_i, nl.half_j};
     for (int *p : ip)
         if (p) hipFree(p);
     double *dp[] = {nl.x_last, nl.y_last, nl.z_last, nl.cell_pos};
     for (double *p : dp)
         if (p) hipFree(p);
     nl.cell_of = nl.cell_count = nl.cell_start = nl.cell_items = nl.row_len = nl.row_start = nl.lower = nullptr;
-    nl.half_start = nl.half_len = nl.nbr = nl.half_i = nl.half_j = nl.fwdpos = nl.revpos = nullptr;
+    nl.half_start = nl.half_len = nl.nbr = nl.half_i = nl.half_j = nullptr;
     nl.x_last = nl.y_last = nl.z_last = nl.cell_pos = nullptr;
     if (nl.lb_sync) hipFree(nl.lb_sync);
     if (nl.lb_rows) hipFree(nl.lb_rows);
@@ -1261,16 +1204,12 @@ static void nl_alloc_lists(sx_ctx *c, long cap_full)
     if (nl.nbr) hipFree(nl.nbr);
     if (nl.half_i) hipFree(nl.half_i);
     if (nl.half_j) hipFree(nl.half_j);
-    if (nl.fwdpos) hipFree(nl.fwdpos);
-    if (nl.revpos) hipFree(nl.revpos);
     if (cap_full > 0x7fffffffL) throw sx_error{SX_ERR_CAPACITY, "neighbour list larger than 2^31 entries"};
     nl.cap_full = cap_full;
     nl.cap_half = cap_full / 2 + 1;
     HIPCHK(hipMalloc(&nl.nbr, sizeof(int) * nl.cap_full));
     HIPCHK(hipMalloc(&nl.half_i, sizeof(int) * nl.cap_half));
     HIPCHK(hipMalloc(&nl.half_j, sizeof(int) * nl.cap_half));
-    HIPCHK(hipMalloc(&nl.fwdpos, sizeof(int) * nl.cap_half));
-    HIPCHK(hipMalloc(&nl.revpos, sizeof(int) * nl.cap_half));
     c->have_graph = false;
 }
 
@@ -1283,7 +1222,7 @@ static void nl_grow_keep(sx_ctx *c, long cap_full)
     if (cap_full > 0x7fffffffL) throw sx_error{SX_ERR_CAPACITY, "neighbour list larger than 2^31 entries"};
     const long cap_half = cap_full / 2 + 1;
     int **full[] = {&nl.nbr};
-    int **half[] = {&nl.half_i, &nl.half_j, &nl.fwdpos, &nl.revpos};
+    int **half[] = {&nl.half_i, &nl.half_j};
     for (int **a : full) {
         int *p = nullptr;
         HIPCHK(hipMalloc(&p, sizeof(int) * cap_full));
