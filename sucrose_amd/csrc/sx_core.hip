@@ -406,7 +406,7 @@ int sx_destroy(sx_ctx *c)
                     c->nl.cell_of, c->nl.cell_count, c->nl.cell_start, c->nl.cell_items, c->nl.row_len,
                     c->nl.row_start, c->nl.lower, c->nl.half_start, c->nl.nbr, c->nl.half_i,
                     c->nl.half_j, c->nl.x_last, c->nl.y_last, c->nl.z_last, c->d_pairbuf, c->d_live, c->d_work,
-                    c->d_ecp_slot, c->d_nht, c->d_dip, c->d_rfl, c->d_ecp_pp, c->d_ecp_scr, c->d_pos4, c->d_atom_term_start, c->d_atom_term, c->d_bbufs, c->d_boff, c->d_step, c->mol.atom_start, c->mol.atoms,
+                    c->d_ecp_slot, c->d_nht, c->d_dip, c->d_rfl, c->d_ecp_pp, c->d_ecp_scr, c->d_atom_term_start, c->d_atom_term, c->d_bbufs, c->d_boff, c->d_step, c->mol.atom_start, c->mol.atoms,
                     c->mol.bond_start, c->mol.bond_a, c->mol.bond_b, c->mol.bond_len, c->mol.bond_gidx, c->mol.bond_lvl, c->mol.sched};
     for (void *p : ptrs)
         if (p) hipFree(p);

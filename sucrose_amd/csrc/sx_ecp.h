@@ -28,10 +28,14 @@ struct Sym3 {
 #define PP_STRIDE 24
 #define PPX(pp, q, i) (pp)[(long)(i) * PP_STRIDE + (q)]
 
-// per-particle records (force.f90:1427-1483 for one particle): FP64 record,
-// FP32 screen record (AR lower triangle, largest semi-axis, smallest) and the
-// packed position
-__device__ __forceinline__ void ecp_pp_particle(const DState &s, int i, double *pp, float *scr, double *pos4)
+// FP32 screen record of a particle: one 64-byte line of 16 floats, the
+// position (x, y, z, 0), the AR lower triangle, the largest and the smallest
+// semi-axis, and 4 unused
+#define ECP_SCR_STRIDE 16
+
+// per-particle records (force.f90:1427-1483 for one particle): FP64 record
+// and FP32 screen record
+__device__ __forceinline__ void ecp_pp_particle(const DState &s, int i, double *pp, float *scr)
 {
     double w = s.f[SX_QW][i], x = s.f[SX_QX][i], y = s.f[SX_QY][i], z = s.f[SX_QZ][i];
     double U[9];
@@ -72,11 +76,10 @@ __device__ __forceinline__ void ecp_pp_particle(const DState &s, int i, double *
     double amax = fmax(fmax(S[0], S[1]), S[2]), amin = fmin(fmin(S[0], S[1]), S[2]);
     rec[PP_AMAX] = amax;
 #undef Ux
-    float4 *s4 = reinterpret_cast<float4 *>(scr) + 2L * i;
-    s4[0] = make_float4((float)ar_l[0], (float)ar_l[1], (float)ar_l[2], (float)ar_l[3]);
-    s4[1] = make_float4((float)ar_l[4], (float)ar_l[5], (float)amax, (float)amin);
-    double4 *p4 = reinterpret_cast<double4 *>(pos4) + i;
-    *p4 = make_double4(s.f[SX_RX][i], s.f[SX_RY][i], s.f[SX_RZ][i], 0.0);
+    float4 *s4 = reinterpret_cast<float4 *>(scr + (long)i * ECP_SCR_STRIDE);
+    s4[0] = make_float4((float)s.f[SX_RX][i], (float)s.f[SX_RY][i], (float)s.f[SX_RZ][i], 0.0f);
+    s4[1] = make_float4((float)ar_l[0], (float)ar_l[1], (float)ar_l[2], (float)ar_l[3]);
+    s4[2] = make_float4((float)ar_l[4], (float)ar_l[5], (float)amax, (float)amin);
 }
 
 __device__ __forceinline__ Sym3 load_sym(const double *pp, long ld, int base, int i)

@@ -188,8 +188,7 @@ template <bool PP, int BS>
 __global__ void __launch_bounds__(BS) k_lgv_step_begin(DState s, LgvConst k, const int *dstep, const double *xl,
                                                         const double *yl, const double *zl, double *part,
                                                         unsigned *ticket, double thresh, int rule, int *flag,
-                                                        double *scal, double *pe_acc, double *pp, float *scr,
-                                                        double *pos4)
+                                                        double *scal, double *pe_acc, double *pp, float *scr)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double m = 0.0;
@@ -203,7 +202,7 @@ __global__ void __launch_bounds__(BS) k_lgv_step_begin(DState s, LgvConst k, con
         s.f[SX_FX][i] = 0.0; s.f[SX_FY][i] = 0.0; s.f[SX_FZ][i] = 0.0;
         s.f[SX_TX][i] = 0.0; s.f[SX_TY][i] = 0.0; s.f[SX_TZ][i] = 0.0;
         pe_acc[i] = 0.0;
-        if (PP) ecp_pp_particle(s, i, pp, scr, pos4);
+        if (PP) ecp_pp_particle(s, i, pp, scr);
     }
     rebuild_flag_from<BS>(m, part, ticket, thresh, rule, flag);
 }
@@ -1616,17 +1615,17 @@ void sx_lgv_step_begin_impl(sx_ctx *c, double dt, double damping, bool ecp_pp)
     int nb = sx_blocks(c->n, 256);
     sx_ensure_part(c, nb, 1);
     double thresh = nl.rule == SX_REBUILD_REFERENCE ? nl.r_skin_sq : (nl.skin / 2.0) * (nl.skin / 2.0);
-    double *pp = nullptr, *pos4 = nullptr;
+    double *pp = nullptr;
     float *scr = nullptr;
-    if (ecp_pp) sx_ecp_records(c, &pp, &scr, &pos4);
+    if (ecp_pp) sx_ecp_records(c, &pp, &scr);
     if (ecp_pp)
         hipLaunchKernelGGL((k_lgv_step_begin<true, 256>), dim3(nb), dim3(256), 0, c->stream, c->dstate(), k, c->d_step,
                            nl.x_last, nl.y_last, nl.z_last, c->d_part, c->d_ticket, thresh, nl.rule, c->d_flag,
-                           c->d_scal, c->d_pe_acc, pp, scr, pos4);
+                           c->d_scal, c->d_pe_acc, pp, scr);
     else
         hipLaunchKernelGGL((k_lgv_step_begin<false, 256>), dim3(nb), dim3(256), 0, c->stream, c->dstate(), k, c->d_step,
                            nl.x_last, nl.y_last, nl.z_last, c->d_part, c->d_ticket, thresh, nl.rule, c->d_flag,
-                           c->d_scal, c->d_pe_acc, pp, scr, pos4);
+                           c->d_scal, c->d_pe_acc, pp, scr);
 }
 
 void sx_lgv_final_impl(sx_ctx *c, double dt, double damping, bool constrained, bool thermo, int *dstep)

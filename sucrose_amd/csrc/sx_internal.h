@@ -174,8 +174,7 @@ struct sx_ctx {
     double *d_nht = nullptr;
     // per-particle ECP precompute
     double *d_ecp_pp = nullptr;   // FP64 records, 24 doubles per particle
-    float *d_ecp_scr = nullptr;   // FP32 screen records, 8 floats per particle
-    double *d_pos4 = nullptr;     // positions packed x y z pad
+    float *d_ecp_scr = nullptr;   // FP32 screen records, ECP_SCR_STRIDE floats per particle
     int ecp_pp_cap = 0;
     // per-particle Gay-Berne factors (GB_TAB doubles per particle)
     double *d_gb_tab = nullptr;
@@ -258,7 +257,7 @@ void sx_lgv_initial_impl(sx_ctx *c, double dt, int step, const int *dstep, doubl
 // (and the ECP per-particle records of the new orientations when ecp_pp)
 void sx_lgv_step_begin_impl(sx_ctx *c, double dt, double damping, bool ecp_pp);
 // the ECP per-particle record buffers (allocated on first use)
-void sx_ecp_records(sx_ctx *c, double **pp, float **scr, double **pos4);
+void sx_ecp_records(sx_ctx *c, double **pp, float **scr);
 // thermo = also run calculate_thermo in the same kernel; dstep = device step
 // counter to advance (graph replays)
 void sx_lgv_final_impl(sx_ctx *c, double dt, double damping, bool constrained, bool thermo = false,

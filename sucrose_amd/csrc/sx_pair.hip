@@ -184,18 +184,19 @@ __global__ void __launch_bounds__(256) k_gather2(DState s, const int *row_start,
 // ---------------------------------------------------------------------------
 // per-particle records of every particle (ecp_pp_particle); thread 0 also
 // clears the work counter of this call
-__global__ void k_ecp_pp(DState s, double *pp, float *scr, double *pos4, int *nwork)
+__global__ void k_ecp_pp(DState s, double *pp, float *scr, int *nwork)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0 && nwork) *nwork = 0;
     if (i >= s.n) return;
-    ecp_pp_particle(s, i, pp, scr, pos4);
+    ecp_pp_particle(s, i, pp, scr);
 }
 
 struct EcpScreen {
     int variant;       // screening class: 0 cut (ecp, ecpw, ecpc, ecpwc), 1 ecpr
     double cutoff;     // reach of the cut forms: cutoff + wf
     double L[3], invL[3];
+    float Lf[3], invLf[3];
     int periodic;
     int fvar;          // force variant: 0 ecp, 1 ecpr, 2 ecpw, 3 ecpc, 4 ecpwc
     EcpPar par;        // the potential's cutoff, weld width, chirality
@@ -217,28 +218,29 @@ static inline bool is_ecp_kind(int kind)
 // (largest/smallest semi-axis)^2 (checked on the host).  Before the solve,
 // F(psi) >= psi(1-psi) r^2 / lambda_max(G(psi)) with lambda_max(G) <=
 // (1-psi) amax_a^2 + psi amax_b^2 rejects distant pairs without a solve.
+// The separation is taken from the FP32 positions of the records: its error
+// (a few 1e-6 in a box of edge 50) changes F by far less than the margin.
 struct ScreenIn {
-    double4 pa, pb;
+    float4 pa, pb;
     float4 A0, A1, B0, B1;
 };
 
-// the screen's inputs of pair (a, b): positions and the FP32 records
-__device__ __forceinline__ ScreenIn ecp_screen_load(const double4 *pos4, const float *scr, int a, int b)
+// the screen's inputs of pair (a, b): one record line each
+__device__ __forceinline__ ScreenIn ecp_screen_load(const float *scr, int a, int b)
 {
-    const float4 *s4 = reinterpret_cast<const float4 *>(scr);
-    return ScreenIn{pos4[a], pos4[b], s4[2L * a], s4[2L * a + 1], s4[2L * b], s4[2L * b + 1]};
+    const float4 *ra = reinterpret_cast<const float4 *>(scr + (long)a * ECP_SCR_STRIDE);
+    const float4 *rb = reinterpret_cast<const float4 *>(scr + (long)b * ECP_SCR_STRIDE);
+    return ScreenIn{ra[0], rb[0], ra[1], ra[2], rb[1], rb[2]};
 }
 
 __device__ __forceinline__ bool ecp_screen32(const ScreenIn &in, const EcpScreen &sc)
 {
-    const double4 pa = in.pa, pb = in.pb;
-    double d0 = pa.x - pb.x, d1 = pa.y - pb.y, d2 = pa.z - pb.z;
+    float r0 = in.pa.x - in.pb.x, r1 = in.pa.y - in.pb.y, r2 = in.pa.z - in.pb.z;
     if (sc.periodic) {
-        d0 = sx_mimg_fast(d0, sc.L[0], sc.invL[0]);
-        d1 = sx_mimg_fast(d1, sc.L[1], sc.invL[1]);
-        d2 = sx_mimg_fast(d2, sc.L[2], sc.invL[2]);
+        r0 = r0 - roundf(r0 * sc.invLf[0]) * sc.Lf[0];
+        r1 = r1 - roundf(r1 * sc.invLf[1]) * sc.Lf[1];
+        r2 = r2 - roundf(r2 * sc.invLf[2]) * sc.Lf[2];
     }
-    float r0 = (float)d0, r1 = (float)d1, r2 = (float)d2;
     float rsq = r0 * r0 + r1 * r1 + r2 * r2;
     float thr;
     if (sc.variant == 1) {
@@ -274,7 +276,7 @@ __device__ __forceinline__ bool ecp_screen32(const ScreenIn &in, const EcpScreen
 
 
 // Compacts the segment's kept pairs in half-list order into the work list
-__global__ void __launch_bounds__(256) k_ecp_screen(const double4 *pos4, const float *scr, const int *half_i,
+__global__ void __launch_bounds__(256) k_ecp_screen(const float *scr, const int *half_i,
                                                     const int *half_j, const int *nnz_half, EcpScreen sc,
                                                     unsigned char *live, int *work, int *segcount)
 {
@@ -288,7 +290,7 @@ __global__ void __launch_bounds__(256) k_ecp_screen(const double4 *pos4, const f
         bool keep = false;
         if (k < end) {
             const int a = half_i[k], b = half_j[k];
-            keep = ecp_screen32(ecp_screen_load(pos4, scr, a, b), sc);
+            keep = ecp_screen32(ecp_screen_load(scr, a, b), sc);
             live[k] = keep ? 1 : 0;
         }
         unsigned long long m = __ballot(keep);
@@ -550,10 +552,8 @@ static void ensure_pp(sx_ctx *c)
     if (c->ecp_pp_cap < c->n) {
         if (c->d_ecp_pp) HIPCHK(hipFree(c->d_ecp_pp));
         if (c->d_ecp_scr) HIPCHK(hipFree(c->d_ecp_scr));
-        if (c->d_pos4) HIPCHK(hipFree(c->d_pos4));
         HIPCHK(hipMalloc(&c->d_ecp_pp, sizeof(double) * PP_STRIDE * (size_t)c->n));
-        HIPCHK(hipMalloc(&c->d_ecp_scr, sizeof(float) * 8 * (size_t)c->n));
-        HIPCHK(hipMalloc(&c->d_pos4, sizeof(double) * 4 * (size_t)c->n));
+        HIPCHK(hipMalloc(&c->d_ecp_scr, sizeof(float) * ECP_SCR_STRIDE * (size_t)c->n));
         c->ecp_pp_cap = c->n;
         c->have_graph = false;
     }
@@ -600,6 +600,8 @@ static EcpScreen make_screen(sx_ctx *c, int kind, const Params &P)
     for (int d = 0; d < 3; ++d) {
         sc.L[d] = c->box[d];
         sc.invL[d] = 1.0 / c->box[d];
+        sc.Lf[d] = (float)sc.L[d];
+        sc.invLf[d] = (float)sc.invL[d];
     }
     return sc;
 }
@@ -627,11 +629,10 @@ static void ecp_stage(sx_ctx *c, int stage, int kind, const EcpScreen &sc, const
     switch (stage) {
     case 0:
         hipLaunchKernelGGL(k_ecp_pp, dim3(sx_blocks(n, 256)), dim3(256), 0, c->stream, s, c->d_ecp_pp, c->d_ecp_scr,
-                           c->d_pos4, (int *)nullptr);
+                           (int *)nullptr);
         break;
     case 1:
-        hipLaunchKernelGGL(k_ecp_screen, dim3(ECP_NSEG), dim3(256), 0, c->stream, (const double4 *)c->d_pos4,
-                           c->d_ecp_scr, nl.half_i, nl.half_j, nnz_half, sc, c->d_live, c->d_work, c->d_segcount);
+        hipLaunchKernelGGL(k_ecp_screen, dim3(ECP_NSEG), dim3(256), 0, c->stream, (const float *)c->d_ecp_scr, nl.half_i, nl.half_j, nnz_half, sc, c->d_live, c->d_work, c->d_segcount);
         break;
     case 2: {
         long *stats = L.stats ? c->d_counts : nullptr;
@@ -836,12 +837,11 @@ static void ecp_forces_impl(sx_ctx *c, int kind, const Params &P)
         ecp_stage(c, stage, kind, sc, L);
 }
 
-void sx_ecp_records(sx_ctx *c, double **pp, float **scr, double **pos4)
+void sx_ecp_records(sx_ctx *c, double **pp, float **scr)
 {
     ensure_pp(c);
     *pp = c->d_ecp_pp;
     *scr = c->d_ecp_scr;
-    *pos4 = c->d_pos4;
 }
 
 template <int K1, int K2, typename T>
@@ -1114,7 +1114,7 @@ int sx_pair_eval(sx_ctx *c, int kind, const double *params, int nparams, long np
         if (is_ecp_kind(kind)) {
             ensure_pp(c);
             hipLaunchKernelGGL(k_ecp_pp, dim3(sx_blocks(c->n, 256)), dim3(256), 0, c->stream, s, c->d_ecp_pp,
-                               c->d_ecp_scr, c->d_pos4, (int *)nullptr);
+                               c->d_ecp_scr, (int *)nullptr);
         }
         int solver = c->ecp_solver == SX_ECP_LEARNED ? SX_ECP_CONVERGED : c->ecp_solver;
         long ld = 0;
