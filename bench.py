@@ -30,26 +30,27 @@ FLOP_OPTIM_DIST = 62        # one optim_dist evaluation (blend + 3x3 Cholesky so
 FLOP_BRENT_ITER = 20        # Brent bookkeeping per iteration
 FLOP_SCREEN = 30 + FLOP_OPTIM_DIST   # min image, bound test, f(psi)
 FLOP_ECP_FORCE = 650        # force/torque evaluation at the solved lambda
-FLOP_APPLY_ENTRY = 8        # per full-list entry of the per-particle apply
+FLOP_APPLY_ENTRY = 7        # per solved pair and particle in the per-particle apply
 # L2-miss bytes per launch of each ECP stage kernel at this workload, from
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch) scaled by a 256 MiB
 # copy measured the same way (FETCH x2.000, WRITE x1.000); the runs and the
 # arithmetic are in profiles/ecp_langevin_32768_fetch_write.txt
 TRAFFIC_BYTES = {
-    "ecp_precompute": (1687.0 * 2.0 + 8192.0) * 1024,
-    "ecp_screen": (10242.1 * 2.0 + 917.6) * 1024,
-    "ecp_solve": (12637.2 * 2.0 + 6209.3) * 1024,
-    "ecp_apply": (12423.7 * 2.0 + 3977.4) * 1024,
+    "ecp_precompute": (1694.6 * 2.0 + 8192.0) * 1024,
+    "ecp_screen": (9848.3 * 2.0 + 866.6) * 1024,
+    "ecp_solve": (10435.6 * 2.0 + 7626.9) * 1024,
+    "ecp_apply": (10120.0 * 2.0 + 2816.0) * 1024,
 }
 # FP64 operations issued per launch of each ECP stage kernel at this workload:
 # rocprofv3 --pmc SQ_INSTS_VALU_FLOPS_FP64 (counted per wavefront instruction,
 # FMA = 2, transcendental included) times the 64 lanes of a wavefront; the run
-# is in profiles/ecp_langevin_32768_fp64_counters.txt
+# is in profiles/ecp_langevin_32768_fp64_counters.txt (the screen computes in
+# FP32 and issues no FP64 operations)
 ISSUED_FP64 = {
     "ecp_precompute": 1.224e5 * 64,
-    "ecp_screen": 1.657e5 * 64,
+    "ecp_screen": 0.0,
     "ecp_solve": 2.565e6 * 64,
-    "ecp_apply": 2.670e5 * 64,
+    "ecp_apply": 3.515e4 * 64,
 }
 
 
@@ -765,12 +766,11 @@ def main():
     # roofline of the dominant kernel, timed live with HIP events on the library stream
     ms, st = dev.profile_ecp(sx.ECPR, (), nrep=20)
     screened, solved, iters = st
-    nnz_full = 2 * screened
     flops = {
         "ecp_precompute": n * 150.0,
         "ecp_screen": screened * FLOP_SCREEN,
         "ecp_solve": solved * (30 + FLOP_OPTIM_DIST + FLOP_ECP_FORCE) + iters * (FLOP_OPTIM_DIST + FLOP_BRENT_ITER),
-        "ecp_apply": nnz_full * FLOP_APPLY_ENTRY,
+        "ecp_apply": 2 * solved * FLOP_APPLY_ENTRY,
     }
     names = list(flops)
     k = max(range(4), key=lambda q: ms[q])
