@@ -421,12 +421,69 @@ __device__ __forceinline__ void rank_row_entries(const int *sh, int len, int len
     }
 }
 
-// row i sorted in place by one wavefront through its LDS buffer sh
+// A row of at most 64 PER entries sorted in place by one wavefront with a
+// bitonic network in registers: lane l holds entries l PER .. l PER + PER - 1
+// (padded with INT_MAX), exchanges at a distance of PER or more go across
+// lanes by shuffles, the others stay in the lane
+template <int PER>
+__device__ __forceinline__ void bitonic_row(int *row, int len, int lane)
+{
+    constexpr int N = SX_WAVE * PER;
+    int v[PER];
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int q = lane * PER + r;
+        v[r] = q < len ? row[q] : 0x7fffffff;
+    }
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= PER) {
+                const int lj = j / PER;
+                const bool lower = (lane & lj) == 0;
+#pragma unroll
+                for (int r = 0; r < PER; ++r) {
+                    const bool up = ((lane * PER + r) & k) == 0;
+                    const int o = __shfl_xor(v[r], lj, SX_WAVE);
+                    v[r] = lower == up ? min(v[r], o) : max(v[r], o);
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < PER; ++r) {
+                    const int p = r ^ j;
+                    if (p > r) {
+                        const bool up = ((lane * PER + r) & k) == 0;
+                        const int a = v[r], b = v[p];
+                        const bool sw = (a > b) == up;
+                        v[r] = sw ? b : a;
+                        v[p] = sw ? a : b;
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int q = lane * PER + r;
+        if (q < len) row[q] = v[r];
+    }
+}
+
+// row i sorted in place by one wavefront: up to 512 entries in registers
+// (bitonic_row), longer rows through its LDS buffer sh by ranks
 __device__ __forceinline__ void sort_row(int i, const int *row_start, int *nbr, int *sh, int lane, int *err)
 {
     {
         const int b = row_start[i], len = row_start[i + 1] - b;
         if (len <= 1) return;
+        if (len <= 8 * SX_WAVE) {
+            if (len <= SX_WAVE) bitonic_row<1>(nbr + b, len, lane);
+            else if (len <= 2 * SX_WAVE) bitonic_row<2>(nbr + b, len, lane);
+            else if (len <= 4 * SX_WAVE) bitonic_row<4>(nbr + b, len, lane);
+            else bitonic_row<8>(nbr + b, len, lane);
+            return;
+        }
         if (len > SORT_CAP) {
             if (lane == 0) atomicCAS(err, 0, DERR_ROW_CAP);
             return;
